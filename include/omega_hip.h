@@ -1,0 +1,142 @@
+/* omega_hip.h -- C ABI of the MI355X (gfx950) projector library libomega_hip.so.
+ *
+ * Plain C: fixed-width integers, float pointers, an opaque geometry handle and
+ * a HIP stream passed as void* (0 = the null stream). Every device pointer is
+ * borrowed from the caller (e.g. torch's data_ptr()); the library never frees
+ * caller memory. Geometry handles own their device copies of the coordinate
+ * tables. Calls are asynchronous on the given stream. Every entry point returns
+ * 0 on success or a negative OMG_ERR_* code; omg_last_error() gives a message
+ * (thread-local).
+ *
+ * Each entry point names the reference interface it replaces (villekf/OMEGA,
+ * paths relative to the repository root).
+ */
+#ifndef OMEGA_HIP_H
+#define OMEGA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMG_OK 0
+#define OMG_ERR_ARG (-1)          /* invalid argument (size, null pointer, unsupported combination) */
+#define OMG_ERR_HIP (-2)          /* HIP runtime error (allocation, launch, copy) */
+#define OMG_ERR_UNSUPPORTED (-3)  /* valid in the reference but not provided by this library */
+
+/* Projection data layouts (source/cpp/projector_functions.h:357-498). */
+#define OMG_MODE_PET_SINO 0   /* x: 4 floats per (radial, angle) bin; z: 2 floats per plane */
+#define OMG_MODE_CT_FLAT 1    /* x: 6 floats per projection; z: uV (2) or pitch vectors (6) per projection */
+#define OMG_MODE_LIST6 2      /* x: 6 floats per LOR */
+
+/* Projector types (source/Python/omegatomo/projector/proj.py:242, projector_type). */
+#define OMG_PROJ_SIDDON 1
+#define OMG_PROJ_ORTH 2
+#define OMG_PROJ_BDD 5
+
+typedef struct omg_geom omg_geom;
+
+/* Geometry description. Mirrors the scalars and tables the reference binds into
+ * paramStruct<T> (source/cpp/projector_functions.h:223-347) and the kernel
+ * arguments of projectorType123 (source/opencl/projectorType123.cl:81-267). */
+typedef struct omg_geom_desc {
+    uint32_t struct_size;      /* sizeof(omg_geom_desc) as seen by the caller */
+    int32_t projector_type;    /* OMG_PROJ_* */
+    int32_t mode;              /* OMG_MODE_* */
+    uint32_t Nx, Ny, Nz;       /* voxels */
+    float dx, dy, dz;          /* voxel size (mm) */
+    float bx, by, bz;          /* FOV lower corner (mm) */
+    uint32_t size_x;           /* nRowsD: radial bins / detector rows */
+    uint32_t size_y;           /* nColsD: angles / detector columns */
+    uint32_t n_proj;           /* planes (PET) or projections (CT) held by the tables */
+    int32_t ct_pitch;          /* CT: z holds 6 floats per projection */
+    float dPitchZ;             /* CT: axial detector pitch when ct_pitch == 0 */
+    float global_factor;       /* PET global correction factor */
+    const float* x; int64_t x_len;        /* host coordinate tables, copied at create */
+    const float* z; int64_t z_len;
+    const float* norm; int64_t norm_len;  /* optional per-measurement normalisation (full data order) */
+    /* orthogonal distance-based projector (projector_type 2) */
+    float orth_width;          /* tube width (FWHM) in mm: tube_width_z for 3D, tube_width_xy for 2D */
+    int32_t orth_3d;           /* 1: 3D (axial) orthogonal distance, 0: transaxial only */
+    /* branchless distance-driven projector (projector_type 5); CT mode only */
+    float dPitchX;             /* transaxial detector pitch */
+    float dPitchY;             /* axial detector pitch */
+    int32_t reserved[8];
+} omg_geom_desc;
+
+/* Library/version info and error text. */
+const char* omg_version(void);
+const char* omg_last_error(void);
+
+/* Geometry handles: replaces the per-projector set-up of projectorClass.initProj /
+ * initProjector (source/Python/omegatomo/projector/proj.py:1913, init.py:39) and the
+ * device buffers ProjectorClass::createBuffers makes (source/cpp/ProjectorClass.h). */
+int omg_geom_create(const omg_geom_desc* desc, omg_geom** out);
+int omg_geom_destroy(omg_geom* g);
+
+/* Forward projection y = A_k x for measurements [first, first + n_meas) of the
+ * subset-ordered data: first is the subset's first plane/projection (list mode: LOR).
+ * y is overwritten. Replaces projectorClass.forwardProject / projfunctions.forwardProjection
+ * (proj.py:1923, projfunctions.py:86) and projectorType123Implementation4(fp=1)
+ * (source/cpp/projector_functions.h:1311). */
+int omg_forward_project(const omg_geom* g, int64_t first, int64_t n_meas, const float* img, float* y, void* stream);
+
+/* Backprojection rhs += A_k^T y; if sens != NULL also sens += A_k^T 1. rhs/sens are
+ * accumulated (the caller zeroes them). Replaces projectorClass.backwardProject
+ * (proj.py:1929, projfunctions.py:735) and projectorType123Implementation4(fp=2)
+ * (projector_functions.h:1311, rhs :633-657). */
+int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_meas, const float* y, float* rhs, float* sens,
+                         void* stream);
+
+/* One OSEM subset data step, fused: yhat = A_k x; r = meas / (yhat + epps) (NaN/Inf -> epps);
+ * rhs += A_k^T r; sens += A_k^T 1 when sens != NULL. PET modes only. Replaces the
+ * forwardProjectionAFOpenCL -> computeForwardStep -> backwardProjectionAFOpenCL sequence
+ * of reconstructionAF (source/cpp/reconstructionAF.h:1089-1264, computeForwardProject.h:20-21,94,400-401). */
+int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_meas, const float* img, const float* meas,
+                       float* rhs, float* sens, float epps, void* stream);
+
+/* Per-LOR voxel lists of the Siddon traversal (verification of the traversal itself).
+ * counts[i] = number of voxels with length > 0 for measurement i. If offsets != NULL
+ * (exclusive prefix sums of counts), voxel indices and lengths are written to idx/len.
+ * Mirrors the index/element lists of improved_siddon_precomputed
+ * (source/cpp/improved_siddon_precomputed.cpp:32-251). */
+int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* counts, const int64_t* offsets,
+                     uint32_t* idx, float* len, void* stream);
+
+/* Relative difference prior gradient: grad += beta * dR/du, 6-neighbourhood.
+ * boundary 0: zero outside the volume (CPU RDPKernel, source/cpp/RDP.h:4-78);
+ * boundary 1: clamp to edge (GPU RDPKernel with images, source/opencl/auxKernels.cl:664-873). */
+int omg_rdp_gradient(const float* u, float* grad, uint32_t Nx, uint32_t Ny, uint32_t Nz, float gamma, float epps,
+                     float beta, int32_t boundary, void* stream);
+
+/* Elementwise image/measurement updates (FP32). */
+/* r = y / (yhat + epps), NaN/Inf -> epps (computeForwardProject.h:20-21,94,400-401); in place on yhat_r. */
+int omg_ratio(const float* y, float* yhat_r, int64_t n, float epps, void* stream);
+/* s = max-clamp: s < epps -> epps (functions.hpp:2996). */
+int omg_sens_clamp(float* s, int64_t n, float epps, void* stream);
+/* OSEM: x = x / S * (rhs + epps); positivity x < epps -> epps; optionally zero rhs afterwards
+ * (algorithms.h:12-15, subiterStep.h:111-121, reconstructionAF.h:1461-1465). */
+int omg_osem_update(float* x, const float* S, float* rhs, int64_t n, float epps, int32_t positivity,
+                    int32_t clear_rhs, void* stream);
+/* OSL-OSEM: x = x / (S + dU) * rhs; positivity; optionally zero rhs (subiterStep.h:219-231). */
+int omg_osl_update(float* x, const float* S, const float* dU, float* rhs, int64_t n, float epps,
+                   int32_t positivity, int32_t clear_rhs, void* stream);
+/* PDHG image update (auxKernels.cl:2034-2059, PDHGUpdate). subsets != 0: im = max(epps, im - tau*rhs).
+ * subsets == 0: u_new = max(epps, u - tau*rhs); im = u_new + theta*(u_new - u); u = u_new. */
+int omg_pdhg_update(float* im, const float* rhs, float* u, int64_t n, float epps, float theta, float tau,
+                    int32_t positivity, int32_t subsets, void* stream);
+/* PDHG dual step for the least-squares data term (computeForwardProject.h:227-316):
+ * p = (p + sigma * (Ax - y)) / (1 + sigma); in place on p, Ax read from ax. */
+int omg_pdhg_dual_l2(float* p, const float* ax, const float* y, int64_t n, float sigma, void* stream);
+
+/* Block-reduced dot products (double accumulation) for CGLS/PDHG scalars:
+ * out[0] = sum a*b. out is a device double. */
+int omg_dot(const float* a, const float* b, int64_t n, double* out, void* stream);
+/* y = a*x + b*y. */
+int omg_axpby(float* y, const float* x, int64_t n, float a, float b, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMEGA_HIP_H */

@@ -1,0 +1,121 @@
+// capi.hip -- geometry handles, error text and version of libomega_hip.so.
+#include "geom.hpp"
+
+#include <cstring>
+#include <new>
+
+namespace omg {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int check_range(const omg_geom* g, int64_t first, int64_t n) {
+    OMG_REQUIRE(first >= 0 && n >= 0, "negative measurement range (first=%lld, n=%lld)", (long long)first, (long long)n);
+    if (g->d.mode == OMG_MODE_LIST6) {
+        OMG_REQUIRE((first + n) * 6 <= g->d.x_len, "list-mode range [%lld, %lld) exceeds the %lld LORs in the table",
+                    (long long)first, (long long)(first + n), (long long)(g->d.x_len / 6));
+        return OMG_OK;
+    }
+    const int64_t per = (int64_t)g->d.size_x * g->d.size_y;
+    OMG_REQUIRE(per > 0, "detector size is zero");
+    OMG_REQUIRE(n % per == 0, "measurement count %lld is not a whole number of %lld-bin planes", (long long)n,
+                (long long)per);
+    OMG_REQUIRE(n <= 0xFFFFFFFFll, "one call handles at most 2^32-1 measurements");
+    OMG_REQUIRE(first + n / per <= (int64_t)g->d.n_proj, "planes [%lld, %lld) exceed the %u planes in the tables",
+                (long long)first, (long long)(first + n / per), g->d.n_proj);
+    return OMG_OK;
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+extern "C" const char* omg_version(void) { return "omega_hip 0.1 (gfx950)"; }
+
+extern "C" const char* omg_last_error(void) { return g_err; }
+
+static int upload(const float* src, int64_t n, float** dst) {
+    *dst = nullptr;
+    if (!src || n <= 0) return OMG_OK;
+    OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(dst), (size_t)n * sizeof(float)));
+    OMG_CHECK_HIP(hipMemcpy(*dst, src, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    return OMG_OK;
+}
+
+int bdd_prepare(omg_geom* g);   // BDD per-geometry set-up (bdd.hip)
+int orth_prepare(omg_geom* g);  // orthogonal projector set-up (orth.hip)
+
+extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
+    OMG_REQUIRE(d && out, "null descriptor or output pointer");
+    OMG_REQUIRE(d->struct_size == sizeof(omg_geom_desc), "omg_geom_desc size mismatch: caller %u, library %zu",
+                d->struct_size, sizeof(omg_geom_desc));
+    OMG_REQUIRE(d->Nx > 0 && d->Ny > 0 && d->Nz > 0, "empty image grid");
+    OMG_REQUIRE((uint64_t)d->Nx * d->Ny * d->Nz < (1ull << 32), "image grid exceeds 2^32 voxels");
+    OMG_REQUIRE(d->dx > 0.f && d->dy > 0.f && d->dz > 0.f, "voxel sizes must be positive");
+    OMG_REQUIRE(d->mode == OMG_MODE_PET_SINO || d->mode == OMG_MODE_CT_FLAT || d->mode == OMG_MODE_LIST6,
+                "unknown data mode %d", d->mode);
+    OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON || d->projector_type == OMG_PROJ_ORTH ||
+                    d->projector_type == OMG_PROJ_BDD,
+                "projector_type %d is not provided (1 Siddon, 2 orthogonal, 5 branchless distance-driven)",
+                d->projector_type);
+    OMG_REQUIRE(d->x && d->x_len > 0, "missing x coordinate table");
+    if (d->mode == OMG_MODE_PET_SINO) {
+        OMG_REQUIRE(d->x_len >= (int64_t)d->size_x * d->size_y * 4, "x table holds %lld floats, need 4*%u*%u",
+                    (long long)d->x_len, d->size_x, d->size_y);
+        OMG_REQUIRE(d->z && d->z_len >= (int64_t)d->n_proj * 2, "z table holds %lld floats, need 2*%u planes",
+                    (long long)d->z_len, d->n_proj);
+    } else if (d->mode == OMG_MODE_CT_FLAT) {
+        OMG_REQUIRE(d->x_len >= (int64_t)d->n_proj * 6, "x table holds %lld floats, need 6*%u projections",
+                    (long long)d->x_len, d->n_proj);
+        OMG_REQUIRE(d->z && d->z_len >= (int64_t)d->n_proj * (d->ct_pitch ? 6 : 2), "z (uV) table too short");
+    } else {
+        OMG_REQUIRE(d->x_len % 6 == 0, "list-mode x table length must be a multiple of 6");
+    }
+    if (d->projector_type == OMG_PROJ_BDD)
+        OMG_REQUIRE(d->mode == OMG_MODE_CT_FLAT, "projector_type 5 (BDD) supports CT flat-panel data only");
+
+    omg_geom* g = new (std::nothrow) omg_geom();
+    OMG_REQUIRE(g != nullptr, "out of host memory");
+    g->d = *d;
+    hipGetDevice(&g->device);
+    // far FOV edges in the reference's float arithmetic (projector_functions.h:1331-1333)
+    g->bmaxz = d->bz + (float)d->Nz * d->dz;
+    g->bmaxy = (float)d->Ny * d->dy + d->by;
+    g->bmaxx = (float)d->Nx * d->dx + d->bx;
+    int rc = upload(d->x, d->x_len, &g->x_dev);
+    if (!rc) rc = upload(d->z, d->z_len, &g->z_dev);
+    if (!rc) rc = upload(d->norm, d->norm_len, &g->norm_dev);
+    if (!rc && d->mode == OMG_MODE_PET_SINO) {
+        g->planar = true;
+        for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
+    }
+    if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
+    if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);
+    g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr;
+    if (rc) {
+        omg_geom_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return OMG_OK;
+}
+
+void bdd_release(omg_geom* g);
+void orth_release(omg_geom* g);
+
+extern "C" int omg_geom_destroy(omg_geom* g) {
+    if (!g) return OMG_OK;
+    hipFree(g->x_dev);
+    hipFree(g->z_dev);
+    hipFree(g->norm_dev);
+    bdd_release(g);
+    orth_release(g);
+    delete g;
+    return OMG_OK;
+}

@@ -1,0 +1,44 @@
+// geom.hpp -- the opaque geometry handle behind omg_geom*.
+#pragma once
+#include "common.hpp"
+#include "siddon_device.hpp"
+
+struct omg_geom {
+    omg_geom_desc d;          // caller's description (host table pointers cleared after upload)
+    float* x_dev = nullptr;   // coordinate tables on the device
+    float* z_dev = nullptr;
+    float* norm_dev = nullptr;
+    float bmaxx = 0.f, bmaxy = 0.f, bmaxz = 0.f;
+    int device = 0;
+    // Planar fast path: every plane's LORs have zs == zd and the sinogram is in PET mode.
+    bool planar = false;
+};
+
+namespace omg {
+
+// Kernel argument block for Siddon-family launches covering measurements [first, first + n).
+static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n) {
+    SidArgs a;
+    a.Nx = g->d.Nx; a.Ny = g->d.Ny; a.Nz = g->d.Nz; a.Nyx = g->d.Nx * g->d.Ny;
+    a.dx = g->d.dx; a.dy = g->d.dy; a.dz = g->d.dz;
+    a.bx = g->d.bx; a.by = g->d.by; a.bz = g->d.bz;
+    a.bmaxx = g->bmaxx; a.bmaxy = g->bmaxy; a.bmaxz = g->bmaxz;
+    a.size_x = g->d.size_x; a.size_y = g->d.size_y;
+    a.mode = g->d.mode; a.ct_pitch = g->d.ct_pitch;
+    a.plane_offset = first;
+    a.nmeas = n;
+    a.dPitchZ = g->d.dPitchZ;
+    a.global_factor = g->d.global_factor;
+    a.x = g->x_dev; a.z = g->z_dev;
+    a.norm = nullptr;
+    if (g->norm_dev) {
+        const int64_t per = (g->d.mode == MODE_LIST6) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
+        a.norm = g->norm_dev + first * per;
+    }
+    return a;
+}
+
+// Validates a [first, first + n) measurement range against the tables; returns OMG_OK or OMG_ERR_ARG.
+int check_range(const omg_geom* g, int64_t first, int64_t n);
+
+}  // namespace omg

@@ -1,0 +1,236 @@
+// image.hip -- image-space kernels: RDP gradient, OSEM/OSL/PDHG updates, ratio, reductions.
+//
+// All of these are HBM-bound streams or 7-point stencils: one lane per voxel
+// (x fastest, matching OMEGA's column-major image), 256 lanes per workgroup,
+// float math written in the reference's operation order.
+#include "common.hpp"
+
+namespace omg {
+
+constexpr int kEB = 256;
+
+// RDP gradient (source/cpp/RDP.h:4-78; source/opencl/auxKernels.cl:802-871).
+// Neighbour pairs are evaluated as the reference's float2 lanes: lane 0 = (+x, +y, +z),
+// lane 1 = (-x, -y, -z), each summed x, y, z; NaN lanes -> 0; grad += beta * (lane0 + lane1).
+__device__ __forceinline__ float rdp_term(float uj, float uk, float gamma, float epps) {
+    const float du = uj - uk;
+    const float den = uj + uk + gamma * fabsf(du);
+    return du * (gamma * fabsf(du) + uj + 3.f * uk + epps * epps) / (den * den + epps);
+}
+
+__global__ __launch_bounds__(kEB) void k_rdp(const float* __restrict__ u, float* __restrict__ grad, uint32_t Nx,
+                                             uint32_t Ny, uint32_t Nz, float gamma, float epps, float beta, int clamp) {
+    const uint32_t Nxy = Nx * Ny;
+    const uint32_t n = blockIdx.x * kEB + threadIdx.x;
+    if (n >= Nxy * Nz) return;
+    const uint32_t z = n / Nxy, rem = n - z * Nxy, y = rem / Nx, x = rem - y * Nx;
+    const float uj = u[n];
+    float xp, xm, yp, ym, zp, zm;
+    if (clamp) {  // clamp-to-edge: the missing neighbour is the voxel itself
+        xp = (x + 1 < Nx) ? u[n + 1] : uj;
+        xm = (x > 0) ? u[n - 1] : uj;
+        yp = (y + 1 < Ny) ? u[n + Nx] : uj;
+        ym = (y > 0) ? u[n - Nx] : uj;
+        zp = (z + 1 < Nz) ? u[n + Nxy] : uj;
+        zm = (z > 0) ? u[n - Nxy] : uj;
+    } else {
+        xp = (x + 1 < Nx) ? u[n + 1] : 0.f;
+        xm = (x > 0) ? u[n - 1] : 0.f;
+        yp = (y + 1 < Ny) ? u[n + Nx] : 0.f;
+        ym = (y > 0) ? u[n - Nx] : 0.f;
+        zp = (z + 1 < Nz) ? u[n + Nxy] : 0.f;
+        zm = (z > 0) ? u[n - Nxy] : 0.f;
+    }
+    float l0 = rdp_term(uj, xp, gamma, epps);
+    l0 = l0 + rdp_term(uj, yp, gamma, epps);
+    l0 = l0 + rdp_term(uj, zp, gamma, epps);
+    float l1 = rdp_term(uj, xm, gamma, epps);
+    l1 = l1 + rdp_term(uj, ym, gamma, epps);
+    l1 = l1 + rdp_term(uj, zm, gamma, epps);
+    if (isnan(l0)) l0 = 0.f;
+    if (isnan(l1)) l1 = 0.f;
+    grad[n] += beta * (l0 + l1);
+}
+
+__global__ __launch_bounds__(kEB) void k_ratio(const float* __restrict__ y, float* __restrict__ r, int64_t n, float epps) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB) {
+        float v = y[i] / (r[i] + epps);
+        if (isnan(v) || isinf(v)) v = epps;
+        r[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(kEB) void k_sens_clamp(float* __restrict__ s, int64_t n, float epps) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB)
+        if (s[i] < epps) s[i] = epps;
+}
+
+__global__ __launch_bounds__(kEB) void k_osem_update(float* __restrict__ x, const float* __restrict__ S,
+                                                     float* __restrict__ rhs, int64_t n, float epps, int pos, int clr) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB) {
+        float v = x[i] / S[i] * (rhs[i] + epps);
+        if (pos && v < epps) v = epps;
+        x[i] = v;
+        if (clr) rhs[i] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(kEB) void k_osl_update(float* __restrict__ x, const float* __restrict__ S,
+                                                    const float* __restrict__ dU, float* __restrict__ rhs, int64_t n,
+                                                    float epps, int pos, int clr) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB) {
+        float v = x[i] / (S[i] + dU[i]) * rhs[i];
+        if (pos && v < epps) v = epps;
+        x[i] = v;
+        if (clr) rhs[i] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(kEB) void k_pdhg_update(float* __restrict__ im, const float* __restrict__ rhs,
+                                                     float* __restrict__ u, int64_t n, float epps, float theta,
+                                                     float tau, int pos, int subsets) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB) {
+        if (subsets) {
+            float v = im[i];
+            v -= tau * rhs[i];
+            if (pos) v = fmaxf(epps, v);
+            im[i] = v;
+        } else {
+            const float up = u[i];
+            float un = up;
+            un -= tau * rhs[i];
+            if (pos) un = fmaxf(epps, un);
+            u[i] = un;
+            im[i] = un + theta * (un - up);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kEB) void k_pdhg_dual_l2(float* __restrict__ p, const float* __restrict__ ax,
+                                                      const float* __restrict__ y, int64_t n, float sigma) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB)
+        p[i] = (p[i] + sigma * (ax[i] - y[i])) / (1.f + sigma);
+}
+
+__global__ __launch_bounds__(kEB) void k_axpby(float* __restrict__ y, const float* __restrict__ x, int64_t n, float a,
+                                               float b) {
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB)
+        y[i] = a * x[i] + b * y[i];
+}
+
+// Dot product with double accumulation: wave64 shuffle reduction, one double atomic per workgroup.
+__global__ void k_zero_f64(double* __restrict__ p) { *p = 0.0; }
+
+__global__ __launch_bounds__(kEB) void k_dot(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
+                                             double* __restrict__ out) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB)
+        acc += (double)a[i] * (double)b[i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    __shared__ double part[kEB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) part[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kEB / 64; k++) t += part[k];
+        atomicAdd(out, t);
+    }
+}
+
+static unsigned stream_grid(int64_t n) {
+    const int64_t g = (n + kEB - 1) / kEB;
+    return (unsigned)(g < 8192 ? (g > 0 ? g : 1) : 8192);
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+extern "C" int omg_rdp_gradient(const float* u, float* grad, uint32_t Nx, uint32_t Ny, uint32_t Nz, float gamma,
+                                float epps, float beta, int32_t boundary, void* stream) {
+    OMG_REQUIRE(u && grad, "null pointer argument");
+    OMG_REQUIRE(Nx > 0 && Ny > 0 && Nz > 0, "empty image");
+    OMG_REQUIRE((uint64_t)Nx * Ny * Nz < (1ull << 32), "image exceeds 2^32 voxels");
+    OMG_REQUIRE(boundary == 0 || boundary == 1, "boundary must be 0 (zero) or 1 (clamp)");
+    hipLaunchKernelGGL(k_rdp, dim3(grid_for((int64_t)Nx * Ny * Nz, kEB)), dim3(kEB), 0, as_stream(stream), u, grad, Nx, Ny, Nz,
+                       gamma, epps, beta, (int)boundary);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_ratio(const float* y, float* yhat_r, int64_t n, float epps, void* stream) {
+    OMG_REQUIRE(y && yhat_r && n >= 0, "bad argument");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_ratio, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), y, yhat_r, n, epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_sens_clamp(float* s, int64_t n, float epps, void* stream) {
+    OMG_REQUIRE(s && n >= 0, "bad argument");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_sens_clamp, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), s, n, epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_osem_update(float* x, const float* S, float* rhs, int64_t n, float epps, int32_t positivity,
+                               int32_t clear_rhs, void* stream) {
+    OMG_REQUIRE(x && S && rhs && n >= 0, "bad argument");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_osem_update, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), x, S, rhs, n, epps,
+                       (int)positivity, (int)clear_rhs);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_osl_update(float* x, const float* S, const float* dU, float* rhs, int64_t n, float epps,
+                              int32_t positivity, int32_t clear_rhs, void* stream) {
+    OMG_REQUIRE(x && S && dU && rhs && n >= 0, "bad argument");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_osl_update, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), x, S, dU, rhs, n, epps,
+                       (int)positivity, (int)clear_rhs);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_pdhg_update(float* im, const float* rhs, float* u, int64_t n, float epps, float theta, float tau,
+                               int32_t positivity, int32_t subsets, void* stream) {
+    OMG_REQUIRE(im && rhs && n >= 0, "bad argument");
+    OMG_REQUIRE(subsets || u, "the no-subset PDHG update needs the u buffer");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_pdhg_update, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), im, rhs, u, n, epps, theta,
+                       tau, (int)positivity, (int)subsets);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_pdhg_dual_l2(float* p, const float* ax, const float* y, int64_t n, float sigma, void* stream) {
+    OMG_REQUIRE(p && ax && y && n >= 0, "bad argument");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_pdhg_dual_l2, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), p, ax, y, n, sigma);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_dot(const float* a, const float* b, int64_t n, double* out, void* stream) {
+    OMG_REQUIRE(a && b && out && n >= 0, "bad argument");
+    hipStream_t s = as_stream(stream);
+    // the accumulator is cleared by a one-lane kernel rather than a memset node, so the call can be graph-captured
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, s, out);
+    OMG_CHECK_LAUNCH();
+    if (!n) return OMG_OK;
+    const int64_t g = (n + kEB - 1) / kEB;
+    hipLaunchKernelGGL(k_dot, dim3((unsigned)(g < 2048 ? g : 2048)), dim3(kEB), 0, s, a, b, n, out);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_axpby(float* y, const float* x, int64_t n, float a, float b, void* stream) {
+    OMG_REQUIRE(y && x && n >= 0, "bad argument");
+    if (!n) return OMG_OK;
+    hipLaunchKernelGGL(k_axpby, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), y, x, n, a, b);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}

@@ -1,0 +1,303 @@
+// siddon_device.hpp -- improved Siddon ray traversal for gfx950 (device side).
+//
+// One lane traces one LOR/detector pixel. The arithmetic is the reference CPU
+// projector's, operation for operation (projectorType123Implementation4,
+// source/cpp/projector_functions.h:1311-2063), so that the visited voxel list and
+// the forward projection are bit-identical to OMEGA's CPU build in FP32. The
+// library is compiled with -ffp-contract=off and IEEE-rounded div/sqrt for this.
+//
+// Differences in form (not in result) from the reference loop:
+//   * the three-way "next plane crossing" choice (:1857-1922) is a single select
+//     of (axis, t_next) followed by one length formula;
+//   * the trace is a template over a visitor so FP, BP, fused OSEM and the
+//     index-list dump share one traversal.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace omg {
+
+enum Mode : int32_t { MODE_PET_SINO = 0, MODE_CT_FLAT = 1, MODE_LIST6 = 2 };
+
+struct SidArgs {
+    uint32_t Nx, Ny, Nz, Nyx;
+    float dx, dy, dz, bx, by, bz;
+    float bmaxx, bmaxy, bmaxz;     // far FOV edges, computed as in projector_functions.h:1331-1333
+    uint32_t size_x, size_y;       // detector rows (radial bins) / columns (angles)
+    int32_t mode;
+    int32_t ct_pitch;
+    int64_t plane_offset;          // subset's first plane/projection (LOR for list mode)
+    int64_t nmeas;                 // measurements in this call
+    float dPitchZ;
+    float global_factor;
+    const float* __restrict__ x;   // coordinate tables (device)
+    const float* __restrict__ z;
+    const float* __restrict__ norm; // per-measurement normalisation, subset-local, or null
+};
+
+struct Ray { float xs, ys, zs, xd, yd, zd; };
+
+__device__ __forceinline__ uint32_t to_u32(float v) { return (uint32_t)(int64_t)v; }
+
+__device__ __forceinline__ void fetch_ray(const SidArgs& a, int64_t lo, Ray& r) {
+    if (a.mode == MODE_LIST6) {
+        const float* p = a.x + (a.plane_offset + lo) * 6;
+        r.xs = p[0]; r.ys = p[1]; r.zs = p[2]; r.xd = p[3]; r.yd = p[4]; r.zd = p[5];
+        return;
+    }
+    const uint32_t per = a.size_x * a.size_y;
+    const uint32_t l32 = (uint32_t)lo;      // a subset call holds < 2^32 measurements
+    uint32_t iz = l32 / per;
+    const uint32_t rem = l32 - iz * per;
+    const uint32_t ix = rem % a.size_x;
+    const uint32_t iy = rem / a.size_x;
+    const int64_t izg = (int64_t)iz + a.plane_offset;
+    if (a.mode == MODE_PET_SINO) {
+        const float4 q = *reinterpret_cast<const float4*>(a.x + (size_t)(ix + iy * a.size_x) * 4);
+        r.xs = q.x; r.ys = q.y; r.xd = q.z; r.yd = q.w;
+        const float2 zz = *reinterpret_cast<const float2*>(a.z + izg * 2);
+        r.zs = zz.x; r.zd = zz.y;
+    } else {
+        const float* p = a.x + izg * 6;
+        r.xs = p[0]; r.ys = p[1]; r.zs = p[2]; r.xd = p[3]; r.yd = p[4]; r.zd = p[5];
+        // (T)ix - (T)size_x / 2. + .5 is exact in float for the sizes used (projector_functions.h:480-481)
+        const float i1 = (float)((double)ix - (double)a.size_x / 2. + .5);
+        const float i2 = (float)((double)iy - (double)a.size_y / 2. + .5);
+        if (a.ct_pitch) {
+            const float* u = a.z + izg * 6;
+            r.xd += u[0] * i1 + u[3] * i2;
+            r.yd += u[1] * i1 + u[4] * i2;
+            r.zd += u[2] * i1 + u[5] * i2;
+        } else {
+            const float2 u = *reinterpret_cast<const float2*>(a.z + izg * 2);
+            r.xd += u.x * i1;
+            r.yd += u.y * i1;
+            r.zd += a.dPitchZ * i2;
+        }
+    }
+}
+
+__device__ __forceinline__ void axis_inc(float tc, float tmin_a, float tmax, float tmax_a, uint32_t& vmin, uint32_t& vmax,
+                                         float& t0, int32_t& step, float diff, float b, float d, float s, uint32_t N) {
+    if (tc == tmin_a) vmin = 1u;
+    else { const float p = s + tc * diff; vmin = to_u32(ceilf((p - b) / d)); }
+    if (tmax == tmax_a) vmax = N;
+    else { const float p = s + tmax * diff; vmax = to_u32(floorf((p - b) / d)); }
+    t0 += (float)vmin * d / diff;
+    step = 1;
+}
+
+__device__ __forceinline__ void axis_dec(float tc, float tmin_a, float tmax, float tmax_a, uint32_t& vmin, uint32_t& vmax,
+                                         float& t0, int32_t& step, float diff, float b, float d, float s, uint32_t N) {
+    if (tc == tmin_a) vmax = N - 1u;
+    else { const float p = s + tc * diff; vmax = to_u32(floorf((p - b) / d)); }
+    if (tmax == tmax_a) vmin = 0u;
+    else { const float p = s + tmax * diff; vmin = to_u32(ceilf((p - b) / d)); }
+    t0 += (float)vmax * d / diff;
+    step = -1;
+}
+
+__device__ __forceinline__ int32_t first_voxel(float pt, float diff, float d, float a) {
+    return (int32_t)floorf((pt * diff - a) / d);
+}
+
+__device__ __forceinline__ bool preloop2(float b1, float b2, float diff1, float diff2, float max1, float max2, float d1, float d2,
+                                         uint32_t N1, uint32_t N2, int32_t& i1, int32_t& i2, float& t1u, float& t2u,
+                                         uint32_t& Np, float s2, float s1, float e2, float e1, float& tc, int32_t& u1,
+                                         int32_t& u2, float& t10, float& t20) {
+    const float a1 = b1 - s1, a2 = b2 - s2;
+    t10 = a1 / diff1;
+    t20 = a2 / diff2;
+    const float t1b = (max1 - s1) / diff1, t2b = (max2 - s2) / diff2;
+    const float t1min = fminf(t10, t1b), t1max = fmaxf(t10, t1b);
+    const float t2min = fminf(t20, t2b), t2max = fmaxf(t20, t2b);
+    tc = fmaxf(t1min, t2min);
+    const float tmax = fminf(t1max, t2max);
+    if (tc >= tmax) return true;
+    uint32_t v1min = 0, v1max = 0, v2min = 0, v2max = 0;
+    if (s1 < e1) axis_inc(tc, t1min, tmax, t1max, v1min, v1max, t10, u1, diff1, b1, d1, s1, N1);
+    else axis_dec(tc, t1min, tmax, t1max, v1min, v1max, t10, u1, diff1, b1, d1, s1, N1);
+    if (s2 < e2) axis_inc(tc, t2min, tmax, t2max, v2min, v2max, t20, u2, diff2, b2, d2, s2, N2);
+    else axis_dec(tc, t2min, tmax, t2max, v2min, v2max, t20, u2, diff2, b2, d2, s2, N2);
+    Np = v1max + 1u + v2max + 1u - v1min - v2min;
+    const float pt = (fminf(t10, t20) + tc) * 0.5f;   // x/2 and x*0.5 are the same IEEE result
+    i1 = first_voxel(pt, diff1, d1, a1);
+    i2 = first_voxel(pt, diff2, d2, a2);
+    if (i1 < 0 || (uint32_t)i1 >= N1 || i2 < 0 || (uint32_t)i2 >= N2) return true;
+    t1u = d1 / fabsf(diff1);
+    t2u = d2 / fabsf(diff2);
+    return false;
+}
+
+__device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float xdf, float ydf, float zdf,
+                                         int32_t& ti, int32_t& tj, int32_t& tk, float& tyu, float& txu, float& tzu,
+                                         uint32_t& Np, float& tc, int32_t& iu, int32_t& ju, int32_t& ku,
+                                         float& tx0, float& ty0, float& tz0) {
+    const float ax = g.bx - r.xs, ay = g.by - r.ys, az = g.bz - r.zs;
+    tx0 = ax / xdf; ty0 = ay / ydf; tz0 = az / zdf;
+    const float txb = (g.bmaxx - r.xs) / xdf, tyb = (g.bmaxy - r.ys) / ydf, tzb = (g.bmaxz - r.zs) / zdf;
+    const float txmin = fminf(tx0, txb), txmax = fmaxf(tx0, txb);
+    const float tymin = fminf(ty0, tyb), tymax = fmaxf(ty0, tyb);
+    const float tzmin = fminf(tz0, tzb), tzmax = fmaxf(tz0, tzb);
+    tc = fmaxf(fmaxf(txmin, tzmin), tymin);
+    const float tmax = fminf(fminf(txmax, tzmax), tymax);
+    if (tc >= tmax) return true;
+    uint32_t imin = 0, imax = 0, jmin = 0, jmax = 0, kmin = 0, kmax = 0;
+    if (r.xs < r.xd) axis_inc(tc, txmin, tmax, txmax, imin, imax, tx0, iu, xdf, g.bx, g.dx, r.xs, g.Nx);
+    else axis_dec(tc, txmin, tmax, txmax, imin, imax, tx0, iu, xdf, g.bx, g.dx, r.xs, g.Nx);
+    if (r.ys < r.yd) axis_inc(tc, tymin, tmax, tymax, jmin, jmax, ty0, ju, ydf, g.by, g.dy, r.ys, g.Ny);
+    else axis_dec(tc, tymin, tmax, tymax, jmin, jmax, ty0, ju, ydf, g.by, g.dy, r.ys, g.Ny);
+    if (r.zs < r.zd) axis_inc(tc, tzmin, tmax, tzmax, kmin, kmax, tz0, ku, zdf, g.bz, g.dz, r.zs, g.Nz);
+    else axis_dec(tc, tzmin, tmax, tzmax, kmin, kmax, tz0, ku, zdf, g.bz, g.dz, r.zs, g.Nz);
+    Np = (kmax - kmin + 1) + (jmax - jmin + 1) + (imax - imin + 1);
+    const float pt = (fminf(fminf(tz0, ty0), tx0) + tc) * 0.5f;
+    ti = first_voxel(pt, xdf, g.dx, ax);
+    tj = first_voxel(pt, ydf, g.dy, ay);
+    tk = first_voxel(pt, zdf, g.dz, az);
+    if (ti < 0 || (uint32_t)ti >= g.Nx || tj < 0 || (uint32_t)tj >= g.Ny || tk < 0 || (uint32_t)tk >= g.Nz) return true;
+    txu = g.dx / fabsf(xdf);
+    tyu = g.dy / fabsf(ydf);
+    tzu = g.dz / fabsf(zdf);
+    return false;
+}
+
+__device__ __forceinline__ int perp_start(float b, float d, float dd, uint32_t N) {
+    int idx = 0;
+    float s = b - d + dd;
+    for (uint32_t i = 0; i < N; i++) {
+        if (s > 0.f) { idx = (int)i; break; }
+        s += dd;
+    }
+    return idx;
+}
+
+// Trace LOR `lo`: calls visit(voxel, length) for each voxel with length > 0 in traversal order.
+// Returns false when the LOR contributes nothing. `w` is the LOR weight (PET: norm*gf/L, CT: 1).
+template <typename Visit>
+__device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float& w, Visit&& visit) {
+    Ray r;
+    fetch_ray(g, lo, r);
+    const float ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
+    if ((ydf == 0.f && xdf == 0.f) || isinf(ydf) || isinf(xdf)) return false;
+    const bool ct = g.mode == MODE_CT_FLAT;
+    const float L = sqrtf(xdf * xdf + ydf * ydf + zdf * zdf);
+
+    // axis-parallel in-plane LOR (projector_functions.h:1508-1700); thresholds are double 1e-8
+    if ((double)fabsf(zdf) < 1e-8 && ((double)fabsf(ydf) < 1e-8 || (double)fabsf(xdf) < 1e-8)) {
+        const int32_t tk = (int32_t)(fabsf(r.zs - g.bz) / g.dz);
+        if (tk < 0 || (uint32_t)tk >= g.Nz) return false;
+        int a1 = 0, a2;
+        float dT1, dT2, db, dd, ddb, dd2;
+        uint32_t Nm, Nrow, Nrun, stride;
+        if ((double)fabsf(ydf) < 1e-8 && r.yd <= g.bmaxy && r.yd >= g.by) {
+            a2 = (int)g.Nx - 1; dT1 = g.dx; dT2 = g.dx; db = g.by; dd = r.yd; ddb = g.dy;
+            float e1, e2;
+            if (r.xs > r.xd) { e1 = g.bx - r.xd; e2 = g.bx + (float)g.Nx * g.dx - r.xs; }
+            else { e1 = g.bx - r.xs; e2 = g.bx + (float)g.Nx * g.dx - r.xd; }
+            for (int k = 0; k < (int)g.Nx; k++) {
+                if (e1 >= 0.f) { a1 = k; dT1 = (k == 0) ? g.dx : fminf(e1, g.dx); break; }
+                e1 += g.dx;
+            }
+            for (int k = (int)g.Nx - 1; k >= a1; k--) {
+                if (e2 <= 0.f) { a2 = k; dT2 = (k == (int)g.Nx - 1) ? g.dx : fminf(-e2, g.dx); break; }
+                e2 -= g.dx;
+            }
+            dd2 = g.dx;
+            // the row start uses Ny as the row stride, as the reference CPU code does (projector_functions.h:1570-1573, :1176)
+            Nm = g.Ny; Nrow = g.Ny; Nrun = g.Nx; stride = 1u;
+        } else if ((double)fabsf(xdf) < 1e-8 && r.xd <= g.bmaxx && r.xd >= g.bx) {
+            a2 = (int)g.Ny - 1; dT1 = g.dx; dT2 = g.dx; db = g.bx; dd = r.xd;
+            float e1, e2;
+            if (r.ys > r.yd) { e1 = g.by - r.yd; e2 = g.by + (float)g.Ny * g.dy - r.ys; }
+            else { e1 = g.by - r.ys; e2 = g.by + (float)g.Ny * g.dy - r.yd; }
+            for (int k = 0; k < (int)g.Ny; k++) {
+                if (e1 >= 0.f) { a1 = k; dT1 = (k == 0) ? g.dy : fminf(e1, g.dy); break; }
+                e1 += g.dy;
+            }
+            for (int k = (int)g.Ny - 1; k >= a1; k--) {
+                if (e2 <= 0.f) { a2 = k; dT2 = (k == (int)g.Ny - 1) ? g.dy : fminf(-e2, g.dy); break; }
+                e2 -= g.dy;
+            }
+            dd2 = g.dy; ddb = g.dx;
+            Nm = 1u; Nrow = g.Nx; Nrun = g.Ny; stride = g.Nx;
+        } else {
+            return false;
+        }
+        uint32_t apu = 0;
+        {
+            float s = db - dd + ddb;
+            for (uint32_t i = 0; i < Nrow; i++) {
+                if (s > 0.f) { apu = i; break; }
+                s += ddb;
+            }
+        }
+        uint32_t ind = apu * Nm + (uint32_t)tk * Nrow * Nrun;
+        if (!ct) {
+            float t = 1.f / L;
+            if (g.norm) t *= g.norm[lo];
+            t *= g.global_factor;
+            w = t;
+        } else {
+            w = 1.f;
+        }
+        for (uint32_t ii = (uint32_t)a1; ii <= (uint32_t)a2; ii++) {
+            float din = dd2;
+            if (ii == (uint32_t)a1) ind += stride * ii;
+            if (a1 > 0 && ii == (uint32_t)a1) din = dT1;
+            else if ((uint32_t)a2 < Nrun - 1 && ii == (uint32_t)a2) din = dT2;
+            visit(ind, din);
+            ind += stride;
+        }
+        return true;
+    }
+
+    int32_t ti = 0, tj = 0, tk = 0, ux = 0, uy = 0, uz = 0;
+    float txu = 0.f, tyu = 0.f, tzu = 0.f, tc = 0.f, tx0 = 1e8f, ty0 = 1e8f, tz0 = 1e8f;
+    uint32_t Np = 0;
+    bool skip;
+    if (fabsf(zdf) < 1e-8f) {
+        tk = (int32_t)(fabsf(r.zs - g.bz) / g.dz);
+        if (tk < 0 || (uint32_t)tk >= g.Nz) return false;
+        skip = preloop2(g.bx, g.by, xdf, ydf, g.bmaxx, g.bmaxy, g.dx, g.dy, g.Nx, g.Ny, ti, tj, txu, tyu, Np,
+                        r.ys, r.xs, r.yd, r.xd, tc, ux, uy, tx0, ty0);
+    } else if (fabsf(ydf) < 1e-8f) {
+        tj = perp_start(g.by, r.yd, g.dy, g.Ny);
+        skip = preloop2(g.bx, g.bz, xdf, zdf, g.bmaxx, g.bmaxz, g.dx, g.dz, g.Nx, g.Nz, ti, tk, txu, tzu, Np,
+                        r.zs, r.xs, r.zd, r.xd, tc, ux, uz, tx0, tz0);
+        if (r.yd > g.bmaxy || r.yd < g.by) skip = true;
+    } else if (fabsf(xdf) < 1e-8f) {
+        ti = perp_start(g.bx, r.xd, g.dx, g.Nx);
+        skip = preloop2(g.by, g.bz, ydf, zdf, g.bmaxy, g.bmaxz, g.dy, g.dz, g.Ny, g.Nz, tj, tk, tyu, tzu, Np,
+                        r.zs, r.ys, r.zd, r.yd, tc, uy, uz, ty0, tz0);
+        if (r.xd > g.bmaxx || r.xd < g.bx) skip = true;
+    } else {
+        skip = preloop3(g, r, xdf, ydf, zdf, ti, tj, tk, tyu, txu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0);
+    }
+    if (skip) return false;
+    if (!ct) {
+        float t = 1.f / L;
+        if (g.norm) t *= g.norm[lo];
+        t *= g.global_factor;
+        w = t;
+    } else {
+        w = 1.f;
+    }
+    for (uint32_t ii = 0; ii < Np; ii++) {
+        const uint32_t ind = (uint32_t)tj * g.Nx + (uint32_t)ti + (uint32_t)tk * g.Nyx;
+        const bool zmin = tz0 < ty0 && tz0 < tx0;
+        const bool ymin = !zmin && ty0 < tx0;
+        const float tn = zmin ? tz0 : (ymin ? ty0 : tx0);
+        float len = 0.f;
+        if (tn >= 0.f && tn <= 1.f) len = (tc < 0.f) ? tn * L : (tn - tc) * L;
+        else if (tc >= 0.f && tc <= 1.f) len = (tn > 1.f) ? (1.f - tc) * L : (tn - tc) * L;
+        tc = tn;
+        if (zmin) { tk += uz; tz0 += tzu; }
+        else if (ymin) { tj += uy; ty0 += tyu; }
+        else { ti += ux; tx0 += txu; }
+        if (len > 0.f) visit(ind, len);
+        if (ti < 0 || (uint32_t)ti >= g.Nx || tj < 0 || (uint32_t)tj >= g.Ny || tk < 0 || (uint32_t)tk >= g.Nz) break;
+    }
+    return true;
+}
+
+}  // namespace omg

@@ -1,0 +1,183 @@
+"""Scanner geometry and coordinate tables in OMEGA's projector formats.
+
+Coordinate tables follow the layouts the reference projector consumes
+(source/cpp/projector_functions.h:357-498, source/opencl/general_opencl_functions.h:1601-1630):
+
+* PET sinogram: ``x`` holds 4 floats per (radial, angle) bin ``[xs, ys, xd, yd]`` with the
+  radial bin fastest; ``z`` holds 2 floats per sinogram plane ``[zs, zd]``.
+* CT flat panel: ``x`` holds 6 floats per projection (source xyz, detector-centre xyz); ``z``
+  holds 2 floats per projection, the transaxial panel direction scaled by the pixel pitch
+  (``uV``, source/Python/omegatomo/projector/detcoord.py:163-185).
+* List mode: 6 floats per LOR ``[xs, ys, zs, xd, yd, zd]``.
+
+Voxel grid placement restates ``computePixelSize`` (source/Python/omegatomo/projector/coordinates.py:42-62):
+boundaries are ``linspace(-FOV/2, FOV/2, N+1)`` in float32, ``d = xx[1] - xx[0]``, ``b = xx[0]``.
+
+The scanner builders are this project's own (the reference reads measured geometry
+from files); they generate rings, sinogram planes and flat-panel trajectories with the
+sizes of the workloads measured here.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Grid:
+    """Voxel grid: counts, voxel sizes and the lower-left corner of the FOV (mm)."""
+    Nx: int
+    Ny: int
+    Nz: int
+    dx: float
+    dy: float
+    dz: float
+    bx: float
+    by: float
+    bz: float
+
+    @property
+    def N(self) -> int:
+        return self.Nx * self.Ny * self.Nz
+
+    @staticmethod
+    def from_fov(Nx: int, Ny: int, Nz: int, FOVa_x: float, FOVa_y: float, axial_fov: float,
+                 oOffsetX: float = 0.0, oOffsetY: float = 0.0, oOffsetZ: float = 0.0) -> "Grid":
+        fov = np.array([FOVa_x, FOVa_y, axial_fov], dtype=np.float32)
+        e = -fov / 2
+        xx = np.linspace(e[0] + oOffsetX, -e[0] + oOffsetX, Nx + 1, dtype=np.float32)
+        yy = np.linspace(e[1] + oOffsetY, -e[1] + oOffsetY, Ny + 1, dtype=np.float32)
+        zz = np.linspace(e[2] + oOffsetZ, -e[2] + oOffsetZ, Nz + 1, dtype=np.float32)
+        return Grid(Nx, Ny, Nz,
+                    float(np.float32(xx[1] - xx[0])), float(np.float32(yy[1] - yy[0])), float(np.float32(zz[1] - zz[0])),
+                    float(xx[0]), float(yy[0]), float(zz[0]))
+
+    def n3(self) -> np.ndarray:
+        return np.array([self.Nx, self.Ny, self.Nz], dtype=np.uint32)
+
+    def d3(self, dtype=np.float32) -> np.ndarray:
+        return np.array([self.dx, self.dy, self.dz], dtype=dtype)
+
+    def b3(self, dtype=np.float32) -> np.ndarray:
+        return np.array([self.bx, self.by, self.bz], dtype=dtype)
+
+    def centers(self):
+        """Voxel centre coordinates per axis (coordinates.py:124-126)."""
+        xc = (np.float32(self.bx) + np.arange(self.Nx, dtype=np.float32) * np.float32(self.dx)
+              + np.float32(self.dx) / np.float32(2)).astype(np.float32)
+        yc = (np.float32(self.by) + np.arange(self.Ny, dtype=np.float32) * np.float32(self.dy)
+              + np.float32(self.dy) / np.float32(2)).astype(np.float32)
+        zc = (np.float32(self.bz) + np.arange(self.Nz, dtype=np.float32) * np.float32(self.dz)
+              + np.float32(self.dz) / np.float32(2)).astype(np.float32)
+        return xc, yc, zc
+
+
+def ring_sinogram_xy(n_radial: int, n_angles: int, radial_pitch: float, radius: float,
+                     angle_span: float = math.pi) -> np.ndarray:
+    """Transaxial sinogram table ``[xs, ys, xd, yd]`` per (radial, angle), radial fastest.
+
+    Bin (r, a) is the chord of the detector ring at signed distance
+    ``s = (r - (n_radial - 1)/2) * radial_pitch`` from the centre, with normal at angle
+    ``phi = a * angle_span / n_angles``. Endpoints lie on the ring. Computed in float64,
+    stored as float32.
+    """
+    r = np.arange(n_radial, dtype=np.float64)
+    a = np.arange(n_angles, dtype=np.float64)
+    s = (r - (n_radial - 1) / 2.0) * radial_pitch
+    phi = a * angle_span / n_angles
+    S, PHI = np.meshgrid(s, phi, indexing="xy")       # shape (n_angles, n_radial): radial fastest
+    t0 = np.sqrt(np.maximum(radius * radius - S * S, 0.0))
+    c, sn = np.cos(PHI), np.sin(PHI)
+    xs = S * c + t0 * sn
+    ys = S * sn - t0 * c
+    xd = S * c - t0 * sn
+    yd = S * sn + t0 * c
+    tab = np.stack([xs, ys, xd, yd], axis=-1)           # (n_angles, n_radial, 4)
+    return np.ascontiguousarray(tab.reshape(-1).astype(np.float32))
+
+
+def direct_planes_z(grid: Grid) -> np.ndarray:
+    """One direct (in-plane) sinogram per voxel slice at the slice centre: ``[zs, zd]`` per plane."""
+    zc = grid.centers()[2]
+    return np.ascontiguousarray(np.stack([zc, zc], axis=-1).reshape(-1).astype(np.float32))
+
+
+def ring_pair_planes_z(n_rings: int, ring_pitch: float, max_ring_diff: int) -> np.ndarray:
+    """Span-1 plane table ``[zs, zd]`` for every ring pair with ``|r1 - r2| <= max_ring_diff``.
+
+    Planes are ordered ring-difference major (0, +1, -1, +2, -2, ...) and by the first ring
+    within a difference, the Michelogram ordering of span-1 segments.
+    """
+    zr = (np.arange(n_rings, dtype=np.float64) - (n_rings - 1) / 2.0) * ring_pitch
+    planes = []
+    diffs = [0]
+    for d in range(1, max_ring_diff + 1):
+        diffs += [d, -d]
+    for d in diffs:
+        for r1 in range(n_rings):
+            r2 = r1 + d
+            if 0 <= r2 < n_rings:
+                planes.append((zr[r1], zr[r2]))
+    return np.ascontiguousarray(np.asarray(planes, dtype=np.float64).reshape(-1).astype(np.float32))
+
+
+def cbct_tables(n_views: int, sod: float, sdd: float, pitch_u: float, span: float = 2 * math.pi,
+                start: float = 0.0):
+    """Circular cone-beam trajectory: ``x`` (6 floats/view) and ``uV`` (2 floats/view).
+
+    The source sits at distance ``sod`` from the rotation axis, the panel centre at
+    ``sdd - sod`` on the far side; ``uV`` is the in-plane panel direction times ``pitch_u``.
+    """
+    th = start + np.arange(n_views, dtype=np.float64) * span / n_views
+    c, s = np.cos(th), np.sin(th)
+    src = np.stack([sod * c, sod * s, np.zeros_like(c)], axis=-1)
+    det = np.stack([-(sdd - sod) * c, -(sdd - sod) * s, np.zeros_like(c)], axis=-1)
+    x = np.concatenate([src, det], axis=-1).reshape(-1).astype(np.float32)
+    uv = np.stack([-s * pitch_u, c * pitch_u], axis=-1).reshape(-1).astype(np.float32)
+    return np.ascontiguousarray(x), np.ascontiguousarray(uv)
+
+
+def subset_index_type8(n_proj: int, subsets: int):
+    """Subset type 8: every ``subsets``-th plane/projection (indices.py:158-170).
+
+    Returns (index, nMeas) where ``index`` lists planes in subset order and ``nMeas[k]`` the
+    number of planes of subset ``k``.
+    """
+    if subsets <= 1:
+        return np.arange(n_proj, dtype=np.int64), np.array([n_proj], dtype=np.int64)
+    idx = [np.arange(k, n_proj, subsets, dtype=np.int64) for k in range(subsets)]
+    return np.concatenate(idx), np.array([len(i) for i in idx], dtype=np.int64)
+
+
+@dataclass
+class PETScanner:
+    """A cylindrical PET scanner sampled as parallel-chord sinograms."""
+    radius: float
+    n_radial: int
+    n_angles: int
+    radial_pitch: float
+    z: np.ndarray = field(repr=False)      # [zs, zd] per plane, float32
+
+    @property
+    def n_planes(self) -> int:
+        return self.z.size // 2
+
+    def xy_table(self) -> np.ndarray:
+        return ring_sinogram_xy(self.n_radial, self.n_angles, self.radial_pitch, self.radius)
+
+
+def pet_2d_planes(grid: Grid, radius: float = 400.0, n_radial: int = 128, n_angles: int = 160,
+                  radial_pitch: float = 2.5) -> PETScanner:
+    """2-D plane PET: one direct sinogram per voxel slice."""
+    return PETScanner(radius, n_radial, n_angles, radial_pitch, direct_planes_z(grid))
+
+
+def pet_3d_rings(radius: float = 400.0, n_rings: int = 32, ring_pitch: float = 4.0, crystals: int = 448,
+                 max_ring_diff: int = 31) -> PETScanner:
+    """3-D PET with span-1 ring-pair planes; ``crystals/2`` angles and radial bins at the
+    arc sampling of the ring (half the crystal pitch)."""
+    n_ang = crystals // 2
+    pitch = math.pi * radius / crystals
+    return PETScanner(radius, n_ang, n_ang, pitch, ring_pair_planes_z(n_rings, ring_pitch, max_ring_diff))

@@ -1,0 +1,368 @@
+"""Drop-in projector operator: the omegatomo ``projectorClass`` surface on the HIP library.
+
+Mirrors ``omegatomo.projector.proj.projectorClass`` (source/Python/omegatomo/projector/proj.py:36)
+for the matrix-free projector path:
+
+* configure attributes, then ``addProjector()`` (proj.py:520) and ``initProj()`` (proj.py:1913);
+* ``forwardProject(f, subset)`` / ``backwardProject(y, subset)`` (proj.py:1923-1933);
+* ``options * f`` and ``options.T() * y`` with ``options.subset = k`` (proj.py:1936-1951);
+* the bookkeeping attributes ``N``, ``Nx/Ny/Nz`` (uint32 arrays), ``nMeas`` (cumulative subset
+  starts), ``nMeasSubset``, ``nProjSubset``, ``nTotMeas``, ``index``, ``x0``.
+
+Tensors are flat float32 CUDA (HIP) tensors in OMEGA's column-major order (x fastest; measurements
+radial-bin fastest, then angle, then plane/projection), as with ``useTorch=True, useCUDA=True``
+(projfunctions.py:182-185). Outputs are newly allocated per call. Work is queued on torch's
+current stream; unlike the reference there is no device-wide synchronisation after each launch.
+
+Coordinates: either the caller provides the projector's coordinate tables directly (``x`` and
+``z``, as in source/Python/custom_detector_exampleSimple.py), or they are generated from
+scanner parameters by :mod:`omega_amd.geometry` (a ring scanner sampled as parallel-chord
+sinograms; a circular cone-beam trajectory for CT).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from . import geometry as G
+
+
+def _stream_ptr(torch):
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Geometry:
+    """Owns an omg_geom handle (device copies of the coordinate tables)."""
+
+    def __init__(self, desc: _lib.GeomDesc, keep=()):
+        self._keep = keep  # host arrays referenced by desc until create returns
+        h = C.c_void_p()
+        _lib.call("omg_geom_create", C.byref(desc), C.byref(h))
+        self.handle = h
+        self.desc = desc
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.load().omg_geom_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class projectorClass:
+    """Projector operator with the omegatomo ``projectorClass`` interface (projector types 1, 2, 5)."""
+
+    def __init__(self):
+        # image
+        self.Nx = 128
+        self.Ny = 128
+        self.Nz = 63
+        self.FOVa_x = 0.0
+        self.FOVa_y = 0.0
+        self.axial_fov = 0.0
+        self.oOffsetX = 0.0
+        self.oOffsetY = 0.0
+        self.oOffsetZ = 0.0
+        # projector
+        self.projector_type = 1
+        self.tube_width_xy = 0.0
+        self.tube_width_z = 0.0
+        # data
+        self.CT = False
+        self.listmode = 0
+        self.subsets = 1
+        self.subsetType = 8
+        self.x = np.empty(0, dtype=np.float32)
+        self.z = np.empty(0, dtype=np.float32)
+        self.Ndist = 0
+        self.Nang = 0
+        self.NSinos = 0
+        self.nRowsD = 0
+        self.nColsD = 0
+        self.nProjections = 0
+        self.SinM = np.empty(0, dtype=np.float32)
+        self.normalization_correction = False
+        self.normalization = np.empty(0, dtype=np.float32)
+        self.global_correction_factor = 1.0
+        self.epps = 1e-5
+        # PET scanner (used when no coordinate tables are given)
+        self.diameter = 0.0
+        self.det_per_ring = 0
+        self.rings = 0
+        self.cr_pz = 0.0
+        self.ring_difference = -1
+        # CT geometry (used when no coordinate tables are given)
+        self.sourceToCRot = 0.0
+        self.sourceToDetector = 0.0
+        self.dPitchX = 0.0
+        self.dPitchY = 0.0
+        self.angles = np.empty(0, dtype=np.float32)
+        self.uV = np.empty(0, dtype=np.float32)
+        self.pitch = False
+        # backend switches kept for API compatibility
+        self.useTorch = True
+        self.useCUDA = True
+        self.useCuPy = False
+        self.x0 = np.empty(0, dtype=np.float32)
+        # state
+        self.subset = 0
+        self.trans = False
+        self.projectorAdded = False
+        self.projectorInitialized = False
+        self.nMultiVolumes = 0
+        self._geom = None
+
+    # ------------------------------------------------------------------ configuration
+    def OMEGAErrorCheck(self):
+        """Configuration checks with the reference's ValueError behaviour (proj.py:1068-1285)."""
+        if self.FOVa_x > 0 and self.FOVa_y == 0:
+            self.FOVa_y = self.FOVa_x
+        if self.projector_type not in (1, 2, 5, 11, 22, 55):
+            raise ValueError("Only projector types 1 (improved Siddon), 2 (orthogonal distance-based) and "
+                             "5 (branchless distance-driven) are available in this implementation!")
+        if self.projector_type in (5, 55) and not self.CT:
+            raise ValueError("Projector type 5 is only supported with CT data!")
+        if self.subsets > 1 and self.subsetType not in (8,) and not self.listmode:
+            raise ValueError("Only subset type 8 (every n-th sinogram/projection) is supported for sinogram data!")
+        if self.subsets < 1:
+            raise ValueError("Number of subsets must be at least 1!")
+        if self.FOVa_x <= 0 or self.axial_fov <= 0:
+            raise ValueError("FOVa_x and axial_fov must be positive!")
+        if int(np.asarray(self.Nx).ravel()[0]) <= 0 or int(np.asarray(self.Ny).ravel()[0]) <= 0 \
+                or int(np.asarray(self.Nz).ravel()[0]) <= 0:
+            raise ValueError("Image dimensions must be positive!")
+        if self.projector_type in (2, 22) and self.tube_width_xy <= 0 and self.tube_width_z <= 0:
+            raise ValueError("Orthogonal distance-based projector needs tube_width_xy or tube_width_z > 0!")
+
+    def _build_tables(self):
+        if self.CT:
+            if self.x.size == 0:
+                if self.angles.size == 0 or self.sourceToCRot <= 0 or self.sourceToDetector <= 0:
+                    raise ValueError("CT data needs either the x/z coordinate tables or angles, sourceToCRot and "
+                                     "sourceToDetector!")
+                ang = np.asarray(self.angles, dtype=np.float64).ravel()
+                if np.max(np.abs(ang)) > 6.0 * math.pi:
+                    ang = ang * (math.pi / 180.0)
+                c, s = np.cos(ang), np.sin(ang)
+                R, D = float(self.sourceToCRot), float(self.sourceToDetector)
+                src = np.stack([R * c, R * s, np.zeros_like(c)], -1)
+                det = np.stack([-(D - R) * c, -(D - R) * s, np.zeros_like(c)], -1)
+                self.x = np.concatenate([src, det], -1).reshape(-1).astype(np.float32)
+                self.z = np.stack([-s * self.dPitchX, c * self.dPitchX], -1).reshape(-1).astype(np.float32)
+                self.nProjections = ang.size
+            if self.nProjections == 0:
+                self.nProjections = self.x.size // 6
+            self.NSinos = self.nProjections
+            self.Ndist, self.Nang = self.nRowsD, self.nColsD
+        elif self.listmode:
+            self.nRowsD, self.nColsD = 1, 1
+            self.nProjections = self.x.size // 6
+            self.NSinos = self.nProjections
+        else:
+            if self.x.size == 0:
+                if self.diameter <= 0 or self.det_per_ring <= 0:
+                    raise ValueError("PET data needs either the x/z coordinate tables or diameter and det_per_ring!")
+                R = self.diameter / 2.0
+                if self.Nang == 0:
+                    self.Nang = self.det_per_ring // 2
+                if self.Ndist == 0:
+                    self.Ndist = self.det_per_ring // 2
+                self.x = G.ring_sinogram_xy(self.Ndist, self.Nang, math.pi * R / self.det_per_ring, R)
+                if self.z.size == 0:
+                    mrd = self.rings - 1 if self.ring_difference < 0 else self.ring_difference
+                    self.z = G.ring_pair_planes_z(self.rings, self.cr_pz, mrd)
+            if self.Ndist == 0 or self.Nang == 0:
+                raise ValueError("Ndist and Nang are required with user-supplied coordinates!")
+            self.nRowsD, self.nColsD = self.Ndist, self.Nang
+            self.NSinos = self.z.size // 2
+            self.nProjections = self.NSinos
+        self.x = np.ascontiguousarray(self.x, dtype=np.float32).ravel()
+        self.z = np.ascontiguousarray(self.z, dtype=np.float32).ravel()
+
+    def addProjector(self):
+        """Validate the configuration, place the grid and form the subsets (proj.py:520-1066)."""
+        self.OMEGAErrorCheck()
+        self._build_tables()
+        Nx, Ny, Nz = (int(np.asarray(v).ravel()[0]) for v in (self.Nx, self.Ny, self.Nz))
+        self.grid = G.Grid.from_fov(Nx, Ny, Nz, float(self.FOVa_x), float(self.FOVa_y), float(self.axial_fov),
+                                    self.oOffsetX, self.oOffsetY, self.oOffsetZ)
+        g = self.grid
+        self.dx = np.array([g.dx], dtype=np.float32)
+        self.dy = np.array([g.dy], dtype=np.float32)
+        self.dz = np.array([g.dz], dtype=np.float32)
+        self.bx = np.array([g.bx], dtype=np.float32)
+        self.by = np.array([g.by], dtype=np.float32)
+        self.bz = np.array([g.bz], dtype=np.float32)
+        self.Nx = np.array(Nx, dtype=np.uint32, ndmin=1)
+        self.Ny = np.array(Ny, dtype=np.uint32, ndmin=1)
+        self.Nz = np.array(Nz, dtype=np.uint32, ndmin=1)
+        self.N = self.Nx.astype(np.uint64) * self.Ny.astype(np.uint64) * self.Nz.astype(np.uint64)
+        if self.listmode:
+            nlor = self.x.size // 6
+            per_subset = np.full(self.subsets, nlor // self.subsets, dtype=np.int64)
+            per_subset[-1] = nlor - per_subset[:-1].sum()
+            self.index = np.arange(nlor, dtype=np.int64)
+            kerroin = 1
+        else:
+            self.index, per_subset = G.subset_index_type8(self.nProjections, self.subsets if self.subsets > 1 else 1)
+            if self.subsets > 1:
+                if self.CT:
+                    self.x = np.ascontiguousarray(self.x.reshape(-1, 6)[self.index].reshape(-1))
+                    w = 6 if self.pitch else 2
+                    self.z = np.ascontiguousarray(self.z.reshape(-1, w)[self.index].reshape(-1))
+                else:
+                    self.z = np.ascontiguousarray(self.z.reshape(-1, 2)[self.index].reshape(-1))
+            kerroin = self.nRowsD * self.nColsD
+        self.nMeas = np.insert(np.cumsum(per_subset), 0, 0).astype(np.int64)
+        self.nMeasSubset = (np.diff(self.nMeas) * kerroin).reshape(-1, 1).astype(np.int64)
+        self.nProjSubset = np.diff(self.nMeas).reshape(-1, 1).astype(np.int64)
+        self.nTotMeas = self.nMeas * kerroin
+        if self.x0.size == 0:
+            self.x0 = np.full(int(self.N[0]), 1e-4 if self.CT else 1.0, dtype=np.float32)
+        self.x0 = np.asarray(self.x0, dtype=np.float32).ravel(order="F")
+        if self.subsets > 1:
+            self.subset = 0
+        self.projectorAdded = True
+
+    def subset_order(self, data):
+        """Reorder full measurement data (planes in scanner order) into subset order."""
+        per = self.nRowsD * self.nColsD
+        d = np.asarray(data, dtype=np.float32).reshape(-1, per)
+        return np.ascontiguousarray(d[self.index].reshape(-1))
+
+    def _descriptor(self):
+        d = _lib.GeomDesc()
+        d.struct_size = C.sizeof(_lib.GeomDesc)
+        pt = self.projector_type % 10 if self.projector_type > 10 else self.projector_type
+        d.projector_type = pt
+        d.mode = _lib.MODE_CT_FLAT if self.CT else (_lib.MODE_LIST6 if self.listmode else _lib.MODE_PET_SINO)
+        g = self.grid
+        d.Nx, d.Ny, d.Nz = g.Nx, g.Ny, g.Nz
+        d.dx, d.dy, d.dz = g.dx, g.dy, g.dz
+        d.bx, d.by, d.bz = g.bx, g.by, g.bz
+        d.size_x, d.size_y = int(self.nRowsD), int(self.nColsD)
+        d.n_proj = int(self.nProjections)
+        d.ct_pitch = int(bool(self.pitch))
+        d.dPitchZ = float(self.dPitchY)
+        d.global_factor = float(self.global_correction_factor)
+        d.x, d.x_len = self.x.ctypes.data, self.x.size
+        d.z, d.z_len = self.z.ctypes.data, self.z.size
+        keep = [self.x, self.z]
+        if self.normalization_correction and self.normalization.size > 0:
+            nrm = self.subset_order(self.normalization) if not self.listmode else \
+                np.ascontiguousarray(self.normalization, dtype=np.float32).ravel()
+            keep.append(nrm)
+            d.norm, d.norm_len = nrm.ctypes.data, nrm.size
+        if pt == 2:
+            width = self.tube_width_z if self.tube_width_z > 0 else self.tube_width_xy
+            d.orth_width = float(width)
+            d.orth_3d = int(self.tube_width_z > 0)
+        d.dPitchX = float(self.dPitchX)
+        d.dPitchY = float(self.dPitchY)
+        return d, keep
+
+    def initProj(self):
+        """Create the device-side geometry (proj.py:1913 -> init.py:39)."""
+        if not self.projectorAdded:
+            self.addProjector()
+        d, keep = self._descriptor()
+        self._geom = Geometry(d, keep)
+        self.projectorInitialized = True
+
+    # ------------------------------------------------------------------ operators
+    def _range(self, subset):
+        if subset == -1:
+            subset = self.subset
+        if self.subsets == 1:
+            subset = 0
+        first = int(self.nMeas[subset])
+        nproj = int(self.nMeas[subset + 1] - self.nMeas[subset])
+        per = 1 if self.listmode else self.nRowsD * self.nColsD
+        return first, nproj * per
+
+    def forwardProject(self, f, subset=-1):
+        """y = A_subset f (projfunctions.forwardProjection, projfunctions.py:86)."""
+        import torch
+        if not self.projectorInitialized:
+            self.initProj()
+        if isinstance(f, list):
+            raise ValueError("Multi-resolution volumes are not supported by this implementation!")
+        first, n = self._range(subset)
+        f = self._as_image(f)
+        y = torch.empty(n, dtype=torch.float32, device=f.device)
+        _lib.call("omg_forward_project", self._geom.handle, first, n, C.c_void_p(f.data_ptr()),
+                  C.c_void_p(y.data_ptr()), _stream_ptr(torch))
+        return y
+
+    def backwardProject(self, y, subset=-1, sens=None):
+        """f = A_subset^T y (projfunctions.backwardProjection, projfunctions.py:735).
+
+        ``sens`` (optional, flat float32 image tensor) accumulates A^T 1 in the same pass.
+        """
+        import torch
+        if not self.projectorInitialized:
+            self.initProj()
+        first, n = self._range(subset)
+        y = self._as_meas(y, n)
+        f = torch.zeros(int(self.N[0]), dtype=torch.float32, device=y.device)
+        sp = C.c_void_p(0)
+        if sens is not None:
+            if sens.dtype != torch.float32 or not sens.is_contiguous() or sens.numel() != int(self.N[0]):
+                raise ValueError("sens must be a contiguous float32 tensor with N elements")
+            sp = C.c_void_p(sens.data_ptr())
+        _lib.call("omg_backward_project", self._geom.handle, first, n, C.c_void_p(y.data_ptr()),
+                  C.c_void_p(f.data_ptr()), sp, _stream_ptr(torch))
+        return f
+
+    def T(self):
+        self.trans = True
+        return self
+
+    def __mul__(self, B):
+        if self.trans:
+            self.trans = False
+            return self.backwardProject(B, self.subset)
+        return self.forwardProject(B, self.subset)
+
+    def __rmul__(self, B):
+        return self
+
+    # ------------------------------------------------------------------ helpers
+    def _as_image(self, f):
+        import torch
+        if not isinstance(f, torch.Tensor):
+            f = torch.as_tensor(np.asarray(f, dtype=np.float32).ravel(order="F"), device="cuda")
+        if f.dtype != torch.float32:
+            raise ValueError("images must be float32")
+        if not f.is_cuda:
+            raise ValueError("images must be device tensors (useTorch=True, useCUDA=True)")
+        f = f.reshape(-1).contiguous()
+        if f.numel() != int(self.N[0]):
+            raise ValueError(f"image has {f.numel()} elements, expected {int(self.N[0])}")
+        return f
+
+    def _as_meas(self, y, n):
+        import torch
+        if not isinstance(y, torch.Tensor):
+            y = torch.as_tensor(np.asarray(y, dtype=np.float32).ravel(), device="cuda")
+        if y.dtype != torch.float32:
+            raise ValueError("measurements must be float32")
+        if not y.is_cuda:
+            raise ValueError("measurements must be device tensors (useTorch=True, useCUDA=True)")
+        y = y.reshape(-1).contiguous()
+        if y.numel() != n:
+            raise ValueError(f"measurement vector has {y.numel()} elements, expected {n} for this subset")
+        return y
+
+    @property
+    def geometry(self) -> Geometry:
+        if not self.projectorInitialized:
+            self.initProj()
+        return self._geom

@@ -1,0 +1,47 @@
+"""The measured workloads, built from their stated sizes with seeded synthetic data.
+
+``pet2d_osem``: 2-D plane PET OSEM with improved Siddon. Ring radius 400 mm, 63 direct sinogram
+planes of 128 radial bins x 160 angles over 180 degrees (2.5 mm radial pitch); image 128x128x63
+with 2.5 mm voxels; phantom of 8 seeded ellipsoids, activity U(0.5, 2.0) on a 0.1 background;
+data Poisson(A x) with A x scaled to 5e6 total counts; 8 subsets (type 8) x 4 iterations.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import geometry as G
+from . import phantoms
+from .projector import projectorClass
+
+
+def pet2d_projector(subsets: int = 8) -> projectorClass:
+    grid = G.Grid.from_fov(128, 128, 63, 320.0, 320.0, 157.5)
+    sc = G.pet_2d_planes(grid, radius=400.0, n_radial=128, n_angles=160, radial_pitch=2.5)
+    p = projectorClass()
+    p.Nx, p.Ny, p.Nz = 128, 128, 63
+    p.FOVa_x, p.FOVa_y, p.axial_fov = 320.0, 320.0, 157.5
+    p.projector_type = 1
+    p.x = sc.xy_table()
+    p.z = sc.z
+    p.Ndist, p.Nang = sc.n_radial, sc.n_angles
+    p.subsets = subsets
+    p.subsetType = 8
+    p.addProjector()
+    return p
+
+
+def pet2d_data(p: projectorClass, seed: int = 0, total_counts: float = 5e6, device="cuda"):
+    """Phantom and Poisson data (subset order) for the 2-D plane PET workload.
+
+    The noiseless projection is computed with the library's forward projector over all subsets.
+    Returns (phantom numpy, measurements as a float32 device tensor in subset order).
+    """
+    import torch
+    ph = phantoms.ellipsoids(p.grid, seed=seed, n_ell=8, background=0.1)
+    x = torch.as_tensor(ph, device=device)
+    parts = [p.forwardProject(x, k) for k in range(p.subsets)]
+    ax = torch.cat(parts).double().cpu().numpy()
+    scale = total_counts / ax.sum()
+    rng = np.random.default_rng(seed + 1)
+    y = rng.poisson(ax * scale).astype(np.float32)
+    return ph, torch.as_tensor(y, device=device)

@@ -1,0 +1,240 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and the reference checker (_ref/libomega_ref.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py. The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libomega_ref.so")
+
+_lib = None
+_ref = None
+
+MODE_PET, MODE_CT, MODE_LIST = 0, 1, 2
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"] + (["-B"] if force else []))
+
+
+def build_ref(force: bool = False) -> bool:
+    """Compile the reference checker if the reference sources are present; returns availability."""
+    src = "/root/reference/source/cpp/projector_functions.h"
+    if os.path.exists(src) and (force or not os.path.exists(REF_PATH)):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"] + (["-B"] if force else []))
+    return os.path.exists(REF_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        for T, suf, p in ((C.c_float, "f32", _f32p), (C.c_double, "f64", _f64p)):
+            geo = [C.c_void_p, p, p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, T, C.c_int32, T, C.c_void_p, C.c_int64]
+            f = getattr(_lib, "oracle_siddon_fp_" + suf)
+            f.argtypes = geo + [p, p, p, p, C.c_int]
+            f.restype = C.c_int
+            f = getattr(_lib, "oracle_siddon_bp_" + suf)
+            f.argtypes = geo + [p, p, p, p, C.c_void_p, C.c_int]
+            f.restype = C.c_int
+            f = getattr(_lib, "oracle_siddon_trace_" + suf)
+            f.argtypes = geo + [p, p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+            f.restype = C.c_int64
+            f = getattr(_lib, "oracle_rdp_gradient_" + suf)
+            f.argtypes = [p, p, C.c_int, C.c_int, C.c_int, T, T, T, C.c_int]
+            f.restype = C.c_int
+        _lib.oracle_ratio_f32.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
+        _lib.oracle_sens_clamp_f32.argtypes = [_f32p, C.c_int64, C.c_float]
+        _lib.oracle_osem_update_f32.argtypes = [_f32p, _f32p, _f32p, C.c_int64, C.c_float, C.c_int]
+        _lib.oracle_osl_update_f32.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int64, C.c_float, C.c_int]
+    return _lib
+
+
+def ref():
+    """The reference checker, or None when it was not built (e.g. no reference sources)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        _ref = C.CDLL(REF_PATH)
+        for T, suf, p in ((C.c_float, "f32", _f32p), (C.c_double, "f64", _f64p)):
+            f = getattr(_ref, "ref_siddon_" + suf)
+            f.argtypes = [C.c_void_p, p, p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, T, C.c_int32, T,
+                          C.c_void_p, C.c_uint32, C.c_int64, p, p, p, p, C.c_void_p, C.c_int, C.c_int]
+            f.restype = C.c_int
+        _ref.ref_rdp_f32.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
+    return _ref
+
+
+class SiddonCase:
+    """Everything a Siddon call needs besides the data vectors."""
+
+    def __init__(self, grid, size_x, size_y, x, z, mode=MODE_PET, plane_offset=0, dPitchZ=0.0, ct_pitch=0,
+                 global_factor=1.0, norm=None, n_proj=None, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.n3 = np.ascontiguousarray(grid.n3())
+        self.d3 = grid.d3(self.dtype)
+        self.b3 = grid.b3(self.dtype)
+        self.grid = grid
+        self.size_x, self.size_y = int(size_x), int(size_y)
+        self.x = np.ascontiguousarray(x, dtype=self.dtype)
+        self.z = np.ascontiguousarray(z, dtype=self.dtype)
+        self.mode = mode
+        self.plane_offset = int(plane_offset)
+        self.dPitchZ = dPitchZ
+        self.ct_pitch = ct_pitch
+        self.global_factor = global_factor
+        self.norm = None if norm is None else np.ascontiguousarray(norm, dtype=self.dtype)
+        self.n_proj = n_proj if n_proj is not None else max(1, self.z.size // 2)
+
+    @property
+    def suf(self):
+        return "f32" if self.dtype == np.float32 else "f64"
+
+    def _geo(self, nmeas):
+        normp = None if self.norm is None else self.norm.ctypes.data
+        return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
+                self.dPitchZ, self.ct_pitch, self.global_factor, normp, nmeas]
+
+    # ---- oracle -------------------------------------------------------------------------
+    def fp(self, img, nmeas, nthreads=0):
+        y = np.zeros(nmeas, dtype=self.dtype)
+        getattr(lib(), "oracle_siddon_fp_" + self.suf)(*self._geo(nmeas), self.x, self.z,
+                                                       np.ascontiguousarray(img, dtype=self.dtype), y, nthreads)
+        return y
+
+    def bp(self, yin, nmeas, sens=False, nthreads=0):
+        n = self.grid.N
+        rhs = np.zeros(n, dtype=self.dtype)
+        s = np.zeros(n, dtype=self.dtype) if sens else None
+        getattr(lib(), "oracle_siddon_bp_" + self.suf)(*self._geo(nmeas), self.x, self.z,
+                                                       np.ascontiguousarray(yin, dtype=self.dtype), rhs,
+                                                       None if s is None else s.ctypes.data, nthreads)
+        return (rhs, s) if sens else rhs
+
+    def trace(self, nmeas):
+        """Per-LOR voxel index lists (traversal order), lengths and LOR weights."""
+        f = getattr(lib(), "oracle_siddon_trace_" + self.suf)
+        counts = np.zeros(nmeas, dtype=np.int64)
+        w = np.zeros(nmeas, dtype=self.dtype)
+        total = f(*self._geo(nmeas), self.x, self.z, counts.ctypes.data, 0, None, None, w.ctypes.data)
+        idx = np.zeros(max(total, 1), dtype=np.uint32)
+        ln = np.zeros(max(total, 1), dtype=self.dtype)
+        f(*self._geo(nmeas), self.x, self.z, None, total, idx.ctypes.data, ln.ctypes.data, None)
+        return counts, idx[:total], ln[:total], w
+
+    # ---- reference checker ---------------------------------------------------------------
+    def ref_fp(self, img, nmeas, nthreads=0):
+        r = ref()
+        y = np.zeros(nmeas, dtype=self.dtype)
+        g = self._geo(nmeas)
+        getattr(r, "ref_siddon_" + self.suf)(*g[:-1], self.n_proj, nmeas, self.x, self.z,
+                                             np.ascontiguousarray(img, dtype=self.dtype).copy(), y, None, 1, nthreads)
+        return y
+
+    def ref_bp(self, yin, nmeas, sens=False, nthreads=0):
+        r = ref()
+        n = self.grid.N
+        rhs = np.zeros(n, dtype=self.dtype)
+        s = np.zeros(n, dtype=self.dtype) if sens else None
+        g = self._geo(nmeas)
+        getattr(r, "ref_siddon_" + self.suf)(*g[:-1], self.n_proj, nmeas, self.x, self.z,
+                                             np.ascontiguousarray(yin, dtype=self.dtype).copy(), rhs,
+                                             None if s is None else s.ctypes.data, 2, nthreads)
+        return (rhs, s) if sens else rhs
+
+
+def rdp_gradient(u, shape, gamma, epps, beta, grad=None, dtype=np.float32, nthreads=0):
+    Nx, Ny, Nz = shape
+    suf = "f32" if np.dtype(dtype) == np.float32 else "f64"
+    u = np.ascontiguousarray(u, dtype=dtype)
+    g = np.zeros_like(u) if grad is None else np.ascontiguousarray(grad, dtype=dtype).copy()
+    getattr(lib(), "oracle_rdp_gradient_" + suf)(g, u, Nx, Ny, Nz, gamma, epps, beta, nthreads)
+    return g
+
+
+def ref_rdp_gradient(u, shape, gamma, epps, beta, grad=None):
+    Nx, Ny, Nz = shape
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    g = np.zeros_like(u) if grad is None else np.ascontiguousarray(grad, dtype=np.float32).copy()
+    ref().ref_rdp_f32(g, u, Nx, Ny, Nz, gamma, epps, beta)
+    return g
+
+
+def ratio(y, yhat, epps):
+    r = np.ascontiguousarray(yhat, dtype=np.float32).copy()
+    lib().oracle_ratio_f32(np.ascontiguousarray(y, dtype=np.float32), r, r.size, epps)
+    return r
+
+
+def sens_clamp(s, epps):
+    s = np.ascontiguousarray(s, dtype=np.float32).copy()
+    lib().oracle_sens_clamp_f32(s, s.size, epps)
+    return s
+
+
+def osem_update(x, S, rhs, epps, positivity=True):
+    x = np.ascontiguousarray(x, dtype=np.float32).copy()
+    lib().oracle_osem_update_f32(x, np.ascontiguousarray(S, dtype=np.float32),
+                                 np.ascontiguousarray(rhs, dtype=np.float32), x.size, epps, int(positivity))
+    return x
+
+
+def osl_update(x, S, dU, rhs, epps, positivity=True):
+    x = np.ascontiguousarray(x, dtype=np.float32).copy()
+    lib().oracle_osl_update_f32(x, np.ascontiguousarray(S, dtype=np.float32), np.ascontiguousarray(dU, dtype=np.float32),
+                                np.ascontiguousarray(rhs, dtype=np.float32), x.size, epps, int(positivity))
+    return x
+
+
+def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter, x0, epps=1e-5, beta=0.0,
+         gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None):
+    """OSEM / OSL-OSEM(RDP) on the CPU in the reference's step order (see omega_amd/recon.py).
+
+    ``meas`` is the full measurement vector in subset order. With ``use_ref`` the forward and
+    backward projections come from the reference build (oracle/_ref), the elementwise steps from
+    the oracle. ``subsets_to_run`` limits the run to the first that many subset-iterations.
+    Returns the image (float32).
+    """
+    per = size_x * size_y
+    starts = np.concatenate([[0], np.cumsum(nmeas_planes)]).astype(np.int64)
+    x = np.ascontiguousarray(x0, dtype=np.float32).copy()
+    S = [None] * len(nmeas_planes)
+    done = 0
+    shape = (grid.Nx, grid.Ny, grid.Nz)
+    for it in range(n_iter):
+        for k in range(len(nmeas_planes)):
+            if subsets_to_run is not None and done >= subsets_to_run:
+                return x
+            n = int(nmeas_planes[k]) * per
+            case = SiddonCase(grid, size_x, size_y, xy, z_subset_ordered, plane_offset=int(starts[k]))
+            yk = meas[starts[k] * per: starts[k] * per + n]
+            yhat = case.ref_fp(x, n, nthreads) if use_ref else case.fp(x, n, nthreads)
+            r = ratio(yk, yhat, epps)
+            if it == 0:
+                rhs, s = (case.ref_bp if use_ref else case.bp)(r, n, sens=True, nthreads=nthreads)
+                S[k] = sens_clamp(s, epps)
+            else:
+                rhs = (case.ref_bp if use_ref else case.bp)(r, n, nthreads=nthreads)
+            if beta != 0.0:
+                dU = rdp_gradient(x, shape, gamma, epps, beta, nthreads=nthreads)
+                x = osl_update(x, S[k], dU, rhs, epps, positivity)
+            else:
+                x = osem_update(x, S[k], rhs, epps, positivity)
+            done += 1
+    return x

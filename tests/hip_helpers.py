@@ -1,0 +1,83 @@
+"""Helpers that drive libomega_hip.so through its C ABI from tests."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from omega_amd import _lib
+from omega_amd.projector import Geometry
+
+
+def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
+              ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_3d=0, dPitchX=0.0, dPitchY=0.0):
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
+    d = _lib.GeomDesc()
+    d.struct_size = C.sizeof(_lib.GeomDesc)
+    d.projector_type = projector_type
+    d.mode = mode
+    d.Nx, d.Ny, d.Nz = grid.Nx, grid.Ny, grid.Nz
+    d.dx, d.dy, d.dz = grid.dx, grid.dy, grid.dz
+    d.bx, d.by, d.bz = grid.bx, grid.by, grid.bz
+    d.size_x, d.size_y = size_x, size_y
+    if n_proj is None:
+        n_proj = (x.size // 6) if mode != _lib.MODE_PET_SINO else z.size // 2
+    d.n_proj = n_proj
+    d.ct_pitch = ct_pitch
+    d.dPitchZ = dPitchZ
+    d.global_factor = global_factor
+    d.x, d.x_len = x.ctypes.data, x.size
+    d.z, d.z_len = z.ctypes.data, z.size
+    keep = [x, z]
+    if norm is not None:
+        norm = np.ascontiguousarray(norm, dtype=np.float32)
+        keep.append(norm)
+        d.norm, d.norm_len = norm.ctypes.data, norm.size
+    d.orth_width = orth_width
+    d.orth_3d = orth_3d
+    d.dPitchX = dPitchX
+    d.dPitchY = dPitchY
+    return Geometry(d, keep)
+
+
+def ptr(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def fp(geom, first, n, img_t):
+    import torch
+    y = torch.empty(n, dtype=torch.float32, device=img_t.device)
+    _lib.call("omg_forward_project", geom.handle, first, n, ptr(img_t), ptr(y), None)
+    return y
+
+
+def bp(geom, first, n, y_t, N, sens=False):
+    import torch
+    rhs = torch.zeros(N, dtype=torch.float32, device=y_t.device)
+    s = torch.zeros(N, dtype=torch.float32, device=y_t.device) if sens else None
+    _lib.call("omg_backward_project", geom.handle, first, n, ptr(y_t), ptr(rhs), ptr(s), None)
+    return (rhs, s) if sens else rhs
+
+
+def trace(geom, first, n, device):
+    """Per-measurement voxel lists from the GPU traversal: (counts, idx, len)."""
+    import torch
+    counts = torch.zeros(n, dtype=torch.int32, device=device)
+    _lib.call("omg_siddon_trace", geom.handle, first, n, ptr(counts), None, None, None, None)
+    c = counts.to(torch.int64)
+    offs = torch.zeros(n, dtype=torch.int64, device=device)
+    if n > 1:
+        offs[1:] = torch.cumsum(c, 0)[:-1]
+    total = int(c.sum().item())
+    idx = torch.zeros(max(total, 1), dtype=torch.int32, device=device)
+    ln = torch.zeros(max(total, 1), dtype=torch.float32, device=device)
+    _lib.call("omg_siddon_trace", geom.handle, first, n, ptr(counts), ptr(offs), ptr(idx), ptr(ln), None)
+    return counts.cpu().numpy().astype(np.int64), idx[:total].cpu().numpy().view(np.uint32), ln[:total].cpu().numpy()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))

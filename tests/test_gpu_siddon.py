@@ -1,0 +1,165 @@
+"""GPU parity of the improved Siddon projector (projector type 1) against the CPU oracle.
+
+Bar: forward projections and traversal lists (voxel indices, lengths) bit-identical to the
+oracle, which is itself bit-identical to the reference CPU build (test_oracle_ref.py,
+test_golden.py). Backprojections (unordered float atomics, as in the reference) within
+1e-5 relative L2 of the FP32 oracle; the FP32 error against the FP64 oracle is printed.
+"""
+import numpy as np
+import pytest
+
+from omega_amd import _lib
+from omega_amd import geometry as G
+from oracle import oracle as O
+from tests import cases
+from tests.hip_helpers import bp, fp, make_geom, rel_l2, trace
+
+pytestmark = pytest.mark.gpu
+
+BP_TOL = 1e-5
+
+
+def _img(N, seed):
+    return np.random.default_rng(seed).random(N).astype(np.float32) + 0.05
+
+
+def test_list_mode_tiny_grid_bitexact(cuda):
+    import torch
+    grid = cases.tiny_grid()
+    x = cases.list_rays(grid, 400, seed=1)
+    n = x.size // 6
+    print(f"[siddon list-mode tiny] n={n}", flush=True)
+    case = O.SiddonCase(grid, 1, 1, x, np.zeros(2, np.float32), mode=O.MODE_LIST)
+    img = _img(grid.N, 2)
+    geom = make_geom(grid, x, None, 1, 1, mode=_lib.MODE_LIST6)
+    y = fp(geom, 0, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    y_ref = case.fp(img, n)
+    assert np.array_equal(y, y_ref), f"max |diff| {np.abs(y - y_ref).max()}"
+    c_g, i_g, l_g = trace(geom, 0, n, cuda)
+    c_o, i_o, l_o, _ = case.trace(n)
+    assert np.array_equal(c_g, c_o)
+    assert np.array_equal(i_g, i_o)
+    assert np.array_equal(l_g, l_o)
+    yin = _img(n, 3)
+    rhs, sens = bp(geom, 0, n, torch.from_numpy(yin).to(cuda), grid.N, sens=True)
+    rhs_o, sens_o = case.bp(yin, n, sens=True)
+    assert rel_l2(rhs.cpu().numpy(), rhs_o) < BP_TOL
+    assert rel_l2(sens.cpu().numpy(), sens_o) < BP_TOL
+
+
+@pytest.mark.parametrize("subset", [0, 3, 7])
+def test_c0_subset_fp_bitexact_bp_close(cuda, subset):
+    import torch
+    grid, sc = cases.c0()
+    xy = sc.xy_table()
+    idx, nmeas = G.subset_index_type8(sc.n_planes, 8)
+    zs = np.ascontiguousarray(sc.z.reshape(-1, 2)[idx].reshape(-1))
+    per = sc.n_radial * sc.n_angles
+    first = int(nmeas[:subset].sum())
+    n = int(nmeas[subset]) * per
+    print(f"[siddon c0] subset={subset} first={first} n={n}", flush=True)
+    case = O.SiddonCase(grid, sc.n_radial, sc.n_angles, xy, zs, plane_offset=first)
+    geom = make_geom(grid, xy, zs, sc.n_radial, sc.n_angles)
+    img = _img(grid.N, 10 + subset)
+    y = fp(geom, first, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    y_ref = case.fp(img, n)
+    assert np.array_equal(y, y_ref), f"{np.count_nonzero(y != y_ref)} of {n} differ"
+    yin = _img(n, 20 + subset)
+    rhs, sens = bp(geom, first, n, torch.from_numpy(yin).to(cuda), grid.N, sens=True)
+    rhs_o, sens_o = case.bp(yin, n, sens=True)
+    e = rel_l2(rhs.cpu().numpy(), rhs_o)
+    case64 = O.SiddonCase(grid, sc.n_radial, sc.n_angles, xy, zs, plane_offset=first, dtype=np.float64)
+    rhs64 = case64.bp(yin.astype(np.float64), n)
+    print(f"  bp rel-L2 vs fp32 oracle {e:.2e}; vs fp64 oracle {rel_l2(rhs.cpu().numpy(), rhs64):.2e}; "
+          f"fp vs fp64 {rel_l2(y, case64.fp(img.astype(np.float64), n)):.2e}")
+    assert e < BP_TOL
+    assert rel_l2(sens.cpu().numpy(), sens_o) < BP_TOL
+
+
+def test_c0_trace_lists_bitexact(cuda):
+    grid, sc = cases.c0()
+    xy = sc.xy_table()
+    per = sc.n_radial * sc.n_angles
+    case = O.SiddonCase(grid, sc.n_radial, sc.n_angles, xy, sc.z, plane_offset=31)
+    geom = make_geom(grid, xy, sc.z, sc.n_radial, sc.n_angles)
+    c_g, i_g, l_g = trace(geom, 31, per, cuda)
+    c_o, i_o, l_o, _ = case.trace(per)
+    assert np.array_equal(c_g, c_o)
+    assert np.array_equal(i_g, i_o)
+    assert np.array_equal(l_g, l_o)
+
+
+def test_3d_ring_pairs_fp_bitexact(cuda):
+    import torch
+    grid, sc = cases.c2_small()
+    xy = sc.xy_table()
+    per = sc.n_radial * sc.n_angles
+    n = sc.n_planes * per
+    print(f"[siddon 3d rings] planes={sc.n_planes} n={n}", flush=True)
+    case = O.SiddonCase(grid, sc.n_radial, sc.n_angles, xy, sc.z)
+    geom = make_geom(grid, xy, sc.z, sc.n_radial, sc.n_angles)
+    img = _img(grid.N, 5)
+    y = fp(geom, 0, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    assert np.array_equal(y, case.fp(img, n))
+    yin = _img(n, 6)
+    rhs = bp(geom, 0, n, torch.from_numpy(yin).to(cuda), grid.N)
+    assert rel_l2(rhs.cpu().numpy(), case.bp(yin, n)) < BP_TOL
+
+
+def test_ct_flat_panel_fp_bitexact(cuda):
+    import torch
+    grid, x, uv, rows, cols, pitch = cases.ct_small()
+    nv = x.size // 6
+    n = nv * rows * cols
+    case = O.SiddonCase(grid, rows, cols, x, uv, mode=O.MODE_CT, dPitchZ=pitch, n_proj=nv)
+    geom = make_geom(grid, x, uv, rows, cols, mode=_lib.MODE_CT_FLAT, n_proj=nv, dPitchZ=pitch)
+    img = _img(grid.N, 7)
+    y = fp(geom, 0, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    assert np.array_equal(y, case.fp(img, n))
+    yin = _img(n, 8)
+    rhs = bp(geom, 0, n, torch.from_numpy(yin).to(cuda), grid.N)
+    assert rel_l2(rhs.cpu().numpy(), case.bp(yin, n)) < BP_TOL
+
+
+def test_adjoint_identity_c0_subset(cuda):
+    """<A x, y> == <x, A^T y> in FP64 accumulation (FP and BP trace identical rays)."""
+    import torch
+    grid, sc = cases.c0()
+    xy = sc.xy_table()
+    per = sc.n_radial * sc.n_angles
+    geom = make_geom(grid, xy, sc.z, sc.n_radial, sc.n_angles)
+    n = 8 * per
+    for seed in range(3):
+        x = _img(grid.N, 100 + seed)
+        yv = _img(n, 200 + seed)
+        ax = fp(geom, 4, n, torch.from_numpy(x).to(cuda)).cpu().numpy().astype(np.float64)
+        aty = bp(geom, 4, n, torch.from_numpy(yv).to(cuda), grid.N).cpu().numpy().astype(np.float64)
+        lhs = float(ax @ yv.astype(np.float64))
+        rhs = float(x.astype(np.float64) @ aty)
+        assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_norm_and_global_factor(cuda):
+    import torch
+    grid, sc = cases.c0()
+    xy = sc.xy_table()
+    per = sc.n_radial * sc.n_angles
+    norm = np.random.default_rng(4).uniform(0.5, 1.5, size=sc.n_planes * per).astype(np.float32)
+    geom = make_geom(grid, xy, sc.z, sc.n_radial, sc.n_angles, norm=norm, global_factor=1.7)
+    case = O.SiddonCase(grid, sc.n_radial, sc.n_angles, xy, sc.z, plane_offset=10, norm=norm[10 * per:12 * per],
+                        global_factor=1.7)
+    img = _img(grid.N, 9)
+    y = fp(geom, 10, 2 * per, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    assert np.array_equal(y, case.fp(img, 2 * per))
+
+
+def test_errors_are_reported(cuda):
+    import torch
+    grid, sc = cases.c0()
+    geom = make_geom(grid, sc.xy_table(), sc.z, sc.n_radial, sc.n_angles)
+    img = torch.zeros(grid.N, device=cuda)
+    with pytest.raises(_lib.OmegaHipError) as e:
+        fp(geom, 62, 2 * sc.n_radial * sc.n_angles, img)   # runs past the last plane
+    assert e.value.code == _lib.OMG_ERR_ARG
+    with pytest.raises(_lib.OmegaHipError):
+        fp(geom, 0, 100, img)                              # not a whole plane
