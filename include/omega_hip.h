@@ -62,6 +62,8 @@ typedef struct omg_geom_desc {
     /* branchless distance-driven projector (projector_type 5); CT mode only */
     float dPitchX;             /* transaxial detector pitch */
     float dPitchY;             /* axial detector pitch */
+    /* launch tuning: reserved[0] != 0 routes all Siddon calls through the one-lane-per-LOR kernels;
+     * reserved[1] > 0 sets the workgroups per plane of the direct-plane (LDS) kernels. Zero = automatic. */
     int32_t reserved[8];
 } omg_geom_desc;
 

@@ -32,6 +32,19 @@ int check_range(const omg_geom* g, int64_t first, int64_t n) {
     return OMG_OK;
 }
 
+int scratch_reserve(omg_geom* g, size_t floats) {
+    if (g->scratch_floats >= floats) return OMG_OK;
+    if (g->scratch) {
+        OMG_CHECK_HIP(hipDeviceSynchronize());
+        OMG_CHECK_HIP(hipFree(g->scratch));
+        g->scratch = nullptr;
+        g->scratch_floats = 0;
+    }
+    OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->scratch), floats * sizeof(float)));
+    g->scratch_floats = floats;
+    return OMG_OK;
+}
+
 }  // namespace omg
 
 using namespace omg;
@@ -83,6 +96,8 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     omg_geom* g = new (std::nothrow) omg_geom();
     OMG_REQUIRE(g != nullptr, "out of host memory");
     g->d = *d;
+    g->force_generic = d->reserved[0] != 0;
+    g->planar_groups = d->reserved[1];
     hipGetDevice(&g->device);
     // far FOV edges in the reference's float arithmetic (projector_functions.h:1331-1333)
     g->bmaxz = d->bz + (float)d->Nz * d->dz;
@@ -114,6 +129,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->x_dev);
     hipFree(g->z_dev);
     hipFree(g->norm_dev);
+    hipFree(g->scratch);
     bdd_release(g);
     orth_release(g);
     delete g;

@@ -12,6 +12,13 @@ struct omg_geom {
     int device = 0;
     // Planar fast path: every plane's LORs have zs == zd and the sinogram is in PET mode.
     bool planar = false;
+    // Route every call through the one-lane-per-LOR kernels (set from omg_geom_desc.reserved[0]).
+    bool force_generic = false;
+    // Workgroups per plane for the planar kernels (omg_geom_desc.reserved[1]; 0 = automatic).
+    int32_t planar_groups = 0;
+    // Library-owned scratch (floats) for partial images; grown on demand, freed with the handle.
+    float* scratch = nullptr;
+    size_t scratch_floats = 0;
 };
 
 namespace omg {
@@ -31,6 +38,7 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.global_factor = g->d.global_factor;
     a.x = g->x_dev; a.z = g->z_dev;
     a.norm = nullptr;
+    a.pitch = g->d.Nx;
     if (g->norm_dev) {
         const int64_t per = (g->d.mode == MODE_LIST6) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
         a.norm = g->norm_dev + first * per;
@@ -40,5 +48,8 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
 
 // Validates a [first, first + n) measurement range against the tables; returns OMG_OK or OMG_ERR_ARG.
 int check_range(const omg_geom* g, int64_t first, int64_t n);
+
+// Ensures g->scratch holds at least `floats` floats (hipMalloc on growth; synchronises the device then).
+int scratch_reserve(omg_geom* g, size_t floats);
 
 }  // namespace omg

@@ -15,20 +15,11 @@ namespace omg {
 constexpr int kBlock = 256;
 
 // ---- forward projection -------------------------------------------------------------------------
-// The gather of step i is issued before the accumulation of step i-1 (a two-deep software
-// pipeline), while the sum itself keeps the traversal order of the reference.
 __global__ __launch_bounds__(kBlock) void k_siddon_fp(SidArgs a, const float* __restrict__ img, float* __restrict__ y) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (lo >= a.nmeas) return;
-    float ax = 0.f, w = 0.f;
-    float pv = 0.f, pl = 0.f;  // pending value and length
-    bool pend = false;
-    const bool hit = siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
-        const float v = img[ind];
-        if (pend) ax += pl * pv;
-        pv = v; pl = len; pend = true;
-    });
-    if (pend) ax += pl * pv;
+    float ax, w = 0.f;
+    const bool hit = siddon_gather<false>(a, lo, img, ax, w);
     y[lo] = hit ? (a.mode == MODE_CT_FLAT ? ax : ax * w) : 0.f;
 }
 
@@ -57,15 +48,8 @@ __global__ __launch_bounds__(kBlock) void k_siddon_osem(SidArgs a, const float* 
                                                         float* __restrict__ sens, float epps) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (lo >= a.nmeas) return;
-    float ax = 0.f, w = 0.f, pv = 0.f, pl = 0.f;
-    bool pend = false;
-    const bool hit = siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
-        const float v = img[ind];
-        if (pend) ax += pl * pv;
-        pv = v; pl = len; pend = true;
-    });
-    if (!hit) return;
-    if (pend) ax += pl * pv;
+    float ax, w = 0.f;
+    if (!siddon_gather<false>(a, lo, img, ax, w)) return;
     const float yhat = ax * w;
     float r = meas[lo] / (yhat + epps);
     if (isnan(r) || isinf(r)) r = epps;
@@ -98,6 +82,174 @@ __global__ __launch_bounds__(kBlock) void k_siddon_trace(SidArgs a, int32_t* __r
     counts[lo] = c;
 }
 
+// ---- planar LDS kernels ---------------------------------------------------------------------------
+// For sinograms whose planes are direct (zs == zd), every LOR of a plane lies in one image slice.
+// A workgroup owns (plane, contiguous LOR chunk of that plane):
+//   FP  stage the slice (Nx*Ny floats) in LDS and gather from it;
+//   BP  accumulate rhs (and the sensitivity image) for the slice in LDS with ds_add_f32, then add
+//       the non-zero entries to global memory with one hardware float atomic each.
+// Global atomics drop from one per (LOR, voxel) to one per (workgroup, touched voxel).
+// Block b handles plane b % n_planes: consecutive blocks go to different XCDs, and with 8 planes
+// per launch all blocks of one plane share one XCD's L2.
+template <int TPB, bool FP, bool BP, bool SENS>
+__global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_planes, uint32_t chunk,
+                                                       const float* __restrict__ img, const float* __restrict__ yin,
+                                                       float* __restrict__ yout, float* __restrict__ rhs,
+                                                       float* __restrict__ sens, float epps) {
+    extern __shared__ float lds[];
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    const uint32_t per = a.size_x * a.size_y;
+    const uint32_t l0 = grp * chunk;
+    if (l0 >= per) return;
+    const uint32_t l1 = min(per, l0 + chunk);
+    const uint32_t Nx = a.Nx, Nyx = a.Nyx, P = a.pitch;   // LDS rows are P = Nx + 1 floats apart
+    const uint32_t S = a.Ny * P;                          // LDS floats per slice
+    const int64_t base = (int64_t)plane * per;
+    const float zs = a.z[(a.plane_offset + plane) * 2];
+    const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+    if (tk < 0 || (uint32_t)tk >= a.Nz) {  // every LOR of this plane misses the image
+        if (FP && !BP)
+            for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) yout[base + l] = 0.f;
+        return;
+    }
+    const size_t slice = (size_t)tk * Nyx;
+    float* acc = lds;                        // image slice (FP), then rhs accumulator (BP)
+    float* acc2 = lds + S;                   // sensitivity accumulator
+    float* R = lds + (SENS ? 2 : 1) * S;     // per-LOR ratio between the FP and BP phases
+    if (FP) {
+        const float4* src = reinterpret_cast<const float4*>(img + slice);
+        for (uint32_t i = threadIdx.x; i < Nyx / 4; i += TPB) {
+            const float4 v = src[i];
+            const uint32_t e = 4 * i, yy = e / Nx, xx = e - yy * Nx;   // Nx % 4 == 0: a float4 stays in one row
+            float* d = acc + yy * P + xx;
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
+        __syncthreads();
+        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+            const int64_t lo = base + l;
+            float ax, w = 0.f;
+            const bool hit = siddon_gather<true>(a, lo, acc, ax, w);
+            const float yv = hit ? ax * w : 0.f;
+            if (!BP) {
+                yout[lo] = yv;
+            } else {
+                float r = yin[lo] / (yv + epps);
+                if (isnan(r) || isinf(r)) r = epps;
+                R[l - l0] = hit ? r : 0.f;
+            }
+        }
+    }
+    if (BP) {
+        if (FP) __syncthreads();
+        for (uint32_t i = threadIdx.x; i < S * (SENS ? 2u : 1u); i += TPB) lds[i] = 0.f;
+        __syncthreads();
+        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+            const int64_t lo = base + l;
+            const float r = FP ? R[l - l0] : yin[lo];
+            if (!SENS && r == 0.f) continue;
+            float w = 0.f;
+            siddon_trace<true>(a, lo, w, [&](uint32_t ind, float len) {
+                const float lw = len * w;
+                atomicAdd(acc + ind, r * lw);
+                if (SENS) atomicAdd(acc2 + ind, lw);
+            });
+        }
+        __syncthreads();
+        // this workgroup's partial slice -> scratch (plain coalesced stores); k_planar_reduce sums them
+        float* part = rhs + (size_t)blockIdx.x * Nyx;
+        float* part2 = sens + (size_t)blockIdx.x * Nyx;
+        for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) {
+            const uint32_t yy = i / Nx, j = i + yy;   // padded LDS index of voxel i
+            part[i] = acc[j];
+            if (SENS) part2[i] = acc2[j];
+        }
+    }
+}
+
+// Launch geometry of the planar kernels: one LOR per lane; a plane's LORs are split into
+// workgroups of up to kPlanarTPB lanes (640 = 10 wave64: with the 2-slice LDS footprint one
+// workgroup fills a CU).
+constexpr int kPlanarTPB = 640;
+
+// Sum of the per-workgroup partial slices of each plane, in workgroup order, added to the
+// plane's image slice. Workgroup b = grp * n_planes + plane wrote partial slot b.
+__global__ __launch_bounds__(256) void k_planar_reduce(SidArgs a, uint32_t n_planes, uint32_t groups,
+                                                       const float* __restrict__ part, float* __restrict__ out) {
+    const uint32_t Nyx = a.Nyx;
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t plane = blockIdx.y;
+    if (v >= Nyx) return;
+    const float zs = a.z[(a.plane_offset + plane) * 2];
+    const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+    if (tk < 0 || (uint32_t)tk >= a.Nz) return;
+    float s = 0.f;
+    for (uint32_t g = 0; g < groups; g++) s += part[(size_t)(g * n_planes + plane) * Nyx + v];
+    if (s != 0.f) unsafeAtomicAdd(out + (size_t)tk * Nyx + v, s);   // atomic: two planes may share a slice
+}
+
+struct PlanarShape { uint32_t n_planes, chunk, blocks; };
+
+static PlanarShape planar_shape(const omg_geom* g, int64_t n) {
+    const uint32_t per = g->d.size_x * g->d.size_y;
+    PlanarShape s;
+    s.n_planes = (uint32_t)(n / per);
+    uint32_t groups = (per + kPlanarTPB - 1) / kPlanarTPB;
+    if (g->planar_groups > 0) groups = (uint32_t)g->planar_groups;
+    const uint32_t max_groups = (per + 63) / 64;
+    if (groups > max_groups) groups = max_groups;
+    if (groups < 1) groups = 1;
+    s.chunk = (per + groups - 1) / groups;
+    groups = (per + s.chunk - 1) / s.chunk;
+    s.blocks = s.n_planes * groups;
+    return s;
+}
+
+// The planar path applies to direct-plane PET sinograms with a square slice that fits twice in LDS.
+bool planar_ok(const omg_geom* g) {
+    return g->planar && g->d.Nx == g->d.Ny && g->d.Ny * (g->d.Nx + 1) <= 16512 && g->d.Nx % 4 == 0;
+}
+
+template <bool FP, bool BP, bool SENS>
+static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* yin, float* yout,
+                         float* rhs, float* sens, float epps, hipStream_t s) {
+    const PlanarShape sh = planar_shape(g, n);
+    const uint32_t S = g->d.Ny * (g->d.Nx + 1);
+    const size_t lds = sizeof(float) * ((size_t)S * (BP ? (SENS ? 2 : 1) : 1) + ((FP && BP) ? sh.chunk : 0));
+    auto kern = k_siddon_planar<kPlanarTPB, FP, BP, SENS>;
+    static size_t attr_bytes = 0;   // largest dynamic LDS size requested so far for this instantiation
+    if (lds > attr_bytes) {
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_bytes = lds;
+    }
+    SidArgs a = make_sid_args(g, first, n);
+    a.pitch = g->d.Nx + 1;
+    float* part = nullptr;
+    float* part2 = nullptr;
+    const uint32_t Nyx = g->d.Nx * g->d.Ny;
+    if (BP) {
+        const size_t need = (size_t)sh.blocks * Nyx * (SENS ? 2 : 1);
+        int rc = scratch_reserve(const_cast<omg_geom*>(g), need);
+        if (rc) return rc;
+        part = g->scratch;
+        part2 = SENS ? g->scratch + (size_t)sh.blocks * Nyx : nullptr;
+    }
+    hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, a, sh.n_planes, sh.chunk, img, yin, yout,
+                       BP ? part : rhs, BP ? part2 : sens, epps);
+    OMG_CHECK_LAUNCH();
+    if (BP) {
+        const uint32_t groups = sh.blocks / sh.n_planes;
+        dim3 rg(grid_for(Nyx, 256), sh.n_planes);
+        hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, part, rhs);
+        OMG_CHECK_LAUNCH();
+        if (SENS) {
+            hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, part2, sens);
+            OMG_CHECK_LAUNCH();
+        }
+    }
+    return OMG_OK;
+}
+
 }  // namespace omg
 
 using namespace omg;
@@ -125,6 +277,7 @@ extern "C" int omg_forward_project(const omg_geom* g, int64_t first, int64_t n_m
     hipStream_t s = as_stream(stream);
     if (g->d.projector_type == OMG_PROJ_ORTH) return orth_forward(g, first, n_meas, img, y, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_forward(g, first, n_meas, img, y, s);
+    if (planar_ok(g) && !g->force_generic) return launch_planar<true, false, false>(g, first, n_meas, img, nullptr, y, nullptr, nullptr, 0.f, s);
     SidArgs a = make_sid_args(g, first, n_meas);
     hipLaunchKernelGGL(k_siddon_fp, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, img, y);
     OMG_CHECK_LAUNCH();
@@ -140,6 +293,10 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
     hipStream_t s = as_stream(stream);
     if (g->d.projector_type == OMG_PROJ_ORTH) return orth_backward(g, first, n_meas, y, rhs, sens, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_backward(g, first, n_meas, y, rhs, sens, s);
+    if (planar_ok(g) && !g->force_generic) {
+        if (sens) return launch_planar<false, true, true>(g, first, n_meas, nullptr, y, nullptr, rhs, sens, 0.f, s);
+        return launch_planar<false, true, false>(g, first, n_meas, nullptr, y, nullptr, rhs, nullptr, 0.f, s);
+    }
     SidArgs a = make_sid_args(g, first, n_meas);
     hipLaunchKernelGGL(k_siddon_bp, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, y, rhs, sens);
     OMG_CHECK_LAUNCH();
@@ -154,6 +311,10 @@ extern "C" int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_me
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "the fused OSEM data step is for emission (PET) data");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "the fused OSEM data step is implemented for projector_type 1");
     if (n_meas == 0) return OMG_OK;
+    if (planar_ok(g) && !g->force_generic) {
+        if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, as_stream(stream));
+        return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, as_stream(stream));
+    }
     SidArgs a = make_sid_args(g, first, n_meas);
     hipLaunchKernelGGL(k_siddon_osem, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, as_stream(stream), a, img, meas,
                        rhs, sens, epps);

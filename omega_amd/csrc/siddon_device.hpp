@@ -33,11 +33,16 @@ struct SidArgs {
     const float* __restrict__ x;   // coordinate tables (device)
     const float* __restrict__ z;
     const float* __restrict__ norm; // per-measurement normalisation, subset-local, or null
+    uint32_t pitch;                // row pitch of slice-relative (planar) voxel indices: Nx, or Nx + 1 in LDS
 };
 
 struct Ray { float xs, ys, zs, xd, yd, zd; };
 
 __device__ __forceinline__ uint32_t to_u32(float v) { return (uint32_t)(int64_t)v; }
+
+// std::min / std::max semantics (first argument wins ties and unordered compares).
+__device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }
 
 __device__ __forceinline__ void fetch_ray(const SidArgs& a, int64_t lo, Ray& r) {
     if (a.mode == MODE_LIST6) {
@@ -109,10 +114,10 @@ __device__ __forceinline__ bool preloop2(float b1, float b2, float diff1, float 
     t10 = a1 / diff1;
     t20 = a2 / diff2;
     const float t1b = (max1 - s1) / diff1, t2b = (max2 - s2) / diff2;
-    const float t1min = fminf(t10, t1b), t1max = fmaxf(t10, t1b);
-    const float t2min = fminf(t20, t2b), t2max = fmaxf(t20, t2b);
-    tc = fmaxf(t1min, t2min);
-    const float tmax = fminf(t1max, t2max);
+    const float t1min = mn(t10, t1b), t1max = mx(t10, t1b);
+    const float t2min = mn(t20, t2b), t2max = mx(t20, t2b);
+    tc = mx(t1min, t2min);
+    const float tmax = mn(t1max, t2max);
     if (tc >= tmax) return true;
     uint32_t v1min = 0, v1max = 0, v2min = 0, v2max = 0;
     if (s1 < e1) axis_inc(tc, t1min, tmax, t1max, v1min, v1max, t10, u1, diff1, b1, d1, s1, N1);
@@ -120,7 +125,7 @@ __device__ __forceinline__ bool preloop2(float b1, float b2, float diff1, float 
     if (s2 < e2) axis_inc(tc, t2min, tmax, t2max, v2min, v2max, t20, u2, diff2, b2, d2, s2, N2);
     else axis_dec(tc, t2min, tmax, t2max, v2min, v2max, t20, u2, diff2, b2, d2, s2, N2);
     Np = v1max + 1u + v2max + 1u - v1min - v2min;
-    const float pt = (fminf(t10, t20) + tc) * 0.5f;   // x/2 and x*0.5 are the same IEEE result
+    const float pt = (mn(t10, t20) + tc) * 0.5f;   // x/2 and x*0.5 are the same IEEE result
     i1 = first_voxel(pt, diff1, d1, a1);
     i2 = first_voxel(pt, diff2, d2, a2);
     if (i1 < 0 || (uint32_t)i1 >= N1 || i2 < 0 || (uint32_t)i2 >= N2) return true;
@@ -136,11 +141,11 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
     const float ax = g.bx - r.xs, ay = g.by - r.ys, az = g.bz - r.zs;
     tx0 = ax / xdf; ty0 = ay / ydf; tz0 = az / zdf;
     const float txb = (g.bmaxx - r.xs) / xdf, tyb = (g.bmaxy - r.ys) / ydf, tzb = (g.bmaxz - r.zs) / zdf;
-    const float txmin = fminf(tx0, txb), txmax = fmaxf(tx0, txb);
-    const float tymin = fminf(ty0, tyb), tymax = fmaxf(ty0, tyb);
-    const float tzmin = fminf(tz0, tzb), tzmax = fmaxf(tz0, tzb);
-    tc = fmaxf(fmaxf(txmin, tzmin), tymin);
-    const float tmax = fminf(fminf(txmax, tzmax), tymax);
+    const float txmin = mn(tx0, txb), txmax = mx(tx0, txb);
+    const float tymin = mn(ty0, tyb), tymax = mx(ty0, tyb);
+    const float tzmin = mn(tz0, tzb), tzmax = mx(tz0, tzb);
+    tc = mx(mx(txmin, tzmin), tymin);
+    const float tmax = mn(mn(txmax, tzmax), tymax);
     if (tc >= tmax) return true;
     uint32_t imin = 0, imax = 0, jmin = 0, jmax = 0, kmin = 0, kmax = 0;
     if (r.xs < r.xd) axis_inc(tc, txmin, tmax, txmax, imin, imax, tx0, iu, xdf, g.bx, g.dx, r.xs, g.Nx);
@@ -150,7 +155,7 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
     if (r.zs < r.zd) axis_inc(tc, tzmin, tmax, tzmax, kmin, kmax, tz0, ku, zdf, g.bz, g.dz, r.zs, g.Nz);
     else axis_dec(tc, tzmin, tmax, tzmax, kmin, kmax, tz0, ku, zdf, g.bz, g.dz, r.zs, g.Nz);
     Np = (kmax - kmin + 1) + (jmax - jmin + 1) + (imax - imin + 1);
-    const float pt = (fminf(fminf(tz0, ty0), tx0) + tc) * 0.5f;
+    const float pt = (mn(mn(tz0, ty0), tx0) + tc) * 0.5f;
     ti = first_voxel(pt, xdf, g.dx, ax);
     tj = first_voxel(pt, ydf, g.dy, ay);
     tk = first_voxel(pt, zdf, g.dz, az);
@@ -159,6 +164,15 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
     tyu = g.dy / fabsf(ydf);
     tzu = g.dz / fabsf(zdf);
     return false;
+}
+
+// LOR weight: PET 1/L * norm * global factor (projector_functions.h:1817-1836, :1181-1211), CT 1.
+__device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct) {
+    if (ct) return 1.f;
+    float t = 1.f / L;
+    if (g.norm) t *= g.norm[lo];
+    t *= g.global_factor;
+    return t;
 }
 
 __device__ __forceinline__ int perp_start(float b, float d, float dd, uint32_t N) {
@@ -171,16 +185,29 @@ __device__ __forceinline__ int perp_start(float b, float d, float dd, uint32_t N
     return idx;
 }
 
-// Trace LOR `lo`: calls visit(voxel, length) for each voxel with length > 0 in traversal order.
-// Returns false when the LOR contributes nothing. `w` is the LOR weight (PET: norm*gf/L, CT: 1).
-template <typename Visit>
-__device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float& w, Visit&& visit) {
+// Traversal state of one LOR. Either an axis-parallel row/column walk (perp) or a Siddon walk.
+struct Walk {
+    float tx0, ty0, tz0, txu, tyu, tzu, tc, L;
+    int32_t ti, tj, tk, ux, uy, uz;
+    uint32_t left;                 // Siddon steps still to take (0 = finished)
+    bool perp;
+    uint32_t p_ind, p_stride, p_a1, p_a2, p_nrun, p_lim;
+    float p_dT1, p_dT2, p_d;
+};
+
+// Set-up of LOR `lo` (projector_functions.h:1421-1840). Returns false when the LOR contributes
+// nothing; otherwise fills the walk and the LOR weight.
+// PLANAR: the caller guarantees zs == zd (a direct sinogram plane); only the in-plane branches
+// are compiled and voxel indices are relative to the LOR's slice (tk * Nx * Ny omitted).
+template <bool PLANAR>
+__device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk& w, float& weight) {
     Ray r;
     fetch_ray(g, lo, r);
     const float ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
     if ((ydf == 0.f && xdf == 0.f) || isinf(ydf) || isinf(xdf)) return false;
     const bool ct = g.mode == MODE_CT_FLAT;
     const float L = sqrtf(xdf * xdf + ydf * ydf + zdf * zdf);
+    w.L = L;
 
     // axis-parallel in-plane LOR (projector_functions.h:1508-1700); thresholds are double 1e-8
     if ((double)fabsf(zdf) < 1e-8 && ((double)fabsf(ydf) < 1e-8 || (double)fabsf(xdf) < 1e-8)) {
@@ -195,34 +222,36 @@ __device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float
             if (r.xs > r.xd) { e1 = g.bx - r.xd; e2 = g.bx + (float)g.Nx * g.dx - r.xs; }
             else { e1 = g.bx - r.xs; e2 = g.bx + (float)g.Nx * g.dx - r.xd; }
             for (int k = 0; k < (int)g.Nx; k++) {
-                if (e1 >= 0.f) { a1 = k; dT1 = (k == 0) ? g.dx : fminf(e1, g.dx); break; }
+                if (e1 >= 0.f) { a1 = k; dT1 = (k == 0) ? g.dx : mn(e1, g.dx); break; }
                 e1 += g.dx;
             }
             for (int k = (int)g.Nx - 1; k >= a1; k--) {
-                if (e2 <= 0.f) { a2 = k; dT2 = (k == (int)g.Nx - 1) ? g.dx : fminf(-e2, g.dx); break; }
+                if (e2 <= 0.f) { a2 = k; dT2 = (k == (int)g.Nx - 1) ? g.dx : mn(-e2, g.dx); break; }
                 e2 -= g.dx;
             }
             dd2 = g.dx;
-            // the row start uses Ny as the row stride, as the reference CPU code does (projector_functions.h:1570-1573, :1176)
-            Nm = g.Ny; Nrow = g.Ny; Nrun = g.Nx; stride = 1u;
+            // the row start uses Ny as the row stride, as the reference CPU code does (projector_functions.h:1570-1573, :1176);
+            // planar (slice-relative) indices require Nx == Ny and use the LDS row pitch
+            Nm = PLANAR ? g.pitch : g.Ny; Nrow = g.Ny; Nrun = g.Nx; stride = 1u;
         } else if ((double)fabsf(xdf) < 1e-8 && r.xd <= g.bmaxx && r.xd >= g.bx) {
             a2 = (int)g.Ny - 1; dT1 = g.dx; dT2 = g.dx; db = g.bx; dd = r.xd;
             float e1, e2;
             if (r.ys > r.yd) { e1 = g.by - r.yd; e2 = g.by + (float)g.Ny * g.dy - r.ys; }
             else { e1 = g.by - r.ys; e2 = g.by + (float)g.Ny * g.dy - r.yd; }
             for (int k = 0; k < (int)g.Ny; k++) {
-                if (e1 >= 0.f) { a1 = k; dT1 = (k == 0) ? g.dy : fminf(e1, g.dy); break; }
+                if (e1 >= 0.f) { a1 = k; dT1 = (k == 0) ? g.dy : mn(e1, g.dy); break; }
                 e1 += g.dy;
             }
             for (int k = (int)g.Ny - 1; k >= a1; k--) {
-                if (e2 <= 0.f) { a2 = k; dT2 = (k == (int)g.Ny - 1) ? g.dy : fminf(-e2, g.dy); break; }
+                if (e2 <= 0.f) { a2 = k; dT2 = (k == (int)g.Ny - 1) ? g.dy : mn(-e2, g.dy); break; }
                 e2 -= g.dy;
             }
             dd2 = g.dy; ddb = g.dx;
-            Nm = 1u; Nrow = g.Nx; Nrun = g.Ny; stride = g.Nx;
+            Nm = 1u; Nrow = g.Nx; Nrun = g.Ny; stride = PLANAR ? g.pitch : g.Nx;
         } else {
             return false;
         }
+        weight = make_weight(g, lo, L, ct);
         uint32_t apu = 0;
         {
             float s = db - dd + ddb;
@@ -231,31 +260,22 @@ __device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float
                 s += ddb;
             }
         }
-        uint32_t ind = apu * Nm + (uint32_t)tk * Nrow * Nrun;
-        if (!ct) {
-            float t = 1.f / L;
-            if (g.norm) t *= g.norm[lo];
-            t *= g.global_factor;
-            w = t;
-        } else {
-            w = 1.f;
-        }
-        for (uint32_t ii = (uint32_t)a1; ii <= (uint32_t)a2; ii++) {
-            float din = dd2;
-            if (ii == (uint32_t)a1) ind += stride * ii;
-            if (a1 > 0 && ii == (uint32_t)a1) din = dT1;
-            else if ((uint32_t)a2 < Nrun - 1 && ii == (uint32_t)a2) din = dT2;
-            visit(ind, din);
-            ind += stride;
-        }
+        w.perp = true;
+        w.p_ind = apu * Nm + (PLANAR ? 0u : (uint32_t)tk * Nrow * Nrun) + stride * (uint32_t)a1;
+        w.p_stride = stride;
+        w.p_a1 = (uint32_t)a1; w.p_a2 = (uint32_t)a2; w.p_nrun = Nrun;
+        // With Nx != Ny the reference's row stride (Ny) can point past the image; such voxels are dropped.
+        w.p_lim = PLANAR ? g.Ny * g.pitch : g.Nyx * g.Nz;
+        w.p_dT1 = dT1; w.p_dT2 = dT2; w.p_d = dd2;
+        w.left = 0;
         return true;
     }
-
+    w.perp = false;
     int32_t ti = 0, tj = 0, tk = 0, ux = 0, uy = 0, uz = 0;
     float txu = 0.f, tyu = 0.f, tzu = 0.f, tc = 0.f, tx0 = 1e8f, ty0 = 1e8f, tz0 = 1e8f;
     uint32_t Np = 0;
     bool skip;
-    if (fabsf(zdf) < 1e-8f) {
+    if (PLANAR || fabsf(zdf) < 1e-8f) {
         tk = (int32_t)(fabsf(r.zs - g.bz) / g.dz);
         if (tk < 0 || (uint32_t)tk >= g.Nz) return false;
         skip = preloop2(g.bx, g.by, xdf, ydf, g.bmaxx, g.bmaxy, g.dx, g.dy, g.Nx, g.Ny, ti, tj, txu, tyu, Np,
@@ -274,28 +294,106 @@ __device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float
         skip = preloop3(g, r, xdf, ydf, zdf, ti, tj, tk, tyu, txu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0);
     }
     if (skip) return false;
-    if (!ct) {
-        float t = 1.f / L;
-        if (g.norm) t *= g.norm[lo];
-        t *= g.global_factor;
-        w = t;
-    } else {
-        w = 1.f;
+    weight = make_weight(g, lo, L, ct);
+    w.tx0 = tx0; w.ty0 = ty0; w.tz0 = tz0; w.txu = txu; w.tyu = tyu; w.tzu = tzu; w.tc = tc;
+    w.ti = ti; w.tj = tj; w.tk = tk; w.ux = ux; w.uy = uy; w.uz = uz;
+    w.left = Np;
+    return true;
+}
+
+// One Siddon step (projector_functions.h:1842-1998) without branches: the voxel the ray is in,
+// the length inside it (0 when it contributes nothing or the walk is over), then the advance to
+// the nearest plane crossing (z, then y, then x). Same float operations as the reference.
+template <bool PLANAR>
+__device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& ind, float& len) {
+    // bitwise & / | on the predicates keep the step free of short-circuit control flow
+    const bool alive = w.left > 0u;
+    ind = PLANAR ? (uint32_t)w.tj * g.pitch + (uint32_t)w.ti
+                 : (uint32_t)w.tj * g.Nx + (uint32_t)w.ti + (uint32_t)w.tk * g.Nyx;
+    const bool zmin = !PLANAR && ((w.tz0 < w.ty0) & (w.tz0 < w.tx0));
+    const bool ymin = (!zmin) & (w.ty0 < w.tx0);
+    const float tn = zmin ? w.tz0 : (ymin ? w.ty0 : w.tx0);
+    const float tc = w.tc;
+    const bool tn_in = (tn >= 0.f) & (tn <= 1.f);
+    const bool tc_in = (tc >= 0.f) & (tc <= 1.f);
+    // operand of the length product, as chosen by the reference's branches
+    const float d_nc = tn - tc, d_1c = 1.f - tc;
+    const float o_in = (tc < 0.f) ? tn : d_nc;
+    const float o_out = (tn > 1.f) ? d_1c : d_nc;
+    const float l = (tn_in ? o_in : o_out) * w.L;
+    len = (alive & (tn_in | tc_in)) ? l : 0.f;
+    const bool sx = alive & (!zmin) & (!ymin), sy = alive & ymin, sz = alive & zmin;
+    w.tc = alive ? tn : tc;
+    w.ti += sx ? w.ux : 0;
+    w.tx0 = sx ? w.tx0 + w.txu : w.tx0;
+    w.tj += sy ? w.uy : 0;
+    w.ty0 = sy ? w.ty0 + w.tyu : w.ty0;
+    if (!PLANAR) {
+        w.tk += sz ? w.uz : 0;
+        w.tz0 = sz ? w.tz0 + w.tzu : w.tz0;
     }
-    for (uint32_t ii = 0; ii < Np; ii++) {
-        const uint32_t ind = (uint32_t)tj * g.Nx + (uint32_t)ti + (uint32_t)tk * g.Nyx;
-        const bool zmin = tz0 < ty0 && tz0 < tx0;
-        const bool ymin = !zmin && ty0 < tx0;
-        const float tn = zmin ? tz0 : (ymin ? ty0 : tx0);
-        float len = 0.f;
-        if (tn >= 0.f && tn <= 1.f) len = (tc < 0.f) ? tn * L : (tn - tc) * L;
-        else if (tc >= 0.f && tc <= 1.f) len = (tn > 1.f) ? (1.f - tc) * L : (tn - tc) * L;
-        tc = tn;
-        if (zmin) { tk += uz; tz0 += tzu; }
-        else if (ymin) { tj += uy; ty0 += tyu; }
-        else { ti += ux; tx0 += txu; }
+    // a negative index wraps to >= N as uint32, so one unsigned compare per axis checks both ends
+    bool out = ((uint32_t)w.ti >= g.Nx) | ((uint32_t)w.tj >= g.Ny);
+    if (!PLANAR) out = out | ((uint32_t)w.tk >= g.Nz);
+    w.left = (alive & !out) ? w.left - 1u : 0u;
+}
+
+// Axis-parallel walk: visit(voxel, length) for each voxel in order.
+template <typename Visit>
+__device__ __forceinline__ void perp_walk(const Walk& w, Visit&& visit) {
+    uint32_t ind = w.p_ind;
+    for (uint32_t ii = w.p_a1; ii <= w.p_a2; ii++) {
+        float din = w.p_d;
+        if (w.p_a1 > 0 && ii == w.p_a1) din = w.p_dT1;
+        else if (w.p_a2 < w.p_nrun - 1 && ii == w.p_a2) din = w.p_dT2;
+        if (ind < w.p_lim) visit(ind, din);
+        ind += w.p_stride;
+    }
+}
+
+// Trace LOR `lo`: visit(voxel, length) for each voxel with length > 0 in traversal order.
+// Returns false when the LOR contributes nothing; `weight` is the LOR weight.
+template <bool PLANAR = false, typename Visit>
+__device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+    Walk w;
+    if (!siddon_setup<PLANAR>(g, lo, w, weight)) return false;
+    if (w.perp) {
+        perp_walk(w, visit);
+        return true;
+    }
+    while (w.left > 0u) {
+        uint32_t ind;
+        float len;
+        walk_step<PLANAR>(g, w, ind, len);
         if (len > 0.f) visit(ind, len);
-        if (ti < 0 || (uint32_t)ti >= g.Nx || tj < 0 || (uint32_t)tj >= g.Ny || tk < 0 || (uint32_t)tk >= g.Nz) break;
+    }
+    return true;
+}
+
+// Forward-projection gather of LOR `lo`: sum of len * src[voxel] in traversal order (the
+// reference's accumulation order), with kBatch steps traced before their loads are consumed so
+// kBatch gathers are in flight at once. Returns false when the LOR contributes nothing.
+constexpr int kBatch = 8;
+
+template <bool PLANAR>
+__device__ __forceinline__ bool siddon_gather(const SidArgs& g, int64_t lo, const float* __restrict__ src, float& ax,
+                                              float& weight) {
+    ax = 0.f;
+    Walk w;
+    if (!siddon_setup<PLANAR>(g, lo, w, weight)) return false;
+    if (w.perp) {
+        perp_walk(w, [&](uint32_t ind, float len) { ax += len * src[ind]; });
+        return true;
+    }
+    while (w.left > 0u) {
+        uint32_t ind[kBatch];
+        float len[kBatch], v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) walk_step<PLANAR>(g, w, ind[k], len[k]);
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) v[k] = src[len[k] > 0.f ? ind[k] : 0u];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) ax = (len[k] > 0.f) ? ax + len[k] * v[k] : ax;
     }
     return true;
 }

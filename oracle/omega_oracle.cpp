@@ -282,7 +282,8 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
             if (ii == (uint32_t)a1) ind += stride * ii;
             if (a1 > 0 && ii == (uint32_t)a1) din = dT1;
             else if ((uint32_t)a2 < Nrun - 1 && ii == (uint32_t)a2) din = dT2;
-            visit(ind, din);
+            // with Nx != Ny the reference's row stride can index past the image (undefined there); dropped here
+            if (ind < Nyx * g.Nz) visit(ind, din);
             ind += stride;
         }
         return true;

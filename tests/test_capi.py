@@ -1,0 +1,96 @@
+"""The C ABI on the CPU: the library loads, exports every entry point the header declares, and
+rejects bad descriptors with OMG_ERR_ARG before touching the device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from omega_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "omega_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(omg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_and_binding_agree():
+    names = declared()
+    assert len(names) >= 15
+    assert sorted(set(_lib.exported_symbols())) == names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared():
+        assert hasattr(lib, name), f"{name} missing from libomega_hip.so"
+    assert b"gfx950" in lib.omg_version()
+
+
+def test_descriptor_layout_matches_header(tmp_path):
+    """ctypes' omg_geom_desc layout equals the C compiler's (sizeof and every field offset)."""
+    import subprocess
+    fields = [f[0] for f in _lib.GeomDesc._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'printf("%s %zu\\n", "{f}", offsetof(omg_geom_desc, {f}));' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "omega_hip.h"\n'
+                   'int main(void){printf("sizeof %zu\\n", sizeof(omg_geom_desc));' + body + 'return 0;}')
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = dict(line.split() for line in subprocess.check_output([str(exe)]).decode().splitlines())
+    assert int(out["sizeof"]) == C.sizeof(_lib.GeomDesc)
+    for f in fields:
+        assert int(out[f]) == getattr(_lib.GeomDesc, f).offset, f
+
+
+def _desc(**kw):
+    x = np.zeros(4 * 4 * 3, np.float32)
+    z = np.zeros(2 * 2, np.float32)
+    d = _lib.GeomDesc()
+    d.struct_size = C.sizeof(_lib.GeomDesc)
+    d.projector_type = 1
+    d.mode = _lib.MODE_PET_SINO
+    d.Nx = d.Ny = d.Nz = 4
+    d.dx = d.dy = d.dz = 1.0
+    d.size_x, d.size_y, d.n_proj = 4, 3, 2
+    d.x, d.x_len = x.ctypes.data, x.size
+    d.z, d.z_len = z.ctypes.data, z.size
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d, (x, z)
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("struct_size", 8, "size mismatch"),
+    ("Nx", 0, "empty image"),
+    ("dx", -1.0, "positive"),
+    ("mode", 7, "unknown data mode"),
+    ("projector_type", 4, "projector_type 4"),
+    ("x_len", 3, "x table"),
+    ("n_proj", 9, "z table"),
+])
+def test_create_rejects_bad_descriptors(field, value, msg):
+    d, keep = _desc(**{field: value})
+    h = C.c_void_p()
+    lib = _lib.load()
+    rc = lib.omg_geom_create(C.byref(d), C.byref(h))
+    assert rc == _lib.OMG_ERR_ARG
+    assert msg in _lib.last_error()
+
+
+def test_bdd_requires_ct():
+    d, keep = _desc(projector_type=5)
+    h = C.c_void_p()
+    assert _lib.load().omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG
+    assert "CT" in _lib.last_error()
+
+
+def test_elementwise_argument_checks():
+    lib = _lib.load()
+    assert lib.omg_ratio(None, None, 10, 1e-5, None) == _lib.OMG_ERR_ARG
+    assert lib.omg_rdp_gradient(None, None, 4, 4, 4, 2.0, 1e-5, 0.1, 0, None) == _lib.OMG_ERR_ARG
+    assert lib.omg_pdhg_update(C.c_void_p(8), C.c_void_p(8), None, 10, 1e-5, 1.0, 0.1, 1, 0, None) == _lib.OMG_ERR_ARG
