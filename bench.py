@@ -6,10 +6,10 @@ Workload (one "step"): a full OSEM reconstruction of the 2-D plane PET data set 
 resident in device memory. value = 32 * steps / wall-seconds of the timed region.
 
 Also reported:
-  roofline      for the dominant kernel (the fused FP/ratio/BP launch): algorithmic bytes per
-                launch / its average duration, timed with HIP events on the launch stream inside
-                the timed region. Algorithmic bytes of one subset launch after iteration 0 =
-                12 * sum(Np) + 8 * M_k  (4 B gather of x per intersected voxel, 8 B atomic
+  roofline      for the dominant kernel (the fused FP/ratio/BP projector launch of iterations >= 1):
+                algorithmic bytes per launch / its average duration, timed with HIP events on the
+                launch stream inside the timed region (the library's omg_timing_* hooks).
+                Algorithmic bytes of one such launch = 12 * sum(Np) + 8 * M_k  (4 B gather of x per intersected voxel, 8 B atomic
                 read-modify-write of rhs per voxel, 4 B measurement + 4 B LOR table per LOR);
                 sum(Np) = exact count of (LOR, voxel) intersections with length > 0.
   cpu_baseline  the reference CPU projector (oracle/_ref, the reference's own headers compiled
@@ -40,8 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_osem_pet2d_traffic.json"),
-                    help="HBM bytes per launch measured by rocprofv3 --pmc (JSON), if present")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_traffic.txt"),
+                    help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
     return ap.parse_args()
 
 
@@ -82,58 +82,51 @@ def main():
 
     runner = OSEMRunner(p, y)
     x0 = torch.ones(int(p.N[0]), dtype=torch.float32, device=dev)
-    events = []
 
     def one_step(timed):
+        """One full OSEM run; with ``timed`` the projector launches of iterations >= 1 are event-timed."""
         runner.sens_done = False
         x = x0.clone()
-        runner.rhs.zero_()
         for it in range(n_iter):
+            if timed and it == 1:
+                _lib.call("omg_timing_enable", 1)
             for k in range(p.subsets):
-                if timed and it > 0:
-                    a = torch.cuda.Event(enable_timing=True)
-                    b = torch.cuda.Event(enable_timing=True)
-                    # time only the fused data-step launch: wrap it by splitting subset_step
-                    first = int(p.nMeas[k]); n = M[k]; off = int(p.nTotMeas[k])
-                    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-                    a.record()
-                    _lib.call("omg_osem_data_step", runner.g, first, n, C.c_void_p(x.data_ptr()),
-                              C.c_void_p(y[off:off + n].data_ptr()), C.c_void_p(runner.rhs.data_ptr()),
-                              None, runner.epps, st)
-                    b.record()
-                    events.append((k, a, b))
-                    _lib.call("omg_osem_update", C.c_void_p(x.data_ptr()), C.c_void_p(runner.S[k].data_ptr()),
-                              C.c_void_p(runner.rhs.data_ptr()), runner.N, runner.epps, runner.pos, 1, st)
-                else:
-                    runner.subset_step(x, k)
+                runner.subset_step(x, k)
             runner.sens_done = True
+        if timed:
+            _lib.call("omg_timing_enable", 0)
         return x
 
     for _ in range(args.warmup):
         one_step(False)
+    # calibration copy for HBM-traffic counters: a plain 256 MiB device-to-device copy whose
+    # FETCH_SIZE / WRITE_SIZE give the counter scale (outside the timed region)
+    cal_src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    cal_dst = torch.empty_like(cal_src)
+    cal_dst.copy_(cal_src)
+    del cal_src, cal_dst
     torch.cuda.synchronize()
+    tot = C.c_double()
+    cnt = C.c_int64()
+    _lib.call("omg_timing_collect", C.byref(tot), C.byref(cnt))   # clear any earlier record
     t0 = time.perf_counter()
     for _ in range(args.steps):
         x_last = one_step(True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = sub_iters * args.steps / dt
+    _lib.call("omg_timing_collect", C.byref(tot), C.byref(cnt))
 
-    # dominant kernel roofline (fused FP/ratio/BP, iterations >= 1)
-    durs = {}
-    for k, a, b in events:
-        durs.setdefault(k, []).append(a.elapsed_time(b) * 1e-3)
-    avg_s = float(np.mean([np.mean(v) for v in durs.values()]))
+    # dominant kernel: the fused FP -> ratio -> BP projector launch of iterations >= 1
+    # (k_siddon_planar<640, true, true, false>), timed with HIP events on its stream
+    avg_s = tot.value * 1e-3 / max(cnt.value, 1)
     bytes_per_launch = float(np.mean([12.0 * np_sum[k] + 8.0 * M[k] for k in range(p.subsets)]))
     achieved = bytes_per_launch / avg_s / 1e9
-    traffic = None
-    if os.path.exists(args.traffic):
-        with open(args.traffic) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = read_traffic(args.traffic)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_siddon_osem", "avg_launch_us": round(avg_s * 1e6, 2),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                "kernel": "k_siddon_planar<640,true,true,false>", "launches_timed": int(cnt.value),
+                "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
     cpu = None
     if not args.no_cpu:
@@ -154,6 +147,16 @@ def main():
     if cpu and cpu.get("value"):
         line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+
+
+def read_traffic(path):
+    """HBM bytes per launch of the dominant kernel from a committed rocprofv3 --pmc summary, if present."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def cpu_baseline(p, y, x0, budget_s, runner, n_iter):
@@ -180,7 +183,6 @@ def cpu_baseline(p, y, x0, budget_s, runner, n_iter):
     dt = time.perf_counter() - t
     # GPU image after the same number of subset-iterations
     runner.sens_done = False
-    runner.rhs.zero_()
     xg = x0.clone()
     done = 0
     for it in range(n_iter):

@@ -98,6 +98,26 @@ int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_meas, const
 int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_meas, const float* img, const float* meas,
                        float* rhs, float* sens, float epps, void* stream);
 
+/* One OSEM (beta == 0) or OSL-OSEM with the RDP prior (beta != 0) subset iteration on image x, in place:
+ * the body of the reference's subset loop (source/cpp/reconstructionAF.h:1073-1466): yhat = A_k x;
+ * r = meas / (yhat + epps) (NaN/Inf -> epps); rhs = A_k^T r; if compute_sens, S = A_k^T 1 with
+ * S < epps -> epps (stored to S; else S is read); OSEM x = x / S * (rhs + epps) (subiterStep.h:111-121)
+ * or OSL x = x / (S + dU) * rhs with dU = beta * grad RDP(x; gamma, epps), zero boundary
+ * (subiterStep.h:219-231, RDP.h:4-78); positivity x < epps -> epps when positivity != 0.
+ * Emission data, projector_type 1. Uses library-owned work buffers of the geometry handle:
+ * calls on one handle must not overlap. */
+int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
+                              int32_t compute_sens, float epps, float beta, float gamma, int32_t positivity,
+                              void* stream);
+
+/* Kernel timing of the projector launches with HIP events, the counterpart of the reference's
+ * verbose >= 3 event timers (source/cpp/ProjectorClass.h:3594-3599). omg_timing_enable(1) records
+ * every following projector kernel launch (Siddon FP/BP/fused kernels) until omg_timing_enable(0);
+ * omg_timing_collect waits for the recorded launches, returns their summed duration (ms) and count,
+ * and clears the record. Process-wide, not thread-safe. */
+int omg_timing_enable(int32_t on);
+int omg_timing_collect(double* total_ms, int64_t* launches);
+
 /* Per-LOR voxel lists of the Siddon traversal (verification of the traversal itself).
  * counts[i] = number of voxels with length > 0 for measurement i. If offsets != NULL
  * (exclusive prefix sums of counts), voxel indices and lengths are written to idx/len.
@@ -137,6 +157,17 @@ int omg_pdhg_dual_l2(float* p, const float* ax, const float* y, int64_t n, float
 int omg_dot(const float* a, const float* b, int64_t n, double* out, void* stream);
 /* y = a*x + b*y. */
 int omg_axpby(float* y, const float* x, int64_t n, float a, float b, void* stream);
+
+/* Objectives (absent in the reference, which never evaluates them; reconstructionAF only
+ * ascends them). Both accumulate in double into the device scalar *out.
+ * Poisson log-likelihood L = sum_i y_i log(yhat_i) - yhat_i (terms with y_i = 0 are -yhat_i). */
+int omg_poisson_loglik(const float* y, const float* yhat, int64_t n, double* out, void* stream);
+/* RDP value R(u) = beta * [ sum over interior 6-neighbour pairs {j,k} of phi(u_j, u_k)
+ * + sum over voxels j and their neighbours outside the volume of phi(u_j, 0) ],
+ * phi(a, b) = (a - b)^2 / (a + b + gamma |a - b|). dR/du is the eps -> 0 limit of
+ * omg_rdp_gradient with boundary 0 (RDPKernel, source/cpp/RDP.h:4-78). */
+int omg_rdp_value(const float* u, uint32_t Nx, uint32_t Ny, uint32_t Nz, float gamma, float beta, double* out,
+                  void* stream);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,9 @@ SIGNATURES = {
     "omg_backward_project": [_P, _I64, _I64, _P, _P, _P, _P],
     "omg_osem_data_step": [_P, _I64, _I64, _P, _P, _P, _P, _F, _P],
     "omg_siddon_trace": [_P, _I64, _I64, _P, _P, _P, _P, _P],
+    "omg_osem_subset_iteration": [_P, _I64, _I64, _P, _P, _P, _I32, _F, _F, _F, _I32, _P],
+    "omg_timing_enable": [_I32],
+    "omg_timing_collect": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "omg_rdp_gradient": [_P, _P, _U32, _U32, _U32, _F, _F, _F, _I32, _P],
     "omg_ratio": [_P, _P, _I64, _F, _P],
     "omg_sens_clamp": [_P, _I64, _F, _P],
@@ -76,6 +79,8 @@ SIGNATURES = {
     "omg_pdhg_dual_l2": [_P, _P, _P, _I64, _F, _P],
     "omg_dot": [_P, _P, _I64, _P, _P],
     "omg_axpby": [_P, _P, _I64, _F, _F, _P],
+    "omg_poisson_loglik": [_P, _P, _I64, _P, _P],
+    "omg_rdp_value": [_P, _U32, _U32, _U32, _F, _F, _P, _P],
 }
 
 _lib = None

@@ -45,6 +45,20 @@ int scratch_reserve(omg_geom* g, size_t floats) {
     return OMG_OK;
 }
 
+int work_reserve(omg_geom* g, uint32_t N) {
+    const size_t need = 2 * (size_t)N + g->d.Nz;
+    if (g->work_floats >= need) return OMG_OK;
+    if (g->work) {
+        OMG_CHECK_HIP(hipDeviceSynchronize());
+        OMG_CHECK_HIP(hipFree(g->work));
+        g->work = nullptr;
+        g->work_floats = 0;
+    }
+    OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->work), need * sizeof(float)));
+    g->work_floats = need;
+    return OMG_OK;
+}
+
 }  // namespace omg
 
 using namespace omg;
@@ -107,6 +121,7 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     if (!rc) rc = upload(d->z, d->z_len, &g->z_dev);
     if (!rc) rc = upload(d->norm, d->norm_len, &g->norm_dev);
     if (!rc && d->mode == OMG_MODE_PET_SINO) {
+        g->z_host.assign(d->z, d->z + d->z_len);
         g->planar = true;
         for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
     }
@@ -130,6 +145,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->z_dev);
     hipFree(g->norm_dev);
     hipFree(g->scratch);
+    hipFree(g->work);
     bdd_release(g);
     orth_release(g);
     delete g;

@@ -3,6 +3,8 @@
 #include "common.hpp"
 #include "siddon_device.hpp"
 
+#include <vector>
+
 struct omg_geom {
     omg_geom_desc d;          // caller's description (host table pointers cleared after upload)
     float* x_dev = nullptr;   // coordinate tables on the device
@@ -19,6 +21,11 @@ struct omg_geom {
     // Library-owned scratch (floats) for partial images; grown on demand, freed with the handle.
     float* scratch = nullptr;
     size_t scratch_floats = 0;
+    // Library-owned work buffers of the fused OSEM iteration: [dU: N][rhs: N][slice map: Nz int32].
+    float* work = nullptr;
+    size_t work_floats = 0;
+    // Host copy of the z table (plane -> slice maps are checked on the host).
+    std::vector<float> z_host;
 };
 
 namespace omg {
@@ -51,5 +58,8 @@ int check_range(const omg_geom* g, int64_t first, int64_t n);
 
 // Ensures g->scratch holds at least `floats` floats (hipMalloc on growth; synchronises the device then).
 int scratch_reserve(omg_geom* g, size_t floats);
+
+// Ensures g->work holds the fused-iteration buffers for an image of N voxels.
+int work_reserve(omg_geom* g, uint32_t N);
 
 }  // namespace omg

@@ -118,9 +118,10 @@ __global__ __launch_bounds__(kEB) void k_axpby(float* __restrict__ y, const floa
         y[i] = a * x[i] + b * y[i];
 }
 
-// Dot product with double accumulation: wave64 shuffle reduction, one double atomic per workgroup.
 __global__ void k_zero_f64(double* __restrict__ p) { *p = 0.0; }
+__global__ void k_scale_f64(double* __restrict__ p, double a) { *p *= a; }
 
+// Dot product with double accumulation: wave64 shuffle reduction, one double atomic per workgroup.
 __global__ __launch_bounds__(kEB) void k_dot(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
                                              double* __restrict__ out) {
     double acc = 0.0;
@@ -136,6 +137,57 @@ __global__ __launch_bounds__(kEB) void k_dot(const float* __restrict__ a, const 
         for (int k = 0; k < kEB / 64; k++) t += part[k];
         atomicAdd(out, t);
     }
+}
+
+// Block sum of a per-lane double into *out (wave64 shuffles, one double atomic per workgroup).
+__device__ __forceinline__ void block_sum_atomic(double acc, double* out) {
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    __shared__ double part[kEB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) part[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kEB / 64; k++) t += part[k];
+        atomicAdd(out, t);
+    }
+}
+
+// Poisson log-likelihood sum_i y_i log(yhat_i) - yhat_i in double; y_i = 0 terms contribute -yhat_i.
+__global__ __launch_bounds__(kEB) void k_poisson_ll(const float* __restrict__ y, const float* __restrict__ yhat,
+                                                    int64_t n, double* __restrict__ out) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kEB) {
+        const double h = yhat[i], yy = y[i];
+        acc += (yy != 0.0 ? yy * log(h) : 0.0) - h;
+    }
+    block_sum_atomic(acc, out);
+}
+
+// RDP value R(u) = beta * [ sum over interior neighbour pairs {j,k} of phi(u_j,u_k)
+//                         + sum over voxels j and their missing (outside) neighbours of phi(u_j, 0) ],
+// phi(a, b) = (a - b)^2 / (a + b + gamma |a - b|). Its gradient is the eps -> 0 limit of the RDP
+// gradient kernel with zero boundary (RDP.h:4-78): interior pairs once, ghost pairs with weight 1.
+__device__ __forceinline__ double rdp_phi(double a, double b, double gamma) {
+    const double d = a - b, den = a + b + gamma * fabs(d);
+    return den > 0.0 ? d * d / den : 0.0;
+}
+
+__global__ __launch_bounds__(kEB) void k_rdp_value(const float* __restrict__ u, uint32_t Nx, uint32_t Ny, uint32_t Nz,
+                                                   double gamma, double* __restrict__ out) {
+    const uint32_t Nxy = Nx * Ny;
+    double acc = 0.0;
+    for (uint32_t n = blockIdx.x * kEB + threadIdx.x; n < Nxy * Nz; n += gridDim.x * kEB) {
+        const uint32_t z = n / Nxy, rem = n - z * Nxy, y = rem / Nx, x = rem - y * Nx;
+        const double a = u[n];
+        acc += (x + 1 < Nx) ? rdp_phi(a, u[n + 1], gamma) : rdp_phi(a, 0.0, gamma);
+        acc += (y + 1 < Ny) ? rdp_phi(a, u[n + Nx], gamma) : rdp_phi(a, 0.0, gamma);
+        acc += (z + 1 < Nz) ? rdp_phi(a, u[n + Nxy], gamma) : rdp_phi(a, 0.0, gamma);
+        if (x == 0) acc += rdp_phi(a, 0.0, gamma);
+        if (y == 0) acc += rdp_phi(a, 0.0, gamma);
+        if (z == 0) acc += rdp_phi(a, 0.0, gamma);
+    }
+    block_sum_atomic(acc, out);
 }
 
 static unsigned stream_grid(int64_t n) {
@@ -231,6 +283,36 @@ extern "C" int omg_axpby(float* y, const float* x, int64_t n, float a, float b, 
     OMG_REQUIRE(y && x && n >= 0, "bad argument");
     if (!n) return OMG_OK;
     hipLaunchKernelGGL(k_axpby, dim3(stream_grid(n)), dim3(kEB), 0, as_stream(stream), y, x, n, a, b);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_poisson_loglik(const float* y, const float* yhat, int64_t n, double* out, void* stream) {
+    OMG_REQUIRE(y && yhat && out && n >= 0, "bad argument");
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, s, out);
+    OMG_CHECK_LAUNCH();
+    if (!n) return OMG_OK;
+    const int64_t g = (n + kEB - 1) / kEB;
+    hipLaunchKernelGGL(k_poisson_ll, dim3((unsigned)(g < 2048 ? g : 2048)), dim3(kEB), 0, s, y, yhat, n, out);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_rdp_value(const float* u, uint32_t Nx, uint32_t Ny, uint32_t Nz, float gamma, float beta,
+                             double* out, void* stream) {
+    OMG_REQUIRE(u && out, "null pointer argument");
+    OMG_REQUIRE(Nx > 0 && Ny > 0 && Nz > 0 && (uint64_t)Nx * Ny * Nz < (1ull << 32), "bad image size");
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, s, out);
+    OMG_CHECK_LAUNCH();
+    const int64_t n = (int64_t)Nx * Ny * Nz;
+    const int64_t g = (n + kEB - 1) / kEB;
+    hipLaunchKernelGGL(k_rdp_value, dim3((unsigned)(g < 2048 ? g : 2048)), dim3(kEB), 0, s, u, Nx, Ny, Nz,
+                       (double)gamma, out);
+    OMG_CHECK_LAUNCH();
+    // the value is scaled by beta on the device with a one-lane kernel (keeps the call asynchronous)
+    hipLaunchKernelGGL(k_scale_f64, dim3(1), dim3(1), 0, s, out, (double)beta);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }

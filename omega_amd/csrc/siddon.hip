@@ -10,9 +10,50 @@
 // uses `omp atomic` (source/cpp/projector_functions.h:652-656).
 #include "geom.hpp"
 
+#include <utility>
+#include <vector>
+
 namespace omg {
 
 constexpr int kBlock = 256;
+
+// Kernel timing around the projector launches, the counterpart of the reference's verbose >= 3
+// HIP event timers (source/cpp/ProjectorClass.h:3594-3599). Off unless enabled.
+struct TimingState {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t used = 0;
+    hipError_t err = hipSuccess;   // first event failure since the last collect
+};
+static TimingState g_timing;
+
+static void timing_begin(hipStream_t s, hipEvent_t* stop) {
+    *stop = nullptr;
+    if (!g_timing.on) return;
+    if (g_timing.used == g_timing.ev.size()) {
+        hipEvent_t a, b;
+        hipError_t e = hipEventCreate(&a);
+        if (e == hipSuccess) e = hipEventCreate(&b);
+        if (e != hipSuccess) {
+            g_timing.err = e;
+            return;
+        }
+        g_timing.ev.push_back({a, b});
+    }
+    auto& pr = g_timing.ev[g_timing.used];
+    const hipError_t e = hipEventRecord(pr.first, s);
+    if (e != hipSuccess) {
+        g_timing.err = e;
+        return;
+    }
+    g_timing.used++;
+    *stop = pr.second;
+}
+
+static void timing_end(hipStream_t s, hipEvent_t stop) {
+    if (stop) hipEventRecord(stop, s);
+}
+
 
 // ---- forward projection -------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_siddon_fp(SidArgs a, const float* __restrict__ img, float* __restrict__ y) {
@@ -234,9 +275,13 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
         part = g->scratch;
         part2 = SENS ? g->scratch + (size_t)sh.blocks * Nyx : nullptr;
     }
+    hipEvent_t stop;
+    timing_begin(s, &stop);
     hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, a, sh.n_planes, sh.chunk, img, yin, yout,
                        BP ? part : rhs, BP ? part2 : sens, epps);
+    timing_end(s, stop);
     OMG_CHECK_LAUNCH();
+    if (BP && !rhs) return OMG_OK;   // the caller reduces the partial slices itself
     if (BP) {
         const uint32_t groups = sh.blocks / sh.n_planes;
         dim3 rg(grid_for(Nyx, 256), sh.n_planes);
@@ -250,9 +295,73 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
     return OMG_OK;
 }
 
+// ---- OSEM / OSL-OSEM update that also reduces the planar partial slices -----------------------------
+// For voxel n in slice z: if z is the slice of launch plane p (slice_plane[z] = p, else -1), rhs and,
+// when sensitivity is computed, S are the workgroup-ordered sums of the plane's partial slices;
+// otherwise rhs = 0 (and S = 0). S is clamped (S < epps -> epps, functions.hpp:2996) and stored.
+// OSEM:     x = x / S * (rhs + epps)            (algorithms.h:12-15, subiterStep.h:111-121)
+// OSL-OSEM: x = x / (S + dU) * rhs              (subiterStep.h:219-231), dU = beta * grad RDP(x)
+// positivity: x < epps -> epps                  (reconstructionAF.h:1461-1465)
+template <bool SENS, bool OSL>
+__global__ __launch_bounds__(256) void k_planar_update(uint32_t Nyx, uint32_t N, uint32_t n_planes, uint32_t groups,
+                                                       const int32_t* __restrict__ slice_plane,
+                                                       const float* __restrict__ part, const float* __restrict__ part2,
+                                                       float* __restrict__ x, float* __restrict__ S,
+                                                       const float* __restrict__ dU, float epps, int pos) {
+    const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    const uint32_t z = n / Nyx, v = n - z * Nyx;
+    const int32_t p = slice_plane[z];
+    float rhs = 0.f, s;
+    if (p >= 0) {
+        for (uint32_t g = 0; g < groups; g++) rhs += part[(size_t)(g * n_planes + (uint32_t)p) * Nyx + v];
+    }
+    if (SENS) {
+        s = 0.f;
+        if (p >= 0)
+            for (uint32_t g = 0; g < groups; g++) s += part2[(size_t)(g * n_planes + (uint32_t)p) * Nyx + v];
+        if (s < epps) s = epps;
+        S[n] = s;
+    } else {
+        s = S[n];
+    }
+    float xn;
+    if (OSL) xn = x[n] / (s + dU[n]) * rhs;
+    else xn = x[n] / s * (rhs + epps);
+    if (pos && xn < epps) xn = epps;
+    x[n] = xn;
+}
+
 }  // namespace omg
 
 using namespace omg;
+
+extern "C" int omg_timing_enable(int32_t on) {
+    g_timing.on = on != 0;
+    return OMG_OK;
+}
+
+extern "C" int omg_timing_collect(double* total_ms, int64_t* launches) {
+    OMG_REQUIRE(total_ms && launches, "null output pointer");
+    if (g_timing.err != hipSuccess) {
+        const hipError_t e = g_timing.err;
+        g_timing.err = hipSuccess;
+        g_timing.used = 0;
+        set_error("recording timing events failed: %s", hipGetErrorString(e));
+        return OMG_ERR_HIP;
+    }
+    double t = 0.0;
+    for (size_t i = 0; i < g_timing.used; i++) {
+        OMG_CHECK_HIP(hipEventSynchronize(g_timing.ev[i].second));
+        float ms = 0.f;
+        OMG_CHECK_HIP(hipEventElapsedTime(&ms, g_timing.ev[i].first, g_timing.ev[i].second));
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = (int64_t)g_timing.used;
+    g_timing.used = 0;
+    return OMG_OK;
+}
 
 static int siddon_common_checks(const omg_geom* g, int64_t first, int64_t n) {
     OMG_REQUIRE(g != nullptr, "null geometry handle");
@@ -279,7 +388,10 @@ extern "C" int omg_forward_project(const omg_geom* g, int64_t first, int64_t n_m
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_forward(g, first, n_meas, img, y, s);
     if (planar_ok(g) && !g->force_generic) return launch_planar<true, false, false>(g, first, n_meas, img, nullptr, y, nullptr, nullptr, 0.f, s);
     SidArgs a = make_sid_args(g, first, n_meas);
+    hipEvent_t stop;
+    timing_begin(s, &stop);
     hipLaunchKernelGGL(k_siddon_fp, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, img, y);
+    timing_end(s, stop);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -298,7 +410,10 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
         return launch_planar<false, true, false>(g, first, n_meas, nullptr, y, nullptr, rhs, nullptr, 0.f, s);
     }
     SidArgs a = make_sid_args(g, first, n_meas);
+    hipEvent_t stop;
+    timing_begin(s, &stop);
     hipLaunchKernelGGL(k_siddon_bp, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, y, rhs, sens);
+    timing_end(s, stop);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -318,6 +433,131 @@ extern "C" int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_me
     SidArgs a = make_sid_args(g, first, n_meas);
     hipLaunchKernelGGL(k_siddon_osem, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, as_stream(stream), a, img, meas,
                        rhs, sens, epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+namespace omg {
+
+__global__ void k_fill_f32(float* __restrict__ p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// slice -> launch plane map for the planar update (-1: slice not in this launch)
+__global__ void k_slice_plane(SidArgs a, uint32_t n_planes, int32_t* __restrict__ map) {
+    const uint32_t zsl = blockIdx.x * 256 + threadIdx.x;
+    if (zsl >= a.Nz) return;
+    int32_t found = -1;
+    for (uint32_t p = 0; p < n_planes; p++) {
+        const float zs = a.z[(a.plane_offset + p) * 2];
+        const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+        if (tk == (int32_t)zsl) { found = (int32_t)p; break; }
+    }
+    map[zsl] = found;
+}
+
+__global__ __launch_bounds__(256) void k_generic_update(uint32_t N, float* __restrict__ x, float* __restrict__ S,
+                                                        float* __restrict__ rhs, const float* __restrict__ dU,
+                                                        float epps, int clamp_s, int pos) {
+    const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    float s = S[n];
+    if (clamp_s) {
+        if (s < epps) s = epps;
+        S[n] = s;
+    }
+    float xn = dU ? x[n] / (s + dU[n]) * rhs[n] : x[n] / s * (rhs[n] + epps);
+    if (pos && xn < epps) xn = epps;
+    x[n] = xn;
+    rhs[n] = 0.f;
+}
+
+// True when the planes [first, first + n_planes) map to distinct image slices.
+static bool unique_slices(const omg_geom* g, int64_t first, uint32_t n_planes) {
+    std::vector<char> seen(g->d.Nz, 0);
+    for (uint32_t p = 0; p < n_planes; p++) {
+        const float zs = g->z_host[(size_t)(first + p) * 2];
+        const int32_t tk = (int32_t)(fabsf(zs - g->d.bz) / g->d.dz);
+        if (tk < 0 || (uint32_t)tk >= g->d.Nz) continue;
+        if (seen[tk]) return false;
+        seen[tk] = 1;
+    }
+    return true;
+}
+
+}  // namespace omg
+
+extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas,
+                                         float* S, int32_t compute_sens, float epps, float beta, float gamma,
+                                         int32_t positivity, void* stream) {
+    int rc = siddon_common_checks(g, first, n_meas);
+    if (rc) return rc;
+    OMG_REQUIRE(x && meas && S, "null pointer argument");
+    OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "OSEM subset iterations here are for emission (PET) data");
+    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "the fused OSEM iteration is implemented for projector_type 1");
+    hipStream_t s = as_stream(stream);
+    omg_geom* gm = const_cast<omg_geom*>(g);
+    const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+    rc = work_reserve(gm, N);
+    if (rc) return rc;
+    float* dU = nullptr;
+    if (beta != 0.f) {
+        dU = gm->work;
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, dU, N, 0.f);
+        OMG_CHECK_LAUNCH();
+        rc = omg_rdp_gradient(x, dU, g->d.Nx, g->d.Ny, g->d.Nz, gamma, epps, beta, 0, stream);
+        if (rc) return rc;
+    }
+    const uint32_t per = g->d.size_x * g->d.size_y;
+    if (n_meas > 0 && planar_ok(g) && !g->force_generic && unique_slices(g, first, (uint32_t)(n_meas / per))) {
+        const PlanarShape sh = planar_shape(g, n_meas);
+        if (compute_sens) rc = launch_planar<true, true, true>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
+        else rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
+        if (rc) return rc;
+        const uint32_t Nyx = g->d.Nx * g->d.Ny;
+        int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
+        SidArgs a = make_sid_args(g, first, n_meas);
+        hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a, sh.n_planes, map);
+        OMG_CHECK_LAUNCH();
+        const float* part = g->scratch;
+        const float* part2 = g->scratch + (size_t)sh.blocks * Nyx;
+        const uint32_t groups = sh.blocks / sh.n_planes;
+        dim3 grid(grid_for(N, 256));
+        if (compute_sens && dU)
+            hipLaunchKernelGGL((k_planar_update<true, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                               part2, x, S, dU, epps, (int)positivity);
+        else if (compute_sens)
+            hipLaunchKernelGGL((k_planar_update<true, false>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                               part2, x, S, dU, epps, (int)positivity);
+        else if (dU)
+            hipLaunchKernelGGL((k_planar_update<false, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                               part2, x, S, dU, epps, (int)positivity);
+        else
+            hipLaunchKernelGGL((k_planar_update<false, false>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                               part2, x, S, dU, epps, (int)positivity);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
+    // one-lane-per-LOR path: rhs (and S) accumulate with global atomics, then the update
+    float* rhs = gm->work + N;
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, rhs, N, 0.f);
+    OMG_CHECK_LAUNCH();
+    if (compute_sens) {
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, S, N, 0.f);
+        OMG_CHECK_LAUNCH();
+    }
+    if (n_meas > 0) {
+        SidArgs a = make_sid_args(g, first, n_meas);
+        hipEvent_t stop;
+        timing_begin(s, &stop);
+        hipLaunchKernelGGL(k_siddon_osem, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, x, meas, rhs,
+                           compute_sens ? S : nullptr, epps);
+        timing_end(s, stop);
+        OMG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_generic_update, dim3(grid_for(N, 256)), dim3(256), 0, s, N, x, S, rhs, dU, epps,
+                       (int)(compute_sens != 0), (int)positivity);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }

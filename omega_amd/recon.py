@@ -11,9 +11,11 @@ Restates the subset loop of the reference's built-in reconstruction
         OSL-OSEM: dU = beta * grad RDP(x);  x = x / (S_k + dU) * rhs    (priors.h:447-451; subiterStep.h:219-231)
         positivity x < epps -> epps                                      (reconstructionAF.h:1461-1465)
 
-The FP -> ratio -> BP part of a subset is one fused kernel launch (omg_osem_data_step); the
-update is a second. Sensitivity images are kept per subset (compute_norm_matrix = 2,
-reconstructionAF.h:313-317) and frozen after the first iteration (:1508-1509).
+A subset iteration is one C-ABI call (omg_osem_subset_iteration): the fused FP -> ratio -> BP
+projector kernel and one update kernel (plus the RDP gradient kernel for OSL-OSEM). Sensitivity
+images are kept per subset (compute_norm_matrix = 2, reconstructionAF.h:313-317) and frozen after
+the first iteration (:1508-1509). The RDP boundary of the fused path is the CPU reference's
+(zero outside the volume).
 """
 from __future__ import annotations
 
@@ -54,7 +56,6 @@ class OSEMRunner:
         dev = self.meas.device
         self.S = torch.zeros((proj.subsets, self.N), dtype=torch.float32, device=dev)
         self.rhs = torch.zeros(self.N, dtype=torch.float32, device=dev)
-        self.dU = torch.zeros(self.N, dtype=torch.float32, device=dev) if self.beta != 0.0 else None
         self.sens_done = False
         self.stream = stream
 
@@ -63,7 +64,17 @@ class OSEMRunner:
         return C.c_void_p(s.cuda_stream)
 
     def subset_step(self, x, k: int) -> None:
-        """One subset iteration on image ``x`` (updated in place)."""
+        """One subset iteration on image ``x`` (updated in place): omg_osem_subset_iteration."""
+        proj = self.proj
+        first = int(proj.nMeas[k])
+        n = int(proj.nMeasSubset[k, 0])
+        off = int(proj.nTotMeas[k])
+        yk = self.meas[off:off + n]
+        _lib.call("omg_osem_subset_iteration", self.g, first, n, _p(x), _p(yk), _p(self.S[k]),
+                  int(not self.sens_done), self.epps, self.beta, self.gamma, self.pos, self._s())
+
+    def subset_step_unfused(self, x, k: int) -> None:
+        """The same iteration through the separate entry points (data step, clamp, RDP, update)."""
         proj = self.proj
         first = int(proj.nMeas[k])
         n = int(proj.nMeasSubset[k, 0])
@@ -73,16 +84,17 @@ class OSEMRunner:
         if sens is not None:
             sens.zero_()
         yk = self.meas[off:off + n]
+        self.rhs.zero_()
         _lib.call("omg_osem_data_step", self.g, first, n, _p(x), _p(yk), _p(self.rhs), _p(sens), self.epps, st)
         if sens is not None:
             _lib.call("omg_sens_clamp", _p(sens), self.N, self.epps, st)
-        if self.dU is None:
+        if self.beta == 0.0:
             _lib.call("omg_osem_update", _p(x), _p(self.S[k]), _p(self.rhs), self.N, self.epps, self.pos, 1, st)
         else:
-            self.dU.zero_()
-            _lib.call("omg_rdp_gradient", _p(x), _p(self.dU), *self.Nxyz, self.gamma, self.epps, self.beta,
+            dU = self.torch.zeros_like(x)
+            _lib.call("omg_rdp_gradient", _p(x), _p(dU), *self.Nxyz, self.gamma, self.epps, self.beta,
                       self.rdp_boundary, st)
-            _lib.call("omg_osl_update", _p(x), _p(self.S[k]), _p(self.dU), _p(self.rhs), self.N, self.epps,
+            _lib.call("omg_osl_update", _p(x), _p(self.S[k]), _p(dU), _p(self.rhs), self.N, self.epps,
                       self.pos, 1, st)
 
     def run(self, n_iter: int, x0=None):

@@ -45,3 +45,37 @@ def pet2d_data(p: projectorClass, seed: int = 0, total_counts: float = 5e6, devi
     rng = np.random.default_rng(seed + 1)
     y = rng.poisson(ax * scale).astype(np.float32)
     return ph, torch.as_tensor(y, device=device)
+
+
+def pet3d_projector(subsets: int = 1, n_rings: int = 32, crystals: int = 448, ring_pitch: float = 4.0,
+                    max_ring_diff: int = 31, radius: float = 400.0, image=(128, 128, 63),
+                    fov_xy: float = 320.0) -> projectorClass:
+    """3-D PET: ``n_rings`` rings at ``ring_pitch`` mm, ``crystals`` per ring, span-1 ring pairs up to
+    ``max_ring_diff``; ``crystals/2`` angles x ``crystals/2`` radial bins; image ``image`` voxels over a
+    ``fov_xy`` mm transaxial FOV and the rings' axial extent. Defaults: 224 x 224 x 1024 sinogram,
+    128 x 128 x 63 image (2.5 mm transaxial, 128/63 mm axial voxels)."""
+    sc = G.pet_3d_rings(radius=radius, n_rings=n_rings, ring_pitch=ring_pitch, crystals=crystals,
+                        max_ring_diff=max_ring_diff)
+    p = projectorClass()
+    p.Nx, p.Ny, p.Nz = image
+    p.FOVa_x, p.FOVa_y, p.axial_fov = fov_xy, fov_xy, n_rings * ring_pitch
+    p.projector_type = 1
+    p.x = sc.xy_table()
+    p.z = sc.z
+    p.Ndist, p.Nang = sc.n_radial, sc.n_angles
+    p.subsets = subsets
+    p.subsetType = 8
+    p.addProjector()
+    return p
+
+
+def poisson_data(p: projectorClass, phantom, total_counts: float, seed: int, device="cuda"):
+    """Poisson data (subset order) of A x scaled to ``total_counts``, x = ``phantom``."""
+    import torch
+    x = torch.as_tensor(phantom, device=device)
+    ax = torch.cat([p.forwardProject(x, k) for k in range(p.subsets)]).double()
+    scale = total_counts / float(ax.sum())
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    y = torch.poisson(ax * scale, generator=g).float()
+    return y, scale

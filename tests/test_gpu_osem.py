@@ -1,0 +1,173 @@
+"""GPU parity of the OSEM / OSL-OSEM(RDP) path and the image kernels against the CPU oracle.
+
+Bars: RDP gradient and the elementwise ratio / clamp / OSEM / OSL updates bit-identical to the
+oracle (same float operations); whole OSEM and OSL-OSEM subset iterations within 1e-5 relative L2
+of the oracle (the backprojection's float-atomic order differs, as it does between runs of the
+reference itself).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from omega_amd import _lib, workloads
+from omega_amd.projector import Geometry
+from omega_amd.recon import OSEMRunner
+from oracle import oracle as O
+from tests.hip_helpers import ptr, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def pet2d(cuda):
+    p = workloads.pet2d_projector(subsets=8)
+    p.initProj()
+    ph, y = workloads.pet2d_data(p, seed=3)
+    return p, y
+
+
+def test_rdp_gradient_bitexact(cuda):
+    import torch
+    shape = (64, 48, 20)
+    rng = np.random.default_rng(0)
+    u = (rng.random(int(np.prod(shape))) * 2).astype(np.float32)
+    u[::53] = 0.0
+    g0 = rng.random(u.size).astype(np.float32)
+    gt = torch.from_numpy(g0).to(cuda)
+    ut = torch.from_numpy(u).to(cuda)
+    _lib.call("omg_rdp_gradient", ptr(ut), ptr(gt), *shape, 2.0, 1e-5, 0.1, 0, None)
+    ref = O.rdp_gradient(u, shape, 2.0, 1e-5, 0.1, grad=g0)
+    assert np.array_equal(gt.cpu().numpy(), ref)
+
+
+def test_rdp_gradient_clamp_boundary(cuda):
+    """Clamp-to-edge neighbours (the reference GPU kernel with image textures)."""
+    import torch
+    shape = (9, 8, 7)
+    u = (np.random.default_rng(1).random(int(np.prod(shape))) + 0.2).astype(np.float32)
+    gt = torch.zeros(u.size, device=cuda)
+    ut = torch.from_numpy(u).to(cuda)
+    _lib.call("omg_rdp_gradient", ptr(ut), ptr(gt), *shape, 2.0, 1e-5, 0.1, 1, None)
+    v = u.reshape(shape[::-1]).astype(np.float64)
+    pad = np.pad(v, 1, mode="edge")
+    c = pad[1:-1, 1:-1, 1:-1]
+    acc = np.zeros_like(c)
+    for sl in [(slice(1, -1), slice(1, -1), slice(2, None)), (slice(1, -1), slice(1, -1), slice(0, -2)),
+               (slice(1, -1), slice(2, None), slice(1, -1)), (slice(1, -1), slice(0, -2), slice(1, -1)),
+               (slice(2, None), slice(1, -1), slice(1, -1)), (slice(0, -2), slice(1, -1), slice(1, -1))]:
+        k = pad[sl]
+        d = c - k
+        den = c + k + 2.0 * np.abs(d)
+        acc += d * (2.0 * np.abs(d) + c + 3 * k + 1e-10) / (den * den + 1e-5)
+    assert rel_l2(gt.cpu().numpy(), 0.1 * acc.reshape(-1)) < 1e-6
+
+
+def test_elementwise_bitexact(cuda):
+    import torch
+    rng = np.random.default_rng(2)
+    n = 100_003
+    y = rng.poisson(3.0, n).astype(np.float32)
+    yhat = rng.random(n).astype(np.float32)
+    yhat[::97] = 0.0
+    yhat[::101] = -1e-5            # yhat + epps == 0: Inf/NaN -> epps
+    r = torch.from_numpy(yhat.copy()).to(cuda)
+    yt = torch.from_numpy(y).to(cuda)
+    _lib.call("omg_ratio", ptr(yt), ptr(r), n, 1e-5, None)
+    assert np.array_equal(r.cpu().numpy(), O.ratio(y, yhat, 1e-5))
+    s = rng.random(n).astype(np.float32) * 1e-4
+    st = torch.from_numpy(s.copy()).to(cuda)
+    _lib.call("omg_sens_clamp", ptr(st), n, 1e-5, None)
+    assert np.array_equal(st.cpu().numpy(), O.sens_clamp(s, 1e-5))
+    x = rng.random(n).astype(np.float32)
+    S = rng.random(n).astype(np.float32) + 0.1
+    rhs = rng.random(n).astype(np.float32)
+    dU = (rng.random(n).astype(np.float32) - 0.5) * 0.01
+    # every device buffer is held by a name for the duration of the call
+    St, dUt = torch.from_numpy(S).to(cuda), torch.from_numpy(dU).to(cuda)
+    xt, rt = torch.from_numpy(x.copy()).to(cuda), torch.from_numpy(rhs.copy()).to(cuda)
+    _lib.call("omg_osem_update", ptr(xt), ptr(St), ptr(rt), n, 1e-5, 1, 1, None)
+    assert np.array_equal(xt.cpu().numpy(), O.osem_update(x, S, rhs, 1e-5))
+    assert float(rt.abs().sum()) == 0.0
+    xt, rt = torch.from_numpy(x.copy()).to(cuda), torch.from_numpy(rhs.copy()).to(cuda)
+    _lib.call("omg_osl_update", ptr(xt), ptr(St), ptr(dUt), ptr(rt), n, 1e-5, 1, 0, None)
+    assert np.array_equal(xt.cpu().numpy(), O.osl_update(x, S, dU, rhs, 1e-5))
+
+
+def _oracle_run(p, y, n_sub, beta=0.0):
+    nmeas = np.diff(p.nMeas).astype(np.int64)
+    return O.osem(p.grid, p.nRowsD, p.nColsD, p.x, p.z, nmeas, y.cpu().numpy(), 4,
+                  np.ones(int(p.N[0]), np.float32), epps=1e-5, beta=beta, gamma=2.0, subsets_to_run=n_sub)
+
+
+def _gpu_run(p, y, n_sub, beta=0.0, generic=False, unfused=False):
+    import torch
+    run = OSEMRunner(p, y, beta=beta, gamma=2.0)
+    if generic:
+        d, keep = p._descriptor()
+        d.reserved[0] = 1
+        gm = Geometry(d, keep)
+        run.g = gm.handle
+        run._keep = gm
+    x = torch.ones(int(p.N[0]), device="cuda")
+    done = 0
+    for it in range(4):
+        for k in range(p.subsets):
+            if done < n_sub:
+                (run.subset_step_unfused if unfused else run.subset_step)(x, k)
+                done += 1
+        run.sens_done = True
+    torch.cuda.synchronize()
+    return x.cpu().numpy()
+
+
+@pytest.mark.parametrize("path", ["planar", "generic", "unfused"])
+def test_osem_iterations_match_oracle(pet2d, path):
+    p, y = pet2d
+    n_sub = 10   # all of iteration 0 (sensitivity images) and two subsets of iteration 1
+    print(f"[osem] path={path} subset-iterations={n_sub}", flush=True)
+    xg = _gpu_run(p, y, n_sub, generic=path == "generic", unfused=path == "unfused")
+    xo = _oracle_run(p, y, n_sub)
+    e = rel_l2(xg, xo)
+    print(f"  rel-L2 vs oracle {e:.2e}")
+    assert e < TOL
+
+
+def test_osl_osem_rdp_iterations_match_oracle(pet2d):
+    p, y = pet2d
+    xg = _gpu_run(p, y, 10, beta=0.1)
+    xo = _oracle_run(p, y, 10, beta=0.1)
+    e = rel_l2(xg, xo)
+    print(f"[osl-osem] rel-L2 vs oracle {e:.2e}")
+    assert e < TOL
+    assert rel_l2(_gpu_run(p, y, 10, beta=0.1, unfused=True), xo) < TOL
+
+
+def test_full_osem_run_reference_quality(pet2d):
+    """The full 8 x 4 run stays within tolerance of the oracle and increases the Poisson log-likelihood."""
+    import torch
+    p, y = pet2d
+    xg = _gpu_run(p, y, 32)
+    xo = _oracle_run(p, y, 32)
+    assert rel_l2(xg, xo) < TOL
+    from omega_amd.recon import poisson_loglik
+    ll = []
+    for img in (np.ones_like(xg), xg):
+        t = torch.from_numpy(img).cuda()
+        yhat = torch.cat([p.forwardProject(t, k) for k in range(p.subsets)]).cpu().numpy()
+        ll.append(poisson_loglik(y.cpu().numpy(), yhat))
+    assert ll[1] > ll[0]
+
+
+def test_timing_hooks(pet2d):
+    import torch
+    p, y = pet2d
+    tot, cnt = C.c_double(), C.c_int64()
+    _lib.call("omg_timing_collect", C.byref(tot), C.byref(cnt))
+    _lib.call("omg_timing_enable", 1)
+    _gpu_run(p, y, 3)
+    _lib.call("omg_timing_enable", 0)
+    _lib.call("omg_timing_collect", C.byref(tot), C.byref(cnt))
+    assert cnt.value == 3 and tot.value > 0.0
