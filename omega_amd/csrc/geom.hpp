@@ -26,6 +26,8 @@ struct omg_geom {
     size_t work_floats = 0;
     // Host copy of the z table (plane -> slice maps are checked on the host).
     std::vector<float> z_host;
+    // Branchless distance-driven projector: per-projection back-projection geometry (16 floats each).
+    float* bdd_geom5 = nullptr;
 };
 
 namespace omg {

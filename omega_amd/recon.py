@@ -118,3 +118,112 @@ def poisson_loglik(y, yhat, epps=0.0) -> float:
     yh = np.asarray(yhat, dtype=np.float64) + epps
     m = yh > 0
     return float(np.sum(y[m] * np.log(yh[m])) - np.sum(yh))
+
+
+def _dot(a, b, out) -> float:
+    """<a, b> in double on the device (omg_dot); returns a Python float (synchronises)."""
+    _lib.call("omg_dot", _p(a), _p(b), a.numel(), _p(out), None)
+    return float(out.item())
+
+
+class CGLSRunner:
+    """CGLS without subsets, restating the reference's recurrences
+    (init functions.hpp:2315-2340; step computeForwardProject.h:197-206; update algorithms.h:243-269):
+
+        r = y; f = x0; p = A^T r; gamma = <p, p>
+        per iteration: q = A p; alpha = gamma / <q, q>; r = r - alpha q; s = A^T r;
+                       gamma' = <s, s>; f = f + alpha p; p = s + (gamma'/gamma) p; gamma = gamma'
+    """
+
+    def __init__(self, proj, meas):
+        import torch
+        self.torch = torch
+        self.proj = proj
+        if not proj.projectorInitialized:
+            proj.initProj()
+        if proj.subsets != 1:
+            raise ValueError("CGLS is defined without subsets (set subsets = 1)")
+        self.meas = meas.reshape(-1).contiguous()
+        self.acc = torch.zeros(1, dtype=torch.float64, device=self.meas.device)
+
+    def run(self, n_iter: int, x0=None):
+        torch = self.torch
+        p_ = self.proj
+        N = int(p_.N[0])
+        f = torch.zeros(N, device=self.meas.device) if x0 is None else x0.reshape(-1).float().clone()
+        r = self.meas.clone()
+        if x0 is not None and float(f.abs().sum()) != 0.0:
+            ax = p_.forwardProject(f, 0)
+            _lib.call("omg_axpby", _p(r), _p(ax), r.numel(), -1.0, 1.0, None)
+        pv = p_.backwardProject(r, 0)
+        gamma = _dot(pv, pv, self.acc)
+        for _ in range(n_iter):
+            q = p_.forwardProject(pv, 0)
+            alpha = gamma / _dot(q, q, self.acc)
+            _lib.call("omg_axpby", _p(r), _p(q), r.numel(), -alpha, 1.0, None)
+            s = p_.backwardProject(r, 0)
+            gamma_new = _dot(s, s, self.acc)
+            beta = gamma_new / gamma
+            _lib.call("omg_axpby", _p(f), _p(pv), N, alpha, 1.0, None)
+            _lib.call("omg_axpby", _p(pv), _p(s), N, 1.0, beta, None)
+            gamma = gamma_new
+        return f
+
+
+def power_method(proj, iterations: int = 10, seed: int = 0, subset: int = 0) -> float:
+    """Step size tau = 1 / L with L = subsets * largest eigenvalue of A_k^T A_k (power iteration),
+    as util/powermethod.py:9-168 computes it for PDHG."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.randn(int(proj.N[0]), device="cuda", generator=g).abs()
+    x = x / torch.linalg.vector_norm(x)
+    L = 1.0
+    for _ in range(iterations):
+        x2 = proj.backwardProject(proj.forwardProject(x, subset), subset)
+        L = float(torch.dot(x, x2) / torch.dot(x, x)) * proj.subsets
+        x = x2 / torch.linalg.vector_norm(x2)
+    return 1.0 / L
+
+
+class PDHGRunner:
+    """PDHG with subsets for the least-squares data term, in the operator-API form of the reference
+    (source/Python/CBCT_custom_algorithm_example_PyTorchCUDA.py:235-264; dual step
+    computeForwardProject.h:227-316; image step PDHGUpdate auxKernels.cl:2034-2059):
+
+        p_k' = (p_k + sigma (A_k f - y_k)) / (1 + sigma);  dg = A_k^T (p_k' - p_k);  p_k = p_k'
+        g1 += dg;  g = g1 + theta * subsets * dg;  f = max(epps, f - tau g)
+    """
+
+    def __init__(self, proj, meas, tau: float, sigma: float = 1.0, theta: float = 1.0, epps: float = 1e-5,
+                 positivity: bool = True):
+        import torch
+        self.torch = torch
+        self.proj = proj
+        if not proj.projectorInitialized:
+            proj.initProj()
+        self.meas = meas.reshape(-1).contiguous()
+        self.tau, self.sigma, self.theta, self.epps = float(tau), float(sigma), float(theta), float(epps)
+        self.pos = int(bool(positivity))
+        self.p = [torch.zeros(int(proj.nMeasSubset[k, 0]), device=self.meas.device) for k in range(proj.subsets)]
+
+    def run(self, n_iter: int, x0=None):
+        torch = self.torch
+        pr = self.proj
+        N = int(pr.N[0])
+        f = torch.zeros(N, device=self.meas.device) if x0 is None else x0.reshape(-1).float().clone()
+        g1 = torch.zeros(N, device=self.meas.device)
+        for _ in range(n_iter):
+            for k in range(pr.subsets):
+                off, n = int(pr.nTotMeas[k]), int(pr.nMeasSubset[k, 0])
+                yk = self.meas[off:off + n]
+                ax = pr.forwardProject(f, k)
+                pl = self.p[k].clone()
+                _lib.call("omg_pdhg_dual_l2", _p(pl), _p(ax), _p(yk), n, self.sigma, None)
+                diff = pl - self.p[k]
+                dg = pr.backwardProject(diff, k)
+                self.p[k] = pl
+                g1 += dg
+                gk = g1 + (self.theta * pr.subsets) * dg
+                _lib.call("omg_pdhg_update", _p(f), _p(gk), None, N, self.epps, 1.0, self.tau, self.pos, 1, None)
+        return f

@@ -508,3 +508,319 @@ extern "C" void oracle_osl_update_f32(float* x, const float* S, const float* dU,
         x[i] = v;
     }
 }
+
+// ------------------------------------------------------------------------------------------------
+// Branchless distance-driven projector (projector type 5), CT flat panel. There is no CPU BDD in
+// the reference: this restates the GPU kernels projectorType5Forward / projectorType5Backward
+// (source/opencl/projectorType5.cl:64-337, :381-756) with the summed-area tables built as the
+// operator API does (projfunctions.py:203-238, :869-878) and texture fetches replaced by exact
+// FP32 bilinear interpolation in normalised coordinates with clamp-to-edge (the slab/projection
+// coordinate selects the slab exactly). The per-projection back-projection geometry restates
+// computeGeom5 (init.py:6-37). Forward projection groups 8 detector columns per thread as the
+// reference does (NVOXELSFP = 8): the footprints of columns 1..7 are the first column's shifted
+// by its own height.
+namespace bdd {
+
+struct BGeo {
+    uint32_t Nx, Ny, Nz, nRows, nCols, nProj;
+    float dx, dy, dz, bx, by, bz, pitchX, pitchY;
+};
+
+// volume SATs: IY[j][z'][x'] (y-slabs, (Nx+1)x(Nz+1)), IX[i][z'][y'] (x-slabs, (Ny+1)x(Nz+1)):
+// first the prefix along z of each column, then the prefix along the table's fast axis.
+static void volume_sats(const BGeo& g, const float* vol, std::vector<float>& IY, std::vector<float>& IX) {
+    const size_t WX = g.Nx + 1, WY = g.Ny + 1, H = g.Nz + 1;
+    IY.assign(WX * H * g.Ny, 0.f);
+    IX.assign(WY * H * g.Nx, 0.f);
+    for (uint32_t j = 0; j < g.Ny; j++)
+        for (uint32_t x = 0; x < g.Nx; x++) {
+            float acc = 0.f;
+            for (uint32_t z = 0; z < g.Nz; z++) {
+                const float v = vol[x + (size_t)j * g.Nx + (size_t)z * g.Nx * g.Ny];
+                acc += v;
+                IY[(size_t)j * WX * H + (size_t)(z + 1) * WX + x + 1] = acc;
+            }
+        }
+    for (uint32_t i = 0; i < g.Nx; i++)
+        for (uint32_t y = 0; y < g.Ny; y++) {
+            float acc = 0.f;
+            for (uint32_t z = 0; z < g.Nz; z++) {
+                acc += vol[i + (size_t)y * g.Nx + (size_t)z * g.Nx * g.Ny];
+                IX[(size_t)i * WY * H + (size_t)(z + 1) * WY + y + 1] = acc;
+            }
+        }
+    for (uint32_t j = 0; j < g.Ny; j++)
+        for (uint32_t z = 1; z <= g.Nz; z++) {
+            float acc = 0.f;
+            float* row = &IY[(size_t)j * WX * H + (size_t)z * WX];
+            for (uint32_t x = 1; x <= g.Nx; x++) { acc += row[x]; row[x] = acc; }
+        }
+    for (uint32_t i = 0; i < g.Nx; i++)
+        for (uint32_t z = 1; z <= g.Nz; z++) {
+            float acc = 0.f;
+            float* row = &IX[(size_t)i * WY * H + (size_t)z * WY];
+            for (uint32_t y = 1; y <= g.Ny; y++) { acc += row[y]; row[y] = acc; }
+        }
+}
+
+// sinogram SAT per projection: S[k][c'][r'] ((nRows+1) x (nCols+1)): prefix along columns, then rows
+static void sino_sat(const BGeo& g, const float* y, std::vector<float>& S) {
+    const size_t W = g.nRows + 1, H = g.nCols + 1;
+    S.assign(W * H * g.nProj, 0.f);
+    for (uint32_t k = 0; k < g.nProj; k++) {
+        for (uint32_t r = 0; r < g.nRows; r++) {
+            float acc = 0.f;
+            for (uint32_t c = 0; c < g.nCols; c++) {
+                acc += y[r + (size_t)c * g.nRows + (size_t)k * g.nRows * g.nCols];
+                S[(size_t)k * W * H + (size_t)(c + 1) * W + r + 1] = acc;
+            }
+        }
+        for (uint32_t c = 1; c <= g.nCols; c++) {
+            float acc = 0.f;
+            float* row = &S[(size_t)k * W * H + (size_t)c * W];
+            for (uint32_t r = 1; r <= g.nRows; r++) { acc += row[r]; row[r] = acc; }
+        }
+    }
+}
+
+// bilinear fetch of a W x H table at normalised (u, v), clamp to edge
+static inline float tex2(const float* T, uint32_t W, uint32_t H, float u, float v) {
+    const float cx = u * (float)W - 0.5f, cy = v * (float)H - 0.5f;
+    const float fx = std::floor(cx), fy = std::floor(cy);
+    const float a = cx - fx, b = cy - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    int x1 = x0 + 1, y1 = y0 + 1;
+    x0 = std::min(std::max(x0, 0), (int)W - 1); x1 = std::min(std::max(x1, 0), (int)W - 1);
+    y0 = std::min(std::max(y0, 0), (int)H - 1); y1 = std::min(std::max(y1, 0), (int)H - 1);
+    const float r0 = (1.f - a) * T[(size_t)y0 * W + x0] + a * T[(size_t)y0 * W + x1];
+    const float r1 = (1.f - a) * T[(size_t)y1 * W + x0] + a * T[(size_t)y1 * W + x1];
+    return (1.f - b) * r0 + b * r1;
+}
+
+static inline float len3(float x, float y, float z) { return std::sqrt(x * x + y * y + z * z); }
+
+constexpr int kCols = 8;
+
+// Forward projection of one group of up to 8 detector columns (projectorType5.cl:155-336).
+static void fp_group(const BGeo& g, const float* xyz, const float* uv, const float* IY, const float* IX,
+                     uint32_t r, uint32_t c0, uint32_t k, float* out) {
+    const float* P = xyz + (size_t)k * 6;
+    const float sx = P[0], sy = P[1], sz = P[2];
+    float dx_ = P[3], dy_ = P[4], dz_ = P[5];
+    const float i1 = (float)r - (float)g.nRows / 2.f + .5f, i2 = (float)c0 - (float)g.nCols / 2.f + .5f;
+    const float ux = uv[(size_t)k * 2], uy = uv[(size_t)k * 2 + 1];
+    dx_ += i1 * ux;
+    dy_ += i1 * uy;
+    dz_ += i2 * g.pitchY;
+    const float dRx = dx_ - ux * 0.5f, dRy = dy_ - uy * 0.5f;
+    const float dLx = dx_ + ux * 0.5f, dLy = dy_ + uy * 0.5f;
+    const float dUz = dz_ + g.pitchY * 0.5f, dDz = dz_ - g.pitchY * 0.5f;
+    const float ex = dx_ - sx, ey = dy_ - sy, ez = dz_ - sz;
+    const float inv = 1.f / len3(ex, ey, ez);
+    const float uvx = std::fabs(ex * inv), uvy = std::fabs(ey * inv);
+    float kerroin = g.dx * g.dy * g.dz;
+    const float apuZ = (g.bz - sz - g.dz / 2.f);
+    const int maxZZ = std::min(kCols, (int)g.nCols - (int)c0);
+    float temp[kCols] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const float scaleX = 1.f / (g.dx * (float)(g.Nx + 1)), scaleY = 1.f / (g.dy * (float)(g.Ny + 1));
+    const float scaleZ = 1.f / (g.dz * (float)(g.Nz + 1));
+    if (std::fabs(sx) <= std::fabs(sy)) {
+        const float X0 = (dLx - sx) / (dLy - sy), X1 = (dRx - sx) / (dRy - sy);
+        const float Z0 = (dUz - sz) / (dy_ - sy), Z1 = (dDz - sz) / (dy_ - sy);
+        kerroin /= uvy;
+        const float apuX = (g.bx - sx - g.dx / 2.f);
+        const float apuV = g.by - sy;
+        const uint32_t W = g.Nx + 1, H = g.Nz + 1;
+        float half = g.dy / 2.f;
+        for (uint32_t jj = 0; jj < g.Ny; jj++) {
+            const float v = half + apuV;
+            half += g.dy;
+            const float a0 = v * X0 - apuX, a1 = v * X1 - apuX;
+            const float b0 = v * Z0 - apuZ, b1 = v * Z1 - apuZ;
+            float xl = std::min(a0, a1), xr = std::max(a0, a1);
+            float zu = std::min(b0, b1), zd = std::max(b0, b1);
+            const float invArea = 1.f / std::fabs((xl - xr) * (zu - zd));
+            xl *= scaleX; xr *= scaleX; zu *= scaleZ; zd *= scaleZ;
+            const float* T = IY + (size_t)jj * W * H;
+            float A = tex2(T, W, H, xr, zu), B = tex2(T, W, H, xl, zd);
+            float C = tex2(T, W, H, xr, zd), D = tex2(T, W, H, xl, zu);
+            temp[0] += (C + D - A - B) * invArea;
+            const float step = std::fabs(zu - zd);
+            for (int q = 1; q < maxZZ; q++) {
+                D = B; A = C;
+                zd += step;
+                B = tex2(T, W, H, xl, zd);
+                C = tex2(T, W, H, xr, zd);
+                temp[q] += (C + D - A - B) * invArea;
+            }
+        }
+    } else {
+        const float Y0 = (dLy - sy) / (dLx - sx), Y1 = (dRy - sy) / (dRx - sx);
+        const float Z0 = (dUz - sz) / (dx_ - sx), Z1 = (dDz - sz) / (dx_ - sx);
+        kerroin /= uvx;
+        const float apuY = (g.by - sy - g.dy / 2.f);
+        const float apuV = g.bx - sx;
+        const uint32_t W = g.Ny + 1, H = g.Nz + 1;
+        float half = g.dx / 2.f;
+        for (uint32_t ii = 0; ii < g.Nx; ii++) {
+            const float v = half + apuV;
+            half += g.dx;
+            const float a0 = v * Y0 - apuY, a1 = v * Y1 - apuY;
+            const float b0 = v * Z0 - apuZ, b1 = v * Z1 - apuZ;
+            float yl = std::min(a0, a1), yr = std::max(a0, a1);
+            float zu = std::min(b0, b1), zd = std::max(b0, b1);
+            const float invArea = 1.f / std::fabs((yl - yr) * (zu - zd));
+            yl *= scaleY; yr *= scaleY; zu *= scaleZ; zd *= scaleZ;
+            const float* T = IX + (size_t)ii * W * H;
+            float A = tex2(T, W, H, yr, zu), B = tex2(T, W, H, yl, zd);
+            float C = tex2(T, W, H, yr, zd), D = tex2(T, W, H, yl, zu);
+            temp[0] += (C + D - A - B) * invArea;
+            const float step = std::fabs(zu - zd);
+            for (int q = 1; q < maxZZ; q++) {
+                D = B; A = C;
+                zd += step;
+                B = tex2(T, W, H, yl, zd);
+                C = tex2(T, W, H, yr, zd);
+                temp[q] += (C + D - A - B) * invArea;
+            }
+        }
+    }
+    for (int q = 0; q < maxZZ; q++) out[r + (size_t)(c0 + q) * g.nRows + (size_t)k * g.nRows * g.nCols] = temp[q] * kerroin;
+}
+
+// Per-projection back-projection geometry (computeGeom5, init.py:6-37): s, d3, normX, normY, crossP, upperPart.
+static void geom5(const BGeo& g, const float* xyz, const float* uv, float* out) {
+    const float indX = (float)g.nRows / 2.f, indY = (float)g.nCols / 2.f;
+    for (uint32_t k = 0; k < g.nProj; k++) {
+        const float* P = xyz + (size_t)k * 6;
+        const float s[3] = {P[0], P[1], P[2]}, d[3] = {P[3], P[4], P[5]};
+        const float ax[3] = {uv[2 * k] * indX, uv[2 * k + 1] * indX, 0.f};
+        const float ay[3] = {0.f, 0.f, indY * g.pitchY};
+        float d3[3], d2[3], nx[3], ny[3], cp[3];
+        for (int a = 0; a < 3; a++) { d3[a] = d[a] - ax[a] - ay[a]; d2[a] = ax[a] - ay[a]; }
+        const float lx = len3(ax[0], ax[1], ax[2]), ly = len3(ay[0], ay[1], ay[2]);
+        for (int a = 0; a < 3; a++) { nx[a] = ax[a] / lx; ny[a] = ay[a] / ly; }
+        const float e[3] = {d3[0] - d[0], d3[1] - d[1], d3[2] - d[2]};
+        cp[0] = d2[1] * e[2] - d2[2] * e[1];
+        cp[1] = d2[2] * e[0] - d2[0] * e[2];
+        cp[2] = d2[0] * e[1] - d2[1] * e[0];
+        const float up = cp[0] * (s[0] - d[0]) + cp[1] * (s[1] - d[1]) + cp[2] * (s[2] - d[2]);
+        float* o = out + (size_t)k * 16;
+        for (int a = 0; a < 3; a++) { o[a] = s[a]; o[3 + a] = d3[a]; o[6 + a] = nx[a]; o[9 + a] = ny[a]; o[12 + a] = cp[a]; }
+        o[15] = up;
+    }
+}
+
+// Back-projection of one voxel over all projections (projectorType5.cl:476-756, NVOXELS5 = 1).
+static void bp_voxel(const BGeo& g, const float* G5, const float* S, uint32_t i, uint32_t j, uint32_t kz, float& out,
+                     float& wsum) {
+    const float vx = (float)i * g.dx + g.dx / 2.f + g.bx;
+    const float vy = (float)j * g.dy + g.dy / 2.f + g.by;
+    const float vz = (float)kz * g.dz + g.dz / 2.f + g.bz;
+    const float kokoX = (float)(g.nRows + 1) * g.pitchX, kokoY = (float)(g.nCols + 1) * g.pitchY;
+    const float invKx = 1.f / kokoX, invKy = 1.f / kokoY;
+    const uint32_t W = g.nRows + 1, H = g.nCols + 1;
+    float temp = 0.f, ws = 0.f;
+    for (uint32_t kk = 0; kk < g.nProj; kk++) {
+        const float* o = G5 + (size_t)kk * 16;
+        const float s[3] = {o[0], o[1], o[2]};
+        const bool sxDim = std::fabs(s[0]) <= std::fabs(s[1]);
+        float lu[3], rd[3];
+        if (sxDim) {
+            if (s[1] > 0.f) { lu[0] = vx - g.dx / 2.f; rd[0] = vx + g.dx / 2.f; }
+            else { lu[0] = vx + g.dx / 2.f; rd[0] = vx - g.dx / 2.f; }
+            lu[1] = vy; rd[1] = vy;
+        } else {
+            if (s[0] > 0.f) { lu[1] = vy + g.dy / 2.f; rd[1] = vy - g.dy / 2.f; }
+            else { lu[1] = vy - g.dy / 2.f; rd[1] = vy + g.dy / 2.f; }
+            lu[0] = vx; rd[0] = vx;
+        }
+        lu[2] = vz - g.dz / 2.f; rd[2] = vz + g.dz / 2.f;
+        for (int a = 0; a < 3; a++) { lu[a] -= s[a]; rd[a] -= s[a]; }
+        const float tLU = o[15] / (-lu[0] * o[12] + -lu[1] * o[13] + -lu[2] * o[14]);
+        const float tRD = o[15] / (-rd[0] * o[12] + -rd[1] * o[13] + -rd[2] * o[14]);
+        float pl[3], pr[3];
+        for (int a = 0; a < 3; a++) {
+            pl[a] = lu[a] * tLU + s[a] - o[3 + a];
+            pr[a] = rd[a] * tRD + s[a] - o[3 + a];
+        }
+        float Ax = pl[0] * o[6] + pl[1] * o[7] + pl[2] * o[8] + g.pitchX / 2.f;
+        float Ay = pl[0] * o[9] + pl[1] * o[10] + pl[2] * o[11] + g.pitchY / 2.f;
+        float Dx = pr[0] * o[6] + pr[1] * o[7] + pr[2] * o[8] + g.pitchX / 2.f;
+        float Dy = pr[0] * o[9] + pr[1] * o[10] + pr[2] * o[11] + g.pitchY / 2.f;
+        if (Dx > Ax) std::swap(Ax, Dx);
+        if (Dy > Ay) std::swap(Ay, Dy);
+        const float* T = S + (size_t)kk * W * H;
+        const float AxN = Ax * invKx, DxN = Dx * invKx, AyN = Ay * invKy, DyN = Dy * invKy;
+        const float A = tex2(T, W, H, AxN, AyN), B = tex2(T, W, H, AxN, DyN);
+        const float C = tex2(T, W, H, DxN, AyN), D = tex2(T, W, H, DxN, DyN);
+        const float vdx = vx - s[0], vdy = vy - s[1], vdz = vz - s[2];
+        const float kerroin = sxDim ? g.dy * len3(vdx, vdy, vdz) / std::fabs(vdy)
+                                    : g.dx * len3(vdx, vdy, vdz) / std::fabs(vdx);
+        const float apu = A + D - C - B;
+        if (apu != 0.f) {
+            temp += apu * kerroin;
+            ws += kerroin;
+        }
+    }
+    out = temp;
+    wsum = ws;
+}
+
+}  // namespace bdd
+
+static bdd::BGeo make_bgeo(const uint32_t* n3, const float* d3, const float* b3, uint32_t nRows, uint32_t nCols,
+                           uint32_t nProj, float pitchX, float pitchY) {
+    bdd::BGeo g;
+    g.Nx = n3[0]; g.Ny = n3[1]; g.Nz = n3[2];
+    g.dx = d3[0]; g.dy = d3[1]; g.dz = d3[2];
+    g.bx = b3[0]; g.by = b3[1]; g.bz = b3[2];
+    g.nRows = nRows; g.nCols = nCols; g.nProj = nProj; g.pitchX = pitchX; g.pitchY = pitchY;
+    return g;
+}
+
+extern "C" int oracle_bdd_fp_f32(const uint32_t* n3, const float* d3, const float* b3, uint32_t nRows, uint32_t nCols,
+                                 uint32_t nProj, float pitchX, float pitchY, const float* xyz, const float* uv,
+                                 const float* vol, float* y, int nthreads) {
+    bdd::BGeo g = make_bgeo(n3, d3, b3, nRows, nCols, nProj, pitchX, pitchY);
+    std::vector<float> IY, IX;
+    bdd::volume_sats(g, vol, IY, IX);
+    set_threads(nthreads);
+    const int64_t groups = (nCols + bdd::kCols - 1) / bdd::kCols;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t t = 0; t < (int64_t)nRows * groups * nProj; t++) {
+        const uint32_t r = (uint32_t)(t % nRows);
+        const uint32_t c0 = (uint32_t)((t / nRows) % groups) * bdd::kCols;
+        const uint32_t k = (uint32_t)(t / (nRows * groups));
+        bdd::fp_group(g, xyz, uv, IY.data(), IX.data(), r, c0, k, y);
+    }
+    return 0;
+}
+
+extern "C" int oracle_bdd_bp_f32(const uint32_t* n3, const float* d3, const float* b3, uint32_t nRows, uint32_t nCols,
+                                 uint32_t nProj, float pitchX, float pitchY, const float* xyz, const float* uv,
+                                 const float* y, float* vol, float* sens, int nthreads) {
+    bdd::BGeo g = make_bgeo(n3, d3, b3, nRows, nCols, nProj, pitchX, pitchY);
+    std::vector<float> S, G5(16 * (size_t)nProj);
+    bdd::sino_sat(g, y, S);
+    bdd::geom5(g, xyz, uv, G5.data());
+    set_threads(nthreads);
+    const int64_t N = (int64_t)g.Nx * g.Ny * g.Nz;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t n = 0; n < N; n++) {
+        const uint32_t i = (uint32_t)(n % g.Nx), j = (uint32_t)((n / g.Nx) % g.Ny), kz = (uint32_t)(n / ((int64_t)g.Nx * g.Ny));
+        float o, w;
+        bdd::bp_voxel(g, G5.data(), S.data(), i, j, kz, o, w);
+        vol[n] += o;
+        if (sens) sens[n] += w;
+    }
+    return 0;
+}
+
+extern "C" int oracle_bdd_geom5_f32(const uint32_t* n3, const float* d3, const float* b3, uint32_t nRows, uint32_t nCols,
+                                    uint32_t nProj, float pitchX, float pitchY, const float* xyz, const float* uv,
+                                    float* out) {
+    bdd::BGeo g = make_bgeo(n3, d3, b3, nRows, nCols, nProj, pitchX, pitchY);
+    bdd::geom5(g, xyz, uv, out);
+    return 0;
+}

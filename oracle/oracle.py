@@ -58,6 +58,10 @@ def lib():
             f = getattr(_lib, "oracle_rdp_gradient_" + suf)
             f.argtypes = [p, p, C.c_int, C.c_int, C.c_int, T, T, T, C.c_int]
             f.restype = C.c_int
+        bgeo = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _f32p, _f32p]
+        _lib.oracle_bdd_fp_f32.argtypes = bgeo + [_f32p, _f32p, C.c_int]
+        _lib.oracle_bdd_bp_f32.argtypes = bgeo + [_f32p, _f32p, C.c_void_p, C.c_int]
+        _lib.oracle_bdd_geom5_f32.argtypes = bgeo + [_f32p]
         _lib.oracle_ratio_f32.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
         _lib.oracle_sens_clamp_f32.argtypes = [_f32p, C.c_int64, C.c_float]
         _lib.oracle_osem_update_f32.argtypes = [_f32p, _f32p, _f32p, C.c_int64, C.c_float, C.c_int]
@@ -238,3 +242,85 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
                 x = osem_update(x, S[k], rhs, epps, positivity)
             done += 1
     return x
+
+
+def cgls(case, meas, n_iter, nmeas=None):
+    """CGLS recurrences (functions.hpp:2315-2340, computeForwardProject.h:197-206, algorithms.h:243-269)
+    with the oracle projector of ``case`` (float64 vectors)."""
+    n = meas.size if nmeas is None else nmeas
+    y = np.asarray(meas, dtype=case.dtype)
+    f = np.zeros(case.grid.N, dtype=case.dtype)
+    r = y.copy()
+    p = case.bp(r, n)
+    gamma = float(np.dot(p.astype(np.float64), p.astype(np.float64)))
+    for _ in range(n_iter):
+        q = case.fp(p, n)
+        alpha = gamma / float(np.dot(q.astype(np.float64), q.astype(np.float64)))
+        r = r - alpha * q
+        s = case.bp(r, n)
+        gamma_new = float(np.dot(s.astype(np.float64), s.astype(np.float64)))
+        f = f + alpha * p
+        p = s + (gamma_new / gamma) * p
+        gamma = gamma_new
+    return f
+
+
+def pdhg(cases, meas_parts, n_iter, tau, sigma=1.0, theta=1.0, epps=1e-5):
+    """PDHG with subsets, least-squares data term (CBCT_custom_algorithm_example_PyTorchCUDA.py:235-264,
+    PDHGUpdate auxKernels.cl:2034-2059), one oracle case per subset."""
+    N = cases[0].grid.N
+    f = np.zeros(N, dtype=np.float64)
+    g1 = np.zeros(N, dtype=np.float64)
+    p = [np.zeros(m.size) for m in meas_parts]
+    S = len(cases)
+    for _ in range(n_iter):
+        for k, c in enumerate(cases):
+            n = meas_parts[k].size
+            ax = c.fp(f.astype(c.dtype), n).astype(np.float64)
+            pl = (p[k] + sigma * (ax - meas_parts[k])) / (1.0 + sigma)
+            dg = c.bp((pl - p[k]).astype(c.dtype), n).astype(np.float64)
+            p[k] = pl
+            g1 = g1 + dg
+            g = g1 + (theta * S) * dg
+            f = np.maximum(epps, f - tau * g)
+    return f
+
+
+class BDDCase:
+    """Branchless distance-driven projector of the oracle on a CT flat-panel geometry."""
+
+    def __init__(self, grid, nRows, nCols, xyz, uv, pitchX, pitchY):
+        self.grid = grid
+        self.n3 = np.ascontiguousarray(grid.n3())
+        self.d3 = grid.d3(np.float32)
+        self.b3 = grid.b3(np.float32)
+        self.nRows, self.nCols = int(nRows), int(nCols)
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        self.uv = np.ascontiguousarray(uv, dtype=np.float32)
+        self.nProj = self.xyz.size // 6
+        self.pitchX, self.pitchY = float(pitchX), float(pitchY)
+
+    def _g(self):
+        return [self.n3.ctypes.data, self.d3, self.b3, self.nRows, self.nCols, self.nProj, self.pitchX, self.pitchY,
+                self.xyz, self.uv]
+
+    @property
+    def M(self):
+        return self.nRows * self.nCols * self.nProj
+
+    def fp(self, vol, nthreads=0):
+        y = np.zeros(self.M, dtype=np.float32)
+        lib().oracle_bdd_fp_f32(*self._g(), np.ascontiguousarray(vol, dtype=np.float32), y, nthreads)
+        return y
+
+    def bp(self, y, sens=False, nthreads=0):
+        v = np.zeros(self.grid.N, dtype=np.float32)
+        s = np.zeros(self.grid.N, dtype=np.float32) if sens else None
+        lib().oracle_bdd_bp_f32(*self._g(), np.ascontiguousarray(y, dtype=np.float32), v,
+                                None if s is None else s.ctypes.data, nthreads)
+        return (v, s) if sens else v
+
+    def geom5(self):
+        out = np.zeros(16 * self.nProj, dtype=np.float32)
+        lib().oracle_bdd_geom5_f32(*self._g(), out)
+        return out
