@@ -57,8 +57,9 @@ typedef struct omg_geom_desc {
     const float* z; int64_t z_len;
     const float* norm; int64_t norm_len;  /* optional per-measurement normalisation (full data order) */
     /* orthogonal distance-based projector (projector_type 2) */
-    float orth_width;          /* tube width (FWHM) in mm: tube_width_z for 3D, tube_width_xy for 2D */
-    int32_t orth_3d;           /* 1: 3D (axial) orthogonal distance, 0: transaxial only */
+    float orth_width;          /* kernel tube width in mm (the reference passes tube_width_z, init.py:652) */
+    int32_t orth_flags;        /* bit 0: transaxial row sweep (-DCRYSTXY, tube_width_xy > 0);
+                                  bit 1: axial plane sweep (-DCRYSTZ, tube_width_z > 0) (init.py:286-290) */
     /* branchless distance-driven projector (projector_type 5); CT mode only */
     float dPitchX;             /* transaxial detector pitch */
     float dPitchY;             /* axial detector pitch */
@@ -92,7 +93,7 @@ int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_meas, const
                          void* stream);
 
 /* One OSEM subset data step, fused: yhat = A_k x; r = meas / (yhat + epps) (NaN/Inf -> epps);
- * rhs += A_k^T r; sens += A_k^T 1 when sens != NULL. PET modes only. Replaces the
+ * rhs += A_k^T r; sens += A_k^T 1 when sens != NULL. PET modes, projector_type 1 or 2. Replaces the
  * forwardProjectionAFOpenCL -> computeForwardStep -> backwardProjectionAFOpenCL sequence
  * of reconstructionAF (source/cpp/reconstructionAF.h:1089-1264, computeForwardProject.h:20-21,94,400-401). */
 int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_meas, const float* img, const float* meas,
@@ -104,7 +105,7 @@ int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_meas, const f
  * S < epps -> epps (stored to S; else S is read); OSEM x = x / S * (rhs + epps) (subiterStep.h:111-121)
  * or OSL x = x / (S + dU) * rhs with dU = beta * grad RDP(x; gamma, epps), zero boundary
  * (subiterStep.h:219-231, RDP.h:4-78); positivity x < epps -> epps when positivity != 0.
- * Emission data, projector_type 1. Uses library-owned work buffers of the geometry handle:
+ * Emission data, projector_type 1 or 2. Uses library-owned work buffers of the geometry handle:
  * calls on one handle must not overlap. */
 int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
                               int32_t compute_sens, float epps, float beta, float gamma, int32_t positivity,
@@ -118,11 +119,12 @@ int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, 
 int omg_timing_enable(int32_t on);
 int omg_timing_collect(double* total_ms, int64_t* launches);
 
-/* Per-LOR voxel lists of the Siddon traversal (verification of the traversal itself).
- * counts[i] = number of voxels with length > 0 for measurement i. If offsets != NULL
- * (exclusive prefix sums of counts), voxel indices and lengths are written to idx/len.
- * Mirrors the index/element lists of improved_siddon_precomputed
- * (source/cpp/improved_siddon_precomputed.cpp:32-251). */
+/* Per-LOR voxel lists of the ray tracer (verification of the traversal itself).
+ * counts[i] = number of voxels visited for measurement i. If offsets != NULL (exclusive prefix sums of
+ * counts), voxel indices are written to idx and per-voxel weights to len: the intersection length for
+ * projector_type 1 (the index/element lists of improved_siddon_precomputed,
+ * source/cpp/improved_siddon_precomputed.cpp:32-251), the tube weight 1 - d/width in visiting order for
+ * projector_type 2 (orthogonalHelper3D, source/opencl/opencl_functions_orth3D.h:85-193). */
 int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* counts, const int64_t* offsets,
                      uint32_t* idx, float* len, void* stream);
 

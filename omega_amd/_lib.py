@@ -46,7 +46,7 @@ class GeomDesc(C.Structure):
         ("z", C.c_void_p), ("z_len", C.c_int64),
         ("norm", C.c_void_p), ("norm_len", C.c_int64),
         ("orth_width", C.c_float),
-        ("orth_3d", C.c_int32),
+        ("orth_flags", C.c_int32),
         ("dPitchX", C.c_float),
         ("dPitchY", C.c_float),
         ("reserved", C.c_int32 * 8),

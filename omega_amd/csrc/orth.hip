@@ -1,14 +1,525 @@
-// orth.hip -- orthogonal distance-based projector (projector type 2).
+// orth.hip -- orthogonal distance-based projector (projector type 2) for gfx950.
+//
+// Semantics are those of the reference's GPU kernel, projectorType123.cl built with -DORTH and
+// -DCRYSTXY / -DCRYSTZ (no TOF, attenuation, masks or multi-ray), because the reference's CPU version
+// of this projector is one-sided (projector_functions.h:1121-1123,1145-1147):
+//   * a Siddon walk (GPU set-up: siddon_pre_loop_2D/3D, general_opencl_functions.h:1825-1957) fixes the
+//     slab axis and, at every crossing of a slab boundary, the row/plane the ray leaves the slab through
+//     (projectorType123.cl:991-1161);
+//   * each slab is swept outwards from that row (and plane, with the axial sweep) while the voxel centre's
+//     distance d to the ray stays below 0.99 * width; each voxel gets w = 1 - d / width
+//     (orthDistance3D / orthogonalHelper3D, opencl_functions_orth3D.h:85-416);
+//   * slabs before the first and after the last crossing are swept until one yields no voxel
+//     (projectorType123.cl:1055-1128,1235-1313);
+//   * rays parallel to x or y walk their row of slabs directly (projectorType123.cl:513-791).
+// FP: y = temp * sum w x; BP: rhs += r * (w * temp), sens += w * temp, temp = norm * global_factor.
+//
+// One lane follows one LOR. FMAD of the reference is fmaf here; everything else is evaluated as written
+// (the library is built with -ffp-contract=off and IEEE division/sqrt), so a forward projection is
+// bit-identical to the oracle's restatement of the same kernel.
 #include "geom.hpp"
 
-int orth_prepare(omg_geom*) { return OMG_OK; }
+namespace omg {
+
+constexpr int kOrthBlock = 256;
+constexpr float kOrthThr = 0.01f;
+constexpr double kCosLo = 0.7071068921727742;    // cos(0.785398f), in double
+constexpr double kCosHi = -0.7071035822349067;   // cos(2.35619f), in double
+
+struct OrthArgs {
+    SidArgs a;
+    float width;        // kernel tube width in mm (tube_width_z)
+    int32_t cxy, cz;    // transaxial row sweep, axial sweep
+};
+
+struct F3 { float x, y, z; };
+
+__device__ __forceinline__ uint32_t f2u_ru(float v) {
+    if (!(v > 0.f)) return 0u;
+    const float c = ceilf(v);
+    return c >= 4294967296.f ? 4294967295u : (uint32_t)c;
+}
+__device__ __forceinline__ uint32_t f2u_rd(float v) {
+    if (!(v > 0.f)) return 0u;
+    const float c = floorf(v);
+    return c >= 4294967296.f ? 4294967295u : (uint32_t)c;
+}
+
+__device__ __forceinline__ void o_dgs(float tmin, float t_min, float tmax, float t_max, uint32_t& vmin, uint32_t& vmax,
+                                      float& t0, int& vu, float diff, float b, float d, float s, uint32_t N) {
+    if (tmin == t_min) vmin = 1u;
+    else { const float pt = s + tmin * diff; vmin = f2u_ru((pt - b) / d); }
+    if (tmax == t_max) vmax = N;
+    else { const float pt = s + tmax * diff; vmax = f2u_rd((pt - b) / d); }
+    t0 += ((float)vmin * d) / diff;
+    vu = 1;
+}
+__device__ __forceinline__ void o_sgd(float tmin, float t_min, float tmax, float t_max, uint32_t& vmin, uint32_t& vmax,
+                                      float& t0, int& vu, float diff, float b, float d, float s, uint32_t N) {
+    if (tmin == t_min) vmax = N - 1u;
+    else { const float pt = s + tmin * diff; vmax = f2u_rd((pt - b) / d); }
+    if (tmax == t_max) vmin = 0u;
+    else { const float pt = s + tmax * diff; vmin = f2u_ru((pt - b) / d); }
+    t0 += ((float)vmax * d) / diff;
+    vu = -1;
+}
+
+__device__ __forceinline__ int o_perp_start(float b, float d, float dd, uint32_t N) {
+    int t = (int)floorf((d - b) / dd);
+    if (t < 0 || t >= (int)N) t = 0;
+    return t;
+}
+
+__device__ __forceinline__ bool o_pre2(float b1, float b2, float diff1, float diff2, float max1, float max2, float d1,
+                                       float d2, uint32_t N1, uint32_t N2, int& temp1, int& temp2, float& t1u,
+                                       float& t2u, uint32_t& Np, float ys, float xs, float yd, float xd, float& tc,
+                                       int& u1, int& u2, float& t10, float& t20, bool& xy) {
+    const float apu_tx = b1 - xs, apu_ty = b2 - ys;
+    t10 = apu_tx / diff1;
+    t20 = apu_ty / diff2;
+    const float txback = (max1 - xs) / diff1, tyback = (max2 - ys) / diff2;
+    const float txmin = fminf(t10, txback), txmax = fmaxf(t10, txback);
+    const float tymin = fminf(t20, tyback), tymax = fmaxf(t20, tyback);
+    tc = fmaxf(txmin, tymin);
+    const float tmax = fminf(txmax, tymax);
+    xy = (tc == t10 || tc == txback);
+    if (tc >= tmax) return true;
+    uint32_t imin, imax, jmin, jmax;
+    if (xs < xd) o_dgs(tc, txmin, tmax, txmax, imin, imax, t10, u1, diff1, b1, d1, xs, N1);
+    else o_sgd(tc, txmin, tmax, txmax, imin, imax, t10, u1, diff1, b1, d1, xs, N1);
+    if (ys < yd) o_dgs(tc, tymin, tmax, tymax, jmin, jmax, t20, u2, diff2, b2, d2, ys, N2);
+    else o_sgd(tc, tymin, tmax, tymax, jmin, jmax, t20, u2, diff2, b2, d2, ys, N2);
+    Np = imax + 1u + jmax + 1u - imin - jmin;
+    const float pt = (fminf(t10, t20) + tc) / 2.f;
+    temp1 = (int)((pt * diff1 - apu_tx) / d1);
+    temp2 = (int)((pt * diff2 - apu_ty) / d2);
+    t1u = d1 / fabsf(diff1);
+    t2u = d2 / fabsf(diff2);
+    return temp1 < 0 || temp2 < 0 || temp1 >= (int)N1 || temp2 >= (int)N2;
+}
+
+__device__ __forceinline__ bool o_pre3(const SidArgs& g, F3 s, F3 d, F3 diff, int& ti, int& tj, int& tk, float& txu,
+                                       float& tyu, float& tzu, uint32_t& Np, float& tc, int& ux, int& uy, int& uz,
+                                       float& tx0, float& ty0, float& tz0, bool& xy) {
+    const F3 apuT{g.bx - s.x, g.by - s.y, g.bz - s.z};
+    const F3 t0{apuT.x / diff.x, apuT.y / diff.y, apuT.z / diff.z};
+    const F3 tB{(g.bmaxx - s.x) / diff.x, (g.bmaxy - s.y) / diff.y, (g.bmaxz - s.z) / diff.z};
+    const F3 tMin{fminf(t0.x, tB.x), fminf(t0.y, tB.y), fminf(t0.z, tB.z)};
+    const F3 tMax{fmaxf(t0.x, tB.x), fmaxf(t0.y, tB.y), fmaxf(t0.z, tB.z)};
+    tc = fmaxf(fmaxf(tMin.x, tMin.z), tMin.y);
+    const float tmax = fminf(fminf(tMax.x, tMax.z), tMax.y);
+    tx0 = t0.x; ty0 = t0.y; tz0 = t0.z;
+    {
+        const float p = d.x - s.x, pY = d.y - s.y;
+        const float cx = p / sqrtf(p * p + pY * pY);
+        xy = ((double)cx > kCosLo && cx < 1.f) || ((double)cx < kCosHi);
+    }
+    if (tc >= tmax) return true;
+    uint32_t imin, imax, jmin, jmax, kmin, kmax;
+    if (s.x < d.x) o_dgs(tc, tMin.x, tmax, tMax.x, imin, imax, tx0, ux, diff.x, g.bx, g.dx, s.x, g.Nx);
+    else o_sgd(tc, tMin.x, tmax, tMax.x, imin, imax, tx0, ux, diff.x, g.bx, g.dx, s.x, g.Nx);
+    if (s.y < d.y) o_dgs(tc, tMin.y, tmax, tMax.y, jmin, jmax, ty0, uy, diff.y, g.by, g.dy, s.y, g.Ny);
+    else o_sgd(tc, tMin.y, tmax, tMax.y, jmin, jmax, ty0, uy, diff.y, g.by, g.dy, s.y, g.Ny);
+    if (s.z < d.z) o_dgs(tc, tMin.z, tmax, tMax.z, kmin, kmax, tz0, uz, diff.z, g.bz, g.dz, s.z, g.Nz);
+    else o_sgd(tc, tMin.z, tmax, tMax.z, kmin, kmax, tz0, uz, diff.z, g.bz, g.dz, s.z, g.Nz);
+    Np = (kmax - kmin + 1u) + (jmax - jmin + 1u) + (imax - imin + 1u);
+    const float pt = (fminf(fminf(tz0, ty0), tx0) + tc) / 2.f;
+    ti = (int)fminf(fmaxf(__builtin_fmaf(pt, diff.x, -apuT.x) / g.dx, 0.f), (float)(g.Nx - 1));
+    tj = (int)fminf(fmaxf(__builtin_fmaf(pt, diff.y, -apuT.y) / g.dy, 0.f), (float)(g.Ny - 1));
+    tk = (int)fminf(fmaxf(__builtin_fmaf(pt, diff.z, -apuT.z) / g.dz, 0.f), (float)(g.Nz - 1));
+    txu = g.dx / fabsf(diff.x);
+    tyu = g.dy / fabsf(diff.y);
+    tzu = g.dz / fabsf(diff.z);
+    return false;
+}
+
+// Slab geometry shared by every sweep of one LOR (after the frame swap: "x" is the slab axis).
+struct Tube {
+    F3 s, diff;
+    float b2, d2, bz, dz, orl;
+    uint32_t Nxy, N1, N2p, N3p, Nz;
+    bool cxy, cz;
+};
+
+// One voxel: true when it lies outside the cut-off (the row sweep stops there).
+template <class V>
+__device__ __forceinline__ bool o_voxel(const Tube& t, int tempi, int uu, int zz, const F3& l, float cy, V& visit) {
+    const float x0 = __builtin_fmaf(-1.f, t.s.y, cy);
+    const float y1 = __builtin_fmaf(t.diff.z, x0, -l.y);
+    const float z1 = __builtin_fmaf(-t.diff.x, x0, l.z);
+    const float norm2 = __builtin_fmaf(l.x, l.x, __builtin_fmaf(y1, y1, z1 * z1));
+    if (norm2 >= (1.f - kOrthThr) * (1.f - kOrthThr) * t.orl * t.orl) return true;
+    const float w = 1.f - sqrtf(norm2) / t.orl;
+    const int64_t ind = (int64_t)((uint32_t)tempi * t.N2p + (uint32_t)uu * t.N3p) + (int64_t)zz * (int64_t)t.Nxy;
+    visit(ind, w);
+    return false;
+}
+
+// Rows of one z plane. `recompute`: the upward rows take their centre from the row index (first z loop);
+// otherwise the centre accumulates from the start row (second z loop), as in the reference.
+template <class V>
+__device__ __forceinline__ void o_rows(const Tube& t, int tempi, int temp2, int zz, const F3& l, float centerY0,
+                                       bool recompute, int& uu, int& uu1, int& uu2, bool& brk, V& visit) {
+    const int maxXY = (int)t.N1;
+    if (t.cxy) {
+        float cy = centerY0;
+        for (uu1 = max(temp2, 0); uu1 < maxXY; uu1++) {
+            if (recompute) cy = t.b2 + (float)uu1 * t.d2 + t.d2 / 2.f;
+            brk = o_voxel(t, tempi, uu1, zz, l, cy, visit);
+            if (brk) break;
+            uu++;
+            cy += t.d2;
+        }
+        cy = centerY0 - t.d2;
+        for (uu2 = min(temp2, maxXY) - 1; uu2 >= 0; uu2--) {
+            brk = o_voxel(t, tempi, uu2, zz, l, cy, visit);
+            if (brk) break;
+            uu++;
+            cy -= t.d2;
+        }
+    } else {
+        uu1 = temp2;
+        const float cy = recompute ? t.b2 + (float)uu1 * t.d2 + t.d2 / 2.f : centerY0;
+        brk = o_voxel(t, tempi, uu1, zz, l, cy, visit);
+        uu2 = temp2 - 1;
+    }
+}
+
+// orthDistance3D: the tube's voxels in slab `tempi` (centre cx), rows from temp2, planes from tempk.
+template <class V>
+__device__ int o_sweep(const Tube& t, int tempi, float cx, int temp2, int tempk, bool preStep, int& k, int ku,
+                       V& visit) {
+    int uu = 0, uu1 = 0, uu2 = 0;
+    bool brk = false;
+    F3 l;
+    const float v0 = cx - t.s.x;
+    l.z = t.diff.y * v0;
+    const float apu1 = t.diff.z * v0;
+    const int maxZ = (int)t.Nz;
+    const float centerY0 = t.b2 + (float)temp2 * t.d2 + t.d2 / 2.f;
+    const float centerZ0 = t.bz + (float)tempk * t.dz + t.dz / 2.f;
+    if (!t.cz) {
+        const float cz = t.bz + (float)tempk * t.dz + t.dz / 2.f;
+        const float z0 = cz - t.s.z;
+        l.x = t.diff.x * z0 - apu1;
+        l.y = t.diff.y * z0;
+        o_rows(t, tempi, temp2, tempk, l, centerY0, true, uu, uu1, uu2, brk, visit);
+        return uu;
+    }
+    int zz;
+    for (zz = max(tempk, 0); zz < maxZ; zz++) {
+        const float cz = t.bz + (float)zz * t.dz + t.dz / 2.f;
+        const float z0 = cz - t.s.z;
+        l.x = t.diff.x * z0 - apu1;
+        l.y = t.diff.y * z0;
+        o_rows(t, tempi, temp2, zz, l, centerY0, true, uu, uu1, uu2, brk, visit);
+        if (uu1 == temp2 && uu2 == temp2 - 1 && brk) break;
+    }
+    k = zz - 1;
+    float cz = centerZ0 - t.dz;
+    for (zz = min(tempk, maxZ) - 1; zz >= 0; zz--) {
+        const float z0 = cz - t.s.z;
+        l.x = t.diff.x * z0 - apu1;
+        l.y = t.diff.y * z0;
+        o_rows(t, tempi, temp2, zz, l, centerY0, false, uu, uu1, uu2, brk, visit);
+        if (uu1 == temp2 && uu2 == temp2 - 1 && brk) break;
+        cz -= t.dz;
+    }
+    if (preStep) k = (ku < 0) ? max(k, zz) - 1 : min(k, zz) + 1;
+    return uu;
+}
+
+// One LOR: visit(ind, w) for every voxel of its tube. Returns the LOR weight temp (0: ray skipped).
+template <class V>
+__device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit) {
+    const SidArgs& g = oa.a;
+    Ray r;
+    fetch_ray(g, lo, r);
+    Tube t;
+    t.s = F3{r.xs, r.ys, r.zs};
+    const F3 d{r.xd, r.yd, r.zd};
+    t.diff = F3{d.x - t.s.x, d.y - t.s.y, d.z - t.s.z};
+    F3& s = t.s;
+    F3& diff = t.diff;
+    if ((diff.x == 0.f && diff.y == 0.f) || !isfinite(diff.x) || !isfinite(diff.y) || !isfinite(diff.z)) return 0.f;
+    const float L = sqrtf(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
+    t.orl = L * oa.width;
+    float w = 1.f;
+    if (g.norm) w *= g.norm[lo];
+    w *= g.global_factor;
+    t.Nxy = g.Nx * g.Ny;
+    t.Nz = g.Nz;
+    t.bz = g.bz;
+    t.dz = g.dz;
+    t.cxy = oa.cxy != 0;
+    t.cz = oa.cz != 0;
+    int tempk_b = 0;
+
+    if (fabsf(diff.z) < 1e-6f && (fabsf(diff.y) < 1e-6f || fabsf(diff.x) < 1e-6f)) {
+        const int tempk = (int)(fabsf(s.z - g.bz) / g.dz);
+        if (tempk < 0 || tempk >= (int)g.Nz) return 0.f;
+        int apuX1, apuX2, indO;
+        float b1, d1;
+        auto span = [&](float sa, float da, float ba, float dd, uint32_t N) {
+            apuX1 = 0;
+            apuX2 = (int)N - 1;
+            float dist1, dist2;
+            if (sa > da) { dist1 = ba - da; dist2 = ba + (float)N * dd - sa; }
+            else { dist1 = ba - sa; dist2 = ba + (float)N * dd - da; }
+            if (dist1 < 0.f) {
+                const int kk = (int)ceilf(-dist1 / dd);
+                if (kk < (int)N) apuX1 = kk;
+            }
+            if (dist2 > 0.f) {
+                const int mm = (int)ceilf(dist2 / dd);
+                const int kk = (int)N - 1 - mm;
+                if (kk >= apuX1) apuX2 = kk;
+            }
+        };
+        if (fabsf(diff.y) < 1e-6f && d.y <= g.bmaxy && d.y >= g.by && s.y <= g.bmaxy && s.y >= g.by) {
+            span(s.x, d.x, g.bx, g.dx, g.Nx);
+            b1 = g.bx; t.b2 = g.by; d1 = g.dx; t.d2 = g.dy;
+            t.N1 = g.Nx; t.N2p = 1u; t.N3p = g.Ny;
+            indO = o_perp_start(g.by, d.y, g.dy, g.Ny);
+        } else if (fabsf(diff.x) < 1e-6f && d.x <= g.bmaxx && d.x >= g.bx && s.x <= g.bmaxx && s.x >= g.bx) {
+            span(s.y, d.y, g.by, g.dy, g.Ny);
+            b1 = g.by; t.b2 = g.bx; d1 = g.dy; t.d2 = g.dx;
+            const float sb = s.x; s.x = s.y; s.y = sb;
+            const float db = diff.x; diff.x = diff.y; diff.y = db;
+            t.N1 = g.Ny; t.N2p = g.Nx; t.N3p = 1u;
+            indO = o_perp_start(g.bx, d.x, g.dx, g.Nx);
+        } else {
+            return 0.f;
+        }
+        for (int ii = apuX1; ii <= apuX2; ii++) {
+            const float cx = b1 + d1 * (float)ii + d1 / 2.f;
+            o_sweep(t, ii, cx, indO, tempk, false, tempk_b, 0, visit);
+        }
+        return w;
+    }
+
+    float txu = 0.f, tyu = 0.f, tzu = 0.f, tc = 0.f, tx0 = 1e8f, ty0 = 1e8f, tz0 = 1e8f;
+    bool skip = false, XY = true;
+    int tempi = 0, tempj = 0, tempk = 0, ux = 0, uy = 0, uz = 0;
+    uint32_t Np = 0u;
+    if (fabsf(diff.z) < 1e-6f) {
+        tempk = (int)(fabsf(s.z - g.bz) / g.dz);
+        if (tempk < 0 || tempk >= (int)g.Nz) return 0.f;
+        skip = o_pre2(g.bx, g.by, diff.x, diff.y, g.bmaxx, g.bmaxy, g.dx, g.dy, g.Nx, g.Ny, tempi, tempj, txu, tyu, Np,
+                      s.y, s.x, d.y, d.x, tc, ux, uy, tx0, ty0, XY);
+    } else if (fabsf(diff.y) < 1e-6f) {
+        tempj = o_perp_start(g.by, d.y, g.dy, g.Ny);
+        skip = o_pre2(g.bx, g.bz, diff.x, diff.z, g.bmaxx, g.bmaxz, g.dx, g.dz, g.Nx, g.Nz, tempi, tempk, txu, tzu, Np,
+                      s.z, s.x, d.z, d.x, tc, ux, uz, tx0, tz0, XY);
+        XY = true;
+        if (d.y > g.bmaxy || d.y < g.by) skip = true;
+    } else if (fabsf(diff.x) < 1e-6f) {
+        tempi = o_perp_start(g.bx, d.x, g.dx, g.Nx);
+        skip = o_pre2(g.by, g.bz, diff.y, diff.z, g.bmaxy, g.bmaxz, g.dy, g.dz, g.Ny, g.Nz, tempj, tempk, tyu, tzu, Np,
+                      s.z, s.y, d.z, d.y, tc, uy, uz, ty0, tz0, XY);
+        XY = false;
+        if (d.x > g.bmaxx || d.x < g.bx) skip = true;
+    } else {
+        skip = o_pre3(g, s, d, diff, tempi, tempj, tempk, txu, tyu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0, XY);
+    }
+    if (skip) return 0.f;
+
+    float b1, d1;
+    uint32_t NNx;
+    if (!XY) {
+        b1 = g.by; t.b2 = g.bx; d1 = g.dy; t.d2 = g.dx;
+        int ib = tempi; tempi = tempj; tempj = ib;
+        float fb = s.x; s.x = s.y; s.y = fb;
+        ib = ux; ux = uy; uy = ib;
+        fb = tx0; tx0 = ty0; ty0 = fb;
+        fb = txu; txu = tyu; tyu = fb;
+        fb = diff.x; diff.x = diff.y; diff.y = fb;
+        NNx = g.Ny;
+        t.N1 = g.Nx; t.N2p = g.Nx; t.N3p = 1u;
+    } else {
+        b1 = g.bx; t.b2 = g.by; d1 = g.dx; t.d2 = g.dy;
+        NNx = g.Nx;
+        t.N1 = g.Ny; t.N2p = 1u; t.N3p = g.Nx;
+    }
+    int tempi_a = tempi, tempj_a = tempj, tempk_a = tempk;
+    bool swept = false;
+    int tempiOld = 0;
+    for (uint32_t ii = 0u; ii < Np; ii++) {
+        const int li = tempi, lj = tempj, lk = tempk;
+        const float tx0_a = tx0, ty0_a = ty0, tz0_a = tz0;
+        tempi_a = tempi; tempj_a = tempj; tempk_a = tempk;
+        if (tz0 < ty0 && tz0 < tx0) { tempk += uz; tz0 += tzu; }
+        else if (ty0 < tx0) { tempj += uy; ty0 += tyu; }
+        else { tempi += ux; tx0 += txu; }
+        if (ii == 0u) {
+            tempk_b = tempk_a;
+            if (ux >= 0) {
+                float cx = b1 + d1 * (float)(tempi_a - 1) + d1 / 2.f;
+                for (int kk = tempi_a - 1; kk >= 0; kk--) {
+                    if (o_sweep(t, kk, cx, tempj_a, tempk_b, true, tempk_b, uz, visit) == 0) break;
+                    cx -= d1;
+                }
+            } else {
+                float cx = b1 + d1 * (float)(tempi_a + 1) + d1 / 2.f;
+                for (int kk = tempi_a + 1; kk < (int)NNx; kk++) {
+                    if (o_sweep(t, kk, cx, tempj_a, tempk_b, true, tempk_b, uz, visit) == 0) break;
+                    cx += d1;
+                }
+            }
+        }
+        if (tz0_a >= tx0_a && ty0_a >= tx0_a) {
+            const float cx = b1 + d1 * (float)li + d1 / 2.f;
+            o_sweep(t, li, cx, lj, lk, false, tempk_b, uz, visit);
+            tempiOld = tempi_a;
+            swept = true;
+        }
+        if (tempi < 0 || tempi >= (int)g.Nx || tempj < 0 || tempj >= (int)g.Ny || tempk < 0 || tempk >= (int)g.Nz)
+            break;
+    }
+    if (ux < 0) {
+        if (!swept || tempiOld != tempi_a) tempi_a++;
+        for (int ii = tempi_a - 1; ii >= 0; ii--) {
+            const float cx = b1 + d1 * (float)ii + d1 / 2.f;
+            if (o_sweep(t, ii, cx, tempj_a, tempk_a, false, tempk_b, uz, visit) == 0) break;
+        }
+    } else {
+        if (!swept || tempiOld != tempi_a) tempi_a--;
+        float cx = b1 + d1 * (float)(tempi_a + 1) + d1 / 2.f;
+        for (int ii = tempi_a + 1; ii < (int)NNx; ii++) {
+            if (o_sweep(t, ii, cx, tempj_a, tempk_a, false, tempk_b, uz, visit) == 0) break;
+            cx += d1;
+        }
+    }
+    return w;
+}
+
+__global__ __launch_bounds__(kOrthBlock) void k_orth_fp(OrthArgs oa, const float* __restrict__ img,
+                                                        float* __restrict__ y) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    float ax = 0.f;
+    const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
+    y[lo] = ax * temp;
+}
+
+__global__ __launch_bounds__(kOrthBlock) void k_orth_bp(OrthArgs oa, const float* __restrict__ yin,
+                                                        float* __restrict__ rhs, float* __restrict__ sens) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    const float r = yin[lo];
+    if (r == 0.f && sens == nullptr) return;
+    float temp = 1.f;
+    if (oa.a.norm) temp *= oa.a.norm[lo];
+    temp *= oa.a.global_factor;
+    if (sens) {
+        orth_lor(oa, lo, [&](int64_t ind, float w) {
+            const float e = w * temp;
+            unsafeAtomicAdd(rhs + ind, r * e);
+            unsafeAtomicAdd(sens + ind, e);
+        });
+    } else {
+        orth_lor(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); });
+    }
+}
+
+// FP, ratio r = y / (A x + eps) (NaN/Inf -> eps), BP of r; optional sensitivity.
+__global__ __launch_bounds__(kOrthBlock) void k_orth_osem(OrthArgs oa, const float* __restrict__ img,
+                                                          const float* __restrict__ meas, float* __restrict__ rhs,
+                                                          float* __restrict__ sens, float epps) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    float ax = 0.f;
+    const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
+    if (temp == 0.f) return;
+    float r = meas[lo] / (ax * temp + epps);
+    if (isnan(r) || isinf(r)) r = epps;
+    if (r == 0.f && sens == nullptr) return;
+    if (sens) {
+        orth_lor(oa, lo, [&](int64_t ind, float w) {
+            const float e = w * temp;
+            unsafeAtomicAdd(rhs + ind, r * e);
+            unsafeAtomicAdd(sens + ind, e);
+        });
+    } else {
+        orth_lor(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); });
+    }
+}
+
+// Per-LOR voxel lists: counts only (offsets == null) or indices and tube weights at offsets[lo].
+__global__ __launch_bounds__(kOrthBlock) void k_orth_trace(OrthArgs oa, int32_t* __restrict__ counts,
+                                                           const int64_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ idx, float* __restrict__ wout) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    int32_t c = 0;
+    if (offsets) {
+        int64_t o = offsets[lo];
+        orth_lor(oa, lo, [&](int64_t ind, float w) { idx[o] = (uint32_t)ind; wout[o] = w; o++; c++; });
+    } else {
+        orth_lor(oa, lo, [&](int64_t, float) { c++; });
+    }
+    counts[lo] = c;
+}
+
+static OrthArgs make_orth_args(const omg_geom* g, int64_t first, int64_t n) {
+    OrthArgs oa;
+    oa.a = make_sid_args(g, first, n);
+    // b + N*d in double, rounded once (init.py:727); the Siddon path keeps the CPU form.
+    oa.a.bmaxx = (float)((double)g->d.bx + (double)g->d.Nx * (double)g->d.dx);
+    oa.a.bmaxy = (float)((double)g->d.by + (double)g->d.Ny * (double)g->d.dy);
+    oa.a.bmaxz = (float)((double)g->d.bz + (double)g->d.Nz * (double)g->d.dz);
+    oa.width = g->d.orth_width;
+    oa.cxy = (g->d.orth_flags & 1) != 0;
+    oa.cz = (g->d.orth_flags & 2) != 0;
+    return oa;
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+int orth_prepare(omg_geom* g) {
+    const omg_geom_desc& d = g->d;
+    OMG_REQUIRE(d.mode == OMG_MODE_PET_SINO || d.mode == OMG_MODE_LIST6,
+                "projector_type 2 is provided for PET sinogram and list-mode data");
+    OMG_REQUIRE(d.Nx == d.Ny, "projector_type 2 needs Nx == Ny (the reference kernel's axis-parallel branch "
+                              "indexes rows with the other axis' extent)");
+    OMG_REQUIRE(d.orth_width > 0.f && std::isfinite(d.orth_width), "orth_width must be a positive tube width");
+    OMG_REQUIRE((d.orth_flags & ~3) == 0, "orth_flags holds bit 0 (transaxial sweep) and bit 1 (axial sweep) only");
+    return OMG_OK;
+}
+
 void orth_release(omg_geom*) {}
 
-int orth_forward(const omg_geom*, int64_t, int64_t, const float*, float*, hipStream_t) {
-    omg::set_error("projector_type 2 is not built yet");
-    return OMG_ERR_UNSUPPORTED;
+int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
+    const OrthArgs oa = make_orth_args(g, first, n);
+    hipLaunchKernelGGL(k_orth_fp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, y);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
 }
-int orth_backward(const omg_geom*, int64_t, int64_t, const float*, float*, float*, hipStream_t) {
-    omg::set_error("projector_type 2 is not built yet");
-    return OMG_ERR_UNSUPPORTED;
+
+int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
+                  hipStream_t s) {
+    const OrthArgs oa = make_orth_args(g, first, n);
+    hipLaunchKernelGGL(k_orth_bp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, yin, rhs, sens);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
+              float* sens, float epps, hipStream_t s) {
+    const OrthArgs oa = make_orth_args(g, first, n);
+    hipLaunchKernelGGL(k_orth_osem, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, meas, rhs, sens,
+                       epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+int orth_trace(const omg_geom* g, int64_t first, int64_t n, int32_t* counts, const int64_t* offsets, uint32_t* idx,
+               float* w, hipStream_t s) {
+    const OrthArgs oa = make_orth_args(g, first, n);
+    hipLaunchKernelGGL(k_orth_trace, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, counts, offsets, idx,
+                       w);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
 }

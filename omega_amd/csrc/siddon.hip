@@ -374,6 +374,10 @@ static int siddon_common_checks(const omg_geom* g, int64_t first, int64_t n) {
 // Forward declarations of the orthogonal / BDD dispatchers (other translation units).
 int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
 int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens, hipStream_t s);
+int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
+              float* sens, float epps, hipStream_t s);
+int orth_trace(const omg_geom* g, int64_t first, int64_t n, int32_t* counts, const int64_t* offsets, uint32_t* idx,
+               float* w, hipStream_t s);
 int bdd_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
 int bdd_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens, hipStream_t s);
 
@@ -424,8 +428,9 @@ extern "C" int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_me
     if (rc) return rc;
     OMG_REQUIRE(img && meas && rhs, "null pointer argument");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "the fused OSEM data step is for emission (PET) data");
-    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "the fused OSEM data step is implemented for projector_type 1");
+    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM data step is implemented for projector_type 1 and 2");
     if (n_meas == 0) return OMG_OK;
+    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_osem(g, first, n_meas, img, meas, rhs, sens, epps, as_stream(stream));
     if (planar_ok(g) && !g->force_generic) {
         if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, as_stream(stream));
         return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, as_stream(stream));
@@ -495,7 +500,7 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
     if (rc) return rc;
     OMG_REQUIRE(x && meas && S, "null pointer argument");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "OSEM subset iterations here are for emission (PET) data");
-    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "the fused OSEM iteration is implemented for projector_type 1");
+    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM iteration is implemented for projector_type 1 and 2");
     hipStream_t s = as_stream(stream);
     omg_geom* gm = const_cast<omg_geom*>(g);
     const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
@@ -510,7 +515,8 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
         if (rc) return rc;
     }
     const uint32_t per = g->d.size_x * g->d.size_y;
-    if (n_meas > 0 && planar_ok(g) && !g->force_generic && unique_slices(g, first, (uint32_t)(n_meas / per))) {
+    if (n_meas > 0 && g->d.projector_type == OMG_PROJ_SIDDON && planar_ok(g) && !g->force_generic &&
+        unique_slices(g, first, (uint32_t)(n_meas / per))) {
         const PlanarShape sh = planar_shape(g, n_meas);
         if (compute_sens) rc = launch_planar<true, true, true>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
         else rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
@@ -547,7 +553,10 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, S, N, 0.f);
         OMG_CHECK_LAUNCH();
     }
-    if (n_meas > 0) {
+    if (n_meas > 0 && g->d.projector_type == OMG_PROJ_ORTH) {
+        rc = orth_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s);
+        if (rc) return rc;
+    } else if (n_meas > 0) {
         SidArgs a = make_sid_args(g, first, n_meas);
         hipEvent_t stop;
         timing_begin(s, &stop);
@@ -566,10 +575,11 @@ extern "C" int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas
                                 const int64_t* offsets, uint32_t* idx, float* len, void* stream) {
     int rc = siddon_common_checks(g, first, n_meas);
     if (rc) return rc;
-    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "trace lists are produced by projector_type 1");
+    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "trace lists are produced by projector_type 1 and 2");
     OMG_REQUIRE(counts != nullptr, "null counts");
     OMG_REQUIRE(offsets == nullptr || (idx && len), "offsets given without idx/len outputs");
     if (n_meas == 0) return OMG_OK;
+    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_trace(g, first, n_meas, counts, offsets, idx, len, as_stream(stream));
     SidArgs a = make_sid_args(g, first, n_meas);
     hipLaunchKernelGGL(k_siddon_trace, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, as_stream(stream), a, counts,
                        offsets, idx, len);

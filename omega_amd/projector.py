@@ -261,9 +261,13 @@ class projectorClass:
             keep.append(nrm)
             d.norm, d.norm_len = nrm.ctypes.data, nrm.size
         if pt == 2:
-            width = self.tube_width_z if self.tube_width_z > 0 else self.tube_width_xy
-            d.orth_width = float(width)
-            d.orth_3d = int(self.tube_width_z > 0)
+            # the kernel's tube width is tube_width_z (init.py:652); the sweeps follow orthTransaxial /
+            # orthAxial (proj.py:1041-1054 -> -DCRYSTXY / -DCRYSTZ, init.py:286-290)
+            if not self.tube_width_z > 0:
+                raise ValueError("The orthogonal projector needs tube_width_z > 0: it is the kernel's tube width "
+                                 "(with tube_width_z = 0 every voxel is outside the tube)")
+            d.orth_width = float(self.tube_width_z)
+            d.orth_flags = int(self.tube_width_xy > 0) | (int(self.tube_width_z > 0) << 1)
         d.dPitchX = float(self.dPitchX)
         d.dPitchY = float(self.dPitchY)
         return d, keep

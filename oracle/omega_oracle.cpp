@@ -824,3 +824,502 @@ extern "C" int oracle_bdd_geom5_f32(const uint32_t* n3, const float* d3, const f
     bdd::geom5(g, xyz, uv, out);
     return 0;
 }
+
+// ---------------------------------------------------------------------------------------------------------------
+// Orthogonal distance-based projector (projector type 2), restated from the reference's GPU kernel
+// (projectorType123.cl compiled with -DORTH [-DCRYSTXY] [-DCRYSTZ], no TOF/attenuation/masks/multi-ray),
+// because the CPU implementation of this projector is one-sided (orthDistance3D passes uu1 instead of uu2 in
+// its second row loop, projector_functions.h:1121-1123,1145-1147) and the reference's Python front end refuses it.
+//
+//   driver, axis-parallel rays     projectorType123.cl:513-791 (apuX1/apuX2 :527-590, indO :617-621, sweep :687-720)
+//   driver, general rays           projectorType123.cl:793-1161 (pre-loops :803-836, frame swap :900-946,
+//                                  first-step slab pre-sweep :1055-1128, per-x-step sweep :1131-1160),
+//                                  post-sweep :1235-1313, FP output :1317-1336
+//   Siddon set-up (GPU flavour)    siddon_pre_loop_2D/3D, d_g_s_precomp, s_g_d_precomp, perpendicular_start
+//                                  general_opencl_functions.h:1657-1664,1742-1800,1825-1957
+//   tube sweep                     orthDistance3D opencl_functions_orth3D.h:196-416
+//   voxel weight                   orthogonalHelper3D opencl_functions_orth3D.h:85-193 (w = 1 - d/(L*width),
+//                                  kept while (d/L)^2 < (0.99*width)^2), compute_ind_orth_3D :78-81
+//   LOR weight                     temp = norm * global_factor (PET), 1 (CT): projectorType123.cl:949-990, :1217-1233
+//
+// Interpretations (where the GPU source leaves a value open):
+//   * tempiOld (uninitialised in the kernel when a ray takes no x-step) starts as "no slab swept yet", so the
+//     post-sweep then starts at the ray's own slab;
+//   * the x/y-dominance test of 3-D rays, |acos(cx)| < 0.785398 || > 2.35619 with cx the x direction cosine,
+//     is evaluated as the equivalent comparison of cx with cos(0.785398) and cos(2.35619), so host and
+//     device decide identically.
+// FMAD(a,b,c) of the kernel is fmaf here; every other expression is evaluated as written, without contraction.
+namespace orth {
+
+struct OGeo {
+    oracle::Geo<float> g;
+    float bmaxx, bmaxy, bmaxz;   // b + N*d computed in double, rounded once (init.py:727)
+    float width;                 // kernel tube width (tube_width_z)
+    bool cxy, cz;                // -DCRYSTXY (transaxial row sweep), -DCRYSTZ (axial sweep)
+};
+
+struct F3 { float x, y, z; };
+
+constexpr float kThr = 0.01f;
+constexpr double kCosLo = 0.7071068921727742;   // cos(0.785398f), in double
+constexpr double kCosHi = -0.7071035822349067;  // cos(2.35619f), in double
+
+static inline uint32_t f2u_ru(float v) {  // __float2uint_ru: ceil, saturating
+    if (!(v > 0.f)) return 0u;
+    const double c = std::ceil((double)v);
+    return c >= 4294967295.0 ? 4294967295u : (uint32_t)c;
+}
+static inline uint32_t f2u_rd(float v) {  // __float2uint_rd: floor, saturating
+    if (!(v > 0.f)) return 0u;
+    const double c = std::floor((double)v);
+    return c >= 4294967295.0 ? 4294967295u : (uint32_t)c;
+}
+
+static inline void dgs(float tmin, float t_min, float tmax, float t_max, uint32_t& vmin, uint32_t& vmax, float& t0,
+                       int& vu, float diff, float b, float d, float s, uint32_t N) {
+    if (tmin == t_min) vmin = 1u;
+    else { const float pt = s + tmin * diff; vmin = f2u_ru((pt - b) / d); }
+    if (tmax == t_max) vmax = N;
+    else { const float pt = s + tmax * diff; vmax = f2u_rd((pt - b) / d); }
+    t0 += ((float)vmin * d) / diff;
+    vu = 1;
+}
+static inline void sgd(float tmin, float t_min, float tmax, float t_max, uint32_t& vmin, uint32_t& vmax, float& t0,
+                       int& vu, float diff, float b, float d, float s, uint32_t N) {
+    if (tmin == t_min) vmax = N - 1u;
+    else { const float pt = s + tmin * diff; vmax = f2u_rd((pt - b) / d); }
+    if (tmax == t_max) vmin = 0u;
+    else { const float pt = s + tmax * diff; vmin = f2u_ru((pt - b) / d); }
+    t0 += ((float)vmax * d) / diff;
+    vu = -1;
+}
+
+static inline int perp_start_gpu(float b, float d, float dd, uint32_t N) {
+    int t = (int)std::floor((d - b) / dd);
+    if (t < 0 || t >= (int)N) t = 0;
+    return t;
+}
+
+static bool pre2(float b1, float b2, float diff1, float diff2, float max1, float max2, float d1, float d2, uint32_t N1,
+                 uint32_t N2, int& temp1, int& temp2, float& t1u, float& t2u, uint32_t& Np, float ys, float xs,
+                 float yd, float xd, float& tc, int& u1, int& u2, float& t10, float& t20, bool& xy) {
+    const float apu_tx = b1 - xs, apu_ty = b2 - ys;
+    t10 = apu_tx / diff1;
+    t20 = apu_ty / diff2;
+    const float txback = (max1 - xs) / diff1, tyback = (max2 - ys) / diff2;
+    const float txmin = std::fmin(t10, txback), txmax = std::fmax(t10, txback);
+    const float tymin = std::fmin(t20, tyback), tymax = std::fmax(t20, tyback);
+    tc = std::fmax(txmin, tymin);
+    const float tmax = std::fmin(txmax, tymax);
+    xy = (tc == t10 || tc == txback);
+    if (tc >= tmax) return true;
+    uint32_t imin, imax, jmin, jmax;
+    if (xs < xd) dgs(tc, txmin, tmax, txmax, imin, imax, t10, u1, diff1, b1, d1, xs, N1);
+    else sgd(tc, txmin, tmax, txmax, imin, imax, t10, u1, diff1, b1, d1, xs, N1);
+    if (ys < yd) dgs(tc, tymin, tmax, tymax, jmin, jmax, t20, u2, diff2, b2, d2, ys, N2);
+    else sgd(tc, tymin, tmax, tymax, jmin, jmax, t20, u2, diff2, b2, d2, ys, N2);
+    Np = imax + 1u + jmax + 1u - imin - jmin;
+    const float pt = (std::fmin(t10, t20) + tc) / 2.f;
+    temp1 = (int)((pt * diff1 - apu_tx) / d1);   // truncation toward zero
+    temp2 = (int)((pt * diff2 - apu_ty) / d2);
+    t1u = d1 / std::fabs(diff1);
+    t2u = d2 / std::fabs(diff2);
+    if (temp1 < 0 || temp2 < 0 || temp1 >= (int)N1 || temp2 >= (int)N2) return true;
+    return false;
+}
+
+static bool pre3(const OGeo& o, F3 s, F3 d, F3 diff, int& ti, int& tj, int& tk, float& txu, float& tyu, float& tzu,
+                 uint32_t& Np, float& tc, int& ux, int& uy, int& uz, float& tx0, float& ty0, float& tz0, bool& xy) {
+    const oracle::Geo<float>& g = o.g;
+    const F3 apuT{g.bx - s.x, g.by - s.y, g.bz - s.z};
+    const F3 t0{apuT.x / diff.x, apuT.y / diff.y, apuT.z / diff.z};
+    const F3 tB{(o.bmaxx - s.x) / diff.x, (o.bmaxy - s.y) / diff.y, (o.bmaxz - s.z) / diff.z};
+    const F3 tMin{std::fmin(t0.x, tB.x), std::fmin(t0.y, tB.y), std::fmin(t0.z, tB.z)};
+    const F3 tMax{std::fmax(t0.x, tB.x), std::fmax(t0.y, tB.y), std::fmax(t0.z, tB.z)};
+    tc = std::fmax(std::fmax(tMin.x, tMin.z), tMin.y);
+    const float tmax = std::fmin(std::fmin(tMax.x, tMax.z), tMax.y);
+    tx0 = t0.x; ty0 = t0.y; tz0 = t0.z;
+    {
+        const float p = d.x - s.x, pY = d.y - s.y;
+        const float cx = p / std::sqrt(p * p + pY * pY);
+        xy = ((double)cx > kCosLo && cx < 1.f) || ((double)cx < kCosHi);
+    }
+    if (tc >= tmax) return true;
+    uint32_t imin, imax, jmin, jmax, kmin, kmax;
+    if (s.x < d.x) dgs(tc, tMin.x, tmax, tMax.x, imin, imax, tx0, ux, diff.x, g.bx, g.dx, s.x, g.Nx);
+    else sgd(tc, tMin.x, tmax, tMax.x, imin, imax, tx0, ux, diff.x, g.bx, g.dx, s.x, g.Nx);
+    if (s.y < d.y) dgs(tc, tMin.y, tmax, tMax.y, jmin, jmax, ty0, uy, diff.y, g.by, g.dy, s.y, g.Ny);
+    else sgd(tc, tMin.y, tmax, tMax.y, jmin, jmax, ty0, uy, diff.y, g.by, g.dy, s.y, g.Ny);
+    if (s.z < d.z) dgs(tc, tMin.z, tmax, tMax.z, kmin, kmax, tz0, uz, diff.z, g.bz, g.dz, s.z, g.Nz);
+    else sgd(tc, tMin.z, tmax, tMax.z, kmin, kmax, tz0, uz, diff.z, g.bz, g.dz, s.z, g.Nz);
+    Np = (kmax - kmin + 1u) + (jmax - jmin + 1u) + (imax - imin + 1u);
+    const float pt = (std::fmin(std::fmin(tz0, ty0), tx0) + tc) / 2.f;
+    auto cl = [](float v, float hi) { return std::fmin(std::fmax(v, 0.f), hi); };
+    ti = (int)cl(std::fma(pt, diff.x, -apuT.x) / g.dx, (float)(g.Nx - 1));
+    tj = (int)cl(std::fma(pt, diff.y, -apuT.y) / g.dy, (float)(g.Ny - 1));
+    tk = (int)cl(std::fma(pt, diff.z, -apuT.z) / g.dz, (float)(g.Nz - 1));
+    txu = g.dx / std::fabs(diff.x);
+    tyu = g.dy / std::fabs(diff.y);
+    tzu = g.dz / std::fabs(diff.z);
+    return false;
+}
+
+// One voxel of the tube: returns true when the voxel is outside the cut-off (the sweep stops there).
+template <class V>
+static inline bool helper(int tempi, int uu, uint32_t N2p, uint32_t N3p, uint32_t Nxy, int zz, F3 s, F3 l, F3 diff,
+                          float orl, F3 center, V& visit) {
+    const float x0 = std::fma(-1.f, s.y, center.y);
+    const float y1 = std::fma(diff.z, x0, -l.y);
+    const float z1 = std::fma(-diff.x, x0, l.z);
+    const float norm2 = std::fma(l.x, l.x, std::fma(y1, y1, z1 * z1));
+    if (norm2 >= (1.f - kThr) * (1.f - kThr) * orl * orl) return true;
+    const float w = 1.f - std::sqrt(norm2) / orl;
+    const int64_t ind = (int64_t)((uint32_t)tempi * N2p + (uint32_t)uu * N3p) + (int64_t)zz * (int64_t)Nxy;
+    visit(ind, w);
+    return false;
+}
+
+// orthDistance3D: rows of slab `tempi` around row temp2, z rows around tempk (when the axial sweep is on).
+template <class V>
+static int sweep(const OGeo& o, int tempi, F3 diff, F3 center, F3 s, float b2, float d2, float bz, float dz,
+                 int temp2, int tempk, uint32_t Nxy, float orl, uint32_t N1, uint32_t N2p, uint32_t N3p,
+                 bool preStep, int& k, int ku, V& visit) {
+    int uu = 0;
+    bool brk = false;
+    F3 l;
+    const float v0 = center.x - s.x;
+    l.z = diff.y * v0;
+    const float apu1 = diff.z * v0;
+    const int maxZ = (int)o.g.Nz, maxXY = (int)N1;
+    int uu1 = 0, uu2 = 0;
+    const float centerY0 = b2 + (float)temp2 * d2 + d2 / 2.f;
+    const float centerZ0 = bz + (float)tempk * dz + dz / 2.f;
+    auto rows_first = [&](int zz) {   // rows of the first (upward) z loop: upward rows recompute their centre
+        if (o.cxy) {
+            center.y = centerY0;
+            for (uu1 = std::max(temp2, 0); uu1 < maxXY; uu1++) {
+                center.y = b2 + (float)uu1 * d2 + d2 / 2.f;
+                brk = helper(tempi, uu1, N2p, N3p, Nxy, zz, s, l, diff, orl, center, visit);
+                if (brk) break;
+                uu++;
+                center.y += d2;
+            }
+            center.y = centerY0 - d2;
+            for (uu2 = std::min(temp2, maxXY) - 1; uu2 >= 0; uu2--) {
+                brk = helper(tempi, uu2, N2p, N3p, Nxy, zz, s, l, diff, orl, center, visit);
+                if (brk) break;
+                uu++;
+                center.y -= d2;
+            }
+        } else {
+            uu1 = temp2;
+            center.y = b2 + (float)uu1 * d2 + d2 / 2.f;
+            brk = helper(tempi, uu1, N2p, N3p, Nxy, zz, s, l, diff, orl, center, visit);
+            uu2 = temp2 - 1;
+        }
+    };
+    auto rows_second = [&](int zz) {  // rows of the second (downward) z loop: centres accumulate
+        if (o.cxy) {
+            center.y = centerY0;
+            for (uu1 = std::max(temp2, 0); uu1 < maxXY; uu1++) {
+                brk = helper(tempi, uu1, N2p, N3p, Nxy, zz, s, l, diff, orl, center, visit);
+                if (brk) break;
+                uu++;
+                center.y += d2;
+            }
+            center.y = centerY0 - d2;
+            for (uu2 = std::min(temp2, maxXY) - 1; uu2 >= 0; uu2--) {
+                brk = helper(tempi, uu2, N2p, N3p, Nxy, zz, s, l, diff, orl, center, visit);
+                if (brk) break;
+                uu++;
+                center.y -= d2;
+            }
+        } else {
+            uu1 = temp2;
+            center.y = centerY0;
+            brk = helper(tempi, uu1, N2p, N3p, Nxy, zz, s, l, diff, orl, center, visit);
+            uu2 = temp2 - 1;
+        }
+    };
+    if (!o.cz) {
+        const int zz = tempk;
+        center.z = bz + (float)zz * dz + dz / 2.f;
+        const float z0 = center.z - s.z;
+        l.x = diff.x * z0 - apu1;
+        l.y = diff.y * z0;
+        rows_first(zz);
+        return uu;
+    }
+    int zz;
+    for (zz = std::max(tempk, 0); zz < maxZ; zz++) {
+        center.z = bz + (float)zz * dz + dz / 2.f;
+        const float z0 = center.z - s.z;
+        l.x = diff.x * z0 - apu1;
+        l.y = diff.y * z0;
+        rows_first(zz);
+        if (uu1 == temp2 && uu2 == temp2 - 1 && brk) break;
+    }
+    k = zz - 1;
+    center.z = centerZ0 - dz;
+    for (zz = std::min(tempk, maxZ) - 1; zz >= 0; zz--) {
+        const float z0 = center.z - s.z;
+        l.x = diff.x * z0 - apu1;
+        l.y = diff.y * z0;
+        rows_second(zz);
+        if (uu1 == temp2 && uu2 == temp2 - 1 && brk) break;
+        center.z -= dz;
+    }
+    if (preStep) {
+        if (ku < 0) k = std::max(k, zz) - 1;
+        else k = std::min(k, zz) + 1;
+    }
+    return uu;
+}
+
+// One LOR: visit(ind, w) for every voxel of its tube; `temp` receives the LOR weight (0 for a skipped ray).
+template <class V>
+static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt, float& temp, V&& visit) {
+    const oracle::Geo<float>& g = o.g;
+    oracle::Ray<float> r;
+    oracle::fetch_ray(g, lo, xt, zt, r);
+    F3 s{r.xs, r.ys, r.zs};
+    const F3 d{r.xd, r.yd, r.zd};
+    F3 diff{d.x - s.x, d.y - s.y, d.z - s.z};
+    temp = 0.f;
+    if ((diff.x == 0.f && diff.y == 0.f) || !std::isfinite(diff.x) || !std::isfinite(diff.y) || !std::isfinite(diff.z))
+        return;
+    const float L = std::sqrt(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
+    const float orl = L * o.width;
+    float w = 1.f;
+    if (g.mode != oracle::CT_FLAT) {
+        if (g.norm) w *= g.norm[lo];
+        w *= g.global_factor;
+    }
+    const uint32_t Nxy = g.Nx * g.Ny;
+    int tempk_b = 0;
+    F3 center{0.f, 0.f, 0.f};
+
+    if (std::fabs(diff.z) < 1e-6f && (std::fabs(diff.y) < 1e-6f || std::fabs(diff.x) < 1e-6f)) {
+        const int tempk = (int)(std::fabs(s.z - g.bz) / g.dz);
+        if (tempk < 0 || tempk >= (int)g.Nz) return;
+        int apuX1, apuX2, indO;
+        float b1, b2, d1, d2;
+        uint32_t N1, N2p, N3p;
+        auto span = [&](float sa, float da, float ba, float dd, uint32_t N) {
+            apuX1 = 0;
+            apuX2 = (int)N - 1;
+            float dist1, dist2;
+            if (sa > da) { dist1 = ba - da; dist2 = ba + (float)N * dd - sa; }
+            else { dist1 = ba - sa; dist2 = ba + (float)N * dd - da; }
+            if (dist1 < 0.f) {
+                const int kk = (int)std::ceil(-dist1 / dd);
+                if (kk < (int)N) apuX1 = kk;
+            }
+            if (dist2 > 0.f) {
+                const int mm = (int)std::ceil(dist2 / dd);
+                const int kk = (int)N - 1 - mm;
+                if (kk >= apuX1) apuX2 = kk;
+            }
+        };
+        if (std::fabs(diff.y) < 1e-6f && d.y <= o.bmaxy && d.y >= g.by && s.y <= o.bmaxy && s.y >= g.by) {
+            span(s.x, d.x, g.bx, g.dx, g.Nx);
+            b1 = g.bx; b2 = g.by; d1 = g.dx; d2 = g.dy;
+            N1 = g.Nx; N2p = 1u; N3p = g.Ny;           // d_N1, d_N3, d_N2 of :694 (row extent/stride as written)
+            indO = perp_start_gpu(g.by, d.y, g.dy, g.Ny);
+        } else if (std::fabs(diff.x) < 1e-6f && d.x <= o.bmaxx && d.x >= g.bx && s.x <= o.bmaxx && s.x >= g.bx) {
+            span(s.y, d.y, g.by, g.dy, g.Ny);
+            b1 = g.by; b2 = g.bx; d1 = g.dy; d2 = g.dx;
+            std::swap(s.x, s.y);
+            std::swap(diff.x, diff.y);
+            N1 = g.Ny; N2p = g.Nx; N3p = 1u;
+            indO = perp_start_gpu(g.bx, d.x, g.dx, g.Nx);
+        } else {
+            return;
+        }
+        temp = w;
+        for (int ii = apuX1; ii <= apuX2; ii++) {
+            center.x = b1 + d1 * (float)ii + d1 / 2.f;
+            sweep(o, ii, diff, center, s, b2, d2, g.bz, g.dz, indO, tempk, Nxy, orl, N1, N2p, N3p, false, tempk_b, 0,
+                  visit);
+        }
+        return;
+    }
+
+    float txu = 0.f, tyu = 0.f, tzu = 0.f, tc = 0.f, tx0 = 1e8f, ty0 = 1e8f, tz0 = 1e8f;
+    bool skip = false, XY = true;
+    int tempi = 0, tempj = 0, tempk = 0, ux = 0, uy = 0, uz = 0;
+    uint32_t Np = 0u;
+    if (std::fabs(diff.z) < 1e-6f) {
+        tempk = (int)(std::fabs(s.z - g.bz) / g.dz);
+        if (tempk < 0 || tempk >= (int)g.Nz) return;
+        skip = pre2(g.bx, g.by, diff.x, diff.y, o.bmaxx, o.bmaxy, g.dx, g.dy, g.Nx, g.Ny, tempi, tempj, txu, tyu, Np,
+                    s.y, s.x, d.y, d.x, tc, ux, uy, tx0, ty0, XY);
+    } else if (std::fabs(diff.y) < 1e-6f) {
+        tempj = perp_start_gpu(g.by, d.y, g.dy, g.Ny);
+        skip = pre2(g.bx, g.bz, diff.x, diff.z, o.bmaxx, o.bmaxz, g.dx, g.dz, g.Nx, g.Nz, tempi, tempk, txu, tzu, Np,
+                    s.z, s.x, d.z, d.x, tc, ux, uz, tx0, tz0, XY);
+        XY = true;
+        if (d.y > o.bmaxy || d.y < g.by) skip = true;
+    } else if (std::fabs(diff.x) < 1e-6f) {
+        tempi = perp_start_gpu(g.bx, d.x, g.dx, g.Nx);
+        skip = pre2(g.by, g.bz, diff.y, diff.z, o.bmaxy, o.bmaxz, g.dy, g.dz, g.Ny, g.Nz, tempj, tempk, tyu, tzu, Np,
+                    s.z, s.y, d.z, d.y, tc, uy, uz, ty0, tz0, XY);
+        XY = false;
+        if (d.x > o.bmaxx || d.x < g.bx) skip = true;
+    } else {
+        skip = pre3(o, s, d, diff, tempi, tempj, tempk, txu, tyu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0, XY);
+    }
+    if (skip) return;
+
+    float b1, b2, d1, d2;
+    uint32_t N1, N2p, N3p, NNx;
+    if (!XY) {
+        b1 = g.by; b2 = g.bx; d1 = g.dy; d2 = g.dx;
+        std::swap(tempi, tempj);
+        std::swap(s.x, s.y);
+        std::swap(ux, uy);
+        std::swap(tx0, ty0);
+        std::swap(txu, tyu);
+        std::swap(diff.x, diff.y);
+        NNx = g.Ny;
+        N1 = g.Nx; N2p = g.Nx; N3p = 1u;
+    } else {
+        b1 = g.bx; b2 = g.by; d1 = g.dx; d2 = g.dy;
+        NNx = g.Nx;
+        N1 = g.Ny; N2p = 1u; N3p = g.Nx;
+    }
+    temp = w;
+    float tx0_a = tx0, ty0_a = ty0, tz0_a = tz0;
+    int tempi_a = tempi, tempj_a = tempj, tempk_a = tempk;
+    bool swept = false;     // tempiOld holds a slab index
+    int tempiOld = 0;
+    for (uint32_t ii = 0u; ii < Np; ii++) {
+        const int li = tempi, lj = tempj, lk = tempk;
+        tx0_a = tx0; ty0_a = ty0; tz0_a = tz0;
+        tempi_a = tempi; tempj_a = tempj; tempk_a = tempk;
+        if (tz0 < ty0 && tz0 < tx0) { tempk += uz; tz0 += tzu; }
+        else if (ty0 < tx0) { tempj += uy; ty0 += tyu; }
+        else { tempi += ux; tx0 += txu; }
+        if (ii == 0u) {
+            tempk_b = tempk_a;
+            if (ux >= 0) {
+                center.x = b1 + d1 * (float)(tempi_a - 1) + d1 / 2.f;
+                for (int kk = tempi_a - 1; kk >= 0; kk--) {
+                    const int uu = sweep(o, kk, diff, center, s, b2, d2, g.bz, g.dz, tempj_a, tempk_b, Nxy, orl, N1,
+                                         N2p, N3p, true, tempk_b, uz, visit);
+                    if (uu == 0) break;
+                    center.x -= d1;
+                }
+            } else {
+                center.x = b1 + d1 * (float)(tempi_a + 1) + d1 / 2.f;
+                for (int kk = tempi_a + 1; kk < (int)NNx; kk++) {
+                    const int uu = sweep(o, kk, diff, center, s, b2, d2, g.bz, g.dz, tempj_a, tempk_b, Nxy, orl, N1,
+                                         N2p, N3p, true, tempk_b, uz, visit);
+                    if (uu == 0) break;
+                    center.x += d1;
+                }
+            }
+        }
+        if (tz0_a >= tx0_a && ty0_a >= tx0_a) {
+            center.x = b1 + d1 * (float)li + d1 / 2.f;
+            sweep(o, li, diff, center, s, b2, d2, g.bz, g.dz, lj, lk, Nxy, orl, N1, N2p, N3p, false, tempk_b, uz, visit);
+            tempiOld = tempi_a;
+            swept = true;
+        }
+        if (tempi < 0 || tempi >= (int)g.Nx || tempj < 0 || tempj >= (int)g.Ny || tempk < 0 ||
+            tempk >= (int)g.Nz)
+            break;
+    }
+    if (ux < 0) {
+        if (!swept || tempiOld != tempi_a) tempi_a++;
+        for (int ii = tempi_a - 1; ii >= 0; ii--) {
+            center.x = b1 + d1 * (float)ii + d1 / 2.f;
+            const int uu = sweep(o, ii, diff, center, s, b2, d2, g.bz, g.dz, tempj_a, tempk_a, Nxy, orl, N1, N2p, N3p,
+                                 false, tempk_b, uz, visit);
+            if (uu == 0) break;
+        }
+    } else {
+        if (!swept || tempiOld != tempi_a) tempi_a--;
+        center.x = b1 + d1 * (float)(tempi_a + 1) + d1 / 2.f;
+        for (int ii = tempi_a + 1; ii < (int)NNx; ii++) {
+            const int uu = sweep(o, ii, diff, center, s, b2, d2, g.bz, g.dz, tempj_a, tempk_a, Nxy, orl, N1, N2p, N3p,
+                                 false, tempk_b, uz, visit);
+            if (uu == 0) break;
+            center.x += d1;
+        }
+    }
+}
+
+}  // namespace orth
+
+static orth::OGeo make_ogeo(const uint32_t* n3, const float* d3, const float* b3, uint32_t size_x, uint32_t size_y,
+                            int32_t mode, int64_t plane_offset, float global_factor, const float* norm, float width,
+                            int32_t flags) {
+    orth::OGeo o;
+    o.g = make_geo<float>(n3, d3, b3, size_x, size_y, mode, plane_offset, 0.f, 0, global_factor, norm);
+    o.bmaxx = (float)((double)b3[0] + (double)n3[0] * (double)d3[0]);
+    o.bmaxy = (float)((double)b3[1] + (double)n3[1] * (double)d3[1]);
+    o.bmaxz = (float)((double)b3[2] + (double)n3[2] * (double)d3[2]);
+    o.width = width;
+    o.cxy = (flags & 1) != 0;
+    o.cz = (flags & 2) != 0;
+    return o;
+}
+
+#define ORACLE_ORTH_ARGS                                                                                              \
+    const uint32_t *n3, const float *d3, const float *b3, uint32_t size_x, uint32_t size_y, int32_t mode,             \
+        int64_t plane_offset, float global_factor, const float *norm, float width, int32_t flags, int64_t nMeas,     \
+        const float *x, const float *z
+
+extern "C" int oracle_orth_fp_f32(ORACLE_ORTH_ARGS, const float* img, float* y, int nthreads) {
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, width, flags);
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        float ax = 0.f, temp = 0.f;
+        orth::orth_lor(o, lo, x, z, temp, [&](int64_t ind, float w) { ax += w * img[ind]; });
+        y[lo] = ax * temp;
+    }
+    return 0;
+}
+
+extern "C" int oracle_orth_bp_f32(ORACLE_ORTH_ARGS, const float* yin, float* rhs, float* sens, int nthreads) {
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, width, flags);
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        const float r = yin[lo];
+        std::vector<std::pair<int64_t, float>> v;
+        float temp = 0.f;
+        orth::orth_lor(o, lo, x, z, temp, [&](int64_t ind, float w) { v.emplace_back(ind, w); });
+        for (auto& p : v) {
+            const float e = p.second * temp;
+#pragma omp atomic
+            rhs[p.first] += r * e;
+            if (sens) {
+#pragma omp atomic
+                sens[p.first] += e;
+            }
+        }
+    }
+    return 0;
+}
+
+// Per-LOR voxel lists in visiting order (indices, tube weights w) and the LOR weight; returns the total count.
+extern "C" int64_t oracle_orth_trace_f32(ORACLE_ORTH_ARGS, int64_t* counts, int64_t cap, int64_t* idx_out,
+                                         float* w_out, float* lor_w) {
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, width, flags);
+    int64_t total = 0;
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        int64_t c = 0;
+        float temp = 0.f;
+        orth::orth_lor(o, lo, x, z, temp, [&](int64_t ind, float w) {
+            if (idx_out && total < cap) { idx_out[total] = ind; w_out[total] = w; }
+            total++;
+            c++;
+        });
+        if (counts) counts[lo] = c;
+        if (lor_w) lor_w[lo] = temp;
+    }
+    return total;
+}

@@ -62,6 +62,12 @@ def lib():
         _lib.oracle_bdd_fp_f32.argtypes = bgeo + [_f32p, _f32p, C.c_int]
         _lib.oracle_bdd_bp_f32.argtypes = bgeo + [_f32p, _f32p, C.c_void_p, C.c_int]
         _lib.oracle_bdd_geom5_f32.argtypes = bgeo + [_f32p]
+        ogeo = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, C.c_float, C.c_void_p,
+                C.c_float, C.c_int32, C.c_int64, _f32p, _f32p]
+        _lib.oracle_orth_fp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_int]
+        _lib.oracle_orth_bp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_void_p, C.c_int]
+        _lib.oracle_orth_trace_f32.argtypes = ogeo + [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.oracle_orth_trace_f32.restype = C.c_int64
         _lib.oracle_ratio_f32.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
         _lib.oracle_sens_clamp_f32.argtypes = [_f32p, C.c_int64, C.c_float]
         _lib.oracle_osem_update_f32.argtypes = [_f32p, _f32p, _f32p, C.c_int64, C.c_float, C.c_int]
@@ -284,6 +290,54 @@ def pdhg(cases, meas_parts, n_iter, tau, sigma=1.0, theta=1.0, epps=1e-5):
             g = g1 + (theta * S) * dg
             f = np.maximum(epps, f - tau * g)
     return f
+
+
+class OrthCase:
+    """Orthogonal distance-based projector (GPU-kernel semantics), float32.
+
+    ``flags``: bit 0 transaxial row sweep (tube_width_xy > 0), bit 1 axial sweep (tube_width_z > 0);
+    ``width`` is the kernel's tube width (tube_width_z)."""
+
+    def __init__(self, grid, size_x, size_y, x, z, width, flags=3, mode=MODE_PET, plane_offset=0,
+                 global_factor=1.0, norm=None):
+        self.n3 = np.ascontiguousarray(grid.n3())
+        self.d3 = grid.d3(np.float32)
+        self.b3 = grid.b3(np.float32)
+        self.grid = grid
+        self.size_x, self.size_y = int(size_x), int(size_y)
+        self.x = np.ascontiguousarray(x, dtype=np.float32)
+        self.z = np.ascontiguousarray(z, dtype=np.float32)
+        self.width, self.flags = float(width), int(flags)
+        self.mode, self.plane_offset, self.global_factor = mode, int(plane_offset), global_factor
+        self.norm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+
+    def _geo(self, nmeas):
+        normp = None if self.norm is None else self.norm.ctypes.data
+        return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
+                self.global_factor, normp, self.width, self.flags, nmeas, self.x, self.z]
+
+    def fp(self, img, nmeas, nthreads=0):
+        y = np.zeros(nmeas, dtype=np.float32)
+        lib().oracle_orth_fp_f32(*self._geo(nmeas), np.ascontiguousarray(img, dtype=np.float32), y, nthreads)
+        return y
+
+    def bp(self, yin, nmeas, sens=False, nthreads=0):
+        rhs = np.zeros(self.grid.N, dtype=np.float32)
+        s = np.zeros(self.grid.N, dtype=np.float32) if sens else None
+        lib().oracle_orth_bp_f32(*self._geo(nmeas), np.ascontiguousarray(yin, dtype=np.float32), rhs,
+                                 None if s is None else s.ctypes.data, nthreads)
+        return (rhs, s) if sens else rhs
+
+    def trace(self, nmeas):
+        """Per-LOR voxel lists in visiting order: (counts, indices, tube weights, LOR weights)."""
+        f = lib().oracle_orth_trace_f32
+        counts = np.zeros(nmeas, dtype=np.int64)
+        lw = np.zeros(nmeas, dtype=np.float32)
+        total = f(*self._geo(nmeas), counts.ctypes.data, 0, None, None, lw.ctypes.data)
+        idx = np.zeros(max(total, 1), dtype=np.int64)
+        w = np.zeros(max(total, 1), dtype=np.float32)
+        f(*self._geo(nmeas), None, total, idx.ctypes.data, w.ctypes.data, None)
+        return counts, idx[:total], w[:total], lw
 
 
 class BDDCase:

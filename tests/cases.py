@@ -71,3 +71,35 @@ def ct_small(n_views=24, rows=40, cols=24):
     grid = G.Grid.from_fov(32, 32, 24, 32.0, 32.0, 24.0)
     x, uv = G.cbct_tables(n_views, sod=120.0, sdd=200.0, pitch_u=1.6)
     return grid, x, uv, rows, cols, 1.6
+
+
+def square_grid():
+    """8x8x5 grid with unequal voxel sizes (the orthogonal projector needs Nx == Ny)."""
+    return G.Grid(8, 8, 5, 1.5, 2.0, 2.5, -6.0, -8.0, -6.25)
+
+
+def orth_pet2d():
+    """Small 2-D plane PET case for the orthogonal projector: 32x32x9 image of 2.5 mm voxels,
+    32 radial bins x 40 angles at 2.5 mm radial pitch on a 100 mm ring, one plane per slice."""
+    grid = G.Grid.from_fov(32, 32, 9, 80.0, 80.0, 22.5)
+    sc = G.pet_2d_planes(grid, radius=100.0, n_radial=32, n_angles=40, radial_pitch=2.5)
+    return grid, sc
+
+
+def tube_brute_force(grid, s, d, width, plane=None):
+    """Every voxel whose centre lies closer than 0.99*width to the infinite line through s and d,
+    with weight 1 - dist/width, in float64 (optionally only slice `plane`). Returns {index: (w, dist/width)}."""
+    Nx, Ny, Nz = grid.Nx, grid.Ny, grid.Nz
+    xc = grid.bx + (np.arange(Nx) + 0.5) * grid.dx
+    yc = grid.by + (np.arange(Ny) + 0.5) * grid.dy
+    zc = grid.bz + (np.arange(Nz) + 0.5) * grid.dz
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    P = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    s = np.asarray(s, np.float64)
+    v = np.asarray(d, np.float64) - s
+    dist = np.linalg.norm(np.cross(v[None, :], P - s[None, :]), axis=1) / np.linalg.norm(v) / width
+    keep = dist < 0.99 + 1e-4
+    if plane is not None:
+        keep &= (np.arange(P.shape[0]) // (Nx * Ny)) == plane
+    ind = np.nonzero(keep)[0]
+    return {int(i): (1.0 - dist[i], dist[i]) for i in ind}

@@ -10,7 +10,7 @@ from omega_amd.projector import Geometry
 
 
 def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
-              ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_3d=0, dPitchX=0.0, dPitchY=0.0):
+              ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_flags=0, dPitchX=0.0, dPitchY=0.0):
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
     d = _lib.GeomDesc()
@@ -35,7 +35,7 @@ def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, 
         keep.append(norm)
         d.norm, d.norm_len = norm.ctypes.data, norm.size
     d.orth_width = orth_width
-    d.orth_3d = orth_3d
+    d.orth_flags = orth_flags
     d.dPitchX = dPitchX
     d.dPitchY = dPitchY
     return Geometry(d, keep)
