@@ -128,6 +128,38 @@ int omg_timing_collect(double* total_ms, int64_t* launches);
 int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* counts, const int64_t* offsets,
                      uint32_t* idx, float* len, void* stream);
 
+/* ---- Batched dynamic-study OSEM (all frames of a study per launch) ----------------------------------
+ * The reference runs the frames of a dynamic study one after another inside every subset
+ * (source/cpp/reconstructionAF.h:1089-1453, per-frame data :429-436). These entry points advance all
+ * n_frames frames through a subset step together, tracing each LOR once per tile of frames.
+ * Layouts (frame fastest): image stack x[N][n_frames]; measurements meas[n_meas][n_frames] in the subset
+ * order of one frame; rhs[N][n_frames] is a work stack that must be zero on the first call and is left
+ * zero by every step. S is the subset's frame-independent sensitivity image (omg_frames_sensitivity).
+ * Siddon projector, emission data. */
+
+/* S = max(A_k^T 1, epps) for the subset's measurements [first, first + n_meas) (reconstructionAF.h:356-381,
+ * functions.hpp:2981-2996). */
+int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t n_meas, float* S, float epps, void* stream);
+
+/* One OSEM subset step of every frame: yhat_f = A_k x_f; r_f = meas_f / (yhat_f + epps) (NaN/Inf -> epps);
+ * x_f = x_f / S * (A_k^T r_f + epps); positivity x < epps -> epps (subiterStep.h:111-121).
+ * Launch shape: rays_per_block in {64,128,256} lanes (one LOR each), frames_per_tile in {1,2,4,8} frames per
+ * lane (n_frames must be a multiple), grid_blocks persistent workgroups (0: one resident wave). */
+int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x,
+                         const float* meas, const float* S, float* rhs, float epps, int32_t positivity,
+                         int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks, void* stream);
+
+/* A captured iteration: n_subsets subset steps (subset k: planes/LORs from first[k], n_meas[k] measurements at
+ * meas + meas_offset[k] * n_frames, sensitivity S + k * N) recorded once in a hipGraph and replayed by
+ * omg_graph_launch. The pointers are baked into the graph: keep the buffers alive and in place. */
+typedef struct omg_graph omg_graph;
+int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t* first, const int64_t* n_meas,
+                            const int64_t* meas_offset, int32_t n_frames, float* x, const float* meas,
+                            const float* S, float* rhs, float epps, int32_t positivity, int32_t rays_per_block,
+                            int32_t frames_per_tile, int32_t grid_blocks, omg_graph** out);
+int omg_graph_launch(omg_graph* h, void* stream);
+int omg_graph_destroy(omg_graph* h);
+
 /* Relative difference prior gradient: grad += beta * dR/du, 6-neighbourhood.
  * boundary 0: zero outside the volume (CPU RDPKernel, source/cpp/RDP.h:4-78);
  * boundary 1: clamp to edge (GPU RDPKernel with images, source/opencl/auxKernels.cl:664-873). */

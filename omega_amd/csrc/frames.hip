@@ -1,0 +1,351 @@
+// frames.hip -- batched dynamic-study OSEM: every frame of a study advances through one subset step in
+// one launch.
+//
+// The reference reconstructs the frames (timesteps) of a dynamic study one after another inside each
+// subset: FP -> ratio -> BP per frame, then the update of all frames (reconstructionAF.h:1089-1453,
+// per-frame data :429-436). The frames share the geometry, so the Siddon traversal of a LOR -- the
+// expensive part -- is the same for all of them. Here a lane traces its LOR once per tile of FT frames
+// and carries FT sums:
+//   * image stacks are frame-interleaved, x[voxel][frame], so the FT values a traversal step needs are
+//     one contiguous vector load (and the FT atomic adds of the BP fall in one cache line);
+//   * measurements are interleaved the same way, meas[lor][frame], in the subset order of one frame;
+//   * the sensitivity image of a subset is frame-independent (omg_frames_sensitivity) and computed once;
+//   * a persistent grid walks (LOR block, frame tile) work items, so one launch covers all frames;
+//   * the update kernel clears rhs behind itself, so a whole iteration (16 subset steps) replays from a
+//     captured hipGraph with no memset nodes.
+// Per frame, each FP sum, ratio and BP term is computed exactly as the single-frame fused kernel
+// computes it (k_siddon_osem in siddon.hip), so frames agree with single-frame runs up to the order of
+// the float atomics.
+#include "geom.hpp"
+
+#include <vector>
+
+namespace omg {
+
+constexpr int kFrameBatch = 4;   // traversal steps traced before their loads are consumed
+
+template <int FT>
+__device__ __forceinline__ void load_frames(const float* __restrict__ p, float (&v)[FT]) {
+    if constexpr (FT == 8) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        const float4 b = *reinterpret_cast<const float4*>(p + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else if constexpr (FT == 4) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else if constexpr (FT == 2) {
+        const float2 a = *reinterpret_cast<const float2*>(p);
+        v[0] = a.x; v[1] = a.y;
+    } else {
+        v[0] = p[0];
+    }
+}
+
+// Sum of len * x[voxel][f0 + f] along LOR lo for FT frames, in traversal order.
+template <int FT>
+__device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, const float* __restrict__ x, uint32_t F,
+                                              float (&ax)[FT], float& weight) {
+#pragma unroll
+    for (int f = 0; f < FT; f++) ax[f] = 0.f;
+    Walk w;
+    if (!siddon_setup<false>(g, lo, w, weight)) return false;
+    if (w.perp) {
+        perp_walk(w, [&](uint32_t ind, float len) {
+            float v[FT];
+            load_frames<FT>(x + (size_t)ind * F, v);
+#pragma unroll
+            for (int f = 0; f < FT; f++) ax[f] += len * v[f];
+        });
+        return true;
+    }
+    while (w.left > 0u) {
+        uint32_t ind[kFrameBatch];
+        float len[kFrameBatch], v[kFrameBatch][FT];
+#pragma unroll
+        for (int k = 0; k < kFrameBatch; k++) walk_step<false>(g, w, ind[k], len[k]);
+#pragma unroll
+        for (int k = 0; k < kFrameBatch; k++) load_frames<FT>(x + (size_t)(len[k] > 0.f ? ind[k] : 0u) * F, v[k]);
+#pragma unroll
+        for (int k = 0; k < kFrameBatch; k++)
+#pragma unroll
+            for (int f = 0; f < FT; f++) ax[f] = (len[k] > 0.f) ? ax[f] + len[k] * v[k][f] : ax[f];
+    }
+    return true;
+}
+
+// One subset step of all frames: FP, ratio, BP into rhs[voxel][frame]. Work item = (frame tile, LOR block).
+// `x`, `meas`, `rhs` point at frame 0 of their stacks; meas holds the subset's LORs only.
+template <int FT, int RPB>
+__global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t F, const float* __restrict__ x,
+                                                     const float* __restrict__ meas, float* __restrict__ rhs,
+                                                     float epps, uint32_t lor_blocks, uint32_t n_items) {
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t tile = item / lor_blocks;
+        const uint32_t lb = item - tile * lor_blocks;
+        const int64_t lo = (int64_t)lb * RPB + threadIdx.x;
+        if (lo >= a.nmeas) continue;
+        const uint32_t f0 = tile * FT;
+        float ax[FT], w = 0.f;
+        if (!frames_gather<FT>(a, lo, x + f0, F, ax, w)) continue;
+        float r[FT];
+        bool any = false;
+#pragma unroll
+        for (int f = 0; f < FT; f++) {
+            const float yhat = ax[f] * w;
+            float q = meas[(size_t)lo * F + f0 + f] / (yhat + epps);
+            if (isnan(q) || isinf(q)) q = epps;
+            r[f] = q;
+            any |= (q != 0.f);
+        }
+        if (!any) continue;
+        siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
+            const float lw = len * w;
+            float* p = rhs + (size_t)ind * F + f0;
+#pragma unroll
+            for (int f = 0; f < FT; f++) unsafeAtomicAdd(p + f, r[f] * lw);
+        });
+    }
+}
+
+// S += A_k^T 1 (frame-independent sensitivity of one subset).
+__global__ __launch_bounds__(256) void k_frames_sens(SidArgs a, float* __restrict__ S) {
+    const int64_t lo = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lo >= a.nmeas) return;
+    float w = 0.f;
+    siddon_trace(a, lo, w, [&](uint32_t ind, float len) { unsafeAtomicAdd(S + ind, len * w); });
+}
+
+__global__ __launch_bounds__(256) void k_frames_clamp(float* __restrict__ S, uint32_t N, float epps) {
+    const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+    if (n < N && S[n] < epps) S[n] = epps;
+}
+
+__global__ __launch_bounds__(256) void k_frames_fill(float* __restrict__ p, size_t n, float v) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// x[n][f] = x / S[n] * (rhs + epps) (subiterStep.h:111-121, EM algorithms.h:12-15), positivity, rhs = 0.
+__global__ __launch_bounds__(256) void k_frames_update(uint32_t N, uint32_t F, float* __restrict__ x,
+                                                       const float* __restrict__ S, float* __restrict__ rhs,
+                                                       float epps, int pos) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)N * F) return;
+    const float s = S[i / F];
+    float xn = x[i] / s * (rhs[i] + epps);
+    if (pos && xn < epps) xn = epps;
+    x[i] = xn;
+    rhs[i] = 0.f;
+}
+
+struct FramesStep {
+    SidArgs a;
+    uint32_t F, N;
+    float* x;
+    const float* meas;
+    const float* S;
+    float* rhs;
+    float epps;
+    int pos;
+    int rpb, ft, grid;
+};
+
+// Workgroups of one resident wave of k_frames_osem<FT, RPB> on the current device.
+template <int FT, int RPB>
+static int resident_blocks() {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_frames_osem<FT, RPB>, RPB, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return cus * per_cu;
+}
+
+template <int RPB>
+static int resident_blocks_ft(int ft) {
+    switch (ft) {
+        case 1: return resident_blocks<1, RPB>();
+        case 2: return resident_blocks<2, RPB>();
+        case 4: return resident_blocks<4, RPB>();
+        default: return resident_blocks<8, RPB>();
+    }
+}
+
+static int resident_blocks_for(int rpb, int ft) {
+    switch (rpb) {
+        case 64: return resident_blocks_ft<64>(ft);
+        case 128: return resident_blocks_ft<128>(ft);
+        default: return resident_blocks_ft<256>(ft);
+    }
+}
+
+template <int FT, int RPB>
+static hipError_t launch_data(const FramesStep& st, hipStream_t s) {
+    const uint32_t lor_blocks = (uint32_t)((st.a.nmeas + RPB - 1) / RPB);
+    const uint32_t n_items = lor_blocks * (st.F / FT);
+    uint32_t grid = (uint32_t)st.grid;
+    if (grid > n_items) grid = n_items;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_frames_osem<FT, RPB>), dim3(grid), dim3(RPB), 0, s, st.a, st.F, st.x, st.meas, st.rhs,
+                       st.epps, lor_blocks, n_items);
+    return hipGetLastError();
+}
+
+template <int RPB>
+static hipError_t launch_data_ft(const FramesStep& st, hipStream_t s) {
+    switch (st.ft) {
+        case 1: return launch_data<1, RPB>(st, s);
+        case 2: return launch_data<2, RPB>(st, s);
+        case 4: return launch_data<4, RPB>(st, s);
+        default: return launch_data<8, RPB>(st, s);
+    }
+}
+
+static int launch_step(const FramesStep& st, hipStream_t s) {
+    hipError_t e;
+    switch (st.rpb) {
+        case 64: e = launch_data_ft<64>(st, s); break;
+        case 128: e = launch_data_ft<128>(st, s); break;
+        default: e = launch_data_ft<256>(st, s); break;
+    }
+    if (e != hipSuccess) {
+        set_error("frames OSEM launch failed: %s", hipGetErrorString(e));
+        return OMG_ERR_HIP;
+    }
+    const size_t NF = (size_t)st.N * st.F;
+    hipLaunchKernelGGL(k_frames_update, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, st.N, st.F, st.x, st.S,
+                       st.rhs, st.epps, st.pos);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x, const float* meas,
+                     const float* S, float* rhs, float epps, int32_t positivity, int32_t rays_per_block,
+                     int32_t frames_per_tile, int32_t grid_blocks, FramesStep& st) {
+    OMG_REQUIRE(g != nullptr, "null geometry");
+    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "batched frames run the improved Siddon projector");
+    OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
+    OMG_REQUIRE(x && meas && S && rhs, "null pointer argument");
+    OMG_REQUIRE(n_frames >= 1, "n_frames must be >= 1");
+    OMG_REQUIRE(rays_per_block == 64 || rays_per_block == 128 || rays_per_block == 256,
+                "rays_per_block must be 64, 128 or 256");
+    OMG_REQUIRE(frames_per_tile == 1 || frames_per_tile == 2 || frames_per_tile == 4 || frames_per_tile == 8,
+                "frames_per_tile must be 1, 2, 4 or 8");
+    OMG_REQUIRE(n_frames % frames_per_tile == 0, "n_frames (%d) must be a multiple of frames_per_tile (%d)", n_frames,
+                frames_per_tile);
+    OMG_REQUIRE(grid_blocks >= 0, "grid_blocks must be >= 0 (0 = one wave of resident workgroups)");
+    OMG_REQUIRE(((uintptr_t)x % (4 * frames_per_tile)) == 0 && ((uintptr_t)meas % 4) == 0,
+                "image stack must be aligned to frames_per_tile floats");
+    const int rc = check_range(g, first, n_meas);
+    if (rc) return rc;
+    st.a = make_sid_args(g, first, n_meas);
+    st.F = (uint32_t)n_frames;
+    st.N = g->d.Nx * g->d.Ny * g->d.Nz;
+    st.x = x; st.meas = meas; st.S = S; st.rhs = rhs;
+    st.epps = epps; st.pos = positivity != 0;
+    st.rpb = rays_per_block; st.ft = frames_per_tile;
+    st.grid = grid_blocks > 0 ? grid_blocks : resident_blocks_for(rays_per_block, frames_per_tile);
+    return OMG_OK;
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+struct omg_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t n_meas, float* S, float epps,
+                                      void* stream) {
+    OMG_REQUIRE(g != nullptr && S != nullptr, "null argument");
+    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
+                "batched frames run the improved Siddon projector on emission data");
+    const int rc = check_range(g, first, n_meas);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+    hipLaunchKernelGGL(k_frames_fill, dim3(grid_for(N, 256)), dim3(256), 0, s, S, (size_t)N, 0.f);
+    OMG_CHECK_LAUNCH();
+    if (n_meas > 0) {
+        const SidArgs a = make_sid_args(g, first, n_meas);
+        hipLaunchKernelGGL(k_frames_sens, dim3(grid_for(n_meas, 256)), dim3(256), 0, s, a, S);
+        OMG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_frames_clamp, dim3(grid_for(N, 256)), dim3(256), 0, s, S, N, epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x,
+                                    const float* meas, const float* S, float* rhs, float epps, int32_t positivity,
+                                    int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
+                                    void* stream) {
+    FramesStep st;
+    int rc = make_step(g, first, n_meas, n_frames, x, meas, S, rhs, epps, positivity, rays_per_block, frames_per_tile,
+                       grid_blocks, st);
+    if (rc) return rc;
+    return launch_step(st, as_stream(stream));
+}
+
+extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t* first,
+                                       const int64_t* n_meas, const int64_t* meas_offset, int32_t n_frames, float* x,
+                                       const float* meas, const float* S, float* rhs, float epps, int32_t positivity,
+                                       int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
+                                       omg_graph** out) {
+    OMG_REQUIRE(out != nullptr, "null output handle");
+    *out = nullptr;
+    OMG_REQUIRE(n_subsets >= 1 && first && n_meas && meas_offset, "subset tables are required");
+    OMG_REQUIRE(g != nullptr, "null geometry");
+    const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+    std::vector<FramesStep> steps((size_t)n_subsets);
+    for (int32_t k = 0; k < n_subsets; k++) {
+        OMG_REQUIRE(meas_offset[k] >= 0, "negative measurement offset");
+        const int rc = make_step(g, first[k], n_meas[k], n_frames, x, meas + (size_t)meas_offset[k] * n_frames,
+                                 S + (size_t)k * N, rhs, epps, positivity, rays_per_block, frames_per_tile, grid_blocks,
+                                 steps[(size_t)k]);
+        if (rc) return rc;
+    }
+    hipStream_t cs;
+    OMG_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    omg_graph* h = new omg_graph;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = OMG_OK;
+    if (e == hipSuccess) {
+        const size_t NF = (size_t)N * n_frames;
+        hipLaunchKernelGGL(k_frames_fill, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, cs, rhs, NF, 0.f);
+        for (const FramesStep& st : steps) {
+            rc = launch_step(st, cs);
+            if (rc) break;
+        }
+        const hipError_t e2 = hipStreamEndCapture(cs, &h->graph);
+        if (!rc && e2 != hipSuccess) e = e2;
+    }
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0);
+    hipStreamDestroy(cs);
+    if (rc || e != hipSuccess) {
+        if (!rc) set_error("hipGraph capture of the frames iteration failed: %s", hipGetErrorString(e));
+        if (h->exec) hipGraphExecDestroy(h->exec);
+        if (h->graph) hipGraphDestroy(h->graph);
+        delete h;
+        return rc ? rc : OMG_ERR_HIP;
+    }
+    *out = h;
+    return OMG_OK;
+}
+
+extern "C" int omg_graph_launch(omg_graph* h, void* stream) {
+    OMG_REQUIRE(h != nullptr && h->exec != nullptr, "null graph");
+    OMG_CHECK_HIP(hipGraphLaunch(h->exec, as_stream(stream)));
+    return OMG_OK;
+}
+
+extern "C" int omg_graph_destroy(omg_graph* h) {
+    if (!h) return OMG_OK;
+    if (h->exec) hipGraphExecDestroy(h->exec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    delete h;
+    return OMG_OK;
+}

@@ -1,5 +1,9 @@
 """The measured workloads, built from their stated sizes with seeded synthetic data.
 
+``dynamic``: the 3-D PET geometry of ``pet3d_projector`` with 24 frames whose images follow
+seeded one-tissue-compartment curves (``phantoms.dynamic_ellipsoids``) and whose total counts are
+lognormal in [1e5, 1e7] (``frame_counts``); 16 subsets x 3 iterations, all frames per launch.
+
 ``pet2d_osem``: 2-D plane PET OSEM with improved Siddon. Ring radius 400 mm, 63 direct sinogram
 planes of 128 radial bins x 160 angles over 180 degrees (2.5 mm radial pitch); image 128x128x63
 with 2.5 mm voxels; phantom of 8 seeded ellipsoids, activity U(0.5, 2.0) on a 0.1 background;
@@ -67,6 +71,29 @@ def pet3d_projector(subsets: int = 1, n_rings: int = 32, crystals: int = 448, ri
     p.subsetType = 8
     p.addProjector()
     return p
+
+
+def frame_counts(n_frames: int, seed: int, lo: float = 1e5, hi: float = 1e7):
+    """Per-frame total counts, lognormal (log10 ~ N(6, 0.6)) clipped to [lo, hi]."""
+    rng = np.random.default_rng(seed + 104729)
+    return 10.0 ** np.clip(rng.normal(6.0, 0.6, n_frames), np.log10(lo), np.log10(hi))
+
+
+def dynamic_data(p: projectorClass, frames, counts, seed: int, device="cuda"):
+    """Frame-interleaved Poisson data [M, F] in subset order: frame f is Poisson(A x_f scaled to counts[f])."""
+    import torch
+    F = frames.shape[1]
+    M = int(p.nTotMeas[-1])
+    y = torch.empty((M, F), dtype=torch.float32, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    for f in range(F):
+        x = torch.as_tensor(np.ascontiguousarray(frames[:, f]), device=device)
+        ax = torch.cat([p.forwardProject(x, k) for k in range(p.subsets)])
+        scale = float(counts[f]) / float(ax.double().sum())
+        y[:, f] = torch.poisson(ax * scale, generator=g)
+        del ax
+    return y
 
 
 def poisson_data(p: projectorClass, phantom, total_counts: float, seed: int, device="cuda"):
