@@ -73,6 +73,22 @@ SIGNATURES = {
     "omg_frames_graph_create": [_P, _I32, _P, _P, _P, _I32, _P, _P, _P, _P, _F, _I32, _I32, _I32, _I32, _P],
     "omg_graph_launch": [_P, _P],
     "omg_graph_destroy": [_P],
+    "omg_conv3d_forward": [_P, _I32, _P, _I32, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P],
+    "omg_conv3d_wgrad": [_P, _I32, _P, _I32, _P, _I32, _P, _I32, _I32, _I32, _P],
+    "omg_conv3d_flipT": [_P, _P, _I32, _I32, _P],
+    "omg_bias_grad": [_P, _P, _I32, _I32, _P],
+    "omg_relu_backward": [_P, _P, _I64, _P],
+    "omg_pool_forward": [_P, _P, _I32, _I32, _I32, _I32, _P],
+    "omg_pool_backward": [_P, _P, _I32, _I32, _I32, _I32, _I32, _P],
+    "omg_upsample_forward": [_P, _P, _I32, _I32, _I32, _I32, _P],
+    "omg_upsample_backward": [_P, _I32, _I32, _P, _I32, _I32, _I32, _I32, _P],
+    "omg_take_channels": [_P, _I32, _I32, _P, _I64, _I32, _I32, _P],
+    "omg_head_forward": [_P, _P, _P, _P, _P, _I32, _I32, _P],
+    "omg_head_backward": [_P, _P, _P, _P, _P, _P, _I32, _I32, _P],
+    "omg_mse": [_P, _P, _P, _P, _P, _I32, _F, _P],
+    "omg_adam": [_P, _P, _P, _P, _P, _I32, _F, _F, _F, _F, _I32, _P],
+    "omg_to_bf16": [_P, _P, _I64, _P],
+    "omg_embed": [_P, _P, _I32, _I32, _I32, _I32, _F, _P],
     "omg_timing_enable": [_I32],
     "omg_timing_collect": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "omg_rdp_gradient": [_P, _P, _U32, _U32, _U32, _F, _F, _F, _I32, _P],

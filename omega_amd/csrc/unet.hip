@@ -1,0 +1,511 @@
+// unet.hip -- kernels of the 3-D U-Net that supplies the OSEM-RDP warm start: BF16 activations and
+// weights, FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16), FP32 master weights.
+//
+// Layout: activations are channels-last, act[voxel][channel] with voxel = (z * H + y) * W + x, channel
+// counts multiples of 16; conv weights are w[tap][cout][cin] with tap = (dz * 3 + dy) * 3 + dx.
+// A 3x3x3 convolution (zero padding 1) is an implicit GEMM: Y[v][co] = sum_{tap, ci} W[tap][co][ci] *
+// X[v + off(tap)][ci]. One wave computes 16 voxels x all output channels; per (tap, 16-channel block)
+// each lane feeds the matrix core 4 consecutive input channels of one voxel (an 8-byte load) and 4
+// consecutive input channels of one weight row. Lane l of a 16x16x16 MFMA holds A[l % 16][4 (l / 16) + j],
+// B[4 (l / 16) + j][l % 16] and receives D[4 (l / 16) + j][l % 16], j = 0..3.
+//
+// Backward: dX is the same convolution of dY with the flipped, transposed weights (k_conv_flipT);
+// dW[tap] = sum_v dY[v]^T X[v + off(tap)] is a GEMM over voxels with partial sums per wave
+// (k_conv3d_wgrad), reduced with float atomics; ReLU, pooling, upsampling, the 1x1 output layer,
+// the loss and Adam are elementwise kernels.
+#include "common.hpp"
+
+namespace omg {
+
+typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even; NaN stays NaN
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+struct Vol { int D, H, W; };
+
+__device__ __forceinline__ bool neighbour(const Vol& g, int v, int tap, int& nb) {
+    const int x = v % g.W, t = v / g.W, y = t % g.H, z = t / g.H;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int zz = z + dz, yy = y + dy, xx = x + dx;
+    nb = (zz * g.H + yy) * g.W + xx;
+    return zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+}
+
+__device__ __forceinline__ bf4 load4(const uint16_t* p) { return *reinterpret_cast<const bf4*>(p); }
+
+// Y = act(conv(X) + b), X = [X1 | X2] (channel concatenation), MT = Cout / 16. 4 waves x 16 voxels per block.
+template <int MT>
+__global__ __launch_bounds__(256) void k_conv3d_fwd(const uint16_t* __restrict__ X1, int C1,
+                                                    const uint16_t* __restrict__ X2, int C2,
+                                                    const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                                    uint16_t* __restrict__ Y, Vol g, int act) {
+    const int V = g.D * g.H * g.W;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = (blockIdx.x * 4 + wave) * 16;
+    if (base >= V) return;
+    const int Cin = C1 + C2, Cout = MT * 16;
+    const int col = lane & 15, kq = (lane >> 4) * 4;
+    const int v = base + col;
+    const bool vin = v < V;
+    f4v acc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) acc[m] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 27; tap++) {
+        int nb = 0;
+        const bool ok = vin && neighbour(g, v, tap, nb);
+        for (int cb = 0; cb < Cin; cb += 16) {
+            const int c = cb + kq;
+            bf4 b = bf4{0, 0, 0, 0};
+            if (ok) b = (c < C1) ? load4(X1 + (size_t)nb * C1 + c) : load4(X2 + (size_t)nb * C2 + (c - C1));
+            const uint16_t* wrow = Wt + ((size_t)tap * Cout + col) * Cin + c;
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const bf4 a = load4(wrow + (size_t)m * 16 * Cin);
+                acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
+            }
+        }
+    }
+    if (!vin) return;
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int co = m * 16 + kq;
+        uint16_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float val = acc[m][j] + (bias ? bias[co + j] : 0.f);
+            if (act == 1) val = fmaxf(val, 0.f);
+            o[j] = f2bf(val);
+        }
+        uint2 pk;
+        pk.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+        pk.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+        *reinterpret_cast<uint2*>(Y + (size_t)v * Cout + co) = pk;
+    }
+}
+
+// dW[tap][co][ci] += sum over this wave's voxels of dY[v][co] * X[v + off(tap)][ci]. grid = (chunks, 27).
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void k_conv3d_wgrad(const uint16_t* __restrict__ dY,
+                                                      const uint16_t* __restrict__ X1, int C1,
+                                                      const uint16_t* __restrict__ X2, int C2,
+                                                      float* __restrict__ dW, Vol g, int vox_per_wave) {
+    const int V = g.D * g.H * g.W;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tap = blockIdx.y;
+    const int Cout = MT * 16, Cin = NT * 16;
+    const int v0 = (blockIdx.x * 4 + wave) * vox_per_wave;
+    if (v0 >= V) return;
+    const int v1 = min(V, v0 + vox_per_wave);
+    const int col = lane & 15, kq = (lane >> 4) * 4;
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[m][n] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int kb = v0; kb < v1; kb += 16) {
+        // this lane's 4 voxels of the K block
+        int nbv[4];
+        bool okv[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int v = kb + kq + j;
+            okv[j] = v < v1 && neighbour(g, v, tap, nbv[j]);
+        }
+        bf4 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const int co = m * 16 + col;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int v = kb + kq + j;
+                a[m][j] = (v < v1) ? (short)dY[(size_t)v * Cout + co] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const int ci = n * 16 + col;
+            bf4 b;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint16_t xv = 0;
+                if (okv[j]) xv = (ci < C1) ? X1[(size_t)nbv[j] * C1 + ci] : X2[(size_t)nbv[j] * C2 + (ci - C1)];
+                b[j] = (short)xv;
+            }
+#pragma unroll
+            for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int co = m * 16 + kq + j, ci = n * 16 + col;
+                unsafeAtomicAdd(dW + ((size_t)tap * Cout + co) * Cin + ci, acc[m][n][j]);
+            }
+}
+
+// Wf[tap][ci][co] = W[26 - tap][co][ci]: dX = conv(dY) with these weights.
+__global__ void k_conv_flipT(const uint16_t* __restrict__ W, uint16_t* __restrict__ Wf, int Cout, int Cin) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 27 * Cout * Cin) return;
+    const int ci = i % Cin, t = i / Cin, co = t % Cout, tap = t / Cout;
+    Wf[((size_t)(26 - tap) * Cin + ci) * Cout + co] = W[i];
+}
+
+// db[c] += sum_v dY[v][c]
+__global__ void k_bias_grad(const uint16_t* __restrict__ dY, float* __restrict__ db, int V, int C) {
+    const int c = threadIdx.x % C;
+    const int lanes = 256 / C;     // C divides 256 for C in {16, 32, 64}
+    const int r = threadIdx.x / C;
+    float s = 0.f;
+    for (int v = blockIdx.x * lanes + r; v < V; v += gridDim.x * lanes) s += bf2f(dY[(size_t)v * C + c]);
+    unsafeAtomicAdd(db + c, s);
+}
+
+// dZ = (Y > 0) ? dY : 0 over n elements (in place allowed)
+__global__ void k_relu_bwd(const uint16_t* __restrict__ Y, uint16_t* dY, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && !(bf2f(Y[i]) > 0.f)) dY[i] = 0;
+}
+
+// 2x2x2 average pooling of X[D][H][W][C] (even D, H, W)
+__global__ void k_pool_fwd(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, Vol g, int C) {
+    const int Ho = g.H / 2, Wo = g.W / 2, Do = g.D / 2;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)Do * Ho * Wo * C) return;
+    const int c = (int)(i % C);
+    const size_t vo = i / C;
+    const int xo = (int)(vo % Wo), yo = (int)((vo / Wo) % Ho), zo = (int)(vo / ((size_t)Wo * Ho));
+    float s = 0.f;
+    for (int d = 0; d < 8; d++) {
+        const int z = 2 * zo + (d >> 2), y = 2 * yo + ((d >> 1) & 1), x = 2 * xo + (d & 1);
+        s += bf2f(X[((size_t)(z * g.H + y) * g.W + x) * C + c]);
+    }
+    Y[i] = f2bf(s * 0.125f);
+}
+
+// dX (fine grid, C channels, stride/offset into dst) (+)= dY(coarse)[..][c] / 8
+__global__ void k_pool_bwd(const uint16_t* __restrict__ dY, uint16_t* __restrict__ dX, Vol g, int C, int accumulate) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)g.D * g.H * g.W * C) return;
+    const int c = (int)(i % C);
+    const size_t v = i / C;
+    const int x = (int)(v % g.W), y = (int)((v / g.W) % g.H), z = (int)(v / ((size_t)g.W * g.H));
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    const float val = bf2f(dY[((size_t)((z / 2) * Ho + y / 2) * Wo + x / 2) * C + c]) * 0.125f;
+    dX[i] = f2bf(accumulate ? bf2f(dX[i]) + val : val);
+}
+
+// nearest-neighbour x2 upsampling: Y(fine)[v][c] = X(coarse)[parent(v)][c]; g = fine grid
+__global__ void k_up_fwd(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, Vol g, int C) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)g.D * g.H * g.W * C) return;
+    const int c = (int)(i % C);
+    const size_t v = i / C;
+    const int x = (int)(v % g.W), y = (int)((v / g.W) % g.H), z = (int)(v / ((size_t)g.W * g.H));
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    Y[i] = X[((size_t)((z / 2) * Ho + y / 2) * Wo + x / 2) * C + c];
+}
+
+// dX(coarse)[vo][c] = sum of the 8 children dY(fine)[v][off + c] (row stride Cs); g = coarse grid
+__global__ void k_up_bwd(const uint16_t* __restrict__ dY, int Cs, int off, uint16_t* __restrict__ dX, Vol g, int C) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)g.D * g.H * g.W * C) return;
+    const int c = (int)(i % C);
+    const size_t vo = i / C;
+    const int xo = (int)(vo % g.W), yo = (int)((vo / g.W) % g.H), zo = (int)(vo / ((size_t)g.W * g.H));
+    const int H2 = g.H * 2, W2 = g.W * 2;
+    float s = 0.f;
+    for (int d = 0; d < 8; d++) {
+        const int z = 2 * zo + (d >> 2), y = 2 * yo + ((d >> 1) & 1), x = 2 * xo + (d & 1);
+        s += bf2f(dY[((size_t)(z * H2 + y) * W2 + x) * Cs + off + c]);
+    }
+    dX[i] = f2bf(s);
+}
+
+// dst[v][c] (+)= src[v][off + c] (row stride Cs), C channels
+__global__ void k_take_channels(const uint16_t* __restrict__ src, int Cs, int off, uint16_t* __restrict__ dst,
+                                size_t V, int C, int accumulate) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= V * C) return;
+    const size_t v = i / C;
+    const int c = (int)(i % C);
+    const float val = bf2f(src[v * Cs + off + c]);
+    dst[i] = f2bf(accumulate ? bf2f(dst[i]) + val : val);
+}
+
+// Output layer: out[v] = base[v] + b + sum_c w[c] * X[v][c]  (FP32)
+__global__ void k_head_fwd(const uint16_t* __restrict__ X, const float* __restrict__ w, const float* __restrict__ b,
+                           const float* __restrict__ base, float* __restrict__ out, int V, int C) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= V) return;
+    float s = b[0];
+    for (int c = 0; c < C; c++) s += w[c] * bf2f(X[(size_t)v * C + c]);
+    out[v] = base[v] + s;
+}
+
+// Head backward: dX[v][c] = g[v] w[c] (BF16); dw[c] += sum g[v] X[v][c]; db += sum g[v]
+__global__ void k_head_bwd(const uint16_t* __restrict__ X, const float* __restrict__ w, const float* __restrict__ gout,
+                           uint16_t* __restrict__ dX, float* __restrict__ dw, float* __restrict__ db, int V, int C) {
+    __shared__ float red[64][17];
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    const float gv = v < V ? gout[v] : 0.f;
+    float part[64];
+    for (int c = 0; c < C; c++) {
+        const float xv = v < V ? bf2f(X[(size_t)v * C + c]) : 0.f;
+        part[c] = gv * xv;
+        if (v < V) dX[(size_t)v * C + c] = f2bf(gv * w[c]);
+    }
+    // reduce part[c] over the block: per wave with shuffles, then atomics
+    for (int c = 0; c < C; c++) {
+        float s = part[c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if ((threadIdx.x & 63) == 0) red[c][threadIdx.x >> 6] = s;
+    }
+    float sg = gv;
+    for (int o = 32; o > 0; o >>= 1) sg += __shfl_xor(sg, o, 64);
+    if ((threadIdx.x & 63) == 0) red[C][threadIdx.x >> 6] = sg;  // C < 64: row C free
+    __syncthreads();
+    if (threadIdx.x < C) {
+        const float s = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+        unsafeAtomicAdd(dw + threadIdx.x, s);
+    }
+    if (threadIdx.x == 0) unsafeAtomicAdd(db, red[C][0] + red[C][1] + red[C][2] + red[C][3]);
+}
+
+// Masked mean squared error: loss += sum m (out - t)^2 / n_valid (FP64 accumulation); g = 2 m (out - t) / n_valid
+__global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t, const float* __restrict__ mask,
+                      float* __restrict__ g, double* __restrict__ loss, int V, float inv_n) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    double e2 = 0.0;
+    if (v < V) {
+        const float m = mask ? mask[v] : 1.f;
+        const float e = (out[v] - t[v]) * m;
+        g[v] = 2.f * e * inv_n;
+        e2 = (double)e * e * inv_n;
+    }
+    for (int o = 32; o > 0; o >>= 1) e2 += __shfl_xor(e2, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, e2);
+}
+
+// Adam on FP32 master weights p with moments m, v; writes the BF16 copy q (q may be null).
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
+                       float* __restrict__ v, uint16_t* __restrict__ q, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (q) q[i] = f2bf(pi);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = f2bf(src[i]);
+}
+
+// Network input: image [Nz][Ny][Nx] * scale into channel 0 of a 16-channel BF16 volume [D][H][W][16]
+// (D >= Nz, H = Ny, W = Nx; planes z >= Nz and channels 1..15 are zero).
+__global__ void k_embed(const float* __restrict__ img, uint16_t* __restrict__ X, Vol g, int Nz, float scale) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t V = (size_t)g.D * g.H * g.W;
+    if (i >= V * 16) return;
+    const int c = (int)(i & 15);
+    const size_t v = i >> 4;
+    const int z = (int)(v / ((size_t)g.H * g.W));
+    X[i] = (c == 0 && z < Nz) ? f2bf(img[v] * scale) : (uint16_t)0;
+}
+
+static inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
+template <int MT>
+static void launch_fwd(const uint16_t* X1, int C1, const uint16_t* X2, int C2, const uint16_t* W, const float* b,
+                       uint16_t* Y, Vol g, int act, hipStream_t s) {
+    const int V = g.D * g.H * g.W;
+    hipLaunchKernelGGL(k_conv3d_fwd<MT>, dim3((unsigned)((V + 63) / 64)), dim3(256), 0, s, X1, C1, X2, C2, W, b, Y, g,
+                       act);
+}
+
+template <int MT, int NT>
+static void launch_wgrad(const uint16_t* dY, const uint16_t* X1, int C1, const uint16_t* X2, int C2, float* dW, Vol g,
+                         hipStream_t s) {
+    const int V = g.D * g.H * g.W;
+    const int vpw = 1024;
+    hipLaunchKernelGGL((k_conv3d_wgrad<MT, NT>), dim3((unsigned)((V + 4 * vpw - 1) / (4 * vpw)), 27), dim3(256), 0, s,
+                       dY, X1, C1, X2, C2, dW, g, vpw);
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+static int vol_ok(int D, int H, int W) { return D > 0 && H > 0 && W > 0 && (int64_t)D * H * W < (1ll << 31); }
+
+extern "C" int omg_conv3d_forward(const uint16_t* X1, int32_t C1, const uint16_t* X2, int32_t C2, const uint16_t* W,
+                                  const float* bias, uint16_t* Y, int32_t Cout, int32_t D, int32_t H, int32_t Wd,
+                                  int32_t relu, void* stream) {
+    OMG_REQUIRE(vol_ok(D, H, Wd), "bad volume size");
+    OMG_REQUIRE(X1 && W && Y && C1 > 0 && C1 % 16 == 0 && C2 >= 0 && C2 % 16 == 0 && (C2 == 0 || X2),
+                "inputs must have a multiple of 16 channels");
+    const Vol g{D, H, Wd};
+    hipStream_t s = as_stream(stream);
+    switch (Cout) {
+        case 16: launch_fwd<1>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;
+        case 32: launch_fwd<2>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;
+        case 48: launch_fwd<3>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;
+        case 64: launch_fwd<4>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;
+        case 96: launch_fwd<6>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;
+        default: OMG_REQUIRE(false, "Cout %d is not one of 16, 32, 48, 64, 96", Cout);
+    }
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_conv3d_wgrad(const uint16_t* dY, int32_t Cout, const uint16_t* X1, int32_t C1, const uint16_t* X2,
+                                int32_t C2, float* dW, int32_t D, int32_t H, int32_t Wd, void* stream) {
+    OMG_REQUIRE(vol_ok(D, H, Wd), "bad volume size");
+    OMG_REQUIRE(dY && X1 && dW && C1 % 16 == 0 && C2 % 16 == 0 && (C2 == 0 || X2), "bad arguments");
+    const Vol g{D, H, Wd};
+    hipStream_t s = as_stream(stream);
+    const int MT = Cout / 16, NT = (C1 + C2) / 16;
+    if (MT == 1 && NT == 1) launch_wgrad<1, 1>(dY, X1, C1, X2, C2, dW, g, s);
+    else if (MT == 2 && NT == 1) launch_wgrad<2, 1>(dY, X1, C1, X2, C2, dW, g, s);
+    else if (MT == 2 && NT == 2) launch_wgrad<2, 2>(dY, X1, C1, X2, C2, dW, g, s);
+    else if (MT == 4 && NT == 2) launch_wgrad<4, 2>(dY, X1, C1, X2, C2, dW, g, s);
+    else if (MT == 4 && NT == 4) launch_wgrad<4, 4>(dY, X1, C1, X2, C2, dW, g, s);
+    else if (MT == 2 && NT == 6) launch_wgrad<2, 6>(dY, X1, C1, X2, C2, dW, g, s);
+    else if (MT == 1 && NT == 3) launch_wgrad<1, 3>(dY, X1, C1, X2, C2, dW, g, s);
+    else OMG_REQUIRE(false, "(Cout, Cin) = (%d, %d) is not a layer shape of the network", Cout, C1 + C2);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_conv3d_flipT(const uint16_t* W, uint16_t* Wf, int32_t Cout, int32_t Cin, void* stream) {
+    OMG_REQUIRE(W && Wf && Cout > 0 && Cin > 0, "bad arguments");
+    hipLaunchKernelGGL(k_conv_flipT, dim3(blocks((size_t)27 * Cout * Cin)), dim3(256), 0, as_stream(stream), W, Wf,
+                       Cout, Cin);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_bias_grad(const uint16_t* dY, float* db, int32_t V, int32_t C, void* stream) {
+    OMG_REQUIRE(dY && db && (C == 16 || C == 32 || C == 64), "bias gradient needs C in {16, 32, 64}");
+    hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), 0, as_stream(stream), dY, db, V, C);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_relu_backward(const uint16_t* Y, uint16_t* dY, int64_t n, void* stream) {
+    OMG_REQUIRE(Y && dY && n >= 0, "bad arguments");
+    if (n) hipLaunchKernelGGL(k_relu_bwd, dim3(blocks((size_t)n)), dim3(256), 0, as_stream(stream), Y, dY, (size_t)n);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_pool_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32_t H, int32_t W, int32_t C,
+                                void* stream) {
+    OMG_REQUIRE(X && Y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "pooling needs even dimensions");
+    hipLaunchKernelGGL(k_pool_fwd, dim3(blocks((size_t)D * H * W * C / 8)), dim3(256), 0, as_stream(stream), X, Y,
+                       Vol{D, H, W}, C);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_pool_backward(const uint16_t* dY, uint16_t* dX, int32_t D, int32_t H, int32_t W, int32_t C,
+                                 int32_t accumulate, void* stream) {
+    OMG_REQUIRE(dY && dX && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "pooling needs even dimensions");
+    hipLaunchKernelGGL(k_pool_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, dX,
+                       Vol{D, H, W}, C, accumulate);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32_t H, int32_t W, int32_t C,
+                                    void* stream) {
+    OMG_REQUIRE(X && Y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "upsampling needs an even fine grid");
+    hipLaunchKernelGGL(k_up_fwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), X, Y,
+                       Vol{D, H, W}, C);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off, uint16_t* dX, int32_t D, int32_t H,
+                                     int32_t W, int32_t C, void* stream) {
+    OMG_REQUIRE(dY && dX && off >= 0 && off + C <= Cs, "bad channel window");
+    hipLaunchKernelGGL(k_up_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, Cs, off, dX,
+                       Vol{D, H, W}, C);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_take_channels(const uint16_t* src, int32_t Cs, int32_t off, uint16_t* dst, int64_t V, int32_t C,
+                                 int32_t accumulate, void* stream) {
+    OMG_REQUIRE(src && dst && off >= 0 && off + C <= Cs, "bad channel window");
+    hipLaunchKernelGGL(k_take_channels, dim3(blocks((size_t)V * C)), dim3(256), 0, as_stream(stream), src, Cs, off, dst,
+                       (size_t)V, C, accumulate);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_head_forward(const uint16_t* X, const float* w, const float* b, const float* base, float* out,
+                                int32_t V, int32_t C, void* stream) {
+    OMG_REQUIRE(X && w && b && base && out && C > 0 && C < 64, "bad arguments");
+    hipLaunchKernelGGL(k_head_fwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, b, base, out, V, C);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_head_backward(const uint16_t* X, const float* w, const float* g, uint16_t* dX, float* dw, float* db,
+                                 int32_t V, int32_t C, void* stream) {
+    OMG_REQUIRE(X && w && g && dX && dw && db && C > 0 && C < 64, "bad arguments");
+    hipLaunchKernelGGL(k_head_bwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, g, dX, dw, db, V, C);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_mse(const float* out, const float* t, const float* mask, float* g, double* loss, int32_t V,
+                       float inv_n, void* stream) {
+    OMG_REQUIRE(out && t && g && loss, "bad arguments");
+    hipLaunchKernelGGL(k_mse, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), out, t, mask, g, loss, V, inv_n);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_adam(float* p, const float* g, float* m, float* v, uint16_t* q, int32_t n, float lr, float b1,
+                        float b2, float eps, int32_t step, void* stream) {
+    OMG_REQUIRE(p && g && m && v && n >= 0 && step >= 1, "bad arguments");
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    if (n) hipLaunchKernelGGL(k_adam, dim3(blocks((size_t)n)), dim3(256), 0, as_stream(stream), p, g, m, v, q, n, lr,
+                              b1, b2, eps, c1, c2);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_to_bf16(const float* src, uint16_t* dst, int64_t n, void* stream) {
+    OMG_REQUIRE(src && dst && n >= 0, "bad arguments");
+    if (n) hipLaunchKernelGGL(k_to_bf16, dim3(blocks((size_t)n)), dim3(256), 0, as_stream(stream), src, dst, (size_t)n);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_embed(const float* img, uint16_t* X, int32_t D, int32_t H, int32_t W, int32_t Nz, float scale,
+                         void* stream) {
+    OMG_REQUIRE(img && X && Nz <= D, "bad arguments");
+    hipLaunchKernelGGL(k_embed, dim3(blocks((size_t)D * H * W * 16)), dim3(256), 0, as_stream(stream), img, X,
+                       Vol{D, H, W}, Nz, scale);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
