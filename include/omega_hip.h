@@ -160,6 +160,53 @@ int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t*
 int omg_graph_launch(omg_graph* h, void* stream);
 int omg_graph_destroy(omg_graph* h);
 
+/* ---- 3-D U-Net warm start (no counterpart in the reference) -------------------------------------------
+ * Kernels of the learned warm start for OSEM-RDP. BF16 tensors are uint16 bit patterns. Activations are
+ * channels-last [D][H][W][C] (C a multiple of 16); conv weights [27][Cout][Cin], tap = (dz*3 + dy)*3 + dx;
+ * 3x3x3 convolutions with zero padding 1 on the matrix cores (BF16 in, FP32 accumulation). */
+
+/* Y = relu?(conv(X) + bias) with X = [X1 | X2] (C1 + C2 input channels; X2 may be null with C2 = 0);
+ * Cout in {16, 32, 48, 64, 96}; bias may be null. Also the data gradient: dX = conv(dY) with the weights
+ * from omg_conv3d_flipT and no bias or ReLU. */
+int omg_conv3d_forward(const uint16_t* X1, int32_t C1, const uint16_t* X2, int32_t C2, const uint16_t* W,
+                       const float* bias, uint16_t* Y, int32_t Cout, int32_t D, int32_t H, int32_t Wd, int32_t relu,
+                       void* stream);
+/* dW[tap][co][ci] += sum_v dY[v][co] * X[v + off(tap)][ci] (FP32 dW). */
+int omg_conv3d_wgrad(const uint16_t* dY, int32_t Cout, const uint16_t* X1, int32_t C1, const uint16_t* X2,
+                     int32_t C2, float* dW, int32_t D, int32_t H, int32_t Wd, void* stream);
+/* Wf[tap][ci][co] = W[26 - tap][co][ci]. */
+int omg_conv3d_flipT(const uint16_t* W, uint16_t* Wf, int32_t Cout, int32_t Cin, void* stream);
+/* db[c] += sum_v dY[v][c], C in {16, 32, 64}. */
+int omg_bias_grad(const uint16_t* dY, float* db, int32_t V, int32_t C, void* stream);
+/* dY = 0 where Y <= 0. */
+int omg_relu_backward(const uint16_t* Y, uint16_t* dY, int64_t n, void* stream);
+/* 2x2x2 average pooling of the fine grid D x H x W and its adjoint ((+)= into dX). */
+int omg_pool_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32_t H, int32_t W, int32_t C, void* stream);
+int omg_pool_backward(const uint16_t* dY, uint16_t* dX, int32_t D, int32_t H, int32_t W, int32_t C,
+                      int32_t accumulate, void* stream);
+/* Nearest x2 upsampling onto the fine grid D x H x W; its adjoint reads channels [off, off + C) of a
+ * fine-grid tensor with Cs channels and writes the coarse grid D x H x W. */
+int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32_t H, int32_t W, int32_t C, void* stream);
+int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off, uint16_t* dX, int32_t D, int32_t H, int32_t W,
+                          int32_t C, void* stream);
+/* dst[v][c] (+)= src[v][off + c] for a source with Cs channels. */
+int omg_take_channels(const uint16_t* src, int32_t Cs, int32_t off, uint16_t* dst, int64_t V, int32_t C,
+                      int32_t accumulate, void* stream);
+/* Output layer out[v] = base[v] + b + sum_c w[c] X[v][c] (FP32) and its backward. */
+int omg_head_forward(const uint16_t* X, const float* w, const float* b, const float* base, float* out, int32_t V,
+                     int32_t C, void* stream);
+int omg_head_backward(const uint16_t* X, const float* w, const float* g, uint16_t* dX, float* dw, float* db, int32_t V,
+                      int32_t C, void* stream);
+/* Masked MSE: loss += sum (mask (out - t))^2 * inv_n (FP64), g = 2 mask (out - t) * inv_n. */
+int omg_mse(const float* out, const float* t, const float* mask, float* g, double* loss, int32_t V, float inv_n,
+            void* stream);
+/* Adam step `step` (1-based) on FP32 parameters; q (optional) receives the BF16 copy. */
+int omg_adam(float* p, const float* g, float* m, float* v, uint16_t* q, int32_t n, float lr, float b1, float b2,
+             float eps, int32_t step, void* stream);
+int omg_to_bf16(const float* src, uint16_t* dst, int64_t n, void* stream);
+/* Network input: img[Nz][H][W] * scale into channel 0 of a 16-channel BF16 volume with D >= Nz planes. */
+int omg_embed(const float* img, uint16_t* X, int32_t D, int32_t H, int32_t W, int32_t Nz, float scale, void* stream);
+
 /* Relative difference prior gradient: grad += beta * dR/du, 6-neighbourhood.
  * boundary 0: zero outside the volume (CPU RDPKernel, source/cpp/RDP.h:4-78);
  * boundary 1: clamp to edge (GPU RDPKernel with images, source/opencl/auxKernels.cl:664-873). */
