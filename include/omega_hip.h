@@ -142,12 +142,15 @@ int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* 
 int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t n_meas, float* S, float epps, void* stream);
 
 /* One OSEM subset step of every frame: yhat_f = A_k x_f; r_f = meas_f / (yhat_f + epps) (NaN/Inf -> epps);
- * x_f = x_f / S * (A_k^T r_f + epps); positivity x < epps -> epps (subiterStep.h:111-121).
+ * x_f = x_f / S * (A_k^T r_f + epps) (subiterStep.h:111-121), or with beta != 0 the OSL-RDP step
+ * x_f = x_f / (S + dU_f) * A_k^T r_f, dU_f = beta * grad RDP(x_f; gamma, epps) with zero boundary
+ * (subiterStep.h:219-231, RDP.h:4-78), dU a work stack [N][n_frames]; positivity x < epps -> epps.
  * Launch shape: rays_per_block in {64,128,256} lanes (one LOR each), frames_per_tile in {1,2,4,8} frames per
  * lane (n_frames must be a multiple), grid_blocks persistent workgroups (0: one resident wave). */
 int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x,
-                         const float* meas, const float* S, float* rhs, float epps, int32_t positivity,
-                         int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks, void* stream);
+                         const float* meas, const float* S, float* rhs, float* dU, float beta, float gamma, float epps,
+                         int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
+                         void* stream);
 
 /* A captured iteration: n_subsets subset steps (subset k: planes/LORs from first[k], n_meas[k] measurements at
  * meas + meas_offset[k] * n_frames, sensitivity S + k * N) recorded once in a hipGraph and replayed by
@@ -155,8 +158,9 @@ int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32
 typedef struct omg_graph omg_graph;
 int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t* first, const int64_t* n_meas,
                             const int64_t* meas_offset, int32_t n_frames, float* x, const float* meas,
-                            const float* S, float* rhs, float epps, int32_t positivity, int32_t rays_per_block,
-                            int32_t frames_per_tile, int32_t grid_blocks, omg_graph** out);
+                            const float* S, float* rhs, float* dU, float beta, float gamma, float epps,
+                            int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
+                            omg_graph** out);
 int omg_graph_launch(omg_graph* h, void* stream);
 int omg_graph_destroy(omg_graph* h);
 

@@ -17,6 +17,7 @@
 // computes it (k_siddon_osem in siddon.hip), so frames agree with single-frame runs up to the order of
 // the float atomics.
 #include "geom.hpp"
+#include "rdp_device.hpp"
 
 #include <vector>
 
@@ -125,14 +126,45 @@ __global__ __launch_bounds__(256) void k_frames_fill(float* __restrict__ p, size
     if (i < n) p[i] = v;
 }
 
-// x[n][f] = x / S[n] * (rhs + epps) (subiterStep.h:111-121, EM algorithms.h:12-15), positivity, rhs = 0.
+// dU[n][f] = beta * grad RDP(x_f)[n], zero outside the volume: the single-frame gradient (omg_rdp_gradient,
+// boundary 0; RDP.h:4-78) on the frame-interleaved stack, lanes summed in the same order.
+__global__ __launch_bounds__(256) void k_frames_rdp(const float* __restrict__ u, float* __restrict__ dU, uint32_t Nx,
+                                                    uint32_t Ny, uint32_t Nz, uint32_t F, float gamma, float epps,
+                                                    float beta) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t Nxy = Nx * Ny;
+    if (i >= (size_t)Nxy * Nz * F) return;
+    const uint32_t n = (uint32_t)(i / F);
+    const uint32_t z = n / Nxy, rem = n - z * Nxy, y = rem / Nx, x = rem - y * Nx;
+    const size_t sF = F;
+    const float uj = u[i];
+    const float xp = (x + 1 < Nx) ? u[i + sF] : 0.f;
+    const float xm = (x > 0) ? u[i - sF] : 0.f;
+    const float yp = (y + 1 < Ny) ? u[i + sF * Nx] : 0.f;
+    const float ym = (y > 0) ? u[i - sF * Nx] : 0.f;
+    const float zp = (z + 1 < Nz) ? u[i + sF * Nxy] : 0.f;
+    const float zm = (z > 0) ? u[i - sF * Nxy] : 0.f;
+    float l0 = rdp_term(uj, xp, gamma, epps);
+    l0 = l0 + rdp_term(uj, yp, gamma, epps);
+    l0 = l0 + rdp_term(uj, zp, gamma, epps);
+    float l1 = rdp_term(uj, xm, gamma, epps);
+    l1 = l1 + rdp_term(uj, ym, gamma, epps);
+    l1 = l1 + rdp_term(uj, zm, gamma, epps);
+    if (isnan(l0)) l0 = 0.f;
+    if (isnan(l1)) l1 = 0.f;
+    dU[i] = 0.f + beta * (l0 + l1);
+}
+
+// OSEM: x[n][f] = x / S[n] * (rhs + epps) (subiterStep.h:111-121, EM algorithms.h:12-15);
+// OSL (dU != null): x = x / (S[n] + dU) * rhs (subiterStep.h:219-231, OSL algorithms.h:24-27);
+// positivity x < epps -> epps; rhs = 0 behind.
 __global__ __launch_bounds__(256) void k_frames_update(uint32_t N, uint32_t F, float* __restrict__ x,
                                                        const float* __restrict__ S, float* __restrict__ rhs,
-                                                       float epps, int pos) {
+                                                       const float* __restrict__ dU, float epps, int pos) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (size_t)N * F) return;
     const float s = S[i / F];
-    float xn = x[i] / s * (rhs[i] + epps);
+    float xn = dU ? x[i] / (s + dU[i]) * rhs[i] : x[i] / s * (rhs[i] + epps);
     if (pos && xn < epps) xn = epps;
     x[i] = xn;
     rhs[i] = 0.f;
@@ -145,6 +177,9 @@ struct FramesStep {
     const float* meas;
     const float* S;
     float* rhs;
+    float* dU;          // OSL-RDP work stack [N][F], or null for OSEM
+    float beta, gamma;
+    uint32_t Nx, Ny, Nz;
     float epps;
     int pos;
     int rpb, ft, grid;
@@ -203,6 +238,12 @@ static hipError_t launch_data_ft(const FramesStep& st, hipStream_t s) {
 }
 
 static int launch_step(const FramesStep& st, hipStream_t s) {
+    const size_t NF = (size_t)st.N * st.F;
+    if (st.dU) {
+        hipLaunchKernelGGL(k_frames_rdp, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, st.x, st.dU, st.Nx, st.Ny,
+                           st.Nz, st.F, st.gamma, st.epps, st.beta);
+        OMG_CHECK_LAUNCH();
+    }
     hipError_t e;
     switch (st.rpb) {
         case 64: e = launch_data_ft<64>(st, s); break;
@@ -213,16 +254,15 @@ static int launch_step(const FramesStep& st, hipStream_t s) {
         set_error("frames OSEM launch failed: %s", hipGetErrorString(e));
         return OMG_ERR_HIP;
     }
-    const size_t NF = (size_t)st.N * st.F;
     hipLaunchKernelGGL(k_frames_update, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, st.N, st.F, st.x, st.S,
-                       st.rhs, st.epps, st.pos);
+                       st.rhs, st.dU, st.epps, st.pos);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
 
 static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x, const float* meas,
-                     const float* S, float* rhs, float epps, int32_t positivity, int32_t rays_per_block,
-                     int32_t frames_per_tile, int32_t grid_blocks, FramesStep& st) {
+                     const float* S, float* rhs, float* dU, float beta, float gamma, float epps, int32_t positivity,
+                     int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks, FramesStep& st) {
     OMG_REQUIRE(g != nullptr, "null geometry");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "batched frames run the improved Siddon projector");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
@@ -235,6 +275,7 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
     OMG_REQUIRE(n_frames % frames_per_tile == 0, "n_frames (%d) must be a multiple of frames_per_tile (%d)", n_frames,
                 frames_per_tile);
     OMG_REQUIRE(grid_blocks >= 0, "grid_blocks must be >= 0 (0 = one wave of resident workgroups)");
+    OMG_REQUIRE(beta == 0.f || dU != nullptr, "OSL-RDP (beta != 0) needs the dU work stack");
     OMG_REQUIRE(((uintptr_t)x % (4 * frames_per_tile)) == 0 && ((uintptr_t)meas % 4) == 0,
                 "image stack must be aligned to frames_per_tile floats");
     const int rc = check_range(g, first, n_meas);
@@ -243,6 +284,9 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
     st.F = (uint32_t)n_frames;
     st.N = g->d.Nx * g->d.Ny * g->d.Nz;
     st.x = x; st.meas = meas; st.S = S; st.rhs = rhs;
+    st.dU = beta != 0.f ? dU : nullptr;
+    st.beta = beta; st.gamma = gamma;
+    st.Nx = g->d.Nx; st.Ny = g->d.Ny; st.Nz = g->d.Nz;
     st.epps = epps; st.pos = positivity != 0;
     st.rpb = rays_per_block; st.ft = frames_per_tile;
     st.grid = grid_blocks > 0 ? grid_blocks : resident_blocks_for(rays_per_block, frames_per_tile);
@@ -280,21 +324,21 @@ extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t 
 }
 
 extern "C" int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x,
-                                    const float* meas, const float* S, float* rhs, float epps, int32_t positivity,
-                                    int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
-                                    void* stream) {
+                                    const float* meas, const float* S, float* rhs, float* dU, float beta, float gamma,
+                                    float epps, int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile,
+                                    int32_t grid_blocks, void* stream) {
     FramesStep st;
-    int rc = make_step(g, first, n_meas, n_frames, x, meas, S, rhs, epps, positivity, rays_per_block, frames_per_tile,
-                       grid_blocks, st);
+    int rc = make_step(g, first, n_meas, n_frames, x, meas, S, rhs, dU, beta, gamma, epps, positivity, rays_per_block,
+                       frames_per_tile, grid_blocks, st);
     if (rc) return rc;
     return launch_step(st, as_stream(stream));
 }
 
 extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t* first,
                                        const int64_t* n_meas, const int64_t* meas_offset, int32_t n_frames, float* x,
-                                       const float* meas, const float* S, float* rhs, float epps, int32_t positivity,
-                                       int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
-                                       omg_graph** out) {
+                                       const float* meas, const float* S, float* rhs, float* dU, float beta,
+                                       float gamma, float epps, int32_t positivity, int32_t rays_per_block,
+                                       int32_t frames_per_tile, int32_t grid_blocks, omg_graph** out) {
     OMG_REQUIRE(out != nullptr, "null output handle");
     *out = nullptr;
     OMG_REQUIRE(n_subsets >= 1 && first && n_meas && meas_offset, "subset tables are required");
@@ -304,8 +348,8 @@ extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, con
     for (int32_t k = 0; k < n_subsets; k++) {
         OMG_REQUIRE(meas_offset[k] >= 0, "negative measurement offset");
         const int rc = make_step(g, first[k], n_meas[k], n_frames, x, meas + (size_t)meas_offset[k] * n_frames,
-                                 S + (size_t)k * N, rhs, epps, positivity, rays_per_block, frames_per_tile, grid_blocks,
-                                 steps[(size_t)k]);
+                                 S + (size_t)k * N, rhs, dU, beta, gamma, epps, positivity, rays_per_block,
+                                 frames_per_tile, grid_blocks, steps[(size_t)k]);
         if (rc) return rc;
     }
     hipStream_t cs;

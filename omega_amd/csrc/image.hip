@@ -4,6 +4,7 @@
 // (x fastest, matching OMEGA's column-major image), 256 lanes per workgroup,
 // float math written in the reference's operation order.
 #include "common.hpp"
+#include "rdp_device.hpp"
 
 namespace omg {
 
@@ -12,12 +13,6 @@ constexpr int kEB = 256;
 // RDP gradient (source/cpp/RDP.h:4-78; source/opencl/auxKernels.cl:802-871).
 // Neighbour pairs are evaluated as the reference's float2 lanes: lane 0 = (+x, +y, +z),
 // lane 1 = (-x, -y, -z), each summed x, y, z; NaN lanes -> 0; grad += beta * (lane0 + lane1).
-__device__ __forceinline__ float rdp_term(float uj, float uk, float gamma, float epps) {
-    const float du = uj - uk;
-    const float den = uj + uk + gamma * fabsf(du);
-    return du * (gamma * fabsf(du) + uj + 3.f * uk + epps * epps) / (den * den + epps);
-}
-
 __global__ __launch_bounds__(kEB) void k_rdp(const float* __restrict__ u, float* __restrict__ grad, uint32_t Nx,
                                              uint32_t Ny, uint32_t Nz, float gamma, float epps, float beta, int clamp) {
     const uint32_t Nxy = Nx * Ny;

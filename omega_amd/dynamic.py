@@ -7,8 +7,9 @@ geometry handle and advance together:
 
 * ``x`` is a frame-interleaved image stack ``[N, F]`` and ``meas`` a frame-interleaved measurement
   stack ``[M, F]`` in the projector's subset order (the order of ``projectorClass.nTotMeas``);
-* one ``omg_frames_osem_step`` launch performs a subset step of every frame; a persistent grid walks
-  (frame tile, LOR block) work items, so rays-per-block and frames-per-tile are launch parameters;
+* one ``omg_frames_osem_step`` launch performs a subset step of every frame (OSEM, or OSL-OSEM with the
+  RDP prior when ``beta != 0``); a persistent grid walks (frame tile, LOR block) work items, so
+  rays-per-block and frames-per-tile are launch parameters;
 * an iteration (all subset steps) is captured once with ``omg_frames_graph_create`` and replayed;
 * :class:`FramesTuner` times every (rays-per-block, frames-per-tile) pair on the real data and keeps
   the fastest, per geometry and frame count.
@@ -34,7 +35,8 @@ def _p(t):
 class FramesOSEM:
     """OSEM of ``F`` frames that share the geometry of ``proj`` (improved Siddon, emission data)."""
 
-    def __init__(self, proj, meas, epps: float = 1e-5, enforce_positivity: bool = True, x0=None):
+    def __init__(self, proj, meas, epps: float = 1e-5, enforce_positivity: bool = True, x0=None, beta: float = 0.0,
+                 gamma: float = 2.0):
         import torch
         self.torch = torch
         if not proj.projectorInitialized:
@@ -53,6 +55,8 @@ class FramesOSEM:
             x0.reshape(self.N, self.F).to(torch.float32).contiguous().clone()
         self.S = torch.zeros((proj.subsets, self.N), dtype=torch.float32, device=dev)
         self.rhs = torch.zeros((self.N, self.F), dtype=torch.float32, device=dev)
+        self.beta, self.gamma = float(beta), float(gamma)
+        self.dU = torch.zeros((self.N, self.F), dtype=torch.float32, device=dev) if self.beta != 0.0 else None
         self.epps = float(epps)
         self.pos = int(bool(enforce_positivity))
         self.sens_done = False
@@ -80,8 +84,8 @@ class FramesOSEM:
         first, n, off = self._subset(k)
         x = self.x if x is None else x
         _lib.call("omg_frames_osem_step", self.g, first, n, self.F, _p(x), _p(self.meas[off:off + n]), _p(self.S[k]),
-                  _p(self.rhs), self.epps, self.pos, int(rays_per_block), int(frames_per_tile), int(grid_blocks),
-                  self._stream())
+                  _p(self.rhs), _p(self.dU), self.beta, self.gamma, self.epps, self.pos, int(rays_per_block),
+                  int(frames_per_tile), int(grid_blocks), self._stream())
 
     def iteration(self, rays_per_block: int = 256, frames_per_tile: int = 8) -> None:
         for k in range(self.proj.subsets):
@@ -102,8 +106,8 @@ class FramesOSEM:
         off = np.array([int(self.proj.nTotMeas[k]) for k in range(K)], dtype=np.int64)
         h = C.c_void_p()
         _lib.call("omg_frames_graph_create", self.g, K, first.ctypes.data, n.ctypes.data, off.ctypes.data, self.F,
-                  _p(self.x), _p(self.meas), _p(self.S), _p(self.rhs), self.epps, self.pos, key[0], key[1], 0,
-                  C.byref(h))
+                  _p(self.x), _p(self.meas), _p(self.S), _p(self.rhs), _p(self.dU), self.beta, self.gamma, self.epps,
+                  self.pos, key[0], key[1], 0, C.byref(h))
         self._graphs[key] = h
         return h
 

@@ -124,3 +124,23 @@ def test_rejects_bad_launch_shapes(study):
     with pytest.raises(_lib.OmegaHipError):
         run.step(0, 256, 3)
     run.close()
+
+
+def test_osl_rdp_frames_match_single_frame(study, cuda):
+    import torch
+    p, y = study
+    N = p.grid.N
+    beta, gamma = 0.1, 2.0
+    run = FramesOSEM(p, y, beta=beta, gamma=gamma)
+    run.iteration(128, 4)
+    got = run.x.cpu().numpy()
+    run.close()
+    for f in (0, 5):
+        x = torch.ones(N, dtype=torch.float32, device=cuda)
+        S = torch.zeros((p.subsets, N), dtype=torch.float32, device=cuda)
+        yf = y[:, f].contiguous()
+        for k in range(p.subsets):
+            first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
+            _lib.call("omg_osem_subset_iteration", p.geometry.handle, first, n, ptr(x), ptr(yf[off:off + n]),
+                      ptr(S[k]), 1, 1e-5, beta, gamma, 1, None)
+        assert rel_l2(got[:, f], x.cpu().numpy()) <= 1e-5, f"frame {f}"
