@@ -141,6 +141,12 @@ int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* 
  * functions.hpp:2981-2996). */
 int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t n_meas, float* S, float epps, void* stream);
 
+/* yhat[i][f] = (A_k x_f)_i for the subset's measurements (Siddon forward projection of every frame). */
+int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, const float* x, float* yhat,
+                       void* stream);
+/* out[f] = sum_i y[i][f] log(yhat[i][f]) - yhat[i][f] (terms with y = 0: -yhat), FP64, for n rows. */
+int omg_frames_loglik(const float* y, const float* yhat, int64_t n, int32_t n_frames, double* out, void* stream);
+
 /* One OSEM subset step of every frame: yhat_f = A_k x_f; r_f = meas_f / (yhat_f + epps) (NaN/Inf -> epps);
  * x_f = x_f / S * (A_k^T r_f + epps) (subiterStep.h:111-121), or with beta != 0 the OSL-RDP step
  * x_f = x_f / (S + dU_f) * A_k^T r_f, dU_f = beta * grad RDP(x_f; gamma, epps) with zero boundary

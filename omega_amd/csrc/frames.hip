@@ -19,6 +19,7 @@
 #include "geom.hpp"
 #include "rdp_device.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace omg {
@@ -106,6 +107,32 @@ __global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t F, cons
             for (int f = 0; f < FT; f++) unsafeAtomicAdd(p + f, r[f] * lw);
         });
     }
+}
+
+// yhat[lo][f] = A_k x_f for every frame of a tile (0 for LORs that miss the image).
+template <int FT>
+__global__ __launch_bounds__(256) void k_frames_fp(SidArgs a, uint32_t F, const float* __restrict__ x,
+                                                   float* __restrict__ yhat) {
+    const int64_t lo = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lo >= a.nmeas) return;
+    const uint32_t f0 = blockIdx.y * FT;
+    float ax[FT], w = 0.f;
+    const bool hit = frames_gather<FT>(a, lo, x + f0, F, ax, w);
+#pragma unroll
+    for (int f = 0; f < FT; f++) yhat[(size_t)lo * F + f0 + f] = hit ? ax[f] * w : 0.f;
+}
+
+// out[f] += sum_i y log(yhat) - yhat over [n][F] stacks, terms with y == 0 contribute -yhat (FP64 sums).
+__global__ __launch_bounds__(256) void k_frames_ll(const float* __restrict__ y, const float* __restrict__ yhat,
+                                                   int64_t n, uint32_t F, double* __restrict__ out) {
+    const uint32_t f = blockIdx.y;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double yi = y[(size_t)i * F + f], hi = yhat[(size_t)i * F + f];
+        s += (yi != 0.0 ? yi * log(hi) : 0.0) - hi;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out + f, s);
 }
 
 // S += A_k^T 1 (frame-independent sensitivity of one subset).
@@ -319,6 +346,42 @@ extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t 
         OMG_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(k_frames_clamp, dim3(grid_for(N, 256)), dim3(256), 0, s, S, N, epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, const float* x,
+                                  float* yhat, void* stream) {
+    OMG_REQUIRE(g != nullptr && x && yhat && n_frames >= 1, "bad arguments");
+    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
+                "batched frames run the improved Siddon projector on emission data");
+    OMG_REQUIRE(((uintptr_t)x % 16) == 0, "image stack must be 16-byte aligned");
+    const int rc = check_range(g, first, n_meas);
+    if (rc) return rc;
+    if (n_meas == 0) return OMG_OK;
+    const SidArgs a = make_sid_args(g, first, n_meas);
+    hipStream_t s = as_stream(stream);
+    const unsigned gx = grid_for(n_meas, 256);
+    if (n_frames % 4 == 0)
+        hipLaunchKernelGGL(k_frames_fp<4>, dim3(gx, (unsigned)(n_frames / 4)), dim3(256), 0, s, a, (uint32_t)n_frames,
+                           x, yhat);
+    else
+        hipLaunchKernelGGL(k_frames_fp<1>, dim3(gx, (unsigned)n_frames), dim3(256), 0, s, a, (uint32_t)n_frames, x,
+                           yhat);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+extern "C" int omg_frames_loglik(const float* y, const float* yhat, int64_t n, int32_t n_frames, double* out,
+                                 void* stream) {
+    OMG_REQUIRE(y && yhat && out && n >= 0 && n_frames >= 1, "bad arguments");
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_frames_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<float*>(out), (size_t)(2 * n_frames),
+                       0.f);
+    OMG_CHECK_LAUNCH();
+    if (n == 0) return OMG_OK;
+    const unsigned gx = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_frames_ll, dim3(gx, (unsigned)n_frames), dim3(256), 0, s, y, yhat, n, (uint32_t)n_frames, out);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
