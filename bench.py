@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_traffic.txt"),
                     help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
+    ap.add_argument("--workload", default="pet2d_osem",
+                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth"],
+                    help="pet2d_osem (default, the headline metric); the others print their own line")
+    ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     return ap.parse_args()
 
 
@@ -51,6 +55,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "pet2d_osem":
+        {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth}[args.workload](args)
+        return
 
     import ctypes as C
 
@@ -147,6 +155,224 @@ def main():
     if cpu and cpu.get("value"):
         line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+
+
+def _common(args, workload_line):
+    line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32"}
+    line.update(workload_line)
+    print(json.dumps(line), flush=True)
+
+
+def _trace_counts(p, k, dev):
+    """Exact (LOR, voxel) visit counts of subset k (omg_siddon_trace)."""
+    import ctypes as C
+    import torch
+    from omega_amd import _lib
+    n = int(p.nMeasSubset[k, 0])
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    _lib.call("omg_siddon_trace", p.geometry.handle, int(p.nMeas[k]), n, C.c_void_p(counts.data_ptr()), None, None,
+              None, None)
+    return int(counts.to(torch.int64).sum().item())
+
+
+def bench_frames(args):
+    """Dynamic 3-D PET: F frames on the 3-D ring geometry, 16 subsets x 3 iterations, all frames per launch,
+    hipGraph-replayed iterations, launch shape from the autotuner. value = frame-subset-iterations/s."""
+    import numpy as np
+    import torch
+    from omega_amd import _lib, phantoms, workloads
+    from omega_amd.dynamic import FramesOSEM, FramesTuner
+    dev = torch.device("cuda:0")
+    _lib.load()
+    p = workloads.pet3d_projector(subsets=16)
+    p.initProj()
+    F = args.frames
+    durations = (phantoms.FRAME_DURATIONS * ((F + 23) // 24))[:F]
+    frames = phantoms.dynamic_ellipsoids(p.grid, seed=0, durations=durations)
+    counts = workloads.frame_counts(F, seed=0)
+    t0 = time.perf_counter()
+    y = workloads.dynamic_data(p, frames, counts, seed=0, device=dev)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    run = FramesOSEM(p, y)
+    tuner = FramesTuner()
+    t0 = time.perf_counter()
+    rpb, ft = tuner.tune(run, k=0, reps=2)
+    t_tune = time.perf_counter() - t0
+    n_iter = 3
+    for _ in range(args.warmup):
+        run.x.fill_(1.0)
+        run.run(1, rpb, ft)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    for _ in range(args.steps):
+        run.x.fill_(1.0)
+        run.run(n_iter, rpb, ft)
+    ev[1].record()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fsi = F * p.subsets * n_iter * args.steps / dt
+    # one subset step timed alone (k_frames_osem + update) for the roofline of the dominant kernel
+    xs = run.x.clone()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    run.step(0, rpb, ft, x=xs)
+    b.record()
+    b.synchronize()
+    step_s = a.elapsed_time(b) * 1e-3
+    np0 = _trace_counts(p, 0, dev)
+    M0 = int(p.nMeasSubset[0, 0])
+    bytes_step = (12.0 * np0 + 8.0 * M0) * F
+    # the same work as F serial single-frame fused iterations (the reference's frame loop), one subset
+    from omega_amd.recon import OSEMRunner
+    t_ser = 0.0
+    for f in range(min(F, 4)):
+        r1 = OSEMRunner(p, y[:, f].contiguous())
+        x1 = torch.ones(p.grid.N, device=dev)
+        r1.subset_step(x1, 0)
+        r1.sens_done = True
+        torch.cuda.synchronize()
+        a.record()
+        r1.subset_step(x1, 0)
+        b.record()
+        b.synchronize()
+        t_ser += a.elapsed_time(b) * 1e-3
+    serial_step_s = t_ser / min(F, 4) * F
+    _common(args, {
+        "metric": "dynamic PET OSEM frame-subset-iterations/s (all frames per launch, hipGraph, autotuned)",
+        "value": round(fsi, 2), "unit": "frame-subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "data": "synthetic (ellipsoid phantom with one-tissue-compartment curves, lognormal frame counts)",
+        "config": {"workload": "pet3d_frames_osem_siddon", "frames": F, "image": [128, 128, 63],
+                   "sinogram": [224, 224, 1024], "rings": 32, "crystals": 448, "subsets": 16, "iterations": n_iter,
+                   "rays_per_block": rpb, "frames_per_tile": ft, "tuner_ms": tuner.last_timings and
+                   {f"{k[0]}x{k[1]}": round(v, 3) for k, v in tuner.last_timings.items()},
+                   "data_gen_s": round(t_data, 1), "tune_s": round(t_tune, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_step / step_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(bytes_step / step_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_frames_osem<{ft},{rpb}> + k_frames_update", "avg_launch_us": round(step_s * 1e6, 1),
+                     "algorithmic_bytes_per_launch": int(bytes_step)},
+        "serial_frames_subset_step_ms": round(serial_step_s * 1e3, 2),
+        "batched_subset_step_ms": round(step_s * 1e3, 2),
+        "speedup_vs_serial_frames": round(serial_step_s / step_s, 2),
+        "cpu_baseline": None,
+    })
+
+
+def bench_cgls(args):
+    """CBCT CGLS with the branchless distance-driven projector: 256^3 volume, 360 views of 512 x 384.
+    value = CGLS iterations/s (10 iterations per step)."""
+    import torch
+    from omega_amd import _lib, workloads
+    from omega_amd.recon import CGLSRunner
+    dev = torch.device("cuda:0")
+    _lib.load()
+    p = workloads.ct_cgls_projector()
+    p.initProj()
+    mu, y = workloads.ct_cgls_data(p, seed=0)
+    runner = CGLSRunner(p, y)
+    n_iter = 10
+    for _ in range(args.warmup):
+        runner.run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f = runner.run(n_iter)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    import numpy as np
+    rel = float(np.linalg.norm(f.cpu().numpy() - mu) / np.linalg.norm(mu))
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    x = torch.rand(p.grid.N, device=dev)
+    a.record()
+    yy = p.forwardProject(x, 0)
+    b.record()
+    b.synchronize()
+    fp_s = a.elapsed_time(b) * 1e-3
+    a.record()
+    p.backwardProject(yy, 0)
+    b.record()
+    b.synchronize()
+    bp_s = a.elapsed_time(b) * 1e-3
+    n = 256
+    sat_bytes = 4.0 * ((n + 1) * (n + 1) * n) * 2
+    fp_bytes = sat_bytes + 4.0 * yy.numel() + 4.0 * p.grid.N
+    _common(args, {
+        "metric": "CBCT CGLS iterations/s (branchless distance-driven FP/BP)", "value": round(n_iter * args.steps / dt, 3),
+        "unit": "CGLS-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "data": "synthetic (12 seeded attenuation ellipsoids, line integrals + 1% Gaussian noise)",
+        "config": {"workload": "ct_cgls_bdd", "volume": [256, 256, 256], "voxel_mm": 0.5, "views": 360,
+                   "detector": [512, 384], "pixel_mm": 0.8, "sod_mm": 700.0, "sdd_mm": 1000.0, "iterations": n_iter,
+                   "image_rel_l2_vs_phantom": round(rel, 4), "fp_ms": round(fp_s * 1e3, 2),
+                   "bp_ms": round(bp_s * 1e3, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(fp_bytes / fp_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(fp_bytes / fp_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "forward projection (k_sat_* + k_bdd_fp)", "avg_launch_us": round(fp_s * 1e6, 1),
+                     "algorithmic_bytes_per_launch": int(fp_bytes)},
+        "cpu_baseline": None,
+    })
+
+
+def bench_orth(args):
+    """2-D plane PET OSEM with the orthogonal distance-based projector (tube width 2.5 mm, transaxial
+    and axial sweeps) on the pet2d_osem geometry. value = subset-iterations/s (8 subsets x 1 iteration per step)."""
+    import numpy as np
+    import torch
+    from omega_amd import _lib, geometry as G, workloads
+    from omega_amd.projector import projectorClass
+    from omega_amd.recon import OSEMRunner
+    dev = torch.device("cuda:0")
+    _lib.load()
+    grid = G.Grid.from_fov(128, 128, 63, 320.0, 320.0, 157.5)
+    sc = G.pet_2d_planes(grid, radius=400.0, n_radial=128, n_angles=160, radial_pitch=2.5)
+    p = projectorClass()
+    p.Nx, p.Ny, p.Nz = 128, 128, 63
+    p.FOVa_x, p.FOVa_y, p.axial_fov = 320.0, 320.0, 157.5
+    p.projector_type = 2
+    p.tube_width_xy = p.tube_width_z = 2.5
+    p.x, p.z = sc.xy_table(), sc.z
+    p.Ndist, p.Nang = sc.n_radial, sc.n_angles
+    p.subsets, p.subsetType = 8, 8
+    p.addProjector()
+    p.initProj()
+    ph, y = workloads.pet2d_data(p, seed=0)
+    runner = OSEMRunner(p, y)
+    x = torch.ones(p.grid.N, device=dev)
+    for _ in range(args.warmup):
+        for k in range(p.subsets):
+            runner.subset_step(x, k)
+        runner.sens_done = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for k in range(p.subsets):
+            runner.subset_step(x, k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ntube = _trace_counts(p, 0, dev)
+    M0 = int(p.nMeasSubset[0, 0])
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    runner.subset_step(x, 0)
+    b.record()
+    b.synchronize()
+    s_s = a.elapsed_time(b) * 1e-3
+    byt = 12.0 * ntube + 8.0 * M0
+    _common(args, {
+        "metric": "OSEM subset-iters/s, orthogonal distance-based projector", "value": round(p.subsets * args.steps / dt, 2),
+        "unit": "subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "data": "synthetic (seeded ellipsoid phantom, Poisson data at 5e6 counts)",
+        "config": {"workload": "pet2d_osem_orth", "image": [128, 128, 63], "sinogram": [128, 160, 63],
+                   "tube_width_mm": 2.5, "sweeps": "transaxial+axial", "subsets": 8,
+                   "tube_visits_subset0": ntube},
+        "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_orth_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
+                     "algorithmic_bytes_per_launch": int(byt)},
+        "cpu_baseline": None,
+    })
 
 
 def read_traffic(path):

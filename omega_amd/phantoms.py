@@ -60,3 +60,13 @@ def dynamic_ellipsoids(grid, seed=0, n_ell=8, background=0.1, durations=None):
         tac = one_tissue_tac(mid, rng.uniform(0.05, 0.5), rng.uniform(0.02, 0.3))
         out[inside] += act * tac / tac.max()
     return np.ascontiguousarray(out.astype(np.float32))
+
+
+def attenuation_ellipsoids(grid, seed=0, n_ell=12, lo=0.01, hi=0.03):
+    """Attenuation phantom: seeded ellipsoids (the geometry of ``ellipsoid_masks``) with
+    mu ~ U(lo, hi) per mm, later ellipsoids overwriting earlier ones, zero elsewhere."""
+    rng = np.random.default_rng(seed + 15485863)
+    img = np.zeros(grid.N, dtype=np.float32)
+    for inside, _ in ellipsoid_masks(grid, seed, n_ell):
+        img[inside] = rng.uniform(lo, hi)
+    return img

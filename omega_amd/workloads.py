@@ -73,6 +73,35 @@ def pet3d_projector(subsets: int = 1, n_rings: int = 32, crystals: int = 448, ri
     return p
 
 
+def ct_cgls_projector(n=256, voxel=0.5, n_views=360, rows=512, cols=384, pixel=0.8, sod=700.0, sdd=1000.0):
+    """CBCT with the branchless distance-driven projector: n^3 volume of `voxel` mm, `n_views` projections
+    over 360 degrees on a rows x cols flat panel of `pixel` mm, source-origin `sod`, source-detector `sdd`."""
+    x, uv = G.cbct_tables(n_views, sod=sod, sdd=sdd, pitch_u=pixel)
+    p = projectorClass()
+    p.CT = True
+    p.projector_type = 5
+    p.Nx = p.Ny = p.Nz = n
+    p.FOVa_x = p.FOVa_y = p.axial_fov = n * voxel
+    p.x, p.z = x, uv
+    p.nRowsD, p.nColsD = rows, cols
+    p.dPitchX = p.dPitchY = pixel
+    p.subsets = 1
+    p.addProjector()
+    return p
+
+
+def ct_cgls_data(p, seed=0, noise=0.01, device="cuda"):
+    """Attenuation phantom (12 seeded ellipsoids, mu ~ U(0.01, 0.03)/mm) and its line integrals A mu plus
+    Gaussian noise of sigma = noise * max."""
+    import torch
+    mu = phantoms.attenuation_ellipsoids(p.grid, seed=seed, n_ell=12)
+    y = p.forwardProject(torch.as_tensor(mu, device=device), 0)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 3)
+    y = y + noise * float(y.max()) * torch.randn(y.shape, device=device, generator=g)
+    return mu, y.contiguous()
+
+
 def frame_counts(n_frames: int, seed: int, lo: float = 1e5, hi: float = 1e7):
     """Per-frame total counts, lognormal (log10 ~ N(6, 0.6)) clipped to [lo, hi]."""
     rng = np.random.default_rng(seed + 104729)
