@@ -119,7 +119,8 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
              ft=8, log=None):
     """Plain vs warm-started OSEM-RDP on held-out seeds (batched). Returns a dict of per-seed results."""
     import torch
-    res = {"seed": [], "ll_plain": [], "ll_warm_start": [], "ll_input": [], "iters_warm": [], "reached": []}
+    res = {"seed": [], "ll_plain": [], "ll_warm_start": [], "ll_input": [], "iters_warm": [], "iters_no_net": [],
+           "reached": []}
     t_plain = t_short = t_net = t_cont = 0.0
     for b0 in range(0, len(seeds), 8):
         sb = list(seeds[b0:b0 + 8])
@@ -169,12 +170,24 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
             iters[newly] = it
             if np.all(iters > 0):
                 break
+        # the same continuation without the network (from the short-OSEM image): what the network adds
+        nonet = FramesOSEM(p, y, beta=beta, gamma=gamma, x0=short.x)
+        nonet.sensitivity()
+        iters_nn = np.full(F, -1)
+        for it in range(1, n_long + 1):
+            nonet.run(1, rpb, tile)
+            ll = loglik(p, y, nonet.x)
+            iters_nn[(iters_nn < 0) & (ll >= target)] = it
+            if np.all(iters_nn > 0):
+                break
+        nonet.close()
         for j in range(F):
             res["seed"].append(int(sb[j]))
             res["ll_plain"].append(float(ll_plain[j]))
             res["ll_warm_start"].append(float(ll_ws[j]))
             res["ll_input"].append(float(ll_in[j]))
             res["iters_warm"].append(int(iters[j]))
+            res["iters_no_net"].append(int(iters_nn[j]))
             res["reached"].append(bool(iters[j] > 0))
         for r in (plain, short, cont):
             r.close()
@@ -227,6 +240,7 @@ def main(argv=None):
     reached = int(np.sum(res["reached"]))
     out = {"pairs": a.pairs, "holdout": a.holdout, "epochs": a.epochs, "loss": hist,
            "reached": reached, "mean_iters_warm": float(np.mean([i for i in res["iters_warm"] if i > 0] or [0])),
+           "mean_iters_no_net": float(np.mean([i for i in res["iters_no_net"] if i > 0] or [0])),
            "ms_plain_total": res["ms_plain_total"], "ms_warm_total": res["ms_warm_total"],
            "speedup": res["ms_plain_total"] / max(res["ms_warm_total"], 1e-9),
            "ll_gain_input_to_net": float(np.mean(np.array(res["ll_warm_start"]) - np.array(res["ll_input"])))}

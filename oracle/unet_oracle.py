@@ -88,17 +88,27 @@ LAYERS = [
 ]
 
 
-def unet_forward(x0, P):
+def _ident(v):
+    return v
+
+
+def bf16_store(v):
+    """What storing a float64 result as BF16 does (through float32, as the kernels round)."""
+    return bf16_round(np.asarray(v, np.float32)).astype(np.float64)
+
+
+def unet_forward(x0, P, rnd=_ident):
     """x0: [D, H, W, 16] embedded input; P: parameters (float arrays). Returns (head output before the
-    residual, activations)."""
+    residual, activations). ``rnd`` is applied wherever the library stores an activation in BF16
+    (``bf16_store`` emulates that; the default keeps float64 throughout)."""
     A = {"x0": np.asarray(x0, np.float64)}
-    r = lambda v: np.maximum(v, 0.0)
+    r = lambda v: rnd(np.maximum(v, 0.0))
     A["a1"] = r(conv3d(A["x0"], P["e1a.w"], P["e1a.b"]))
     A["s1"] = r(conv3d(A["a1"], P["e1b.w"], P["e1b.b"]))
-    A["p1"] = pool(A["s1"])
+    A["p1"] = rnd(pool(A["s1"]))
     A["a2"] = r(conv3d(A["p1"], P["e2a.w"], P["e2a.b"]))
     A["s2"] = r(conv3d(A["a2"], P["e2b.w"], P["e2b.b"]))
-    A["p2"] = pool(A["s2"])
+    A["p2"] = rnd(pool(A["s2"]))
     A["a3"] = r(conv3d(A["p2"], P["bta.w"], P["bta.b"]))
     A["b3"] = r(conv3d(A["a3"], P["btb.w"], P["btb.b"]))
     A["u2"] = up(A["b3"])
@@ -111,31 +121,32 @@ def unet_forward(x0, P):
     return head, A
 
 
-def unet_backward(gout, P, A):
-    """Parameter gradients for d(loss)/d(head) = gout [D, H, W]."""
+def unet_backward(gout, P, A, rnd=_ident):
+    """Parameter gradients for d(loss)/d(head) = gout [D, H, W]; ``rnd`` as in :func:`unet_forward`, at the
+    points where the library stores a gradient in BF16."""
     G = {}
     g = np.asarray(gout, np.float64)
     G["head.w"] = np.einsum("zyxc,zyx->c", A["d1"], g)
     G["head.b"] = np.array([g.sum()])
-    g_d1 = g[..., None] * np.asarray(P["head.w"], np.float64) * (A["d1"] > 0)
+    g_d1 = rnd(g[..., None] * np.asarray(P["head.w"], np.float64)) * (A["d1"] > 0)
 
     def layer(name, gy, xin):
         G[name + ".w"] = conv3d_dw(gy, xin)
         G[name + ".b"] = gy.reshape(-1, gy.shape[-1]).sum(0)
-        return conv3d_dx(gy, P[name + ".w"])
+        return rnd(conv3d_dx(gy, P[name + ".w"]))
 
     g_a5 = layer("d1b", g_d1, A["a5"]) * (A["a5"] > 0)
     g_cat1 = layer("d1a", g_a5, np.concatenate([A["u1"], A["s1"]], -1))
-    g_d2 = up_bwd(g_cat1[..., :32]) * (A["d2"] > 0)
+    g_d2 = rnd(up_bwd(g_cat1[..., :32])) * (A["d2"] > 0)
     g_a4 = layer("d2b", g_d2, A["a4"]) * (A["a4"] > 0)
     g_cat2 = layer("d2a", g_a4, np.concatenate([A["u2"], A["s2"]], -1))
-    g_b3 = up_bwd(g_cat2[..., :64]) * (A["b3"] > 0)
+    g_b3 = rnd(up_bwd(g_cat2[..., :64])) * (A["b3"] > 0)
     g_a3 = layer("btb", g_b3, A["a3"]) * (A["a3"] > 0)
     g_p2 = layer("bta", g_a3, A["p2"])
-    g_s2 = (pool_bwd(g_p2) + g_cat2[..., 64:]) * (A["s2"] > 0)
+    g_s2 = rnd(rnd(pool_bwd(g_p2)) + g_cat2[..., 64:]) * (A["s2"] > 0)
     g_a2 = layer("e2b", g_s2, A["a2"]) * (A["a2"] > 0)
     g_p1 = layer("e2a", g_a2, A["p1"])
-    g_s1 = (pool_bwd(g_p1) + g_cat1[..., 32:]) * (A["s1"] > 0)
+    g_s1 = rnd(rnd(pool_bwd(g_p1)) + g_cat1[..., 32:]) * (A["s1"] > 0)
     g_a1 = layer("e1b", g_s1, A["a1"]) * (A["a1"] > 0)
     layer("e1a", g_a1, A["x0"])
     return G

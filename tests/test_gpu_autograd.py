@@ -27,7 +27,9 @@ def test_backward_is_backprojection(pet3d_small):
     y = AG.forward_project(x, p)
     v = torch.rand(y.numel(), device="cuda", generator=g)
     (y * v).sum().backward()
-    assert torch.equal(x.grad, p.backwardProject(v, 0))
+    ref = p.backwardProject(v, 0)
+    # float atomics: equal up to the order of the additions
+    assert float(torch.linalg.vector_norm(x.grad - ref) / torch.linalg.vector_norm(ref)) <= 1e-6
 
 
 def test_adjoint_identity_20_pairs(pet3d_small):
@@ -60,7 +62,8 @@ def test_rdp_value_matches_float64(cuda):
         for idx in (0, n - 1):
             a = np.take(w, idx, axis=ax)
             ghost += np.sum(a * a / (a + 2.0 * a))
-    expect = 0.3 * ((full - ghost) / 2.0 + ghost)
+    # beta crosses the C ABI as float32
+    expect = float(np.float32(0.3)) * ((full - ghost) / 2.0 + ghost)
     assert abs(float(v) - expect) < 1e-10 * abs(expect)
 
 

@@ -2,8 +2,9 @@
 
 Bar: a convolution's BF16 output within one BF16 rounding (2^-8 relative, plus 1e-6 absolute) of the
 oracle on BF16 inputs; dX likewise; dW (FP32 output) within 1e-4 relative to its largest entry; the
-whole network's output and every parameter gradient within 5e-2 relative L2 of the float64 oracle
-(BF16 activations between layers); Adam training lowers the loss.
+whole network's output and every parameter gradient within 1e-2 relative L2 of the oracle run with
+BF16 storage emulated at the same points (float64 arithmetic in between), and within 1e-1 of the
+pure float64 network; Adam training lowers the loss.
 """
 import ctypes as C
 
@@ -95,18 +96,19 @@ def test_network_forward_and_gradients(cuda):
     P["head.b"] = net.p["head.b"].cpu().numpy().astype(np.float64)
     D = net.D[0]
     x0 = net.act["x0"].float().cpu().numpy().reshape(D, ny, nx, 16)
-    head, A = U.unet_forward(x0, P)
     base = np.zeros(D * ny * nx); base[: nx * ny * nz] = img.cpu().numpy() * scale
-    ref_out = head.reshape(-1) + base
-    assert rel_l2(out, ref_out) <= 5e-2
     tgt = np.zeros(D * ny * nx); tgt[: nx * ny * nz] = target.cpu().numpy() * scale
     m = mask.cpu().numpy()
-    gout = 2.0 * (ref_out - tgt) * m / m.sum()
-    G = U.unet_backward(gout.reshape(D, ny, nx), P, A)
-    for k in G:
-        err = rel_l2(net.g[k].cpu().numpy(), G[k])
-        print(f"  grad {k}: rel L2 {err:.2e}", flush=True)
-        assert err <= 5e-2, k
+    for rnd, tol in ((U.bf16_store, 1e-2), (U._ident, 1e-1)):
+        head, A = U.unet_forward(x0, P, rnd)
+        ref_out = head.reshape(-1) + base
+        assert rel_l2(out, ref_out) <= tol / 10
+        gout = 2.0 * (ref_out - tgt) * m / m.sum()
+        G = U.unet_backward(gout.reshape(D, ny, nx), P, A, rnd)
+        for k in G:
+            err = rel_l2(net.g[k].cpu().numpy(), G[k])
+            print(f"  grad {k} ({'bf16 storage' if rnd is U.bf16_store else 'float64'}): rel L2 {err:.2e}", flush=True)
+            assert err <= tol, k
 
 
 def test_training_lowers_loss(cuda):

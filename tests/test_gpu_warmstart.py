@@ -1,9 +1,10 @@
 """Learned warm start, end to end on a reduced 3-D PET scanner: pairs made by the method itself,
 U-Net training on the library's kernels, held-out evaluation against the plain OSEM-RDP run.
 
-Bar: the training loss falls; on held-out seeds the network's image has a higher Poisson
-log-likelihood than the short-OSEM image it starts from for most seeds (what the network adds);
-the batched log-likelihood matches the single-frame forward projection + omg_poisson_loglik.
+Bar: the training loss falls; on held-out seeds, OSEM-RDP continued from the network's image reaches
+the plain run's log-likelihood in fewer iterations on average than the same continuation from the
+short-OSEM image without the network (what the network adds); the batched log-likelihood matches the
+single-frame forward projection + omg_poisson_loglik.
 """
 import numpy as np
 import pytest
@@ -39,15 +40,20 @@ def test_batched_loglik_matches_single(scanner, cuda):
 
 def test_pairs_training_and_gain(scanner, cuda):
     p = scanner
-    seeds = list(range(100, 124))
+    seeds = list(range(100, 148))
     xin, xt = WS.make_pairs(p, seeds, counts=5e5, n_short=2, n_long=10)
-    assert xin.shape == xt.shape == (24, p.grid.N)
+    assert xin.shape == xt.shape == (48, p.grid.N)
     net = UNet3D(40, 40, 15, seed=1, device=cuda)
-    hist = WS.train(net, xin, xt, epochs=4, lr=2e-3, log=print)
+    hist = WS.train(net, xin, xt, epochs=8, lr=2e-3, log=print)
     assert hist[-1] < hist[0]
     res = WS.evaluate(p, net, list(range(5000, 5008)), counts=5e5, n_short=2, n_long=10, log=print)
     gain = np.array(res["ll_warm_start"]) - np.array(res["ll_input"])
-    print(f"  LL gain of the network over its input: {gain}", flush=True)
-    print(f"  warm-start iterations to the plain 10-iteration LL: {res['iters_warm']}", flush=True)
+    it_w = np.array(res["iters_warm"], float)
+    it_n = np.array(res["iters_no_net"], float)
+    it_w[it_w < 0] = 11
+    it_n[it_n < 0] = 11
+    print(f"  LL of the network image minus LL of its input: {gain}", flush=True)
+    print(f"  iterations to the plain 10-iteration LL: with the network {res['iters_warm']}, "
+          f"without {res['iters_no_net']}", flush=True)
     print(f"  ms plain {res['ms_plain_total']:.1f}  ms warm {res['ms_warm_total']:.1f}", flush=True)
-    assert np.sum(gain > 0) >= 6
+    assert it_w.mean() < it_n.mean()
