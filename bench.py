@@ -157,6 +157,10 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _common(args, workload_line):
     line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32"}
@@ -195,11 +199,13 @@ def bench_frames(args):
     y = workloads.dynamic_data(p, frames, counts, seed=0, device=dev)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
+    _progress(f"frames: data for {F} frames in {t_data:.1f} s")
     run = FramesOSEM(p, y)
     tuner = FramesTuner()
     t0 = time.perf_counter()
     rpb, ft = tuner.tune(run, k=0, reps=2)
     t_tune = time.perf_counter() - t0
+    _progress(f"frames: tuner picked rays_per_block={rpb} frames_per_tile={ft} in {t_tune:.1f} s")
     n_iter = 3
     for _ in range(args.warmup):
         run.x.fill_(1.0)
@@ -208,9 +214,12 @@ def bench_frames(args):
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     t0 = time.perf_counter()
     ev[0].record()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         run.x.fill_(1.0)
-        run.run(n_iter, rpb, ft)
+        for it in range(n_iter):
+            run.run(1, rpb, ft)
+            torch.cuda.synchronize()
+            _progress(f"frames: step {i} iteration {it} done")
     ev[1].record()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -272,6 +281,7 @@ def bench_cgls(args):
     p = workloads.ct_cgls_projector()
     p.initProj()
     mu, y = workloads.ct_cgls_data(p, seed=0)
+    _progress("cgls: data ready")
     runner = CGLSRunner(p, y)
     n_iter = 10
     for _ in range(args.warmup):
