@@ -239,7 +239,7 @@ def bench_frames(args):
     from omega_amd.recon import OSEMRunner
     t_ser = 0.0
     for f in range(min(F, 4)):
-        r1 = OSEMRunner(p, y[:, f].contiguous())
+        r1 = OSEMRunner(p, y[f].contiguous())
         x1 = torch.ones(p.grid.N, device=dev)
         r1.subset_step(x1, 0)
         r1.sens_done = True

@@ -132,39 +132,41 @@ int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* 
  * The reference runs the frames of a dynamic study one after another inside every subset
  * (source/cpp/reconstructionAF.h:1089-1453, per-frame data :429-436). These entry points advance all
  * n_frames frames through a subset step together, tracing each LOR once per tile of frames.
- * Layouts (frame fastest): image stack x[N][n_frames]; measurements meas[n_meas][n_frames] in the subset
- * order of one frame; rhs[N][n_frames] is a work stack that must be zero on the first call and is left
- * zero by every step. S is the subset's frame-independent sensitivity image (omg_frames_sensitivity).
+ * Layouts (frame-major): image stack x[n_frames][N]; measurements meas[n_frames][...] with frame stride
+ * meas_stride, each frame in the subset order of a single-frame run, meas pointing at frame 0's first LOR
+ * of the subset; rhs[n_frames][N] is a work stack that must be zero on the first call and is left zero by
+ * every step. S is the subset's frame-independent sensitivity image (omg_frames_sensitivity).
  * Siddon projector, emission data. */
 
 /* S = max(A_k^T 1, epps) for the subset's measurements [first, first + n_meas) (reconstructionAF.h:356-381,
  * functions.hpp:2981-2996). */
 int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t n_meas, float* S, float epps, void* stream);
 
-/* yhat[i][f] = (A_k x_f)_i for the subset's measurements (Siddon forward projection of every frame). */
+/* yhat[f * yhat_stride + i] = (A_k x_f)_i for the subset's measurements (Siddon FP of every frame). */
 int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, const float* x, float* yhat,
-                       void* stream);
-/* out[f] = sum_i y[i][f] log(yhat[i][f]) - yhat[i][f] (terms with y = 0: -yhat), FP64, for n rows. */
+                       int64_t yhat_stride, void* stream);
+/* out[f] = sum_i y[f][i] log(yhat[f][i]) - yhat[f][i] (terms with y = 0: -yhat), FP64, [n_frames][n] stacks. */
 int omg_frames_loglik(const float* y, const float* yhat, int64_t n, int32_t n_frames, double* out, void* stream);
 
 /* One OSEM subset step of every frame: yhat_f = A_k x_f; r_f = meas_f / (yhat_f + epps) (NaN/Inf -> epps);
  * x_f = x_f / S * (A_k^T r_f + epps) (subiterStep.h:111-121), or with beta != 0 the OSL-RDP step
  * x_f = x_f / (S + dU_f) * A_k^T r_f, dU_f = beta * grad RDP(x_f; gamma, epps) with zero boundary
- * (subiterStep.h:219-231, RDP.h:4-78), dU a work stack [N][n_frames]; positivity x < epps -> epps.
+ * (subiterStep.h:219-231, RDP.h:4-78), dU a work stack [n_frames][N]; positivity x < epps -> epps.
  * Launch shape: rays_per_block in {64,128,256} lanes (one LOR each), frames_per_tile in {1,2,4,8} frames per
  * lane (n_frames must be a multiple), grid_blocks persistent workgroups (0: one resident wave). */
 int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x,
-                         const float* meas, const float* S, float* rhs, float* dU, float beta, float gamma, float epps,
-                         int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
-                         void* stream);
+                         const float* meas, int64_t meas_stride, const float* S, float* rhs, float* dU, float beta,
+                         float gamma, float epps, int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile,
+                         int32_t grid_blocks, void* stream);
 
 /* A captured iteration: n_subsets subset steps (subset k: planes/LORs from first[k], n_meas[k] measurements at
- * meas + meas_offset[k] * n_frames, sensitivity S + k * N) recorded once in a hipGraph and replayed by
+ * meas + meas_offset[k] (frame stride meas_stride), sensitivity S + k * N) recorded once in a hipGraph and replayed by
  * omg_graph_launch. The pointers are baked into the graph: keep the buffers alive and in place. */
 typedef struct omg_graph omg_graph;
 int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t* first, const int64_t* n_meas,
                             const int64_t* meas_offset, int32_t n_frames, float* x, const float* meas,
-                            const float* S, float* rhs, float* dU, float beta, float gamma, float epps,
+                            int64_t meas_stride, const float* S, float* rhs, float* dU, float beta, float gamma,
+                            float epps,
                             int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
                             omg_graph** out);
 int omg_graph_launch(omg_graph* h, void* stream);

@@ -69,11 +69,11 @@ SIGNATURES = {
     "omg_siddon_trace": [_P, _I64, _I64, _P, _P, _P, _P, _P],
     "omg_osem_subset_iteration": [_P, _I64, _I64, _P, _P, _P, _I32, _F, _F, _F, _I32, _P],
     "omg_frames_sensitivity": [_P, _I64, _I64, _P, _F, _P],
-    "omg_frames_forward": [_P, _I64, _I64, _I32, _P, _P, _P],
+    "omg_frames_forward": [_P, _I64, _I64, _I32, _P, _P, _I64, _P],
     "omg_frames_loglik": [_P, _P, _I64, _I32, _P, _P],
-    "omg_frames_osem_step": [_P, _I64, _I64, _I32, _P, _P, _P, _P, _P, _F, _F, _F, _I32, _I32, _I32, _I32, _P],
-    "omg_frames_graph_create": [_P, _I32, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _F, _F, _F, _I32, _I32, _I32, _I32,
-                                _P],
+    "omg_frames_osem_step": [_P, _I64, _I64, _I32, _P, _P, _I64, _P, _P, _P, _F, _F, _F, _I32, _I32, _I32, _I32, _P],
+    "omg_frames_graph_create": [_P, _I32, _P, _P, _P, _I32, _P, _P, _I64, _P, _P, _P, _F, _F, _F, _I32, _I32, _I32,
+                                _I32, _P],
     "omg_graph_launch": [_P, _P],
     "omg_graph_destroy": [_P],
     "omg_conv3d_forward": [_P, _I32, _P, _I32, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P],

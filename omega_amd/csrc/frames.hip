@@ -6,9 +6,10 @@
 // per-frame data :429-436). The frames share the geometry, so the Siddon traversal of a LOR -- the
 // expensive part -- is the same for all of them. Here a lane traces its LOR once per tile of FT frames
 // and carries FT sums:
-//   * image stacks are frame-interleaved, x[voxel][frame], so the FT values a traversal step needs are
-//     one contiguous vector load (and the FT atomic adds of the BP fall in one cache line);
-//   * measurements are interleaved the same way, meas[lor][frame], in the subset order of one frame;
+//   * stacks are frame-major: x[frame][voxel], rhs[frame][voxel], meas[frame][lor] (each frame in the
+//     subset order of a single-frame run), so every frame keeps the cache footprint of a single-frame
+//     reconstruction and a tile of FT frames touches FT such images (an interleaved [voxel][frame] layout
+//     was measured 2x slower: each load then pulls a cache line of other frames' data);
 //   * the sensitivity image of a subset is frame-independent (omg_frames_sensitivity) and computed once;
 //   * a persistent grid walks (LOR block, frame tile) work items, so one launch covers all frames;
 //   * the update kernel clears rhs behind itself, so a whole iteration (16 subset steps) replays from a
@@ -27,25 +28,14 @@ namespace omg {
 constexpr int kFrameBatch = 4;   // traversal steps traced before their loads are consumed
 
 template <int FT>
-__device__ __forceinline__ void load_frames(const float* __restrict__ p, float (&v)[FT]) {
-    if constexpr (FT == 8) {
-        const float4 a = *reinterpret_cast<const float4*>(p);
-        const float4 b = *reinterpret_cast<const float4*>(p + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else if constexpr (FT == 4) {
-        const float4 a = *reinterpret_cast<const float4*>(p);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    } else if constexpr (FT == 2) {
-        const float2 a = *reinterpret_cast<const float2*>(p);
-        v[0] = a.x; v[1] = a.y;
-    } else {
-        v[0] = p[0];
-    }
+__device__ __forceinline__ void load_frames(const float* __restrict__ p, size_t stride, float (&v)[FT]) {
+#pragma unroll
+    for (int f = 0; f < FT; f++) v[f] = p[(size_t)f * stride];
 }
 
-// Sum of len * x[voxel][f0 + f] along LOR lo for FT frames, in traversal order.
+// Sum of len * x[f][voxel] along LOR lo for the FT frames at x, x + N, ..., in traversal order.
 template <int FT>
-__device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, const float* __restrict__ x, uint32_t F,
+__device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, const float* __restrict__ x, uint32_t N,
                                               float (&ax)[FT], float& weight) {
 #pragma unroll
     for (int f = 0; f < FT; f++) ax[f] = 0.f;
@@ -54,7 +44,7 @@ __device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, cons
     if (w.perp) {
         perp_walk(w, [&](uint32_t ind, float len) {
             float v[FT];
-            load_frames<FT>(x + (size_t)ind * F, v);
+            load_frames<FT>(x + ind, N, v);
 #pragma unroll
             for (int f = 0; f < FT; f++) ax[f] += len * v[f];
         });
@@ -66,7 +56,7 @@ __device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, cons
 #pragma unroll
         for (int k = 0; k < kFrameBatch; k++) walk_step<false>(g, w, ind[k], len[k]);
 #pragma unroll
-        for (int k = 0; k < kFrameBatch; k++) load_frames<FT>(x + (size_t)(len[k] > 0.f ? ind[k] : 0u) * F, v[k]);
+        for (int k = 0; k < kFrameBatch; k++) load_frames<FT>(x + (len[k] > 0.f ? ind[k] : 0u), N, v[k]);
 #pragma unroll
         for (int k = 0; k < kFrameBatch; k++)
 #pragma unroll
@@ -75,12 +65,14 @@ __device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, cons
     return true;
 }
 
-// One subset step of all frames: FP, ratio, BP into rhs[voxel][frame]. Work item = (frame tile, LOR block).
-// `x`, `meas`, `rhs` point at frame 0 of their stacks; meas holds the subset's LORs only.
+// One subset step of all frames: FP, ratio, BP into rhs[frame][voxel]. Work item = (frame tile, LOR block).
+// `x`, `rhs` point at frame 0 of their stacks (frame stride N); `meas` at frame 0's first LOR of the subset
+// (frame stride meas_stride).
 template <int FT, int RPB>
-__global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t F, const float* __restrict__ x,
-                                                     const float* __restrict__ meas, float* __restrict__ rhs,
-                                                     float epps, uint32_t lor_blocks, uint32_t n_items) {
+__global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t N, const float* __restrict__ x,
+                                                     const float* __restrict__ meas, int64_t meas_stride,
+                                                     float* __restrict__ rhs, float epps, uint32_t lor_blocks,
+                                                     uint32_t n_items) {
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t tile = item / lor_blocks;
         const uint32_t lb = item - tile * lor_blocks;
@@ -88,13 +80,13 @@ __global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t F, cons
         if (lo >= a.nmeas) continue;
         const uint32_t f0 = tile * FT;
         float ax[FT], w = 0.f;
-        if (!frames_gather<FT>(a, lo, x + f0, F, ax, w)) continue;
+        if (!frames_gather<FT>(a, lo, x + (size_t)f0 * N, N, ax, w)) continue;
         float r[FT];
         bool any = false;
 #pragma unroll
         for (int f = 0; f < FT; f++) {
             const float yhat = ax[f] * w;
-            float q = meas[(size_t)lo * F + f0 + f] / (yhat + epps);
+            float q = meas[(size_t)(f0 + f) * meas_stride + lo] / (yhat + epps);
             if (isnan(q) || isinf(q)) q = epps;
             r[f] = q;
             any |= (q != 0.f);
@@ -102,33 +94,33 @@ __global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t F, cons
         if (!any) continue;
         siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
             const float lw = len * w;
-            float* p = rhs + (size_t)ind * F + f0;
+            float* p = rhs + (size_t)f0 * N + ind;
 #pragma unroll
-            for (int f = 0; f < FT; f++) unsafeAtomicAdd(p + f, r[f] * lw);
+            for (int f = 0; f < FT; f++) unsafeAtomicAdd(p + (size_t)f * N, r[f] * lw);
         });
     }
 }
 
-// yhat[lo][f] = A_k x_f for every frame of a tile (0 for LORs that miss the image).
+// yhat[f][lo] = A_k x_f for every frame of a tile (0 for LORs that miss the image); frame stride y_stride.
 template <int FT>
-__global__ __launch_bounds__(256) void k_frames_fp(SidArgs a, uint32_t F, const float* __restrict__ x,
-                                                   float* __restrict__ yhat) {
+__global__ __launch_bounds__(256) void k_frames_fp(SidArgs a, uint32_t N, const float* __restrict__ x,
+                                                   float* __restrict__ yhat, int64_t y_stride) {
     const int64_t lo = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (lo >= a.nmeas) return;
     const uint32_t f0 = blockIdx.y * FT;
     float ax[FT], w = 0.f;
-    const bool hit = frames_gather<FT>(a, lo, x + f0, F, ax, w);
+    const bool hit = frames_gather<FT>(a, lo, x + (size_t)f0 * N, N, ax, w);
 #pragma unroll
-    for (int f = 0; f < FT; f++) yhat[(size_t)lo * F + f0 + f] = hit ? ax[f] * w : 0.f;
+    for (int f = 0; f < FT; f++) yhat[(size_t)(f0 + f) * y_stride + lo] = hit ? ax[f] * w : 0.f;
 }
 
-// out[f] += sum_i y log(yhat) - yhat over [n][F] stacks, terms with y == 0 contribute -yhat (FP64 sums).
+// out[f] += sum_i y log(yhat) - yhat over [F][n] stacks, terms with y == 0 contribute -yhat (FP64 sums).
 __global__ __launch_bounds__(256) void k_frames_ll(const float* __restrict__ y, const float* __restrict__ yhat,
                                                    int64_t n, uint32_t F, double* __restrict__ out) {
     const uint32_t f = blockIdx.y;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const double yi = y[(size_t)i * F + f], hi = yhat[(size_t)i * F + f];
+        const double yi = y[(size_t)f * n + i], hi = yhat[(size_t)f * n + i];
         s += (yi != 0.0 ? yi * log(hi) : 0.0) - hi;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -153,24 +145,24 @@ __global__ __launch_bounds__(256) void k_frames_fill(float* __restrict__ p, size
     if (i < n) p[i] = v;
 }
 
-// dU[n][f] = beta * grad RDP(x_f)[n], zero outside the volume: the single-frame gradient (omg_rdp_gradient,
-// boundary 0; RDP.h:4-78) on the frame-interleaved stack, lanes summed in the same order.
+// dU[f][n] = beta * grad RDP(x_f)[n], zero outside the volume: the single-frame gradient (omg_rdp_gradient,
+// boundary 0; RDP.h:4-78) on every frame of the stack, lanes summed in the same order.
 __global__ __launch_bounds__(256) void k_frames_rdp(const float* __restrict__ u, float* __restrict__ dU, uint32_t Nx,
                                                     uint32_t Ny, uint32_t Nz, uint32_t F, float gamma, float epps,
                                                     float beta) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     const uint32_t Nxy = Nx * Ny;
-    if (i >= (size_t)Nxy * Nz * F) return;
-    const uint32_t n = (uint32_t)(i / F);
+    const size_t N = (size_t)Nxy * Nz;
+    if (i >= N * F) return;
+    const uint32_t n = (uint32_t)(i % N);
     const uint32_t z = n / Nxy, rem = n - z * Nxy, y = rem / Nx, x = rem - y * Nx;
-    const size_t sF = F;
     const float uj = u[i];
-    const float xp = (x + 1 < Nx) ? u[i + sF] : 0.f;
-    const float xm = (x > 0) ? u[i - sF] : 0.f;
-    const float yp = (y + 1 < Ny) ? u[i + sF * Nx] : 0.f;
-    const float ym = (y > 0) ? u[i - sF * Nx] : 0.f;
-    const float zp = (z + 1 < Nz) ? u[i + sF * Nxy] : 0.f;
-    const float zm = (z > 0) ? u[i - sF * Nxy] : 0.f;
+    const float xp = (x + 1 < Nx) ? u[i + 1] : 0.f;
+    const float xm = (x > 0) ? u[i - 1] : 0.f;
+    const float yp = (y + 1 < Ny) ? u[i + Nx] : 0.f;
+    const float ym = (y > 0) ? u[i - Nx] : 0.f;
+    const float zp = (z + 1 < Nz) ? u[i + Nxy] : 0.f;
+    const float zm = (z > 0) ? u[i - Nxy] : 0.f;
     float l0 = rdp_term(uj, xp, gamma, epps);
     l0 = l0 + rdp_term(uj, yp, gamma, epps);
     l0 = l0 + rdp_term(uj, zp, gamma, epps);
@@ -182,7 +174,7 @@ __global__ __launch_bounds__(256) void k_frames_rdp(const float* __restrict__ u,
     dU[i] = 0.f + beta * (l0 + l1);
 }
 
-// OSEM: x[n][f] = x / S[n] * (rhs + epps) (subiterStep.h:111-121, EM algorithms.h:12-15);
+// OSEM: x[f][n] = x / S[n] * (rhs + epps) (subiterStep.h:111-121, EM algorithms.h:12-15);
 // OSL (dU != null): x = x / (S[n] + dU) * rhs (subiterStep.h:219-231, OSL algorithms.h:24-27);
 // positivity x < epps -> epps; rhs = 0 behind.
 __global__ __launch_bounds__(256) void k_frames_update(uint32_t N, uint32_t F, float* __restrict__ x,
@@ -190,7 +182,7 @@ __global__ __launch_bounds__(256) void k_frames_update(uint32_t N, uint32_t F, f
                                                        const float* __restrict__ dU, float epps, int pos) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (size_t)N * F) return;
-    const float s = S[i / F];
+    const float s = S[i % N];
     float xn = dU ? x[i] / (s + dU[i]) * rhs[i] : x[i] / s * (rhs[i] + epps);
     if (pos && xn < epps) xn = epps;
     x[i] = xn;
@@ -202,9 +194,10 @@ struct FramesStep {
     uint32_t F, N;
     float* x;
     const float* meas;
+    int64_t meas_stride;
     const float* S;
     float* rhs;
-    float* dU;          // OSL-RDP work stack [N][F], or null for OSEM
+    float* dU;          // OSL-RDP work stack [F][N], or null for OSEM
     float beta, gamma;
     uint32_t Nx, Ny, Nz;
     float epps;
@@ -249,8 +242,8 @@ static hipError_t launch_data(const FramesStep& st, hipStream_t s) {
     uint32_t grid = (uint32_t)st.grid;
     if (grid > n_items) grid = n_items;
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_frames_osem<FT, RPB>), dim3(grid), dim3(RPB), 0, s, st.a, st.F, st.x, st.meas, st.rhs,
-                       st.epps, lor_blocks, n_items);
+    hipLaunchKernelGGL((k_frames_osem<FT, RPB>), dim3(grid), dim3(RPB), 0, s, st.a, st.N, st.x, st.meas,
+                       st.meas_stride, st.rhs, st.epps, lor_blocks, n_items);
     return hipGetLastError();
 }
 
@@ -288,8 +281,9 @@ static int launch_step(const FramesStep& st, hipStream_t s) {
 }
 
 static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x, const float* meas,
-                     const float* S, float* rhs, float* dU, float beta, float gamma, float epps, int32_t positivity,
-                     int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks, FramesStep& st) {
+                     int64_t meas_stride, const float* S, float* rhs, float* dU, float beta, float gamma, float epps,
+                     int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
+                     FramesStep& st) {
     OMG_REQUIRE(g != nullptr, "null geometry");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "batched frames run the improved Siddon projector");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
@@ -303,14 +297,13 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
                 frames_per_tile);
     OMG_REQUIRE(grid_blocks >= 0, "grid_blocks must be >= 0 (0 = one wave of resident workgroups)");
     OMG_REQUIRE(beta == 0.f || dU != nullptr, "OSL-RDP (beta != 0) needs the dU work stack");
-    OMG_REQUIRE(((uintptr_t)x % (4 * frames_per_tile)) == 0 && ((uintptr_t)meas % 4) == 0,
-                "image stack must be aligned to frames_per_tile floats");
+    OMG_REQUIRE(n_frames == 1 || meas_stride >= n_meas, "meas_stride must cover the subset's measurements");
     const int rc = check_range(g, first, n_meas);
     if (rc) return rc;
     st.a = make_sid_args(g, first, n_meas);
     st.F = (uint32_t)n_frames;
     st.N = g->d.Nx * g->d.Ny * g->d.Nz;
-    st.x = x; st.meas = meas; st.S = S; st.rhs = rhs;
+    st.x = x; st.meas = meas; st.meas_stride = meas_stride; st.S = S; st.rhs = rhs;
     st.dU = beta != 0.f ? dU : nullptr;
     st.beta = beta; st.gamma = gamma;
     st.Nx = g->d.Nx; st.Ny = g->d.Ny; st.Nz = g->d.Nz;
@@ -351,23 +344,23 @@ extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t 
 }
 
 extern "C" int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, const float* x,
-                                  float* yhat, void* stream) {
+                                  float* yhat, int64_t yhat_stride, void* stream) {
     OMG_REQUIRE(g != nullptr && x && yhat && n_frames >= 1, "bad arguments");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
                 "batched frames run the improved Siddon projector on emission data");
-    OMG_REQUIRE(((uintptr_t)x % 16) == 0, "image stack must be 16-byte aligned");
+    OMG_REQUIRE(n_frames == 1 || yhat_stride >= n_meas, "yhat_stride must cover the subset's measurements");
     const int rc = check_range(g, first, n_meas);
     if (rc) return rc;
     if (n_meas == 0) return OMG_OK;
     const SidArgs a = make_sid_args(g, first, n_meas);
     hipStream_t s = as_stream(stream);
     const unsigned gx = grid_for(n_meas, 256);
+    const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
     if (n_frames % 4 == 0)
-        hipLaunchKernelGGL(k_frames_fp<4>, dim3(gx, (unsigned)(n_frames / 4)), dim3(256), 0, s, a, (uint32_t)n_frames,
-                           x, yhat);
+        hipLaunchKernelGGL(k_frames_fp<4>, dim3(gx, (unsigned)(n_frames / 4)), dim3(256), 0, s, a, N, x, yhat,
+                           yhat_stride);
     else
-        hipLaunchKernelGGL(k_frames_fp<1>, dim3(gx, (unsigned)n_frames), dim3(256), 0, s, a, (uint32_t)n_frames, x,
-                           yhat);
+        hipLaunchKernelGGL(k_frames_fp<1>, dim3(gx, (unsigned)n_frames), dim3(256), 0, s, a, N, x, yhat, yhat_stride);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -387,19 +380,20 @@ extern "C" int omg_frames_loglik(const float* y, const float* yhat, int64_t n, i
 }
 
 extern "C" int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x,
-                                    const float* meas, const float* S, float* rhs, float* dU, float beta, float gamma,
-                                    float epps, int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile,
-                                    int32_t grid_blocks, void* stream) {
+                                    const float* meas, int64_t meas_stride, const float* S, float* rhs, float* dU,
+                                    float beta, float gamma, float epps, int32_t positivity, int32_t rays_per_block,
+                                    int32_t frames_per_tile, int32_t grid_blocks, void* stream) {
     FramesStep st;
-    int rc = make_step(g, first, n_meas, n_frames, x, meas, S, rhs, dU, beta, gamma, epps, positivity, rays_per_block,
-                       frames_per_tile, grid_blocks, st);
+    int rc = make_step(g, first, n_meas, n_frames, x, meas, meas_stride, S, rhs, dU, beta, gamma, epps, positivity,
+                       rays_per_block, frames_per_tile, grid_blocks, st);
     if (rc) return rc;
     return launch_step(st, as_stream(stream));
 }
 
 extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, const int64_t* first,
                                        const int64_t* n_meas, const int64_t* meas_offset, int32_t n_frames, float* x,
-                                       const float* meas, const float* S, float* rhs, float* dU, float beta,
+                                       const float* meas, int64_t meas_stride, const float* S, float* rhs, float* dU,
+                                       float beta,
                                        float gamma, float epps, int32_t positivity, int32_t rays_per_block,
                                        int32_t frames_per_tile, int32_t grid_blocks, omg_graph** out) {
     OMG_REQUIRE(out != nullptr, "null output handle");
@@ -410,7 +404,7 @@ extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, con
     std::vector<FramesStep> steps((size_t)n_subsets);
     for (int32_t k = 0; k < n_subsets; k++) {
         OMG_REQUIRE(meas_offset[k] >= 0, "negative measurement offset");
-        const int rc = make_step(g, first[k], n_meas[k], n_frames, x, meas + (size_t)meas_offset[k] * n_frames,
+        const int rc = make_step(g, first[k], n_meas[k], n_frames, x, meas + meas_offset[k], meas_stride,
                                  S + (size_t)k * N, rhs, dU, beta, gamma, epps, positivity, rays_per_block,
                                  frames_per_tile, grid_blocks, steps[(size_t)k]);
         if (rc) return rc;

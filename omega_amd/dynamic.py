@@ -5,8 +5,9 @@ subset (source/cpp/reconstructionAF.h:1089-1453; per-frame measurements :429-436
 ``nMeas`` in source/Python/omegatomo/projector/indices.py:271-272). Here the frames share one
 geometry handle and advance together:
 
-* ``x`` is a frame-interleaved image stack ``[N, F]`` and ``meas`` a frame-interleaved measurement
-  stack ``[M, F]`` in the projector's subset order (the order of ``projectorClass.nTotMeas``);
+* ``x`` is a frame-major image stack ``[F, N]`` and ``meas`` a frame-major measurement stack ``[F, M]``,
+  each frame in the projector's subset order (the order of ``projectorClass.nTotMeas``), so every frame
+  keeps the cache footprint of a single-frame reconstruction;
 * one ``omg_frames_osem_step`` launch performs a subset step of every frame (OSEM, or OSL-OSEM with the
   RDP prior when ``beta != 0``); a persistent grid walks (frame tile, LOR block) work items, so
   rays-per-block and frames-per-tile are launch parameters;
@@ -45,18 +46,20 @@ class FramesOSEM:
         self.g = proj.geometry.handle
         self.N = int(proj.N[0])
         if meas.dim() != 2 or meas.dtype != torch.float32 or not meas.is_cuda:
-            raise ValueError("meas must be a float32 device tensor of shape [n_measurements, n_frames]")
-        if meas.shape[0] != int(proj.nTotMeas[-1]):
-            raise ValueError(f"meas has {meas.shape[0]} rows, the projector has {int(proj.nTotMeas[-1])} measurements")
+            raise ValueError("meas must be a float32 device tensor of shape [n_frames, n_measurements]")
+        if meas.shape[1] != int(proj.nTotMeas[-1]):
+            raise ValueError(f"meas has {meas.shape[1]} columns, the projector has {int(proj.nTotMeas[-1])} "
+                             "measurements")
         self.meas = meas.contiguous()
-        self.F = int(meas.shape[1])
+        self.F = int(meas.shape[0])
+        self.M = int(meas.shape[1])
         dev = meas.device
-        self.x = torch.ones((self.N, self.F), dtype=torch.float32, device=dev) if x0 is None else \
-            x0.reshape(self.N, self.F).to(torch.float32).contiguous().clone()
+        self.x = torch.ones((self.F, self.N), dtype=torch.float32, device=dev) if x0 is None else \
+            x0.reshape(self.F, self.N).to(torch.float32).contiguous().clone()
         self.S = torch.zeros((proj.subsets, self.N), dtype=torch.float32, device=dev)
-        self.rhs = torch.zeros((self.N, self.F), dtype=torch.float32, device=dev)
+        self.rhs = torch.zeros((self.F, self.N), dtype=torch.float32, device=dev)
         self.beta, self.gamma = float(beta), float(gamma)
-        self.dU = torch.zeros((self.N, self.F), dtype=torch.float32, device=dev) if self.beta != 0.0 else None
+        self.dU = torch.zeros((self.F, self.N), dtype=torch.float32, device=dev) if self.beta != 0.0 else None
         self.epps = float(epps)
         self.pos = int(bool(enforce_positivity))
         self.sens_done = False
@@ -83,9 +86,9 @@ class FramesOSEM:
             self.sensitivity()
         first, n, off = self._subset(k)
         x = self.x if x is None else x
-        _lib.call("omg_frames_osem_step", self.g, first, n, self.F, _p(x), _p(self.meas[off:off + n]), _p(self.S[k]),
-                  _p(self.rhs), _p(self.dU), self.beta, self.gamma, self.epps, self.pos, int(rays_per_block),
-                  int(frames_per_tile), int(grid_blocks), self._stream())
+        _lib.call("omg_frames_osem_step", self.g, first, n, self.F, _p(x), _p(self.meas[0, off:]), self.M,
+                  _p(self.S[k]), _p(self.rhs), _p(self.dU), self.beta, self.gamma, self.epps, self.pos,
+                  int(rays_per_block), int(frames_per_tile), int(grid_blocks), self._stream())
 
     def iteration(self, rays_per_block: int = 256, frames_per_tile: int = 8) -> None:
         for k in range(self.proj.subsets):
@@ -106,8 +109,8 @@ class FramesOSEM:
         off = np.array([int(self.proj.nTotMeas[k]) for k in range(K)], dtype=np.int64)
         h = C.c_void_p()
         _lib.call("omg_frames_graph_create", self.g, K, first.ctypes.data, n.ctypes.data, off.ctypes.data, self.F,
-                  _p(self.x), _p(self.meas), _p(self.S), _p(self.rhs), _p(self.dU), self.beta, self.gamma, self.epps,
-                  self.pos, key[0], key[1], 0, C.byref(h))
+                  _p(self.x), _p(self.meas), self.M, _p(self.S), _p(self.rhs), _p(self.dU), self.beta, self.gamma,
+                  self.epps, self.pos, key[0], key[1], 0, C.byref(h))
         self._graphs[key] = h
         return h
 

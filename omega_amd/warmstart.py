@@ -33,38 +33,39 @@ def _stream(torch):
 
 
 def batch_data(p, seeds, counts, device="cuda"):
-    """Poisson data [M, len(seeds)] (subset order) of the seeded ellipsoid phantoms, and the phantoms."""
+    """Poisson data [len(seeds), M] (subset order) of the seeded ellipsoid phantoms, and the phantoms [F, N]."""
     import torch
     N = p.grid.N
-    X = torch.empty((N, len(seeds)), dtype=torch.float32, device=device)
+    X = torch.empty((len(seeds), N), dtype=torch.float32, device=device)
     for j, s in enumerate(seeds):
-        X[:, j] = torch.from_numpy(phantoms.ellipsoids(p.grid, seed=int(s), n_ell=8, background=0.1)).to(device)
+        X[j] = torch.from_numpy(phantoms.ellipsoids(p.grid, seed=int(s), n_ell=8, background=0.1)).to(device)
     y = frames_forward(p, X)
-    scale = counts / y.double().sum(0)
+    scale = counts / y.double().sum(1)
     g = torch.Generator(device=device)
     g.manual_seed(int(seeds[0]) * 7919 + 1)
-    y = torch.poisson((y.double() * scale[None, :]).float(), generator=g)
+    y = torch.poisson((y.double() * scale[:, None]).float(), generator=g)
     return y, X
 
 
 def frames_forward(p, X):
-    """A x_f for every column of X [N, F] over all subsets, in subset order [M, F]."""
+    """A x_f for every row of X [F, N] over all subsets, in subset order [F, M]."""
     import torch
-    F = X.shape[1]
+    F = X.shape[0]
     Xc = X.contiguous()
-    out = torch.empty((int(p.nTotMeas[-1]), F), dtype=torch.float32, device=X.device)
+    M = int(p.nTotMeas[-1])
+    out = torch.empty((F, M), dtype=torch.float32, device=X.device)
     for k in range(p.subsets):
         first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
-        _lib.call("omg_frames_forward", p.geometry.handle, first, n, F, _p(Xc), _p(out[off:off + n]), _stream(torch))
+        _lib.call("omg_frames_forward", p.geometry.handle, first, n, F, _p(Xc), _p(out[0, off:]), M, _stream(torch))
     return out
 
 
 def loglik(p, y, X):
-    """Poisson log-likelihood of each column of X [N, F] for data y [M, F] (float64 numpy array)."""
+    """Poisson log-likelihood of each row of X [F, N] for data y [F, M] (float64 numpy array)."""
     import torch
     yhat = frames_forward(p, X)
-    out = torch.zeros(X.shape[1], dtype=torch.float64, device=X.device)
-    _lib.call("omg_frames_loglik", _p(y), _p(yhat), y.shape[0], y.shape[1], _p(out), _stream(torch))
+    out = torch.zeros(X.shape[0], dtype=torch.float64, device=X.device)
+    _lib.call("omg_frames_loglik", _p(y), _p(yhat), y.shape[1], y.shape[0], _p(out), _stream(torch))
     return out.cpu().numpy()
 
 
@@ -79,8 +80,8 @@ def make_pairs(p, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, gamma=2.0, 
         short.run(n_short, rpb, min(ft, _tile(len(sb))))
         long = FramesOSEM(p, y, beta=beta, gamma=gamma)
         long.run(n_long, rpb, min(ft, _tile(len(sb))))
-        ins.append(short.x.t().contiguous())
-        outs.append(long.x.t().contiguous())
+        ins.append(short.x.clone())
+        outs.append(long.x.clone())
         short.close()
         long.close()
         del y
@@ -149,8 +150,8 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
         ws = torch.empty_like(short.x)
         ev0.record()
         for j in range(F):
-            x = short.x[:, j].contiguous()
-            ws[:, j] = net.predict(x, float(1.0 / x.mean())).clamp_min(1e-5)
+            x = short.x[j]
+            ws[j] = net.predict(x, float(1.0 / x.mean())).clamp_min(1e-5)
         ev1.record()
         ev1.synchronize()
         t_net += ev0.elapsed_time(ev1)

@@ -109,18 +109,19 @@ def frame_counts(n_frames: int, seed: int, lo: float = 1e5, hi: float = 1e7):
 
 
 def dynamic_data(p: projectorClass, frames, counts, seed: int, device="cuda"):
-    """Frame-interleaved Poisson data [M, F] in subset order: frame f is Poisson(A x_f scaled to counts[f])."""
+    """Frame-major Poisson data [F, M] in subset order: frame f is Poisson(A x_f scaled to counts[f]);
+    ``frames`` is [N, F] (phantoms.dynamic_ellipsoids)."""
     import torch
     F = frames.shape[1]
     M = int(p.nTotMeas[-1])
-    y = torch.empty((M, F), dtype=torch.float32, device=device)
+    y = torch.empty((F, M), dtype=torch.float32, device=device)
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     for f in range(F):
         x = torch.as_tensor(np.ascontiguousarray(frames[:, f]), device=device)
         ax = torch.cat([p.forwardProject(x, k) for k in range(p.subsets)])
         scale = float(counts[f]) / float(ax.double().sum())
-        y[:, f] = torch.poisson(ax * scale, generator=g)
+        y[f] = torch.poisson(ax * scale, generator=g)
         del ax
     return y
 

@@ -32,23 +32,23 @@ def study(cuda):
     assert frames.shape[1] == F
     counts = workloads.frame_counts(F, seed=5, lo=2e4, hi=2e5)
     y = workloads.dynamic_data(p, frames, counts, seed=5, device=cuda)
-    print(f"[frames study] N={p.grid.N} M={y.shape[0]} F={F}", flush=True)
+    print(f"[frames study] N={p.grid.N} M={y.shape[1]} F={F}", flush=True)
     return p, y
 
 
 def _single_frame_iteration(p, y, cuda):
     import torch
     N = p.grid.N
-    out = np.zeros((N, F), np.float32)
+    out = np.zeros((F, N), np.float32)
     for f in range(F):
         x = torch.ones(N, dtype=torch.float32, device=cuda)
         S = torch.zeros((p.subsets, N), dtype=torch.float32, device=cuda)
-        yf = y[:, f].contiguous()
+        yf = y[f].contiguous()
         for k in range(p.subsets):
             first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
             _lib.call("omg_osem_subset_iteration", p.geometry.handle, first, n, ptr(x), ptr(yf[off:off + n]),
                       ptr(S[k]), 1, 1e-5, 0.0, 2.0, 1, None)
-        out[:, f] = x.cpu().numpy()
+        out[f] = x.cpu().numpy()
     return out
 
 
@@ -60,7 +60,7 @@ def test_frames_match_single_frame_path(study, cuda):
         run.iteration(rpb, ft)
         got = run.x.cpu().numpy()
         for f in range(F):
-            assert rel_l2(got[:, f], ref[:, f]) <= 1e-5, f"rpb={rpb} ft={ft} frame {f}"
+            assert rel_l2(got[f], ref[f]) <= 1e-5, f"rpb={rpb} ft={ft} frame {f}"
         run.close()
 
 
@@ -91,14 +91,14 @@ def test_step_matches_oracle_frame0(study, cuda):
     first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
     case = O.SiddonCase(p.grid, int(p.nRowsD), int(p.nColsD), p.x, p.z, plane_offset=first)
     x = np.ones(p.grid.N, np.float32)
-    meas = y[off:off + n, 0].cpu().numpy()
+    meas = y[0, off:off + n].cpu().numpy()
     r = case.fp(x, n)
     O.lib().oracle_ratio_f32(meas, r, n, 1e-5)
     rhs, sens = case.bp(r, n, sens=True)
     O.lib().oracle_sens_clamp_f32(sens, p.grid.N, 1e-5)
     O.lib().oracle_osem_update_f32(x, sens, rhs, p.grid.N, 1e-5, 1)
     assert rel_l2(run.S[0].cpu().numpy(), sens) <= 1e-5
-    assert rel_l2(run.x[:, 0].cpu().numpy(), x) <= 1e-5
+    assert rel_l2(run.x[0].cpu().numpy(), x) <= 1e-5
     run.close()
 
 
@@ -138,9 +138,9 @@ def test_osl_rdp_frames_match_single_frame(study, cuda):
     for f in (0, 5):
         x = torch.ones(N, dtype=torch.float32, device=cuda)
         S = torch.zeros((p.subsets, N), dtype=torch.float32, device=cuda)
-        yf = y[:, f].contiguous()
+        yf = y[f].contiguous()
         for k in range(p.subsets):
             first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
             _lib.call("omg_osem_subset_iteration", p.geometry.handle, first, n, ptr(x), ptr(yf[off:off + n]),
                       ptr(S[k]), 1, 1e-5, beta, gamma, 1, None)
-        assert rel_l2(got[:, f], x.cpu().numpy()) <= 1e-5, f"frame {f}"
+        assert rel_l2(got[f], x.cpu().numpy()) <= 1e-5, f"frame {f}"

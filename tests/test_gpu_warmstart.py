@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def scanner(cuda):
-    p = workloads.pet3d_projector(subsets=4, n_rings=8, crystals=96, max_ring_diff=7, radius=150.0,
+    p = workloads.pet3d_projector(subsets=16, n_rings=8, crystals=96, max_ring_diff=7, radius=150.0,
                                   image=(40, 40, 15), fov_xy=160.0)
     p.initProj()
     return p
@@ -30,21 +30,21 @@ def test_batched_loglik_matches_single(scanner, cuda):
     y, X = WS.batch_data(p, [11, 12, 13, 14], counts=2e5)
     ll = WS.loglik(p, y, X)
     for j in range(4):
-        x = X[:, j].contiguous()
+        x = X[j].contiguous()
         yhat = torch.cat([p.forwardProject(x, k) for k in range(p.subsets)])
         out = torch.zeros(1, dtype=torch.float64, device=cuda)
-        yj = y[:, j].contiguous()
+        yj = y[j].contiguous()
         _lib.call("omg_poisson_loglik", ptr(yj), ptr(yhat), yj.numel(), ptr(out), None)
         assert abs(float(out) - ll[j]) <= 1e-6 * abs(ll[j])
 
 
 def test_pairs_training_and_gain(scanner, cuda):
     p = scanner
-    seeds = list(range(100, 148))
+    seeds = list(range(100, 164))
     xin, xt = WS.make_pairs(p, seeds, counts=5e5, n_short=2, n_long=10)
-    assert xin.shape == xt.shape == (48, p.grid.N)
+    assert xin.shape == xt.shape == (64, p.grid.N)
     net = UNet3D(40, 40, 15, seed=1, device=cuda)
-    hist = WS.train(net, xin, xt, epochs=8, lr=2e-3, log=print)
+    hist = WS.train(net, xin, xt, epochs=10, lr=1e-3, log=print)
     assert hist[-1] < hist[0]
     res = WS.evaluate(p, net, list(range(5000, 5008)), counts=5e5, n_short=2, n_long=10, log=print)
     gain = np.array(res["ll_warm_start"]) - np.array(res["ll_input"])
