@@ -203,13 +203,13 @@ def bench_frames(args):
     run = FramesOSEM(p, y)
     tuner = FramesTuner()
     t0 = time.perf_counter()
-    rpb, ft = tuner.tune(run, k=0, reps=2)
+    rpb, ft, gm = tuner.tune(run, k=0, reps=2)
     t_tune = time.perf_counter() - t0
-    _progress(f"frames: tuner picked rays_per_block={rpb} frames_per_tile={ft} in {t_tune:.1f} s")
+    _progress(f"frames: tuner picked rays_per_block={rpb} frames_per_tile={ft} grid={gm} in {t_tune:.1f} s")
     n_iter = 3
     for _ in range(args.warmup):
         run.x.fill_(1.0)
-        run.run(1, rpb, ft)
+        run.run(1, rpb, ft, grid_blocks=gm)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     t0 = time.perf_counter()
@@ -217,7 +217,7 @@ def bench_frames(args):
     for i in range(args.steps):
         run.x.fill_(1.0)
         for it in range(n_iter):
-            run.run(1, rpb, ft)
+            run.run(1, rpb, ft, grid_blocks=gm)
             torch.cuda.synchronize()
             _progress(f"frames: step {i} iteration {it} done")
     ev[1].record()
@@ -228,7 +228,7 @@ def bench_frames(args):
     xs = run.x.clone()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record()
-    run.step(0, rpb, ft, x=xs)
+    run.step(0, rpb, ft, gm, x=xs)
     b.record()
     b.synchronize()
     step_s = a.elapsed_time(b) * 1e-3
@@ -256,8 +256,9 @@ def bench_frames(args):
         "data": "synthetic (ellipsoid phantom with one-tissue-compartment curves, lognormal frame counts)",
         "config": {"workload": "pet3d_frames_osem_siddon", "frames": F, "image": [128, 128, 63],
                    "sinogram": [224, 224, 1024], "rings": 32, "crystals": 448, "subsets": 16, "iterations": n_iter,
-                   "rays_per_block": rpb, "frames_per_tile": ft, "tuner_ms": tuner.last_timings and
-                   {f"{k[0]}x{k[1]}": round(v, 3) for k, v in tuner.last_timings.items()},
+                   "rays_per_block": rpb, "frames_per_tile": ft, "grid": "persistent" if gm == 0 else "per-item",
+                   "tuner_ms": tuner.last_timings and
+                   {f"{k[0]}x{k[1]}{'p' if k[2] == 0 else 'i'}": round(v, 3) for k, v in tuner.last_timings.items()},
                    "data_gen_s": round(t_data, 1), "tune_s": round(t_tune, 1)},
         "roofline": {"bound": "hbm", "achieved": round(bytes_step / step_s / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(bytes_step / step_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,

@@ -25,7 +25,10 @@
 
 namespace omg {
 
-constexpr int kFrameBatch = 4;   // traversal steps traced before their loads are consumed
+// traversal steps traced before their loads are consumed: 8 like the single-frame gather for up to two
+// frames per lane, 4 above (register budget)
+template <int FT>
+constexpr int frame_batch() { return FT <= 2 ? 8 : 4; }
 
 template <int FT>
 __device__ __forceinline__ void load_frames(const float* __restrict__ p, size_t stride, float (&v)[FT]) {
@@ -50,15 +53,16 @@ __device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, cons
         });
         return true;
     }
+    constexpr int kFB = frame_batch<FT>();
     while (w.left > 0u) {
-        uint32_t ind[kFrameBatch];
-        float len[kFrameBatch], v[kFrameBatch][FT];
+        uint32_t ind[kFB];
+        float len[kFB], v[kFB][FT];
 #pragma unroll
-        for (int k = 0; k < kFrameBatch; k++) walk_step<false>(g, w, ind[k], len[k]);
+        for (int k = 0; k < kFB; k++) walk_step<false>(g, w, ind[k], len[k]);
 #pragma unroll
-        for (int k = 0; k < kFrameBatch; k++) load_frames<FT>(x + (len[k] > 0.f ? ind[k] : 0u), N, v[k]);
+        for (int k = 0; k < kFB; k++) load_frames<FT>(x + (len[k] > 0.f ? ind[k] : 0u), N, v[k]);
 #pragma unroll
-        for (int k = 0; k < kFrameBatch; k++)
+        for (int k = 0; k < kFB; k++)
 #pragma unroll
             for (int f = 0; f < FT; f++) ax[f] = (len[k] > 0.f) ? ax[f] + len[k] * v[k][f] : ax[f];
     }

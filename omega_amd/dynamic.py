@@ -27,6 +27,8 @@ from . import _lib
 
 RAYS_PER_BLOCK = (64, 128, 256)
 FRAMES_PER_TILE = (1, 2, 4, 8)
+# persistent grid (one resident wave of workgroups walking the work items) or one workgroup per item
+GRID_MODES = (0, 1 << 30)
 
 
 def _p(t):
@@ -90,14 +92,14 @@ class FramesOSEM:
                   _p(self.S[k]), _p(self.rhs), _p(self.dU), self.beta, self.gamma, self.epps, self.pos,
                   int(rays_per_block), int(frames_per_tile), int(grid_blocks), self._stream())
 
-    def iteration(self, rays_per_block: int = 256, frames_per_tile: int = 8) -> None:
+    def iteration(self, rays_per_block: int = 256, frames_per_tile: int = 8, grid_blocks: int = 0) -> None:
         for k in range(self.proj.subsets):
-            self.step(k, rays_per_block, frames_per_tile)
+            self.step(k, rays_per_block, frames_per_tile, grid_blocks)
 
     # ---- graph capture ------------------------------------------------------------------------------
-    def capture(self, rays_per_block: int = 256, frames_per_tile: int = 8):
+    def capture(self, rays_per_block: int = 256, frames_per_tile: int = 8, grid_blocks: int = 0):
         """Capture one iteration (every subset step of every frame) as a hipGraph."""
-        key = (int(rays_per_block), int(frames_per_tile))
+        key = (int(rays_per_block), int(frames_per_tile), int(grid_blocks))
         if key in self._graphs:
             return self._graphs[key]
         if not self.sens_done:
@@ -110,20 +112,21 @@ class FramesOSEM:
         h = C.c_void_p()
         _lib.call("omg_frames_graph_create", self.g, K, first.ctypes.data, n.ctypes.data, off.ctypes.data, self.F,
                   _p(self.x), _p(self.meas), self.M, _p(self.S), _p(self.rhs), _p(self.dU), self.beta, self.gamma,
-                  self.epps, self.pos, key[0], key[1], 0, C.byref(h))
+                  self.epps, self.pos, key[0], key[1], key[2], C.byref(h))
         self._graphs[key] = h
         return h
 
-    def replay(self, rays_per_block: int = 256, frames_per_tile: int = 8) -> None:
-        h = self.capture(rays_per_block, frames_per_tile)
+    def replay(self, rays_per_block: int = 256, frames_per_tile: int = 8, grid_blocks: int = 0) -> None:
+        h = self.capture(rays_per_block, frames_per_tile, grid_blocks)
         _lib.call("omg_graph_launch", h, self._stream())
 
-    def run(self, n_iter: int, rays_per_block: int = 256, frames_per_tile: int = 8, graph: bool = True) -> None:
+    def run(self, n_iter: int, rays_per_block: int = 256, frames_per_tile: int = 8, graph: bool = True,
+            grid_blocks: int = 0) -> None:
         for _ in range(n_iter):
             if graph:
-                self.replay(rays_per_block, frames_per_tile)
+                self.replay(rays_per_block, frames_per_tile, grid_blocks)
             else:
-                self.iteration(rays_per_block, frames_per_tile)
+                self.iteration(rays_per_block, frames_per_tile, grid_blocks)
 
     def close(self) -> None:
         for h in self._graphs.values():
@@ -138,7 +141,8 @@ class FramesOSEM:
 
 
 class FramesTuner:
-    """Picks rays-per-block x frames-per-tile for :class:`FramesOSEM` by timing them on the real data.
+    """Picks rays-per-block x frames-per-tile x grid mode for :class:`FramesOSEM` by timing them on the real
+    data (grid mode: a persistent grid of one resident wave, or one workgroup per work item).
 
     Every candidate runs ``reps`` eager subset steps (subset ``k``) on a scratch copy of the image
     stack after one warm-up step; the median event-timed step wins. Results are cached per key
@@ -159,7 +163,7 @@ class FramesTuner:
         p = run.proj
         return f"N{run.N}-M{int(p.nMeasSubset[k, 0])}-F{run.F}-z{int(p.Nz[0])}"
 
-    def tune(self, run: FramesOSEM, k: int = 0, reps: int = 3, candidates=None) -> tuple[int, int]:
+    def tune(self, run: FramesOSEM, k: int = 0, reps: int = 3, candidates=None) -> tuple[int, int, int]:
         torch = run.torch
         key = self.key(run, k)
         if key in self.cache:
@@ -167,22 +171,23 @@ class FramesTuner:
         if not run.sens_done:
             run.sensitivity()
         if candidates is None:
-            candidates = [(r, f) for r in RAYS_PER_BLOCK for f in FRAMES_PER_TILE if run.F % f == 0]
+            candidates = [(r, f, gm) for r in RAYS_PER_BLOCK for f in FRAMES_PER_TILE for gm in GRID_MODES
+                          if run.F % f == 0]
         xs = run.x.clone()
         timings = {}
-        for rpb, ft in candidates:
+        for rpb, ft, gm in candidates:
             times = []
             for rep in range(reps + 1):
                 xs.copy_(run.x)
                 a = torch.cuda.Event(enable_timing=True)
                 b = torch.cuda.Event(enable_timing=True)
                 a.record()
-                run.step(k, rpb, ft, x=xs)
+                run.step(k, rpb, ft, gm, x=xs)
                 b.record()
                 b.synchronize()
                 if rep:
                     times.append(a.elapsed_time(b))
-            timings[(rpb, ft)] = float(np.median(times))
+            timings[(rpb, ft, gm)] = float(np.median(times))
         run.rhs.zero_()
         best = min(timings, key=timings.get)
         self.last_timings = timings

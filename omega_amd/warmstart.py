@@ -80,8 +80,10 @@ def make_pairs(p, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, gamma=2.0, 
         short.run(n_short, rpb, min(ft, _tile(len(sb))))
         long = FramesOSEM(p, y, beta=beta, gamma=gamma)
         long.run(n_long, rpb, min(ft, _tile(len(sb))))
-        ins.append(short.x.clone())
-        outs.append(long.x.clone())
+        # OSL-OSEM divides by S + beta grad R; a run that left the finite range is not a usable target
+        ok = torch.isfinite(short.x).all(dim=1) & torch.isfinite(long.x).all(dim=1)
+        ins.append(short.x[ok].clone())
+        outs.append(long.x[ok].clone())
         short.close()
         long.close()
         del y

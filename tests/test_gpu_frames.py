@@ -55,12 +55,12 @@ def _single_frame_iteration(p, y, cuda):
 def test_frames_match_single_frame_path(study, cuda):
     p, y = study
     ref = _single_frame_iteration(p, y, cuda)
-    for rpb, ft in [(256, 8), (64, 1), (128, 4), (64, 2)]:
+    for rpb, ft, gm in [(256, 8, 0), (64, 1, 1 << 30), (128, 4, 0), (64, 2, 1 << 30)]:
         run = FramesOSEM(p, y)
-        run.iteration(rpb, ft)
+        run.iteration(rpb, ft, gm)
         got = run.x.cpu().numpy()
         for f in range(F):
-            assert rel_l2(got[f], ref[f]) <= 1e-5, f"rpb={rpb} ft={ft} frame {f}"
+            assert rel_l2(got[f], ref[f]) <= 1e-5, f"rpb={rpb} ft={ft} grid={gm} frame {f}"
         run.close()
 
 
@@ -107,7 +107,7 @@ def test_autotuner_picks_a_valid_shape(study, tmp_path):
     run = FramesOSEM(p, y)
     tuner = FramesTuner(str(tmp_path / "tune.json"))
     best = tuner.tune(run, k=1, reps=2)
-    assert best in tuner.last_timings and len(tuner.last_timings) == 12
+    assert best in tuner.last_timings and len(tuner.last_timings) == 24
     assert min(tuner.last_timings.values()) == tuner.last_timings[best]
     again = FramesTuner(str(tmp_path / "tune.json"))
     assert again.tune(run, k=1) == best
@@ -127,20 +127,25 @@ def test_rejects_bad_launch_shapes(study):
 
 
 def test_osl_rdp_frames_match_single_frame(study, cuda):
+    """OSL-RDP: one subset step to 1e-5; a full iteration to 5e-5 -- the OSL update divides by S + beta grad R,
+    which amplifies the float-atomic ordering differences of the two paths where the prior gradient
+    nearly cancels the sensitivity."""
     import torch
     p, y = study
     N = p.grid.N
     beta, gamma = 0.1, 2.0
-    run = FramesOSEM(p, y, beta=beta, gamma=gamma)
-    run.iteration(128, 4)
-    got = run.x.cpu().numpy()
-    run.close()
-    for f in (0, 5):
-        x = torch.ones(N, dtype=torch.float32, device=cuda)
-        S = torch.zeros((p.subsets, N), dtype=torch.float32, device=cuda)
-        yf = y[f].contiguous()
-        for k in range(p.subsets):
-            first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
-            _lib.call("omg_osem_subset_iteration", p.geometry.handle, first, n, ptr(x), ptr(yf[off:off + n]),
-                      ptr(S[k]), 1, 1e-5, beta, gamma, 1, None)
-        assert rel_l2(got[f], x.cpu().numpy()) <= 1e-5, f"frame {f}"
+    for n_steps, tol in ((1, 1e-5), (p.subsets, 5e-5)):
+        run = FramesOSEM(p, y, beta=beta, gamma=gamma)
+        for k in range(n_steps):
+            run.step(k, 128, 4)
+        got = run.x.cpu().numpy()
+        run.close()
+        for f in (0, 5):
+            x = torch.ones(N, dtype=torch.float32, device=cuda)
+            S = torch.zeros((p.subsets, N), dtype=torch.float32, device=cuda)
+            yf = y[f].contiguous()
+            for k in range(n_steps):
+                first, n, off = int(p.nMeas[k]), int(p.nMeasSubset[k, 0]), int(p.nTotMeas[k])
+                _lib.call("omg_osem_subset_iteration", p.geometry.handle, first, n, ptr(x), ptr(yf[off:off + n]),
+                          ptr(S[k]), 1, 1e-5, beta, gamma, 1, None)
+            assert rel_l2(got[f], x.cpu().numpy()) <= tol, f"frame {f} after {n_steps} subset steps"
