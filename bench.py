@@ -9,8 +9,9 @@ Also reported:
   roofline      for the dominant kernel (the fused FP/ratio/BP projector launch of iterations >= 1):
                 algorithmic bytes per launch / its average duration, timed with HIP events on the
                 launch stream inside the timed region (the library's omg_timing_* hooks).
-                Algorithmic bytes of one such launch = 12 * sum(Np) + 8 * M_k  (4 B gather of x per intersected voxel, 8 B atomic
-                read-modify-write of rhs per voxel, 4 B measurement + 4 B LOR table per LOR);
+                Algorithmic bytes of one such launch = 4 * sum(Np) + 8 * sum(Np, y != 0) + 8 * M_k (4 B gather
+                of x per intersected voxel, 8 B read-modify-write of rhs per voxel of the LORs with counts --
+                a zero ratio is not backprojected --, 4 B measurement + 4 B LOR table per LOR);
                 sum(Np) = exact count of (LOR, voxel) intersections with length > 0.
   cpu_baseline  the reference CPU projector (oracle/_ref, the reference's own headers compiled
                 with OpenMP) or, without it, the oracle restatement, on the host cores, on the
@@ -78,14 +79,18 @@ def main():
     n_iter = 4
     sub_iters = p.subsets * n_iter
 
-    # exact intersection counts per subset (GPU traversal; equal to the oracle's, see tests)
-    np_sum = []
+    # exact intersection counts per subset (GPU traversal; equal to the oracle's, see tests), all LORs and
+    # the LORs with counts (only those are backprojected: a zero ratio adds nothing and is skipped)
+    np_sum, np_nz = [], []
     for k in range(p.subsets):
         n = int(p.nMeasSubset[k, 0])
         counts = torch.zeros(n, dtype=torch.int32, device=dev)
         _lib.call("omg_siddon_trace", p.geometry.handle, int(p.nMeas[k]), n, C.c_void_p(counts.data_ptr()),
                   None, None, None, None)
-        np_sum.append(int(counts.to(torch.int64).sum().item()))
+        c64 = counts.to(torch.int64)
+        off = int(p.nTotMeas[k])
+        np_sum.append(int(c64.sum().item()))
+        np_nz.append(int((c64 * (y[off:off + n] != 0)).sum().item()))
     M = [int(p.nMeasSubset[k, 0]) for k in range(p.subsets)]
 
     runner = OSEMRunner(p, y)
@@ -128,7 +133,7 @@ def main():
     # dominant kernel: the fused FP -> ratio -> BP projector launch of iterations >= 1
     # (k_siddon_planar<640, true, true, false>), timed with HIP events on its stream
     avg_s = tot.value * 1e-3 / max(cnt.value, 1)
-    bytes_per_launch = float(np.mean([12.0 * np_sum[k] + 8.0 * M[k] for k in range(p.subsets)]))
+    bytes_per_launch = float(np.mean([4.0 * np_sum[k] + 8.0 * np_nz[k] + 8.0 * M[k] for k in range(p.subsets)]))
     achieved = bytes_per_launch / avg_s / 1e9
     traffic = read_traffic(args.traffic)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -232,9 +237,15 @@ def bench_frames(args):
     b.record()
     b.synchronize()
     step_s = a.elapsed_time(b) * 1e-3
-    np0 = _trace_counts(p, 0, dev)
+    import ctypes as C
     M0 = int(p.nMeasSubset[0, 0])
-    bytes_step = (12.0 * np0 + 8.0 * M0) * F
+    cnt = torch.zeros(M0, dtype=torch.int32, device=dev)
+    _lib.call("omg_siddon_trace", p.geometry.handle, int(p.nMeas[0]), M0, C.c_void_p(cnt.data_ptr()), None, None,
+              None, None)
+    c64 = cnt.to(torch.int64)
+    np0 = int(c64.sum().item())
+    np_nz = int((c64[None, :] * (y[:, :M0] != 0)).sum().item())    # summed over frames
+    bytes_step = 4.0 * np0 * F + 8.0 * np_nz + 8.0 * M0 * F
     # the same work as F serial single-frame fused iterations (the reference's frame loop), one subset
     from omega_amd.recon import OSEMRunner
     t_ser = 0.0

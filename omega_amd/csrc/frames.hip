@@ -96,11 +96,14 @@ __global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t N, cons
             any |= (q != 0.f);
         }
         if (!any) continue;
+        // frames whose ratio is 0 (no counts in this LOR) add nothing: skip their atomics, as the
+        // single-frame kernel skips the whole LOR
         siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
             const float lw = len * w;
             float* p = rhs + (size_t)f0 * N + ind;
 #pragma unroll
-            for (int f = 0; f < FT; f++) unsafeAtomicAdd(p + (size_t)f * N, r[f] * lw);
+            for (int f = 0; f < FT; f++)
+                if (r[f] != 0.f) unsafeAtomicAdd(p + (size_t)f * N, r[f] * lw);
         });
     }
 }

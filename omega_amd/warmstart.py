@@ -8,7 +8,9 @@ library's own kernels, and the evaluation that keeps the plain method as fallbac
 * Training: masked MSE on images scaled by 1 / mean(input), Adam, one volume per step.
 * Evaluation on held-out seeds: the plain run is ``n_long`` OSEM-RDP iterations from a uniform
   image; the warm-started run is ``n_short`` OSEM iterations, the network, then OSEM-RDP iterations
-  until the Poisson log-likelihood reaches the plain run's final value within ``tol`` relative.
+  until the OSEM-RDP objective L(x) - beta R(x) (Poisson log-likelihood minus the RDP value) reaches
+  the plain run's final value within ``tol`` relative; the same continuation from the short-OSEM
+  image without the network is run as the control.
   Wall time is compared on the GPU's own clock (HIP events); when the warm start does not reach the
   target within ``n_long`` iterations the plain result stands.
 """
@@ -67,6 +69,22 @@ def loglik(p, y, X):
     out = torch.zeros(X.shape[0], dtype=torch.float64, device=X.device)
     _lib.call("omg_frames_loglik", _p(y), _p(yhat), y.shape[1], y.shape[0], _p(out), _stream(torch))
     return out.cpu().numpy()
+
+
+def rdp_values(p, X, beta, gamma):
+    """beta * R(x_f) for every row of X [F, N] (omg_rdp_value, zero boundary), float64 numpy array."""
+    import torch
+    out = torch.zeros(X.shape[0], dtype=torch.float64, device=X.device)
+    Nx, Ny, Nz = int(p.Nx[0]), int(p.Ny[0]), int(p.Nz[0])
+    for j in range(X.shape[0]):
+        xj = X[j].contiguous()
+        _lib.call("omg_rdp_value", _p(xj), Nx, Ny, Nz, float(gamma), float(beta), _p(out[j:j + 1]), _stream(torch))
+    return out.cpu().numpy()
+
+
+def objective(p, y, X, beta, gamma):
+    """The OSEM-RDP objective L(x) - beta R(x) of every row of X [F, N] (float64 numpy array)."""
+    return loglik(p, y, X) - rdp_values(p, X, beta, gamma)
 
 
 def make_pairs(p, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, gamma=2.0, batch=8, rpb=256, ft=8):
@@ -139,7 +157,7 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
         ev1.record()
         ev1.synchronize()
         t_plain += ev0.elapsed_time(ev1)
-        ll_plain = loglik(p, y, plain.x)
+        ll_plain = objective(p, y, plain.x, beta, gamma)
         # warm start: n_short OSEM iterations, the network, then OSEM-RDP until the plain log-likelihood
         short = FramesOSEM(p, y)
         short.sensitivity()
@@ -148,7 +166,7 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
         ev1.record()
         ev1.synchronize()
         t_short += ev0.elapsed_time(ev1)
-        ll_in = loglik(p, y, short.x)
+        ll_in = objective(p, y, short.x, beta, gamma)
         ws = torch.empty_like(short.x)
         ev0.record()
         for j in range(F):
@@ -157,7 +175,7 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
         ev1.record()
         ev1.synchronize()
         t_net += ev0.elapsed_time(ev1)
-        ll_ws = loglik(p, y, ws)
+        ll_ws = objective(p, y, ws, beta, gamma)
         cont = FramesOSEM(p, y, beta=beta, gamma=gamma, x0=ws)
         cont.sensitivity()
         target = ll_plain - tol * np.abs(ll_plain)
@@ -168,7 +186,7 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
             ev1.record()
             ev1.synchronize()
             t_cont += ev0.elapsed_time(ev1)
-            ll = loglik(p, y, cont.x)
+            ll = objective(p, y, cont.x, beta, gamma)
             newly = (iters < 0) & (ll >= target)
             iters[newly] = it
             if np.all(iters > 0):
@@ -179,7 +197,7 @@ def evaluate(p, net: UNet3D, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, 
         iters_nn = np.full(F, -1)
         for it in range(1, n_long + 1):
             nonet.run(1, rpb, tile)
-            ll = loglik(p, y, nonet.x)
+            ll = objective(p, y, nonet.x, beta, gamma)
             iters_nn[(iters_nn < 0) & (ll >= target)] = it
             if np.all(iters_nn > 0):
                 break
@@ -219,6 +237,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--counts", type=float, default=5e6)
     ap.add_argument("--subsets", type=int, default=16)
+    ap.add_argument("--n-long", type=int, default=40)
+    ap.add_argument("--n-short", type=int, default=2)
     ap.add_argument("--small", action="store_true", help="reduced scanner (8 rings, 96 crystals, 40x40x15 image)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -231,15 +251,15 @@ def main(argv=None):
     t0 = time.time()
     train_seeds = list(range(1000, 1000 + a.pairs))
     hold_seeds = list(range(900000, 900000 + a.holdout))
-    xin, xt = make_pairs(p, train_seeds, counts=a.counts)
+    xin, xt = make_pairs(p, train_seeds, counts=a.counts, n_short=a.n_short, n_long=a.n_long)
     torch.cuda.synchronize()
-    log(f"pairs: {a.pairs} in {time.time() - t0:.1f} s")
+    log(f"pairs: {xin.shape[0]} usable of {a.pairs} in {time.time() - t0:.1f} s")
     net = UNet3D(int(p.Nx[0]), int(p.Ny[0]), int(p.Nz[0]), seed=0)
     t1 = time.time()
     hist = train(net, xin, xt, epochs=a.epochs, lr=a.lr, log=log)
     torch.cuda.synchronize()
     log(f"training: {time.time() - t1:.1f} s")
-    res = evaluate(p, net, hold_seeds, counts=a.counts, log=log)
+    res = evaluate(p, net, hold_seeds, counts=a.counts, n_short=a.n_short, n_long=a.n_long, log=log)
     reached = int(np.sum(res["reached"]))
     out = {"pairs": a.pairs, "holdout": a.holdout, "epochs": a.epochs, "loss": hist,
            "reached": reached, "mean_iters_warm": float(np.mean([i for i in res["iters_warm"] if i > 0] or [0])),
