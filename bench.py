@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_traffic.txt"),
                     help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
-                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth"],
+                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet3d_autograd"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     return ap.parse_args()
@@ -58,7 +58,8 @@ def main():
         sys.exit(2)
 
     if args.workload != "pet2d_osem":
-        {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth}[args.workload](args)
+        {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth,
+         "pet3d_autograd": bench_autograd}[args.workload](args)
         return
 
     import ctypes as C
@@ -394,6 +395,65 @@ def bench_orth(args):
                      "kernel": "k_orth_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt)},
         "cpu_baseline": None,
+    })
+
+
+def bench_autograd(args):
+    """Differentiable 3-D PET projector on the full ring geometry (224 x 224 x 1024 sinogram, 128 x 128 x 63
+    image): adjoint identity on 20 seeded (x, y) pairs, central differences of -L(x) + R(x) (y = Poisson(A x),
+    2e7 counts, gamma 2, beta 0.1) at 64 random voxels with h = 1e-3 against the autograd gradient, and the
+    time of one forward + backward. value = forward+backward passes per second."""
+    import numpy as np
+    import torch
+    from omega_amd import _lib, autograd as AG, phantoms, workloads
+    dev = torch.device("cuda:0")
+    _lib.load()
+    p = workloads.pet3d_projector(subsets=1)
+    p.initProj()
+    ph = phantoms.ellipsoids(p.grid, seed=0) + 0.2
+    meas, _ = workloads.poisson_data(p, ph, 2e7, seed=1)
+    _progress("autograd: data ready")
+    worst = 0.0
+    for seed in range(20):
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        x = torch.rand(p.grid.N, device=dev, generator=g)
+        yv = torch.rand(int(p.nMeasSubset[0, 0]), device=dev, generator=g)
+        lhs = float((p.forwardProject(x, 0).double() * yv.double()).sum())
+        rhs = float((x.double() * p.backwardProject(yv, 0).double()).sum())
+        worst = max(worst, abs(lhs - rhs) / abs(lhs))
+    _progress(f"autograd: adjoint worst {worst:.2e}")
+    x = torch.as_tensor(ph, device=dev).clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, args.steps)
+    for _ in range(reps):
+        x.grad = None
+        loss = AG.penalised_nll(x, p, meas, gamma=2.0, beta=0.1)
+        loss.backward()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    grad = x.grad.double().cpu().numpy()
+    rng = np.random.default_rng(7)
+    h = 1e-3
+    errs = []
+    for j in rng.choice(p.grid.N, 64, replace=False):
+        xp = x.detach().clone()
+        xm = x.detach().clone()
+        xp[j] += h
+        xm[j] -= h
+        fd = (float(AG.penalised_nll(xp, p, meas)) - float(AG.penalised_nll(xm, p, meas))) / (2 * h)
+        errs.append(abs(fd - grad[j]) / max(1.0, abs(grad[j])))
+    _progress(f"autograd: finite differences max {max(errs):.2e}")
+    _common(args, {
+        "metric": "differentiable 3-D PET projector: forward+backward of -L(x)+R(x) per second",
+        "value": round(1.0 / dt, 3), "unit": "fwd+bwd/s", "ms_per_step": round(dt * 1e3, 2),
+        "data": "synthetic (ellipsoid phantom + 0.2, Poisson data at 2e7 counts)",
+        "config": {"workload": "pet3d_autograd", "image": [128, 128, 63], "sinogram": [224, 224, 1024],
+                   "rings": 32, "crystals": 448, "max_ring_difference": 31, "gamma": 2.0, "beta": 0.1},
+        "adjoint_pairs": 20, "adjoint_worst_rel_mismatch": worst,
+        "fd_voxels": 64, "fd_h": h, "fd_max_rel_error": max(errs), "fd_median_rel_error": float(np.median(errs)),
+        "roofline": None, "cpu_baseline": None,
     })
 
 

@@ -163,3 +163,23 @@ def test_errors_are_reported(cuda):
     assert e.value.code == _lib.OMG_ERR_ARG
     with pytest.raises(_lib.OmegaHipError):
         fp(geom, 0, 100, img)                              # not a whole plane
+
+
+def test_c2_geometry_fp32_vs_fp64(cuda):
+    """Full 3-D ring geometry (224 x 224 x 1024 sinogram, 128 x 128 x 63 image), 8 planes of it: the FP32
+    forward projection bit-identical to the FP32 oracle; its error against the FP64 oracle is stated."""
+    import torch
+    from omega_amd import workloads
+    p = workloads.pet3d_projector(subsets=1)
+    p.initProj()
+    per = int(p.nRowsD) * int(p.nColsD)
+    first, planes = 500, 8
+    n = per * planes
+    img = _img(p.grid.N, 9)
+    y = fp(p.geometry, first, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    c32 = O.SiddonCase(p.grid, int(p.nRowsD), int(p.nColsD), p.x, p.z, plane_offset=first)
+    c64 = O.SiddonCase(p.grid, int(p.nRowsD), int(p.nColsD), p.x, p.z, plane_offset=first, dtype=np.float64)
+    assert np.array_equal(y, c32.fp(img, n))
+    e = rel_l2(y, c64.fp(img.astype(np.float64), n))
+    print(f"[c2 fp] FP32 vs FP64 relative L2 {e:.2e} over {n} LORs", flush=True)
+    assert e < 1e-5
