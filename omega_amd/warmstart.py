@@ -87,11 +87,13 @@ def objective(p, y, X, beta, gamma):
     return loglik(p, y, X) - rdp_values(p, X, beta, gamma)
 
 
-def make_pairs(p, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, gamma=2.0, batch=8, rpb=256, ft=8):
+def make_pairs(p, seeds, counts=5e6, n_short=2, n_long=40, beta=0.1, gamma=2.0, batch=8, rpb=256, ft=8, log=None):
     """(inputs [n, N], targets [n, N]) device tensors for the seeds, `batch` phantoms per launch."""
     import torch
     ins, outs = [], []
     for b0 in range(0, len(seeds), batch):
+        if log and (b0 // batch) % 8 == 0:
+            log(f"pairs: batch {b0 // batch} of {(len(seeds) + batch - 1) // batch}")
         sb = seeds[b0:b0 + batch]
         y, _ = batch_data(p, sb, counts)
         short = FramesOSEM(p, y)
@@ -266,7 +268,7 @@ def main(argv=None):
     t0 = time.time()
     train_seeds = list(range(1000, 1000 + a.pairs))
     hold_seeds = list(range(900000, 900000 + a.holdout))
-    xin, xt = make_pairs(p, train_seeds, counts=a.counts, n_short=a.n_short, n_long=a.n_long, beta=a.beta)
+    xin, xt = make_pairs(p, train_seeds, counts=a.counts, n_short=a.n_short, n_long=a.n_long, beta=a.beta, log=log)
     torch.cuda.synchronize()
     log(f"pairs: {xin.shape[0]} usable of {a.pairs} in {time.time() - t0:.1f} s")
     ratio = (xt.amax(dim=1) / xt.mean(dim=1)).cpu().numpy()
