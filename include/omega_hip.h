@@ -64,7 +64,9 @@ typedef struct omg_geom_desc {
     float dPitchX;             /* transaxial detector pitch */
     float dPitchY;             /* axial detector pitch */
     /* launch tuning: reserved[0] != 0 routes all Siddon calls through the one-lane-per-LOR kernels;
-     * reserved[1] > 0 sets the workgroups per plane of the direct-plane (LDS) kernels. Zero = automatic. */
+     * reserved[1] > 0 sets the workgroups per plane of the direct-plane (LDS) kernels; reserved[2] != 0
+     * disables the voxel-driven backprojection of the fused subset iteration on parallel-beam ring
+     * sinograms (the LDS-atomic planar kernel then runs). Zero = automatic. */
     int32_t reserved[8];
 } omg_geom_desc;
 

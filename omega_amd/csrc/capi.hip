@@ -76,6 +76,7 @@ static int upload(const float* src, int64_t n, float** dst) {
 }
 
 int bdd_prepare(omg_geom* g);   // BDD per-geometry set-up (bdd.hip)
+int vbp_prepare(omg_geom* g, const float* xt);  // voxel-driven planar backprojection set-up (vbp.hip)
 int orth_prepare(omg_geom* g);  // orthogonal projector set-up (orth.hip)
 
 extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
@@ -125,6 +126,7 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
         g->planar = true;
         for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
     }
+    if (!rc && d->mode == OMG_MODE_PET_SINO) rc = vbp_prepare(g, d->x);
     if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
     if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);
     g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr;
@@ -138,6 +140,7 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
 
 void bdd_release(omg_geom* g);
 void orth_release(omg_geom* g);
+void vbp_release(omg_geom* g);
 
 extern "C" int omg_geom_destroy(omg_geom* g) {
     if (!g) return OMG_OK;
@@ -148,6 +151,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->work);
     bdd_release(g);
     orth_release(g);
+    vbp_release(g);
     delete g;
     return OMG_OK;
 }

@@ -28,6 +28,12 @@ struct omg_geom {
     std::vector<float> z_host;
     // Branchless distance-driven projector: per-projection back-projection geometry (16 floats each).
     float* bdd_geom5 = nullptr;
+    // Voxel-driven planar backprojection (vbp.hip): usable when every angle column is a parallel-beam set
+    // of evenly spaced LORs with endpoints outside the image; per-angle parameters and q/w work buffers.
+    bool vbp_ok = false;
+    struct VbpAngle* vbp_angles = nullptr;
+    float* vbp_q = nullptr;
+    size_t vbp_q_floats = 0;
 };
 
 namespace omg {

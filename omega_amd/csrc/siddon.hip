@@ -378,6 +378,9 @@ int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, con
               float* sens, float epps, hipStream_t s);
 int orth_trace(const omg_geom* g, int64_t first, int64_t n, int32_t* counts, const int64_t* offsets, uint32_t* idx,
                float* w, hipStream_t s);
+int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
+                         int32_t compute_sens, float epps, const float* dU, int32_t positivity, const int32_t* map,
+                         hipStream_t s);
 int bdd_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
 int bdd_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens, hipStream_t s);
 
@@ -517,6 +520,19 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
     const uint32_t per = g->d.size_x * g->d.size_y;
     if (n_meas > 0 && g->d.projector_type == OMG_PROJ_SIDDON && planar_ok(g) && !g->force_generic &&
         unique_slices(g, first, (uint32_t)(n_meas / per))) {
+        if (g->vbp_ok) {
+            // scatter-free path: ray-driven FP -> q = r w, voxel-driven BP with the update fused (vbp.hip)
+            int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
+            SidArgs a = make_sid_args(g, first, n_meas);
+            hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a,
+                               (uint32_t)(n_meas / per), map);
+            OMG_CHECK_LAUNCH();
+            hipEvent_t stop;
+            timing_begin(s, &stop);
+            rc = vbp_subset_iteration(gm, first, n_meas, x, meas, S, compute_sens, epps, dU, positivity, map, s);
+            timing_end(s, stop);
+            return rc;
+        }
         const PlanarShape sh = planar_shape(g, n_meas);
         if (compute_sens) rc = launch_planar<true, true, true>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
         else rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);

@@ -1,0 +1,273 @@
+// vbp.hip -- scatter-free OSEM subset iteration for direct-plane sinograms with parallel-beam angles.
+//
+// The fused planar kernel (siddon.hip) backprojects with one LDS atomic add per (LOR, voxel); on
+// gfx950 that phase is bound by LDS atomic issue. When every angle column of the sinogram is a set
+// of parallel, evenly spaced LORs whose endpoints lie outside the image (a ring sinogram), the
+// backprojection of a plane can be computed voxel by voxel instead:
+//
+//   rhs(v) = sum_angles sum_{bins i crossing v} q_i * len_i(v),   q_i = r_i * w_i
+//
+// len_i(v) is the length of LOR i inside the square voxel v -- the quantity the Siddon traversal
+// produces as a difference of plane-crossing parameters (projector_functions.h:1842-1998) --
+// evaluated in closed form: for a line with unit direction u at signed distance delta from the
+// voxel centre, the chord of a dx x dy square is a trapezoid in delta (flat Lmax = min(dx/|ux|,
+// dy/|uy|) for |delta| <= H', linear to 0 at |delta| = H, H/H' = hx|uy| +/- hy|ux|). No atomics, a
+// fixed summation order (deterministic), and the EM / OSL update of each voxel fused behind it.
+//
+// The forward projection stays the ray-driven Siddon gather (bit-identical to the oracle); only
+// the order and the rounding of the backprojection's terms differ from the LOR-driven kernels,
+// within float rounding of the lengths.
+#include "geom.hpp"
+
+#include <cmath>
+#include <vector>
+
+// per-angle parameters (global scope: omg_geom holds a pointer to them)
+struct VbpAngle {
+    float nx, ny;        // unit normal of the angle's LORs
+    float rho0, drho;    // signed distance of bin 0 from the origin along n, bin spacing
+    float inv_drho;
+    float H, Hp;         // outer / inner half-widths of the voxel's projection on n
+    float Lmax, slope;   // chord plateau and ramp slope
+    float pad[3];
+};
+
+namespace omg {
+
+
+__device__ __forceinline__ float vbp_chord(const VbpAngle& A, float delta) {
+    const float a = fabsf(delta);
+    const float ramp = A.slope * (A.H - a);
+    return a >= A.H ? 0.f : (a <= A.Hp ? A.Lmax : fminf(ramp, A.Lmax));
+}
+
+constexpr int kVbpTPB = 512;         // 8 waves; with ~88 KB of LDS one workgroup per CU
+constexpr int kVbpVPT = 1;           // voxels per lane
+constexpr int kFpqTPB = 640;
+
+// FP (Siddon gather from the LDS-staged slice, as k_siddon_planar) -> ratio -> q = r * w per LOR;
+// with SENS also wq = w. Workgroup = (plane, chunk of the plane's LORs).
+template <int TPB, bool SENS>
+__global__ __launch_bounds__(TPB) void k_planar_fpq(SidArgs a, uint32_t n_planes, uint32_t chunk,
+                                                    const float* __restrict__ img, const float* __restrict__ meas,
+                                                    float* __restrict__ q, float* __restrict__ wq, float epps) {
+    extern __shared__ float lds[];
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    const uint32_t per = a.size_x * a.size_y;
+    const uint32_t l0 = grp * chunk;
+    if (l0 >= per) return;
+    const uint32_t l1 = min(per, l0 + chunk);
+    const uint32_t Nx = a.Nx, Nyx = a.Nyx, P = a.pitch;
+    const int64_t base = (int64_t)plane * per;
+    const float zs = a.z[(a.plane_offset + plane) * 2];
+    const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+    if (tk < 0 || (uint32_t)tk >= a.Nz) {
+        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+            q[base + l] = 0.f;
+            if (SENS) wq[base + l] = 0.f;
+        }
+        return;
+    }
+    const float4* src = reinterpret_cast<const float4*>(img + (size_t)tk * Nyx);
+    for (uint32_t i = threadIdx.x; i < Nyx / 4; i += TPB) {
+        const float4 v = src[i];
+        const uint32_t e = 4 * i, yy = e / Nx, xx = e - yy * Nx;
+        float* d = lds + yy * P + xx;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    __syncthreads();
+    for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+        const int64_t lo = base + l;
+        float ax, w = 0.f;
+        const bool hit = siddon_gather<true>(a, lo, lds, ax, w);
+        const float yv = hit ? ax * w : 0.f;
+        float r = meas[lo] / (yv + epps);
+        if (isnan(r) || isinf(r)) r = epps;
+        q[lo] = hit ? r * w : 0.f;
+        if (SENS) wq[lo] = hit ? w : 0.f;
+    }
+}
+
+// Voxel-driven backprojection of one slice's plane and the EM (OSL) update of its voxels.
+// grid = (tiles of kVbpTPB * kVbpVPT voxels, Nz); map[z] = the plane of the launch that lies in slice z, or -1.
+template <bool SENS, bool OSL>
+__global__ __launch_bounds__(kVbpTPB) void k_planar_vbp(SidArgs a, const VbpAngle* __restrict__ ang,
+                                                        const int32_t* __restrict__ map, const float* __restrict__ q,
+                                                        const float* __restrict__ wq, float* __restrict__ x,
+                                                        float* __restrict__ S, const float* __restrict__ dU,
+                                                        float epps, int pos) {
+    extern __shared__ float lds[];
+    const uint32_t z = blockIdx.y;
+    const uint32_t Nx = a.Nx, Nyx = a.Nyx;
+    const uint32_t nr = a.size_x, na = a.size_y, per = nr * na;
+    const int32_t plane = map[z];
+    float* qs = lds;                                            // the plane's q, [angle][bin]
+    VbpAngle* as = reinterpret_cast<VbpAngle*>(lds + per);      // per-angle parameters
+    if (plane >= 0) {
+        const float* qp = q + (size_t)plane * per;
+        for (uint32_t i = threadIdx.x; i < per; i += kVbpTPB) qs[i] = qp[i];
+        const float* src = reinterpret_cast<const float*>(ang);
+        float* dst = reinterpret_cast<float*>(as);
+        for (uint32_t i = threadIdx.x; i < na * (sizeof(VbpAngle) / 4); i += kVbpTPB) dst[i] = src[i];
+    }
+    __syncthreads();
+    const float* wp = SENS && plane >= 0 ? wq + (size_t)plane * per : nullptr;
+    for (int k = 0; k < kVbpVPT; k++) {
+        const uint32_t vi = (blockIdx.x * kVbpVPT + k) * kVbpTPB + threadIdx.x;
+        if (vi >= Nyx) continue;
+        float acc = 0.f, accs = 0.f;
+        if (plane >= 0) {
+            const uint32_t yy = vi / Nx, xx = vi - yy * Nx;
+            const float cx = a.bx + ((float)xx + 0.5f) * a.dx;
+            const float cy = a.by + ((float)yy + 0.5f) * a.dy;
+            for (uint32_t an = 0; an < na; an++) {
+                const VbpAngle A = as[an];
+                const float p = __builtin_fmaf(A.nx, cx, A.ny * cy);
+                const float t0 = (p - A.H - A.rho0) * A.inv_drho, t1 = (p + A.H - A.rho0) * A.inv_drho;
+                const int i0 = max(0, (int)ceilf(fminf(t0, t1)));
+                const int i1 = min((int)nr - 1, (int)floorf(fmaxf(t0, t1)));
+                for (int i = i0; i <= i1; i++) {
+                    const float len = vbp_chord(A, __builtin_fmaf((float)i, A.drho, A.rho0) - p);
+                    acc += qs[an * nr + i] * len;
+                    if (SENS) accs += wp[an * nr + i] * len;
+                }
+            }
+        }
+        const size_t n = (size_t)z * Nyx + vi;
+        float s;
+        if (SENS) {
+            s = accs < epps ? epps : accs;
+            S[n] = s;
+        } else {
+            s = S[n];
+        }
+        float xn = OSL ? x[n] / (s + dU[n]) * acc : x[n] / s * (acc + epps);
+        if (pos && xn < epps) xn = epps;
+        x[n] = xn;
+    }
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+// Host check of the parallel-beam ring-sinogram structure; fills g->vbp_angles on success.
+int vbp_prepare(omg_geom* g, const float* xt) {
+    g->vbp_ok = false;
+    const omg_geom_desc& d = g->d;
+    if (!g->planar || d.projector_type != OMG_PROJ_SIDDON || d.reserved[2] != 0) return OMG_OK;
+    const uint32_t nr = d.size_x, na = d.size_y;
+    if (nr < 2 || na < 1) return OMG_OK;
+    const double bx = d.bx, by = d.by, ex = (double)d.bx + (double)d.Nx * d.dx, ey = (double)d.by + (double)d.Ny * d.dy;
+    const double hx = 0.5 * d.dx, hy = 0.5 * d.dy;
+    std::vector<VbpAngle> A(na);
+    for (uint32_t a = 0; a < na; a++) {
+        const float* t0 = xt + (size_t)a * nr * 4;
+        double ux0 = (double)t0[2] - t0[0], uy0 = (double)t0[3] - t0[1];
+        const double l0 = std::sqrt(ux0 * ux0 + uy0 * uy0);
+        if (!(l0 > 0.0)) return OMG_OK;
+        ux0 /= l0; uy0 /= l0;
+        const double nx = -uy0, ny = ux0;
+        std::vector<double> rho(nr);
+        for (uint32_t i = 0; i < nr; i++) {
+            const float* t = t0 + (size_t)i * 4;
+            const double xs = t[0], ys = t[1], xd = t[2], yd = t[3];
+            double ux = xd - xs, uy = yd - ys;
+            const double l = std::sqrt(ux * ux + uy * uy);
+            if (!(l > 0.0)) return OMG_OK;
+            ux /= l; uy /= l;
+            if (ux * ux0 + uy * uy0 < 1.0 - 1e-10) return OMG_OK;                 // not parallel
+            const bool s_in = xs > bx && xs < ex && ys > by && ys < ey;
+            const bool d_in = xd > bx && xd < ex && yd > by && yd < ey;
+            if (s_in || d_in) return OMG_OK;                                       // an endpoint inside the image
+            rho[i] = 0.5 * (xs + xd) * nx + 0.5 * (ys + yd) * ny;
+            // axis-parallel LORs take the reference's row/column walk: keep them off voxel boundaries
+            if (std::fabs((double)((float)xd - (float)xs)) < 1e-8 || std::fabs((double)((float)yd - (float)ys)) < 1e-8) {
+                const bool vertical = std::fabs((double)((float)xd - (float)xs)) < 1e-8;
+                const double pos = vertical ? xs : ys, b0 = vertical ? bx : by, dd = vertical ? d.dx : d.dy;
+                const double f = (pos - b0) / dd - std::floor((pos - b0) / dd);
+                if (f < 1e-3 || f > 1.0 - 1e-3) return OMG_OK;
+            }
+        }
+        const double drho = (rho[nr - 1] - rho[0]) / (double)(nr - 1);
+        if (!(std::fabs(drho) > 0.0)) return OMG_OK;
+        for (uint32_t i = 0; i < nr; i++)
+            if (std::fabs(rho[i] - (rho[0] + i * drho)) > 2e-4 * std::fabs(drho)) return OMG_OK;   // uneven bins
+        const double aux = std::fabs(ux0), auy = std::fabs(uy0);
+        const double H = hx * auy + hy * aux, Hp = std::fabs(hx * auy - hy * aux);
+        const double Lmax = std::fmin(aux > 0 ? d.dx / aux : INFINITY, auy > 0 ? d.dy / auy : INFINITY);
+        VbpAngle& v = A[a];
+        v.nx = (float)nx; v.ny = (float)ny;
+        v.rho0 = (float)rho[0]; v.drho = (float)drho; v.inv_drho = (float)(1.0 / drho);
+        v.H = (float)H; v.Hp = (float)Hp; v.Lmax = (float)Lmax;
+        v.slope = H > Hp ? (float)(Lmax / (H - Hp)) : 0.f;
+        v.pad[0] = v.pad[1] = v.pad[2] = 0.f;
+    }
+    // one plane's q plus the angle table must fit in the LDS of a workgroup
+    if ((size_t)nr * na * 4 + (size_t)na * sizeof(VbpAngle) > 150 * 1024) return OMG_OK;
+    OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->vbp_angles), (size_t)na * sizeof(VbpAngle)));
+    OMG_CHECK_HIP(hipMemcpy(g->vbp_angles, A.data(), (size_t)na * sizeof(VbpAngle), hipMemcpyHostToDevice));
+    g->vbp_ok = true;
+    return OMG_OK;
+}
+
+void vbp_release(omg_geom* g) {
+    hipFree(g->vbp_angles);
+    hipFree(g->vbp_q);
+    g->vbp_angles = nullptr;
+    g->vbp_q = nullptr;
+}
+
+// One OSEM / OSL subset iteration through k_planar_fpq + k_planar_vbp. map: slice -> plane (device,
+// filled by the caller); dU: OSL gradient or null.
+int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
+                         int32_t compute_sens, float epps, const float* dU, int32_t positivity, const int32_t* map,
+                         hipStream_t s) {
+    const uint32_t per = g->d.size_x * g->d.size_y;
+    const uint32_t n_planes = (uint32_t)(n_meas / per);
+    const size_t need = 2 * (size_t)n_meas;
+    if (g->vbp_q_floats < need) {
+        OMG_CHECK_HIP(hipDeviceSynchronize());
+        hipFree(g->vbp_q);
+        g->vbp_q = nullptr;
+        g->vbp_q_floats = 0;
+        OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->vbp_q), need * sizeof(float)));
+        g->vbp_q_floats = need;
+    }
+    float* q = g->vbp_q;
+    float* wq = g->vbp_q + n_meas;
+    SidArgs a = make_sid_args(g, first, n_meas);
+    a.pitch = g->d.Nx + 1;
+    const uint32_t groups = (per + kFpqTPB - 1) / kFpqTPB;
+    const uint32_t chunk = (per + groups - 1) / groups;
+    const size_t lds_fp = (size_t)g->d.Ny * (g->d.Nx + 1) * sizeof(float);
+    if (compute_sens) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_planar_fpq<kFpqTPB, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fp);
+        hipLaunchKernelGGL((k_planar_fpq<kFpqTPB, true>), dim3(groups * n_planes), dim3(kFpqTPB), lds_fp, s, a,
+                           n_planes, chunk, x, meas, q, wq, epps);
+    } else {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_planar_fpq<kFpqTPB, false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fp);
+        hipLaunchKernelGGL((k_planar_fpq<kFpqTPB, false>), dim3(groups * n_planes), dim3(kFpqTPB), lds_fp, s, a,
+                           n_planes, chunk, x, meas, q, wq, epps);
+    }
+    OMG_CHECK_LAUNCH();
+    const size_t lds_bp = (size_t)per * sizeof(float) + (size_t)g->d.size_y * sizeof(VbpAngle);
+    const dim3 grid((g->d.Nx * g->d.Ny + kVbpTPB * kVbpVPT - 1) / (kVbpTPB * kVbpVPT), g->d.Nz);
+    const int pos = positivity != 0;
+#define OMG_VBP_LAUNCH(SENS, OSL)                                                                              \
+    do {                                                                                                       \
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_planar_vbp<SENS, OSL>),                            \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bp);                          \
+        hipLaunchKernelGGL((k_planar_vbp<SENS, OSL>), grid, dim3(kVbpTPB), lds_bp, s, a, g->vbp_angles, map, q, \
+                           wq, x, S, dU, epps, pos);                                                           \
+    } while (0)
+    if (compute_sens && dU) OMG_VBP_LAUNCH(true, true);
+    else if (compute_sens) OMG_VBP_LAUNCH(true, false);
+    else if (dU) OMG_VBP_LAUNCH(false, true);
+    else OMG_VBP_LAUNCH(false, false);
+#undef OMG_VBP_LAUNCH
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}

@@ -102,12 +102,15 @@ def _oracle_run(p, y, n_sub, beta=0.0):
                   np.ones(int(p.N[0]), np.float32), epps=1e-5, beta=beta, gamma=2.0, subsets_to_run=n_sub)
 
 
-def _gpu_run(p, y, n_sub, beta=0.0, generic=False, unfused=False):
+def _gpu_run(p, y, n_sub, beta=0.0, generic=False, unfused=False, lds_bp=False):
     import torch
     run = OSEMRunner(p, y, beta=beta, gamma=2.0)
-    if generic:
+    if generic or lds_bp:
         d, keep = p._descriptor()
-        d.reserved[0] = 1
+        if generic:
+            d.reserved[0] = 1
+        if lds_bp:
+            d.reserved[2] = 1
         gm = Geometry(d, keep)
         run.g = gm.handle
         run._keep = gm
@@ -123,12 +126,12 @@ def _gpu_run(p, y, n_sub, beta=0.0, generic=False, unfused=False):
     return x.cpu().numpy()
 
 
-@pytest.mark.parametrize("path", ["planar", "generic", "unfused"])
+@pytest.mark.parametrize("path", ["voxel_bp", "planar_lds", "generic", "unfused"])
 def test_osem_iterations_match_oracle(pet2d, path):
     p, y = pet2d
     n_sub = 10   # all of iteration 0 (sensitivity images) and two subsets of iteration 1
     print(f"[osem] path={path} subset-iterations={n_sub}", flush=True)
-    xg = _gpu_run(p, y, n_sub, generic=path == "generic", unfused=path == "unfused")
+    xg = _gpu_run(p, y, n_sub, generic=path == "generic", unfused=path == "unfused", lds_bp=path == "planar_lds")
     xo = _oracle_run(p, y, n_sub)
     e = rel_l2(xg, xo)
     print(f"  rel-L2 vs oracle {e:.2e}")
@@ -143,6 +146,20 @@ def test_osl_osem_rdp_iterations_match_oracle(pet2d):
     print(f"[osl-osem] rel-L2 vs oracle {e:.2e}")
     assert e < TOL
     assert rel_l2(_gpu_run(p, y, 10, beta=0.1, unfused=True), xo) < TOL
+    assert rel_l2(_gpu_run(p, y, 10, beta=0.1, lds_bp=True), xo) < TOL
+
+
+def test_voxel_bp_is_deterministic_and_matches_lds_path(pet2d):
+    """The voxel-driven backprojection sums in a fixed order: two runs are bit-identical; it agrees with
+    the LDS-atomic planar path to 1e-5 after the sensitivity iteration and a later one."""
+    p, y = pet2d
+    a = _gpu_run(p, y, 9)
+    b = _gpu_run(p, y, 9)
+    assert np.array_equal(a, b)
+    c = _gpu_run(p, y, 9, lds_bp=True)
+    e = rel_l2(a, c)
+    print(f"[voxel bp] rel-L2 vs LDS-atomic path {e:.2e}")
+    assert e < TOL
 
 
 def test_full_osem_run_reference_quality(pet2d):
