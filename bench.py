@@ -47,8 +47,9 @@ def parse():
                     choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet3d_autograd"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
-    ap.add_argument("--lds-bp", action="store_true",
-                    help="pet2d_osem: use the LDS-atomic planar backprojection instead of the voxel-driven one")
+    ap.add_argument("--voxel-bp", action="store_true",
+                    help="pet2d_osem: opt into the voxel-driven planar backprojection (closed-form chords; "
+                         "matches an FP64 Siddon re-run, not the FP32 walk's lengths) instead of the LDS-atomic one")
     return ap.parse_args()
 
 
@@ -77,8 +78,8 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     p = workloads.pet2d_projector(subsets=8)
-    if args.lds_bp:
-        p.lds_backprojection = True
+    if args.voxel_bp:
+        p.voxel_backprojection = True
     p.initProj()
     ph, y = workloads.pet2d_data(p, seed=0)
     n_iter = 4
@@ -136,14 +137,14 @@ def main():
     _lib.call("omg_timing_collect", C.byref(tot), C.byref(cnt))
 
     # dominant kernel: the fused FP -> ratio -> BP projector launch(es) of iterations >= 1, timed with HIP
-    # events on their stream: k_planar_fpq + k_planar_vbp (voxel-driven BP, ring sinograms) or
-    # k_siddon_planar<640, true, true, false> (LDS-atomic BP)
-    kernel_name = "k_planar_fpq<640,false> + k_planar_vbp<false,false>" if not args.lds_bp else \
+    # events on their stream: k_siddon_planar<640, true, true, false> (LDS-atomic BP, the default) or,
+    # opted in, k_planar_fpq + k_planar_vbp (voxel-driven BP, ring sinograms)
+    kernel_name = "k_planar_fpq<640,false> + k_planar_vbp<false,false>" if args.voxel_bp else \
         "k_siddon_planar<640,true,true,false>"
     avg_s = tot.value * 1e-3 / max(cnt.value, 1)
     bytes_per_launch = float(np.mean([4.0 * np_sum[k] + 8.0 * np_nz[k] + 8.0 * M[k] for k in range(p.subsets)]))
     achieved = bytes_per_launch / avg_s / 1e9
-    traffic = read_traffic(args.traffic)
+    traffic = None if args.voxel_bp else read_traffic(args.traffic)   # the summary covers the LDS kernel
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel_name, "launches_timed": int(cnt.value),

@@ -64,9 +64,11 @@ typedef struct omg_geom_desc {
     float dPitchX;             /* transaxial detector pitch */
     float dPitchY;             /* axial detector pitch */
     /* launch tuning: reserved[0] != 0 routes all Siddon calls through the one-lane-per-LOR kernels;
-     * reserved[1] > 0 sets the workgroups per plane of the direct-plane (LDS) kernels; reserved[2] != 0
-     * disables the voxel-driven backprojection of the fused subset iteration on parallel-beam ring
-     * sinograms (the LDS-atomic planar kernel then runs). Zero = automatic. */
+     * reserved[1] > 0 sets the workgroups per plane of the direct-plane (LDS) kernels; reserved[2] == 1
+     * opts the fused subset iteration into the voxel-driven backprojection on parallel-beam ring
+     * sinograms (closed-form chord lengths: within 5e-7 of an FP64 Siddon backprojection, but not the
+     * reference's FP32 walk lengths, which are themselves ~1e-5 from FP64); geometries that do not
+     * qualify keep the LDS-atomic planar kernel. Zero = automatic (LDS-atomic). */
     int32_t reserved[8];
 } omg_geom_desc;
 

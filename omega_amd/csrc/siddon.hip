@@ -127,11 +127,45 @@ __global__ __launch_bounds__(kBlock) void k_siddon_trace(SidArgs a, int32_t* __r
 // For sinograms whose planes are direct (zs == zd), every LOR of a plane lies in one image slice.
 // A workgroup owns (plane, contiguous LOR chunk of that plane):
 //   FP  stage the slice (Nx*Ny floats) in LDS and gather from it;
-//   BP  accumulate rhs (and the sensitivity image) for the slice in LDS with ds_add_f32, then add
-//       the non-zero entries to global memory with one hardware float atomic each.
-// Global atomics drop from one per (LOR, voxel) to one per (workgroup, touched voxel).
+//   BP  accumulate the workgroup's share of rhs (and of the sensitivity image) for the slice in
+//       LDS, then store it as a partial slice; k_planar_reduce / k_planar_update sum the partial
+//       slices of a plane in workgroup order.
+// The LDS accumulator is 64-bit fixed point: on gfx950 a float LDS atomic (ds_add_f32) retires at
+// ~0.33 lanes per clock per CU even without bank conflicts, a 64-bit integer one (ds_add_u64) at
+// ~5, so the float version is bound by the atomic rate. Each term t = r * (len * w) (the float
+// product of the reference's backprojection) is added as round(t * 2^F) with F chosen per
+// workgroup so that no sum can leave +-2^62: with B = (LORs of the workgroup) * max |r w| *
+// (voxel diagonal) * (1 + 2^-10) >= any voxel's sum of |t|, F = 62 - e where B < 2^e. A term is
+// represented to 2^-F absolute (exactly when |t| 2^F >= 2^23), integer sums are exact and do not
+// depend on the order in which the atomics retire, and the one float rounding left is that of the
+// final sum. Inputs must be finite (an Inf or NaN term has no fixed-point value).
 // Block b handles plane b % n_planes: consecutive blocks go to different XCDs, and with 8 planes
 // per launch all blocks of one plane share one XCD's L2.
+
+// 2^F and 2^-F for the bound B of a workgroup's sums (F clamped to [-126, 126]).
+__device__ __forceinline__ void fixed_scale(float B, float& up, float& down) {
+    const int32_t E = (int32_t)((__float_as_uint(B) >> 23) & 0xffu);   // B < 2^(E - 126) for finite B >= 0
+    int32_t F = 62 - (E - 126);
+    F = F > 126 ? 126 : (F < -126 ? -126 : F);
+    up = __uint_as_float((uint32_t)(F + 127) << 23);
+    down = __uint_as_float((uint32_t)(127 - F) << 23);
+}
+
+// Maximum of v over the workgroup (red: TPB / 64 floats of LDS). Contains __syncthreads.
+template <int TPB>
+__device__ __forceinline__ float block_max(float v, float* red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float m = red[0];
+    for (int i = 1; i < TPB / 64; i++) m = fmaxf(m, red[i]);
+    return m;
+}
+
+__device__ __forceinline__ void lds_add_fixed(unsigned long long* p, float t, float up) {
+    atomicAdd(p, (unsigned long long)__float2ll_rn(t * up));
+}
+
 template <int TPB, bool FP, bool BP, bool SENS>
 __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_planes, uint32_t chunk,
                                                        const float* __restrict__ img, const float* __restrict__ yin,
@@ -143,8 +177,8 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
     const uint32_t l0 = grp * chunk;
     if (l0 >= per) return;
     const uint32_t l1 = min(per, l0 + chunk);
-    const uint32_t Nx = a.Nx, Nyx = a.Nyx, P = a.pitch;   // LDS rows are P = Nx + 1 floats apart
-    const uint32_t S = a.Ny * P;                          // LDS floats per slice
+    const uint32_t Nx = a.Nx, Nyx = a.Nyx, P = a.pitch;   // LDS rows are P = Nx + 1 entries apart
+    const uint32_t S = a.Ny * P;                          // LDS entries per slice
     const int64_t base = (int64_t)plane * per;
     const float zs = a.z[(a.plane_offset + plane) * 2];
     const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
@@ -154,22 +188,24 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
         return;
     }
     const size_t slice = (size_t)tk * Nyx;
-    float* acc = lds;                        // image slice (FP), then rhs accumulator (BP)
-    float* acc2 = lds + S;                   // sensitivity accumulator
-    float* R = lds + (SENS ? 2 : 1) * S;     // per-LOR ratio between the FP and BP phases
+    // LDS: [slice: S floats (FP) / S int64 (BP)][R: chunk floats][red: TPB / 64 floats]
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds);
+    float* R = lds + (BP ? 2 : 1) * S;       // per-LOR ratio between the FP and BP phases
+    float* red = R + chunk;
+    float mr = 0.f, ms = 0.f;                // this lane's max |r w| and max w
     if (FP) {
         const float4* src = reinterpret_cast<const float4*>(img + slice);
         for (uint32_t i = threadIdx.x; i < Nyx / 4; i += TPB) {
             const float4 v = src[i];
             const uint32_t e = 4 * i, yy = e / Nx, xx = e - yy * Nx;   // Nx % 4 == 0: a float4 stays in one row
-            float* d = acc + yy * P + xx;
+            float* d = lds + yy * P + xx;
             d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
         }
         __syncthreads();
         for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
             const int64_t lo = base + l;
             float ax, w = 0.f;
-            const bool hit = siddon_gather<true>(a, lo, acc, ax, w);
+            const bool hit = siddon_gather<true>(a, lo, lds, ax, w);
             const float yv = hit ? ax * w : 0.f;
             if (!BP) {
                 yout[lo] = yv;
@@ -177,32 +213,60 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
                 float r = yin[lo] / (yv + epps);
                 if (isnan(r) || isinf(r)) r = epps;
                 R[l - l0] = hit ? r : 0.f;
+                if (hit) {
+                    mr = fmaxf(mr, fabsf(r * w));
+                    ms = fmaxf(ms, w);
+                }
+            }
+        }
+    } else if (BP) {
+        // bound pass: the LOR weights from the set-up alone
+        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+            const int64_t lo = base + l;
+            Walk wk;
+            float w = 0.f;
+            if (siddon_setup<true>(a, lo, wk, w)) {
+                mr = fmaxf(mr, fabsf(yin[lo] * w));
+                ms = fmaxf(ms, w);
             }
         }
     }
     if (BP) {
-        if (FP) __syncthreads();
-        for (uint32_t i = threadIdx.x; i < S * (SENS ? 2u : 1u); i += TPB) lds[i] = 0.f;
-        __syncthreads();
-        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
-            const int64_t lo = base + l;
-            const float r = FP ? R[l - l0] : yin[lo];
-            if (!SENS && r == 0.f) continue;
-            float w = 0.f;
-            siddon_trace<true>(a, lo, w, [&](uint32_t ind, float len) {
-                const float lw = len * w;
-                atomicAdd(acc + ind, r * lw);
-                if (SENS) atomicAdd(acc2 + ind, lw);
-            });
+        const float span = sqrtf(a.dx * a.dx + a.dy * a.dy) * (float)(l1 - l0) * (1.f + 0x1p-10f);
+        float up_r, down_r, up_s = 1.f, down_s = 1.f;
+        fixed_scale(block_max<TPB>(mr, red) * span, up_r, down_r);   // (syncs: the FP slice is free after this)
+        if (SENS) {
+            __syncthreads();
+            fixed_scale(block_max<TPB>(ms, red) * span, up_s, down_s);
         }
-        __syncthreads();
-        // this workgroup's partial slice -> scratch (plain coalesced stores); k_planar_reduce sums them
-        float* part = rhs + (size_t)blockIdx.x * Nyx;
-        float* part2 = sens + (size_t)blockIdx.x * Nyx;
-        for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) {
-            const uint32_t yy = i / Nx, j = i + yy;   // padded LDS index of voxel i
-            part[i] = acc[j];
-            if (SENS) part2[i] = acc2[j];
+        for (int pass = 0; pass < (SENS ? 2 : 1); pass++) {
+            for (uint32_t i = threadIdx.x; i < S; i += TPB) acc[i] = 0ull;
+            __syncthreads();
+            for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+                const int64_t lo = base + l;
+                const float r = FP ? R[l - l0] : yin[lo];
+                if (pass == 0) {
+                    if (r == 0.f) continue;
+                    float w = 0.f;
+                    siddon_trace<true>(a, lo, w, [&](uint32_t ind, float len) {
+                        lds_add_fixed(acc + ind, r * (len * w), up_r);
+                    });
+                } else {
+                    float w = 0.f;
+                    siddon_trace<true>(a, lo, w, [&](uint32_t ind, float len) {
+                        lds_add_fixed(acc + ind, len * w, up_s);
+                    });
+                }
+            }
+            __syncthreads();
+            // this workgroup's partial slice -> scratch (plain coalesced stores)
+            float* part = (pass == 0 ? rhs : sens) + (size_t)blockIdx.x * Nyx;
+            const float down = pass == 0 ? down_r : down_s;
+            for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) {
+                const uint32_t yy = i / Nx, j = i + yy;   // padded LDS index of voxel i
+                part[i] = (float)(long long)acc[j] * down;
+            }
+            if (SENS && pass == 0) __syncthreads();
         }
     }
 }
@@ -255,7 +319,8 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
                          float* rhs, float* sens, float epps, hipStream_t s) {
     const PlanarShape sh = planar_shape(g, n);
     const uint32_t S = g->d.Ny * (g->d.Nx + 1);
-    const size_t lds = sizeof(float) * ((size_t)S * (BP ? (SENS ? 2 : 1) : 1) + ((FP && BP) ? sh.chunk : 0));
+    // [slice: S floats (FP only) or S int64 (BP)][R: chunk floats][block_max scratch]
+    const size_t lds = BP ? (size_t)S * 8 + sizeof(float) * ((size_t)sh.chunk + kPlanarTPB / 64) : sizeof(float) * (size_t)S;
     auto kern = k_siddon_planar<kPlanarTPB, FP, BP, SENS>;
     static size_t attr_bytes = 0;   // largest dynamic LDS size requested so far for this instantiation
     if (lds > attr_bytes) {

@@ -268,7 +268,7 @@ class projectorClass:
                                  "(with tube_width_z = 0 every voxel is outside the tube)")
             d.orth_width = float(self.tube_width_z)
             d.orth_flags = int(self.tube_width_xy > 0) | (int(self.tube_width_z > 0) << 1)
-        if getattr(self, "lds_backprojection", False):
+        if getattr(self, "voxel_backprojection", False):
             d.reserved[2] = 1
         d.dPitchX = float(self.dPitchX)
         d.dPitchY = float(self.dPitchY)

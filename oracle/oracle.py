@@ -212,18 +212,43 @@ def osl_update(x, S, dU, rhs, epps, positivity=True):
     return x
 
 
+def _ratio64(y, yhat, epps):
+    """oracle_ratio_f32's steps in float64: r = y / (yhat + epps), NaN/Inf -> epps."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.asarray(y, np.float64) / (np.asarray(yhat, np.float64) + epps)
+    r[~np.isfinite(r)] = epps
+    return r
+
+
+def _sens_clamp64(s, epps):
+    s = np.asarray(s, np.float64).copy()
+    s[s < epps] = epps
+    return s
+
+
+def _update64(x, S, dU, rhs, epps, positivity):
+    """oracle_osem_update_f32 (dU None) / oracle_osl_update_f32 in float64, same operation order."""
+    v = x / S * (rhs + epps) if dU is None else x / (S + dU) * rhs
+    if positivity:
+        v = np.where(v < epps, epps, v)
+    return v
+
+
 def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter, x0, epps=1e-5, beta=0.0,
-         gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None):
+         gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None, dtype=np.float32):
     """OSEM / OSL-OSEM(RDP) on the CPU in the reference's step order (see omega_amd/recon.py).
 
     ``meas`` is the full measurement vector in subset order. With ``use_ref`` the forward and
     backward projections come from the reference build (oracle/_ref), the elementwise steps from
     the oracle. ``subsets_to_run`` limits the run to the first that many subset-iterations.
-    Returns the image (float32).
+    ``dtype=np.float64`` is the FP64 re-run: the same projector and steps in double precision (not
+    with ``use_ref``). Returns the image (float32, or float64 for the FP64 re-run).
     """
+    f64 = np.dtype(dtype) == np.float64
+    assert not (f64 and use_ref), "the reference build is FP32 only"
     per = size_x * size_y
     starts = np.concatenate([[0], np.cumsum(nmeas_planes)]).astype(np.int64)
-    x = np.ascontiguousarray(x0, dtype=np.float32).copy()
+    x = np.ascontiguousarray(x0, dtype=dtype).copy()
     S = [None] * len(nmeas_planes)
     done = 0
     shape = (grid.Nx, grid.Ny, grid.Nz)
@@ -232,17 +257,19 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
             if subsets_to_run is not None and done >= subsets_to_run:
                 return x
             n = int(nmeas_planes[k]) * per
-            case = SiddonCase(grid, size_x, size_y, xy, z_subset_ordered, plane_offset=int(starts[k]))
+            case = SiddonCase(grid, size_x, size_y, xy, z_subset_ordered, plane_offset=int(starts[k]), dtype=dtype)
             yk = meas[starts[k] * per: starts[k] * per + n]
             yhat = case.ref_fp(x, n, nthreads) if use_ref else case.fp(x, n, nthreads)
-            r = ratio(yk, yhat, epps)
+            r = _ratio64(yk, yhat, epps) if f64 else ratio(yk, yhat, epps)
             if it == 0:
                 rhs, s = (case.ref_bp if use_ref else case.bp)(r, n, sens=True, nthreads=nthreads)
-                S[k] = sens_clamp(s, epps)
+                S[k] = _sens_clamp64(s, epps) if f64 else sens_clamp(s, epps)
             else:
                 rhs = (case.ref_bp if use_ref else case.bp)(r, n, nthreads=nthreads)
-            if beta != 0.0:
-                dU = rdp_gradient(x, shape, gamma, epps, beta, nthreads=nthreads)
+            dU = rdp_gradient(x, shape, gamma, epps, beta, dtype=dtype, nthreads=nthreads) if beta != 0.0 else None
+            if f64:
+                x = _update64(x, S[k], dU, rhs, epps, positivity)
+            elif dU is not None:
                 x = osl_update(x, S[k], dU, rhs, epps, positivity)
             else:
                 x = osem_update(x, S[k], rhs, epps, positivity)

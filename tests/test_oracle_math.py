@@ -144,6 +144,17 @@ def test_osem_fixed_point():
     seen = c0.bp(np.ones(int(nmeas[0]) * per, np.float32), int(nmeas[0]) * per) > 1.0
     rel = np.abs(x1[seen] - xstar[seen]) / xstar[seen]
     assert rel.max() < 1e-3 and np.median(rel) < 1e-5
+    # the FP64 re-run (y exact in float64, epsilon terms made negligible) holds x* to double rounding
+    y64 = np.concatenate(parts)
+    x64 = O.osem(grid, sc.n_radial, sc.n_angles, xy, zs, nmeas, y64, 1, xstar, epps=1e-14, subsets_to_run=1,
+                 dtype=np.float64)
+    assert x64.dtype == np.float64
+    rel64 = np.abs(x64[seen] - xstar[seen]) / xstar[seen]
+    assert rel64.max() < 1e-9
+    # with the same epsilon the FP32 run is within its rounding of the FP64 re-run
+    x64e = O.osem(grid, sc.n_radial, sc.n_angles, xy, zs, nmeas, y, 1, xstar, epps=1e-5, subsets_to_run=1,
+                  dtype=np.float64)
+    assert np.linalg.norm(x1 - x64e) / np.linalg.norm(x64e) < 1e-5
 
 
 def rdp_value(u, shape, gamma):
