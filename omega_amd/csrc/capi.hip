@@ -1,6 +1,7 @@
 // capi.hip -- geometry handles, error text and version of libomega_hip.so.
 #include "geom.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -79,6 +80,34 @@ int bdd_prepare(omg_geom* g);   // BDD per-geometry set-up (bdd.hip)
 int vbp_prepare(omg_geom* g, const float* xt);  // voxel-driven planar backprojection set-up (vbp.hip)
 int orth_prepare(omg_geom* g);  // orthogonal projector set-up (orth.hip)
 
+// True when every LOR of a sinogram's transaxial table [xs, ys, xd, yd] either misses the image or
+// enters it after t = 1e-3 and leaves it before t = 1 - 1e-3 of the segment source -> detector
+// (slab clipping in double against the float image box). Every crossing parameter of such a walk
+// then lies in [0, 1], also after the float accumulation of up to Nx + Ny crossings.
+static bool ring_table(const omg_geom_desc& d, const float* xt, float bmaxx, float bmaxy) {
+    const int64_t n = (int64_t)d.size_x * d.size_y;
+    const double lo[2] = {d.bx, d.by}, hi[2] = {bmaxx, bmaxy};
+    for (int64_t i = 0; i < n; i++) {
+        const double s[2] = {xt[4 * i], xt[4 * i + 1]}, e[2] = {xt[4 * i + 2], xt[4 * i + 3]};
+        double t0 = 0.0, t1 = 1.0;
+        bool miss = false;
+        for (int k = 0; k < 2; k++) {
+            const double dd = e[k] - s[k];
+            if (dd == 0.0) {
+                if (s[k] < lo[k] || s[k] > hi[k]) miss = true;
+                continue;
+            }
+            double ta = (lo[k] - s[k]) / dd, tb = (hi[k] - s[k]) / dd;
+            if (ta > tb) std::swap(ta, tb);
+            t0 = std::max(t0, ta);
+            t1 = std::min(t1, tb);
+        }
+        if (miss || t0 >= t1) continue;
+        if (t0 < 1e-3 || t1 > 1.0 - 1e-3) return false;
+    }
+    return true;
+}
+
 extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     OMG_REQUIRE(d && out, "null descriptor or output pointer");
     OMG_REQUIRE(d->struct_size == sizeof(omg_geom_desc), "omg_geom_desc size mismatch: caller %u, library %zu",
@@ -126,6 +155,7 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
         g->planar = true;
         for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
     }
+    if (!rc && g->planar) g->ring = ring_table(*d, d->x, g->bmaxx, g->bmaxy);
     if (!rc && d->mode == OMG_MODE_PET_SINO) rc = vbp_prepare(g, d->x);
     if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
     if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);

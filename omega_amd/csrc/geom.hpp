@@ -16,6 +16,9 @@ struct omg_geom {
     bool planar = false;
     // Route every call through the one-lane-per-LOR kernels (set from omg_geom_desc.reserved[0]).
     bool force_generic = false;
+    // Every LOR of the (planar) sinogram has both endpoints outside the image with margin, so every
+    // crossing parameter of its walk lies in [0, 1] (the planar kernels' short length expression applies).
+    bool ring = false;
     // Workgroups per plane for the planar kernels (omg_geom_desc.reserved[1]; 0 = automatic).
     int32_t planar_groups = 0;
     // Library-owned scratch (floats) for partial images; grown on demand, freed with the handle.

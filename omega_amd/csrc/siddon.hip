@@ -166,7 +166,7 @@ __device__ __forceinline__ void lds_add_fixed(unsigned long long* p, float t, fl
     atomicAdd(p, (unsigned long long)__float2ll_rn(t * up));
 }
 
-template <int TPB, bool FP, bool BP, bool SENS>
+template <int TPB, bool FP, bool BP, bool SENS, bool RING>
 __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_planes, uint32_t chunk,
                                                        const float* __restrict__ img, const float* __restrict__ yin,
                                                        float* __restrict__ yout, float* __restrict__ rhs,
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
         for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
             const int64_t lo = base + l;
             float ax, w = 0.f;
-            const bool hit = siddon_gather<true>(a, lo, lds, ax, w);
+            const bool hit = siddon_gather<true, RING>(a, lo, lds, ax, w);
             const float yv = hit ? ax * w : 0.f;
             if (!BP) {
                 yout[lo] = yv;
@@ -248,12 +248,12 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
                 if (pass == 0) {
                     if (r == 0.f) continue;
                     float w = 0.f;
-                    siddon_trace<true>(a, lo, w, [&](uint32_t ind, float len) {
+                    siddon_scatter<true, RING>(a, lo, w, [&](uint32_t ind, float len) {
                         lds_add_fixed(acc + ind, r * (len * w), up_r);
                     });
                 } else {
                     float w = 0.f;
-                    siddon_trace<true>(a, lo, w, [&](uint32_t ind, float len) {
+                    siddon_scatter<true, RING>(a, lo, w, [&](uint32_t ind, float len) {
                         lds_add_fixed(acc + ind, len * w, up_s);
                     });
                 }
@@ -321,12 +321,13 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
     const uint32_t S = g->d.Ny * (g->d.Nx + 1);
     // [slice: S floats (FP only) or S int64 (BP)][R: chunk floats][block_max scratch]
     const size_t lds = BP ? (size_t)S * 8 + sizeof(float) * ((size_t)sh.chunk + kPlanarTPB / 64) : sizeof(float) * (size_t)S;
-    auto kern = k_siddon_planar<kPlanarTPB, FP, BP, SENS>;
-    static size_t attr_bytes = 0;   // largest dynamic LDS size requested so far for this instantiation
-    if (lds > attr_bytes) {
+    const int ring = g->ring ? 1 : 0;
+    auto kern = ring ? k_siddon_planar<kPlanarTPB, FP, BP, SENS, true> : k_siddon_planar<kPlanarTPB, FP, BP, SENS, false>;
+    static size_t attr_bytes[2] = {0, 0};   // largest dynamic LDS size requested so far per instantiation
+    if (lds > attr_bytes[ring]) {
         OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_bytes = lds;
+        attr_bytes[ring] = lds;
     }
     SidArgs a = make_sid_args(g, first, n);
     a.pitch = g->d.Nx + 1;

@@ -304,7 +304,10 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
 // One Siddon step (projector_functions.h:1842-1998) without branches: the voxel the ray is in,
 // the length inside it (0 when it contributes nothing or the walk is over), then the advance to
 // the nearest plane crossing (z, then y, then x). Same float operations as the reference.
-template <bool PLANAR>
+// RING: the caller guarantees that every crossing parameter of the walk lies in [0, 1] (both
+// endpoints of the LOR outside the image, with margin); the reference's length expression then
+// reduces to (tn - tc) * L, the same float result without its range selections.
+template <bool PLANAR, bool RING = false>
 __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& ind, float& len) {
     // bitwise & / | on the predicates keep the step free of short-circuit control flow
     const bool alive = w.left > 0u;
@@ -314,14 +317,18 @@ __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& i
     const bool ymin = (!zmin) & (w.ty0 < w.tx0);
     const float tn = zmin ? w.tz0 : (ymin ? w.ty0 : w.tx0);
     const float tc = w.tc;
-    const bool tn_in = (tn >= 0.f) & (tn <= 1.f);
-    const bool tc_in = (tc >= 0.f) & (tc <= 1.f);
-    // operand of the length product, as chosen by the reference's branches
-    const float d_nc = tn - tc, d_1c = 1.f - tc;
-    const float o_in = (tc < 0.f) ? tn : d_nc;
-    const float o_out = (tn > 1.f) ? d_1c : d_nc;
-    const float l = (tn_in ? o_in : o_out) * w.L;
-    len = (alive & (tn_in | tc_in)) ? l : 0.f;
+    if (RING) {
+        len = alive ? (tn - tc) * w.L : 0.f;
+    } else {
+        const bool tn_in = (tn >= 0.f) & (tn <= 1.f);
+        const bool tc_in = (tc >= 0.f) & (tc <= 1.f);
+        // operand of the length product, as chosen by the reference's branches
+        const float d_nc = tn - tc, d_1c = 1.f - tc;
+        const float o_in = (tc < 0.f) ? tn : d_nc;
+        const float o_out = (tn > 1.f) ? d_1c : d_nc;
+        const float l = (tn_in ? o_in : o_out) * w.L;
+        len = (alive & (tn_in | tc_in)) ? l : 0.f;
+    }
     const bool sx = alive & (!zmin) & (!ymin), sy = alive & ymin, sz = alive & zmin;
     w.tc = alive ? tn : tc;
     w.ti += sx ? w.ux : 0;
@@ -375,7 +382,7 @@ __device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float
 // kBatch gathers are in flight at once. Returns false when the LOR contributes nothing.
 constexpr int kBatch = 8;
 
-template <bool PLANAR>
+template <bool PLANAR, bool RING = false>
 __device__ __forceinline__ bool siddon_gather(const SidArgs& g, int64_t lo, const float* __restrict__ src, float& ax,
                                               float& weight) {
     ax = 0.f;
@@ -389,11 +396,33 @@ __device__ __forceinline__ bool siddon_gather(const SidArgs& g, int64_t lo, cons
         uint32_t ind[kBatch];
         float len[kBatch], v[kBatch];
 #pragma unroll
-        for (int k = 0; k < kBatch; k++) walk_step<PLANAR>(g, w, ind[k], len[k]);
+        for (int k = 0; k < kBatch; k++) walk_step<PLANAR, RING>(g, w, ind[k], len[k]);
 #pragma unroll
         for (int k = 0; k < kBatch; k++) v[k] = src[len[k] > 0.f ? ind[k] : 0u];
 #pragma unroll
         for (int k = 0; k < kBatch; k++) ax = (len[k] > 0.f) ? ax + len[k] * v[k] : ax;
+    }
+    return true;
+}
+
+// Backprojection walk of LOR `lo`: visit(voxel, length) for each voxel with length > 0, kBatch steps
+// traced per loop trip (one loop test per kBatch steps). Returns false when the LOR contributes nothing.
+template <bool PLANAR, bool RING = false, typename Visit>
+__device__ __forceinline__ bool siddon_scatter(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+    Walk w;
+    if (!siddon_setup<PLANAR>(g, lo, w, weight)) return false;
+    if (w.perp) {
+        perp_walk(w, visit);
+        return true;
+    }
+    while (w.left > 0u) {
+        uint32_t ind[kBatch];
+        float len[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) walk_step<PLANAR, RING>(g, w, ind[k], len[k]);
+#pragma unroll
+        for (int k = 0; k < kBatch; k++)
+            if (len[k] > 0.f) visit(ind[k], len[k]);
     }
     return true;
 }

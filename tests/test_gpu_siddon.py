@@ -2,8 +2,9 @@
 
 Bar: forward projections and traversal lists (voxel indices, lengths) bit-identical to the
 oracle, which is itself bit-identical to the reference CPU build (test_oracle_ref.py,
-test_golden.py). Backprojections (unordered float atomics, as in the reference) within
-1e-5 relative L2 of the FP32 oracle; the FP32 error against the FP64 oracle is printed.
+test_golden.py). Backprojections (unordered float atomics in the one-lane-per-LOR kernels, as in
+the reference; exact 64-bit fixed-point sums of the float terms in the direct-plane LDS kernels)
+within 1e-5 relative L2 of the FP32 oracle; the FP32 error against the FP64 oracle is printed.
 """
 import numpy as np
 import pytest
@@ -12,7 +13,7 @@ from omega_amd import _lib
 from omega_amd import geometry as G
 from oracle import oracle as O
 from tests import cases
-from tests.hip_helpers import bp, fp, make_geom, rel_l2, trace
+from tests.hip_helpers import bp, fp, make_geom, ptr, rel_l2, trace
 
 pytestmark = pytest.mark.gpu
 
@@ -137,6 +138,44 @@ def test_adjoint_identity_c0_subset(cuda):
         lhs = float(ax @ yv.astype(np.float64))
         rhs = float(x.astype(np.float64) @ aty)
         assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+@pytest.mark.parametrize("radius", [100.0, 400.0])
+def test_planar_endpoints_inside_and_outside(cuda, radius):
+    """Direct-plane sinograms whose LOR endpoints lie inside the image (ring radius 100 mm, image
+    half-width 80 mm: the planar kernels keep the reference's full length expression) and outside it
+    (400 mm: every crossing parameter is in [0, 1] and the short expression applies). FP and the
+    traversal lists bit-identical to the oracle either way; BP, sensitivity and the fused
+    FP -> ratio -> BP step within 1e-5."""
+    import torch
+    grid = G.Grid.from_fov(64, 64, 4, 160.0, 160.0, 10.0)
+    sc = G.pet_2d_planes(grid, radius=radius, n_radial=64, n_angles=48, radial_pitch=2.5)
+    xy = sc.xy_table()
+    per = sc.n_radial * sc.n_angles
+    n = 4 * per
+    print(f"[planar] radius={radius} LORs={n}", flush=True)
+    case = O.SiddonCase(grid, sc.n_radial, sc.n_angles, xy, sc.z)
+    geom = make_geom(grid, xy, sc.z, sc.n_radial, sc.n_angles)
+    img = _img(grid.N, 31)
+    y = fp(geom, 0, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+    assert np.array_equal(y, case.fp(img, n))
+    c_g, i_g, l_g = trace(geom, 0, n, cuda)
+    c_o, i_o, l_o, _ = case.trace(n)
+    assert np.array_equal(c_g, c_o) and np.array_equal(i_g, i_o) and np.array_equal(l_g, l_o)
+    yin = _img(n, 32)
+    rhs, sens = bp(geom, 0, n, torch.from_numpy(yin).to(cuda), grid.N, sens=True)
+    rhs_o, sens_o = case.bp(yin, n, sens=True)
+    assert rel_l2(rhs.cpu().numpy(), rhs_o) < BP_TOL
+    assert rel_l2(sens.cpu().numpy(), sens_o) < BP_TOL
+    meas = (yin * 3).astype(np.float32)
+    rhs_t = torch.zeros(grid.N, device=cuda)
+    sens_t = torch.zeros(grid.N, device=cuda)
+    img_t, meas_t = torch.from_numpy(img).to(cuda), torch.from_numpy(meas).to(cuda)
+    _lib.call("omg_osem_data_step", geom.handle, 0, n, ptr(img_t), ptr(meas_t), ptr(rhs_t), ptr(sens_t), 1e-5, None)
+    r_o = O.ratio(meas, case.fp(img, n), 1e-5)
+    rhs_o, sens_o = case.bp(r_o, n, sens=True)
+    assert rel_l2(rhs_t.cpu().numpy(), rhs_o) < BP_TOL
+    assert rel_l2(sens_t.cpu().numpy(), sens_o) < BP_TOL
 
 
 def test_norm_and_global_factor(cuda):
