@@ -398,6 +398,72 @@ __global__ __launch_bounds__(256) void k_planar_update(uint32_t Nyx, uint32_t N,
     x[n] = xn;
 }
 
+// Sum over workgroups g = 0, 1, ... of plane p's partial slices at float4 index v, in that order;
+// eight loads in flight per lane.
+__device__ __forceinline__ float4 sum_partials(const float4* __restrict__ part, uint32_t groups, uint32_t n_planes,
+                                               uint32_t p, uint32_t Nyx4, uint32_t v) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t g = 0;
+    for (; g + 8 <= groups; g += 8) {
+        float4 t[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) t[k] = part[(size_t)((g + k) * n_planes + p) * Nyx4 + v];
+#pragma unroll
+        for (int k = 0; k < 8; k++) { acc.x += t[k].x; acc.y += t[k].y; acc.z += t[k].z; acc.w += t[k].w; }
+    }
+    for (; g < groups; g++) {
+        const float4 t = part[(size_t)(g * n_planes + p) * Nyx4 + v];
+        acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+    }
+    return acc;
+}
+
+// The same update four voxels per lane (float4 loads and stores; the caller checks 16-byte alignment
+// and Nx % 4 == 0). grid = (ceil(Nyx / 4 / 256), Nz): the slice's launch plane is found per workgroup
+// from the z table, so no slice -> plane map is built.
+template <bool SENS, bool OSL>
+__global__ __launch_bounds__(256) void k_planar_update4(SidArgs a, uint32_t n_planes, uint32_t groups,
+                                                        const float4* __restrict__ part, const float4* __restrict__ part2,
+                                                        float4* __restrict__ x, float4* __restrict__ S,
+                                                        const float4* __restrict__ dU, float epps, int pos) {
+    const uint32_t Nyx4 = a.Nyx / 4;
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t z = blockIdx.y;
+    int32_t p = -1;
+    for (uint32_t q = 0; q < n_planes; q++) {
+        const float zs = a.z[(a.plane_offset + q) * 2];
+        if ((int32_t)(fabsf(zs - a.bz) / a.dz) == (int32_t)z) { p = (int32_t)q; break; }
+    }
+    if (v >= Nyx4) return;
+    const size_t n = (size_t)z * Nyx4 + v;
+    float4 rhs = make_float4(0.f, 0.f, 0.f, 0.f), s;
+    if (p >= 0) rhs = sum_partials(part, groups, n_planes, (uint32_t)p, Nyx4, v);
+    if (SENS) {
+        s = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p >= 0) s = sum_partials(part2, groups, n_planes, (uint32_t)p, Nyx4, v);
+        s.x = s.x < epps ? epps : s.x; s.y = s.y < epps ? epps : s.y;
+        s.z = s.z < epps ? epps : s.z; s.w = s.w < epps ? epps : s.w;
+        S[n] = s;
+    } else {
+        s = S[n];
+    }
+    const float4 xo = x[n];
+    float4 xn;
+    if (OSL) {
+        const float4 d = dU[n];
+        xn.x = xo.x / (s.x + d.x) * rhs.x; xn.y = xo.y / (s.y + d.y) * rhs.y;
+        xn.z = xo.z / (s.z + d.z) * rhs.z; xn.w = xo.w / (s.w + d.w) * rhs.w;
+    } else {
+        xn.x = xo.x / s.x * (rhs.x + epps); xn.y = xo.y / s.y * (rhs.y + epps);
+        xn.z = xo.z / s.z * (rhs.z + epps); xn.w = xo.w / s.w * (rhs.w + epps);
+    }
+    if (pos) {
+        xn.x = xn.x < epps ? epps : xn.x; xn.y = xn.y < epps ? epps : xn.y;
+        xn.z = xn.z < epps ? epps : xn.z; xn.w = xn.w < epps ? epps : xn.w;
+    }
+    x[n] = xn;
+}
+
 }  // namespace omg
 
 using namespace omg;
@@ -604,13 +670,37 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
         else rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
         if (rc) return rc;
         const uint32_t Nyx = g->d.Nx * g->d.Ny;
-        int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
-        SidArgs a = make_sid_args(g, first, n_meas);
-        hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a, sh.n_planes, map);
-        OMG_CHECK_LAUNCH();
         const float* part = g->scratch;
         const float* part2 = g->scratch + (size_t)sh.blocks * Nyx;
         const uint32_t groups = sh.blocks / sh.n_planes;
+        SidArgs a = make_sid_args(g, first, n_meas);
+        auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+        if (al16(x) && al16(S) && (!dU || al16(dU)) && al16(part2)) {
+            // Nx % 4 == 0 on the planar path, so Nyx % 4 == 0 and the partial slices stay 16-byte aligned
+            const dim3 g4(grid_for(Nyx / 4, 256), g->d.Nz);
+            const float4* p4 = reinterpret_cast<const float4*>(part);
+            const float4* q4 = reinterpret_cast<const float4*>(part2);
+            float4* x4 = reinterpret_cast<float4*>(x);
+            float4* S4 = reinterpret_cast<float4*>(S);
+            const float4* d4 = reinterpret_cast<const float4*>(dU);
+            if (compute_sens && dU)
+                hipLaunchKernelGGL((k_planar_update4<true, true>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4, x4,
+                                   S4, d4, epps, (int)positivity);
+            else if (compute_sens)
+                hipLaunchKernelGGL((k_planar_update4<true, false>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
+                                   x4, S4, d4, epps, (int)positivity);
+            else if (dU)
+                hipLaunchKernelGGL((k_planar_update4<false, true>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
+                                   x4, S4, d4, epps, (int)positivity);
+            else
+                hipLaunchKernelGGL((k_planar_update4<false, false>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
+                                   x4, S4, d4, epps, (int)positivity);
+            OMG_CHECK_LAUNCH();
+            return OMG_OK;
+        }
+        int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
+        hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a, sh.n_planes, map);
+        OMG_CHECK_LAUNCH();
         dim3 grid(grid_for(N, 256));
         if (compute_sens && dU)
             hipLaunchKernelGGL((k_planar_update<true, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
