@@ -189,6 +189,7 @@ __device__ __forceinline__ int perp_start(float b, float d, float dd, uint32_t N
 struct Walk {
     float tx0, ty0, tz0, txu, tyu, tzu, tc, L;
     int32_t ti, tj, tk, ux, uy, uz;
+    uint32_t pos, uyp;             // PLANAR: LDS index tj * pitch + ti, kept incrementally; uy * pitch
     uint32_t left;                 // Siddon steps still to take (0 = finished)
     bool perp;
     uint32_t p_ind, p_stride, p_a1, p_a2, p_nrun, p_lim;
@@ -297,6 +298,10 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
     weight = make_weight(g, lo, L, ct);
     w.tx0 = tx0; w.ty0 = ty0; w.tz0 = tz0; w.txu = txu; w.tyu = tyu; w.tzu = tzu; w.tc = tc;
     w.ti = ti; w.tj = tj; w.tk = tk; w.ux = ux; w.uy = uy; w.uz = uz;
+    if (PLANAR) {
+        w.pos = (uint32_t)tj * g.pitch + (uint32_t)ti;
+        w.uyp = (uint32_t)(uy * (int32_t)g.pitch);
+    }
     w.left = Np;
     return true;
 }
@@ -311,8 +316,7 @@ template <bool PLANAR, bool RING = false>
 __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& ind, float& len) {
     // bitwise & / | on the predicates keep the step free of short-circuit control flow
     const bool alive = w.left > 0u;
-    ind = PLANAR ? (uint32_t)w.tj * g.pitch + (uint32_t)w.ti
-                 : (uint32_t)w.tj * g.Nx + (uint32_t)w.ti + (uint32_t)w.tk * g.Nyx;
+    ind = PLANAR ? w.pos : (uint32_t)w.tj * g.Nx + (uint32_t)w.ti + (uint32_t)w.tk * g.Nyx;
     const bool zmin = !PLANAR && ((w.tz0 < w.ty0) & (w.tz0 < w.tx0));
     const bool ymin = (!zmin) & (w.ty0 < w.tx0);
     const float tn = zmin ? w.tz0 : (ymin ? w.ty0 : w.tx0);
@@ -335,6 +339,7 @@ __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& i
     w.tx0 = sx ? w.tx0 + w.txu : w.tx0;
     w.tj += sy ? w.uy : 0;
     w.ty0 = sy ? w.ty0 + w.tyu : w.ty0;
+    if (PLANAR) w.pos += sx ? (uint32_t)w.ux : (sy ? w.uyp : 0u);
     if (!PLANAR) {
         w.tk += sz ? w.uz : 0;
         w.tz0 = sz ? w.tz0 + w.tzu : w.tz0;
