@@ -19,6 +19,8 @@ struct omg_geom {
     // Every LOR of the (planar) sinogram has both endpoints outside the image with margin, so every
     // crossing parameter of its walk lies in [0, 1] (the planar kernels' short length expression applies).
     bool ring = false;
+    // Orthogonal projector: direct-plane sinogram whose tubes stay in their slice (orth.hip planar kernels).
+    bool orth_planar = false;
     // Workgroups per plane for the planar kernels (omg_geom_desc.reserved[1]; 0 = automatic).
     int32_t planar_groups = 0;
     // Library-owned scratch (floats) for partial images; grown on demand, freed with the handle.
@@ -72,5 +74,22 @@ int scratch_reserve(omg_geom* g, size_t floats);
 
 // Ensures g->work holds the fused-iteration buffers for an image of N voxels.
 int work_reserve(omg_geom* g, uint32_t N);
+
+// ---- direct-plane (planar) launches, shared by the Siddon and orthogonal projectors (siddon.hip) ----
+// Workgroups of a planar launch over [first, first + n): n_planes planes, `chunk` LORs per workgroup,
+// `blocks` = n_planes * groups workgroups; block b owns plane b % n_planes, LORs from (b / n_planes) * chunk.
+struct PlanarShape { uint32_t n_planes, chunk, blocks; };
+PlanarShape planar_shape(const omg_geom* g, int64_t n);
+constexpr int kPlanarTPB = 640;   // lanes per planar workgroup (10 wave64)
+// Adds the partial slices in g->scratch into rhs (and sens when non-null).
+int planar_reduce_partials(const omg_geom* g, int64_t first, int64_t n, float* rhs, float* sens, hipStream_t s);
+// EM / OSL update of x from the partial slices of a fused planar launch (needs work_reserve).
+int planar_update(const omg_geom* g, int64_t first, int64_t n_meas, float* x, float* S, const float* dU,
+                  int32_t compute_sens, float epps, int32_t positivity, hipStream_t s);
+// Kernel timing hooks (omg_timing_enable / omg_timing_collect): HIP events around one launch.
+void timing_begin(hipStream_t s, hipEvent_t* stop);
+void timing_end(hipStream_t s, hipEvent_t stop);
+// True when no two planes of [first, first + n_planes) lie in the same image slice.
+bool unique_slices(const omg_geom* g, int64_t first, uint32_t n_planes);
 
 }  // namespace omg

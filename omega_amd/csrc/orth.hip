@@ -461,6 +461,116 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_trace(OrthArgs oa, int32_t*
     counts[lo] = c;
 }
 
+// ---- direct-plane (planar) kernels ---------------------------------------------------------------
+// For direct-plane sinograms (zs == zd) whose tube cannot reach a neighbouring slice -- transaxial sweep
+// only, or an axial sweep whose neighbouring slice centres all lie farther than the 0.99 * width
+// cut-off from the plane (checked on the host, orth_prepare) -- every tube voxel of a LOR lies in the
+// LOR's slice tk = (int)(|zs - bz| / dz), the slice of orth_lor's in-plane branch. The Siddon planar
+// design then applies: a workgroup owns (plane, chunk of LORs); the FP gathers from the slice staged
+// in LDS (same visit order: bit-identical to k_orth_fp); the BP accumulates t = r * (w * temp) in a
+// 64-bit fixed-point LDS slice (the scale bound uses w <= 1 and, as margin, up to four visits of a
+// voxel per LOR) and stores a partial slice, reduced by the shared planar reduce / update.
+template <int TPB, bool FP, bool BP, bool SENS>
+__global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_planes, uint32_t chunk,
+                                                     const float* __restrict__ img, const float* __restrict__ yin,
+                                                     float* __restrict__ yout, float* __restrict__ rhs,
+                                                     float* __restrict__ sens, float epps) {
+    extern __shared__ float lds[];
+    const SidArgs& a = oa.a;
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    const uint32_t per = a.size_x * a.size_y;
+    const uint32_t l0 = grp * chunk;
+    if (l0 >= per) return;
+    const uint32_t l1 = min(per, l0 + chunk);
+    const uint32_t Nyx = a.Nyx;
+    const int64_t base = (int64_t)plane * per;
+    const float zs = a.z[(a.plane_offset + plane) * 2];
+    const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+    if (tk < 0 || (uint32_t)tk >= a.Nz) {   // every LOR of this plane misses the image
+        if (FP && !BP)
+            for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) yout[base + l] = 0.f;
+        return;
+    }
+    const int64_t sbase = (int64_t)tk * Nyx;
+    // LDS: [slice: Nyx floats (FP) / Nyx int64 (BP)][R: chunk floats][red: TPB / 64 floats]
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds);
+    float* R = lds + (BP ? 2 : 1) * Nyx;
+    float* red = R + chunk;
+    float mr = 0.f, ms = 0.f;   // this lane's max |r temp| and max temp
+    if (FP) {
+        const float4* src = reinterpret_cast<const float4*>(img + sbase);
+        float4* dst = reinterpret_cast<float4*>(lds);
+        for (uint32_t i = threadIdx.x; i < Nyx / 4; i += TPB) dst[i] = src[i];
+        __syncthreads();
+        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+            const int64_t lo = base + l;
+            float ax = 0.f;
+            const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) {
+                const uint64_t j = (uint64_t)(ind - sbase);
+                if (j < Nyx) ax += w * lds[j];
+            });
+            if (!BP) {
+                yout[lo] = ax * temp;
+            } else {
+                float r = 0.f;
+                if (temp != 0.f) {
+                    r = yin[lo] / (ax * temp + epps);
+                    if (isnan(r) || isinf(r)) r = epps;
+                    mr = fmaxf(mr, fabsf(r * temp));
+                    ms = fmaxf(ms, fabsf(temp));
+                }
+                R[l - l0] = r;
+            }
+        }
+    } else if (BP) {
+        for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+            const int64_t lo = base + l;
+            float temp = 1.f;
+            if (a.norm) temp *= a.norm[lo];
+            temp *= a.global_factor;
+            mr = fmaxf(mr, fabsf(yin[lo] * temp));
+            ms = fmaxf(ms, fabsf(temp));
+        }
+    }
+    if (BP) {
+        const float span = 4.f * (float)(l1 - l0) * (1.f + 0x1p-10f);
+        float up_r, down_r, up_s = 1.f, down_s = 1.f;
+        fixed_scale(block_max<TPB>(mr, red) * span, up_r, down_r);
+        if (SENS) {
+            __syncthreads();
+            fixed_scale(block_max<TPB>(ms, red) * span, up_s, down_s);
+        }
+        for (int pass = 0; pass < (SENS ? 2 : 1); pass++) {
+            for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) acc[i] = 0ull;
+            __syncthreads();
+            for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+                const int64_t lo = base + l;
+                const float r = FP ? R[l - l0] : yin[lo];
+                if (pass == 0 && r == 0.f) continue;
+                float temp = 1.f;
+                if (a.norm) temp *= a.norm[lo];
+                temp *= a.global_factor;
+                if (pass == 0) {
+                    orth_lor(oa, lo, [&](int64_t ind, float w) {
+                        const uint64_t j = (uint64_t)(ind - sbase);
+                        if (j < Nyx) lds_add_fixed(acc + j, r * (w * temp), up_r);
+                    });
+                } else {
+                    orth_lor(oa, lo, [&](int64_t ind, float w) {
+                        const uint64_t j = (uint64_t)(ind - sbase);
+                        if (j < Nyx) lds_add_fixed(acc + j, w * temp, up_s);
+                    });
+                }
+            }
+            __syncthreads();
+            float* part = (pass == 0 ? rhs : sens) + (size_t)blockIdx.x * Nyx;
+            const float down = pass == 0 ? down_r : down_s;
+            for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) part[i] = (float)(long long)acc[i] * down;
+            if (SENS && pass == 0) __syncthreads();
+        }
+    }
+}
+
 static OrthArgs make_orth_args(const omg_geom* g, int64_t first, int64_t n) {
     OrthArgs oa;
     oa.a = make_sid_args(g, first, n);
@@ -478,6 +588,72 @@ static OrthArgs make_orth_args(const omg_geom* g, int64_t first, int64_t n) {
 
 using namespace omg;
 
+// The planar kernels apply when the sinogram is direct-plane, the slice fits the LDS accumulator and,
+// with the axial sweep, the centres of the slices next to every plane's slice lie at least
+// 0.99 * width * (1 + 1e-3) from the plane, so no voxel of another slice passes the tube cut-off
+// (a voxel's distance to an in-plane LOR is at least its axial offset).
+static bool orth_planar_check(const omg_geom* g) {
+    const omg_geom_desc& d = g->d;
+    if (!g->planar || d.mode != OMG_MODE_PET_SINO || d.Nx % 4 != 0) return false;
+    const size_t Nyx = (size_t)d.Nx * d.Ny;
+    if (Nyx * 8 + sizeof(float) * (kPlanarTPB + kPlanarTPB / 64) > 160 * 1024) return false;
+    if (!(d.orth_flags & 2)) return true;
+    const double cut = (1.0 - (double)kOrthThr) * (double)d.orth_width * (1.0 + 1e-3);
+    for (uint32_t p = 0; p < d.n_proj; p++) {
+        const float zs = g->z_host[2 * (size_t)p];
+        const int32_t tk = (int32_t)(fabsf(zs - d.bz) / d.dz);
+        if (tk < 0 || (uint32_t)tk >= d.Nz) continue;
+        for (int k = tk - 1; k <= tk + 1; k += 2) {
+            if (k < 0 || (uint32_t)k >= d.Nz) continue;
+            const float c = d.bz + (float)k * d.dz + d.dz / 2.f;   // the kernel's slice centre
+            if (std::fabs((double)c - (double)zs) < cut) return false;
+        }
+    }
+    return true;
+}
+
+template <bool FP, bool BP, bool SENS>
+static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* yin,
+                              float* yout, float* rhs, float* sens, float epps, hipStream_t s) {
+    const PlanarShape sh = planar_shape(g, n);
+    const uint32_t Nyx = g->d.Nx * g->d.Ny;
+    const size_t lds = BP ? (size_t)Nyx * 8 + sizeof(float) * ((size_t)sh.chunk + kPlanarTPB / 64)
+                          : sizeof(float) * (size_t)Nyx;
+    auto kern = k_orth_planar<kPlanarTPB, FP, BP, SENS>;
+    static size_t attr_bytes = 0;
+    if (lds > attr_bytes) {
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_bytes = lds;
+    }
+    const OrthArgs oa = make_orth_args(g, first, n);
+    float* part = nullptr;
+    float* part2 = nullptr;
+    if (BP) {
+        const int rc = scratch_reserve(const_cast<omg_geom*>(g), (size_t)sh.blocks * Nyx * (SENS ? 2 : 1));
+        if (rc) return rc;
+        part = g->scratch;
+        part2 = SENS ? g->scratch + (size_t)sh.blocks * Nyx : nullptr;
+    }
+    hipEvent_t stop;
+    timing_begin(s, &stop);
+    hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, oa, sh.n_planes, sh.chunk, img, yin, yout,
+                       part, part2, epps);
+    timing_end(s, stop);
+    OMG_CHECK_LAUNCH();
+    if (BP && rhs) return planar_reduce_partials(g, first, n, rhs, SENS ? sens : nullptr, s);
+    return OMG_OK;
+}
+
+// Fused planar FP -> ratio -> BP of [first, first + n) into the partial slices of g->scratch (the caller
+// applies planar_update). Returns OMG_ERR_ARG when the planar kernels do not apply.
+int orth_planar_fused(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas,
+                      int32_t compute_sens, float epps, hipStream_t s) {
+    OMG_REQUIRE(g->orth_planar, "the orthogonal planar kernels do not apply to this geometry");
+    if (compute_sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, nullptr, nullptr, epps, s);
+    return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, nullptr, nullptr, epps, s);
+}
+
 int orth_prepare(omg_geom* g) {
     const omg_geom_desc& d = g->d;
     OMG_REQUIRE(d.mode == OMG_MODE_PET_SINO || d.mode == OMG_MODE_LIST6,
@@ -486,12 +662,15 @@ int orth_prepare(omg_geom* g) {
                               "indexes rows with the other axis' extent)");
     OMG_REQUIRE(d.orth_width > 0.f && std::isfinite(d.orth_width), "orth_width must be a positive tube width");
     OMG_REQUIRE((d.orth_flags & ~3) == 0, "orth_flags holds bit 0 (transaxial sweep) and bit 1 (axial sweep) only");
+    g->orth_planar = orth_planar_check(g);
     return OMG_OK;
 }
 
 void orth_release(omg_geom*) {}
 
 int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
+    if (g->orth_planar && !g->force_generic)
+        return orth_launch_planar<true, false, false>(g, first, n, img, nullptr, y, nullptr, nullptr, 0.f, s);
     const OrthArgs oa = make_orth_args(g, first, n);
     hipLaunchKernelGGL(k_orth_fp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, y);
     OMG_CHECK_LAUNCH();
@@ -500,6 +679,10 @@ int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, 
 
 int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
                   hipStream_t s) {
+    if (g->orth_planar && !g->force_generic) {
+        if (sens) return orth_launch_planar<false, true, true>(g, first, n, nullptr, yin, nullptr, rhs, sens, 0.f, s);
+        return orth_launch_planar<false, true, false>(g, first, n, nullptr, yin, nullptr, rhs, nullptr, 0.f, s);
+    }
     const OrthArgs oa = make_orth_args(g, first, n);
     hipLaunchKernelGGL(k_orth_bp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, yin, rhs, sens);
     OMG_CHECK_LAUNCH();
@@ -508,6 +691,10 @@ int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin,
 
 int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
               float* sens, float epps, hipStream_t s) {
+    if (g->orth_planar && !g->force_generic) {
+        if (sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, rhs, sens, epps, s);
+        return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, rhs, nullptr, epps, s);
+    }
     const OrthArgs oa = make_orth_args(g, first, n);
     hipLaunchKernelGGL(k_orth_osem, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, meas, rhs, sens,
                        epps);

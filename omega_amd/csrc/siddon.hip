@@ -27,7 +27,7 @@ struct TimingState {
 };
 static TimingState g_timing;
 
-static void timing_begin(hipStream_t s, hipEvent_t* stop) {
+void timing_begin(hipStream_t s, hipEvent_t* stop) {
     *stop = nullptr;
     if (!g_timing.on) return;
     if (g_timing.used == g_timing.ev.size()) {
@@ -50,7 +50,7 @@ static void timing_begin(hipStream_t s, hipEvent_t* stop) {
     *stop = pr.second;
 }
 
-static void timing_end(hipStream_t s, hipEvent_t stop) {
+void timing_end(hipStream_t s, hipEvent_t stop) {
     if (stop) hipEventRecord(stop, s);
 }
 
@@ -141,30 +141,6 @@ __global__ __launch_bounds__(kBlock) void k_siddon_trace(SidArgs a, int32_t* __r
 // final sum. Inputs must be finite (an Inf or NaN term has no fixed-point value).
 // Block b handles plane b % n_planes: consecutive blocks go to different XCDs, and with 8 planes
 // per launch all blocks of one plane share one XCD's L2.
-
-// 2^F and 2^-F for the bound B of a workgroup's sums (F clamped to [-126, 126]).
-__device__ __forceinline__ void fixed_scale(float B, float& up, float& down) {
-    const int32_t E = (int32_t)((__float_as_uint(B) >> 23) & 0xffu);   // B < 2^(E - 126) for finite B >= 0
-    int32_t F = 62 - (E - 126);
-    F = F > 126 ? 126 : (F < -126 ? -126 : F);
-    up = __uint_as_float((uint32_t)(F + 127) << 23);
-    down = __uint_as_float((uint32_t)(127 - F) << 23);
-}
-
-// Maximum of v over the workgroup (red: TPB / 64 floats of LDS). Contains __syncthreads.
-template <int TPB>
-__device__ __forceinline__ float block_max(float v, float* red) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float m = red[0];
-    for (int i = 1; i < TPB / 64; i++) m = fmaxf(m, red[i]);
-    return m;
-}
-
-__device__ __forceinline__ void lds_add_fixed(unsigned long long* p, float t, float up) {
-    atomicAdd(p, (unsigned long long)__float2ll_rn(t * up));
-}
 
 template <int TPB, bool FP, bool BP, bool SENS, bool RING>
 __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_planes, uint32_t chunk,
@@ -272,9 +248,8 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
 }
 
 // Launch geometry of the planar kernels: one LOR per lane; a plane's LORs are split into
-// workgroups of up to kPlanarTPB lanes (640 = 10 wave64: with the 2-slice LDS footprint one
-// workgroup fills a CU).
-constexpr int kPlanarTPB = 640;
+// workgroups of up to kPlanarTPB lanes (640 = 10 wave64; with the 132 KB int64 accumulator of a
+// 128 x 128 slice one workgroup fills a CU).
 
 // Sum of the per-workgroup partial slices of each plane, in workgroup order, added to the
 // plane's image slice. Workgroup b = grp * n_planes + plane wrote partial slot b.
@@ -292,9 +267,7 @@ __global__ __launch_bounds__(256) void k_planar_reduce(SidArgs a, uint32_t n_pla
     if (s != 0.f) unsafeAtomicAdd(out + (size_t)tk * Nyx + v, s);   // atomic: two planes may share a slice
 }
 
-struct PlanarShape { uint32_t n_planes, chunk, blocks; };
-
-static PlanarShape planar_shape(const omg_geom* g, int64_t n) {
+PlanarShape planar_shape(const omg_geom* g, int64_t n) {
     const uint32_t per = g->d.size_x * g->d.size_y;
     PlanarShape s;
     s.n_planes = (uint32_t)(n / per);
@@ -307,6 +280,24 @@ static PlanarShape planar_shape(const omg_geom* g, int64_t n) {
     groups = (per + s.chunk - 1) / s.chunk;
     s.blocks = s.n_planes * groups;
     return s;
+}
+
+// Adds the partial slices in g->scratch (layout of planar_shape(g, n): slot b = group * n_planes + plane;
+// sensitivity partials after all rhs partials) of launch [first, first + n) into rhs (and sens).
+int planar_reduce_partials(const omg_geom* g, int64_t first, int64_t n, float* rhs, float* sens, hipStream_t s) {
+    const PlanarShape sh = planar_shape(g, n);
+    const uint32_t Nyx = g->d.Nx * g->d.Ny;
+    const uint32_t groups = sh.blocks / sh.n_planes;
+    const SidArgs a = make_sid_args(g, first, n);
+    const dim3 rg(grid_for(Nyx, 256), sh.n_planes);
+    hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, g->scratch, rhs);
+    OMG_CHECK_LAUNCH();
+    if (sens) {
+        hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups,
+                           g->scratch + (size_t)sh.blocks * Nyx, sens);
+        OMG_CHECK_LAUNCH();
+    }
+    return OMG_OK;
 }
 
 // The planar path applies to direct-plane PET sinograms with a square slice that fits twice in LDS.
@@ -348,16 +339,7 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
     timing_end(s, stop);
     OMG_CHECK_LAUNCH();
     if (BP && !rhs) return OMG_OK;   // the caller reduces the partial slices itself
-    if (BP) {
-        const uint32_t groups = sh.blocks / sh.n_planes;
-        dim3 rg(grid_for(Nyx, 256), sh.n_planes);
-        hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, part, rhs);
-        OMG_CHECK_LAUNCH();
-        if (SENS) {
-            hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, part2, sens);
-            OMG_CHECK_LAUNCH();
-        }
-    }
+    if (BP) return planar_reduce_partials(g, first, n, rhs, SENS ? sens : nullptr, s);
     return OMG_OK;
 }
 
@@ -510,6 +492,8 @@ int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, con
               float* sens, float epps, hipStream_t s);
 int orth_trace(const omg_geom* g, int64_t first, int64_t n, int32_t* counts, const int64_t* offsets, uint32_t* idx,
                float* w, hipStream_t s);
+int orth_planar_fused(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas,
+                      int32_t compute_sens, float epps, hipStream_t s);
 int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
                          int32_t compute_sens, float epps, const float* dU, int32_t positivity, const int32_t* map,
                          hipStream_t s);
@@ -614,7 +598,64 @@ __global__ __launch_bounds__(256) void k_generic_update(uint32_t N, float* __res
 }
 
 // True when the planes [first, first + n_planes) map to distinct image slices.
-static bool unique_slices(const omg_geom* g, int64_t first, uint32_t n_planes) {
+// EM / OSL update of image x from the partial slices of a planar fused launch [first, first + n):
+// k_planar_update4 when x, S, dU and the partials are 16-byte aligned, else the slice map and
+// k_planar_update. gm->work must hold the fused-iteration buffers (work_reserve).
+int planar_update(const omg_geom* g, int64_t first, int64_t n_meas, float* x, float* S, const float* dU,
+                  int32_t compute_sens, float epps, int32_t positivity, hipStream_t s) {
+    omg_geom* gm = const_cast<omg_geom*>(g);
+    const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+    const PlanarShape sh = planar_shape(g, n_meas);
+    const uint32_t Nyx = g->d.Nx * g->d.Ny;
+    const float* part = g->scratch;
+    const float* part2 = g->scratch + (size_t)sh.blocks * Nyx;
+    const uint32_t groups = sh.blocks / sh.n_planes;
+    SidArgs a = make_sid_args(g, first, n_meas);
+    auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+    if (al16(x) && al16(S) && (!dU || al16(dU)) && al16(part2)) {
+        // Nx % 4 == 0 on the planar path, so Nyx % 4 == 0 and the partial slices stay 16-byte aligned
+        const dim3 g4(grid_for(Nyx / 4, 256), g->d.Nz);
+        const float4* p4 = reinterpret_cast<const float4*>(part);
+        const float4* q4 = reinterpret_cast<const float4*>(part2);
+        float4* x4 = reinterpret_cast<float4*>(x);
+        float4* S4 = reinterpret_cast<float4*>(S);
+        const float4* d4 = reinterpret_cast<const float4*>(dU);
+        if (compute_sens && dU)
+            hipLaunchKernelGGL((k_planar_update4<true, true>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4, x4,
+                               S4, d4, epps, (int)positivity);
+        else if (compute_sens)
+            hipLaunchKernelGGL((k_planar_update4<true, false>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
+                               x4, S4, d4, epps, (int)positivity);
+        else if (dU)
+            hipLaunchKernelGGL((k_planar_update4<false, true>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
+                               x4, S4, d4, epps, (int)positivity);
+        else
+            hipLaunchKernelGGL((k_planar_update4<false, false>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
+                               x4, S4, d4, epps, (int)positivity);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
+    int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
+    hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a, sh.n_planes, map);
+    OMG_CHECK_LAUNCH();
+    dim3 grid(grid_for(N, 256));
+    if (compute_sens && dU)
+        hipLaunchKernelGGL((k_planar_update<true, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                           part2, x, S, dU, epps, (int)positivity);
+    else if (compute_sens)
+        hipLaunchKernelGGL((k_planar_update<true, false>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                           part2, x, S, dU, epps, (int)positivity);
+    else if (dU)
+        hipLaunchKernelGGL((k_planar_update<false, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                           part2, x, S, dU, epps, (int)positivity);
+    else
+        hipLaunchKernelGGL((k_planar_update<false, false>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
+                           part2, x, S, dU, epps, (int)positivity);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+bool unique_slices(const omg_geom* g, int64_t first, uint32_t n_planes) {
     std::vector<char> seen(g->d.Nz, 0);
     for (uint32_t p = 0; p < n_planes; p++) {
         const float zs = g->z_host[(size_t)(first + p) * 2];
@@ -665,57 +706,16 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
             timing_end(s, stop);
             return rc;
         }
-        const PlanarShape sh = planar_shape(g, n_meas);
         if (compute_sens) rc = launch_planar<true, true, true>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
         else rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
         if (rc) return rc;
-        const uint32_t Nyx = g->d.Nx * g->d.Ny;
-        const float* part = g->scratch;
-        const float* part2 = g->scratch + (size_t)sh.blocks * Nyx;
-        const uint32_t groups = sh.blocks / sh.n_planes;
-        SidArgs a = make_sid_args(g, first, n_meas);
-        auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
-        if (al16(x) && al16(S) && (!dU || al16(dU)) && al16(part2)) {
-            // Nx % 4 == 0 on the planar path, so Nyx % 4 == 0 and the partial slices stay 16-byte aligned
-            const dim3 g4(grid_for(Nyx / 4, 256), g->d.Nz);
-            const float4* p4 = reinterpret_cast<const float4*>(part);
-            const float4* q4 = reinterpret_cast<const float4*>(part2);
-            float4* x4 = reinterpret_cast<float4*>(x);
-            float4* S4 = reinterpret_cast<float4*>(S);
-            const float4* d4 = reinterpret_cast<const float4*>(dU);
-            if (compute_sens && dU)
-                hipLaunchKernelGGL((k_planar_update4<true, true>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4, x4,
-                                   S4, d4, epps, (int)positivity);
-            else if (compute_sens)
-                hipLaunchKernelGGL((k_planar_update4<true, false>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
-                                   x4, S4, d4, epps, (int)positivity);
-            else if (dU)
-                hipLaunchKernelGGL((k_planar_update4<false, true>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
-                                   x4, S4, d4, epps, (int)positivity);
-            else
-                hipLaunchKernelGGL((k_planar_update4<false, false>), g4, dim3(256), 0, s, a, sh.n_planes, groups, p4, q4,
-                                   x4, S4, d4, epps, (int)positivity);
-            OMG_CHECK_LAUNCH();
-            return OMG_OK;
-        }
-        int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
-        hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a, sh.n_planes, map);
-        OMG_CHECK_LAUNCH();
-        dim3 grid(grid_for(N, 256));
-        if (compute_sens && dU)
-            hipLaunchKernelGGL((k_planar_update<true, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
-                               part2, x, S, dU, epps, (int)positivity);
-        else if (compute_sens)
-            hipLaunchKernelGGL((k_planar_update<true, false>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
-                               part2, x, S, dU, epps, (int)positivity);
-        else if (dU)
-            hipLaunchKernelGGL((k_planar_update<false, true>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
-                               part2, x, S, dU, epps, (int)positivity);
-        else
-            hipLaunchKernelGGL((k_planar_update<false, false>), grid, dim3(256), 0, s, Nyx, N, sh.n_planes, groups, map, part,
-                               part2, x, S, dU, epps, (int)positivity);
-        OMG_CHECK_LAUNCH();
-        return OMG_OK;
+        return planar_update(g, first, n_meas, x, S, dU, compute_sens, epps, positivity, s);
+    }
+    if (n_meas > 0 && g->d.projector_type == OMG_PROJ_ORTH && g->orth_planar && !g->force_generic &&
+        unique_slices(g, first, (uint32_t)(n_meas / per))) {
+        rc = orth_planar_fused(g, first, n_meas, x, meas, compute_sens, epps, s);
+        if (rc) return rc;
+        return planar_update(g, first, n_meas, x, S, dU, compute_sens, epps, positivity, s);
     }
     // one-lane-per-LOR path: rhs (and S) accumulate with global atomics, then the update
     float* rhs = gm->work + N;

@@ -432,4 +432,29 @@ __device__ __forceinline__ bool siddon_scatter(const SidArgs& g, int64_t lo, flo
     return true;
 }
 
+// ---- fixed-point LDS accumulation of the direct-plane (planar) kernels ----------------------------
+// 2^F and 2^-F for the bound B of a workgroup's sums (F clamped to [-126, 126]).
+__device__ __forceinline__ void fixed_scale(float B, float& up, float& down) {
+    const int32_t E = (int32_t)((__float_as_uint(B) >> 23) & 0xffu);   // B < 2^(E - 126) for finite B >= 0
+    int32_t F = 62 - (E - 126);
+    F = F > 126 ? 126 : (F < -126 ? -126 : F);
+    up = __uint_as_float((uint32_t)(F + 127) << 23);
+    down = __uint_as_float((uint32_t)(127 - F) << 23);
+}
+
+// Maximum of v over the workgroup (red: TPB / 64 floats of LDS). Contains __syncthreads.
+template <int TPB>
+__device__ __forceinline__ float block_max(float v, float* red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float m = red[0];
+    for (int i = 1; i < TPB / 64; i++) m = fmaxf(m, red[i]);
+    return m;
+}
+
+__device__ __forceinline__ void lds_add_fixed(unsigned long long* p, float t, float up) {
+    atomicAdd(p, (unsigned long long)__float2ll_rn(t * up));
+}
+
 }  // namespace omg
