@@ -68,8 +68,16 @@ typedef struct omg_geom_desc {
      * opts the fused subset iteration into the voxel-driven backprojection on parallel-beam ring
      * sinograms (closed-form chord lengths: within 5e-7 of an FP64 Siddon backprojection, but not the
      * reference's FP32 walk lengths, which are themselves ~1e-5 from FP64); geometries that do not
-     * qualify keep the LDS-atomic planar kernel. Zero = automatic (LDS-atomic). */
+     * qualify keep the LDS fixed-point planar kernel. Zero = automatic. */
     int32_t reserved[8];
+    /* measurement-level subsets of PET sinograms (subset types 1-5: indexMaker's `index`,
+     * source/Python/omegatomo/projector/indices.py:51-123, read by getIndex,
+     * source/opencl/general_opencl_functions.h:913-941): optional host table of sinogram bins, one per
+     * measurement in subset order, copied at create. With it, PET sinogram calls take `first` and `n`
+     * in measurements of this table instead of planes: measurement i of a call is sinogram bin
+     * meas_index[first + i] (radial bin fastest, then angle, then plane), and `norm` is in the same
+     * (subset) order. The direct-plane kernels do not apply; frame batching is not offered. */
+    const uint32_t* meas_index; int64_t meas_index_len;
 } omg_geom_desc;
 
 /* Library/version info and error text. */

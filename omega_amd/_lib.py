@@ -50,6 +50,7 @@ class GeomDesc(C.Structure):
         ("dPitchX", C.c_float),
         ("dPitchY", C.c_float),
         ("reserved", C.c_int32 * 8),
+        ("meas_index", C.c_void_p), ("meas_index_len", C.c_int64),
     ]
 
 

@@ -23,6 +23,12 @@ int check_range(const omg_geom* g, int64_t first, int64_t n) {
                     (long long)first, (long long)(first + n), (long long)(g->d.x_len / 6));
         return OMG_OK;
     }
+    if (g->index_dev) {
+        OMG_REQUIRE(first + n <= g->d.meas_index_len, "measurement range [%lld, %lld) exceeds the %lld entries of "
+                    "the subset index table", (long long)first, (long long)(first + n), (long long)g->d.meas_index_len);
+        OMG_REQUIRE(n <= 0xFFFFFFFFll, "one call handles at most 2^32-1 measurements");
+        return OMG_OK;
+    }
     const int64_t per = (int64_t)g->d.size_x * g->d.size_y;
     OMG_REQUIRE(per > 0, "detector size is zero");
     OMG_REQUIRE(n % per == 0, "measurement count %lld is not a whole number of %lld-bin planes", (long long)n,
@@ -136,6 +142,15 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     }
     if (d->projector_type == OMG_PROJ_BDD)
         OMG_REQUIRE(d->mode == OMG_MODE_CT_FLAT, "projector_type 5 (BDD) supports CT flat-panel data only");
+    if (d->meas_index) {
+        OMG_REQUIRE(d->mode == OMG_MODE_PET_SINO, "meas_index (subset types 1-5) is for PET sinogram data");
+        OMG_REQUIRE(d->meas_index_len > 0 && d->meas_index_len <= 0xFFFFFFFFll,
+                    "meas_index length %lld outside [1, 2^32)", (long long)d->meas_index_len);
+        const uint64_t bins = (uint64_t)d->size_x * d->size_y * d->n_proj;
+        for (int64_t i = 0; i < d->meas_index_len; i++)
+            OMG_REQUIRE((uint64_t)d->meas_index[i] < bins, "meas_index[%lld] = %u is outside the %llu sinogram bins",
+                        (long long)i, d->meas_index[i], (unsigned long long)bins);
+    }
 
     omg_geom* g = new (std::nothrow) omg_geom();
     OMG_REQUIRE(g != nullptr, "out of host memory");
@@ -150,16 +165,24 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     int rc = upload(d->x, d->x_len, &g->x_dev);
     if (!rc) rc = upload(d->z, d->z_len, &g->z_dev);
     if (!rc) rc = upload(d->norm, d->norm_len, &g->norm_dev);
+    if (!rc && d->meas_index) {
+        const size_t bytes = (size_t)d->meas_index_len * sizeof(uint32_t);
+        if (hipMalloc(reinterpret_cast<void**>(&g->index_dev), bytes) != hipSuccess ||
+            hipMemcpy(g->index_dev, d->meas_index, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("could not copy the %lld-entry subset index table to the device", (long long)d->meas_index_len);
+            rc = OMG_ERR_HIP;
+        }
+    }
     if (!rc && d->mode == OMG_MODE_PET_SINO) {
         g->z_host.assign(d->z, d->z + d->z_len);
-        g->planar = true;
+        g->planar = d->meas_index == nullptr;
         for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
     }
     if (!rc && g->planar) g->ring = ring_table(*d, d->x, g->bmaxx, g->bmaxy);
     if (!rc && d->mode == OMG_MODE_PET_SINO) rc = vbp_prepare(g, d->x);
     if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
     if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);
-    g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr;
+    g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr; g->d.meas_index = nullptr;
     if (rc) {
         omg_geom_destroy(g);
         return rc;
@@ -177,6 +200,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->x_dev);
     hipFree(g->z_dev);
     hipFree(g->norm_dev);
+    hipFree(g->index_dev);
     hipFree(g->scratch);
     hipFree(g->work);
     bdd_release(g);

@@ -294,6 +294,7 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
     OMG_REQUIRE(g != nullptr, "null geometry");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "batched frames run the improved Siddon projector");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
+    OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(x && meas && S && rhs, "null pointer argument");
     OMG_REQUIRE(n_frames >= 1, "n_frames must be >= 1");
     OMG_REQUIRE(rays_per_block == 64 || rays_per_block == 128 || rays_per_block == 256,
@@ -334,6 +335,7 @@ extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t 
     OMG_REQUIRE(g != nullptr && S != nullptr, "null argument");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
                 "batched frames run the improved Siddon projector on emission data");
+    OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     const int rc = check_range(g, first, n_meas);
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
@@ -355,6 +357,7 @@ extern "C" int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_me
     OMG_REQUIRE(g != nullptr && x && yhat && n_frames >= 1, "bad arguments");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
                 "batched frames run the improved Siddon projector on emission data");
+    OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(n_frames == 1 || yhat_stride >= n_meas, "yhat_stride must cover the subset's measurements");
     const int rc = check_range(g, first, n_meas);
     if (rc) return rc;

@@ -10,6 +10,7 @@ struct omg_geom {
     float* x_dev = nullptr;   // coordinate tables on the device
     float* z_dev = nullptr;
     float* norm_dev = nullptr;
+    uint32_t* index_dev = nullptr;   // measurement-level subsets: sinogram bin per measurement (subset order)
     float bmaxx = 0.f, bmaxy = 0.f, bmaxz = 0.f;
     int device = 0;
     // Planar fast path: every plane's LORs have zs == zd and the sinogram is in PET mode.
@@ -59,8 +60,9 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.x = g->x_dev; a.z = g->z_dev;
     a.norm = nullptr;
     a.pitch = g->d.Nx;
+    a.index = g->index_dev;
     if (g->norm_dev) {
-        const int64_t per = (g->d.mode == MODE_LIST6) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
+        const int64_t per = (g->d.mode == MODE_LIST6 || g->index_dev) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
         a.norm = g->norm_dev + first * per;
     }
     return a;

@@ -34,6 +34,8 @@ struct SidArgs {
     const float* __restrict__ z;
     const float* __restrict__ norm; // per-measurement normalisation, subset-local, or null
     uint32_t pitch;                // row pitch of slice-relative (planar) voxel indices: Nx, or Nx + 1 in LDS
+    const uint32_t* __restrict__ index;  // PET sinogram: sinogram bin per measurement (subset types 1-5), or null;
+                                         // plane_offset is then the call's first measurement in this table
 };
 
 struct Ray { float xs, ys, zs, xd, yd, zd; };
@@ -51,6 +53,15 @@ __device__ __forceinline__ void fetch_ray(const SidArgs& a, int64_t lo, Ray& r) 
         return;
     }
     const uint32_t per = a.size_x * a.size_y;
+    if (a.index) {   // measurement-level subset: the bin comes from the index table (getIndex)
+        const uint32_t b = a.index[a.plane_offset + lo];
+        const uint32_t iz = b / per, rem = b - iz * per;
+        const float4 q = *reinterpret_cast<const float4*>(a.x + (size_t)rem * 4);
+        r.xs = q.x; r.ys = q.y; r.xd = q.z; r.yd = q.w;
+        const float2 zz = *reinterpret_cast<const float2*>(a.z + (size_t)iz * 2);
+        r.zs = zz.x; r.zd = zz.y;
+        return;
+    }
     const uint32_t l32 = (uint32_t)lo;      // a subset call holds < 2^32 measurements
     uint32_t iz = l32 / per;
     const uint32_t rem = l32 - iz * per;

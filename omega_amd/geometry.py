@@ -151,6 +151,60 @@ def subset_index_type8(n_proj: int, subsets: int):
     return np.concatenate(idx), np.array([len(i) for i in idx], dtype=np.int64)
 
 
+def subset_index_measurements(subset_type: int, subsets: int, n_dist: int, n_ang: int, n_sinos: int,
+                              seed: int = -1):
+    """Measurement-level subsets of a sinogram (indexMaker, indices.py:51-123).
+
+    Bins are numbered radial bin fastest, then angle, then plane (``r + a*n_dist + s*n_dist*n_ang``).
+    Subset ``i`` of
+
+    * type 1 takes every ``subsets``-th bin counted angle-fastest, starting at ``i``;
+    * type 2 every ``subsets``-th bin in sinogram order, starting at ``i``;
+    * type 3 the ``i``-th block of ``total // subsets`` bins of a permutation drawn with
+      ``numpy.random.default_rng(seed)`` (seed < 0: unseeded); the last block ends one bin before the
+      end of the permutation, as the reference's slice ``[port*i:-1]`` does;
+    * type 4 every ``subsets``-th angle starting at ``i`` (whole radial rows), plane by plane; the
+      reference fails with ``n_ang % subsets != 0`` (its remainder branch does not broadcast), so that
+      case raises ``ValueError`` here;
+    * type 5 every ``subsets``-th (radial bin, plane) row counted radial-fastest, each with all angles.
+
+    Returns (index, counts): the subsets' bins concatenated in subset order (int64) and the number of
+    bins per subset.
+    """
+    total = n_dist * n_ang * n_sinos
+    parts = []
+    if subset_type == 1:
+        for i in range(subsets):
+            k = np.arange(i, total, subsets, dtype=np.int64)
+            a, rest = k % n_ang, k // n_ang
+            parts.append(rest % n_dist + a * n_dist + (rest // n_dist) * (n_dist * n_ang))
+    elif subset_type == 2:
+        parts = [np.arange(i, total, subsets, dtype=np.int64) for i in range(subsets)]
+    elif subset_type == 3:
+        rng = np.random.default_rng(seed) if seed >= 0 else np.random.default_rng()
+        perm = rng.permutation(total).astype(np.int64)
+        port = total // subsets
+        parts = [perm[port * i: port * (i + 1)] for i in range(subsets - 1)] + [perm[port * (subsets - 1): -1]]
+    elif subset_type == 4:
+        if n_ang % subsets:
+            raise ValueError("Subset type 4 needs the number of angles to be divisible by the number of subsets "
+                             "(the reference's indexMaker fails otherwise)")
+        r = np.arange(n_dist, dtype=np.int64)
+        s = np.arange(n_sinos, dtype=np.int64)
+        for i in range(subsets):
+            a = np.arange(i, n_ang, subsets, dtype=np.int64)
+            parts.append((s[:, None, None] * (n_dist * n_ang) + a[None, :, None] * n_dist + r[None, None, :]).ravel())
+    elif subset_type == 5:
+        a = np.arange(n_ang, dtype=np.int64)
+        for i in range(subsets):
+            rows = np.arange(i, n_dist * n_sinos, subsets, dtype=np.int64)
+            r, s = rows % n_dist, rows // n_dist
+            parts.append((r[:, None] + a[None, :] * n_dist + s[:, None] * (n_dist * n_ang)).ravel())
+    else:
+        raise ValueError(f"subset type {subset_type} is not a measurement-level subset type (1-5)")
+    return np.concatenate(parts), np.array([p.size for p in parts], dtype=np.int64)
+
+
 @dataclass
 class PETScanner:
     """A cylindrical PET scanner sampled as parallel-chord sinograms."""

@@ -79,6 +79,7 @@ class projectorClass:
         self.listmode = 0
         self.subsets = 1
         self.subsetType = 8
+        self.seed = -1                  # subset type 3: numpy default_rng seed (< 0: unseeded)
         self.x = np.empty(0, dtype=np.float32)
         self.z = np.empty(0, dtype=np.float32)
         self.Ndist = 0
@@ -129,8 +130,11 @@ class projectorClass:
                              "5 (branchless distance-driven) are available in this implementation!")
         if self.projector_type in (5, 55) and not self.CT:
             raise ValueError("Projector type 5 is only supported with CT data!")
-        if self.subsets > 1 and self.subsetType not in (8,) and not self.listmode:
-            raise ValueError("Only subset type 8 (every n-th sinogram/projection) is supported for sinogram data!")
+        if self.subsets > 1 and not self.listmode:
+            if self.CT and self.subsetType != 8:
+                raise ValueError("Only subset type 8 (every n-th projection) is supported for CT data!")
+            if self.subsetType not in (1, 2, 3, 4, 5, 8):
+                raise ValueError("Subset types 1-5 and 8 are supported for sinogram data!")
         if self.subsets < 1:
             raise ValueError("Number of subsets must be at least 1!")
         if self.FOVa_x <= 0 or self.axial_fov <= 0:
@@ -210,6 +214,11 @@ class projectorClass:
             per_subset[-1] = nlor - per_subset[:-1].sum()
             self.index = np.arange(nlor, dtype=np.int64)
             kerroin = 1
+        elif self.subsets > 1 and self.subsetType < 8:
+            # measurement-level subsets: the library looks each measurement's bin up in the index table
+            self.index, per_subset = G.subset_index_measurements(self.subsetType, self.subsets, int(self.nRowsD),
+                                                                 int(self.nColsD), int(self.NSinos), self.seed)
+            kerroin = 1
         else:
             self.index, per_subset = G.subset_index_type8(self.nProjections, self.subsets if self.subsets > 1 else 1)
             if self.subsets > 1:
@@ -231,8 +240,13 @@ class projectorClass:
             self.subset = 0
         self.projectorAdded = True
 
+    def _measurement_subsets(self):
+        return self.subsets > 1 and not self.listmode and not self.CT and self.subsetType < 8
+
     def subset_order(self, data):
-        """Reorder full measurement data (planes in scanner order) into subset order."""
+        """Reorder full measurement data (sinogram order) into subset order."""
+        if self._measurement_subsets():
+            return np.ascontiguousarray(np.asarray(data, dtype=np.float32).ravel()[self.index])
         per = self.nRowsD * self.nColsD
         d = np.asarray(data, dtype=np.float32).reshape(-1, per)
         return np.ascontiguousarray(d[self.index].reshape(-1))
@@ -270,6 +284,10 @@ class projectorClass:
             d.orth_flags = int(self.tube_width_xy > 0) | (int(self.tube_width_z > 0) << 1)
         if getattr(self, "voxel_backprojection", False):
             d.reserved[2] = 1
+        if self._measurement_subsets():
+            idx = np.ascontiguousarray(self.index, dtype=np.uint32)
+            keep.append(idx)
+            d.meas_index, d.meas_index_len = idx.ctypes.data, idx.size
         d.dPitchX = float(self.dPitchX)
         d.dPitchY = float(self.dPitchY)
         return d, keep
@@ -290,7 +308,7 @@ class projectorClass:
             subset = 0
         first = int(self.nMeas[subset])
         nproj = int(self.nMeas[subset + 1] - self.nMeas[subset])
-        per = 1 if self.listmode else self.nRowsD * self.nColsD
+        per = 1 if (self.listmode or self._measurement_subsets()) else self.nRowsD * self.nColsD
         return first, nproj * per
 
     def forwardProject(self, f, subset=-1):

@@ -82,6 +82,21 @@ def test_create_rejects_bad_descriptors(field, value, msg):
     assert msg in _lib.last_error()
 
 
+def test_create_rejects_bad_subset_index():
+    """A measurement index table must point into the sinogram (4 x 3 bins x 2 planes = 24 bins) and is
+    for PET sinogram data only."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    bad = np.array([0, 5, 24], np.uint32)
+    d, keep = _desc(meas_index=bad.ctypes.data, meas_index_len=bad.size)
+    assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG
+    assert "meas_index[2] = 24" in _lib.last_error()
+    ok = np.array([0, 5, 23], np.uint32)
+    d, keep = _desc(meas_index=ok.ctypes.data, meas_index_len=ok.size, mode=_lib.MODE_LIST6, x_len=12)
+    assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG
+    assert "PET sinogram" in _lib.last_error()
+
+
 def test_bdd_requires_ct():
     d, keep = _desc(projector_type=5)
     h = C.c_void_p()
