@@ -533,12 +533,16 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
         }
     }
     if (BP) {
-        const float span = 4.f * (float)(l1 - l0) * (1.f + 0x1p-10f);
-        float up_r, down_r, up_s = 1.f, down_s = 1.f;
-        fixed_scale(block_max<TPB>(mr, red) * span, up_r, down_r);
+        // |r (w temp)| <= |r temp| (w <= 1); a voxel's sum is bounded, with a margin of four visits per
+        // LOR, by 4 * (LORs of the workgroup) * that
+        const float nl4 = 4.f * (float)(l1 - l0);
+        double up_r, down_r, up_s = 1.0, down_s = 1.0;
+        const float br = block_max<TPB>(mr, red) * (1.f + 0x1p-10f);
+        fixed_scale(br, br * nl4, up_r, down_r);
         if (SENS) {
             __syncthreads();
-            fixed_scale(block_max<TPB>(ms, red) * span, up_s, down_s);
+            const float bs = block_max<TPB>(ms, red) * (1.f + 0x1p-10f);
+            fixed_scale(bs, bs * nl4, up_s, down_s);
         }
         for (int pass = 0; pass < (SENS ? 2 : 1); pass++) {
             for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) acc[i] = 0ull;
@@ -564,8 +568,8 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
             }
             __syncthreads();
             float* part = (pass == 0 ? rhs : sens) + (size_t)blockIdx.x * Nyx;
-            const float down = pass == 0 ? down_r : down_s;
-            for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) part[i] = (float)(long long)acc[i] * down;
+            const double down = pass == 0 ? down_r : down_s;
+            for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) part[i] = fixed_value(acc[i], down);
             if (SENS && pass == 0) __syncthreads();
         }
     }

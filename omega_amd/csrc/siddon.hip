@@ -208,12 +208,17 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
         }
     }
     if (BP) {
-        const float span = sqrtf(a.dx * a.dx + a.dy * a.dy) * (float)(l1 - l0) * (1.f + 0x1p-10f);
-        float up_r, down_r, up_s = 1.f, down_s = 1.f;
-        fixed_scale(block_max<TPB>(mr, red) * span, up_r, down_r);   // (syncs: the FP slice is free after this)
+        // |r (len w)| <= |r w| * voxel diagonal (float rounding inside the 2^-10 margin); a LOR adds at
+        // most one term to a voxel, so a voxel's sum is bounded by (LORs of the workgroup) * that
+        const float diag = sqrtf(a.dx * a.dx + a.dy * a.dy) * (1.f + 0x1p-10f);
+        const float nl = (float)(l1 - l0);
+        double up_r, down_r, up_s = 1.0, down_s = 1.0;
+        const float br = block_max<TPB>(mr, red) * diag;   // (syncs: the FP slice is free after this)
+        fixed_scale(br, br * nl, up_r, down_r);
         if (SENS) {
             __syncthreads();
-            fixed_scale(block_max<TPB>(ms, red) * span, up_s, down_s);
+            const float bs = block_max<TPB>(ms, red) * diag;
+            fixed_scale(bs, bs * nl, up_s, down_s);
         }
         for (int pass = 0; pass < (SENS ? 2 : 1); pass++) {
             for (uint32_t i = threadIdx.x; i < S; i += TPB) acc[i] = 0ull;
@@ -237,10 +242,10 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
             __syncthreads();
             // this workgroup's partial slice -> scratch (plain coalesced stores)
             float* part = (pass == 0 ? rhs : sens) + (size_t)blockIdx.x * Nyx;
-            const float down = pass == 0 ? down_r : down_s;
+            const double down = pass == 0 ? down_r : down_s;
             for (uint32_t i = threadIdx.x; i < Nyx; i += TPB) {
                 const uint32_t yy = i / Nx, j = i + yy;   // padded LDS index of voxel i
-                part[i] = (float)(long long)acc[j] * down;
+                part[i] = fixed_value(acc[j], down);
             }
             if (SENS && pass == 0) __syncthreads();
         }

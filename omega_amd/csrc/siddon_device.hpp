@@ -444,13 +444,17 @@ __device__ __forceinline__ bool siddon_scatter(const SidArgs& g, int64_t lo, flo
 }
 
 // ---- fixed-point LDS accumulation of the direct-plane (planar) kernels ----------------------------
-// 2^F and 2^-F for the bound B of a workgroup's sums (F clamped to [-126, 126]).
-__device__ __forceinline__ void fixed_scale(float B, float& up, float& down) {
-    const int32_t E = (int32_t)((__float_as_uint(B) >> 23) & 0xffu);   // B < 2^(E - 126) for finite B >= 0
-    int32_t F = 62 - (E - 126);
-    F = F > 126 ? 126 : (F < -126 ? -126 : F);
-    up = __uint_as_float((uint32_t)(F + 127) << 23);
-    down = __uint_as_float((uint32_t)(127 - F) << 23);
+// A bound B >= 0 satisfies B < 2^(exp_bound(B)) (finite B; exp_bound(0) = -126).
+__device__ __forceinline__ int32_t exp_bound(float B) {
+    return (int32_t)((__float_as_uint(B) >> 23) & 0xffu) - 126;
+}
+
+// Scale 2^F of a workgroup's fixed-point sums and its inverse: every term |t| <= Bt maps below 2^50
+// (the conversion in lds_add_fixed needs |t 2^F| < 2^51) and every sum |s| <= Bs below 2^62.
+__device__ __forceinline__ void fixed_scale(float Bt, float Bs, double& up, double& down) {
+    const int32_t F = min(50 - exp_bound(Bt), 62 - exp_bound(Bs));   // in [-79, 176]
+    up = __longlong_as_double((long long)(1023 + F) << 52);
+    down = __longlong_as_double((long long)(1023 - F) << 52);
 }
 
 // Maximum of v over the workgroup (red: TPB / 64 floats of LDS). Contains __syncthreads.
@@ -464,8 +468,18 @@ __device__ __forceinline__ float block_max(float v, float* red) {
     return m;
 }
 
-__device__ __forceinline__ void lds_add_fixed(unsigned long long* p, float t, float up) {
-    atomicAdd(p, (unsigned long long)__float2ll_rn(t * up));
+// Adds round(t 2^F) to a 64-bit LDS accumulator (ds_add_u64). The double fma t * 2^F + 1.5 * 2^52
+// rounds t 2^F to an integer k (|k| < 2^51 keeps the exponent at 52), so the bits of the result
+// less those of 1.5 * 2^52 are k in two's complement: two VALU operations and an integer subtract
+// instead of a float -> int64 conversion sequence.
+__device__ __forceinline__ void lds_add_fixed(unsigned long long* p, float t, double up) {
+    const double q = __builtin_fma((double)t, up, 0x1.8p52);
+    atomicAdd(p, (unsigned long long)(__double_as_longlong(q) - 0x4338000000000000ll));
+}
+
+// The float value of a fixed-point sum.
+__device__ __forceinline__ float fixed_value(unsigned long long v, double down) {
+    return (float)((double)(long long)v * down);
 }
 
 }  // namespace omg
