@@ -68,7 +68,10 @@ typedef struct omg_geom_desc {
      * opts the fused subset iteration into the voxel-driven backprojection on parallel-beam ring
      * sinograms (closed-form chord lengths: within 5e-7 of an FP64 Siddon backprojection, but not the
      * reference's FP32 walk lengths, which are themselves ~1e-5 from FP64); geometries that do not
-     * qualify keep the LDS fixed-point planar kernel. Zero = automatic. */
+     * qualify keep the LDS fixed-point planar kernel; reserved[3] != 0 keeps the one-lane-per-LOR kernels
+     * on PET sinograms in the x-fastest image only (by default LORs closer to the x axis accumulate
+     * their backprojection into y-fastest buffers, where the lanes of a wave share cache lines).
+     * Zero = automatic. */
     int32_t reserved[8];
     /* measurement-level subsets of PET sinograms (subset types 1-5: indexMaker's `index`,
      * source/Python/omegatomo/projector/indices.py:51-123, read by getIndex,

@@ -201,6 +201,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->z_dev);
     hipFree(g->norm_dev);
     hipFree(g->index_dev);
+    hipFree(g->frames_t);
     hipFree(g->scratch);
     hipFree(g->work);
     bdd_release(g);

@@ -72,11 +72,13 @@ __device__ __forceinline__ bool frames_gather(const SidArgs& g, int64_t lo, cons
 // One subset step of all frames: FP, ratio, BP into rhs[frame][voxel]. Work item = (frame tile, LOR block).
 // `x`, `rhs` point at frame 0 of their stacks (frame stride N); `meas` at frame 0's first LOR of the subset
 // (frame stride meas_stride).
+// rhs_t (or null): y-fastest stack [frame][z][x][y] that x-major LORs accumulate into (siddon_trace_split),
+// folded into rhs by k_frames_update_t.
 template <int FT, int RPB>
 __global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t N, const float* __restrict__ x,
                                                      const float* __restrict__ meas, int64_t meas_stride,
-                                                     float* __restrict__ rhs, float epps, uint32_t lor_blocks,
-                                                     uint32_t n_items) {
+                                                     float* __restrict__ rhs, float* __restrict__ rhs_t, float epps,
+                                                     uint32_t lor_blocks, uint32_t n_items) {
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t tile = item / lor_blocks;
         const uint32_t lb = item - tile * lor_blocks;
@@ -98,13 +100,23 @@ __global__ __launch_bounds__(RPB) void k_frames_osem(SidArgs a, uint32_t N, cons
         if (!any) continue;
         // frames whose ratio is 0 (no counts in this LOR) add nothing: skip their atomics, as the
         // single-frame kernel skips the whole LOR
-        siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
-            const float lw = len * w;
-            float* p = rhs + (size_t)f0 * N + ind;
+        if (rhs_t) {
+            siddon_trace_split(a, lo, w, [&](uint32_t ind, float len, bool t) {
+                const float lw = len * w;
+                float* p = (t ? rhs_t : rhs) + (size_t)f0 * N + ind;
 #pragma unroll
-            for (int f = 0; f < FT; f++)
-                if (r[f] != 0.f) unsafeAtomicAdd(p + (size_t)f * N, r[f] * lw);
-        });
+                for (int f = 0; f < FT; f++)
+                    if (r[f] != 0.f) unsafeAtomicAdd(p + (size_t)f * N, r[f] * lw);
+            });
+        } else {
+            siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
+                const float lw = len * w;
+                float* p = rhs + (size_t)f0 * N + ind;
+#pragma unroll
+                for (int f = 0; f < FT; f++)
+                    if (r[f] != 0.f) unsafeAtomicAdd(p + (size_t)f * N, r[f] * lw);
+            });
+        }
     }
 }
 
@@ -196,6 +208,43 @@ __global__ __launch_bounds__(256) void k_frames_update(uint32_t N, uint32_t F, f
     rhs[i] = 0.f;
 }
 
+// k_frames_update with the y-fastest accumulator rhs_t folded in. Per 32 x 32 (x, y) tile of one
+// (frame, slice) plane, rhs_t's tile is read along y, transposed through LDS and cleared behind; then
+// x, rhs (cleared behind) and dU are read along x. grid = (ceil(Nx / 32), ceil(Ny / 32), Nz * F),
+// block = (32, 8).
+__global__ __launch_bounds__(256) void k_frames_update_t(uint32_t Nx, uint32_t Ny, uint32_t Nz, float* __restrict__ x,
+                                                         const float* __restrict__ S, float* __restrict__ rhs,
+                                                         float* __restrict__ rhs_t, const float* __restrict__ dU,
+                                                         float epps, int pos) {
+    __shared__ float tile[32][33];
+    const uint32_t x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const size_t Nyx = (size_t)Nx * Ny;
+    const size_t off = (size_t)blockIdx.z * Nyx;              // (frame, slice) plane
+    const size_t soff = (size_t)(blockIdx.z % Nz) * Nyx;      // the slice in S
+    for (uint32_t k = threadIdx.y; k < 32; k += 8) {
+        const uint32_t xx = x0 + k, yy = y0 + threadIdx.x;
+        float v = 0.f;
+        if (xx < Nx && yy < Ny) {
+            float* p = rhs_t + off + (size_t)xx * Ny + yy;
+            v = *p;
+            *p = 0.f;
+        }
+        tile[k][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.y; k < 32; k += 8) {
+        const uint32_t yy = y0 + k, xx = x0 + threadIdx.x;
+        if (xx >= Nx || yy >= Ny) continue;
+        const size_t i = off + (size_t)yy * Nx + xx;
+        const float r = rhs[i] + tile[threadIdx.x][k];
+        const float sv = S[soff + (size_t)yy * Nx + xx];
+        float xn = dU ? x[i] / (sv + dU[i]) * r : x[i] / sv * (r + epps);
+        if (pos && xn < epps) xn = epps;
+        x[i] = xn;
+        rhs[i] = 0.f;
+    }
+}
+
 struct FramesStep {
     SidArgs a;
     uint32_t F, N;
@@ -204,6 +253,7 @@ struct FramesStep {
     int64_t meas_stride;
     const float* S;
     float* rhs;
+    float* rhs_t;       // y-fastest accumulator stack [F][N] (zero between steps), or null
     float* dU;          // OSL-RDP work stack [F][N], or null for OSEM
     float beta, gamma;
     uint32_t Nx, Ny, Nz;
@@ -250,7 +300,7 @@ static hipError_t launch_data(const FramesStep& st, hipStream_t s) {
     if (grid > n_items) grid = n_items;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((k_frames_osem<FT, RPB>), dim3(grid), dim3(RPB), 0, s, st.a, st.N, st.x, st.meas,
-                       st.meas_stride, st.rhs, st.epps, lor_blocks, n_items);
+                       st.meas_stride, st.rhs, st.rhs_t, st.epps, lor_blocks, n_items);
     return hipGetLastError();
 }
 
@@ -281,16 +331,26 @@ static int launch_step(const FramesStep& st, hipStream_t s) {
         set_error("frames OSEM launch failed: %s", hipGetErrorString(e));
         return OMG_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_frames_update, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, st.N, st.F, st.x, st.S,
-                       st.rhs, st.dU, st.epps, st.pos);
+    if (st.rhs_t) {
+        const dim3 tg((st.Nx + 31) / 32, (st.Ny + 31) / 32, st.Nz * st.F);
+        hipLaunchKernelGGL(k_frames_update_t, tg, dim3(32, 8), 0, s, st.Nx, st.Ny, st.Nz, st.x, st.S, st.rhs, st.rhs_t,
+                           st.dU, st.epps, st.pos);
+    } else {
+        hipLaunchKernelGGL(k_frames_update, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, st.N, st.F, st.x,
+                           st.S, st.rhs, st.dU, st.epps, st.pos);
+    }
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
 
+// The frames kernels accumulate x-major LORs into a y-fastest stack on PET sinograms unless
+// omg_geom_desc.reserved[3] != 0 (as the single-frame one-lane-per-LOR kernels do).
+static bool frames_split(const omg_geom* g) { return g->d.mode == OMG_MODE_PET_SINO && g->d.reserved[3] == 0; }
+
 static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n_frames, float* x, const float* meas,
-                     int64_t meas_stride, const float* S, float* rhs, float* dU, float beta, float gamma, float epps,
-                     int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile, int32_t grid_blocks,
-                     FramesStep& st) {
+                     int64_t meas_stride, const float* S, float* rhs, float* rhs_t, float* dU, float beta, float gamma,
+                     float epps, int32_t positivity, int32_t rays_per_block, int32_t frames_per_tile,
+                     int32_t grid_blocks, FramesStep& st) {
     OMG_REQUIRE(g != nullptr, "null geometry");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "batched frames run the improved Siddon projector");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
@@ -311,7 +371,7 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
     st.a = make_sid_args(g, first, n_meas);
     st.F = (uint32_t)n_frames;
     st.N = g->d.Nx * g->d.Ny * g->d.Nz;
-    st.x = x; st.meas = meas; st.meas_stride = meas_stride; st.S = S; st.rhs = rhs;
+    st.x = x; st.meas = meas; st.meas_stride = meas_stride; st.S = S; st.rhs = rhs; st.rhs_t = rhs_t;
     st.dU = beta != 0.f ? dU : nullptr;
     st.beta = beta; st.gamma = gamma;
     st.Nx = g->d.Nx; st.Ny = g->d.Ny; st.Nz = g->d.Nz;
@@ -328,7 +388,22 @@ using namespace omg;
 struct omg_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    float* rhs_t = nullptr;   // the graph's own y-fastest accumulator stack (frames_split), zeroed by the graph
 };
+
+// Ensures g->frames_t holds n floats of y-fastest accumulator, zero (allocated and cleared on growth).
+static int frames_t_reserve(omg_geom* g, size_t n, hipStream_t s) {
+    if (g->frames_t_floats >= n) return OMG_OK;
+    OMG_CHECK_HIP(hipDeviceSynchronize());
+    OMG_CHECK_HIP(hipFree(g->frames_t));
+    g->frames_t = nullptr;
+    g->frames_t_floats = 0;
+    OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->frames_t), n * sizeof(float)));
+    hipLaunchKernelGGL(k_frames_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g->frames_t, n, 0.f);
+    OMG_CHECK_LAUNCH();
+    g->frames_t_floats = n;
+    return OMG_OK;
+}
 
 extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t n_meas, float* S, float epps,
                                       void* stream) {
@@ -394,8 +469,15 @@ extern "C" int omg_frames_osem_step(const omg_geom* g, int64_t first, int64_t n_
                                     float beta, float gamma, float epps, int32_t positivity, int32_t rays_per_block,
                                     int32_t frames_per_tile, int32_t grid_blocks, void* stream) {
     FramesStep st;
-    int rc = make_step(g, first, n_meas, n_frames, x, meas, meas_stride, S, rhs, dU, beta, gamma, epps, positivity,
-                       rays_per_block, frames_per_tile, grid_blocks, st);
+    float* rhs_t = nullptr;
+    if (g && n_frames >= 1 && frames_split(g)) {
+        const size_t NF = (size_t)g->d.Nx * g->d.Ny * g->d.Nz * (size_t)n_frames;
+        const int rc = frames_t_reserve(const_cast<omg_geom*>(g), NF, as_stream(stream));
+        if (rc) return rc;
+        rhs_t = g->frames_t;
+    }
+    int rc = make_step(g, first, n_meas, n_frames, x, meas, meas_stride, S, rhs, rhs_t, dU, beta, gamma, epps,
+                       positivity, rays_per_block, frames_per_tile, grid_blocks, st);
     if (rc) return rc;
     return launch_step(st, as_stream(stream));
 }
@@ -411,22 +493,33 @@ extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, con
     OMG_REQUIRE(n_subsets >= 1 && first && n_meas && meas_offset, "subset tables are required");
     OMG_REQUIRE(g != nullptr, "null geometry");
     const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+    OMG_REQUIRE(n_frames >= 1, "n_frames must be >= 1");
+    float* rhs_t = nullptr;
+    const size_t NF = (size_t)N * (size_t)n_frames;
+    if (frames_split(g)) OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&rhs_t), NF * sizeof(float)));
     std::vector<FramesStep> steps((size_t)n_subsets);
     for (int32_t k = 0; k < n_subsets; k++) {
-        OMG_REQUIRE(meas_offset[k] >= 0, "negative measurement offset");
-        const int rc = make_step(g, first[k], n_meas[k], n_frames, x, meas + meas_offset[k], meas_stride,
-                                 S + (size_t)k * N, rhs, dU, beta, gamma, epps, positivity, rays_per_block,
-                                 frames_per_tile, grid_blocks, steps[(size_t)k]);
-        if (rc) return rc;
+        int rc = meas_offset[k] >= 0 ? OMG_OK : OMG_ERR_ARG;
+        if (rc) set_error("negative measurement offset");
+        if (!rc)
+            rc = make_step(g, first[k], n_meas[k], n_frames, x, meas + meas_offset[k], meas_stride, S + (size_t)k * N,
+                           rhs, rhs_t, dU, beta, gamma, epps, positivity, rays_per_block, frames_per_tile,
+                           grid_blocks, steps[(size_t)k]);
+        if (rc) {
+            hipFree(rhs_t);
+            return rc;
+        }
     }
     hipStream_t cs;
     OMG_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     omg_graph* h = new omg_graph;
+    h->rhs_t = rhs_t;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     int rc = OMG_OK;
     if (e == hipSuccess) {
-        const size_t NF = (size_t)N * n_frames;
         hipLaunchKernelGGL(k_frames_fill, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, cs, rhs, NF, 0.f);
+        if (rhs_t)
+            hipLaunchKernelGGL(k_frames_fill, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, cs, rhs_t, NF, 0.f);
         for (const FramesStep& st : steps) {
             rc = launch_step(st, cs);
             if (rc) break;
@@ -440,6 +533,7 @@ extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, con
         if (!rc) set_error("hipGraph capture of the frames iteration failed: %s", hipGetErrorString(e));
         if (h->exec) hipGraphExecDestroy(h->exec);
         if (h->graph) hipGraphDestroy(h->graph);
+        hipFree(h->rhs_t);
         delete h;
         return rc ? rc : OMG_ERR_HIP;
     }
@@ -457,6 +551,7 @@ extern "C" int omg_graph_destroy(omg_graph* h) {
     if (!h) return OMG_OK;
     if (h->exec) hipGraphExecDestroy(h->exec);
     if (h->graph) hipGraphDestroy(h->graph);
+    hipFree(h->rhs_t);
     delete h;
     return OMG_OK;
 }

@@ -20,6 +20,9 @@ struct omg_geom {
     // Every LOR of the (planar) sinogram has both endpoints outside the image with margin, so every
     // crossing parameter of its walk lies in [0, 1] (the planar kernels' short length expression applies).
     bool ring = false;
+    // Batched frames: y-fastest accumulator stack of direct omg_frames_osem_step calls (zero between calls).
+    float* frames_t = nullptr;
+    size_t frames_t_floats = 0;
     // Orthogonal projector: direct-plane sinogram whose tubes stay in their slice (orth.hip planar kernels).
     bool orth_planar = false;
     // Workgroups per plane for the planar kernels (omg_geom_desc.reserved[1]; 0 = automatic).

@@ -55,6 +55,11 @@ void timing_end(hipStream_t s, hipEvent_t stop) {
 }
 
 
+__global__ void k_fill_f32(float* __restrict__ p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---- forward projection -------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_siddon_fp(SidArgs a, const float* __restrict__ img, float* __restrict__ y) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -83,10 +88,49 @@ __global__ __launch_bounds__(kBlock) void k_siddon_bp(SidArgs a, const float* __
     }
 }
 
+// The same with x-major LORs accumulated into y-fastest buffers rhs_t / sens_t (siddon_trace_split),
+// added into rhs / sens by k_add_transposed afterwards.
+template <bool SENS>
+__global__ __launch_bounds__(kBlock) void k_siddon_bp_split(SidArgs a, const float* __restrict__ yin,
+                                                            float* __restrict__ rhs, float* __restrict__ sens,
+                                                            float* __restrict__ rhs_t, float* __restrict__ sens_t) {
+    const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (lo >= a.nmeas) return;
+    const float yi = yin[lo];
+    if (!SENS && yi == 0.f) return;
+    float w = 0.f;
+    siddon_trace_split(a, lo, w, [&](uint32_t ind, float len, bool t) {
+        const float lw = len * w;
+        unsafeAtomicAdd((t ? rhs_t : rhs) + ind, yi * lw);
+        if (SENS) unsafeAtomicAdd((t ? sens_t : sens) + ind, lw);
+    });
+}
+
+// dst[x + y*Nx + z*Nx*Ny] += src_t[y + x*Ny + z*Nx*Ny], through 32 x 32 LDS tiles (both sides coalesced).
+// grid = (ceil(Nx / 32), ceil(Ny / 32), Nz), block = (32, 8).
+__global__ __launch_bounds__(256) void k_add_transposed(const float* __restrict__ src_t, float* __restrict__ dst,
+                                                        uint32_t Nx, uint32_t Ny) {
+    __shared__ float tile[32][33];
+    const uint32_t x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const size_t zoff = (size_t)blockIdx.z * Nx * Ny;
+    for (uint32_t k = threadIdx.y; k < 32; k += 8) {   // read: y fastest
+        const uint32_t x = x0 + k, y = y0 + threadIdx.x;
+        tile[k][threadIdx.x] = (x < Nx && y < Ny) ? src_t[zoff + (size_t)x * Ny + y] : 0.f;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.y; k < 32; k += 8) {   // write: x fastest
+        const uint32_t y = y0 + k, x = x0 + threadIdx.x;
+        if (x < Nx && y < Ny) dst[zoff + (size_t)y * Nx + x] += tile[threadIdx.x][k];
+    }
+}
+
 // ---- fused OSEM data step: FP, ratio, BP for one LOR ---------------------------------------------
+// With rhs_t (/ sens_t) non-null, x-major LORs accumulate into y-fastest buffers, which the caller adds
+// back (k_add_transposed).
 __global__ __launch_bounds__(kBlock) void k_siddon_osem(SidArgs a, const float* __restrict__ img,
                                                         const float* __restrict__ meas, float* __restrict__ rhs,
-                                                        float* __restrict__ sens, float epps) {
+                                                        float* __restrict__ sens, float* __restrict__ rhs_t,
+                                                        float* __restrict__ sens_t, float epps) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (lo >= a.nmeas) return;
     float ax, w = 0.f;
@@ -95,7 +139,13 @@ __global__ __launch_bounds__(kBlock) void k_siddon_osem(SidArgs a, const float* 
     float r = meas[lo] / (yhat + epps);
     if (isnan(r) || isinf(r)) r = epps;
     if (r == 0.f && sens == nullptr) return;
-    if (sens) {
+    if (rhs_t) {
+        siddon_trace_split(a, lo, w, [&](uint32_t ind, float len, bool t) {
+            const float lw = len * w;
+            unsafeAtomicAdd((t ? rhs_t : rhs) + ind, r * lw);
+            if (sens) unsafeAtomicAdd((t ? sens_t : sens) + ind, lw);
+        });
+    } else if (sens) {
         siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
             const float lw = len * w;
             unsafeAtomicAdd(rhs + ind, r * lw);
@@ -482,6 +532,59 @@ extern "C" int omg_timing_collect(double* total_ms, int64_t* launches) {
     return OMG_OK;
 }
 
+// One-lane-per-LOR launches on PET sinograms walk x-major LORs in y-fastest copies of the image and of
+// the accumulators (siddon_setup's allow_trans); omg_geom_desc.reserved[3] != 0 turns this off.
+static bool split_layout(const omg_geom* g) {
+    return g->d.mode == OMG_MODE_PET_SINO && g->d.reserved[3] == 0;
+}
+
+static dim3 xy_tiles(const omg_geom* g) { return dim3((g->d.Nx + 31) / 32, (g->d.Ny + 31) / 32, g->d.Nz); }
+
+// The forward projection gathers from the x-fastest image only: gathering x-major LORs from a y-fastest
+// copy measured slower (9.6 ms against 7.4 ms for a C2 forward projection).
+static int generic_fp(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
+    const SidArgs a = make_sid_args(g, first, n);
+    hipEvent_t stop;
+    timing_begin(s, &stop);
+    hipLaunchKernelGGL(k_siddon_fp, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a, img, y);
+    timing_end(s, stop);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+// FP -> ratio -> BP into rhs (and sens), both already holding what the caller wants added to.
+static int generic_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
+                        float* sens, float epps, hipStream_t s) {
+    const SidArgs a = make_sid_args(g, first, n);
+    float *rhs_t = nullptr, *sens_t = nullptr;
+    const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+    if (split_layout(g)) {   // y-fastest accumulators only (see generic_fp)
+        const int rc = scratch_reserve(const_cast<omg_geom*>(g), (size_t)N * (sens ? 2 : 1));
+        if (rc) return rc;
+        float* t = g->scratch;
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for((int64_t)N * (sens ? 2 : 1), 256)), dim3(256), 0, s, t,
+                           N * (sens ? 2u : 1u), 0.f);
+        OMG_CHECK_LAUNCH();
+        rhs_t = t;
+        sens_t = sens ? t + N : nullptr;
+    }
+    hipEvent_t stop;
+    timing_begin(s, &stop);
+    hipLaunchKernelGGL(k_siddon_osem, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a, img, meas, rhs, sens,
+                       rhs_t, sens_t, epps);
+    timing_end(s, stop);
+    OMG_CHECK_LAUNCH();
+    if (rhs_t) {
+        hipLaunchKernelGGL(k_add_transposed, xy_tiles(g), dim3(32, 8), 0, s, rhs_t, rhs, g->d.Nx, g->d.Ny);
+        OMG_CHECK_LAUNCH();
+        if (sens) {
+            hipLaunchKernelGGL(k_add_transposed, xy_tiles(g), dim3(32, 8), 0, s, sens_t, sens, g->d.Nx, g->d.Ny);
+            OMG_CHECK_LAUNCH();
+        }
+    }
+    return OMG_OK;
+}
+
 static int siddon_common_checks(const omg_geom* g, int64_t first, int64_t n) {
     OMG_REQUIRE(g != nullptr, "null geometry handle");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON || g->d.projector_type == OMG_PROJ_ORTH ||
@@ -515,13 +618,7 @@ extern "C" int omg_forward_project(const omg_geom* g, int64_t first, int64_t n_m
     if (g->d.projector_type == OMG_PROJ_ORTH) return orth_forward(g, first, n_meas, img, y, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_forward(g, first, n_meas, img, y, s);
     if (planar_ok(g) && !g->force_generic) return launch_planar<true, false, false>(g, first, n_meas, img, nullptr, y, nullptr, nullptr, 0.f, s);
-    SidArgs a = make_sid_args(g, first, n_meas);
-    hipEvent_t stop;
-    timing_begin(s, &stop);
-    hipLaunchKernelGGL(k_siddon_fp, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, img, y);
-    timing_end(s, stop);
-    OMG_CHECK_LAUNCH();
-    return OMG_OK;
+    return generic_fp(g, first, n_meas, img, y, s);
 }
 
 extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_meas, const float* y, float* rhs,
@@ -538,6 +635,35 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
         return launch_planar<false, true, false>(g, first, n_meas, nullptr, y, nullptr, rhs, nullptr, 0.f, s);
     }
     SidArgs a = make_sid_args(g, first, n_meas);
+    if (split_layout(g)) {
+        // x-major LORs into y-fastest buffers (scratch), added back transposed
+        const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
+        rc = scratch_reserve(const_cast<omg_geom*>(g), (size_t)N * (sens ? 2 : 1));
+        if (rc) return rc;
+        float* rt = g->scratch;
+        float* st = sens ? g->scratch + N : nullptr;
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for((int64_t)N * (sens ? 2 : 1), 256)), dim3(256), 0, s, rt,
+                           N * (sens ? 2u : 1u), 0.f);
+        OMG_CHECK_LAUNCH();
+        hipEvent_t stop;
+        timing_begin(s, &stop);
+        if (sens)
+            hipLaunchKernelGGL(k_siddon_bp_split<true>, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, y, rhs,
+                               sens, rt, st);
+        else
+            hipLaunchKernelGGL(k_siddon_bp_split<false>, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, y, rhs,
+                               nullptr, rt, nullptr);
+        timing_end(s, stop);
+        OMG_CHECK_LAUNCH();
+        const dim3 tg = xy_tiles(g);
+        hipLaunchKernelGGL(k_add_transposed, tg, dim3(32, 8), 0, s, rt, rhs, g->d.Nx, g->d.Ny);
+        OMG_CHECK_LAUNCH();
+        if (sens) {
+            hipLaunchKernelGGL(k_add_transposed, tg, dim3(32, 8), 0, s, st, sens, g->d.Nx, g->d.Ny);
+            OMG_CHECK_LAUNCH();
+        }
+        return OMG_OK;
+    }
     hipEvent_t stop;
     timing_begin(s, &stop);
     hipLaunchKernelGGL(k_siddon_bp, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, y, rhs, sens);
@@ -559,19 +685,10 @@ extern "C" int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_me
         if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, as_stream(stream));
         return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, as_stream(stream));
     }
-    SidArgs a = make_sid_args(g, first, n_meas);
-    hipLaunchKernelGGL(k_siddon_osem, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, as_stream(stream), a, img, meas,
-                       rhs, sens, epps);
-    OMG_CHECK_LAUNCH();
-    return OMG_OK;
+    return generic_osem(g, first, n_meas, img, meas, rhs, sens, epps, as_stream(stream));
 }
 
 namespace omg {
-
-__global__ void k_fill_f32(float* __restrict__ p, uint32_t n, float v) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = v;
-}
 
 // slice -> launch plane map for the planar update (-1: slice not in this launch)
 __global__ void k_slice_plane(SidArgs a, uint32_t n_planes, int32_t* __restrict__ map) {
@@ -734,13 +851,8 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
         rc = orth_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s);
         if (rc) return rc;
     } else if (n_meas > 0) {
-        SidArgs a = make_sid_args(g, first, n_meas);
-        hipEvent_t stop;
-        timing_begin(s, &stop);
-        hipLaunchKernelGGL(k_siddon_osem, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, s, a, x, meas, rhs,
-                           compute_sens ? S : nullptr, epps);
-        timing_end(s, stop);
-        OMG_CHECK_LAUNCH();
+        rc = generic_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s);
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(k_generic_update, dim3(grid_for(N, 256)), dim3(256), 0, s, N, x, S, rhs, dU, epps,
                        (int)(compute_sens != 0), (int)positivity);

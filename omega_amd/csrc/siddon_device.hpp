@@ -200,7 +200,10 @@ __device__ __forceinline__ int perp_start(float b, float d, float dd, uint32_t N
 struct Walk {
     float tx0, ty0, tz0, txu, tyu, tzu, tc, L;
     int32_t ti, tj, tk, ux, uy, uz;
-    uint32_t pos, uyp;             // PLANAR: LDS index tj * pitch + ti, kept incrementally; uy * pitch
+    uint32_t pos;                  // PLANAR: LDS index of (ti, tj), kept incrementally (uint32 wrap-around)
+    uint32_t sxs, sys;             // PLANAR: index steps of an x / y plane crossing;
+                                   // otherwise the x / y strides of the walked image (1, Nx) or (Ny, 1)
+    bool trans;                    // the walked image is y-fastest (transposed): y + x*Ny + z*Nx*Ny
     uint32_t left;                 // Siddon steps still to take (0 = finished)
     bool perp;
     uint32_t p_ind, p_stride, p_a1, p_a2, p_nrun, p_lim;
@@ -211,8 +214,12 @@ struct Walk {
 // nothing; otherwise fills the walk and the LOR weight.
 // PLANAR: the caller guarantees zs == zd (a direct sinogram plane); only the in-plane branches
 // are compiled and voxel indices are relative to the LOR's slice (tk * Nx * Ny omitted).
+// allow_trans: an x-major LOR (|dx| > |dy| in voxel steps) walks the y-fastest image, where the
+// neighbouring radial bins held by the lanes of a wave -- side by side along y for such LORs -- share
+// cache lines (w.trans is set; the caller supplies the transposed image / accumulator).
 template <bool PLANAR>
-__device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk& w, float& weight) {
+__device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk& w, float& weight,
+                                             bool allow_trans = false) {
     Ray r;
     fetch_ray(g, lo, r);
     const float ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
@@ -273,6 +280,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
             }
         }
         w.perp = true;
+        w.trans = false;
         w.p_ind = apu * Nm + (PLANAR ? 0u : (uint32_t)tk * Nrow * Nrun) + stride * (uint32_t)a1;
         w.p_stride = stride;
         w.p_a1 = (uint32_t)a1; w.p_a2 = (uint32_t)a2; w.p_nrun = Nrun;
@@ -310,8 +318,14 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
     w.tx0 = tx0; w.ty0 = ty0; w.tz0 = tz0; w.txu = txu; w.tyu = tyu; w.tzu = tzu; w.tc = tc;
     w.ti = ti; w.tj = tj; w.tk = tk; w.ux = ux; w.uy = uy; w.uz = uz;
     if (PLANAR) {
+        w.trans = false;
         w.pos = (uint32_t)tj * g.pitch + (uint32_t)ti;
-        w.uyp = (uint32_t)(uy * (int32_t)g.pitch);
+        w.sxs = (uint32_t)ux;
+        w.sys = (uint32_t)(uy * (int32_t)g.pitch);
+    } else {
+        w.trans = allow_trans && txu > 0.f && (tyu == 0.f || g.dx * tyu > g.dy * txu);
+        w.sxs = w.trans ? g.Ny : 1u;
+        w.sys = w.trans ? 1u : g.Nx;
     }
     w.left = Np;
     return true;
@@ -323,11 +337,15 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
 // RING: the caller guarantees that every crossing parameter of the walk lies in [0, 1] (both
 // endpoints of the LOR outside the image, with margin); the reference's length expression then
 // reduces to (tn - tc) * L, the same float result without its range selections.
-template <bool PLANAR, bool RING = false>
+// STRIDED (not PLANAR): the index uses the walk's per-LOR strides (siddon_setup's allow_trans);
+// otherwise the x-fastest image's tj * Nx + ti + tk * Nx*Ny with the grid's uniform strides.
+template <bool PLANAR, bool RING = false, bool STRIDED = false>
 __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& ind, float& len) {
     // bitwise & / | on the predicates keep the step free of short-circuit control flow
     const bool alive = w.left > 0u;
-    ind = PLANAR ? w.pos : (uint32_t)w.tj * g.Nx + (uint32_t)w.ti + (uint32_t)w.tk * g.Nyx;
+    if (PLANAR) ind = w.pos;
+    else if (STRIDED) ind = (uint32_t)w.ti * w.sxs + (uint32_t)w.tj * w.sys + (uint32_t)w.tk * g.Nyx;
+    else ind = (uint32_t)w.tj * g.Nx + (uint32_t)w.ti + (uint32_t)w.tk * g.Nyx;
     const bool zmin = !PLANAR && ((w.tz0 < w.ty0) & (w.tz0 < w.tx0));
     const bool ymin = (!zmin) & (w.ty0 < w.tx0);
     const float tn = zmin ? w.tz0 : (ymin ? w.ty0 : w.tx0);
@@ -350,7 +368,7 @@ __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& i
     w.tx0 = sx ? w.tx0 + w.txu : w.tx0;
     w.tj += sy ? w.uy : 0;
     w.ty0 = sy ? w.ty0 + w.tyu : w.ty0;
-    if (PLANAR) w.pos += sx ? (uint32_t)w.ux : (sy ? w.uyp : 0u);
+    if (PLANAR) w.pos += sx ? w.sxs : (sy ? w.sys : 0u);
     if (!PLANAR) {
         w.tk += sz ? w.uz : 0;
         w.tz0 = sz ? w.tz0 + w.tzu : w.tz0;
@@ -439,6 +457,27 @@ __device__ __forceinline__ bool siddon_scatter(const SidArgs& g, int64_t lo, flo
 #pragma unroll
         for (int k = 0; k < kBatch; k++)
             if (len[k] > 0.f) visit(ind[k], len[k]);
+    }
+    return true;
+}
+
+// Backprojection walk that lets x-major LORs accumulate into a y-fastest (transposed) image:
+// visit(ind, len, trans) for each voxel with len > 0, ind in the layout named by trans (see
+// siddon_setup's allow_trans). Returns false when the LOR contributes nothing.
+template <typename Visit>
+__device__ __forceinline__ bool siddon_trace_split(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+    Walk w;
+    if (!siddon_setup<false>(g, lo, w, weight, true)) return false;
+    if (w.perp) {
+        perp_walk(w, [&](uint32_t ind, float len) { visit(ind, len, false); });
+        return true;
+    }
+    const bool t = w.trans;
+    while (w.left > 0u) {
+        uint32_t ind;
+        float len;
+        walk_step<false, false, true>(g, w, ind, len);
+        if (len > 0.f) visit(ind, len, t);
     }
     return true;
 }

@@ -11,9 +11,10 @@ from omega_amd.projector import Geometry
 
 def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
               ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_flags=0, dPitchX=0.0, dPitchY=0.0,
-              generic=False):
+              generic=False, x_fastest_only=False):
     """Geometry handle through the C ABI; ``generic`` routes every call through the one-lane-per-LOR
-    kernels (omg_geom_desc.reserved[0])."""
+    kernels (omg_geom_desc.reserved[0]); ``x_fastest_only`` keeps those kernels in the x-fastest image
+    (reserved[3])."""
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
     d = _lib.GeomDesc()
@@ -43,6 +44,8 @@ def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, 
     d.dPitchY = dPitchY
     if generic:
         d.reserved[0] = 1
+    if x_fastest_only:
+        d.reserved[3] = 1
     return Geometry(d, keep)
 
 
