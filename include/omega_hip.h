@@ -126,6 +126,16 @@ int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, 
                               int32_t compute_sens, float epps, float beta, float gamma, int32_t positivity,
                               void* stream);
 
+/* The data step and the subset iteration with the randoms / scatter term of the ratio: the denominator
+ * becomes (A_k x + epps) + additive[i] (computeForwardProject.h:20-21, 66-69: input += epps; input +=
+ * randomsData), additive holding one float per measurement of the call in the order of meas, or NULL
+ * (the plain entry points above). Not offered for the batched frames. */
+int omg_osem_data_step_ex(const omg_geom* g, int64_t first, int64_t n_meas, const float* img, const float* meas,
+                          const float* additive, float* rhs, float* sens, float epps, void* stream);
+int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas,
+                                 const float* additive, float* S, int32_t compute_sens, float epps, float beta,
+                                 float gamma, int32_t positivity, void* stream);
+
 /* Kernel timing of the projector launches with HIP events, the counterpart of the reference's
  * verbose >= 3 event timers (source/cpp/ProjectorClass.h:3594-3599). omg_timing_enable(1) records
  * every following projector kernel launch (Siddon FP/BP/fused kernels) until omg_timing_enable(0);

@@ -64,6 +64,7 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.norm = nullptr;
     a.pitch = g->d.Nx;
     a.index = g->index_dev;
+    a.add = nullptr;
     if (g->norm_dev) {
         const int64_t per = (g->d.mode == MODE_LIST6 || g->index_dev) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
         a.norm = g->norm_dev + first * per;

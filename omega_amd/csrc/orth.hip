@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_osem(OrthArgs oa, const flo
     float ax = 0.f;
     const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
     if (temp == 0.f) return;
-    float r = meas[lo] / (ax * temp + epps);
+    float r = meas[lo] / ratio_den(oa.a, lo, ax * temp, epps);
     if (isnan(r) || isinf(r)) r = epps;
     if (r == 0.f && sens == nullptr) return;
     if (sens) {
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
             } else {
                 float r = 0.f;
                 if (temp != 0.f) {
-                    r = yin[lo] / (ax * temp + epps);
+                    r = yin[lo] / ratio_den(a, lo, ax * temp, epps);
                     if (isnan(r) || isinf(r)) r = epps;
                     mr = fmaxf(mr, fabsf(r * temp));
                     ms = fmaxf(ms, fabsf(temp));
@@ -618,7 +618,8 @@ static bool orth_planar_check(const omg_geom* g) {
 
 template <bool FP, bool BP, bool SENS>
 static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* yin,
-                              float* yout, float* rhs, float* sens, float epps, hipStream_t s) {
+                              float* yout, float* rhs, float* sens, float epps, hipStream_t s,
+                              const float* add = nullptr) {
     const PlanarShape sh = planar_shape(g, n);
     const uint32_t Nyx = g->d.Nx * g->d.Ny;
     const size_t lds = BP ? (size_t)Nyx * 8 + sizeof(float) * ((size_t)sh.chunk + kPlanarTPB / 64)
@@ -630,7 +631,8 @@ static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr_bytes = lds;
     }
-    const OrthArgs oa = make_orth_args(g, first, n);
+    OrthArgs oa = make_orth_args(g, first, n);
+    oa.a.add = add;
     float* part = nullptr;
     float* part2 = nullptr;
     if (BP) {
@@ -652,10 +654,11 @@ static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const
 // Fused planar FP -> ratio -> BP of [first, first + n) into the partial slices of g->scratch (the caller
 // applies planar_update). Returns OMG_ERR_ARG when the planar kernels do not apply.
 int orth_planar_fused(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas,
-                      int32_t compute_sens, float epps, hipStream_t s) {
+                      int32_t compute_sens, float epps, hipStream_t s, const float* add) {
     OMG_REQUIRE(g->orth_planar, "the orthogonal planar kernels do not apply to this geometry");
-    if (compute_sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, nullptr, nullptr, epps, s);
-    return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, nullptr, nullptr, epps, s);
+    if (compute_sens)
+        return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, nullptr, nullptr, epps, s, add);
+    return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, nullptr, nullptr, epps, s, add);
 }
 
 int orth_prepare(omg_geom* g) {
@@ -694,12 +697,13 @@ int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin,
 }
 
 int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
-              float* sens, float epps, hipStream_t s) {
+              float* sens, float epps, hipStream_t s, const float* add) {
     if (g->orth_planar && !g->force_generic) {
-        if (sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, rhs, sens, epps, s);
-        return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, rhs, nullptr, epps, s);
+        if (sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, rhs, sens, epps, s, add);
+        return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, rhs, nullptr, epps, s, add);
     }
-    const OrthArgs oa = make_orth_args(g, first, n);
+    OrthArgs oa = make_orth_args(g, first, n);
+    oa.a.add = add;
     hipLaunchKernelGGL(k_orth_osem, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, meas, rhs, sens,
                        epps);
     OMG_CHECK_LAUNCH();

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_siddon_osem(SidArgs a, const float* 
     float ax, w = 0.f;
     if (!siddon_gather<false>(a, lo, img, ax, w)) return;
     const float yhat = ax * w;
-    float r = meas[lo] / (yhat + epps);
+    float r = meas[lo] / ratio_den(a, lo, yhat, epps);
     if (isnan(r) || isinf(r)) r = epps;
     if (r == 0.f && sens == nullptr) return;
     if (rhs_t) {
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(TPB) void k_siddon_planar(SidArgs a, uint32_t n_pla
             if (!BP) {
                 yout[lo] = yv;
             } else {
-                float r = yin[lo] / (yv + epps);
+                float r = yin[lo] / ratio_den(a, lo, yv, epps);
                 if (isnan(r) || isinf(r)) r = epps;
                 R[l - l0] = hit ? r : 0.f;
                 if (hit) {
@@ -362,7 +362,7 @@ bool planar_ok(const omg_geom* g) {
 
 template <bool FP, bool BP, bool SENS>
 static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* yin, float* yout,
-                         float* rhs, float* sens, float epps, hipStream_t s) {
+                         float* rhs, float* sens, float epps, hipStream_t s, const float* add = nullptr) {
     const PlanarShape sh = planar_shape(g, n);
     const uint32_t S = g->d.Ny * (g->d.Nx + 1);
     // [slice: S floats (FP only) or S int64 (BP)][R: chunk floats][block_max scratch]
@@ -377,6 +377,7 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
     }
     SidArgs a = make_sid_args(g, first, n);
     a.pitch = g->d.Nx + 1;
+    a.add = add;
     float* part = nullptr;
     float* part2 = nullptr;
     const uint32_t Nyx = g->d.Nx * g->d.Ny;
@@ -554,8 +555,9 @@ static int generic_fp(const omg_geom* g, int64_t first, int64_t n, const float* 
 
 // FP -> ratio -> BP into rhs (and sens), both already holding what the caller wants added to.
 static int generic_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
-                        float* sens, float epps, hipStream_t s) {
-    const SidArgs a = make_sid_args(g, first, n);
+                        float* sens, float epps, hipStream_t s, const float* add) {
+    SidArgs a = make_sid_args(g, first, n);
+    a.add = add;
     float *rhs_t = nullptr, *sens_t = nullptr;
     const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
     if (split_layout(g)) {   // y-fastest accumulators only (see generic_fp)
@@ -597,14 +599,14 @@ static int siddon_common_checks(const omg_geom* g, int64_t first, int64_t n) {
 int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
 int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens, hipStream_t s);
 int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
-              float* sens, float epps, hipStream_t s);
+              float* sens, float epps, hipStream_t s, const float* add);
 int orth_trace(const omg_geom* g, int64_t first, int64_t n, int32_t* counts, const int64_t* offsets, uint32_t* idx,
                float* w, hipStream_t s);
 int orth_planar_fused(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas,
-                      int32_t compute_sens, float epps, hipStream_t s);
+                      int32_t compute_sens, float epps, hipStream_t s, const float* add);
 int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
                          int32_t compute_sens, float epps, const float* dU, int32_t positivity, const int32_t* map,
-                         hipStream_t s);
+                         hipStream_t s, const float* add);
 int bdd_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
 int bdd_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens, hipStream_t s);
 
@@ -672,20 +674,27 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
     return OMG_OK;
 }
 
-extern "C" int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_meas, const float* img,
-                                  const float* meas, float* rhs, float* sens, float epps, void* stream) {
+extern "C" int omg_osem_data_step_ex(const omg_geom* g, int64_t first, int64_t n_meas, const float* img,
+                                     const float* meas, const float* additive, float* rhs, float* sens, float epps,
+                                     void* stream) {
     int rc = siddon_common_checks(g, first, n_meas);
     if (rc) return rc;
     OMG_REQUIRE(img && meas && rhs, "null pointer argument");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "the fused OSEM data step is for emission (PET) data");
     OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM data step is implemented for projector_type 1 and 2");
     if (n_meas == 0) return OMG_OK;
-    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_osem(g, first, n_meas, img, meas, rhs, sens, epps, as_stream(stream));
+    hipStream_t s = as_stream(stream);
+    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_osem(g, first, n_meas, img, meas, rhs, sens, epps, s, additive);
     if (planar_ok(g) && !g->force_generic) {
-        if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, as_stream(stream));
-        return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, as_stream(stream));
+        if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, s, additive);
+        return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, s, additive);
     }
-    return generic_osem(g, first, n_meas, img, meas, rhs, sens, epps, as_stream(stream));
+    return generic_osem(g, first, n_meas, img, meas, rhs, sens, epps, s, additive);
+}
+
+extern "C" int omg_osem_data_step(const omg_geom* g, int64_t first, int64_t n_meas, const float* img,
+                                  const float* meas, float* rhs, float* sens, float epps, void* stream) {
+    return omg_osem_data_step_ex(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, stream);
 }
 
 namespace omg {
@@ -794,6 +803,13 @@ bool unique_slices(const omg_geom* g, int64_t first, uint32_t n_planes) {
 extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas,
                                          float* S, int32_t compute_sens, float epps, float beta, float gamma,
                                          int32_t positivity, void* stream) {
+    return omg_osem_subset_iteration_ex(g, first, n_meas, x, meas, nullptr, S, compute_sens, epps, beta, gamma,
+                                        positivity, stream);
+}
+
+extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, int64_t n_meas, float* x,
+                                            const float* meas, const float* additive, float* S, int32_t compute_sens,
+                                            float epps, float beta, float gamma, int32_t positivity, void* stream) {
     int rc = siddon_common_checks(g, first, n_meas);
     if (rc) return rc;
     OMG_REQUIRE(x && meas && S, "null pointer argument");
@@ -824,18 +840,21 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
             OMG_CHECK_LAUNCH();
             hipEvent_t stop;
             timing_begin(s, &stop);
-            rc = vbp_subset_iteration(gm, first, n_meas, x, meas, S, compute_sens, epps, dU, positivity, map, s);
+            rc = vbp_subset_iteration(gm, first, n_meas, x, meas, S, compute_sens, epps, dU, positivity, map, s,
+                                      additive);
             timing_end(s, stop);
             return rc;
         }
-        if (compute_sens) rc = launch_planar<true, true, true>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
-        else rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s);
+        if (compute_sens)
+            rc = launch_planar<true, true, true>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s, additive);
+        else
+            rc = launch_planar<true, true, false>(g, first, n_meas, x, meas, nullptr, nullptr, nullptr, epps, s, additive);
         if (rc) return rc;
         return planar_update(g, first, n_meas, x, S, dU, compute_sens, epps, positivity, s);
     }
     if (n_meas > 0 && g->d.projector_type == OMG_PROJ_ORTH && g->orth_planar && !g->force_generic &&
         unique_slices(g, first, (uint32_t)(n_meas / per))) {
-        rc = orth_planar_fused(g, first, n_meas, x, meas, compute_sens, epps, s);
+        rc = orth_planar_fused(g, first, n_meas, x, meas, compute_sens, epps, s, additive);
         if (rc) return rc;
         return planar_update(g, first, n_meas, x, S, dU, compute_sens, epps, positivity, s);
     }
@@ -848,10 +867,10 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
         OMG_CHECK_LAUNCH();
     }
     if (n_meas > 0 && g->d.projector_type == OMG_PROJ_ORTH) {
-        rc = orth_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s);
+        rc = orth_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s, additive);
         if (rc) return rc;
     } else if (n_meas > 0) {
-        rc = generic_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s);
+        rc = generic_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s, additive);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_generic_update, dim3(grid_for(N, 256)), dim3(256), 0, s, N, x, S, rhs, dU, epps,

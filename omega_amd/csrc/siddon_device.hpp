@@ -36,7 +36,15 @@ struct SidArgs {
     uint32_t pitch;                // row pitch of slice-relative (planar) voxel indices: Nx, or Nx + 1 in LDS
     const uint32_t* __restrict__ index;  // PET sinogram: sinogram bin per measurement (subset types 1-5), or null;
                                          // plane_offset is then the call's first measurement in this table
+    const float* __restrict__ add;  // additive term of the fused ratio (randoms / scatter), call-local, or null
 };
+
+// Denominator of the fused EM ratio: (A x + epps) + additive (computeForwardProject.h:20-21, 66-69).
+__device__ __forceinline__ float ratio_den(const SidArgs& a, int64_t lo, float yv, float epps) {
+    float d = yv + epps;
+    if (a.add) d += a.add[lo];
+    return d;
+}
 
 struct Ray { float xs, ys, zs, xd, yd, zd; };
 

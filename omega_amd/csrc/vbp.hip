@@ -85,7 +85,7 @@ __global__ __launch_bounds__(TPB) void k_planar_fpq(SidArgs a, uint32_t n_planes
         float ax, w = 0.f;
         const bool hit = siddon_gather<true>(a, lo, lds, ax, w);
         const float yv = hit ? ax * w : 0.f;
-        float r = meas[lo] / (yv + epps);
+        float r = meas[lo] / ratio_den(a, lo, yv, epps);
         if (isnan(r) || isinf(r)) r = epps;
         q[lo] = hit ? r * w : 0.f;
         if (SENS) wq[lo] = hit ? w : 0.f;
@@ -226,7 +226,7 @@ void vbp_release(omg_geom* g) {
 // filled by the caller); dU: OSL gradient or null.
 int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas, float* S,
                          int32_t compute_sens, float epps, const float* dU, int32_t positivity, const int32_t* map,
-                         hipStream_t s) {
+                         hipStream_t s, const float* add) {
     const uint32_t per = g->d.size_x * g->d.size_y;
     const uint32_t n_planes = (uint32_t)(n_meas / per);
     const size_t need = 2 * (size_t)n_meas;
@@ -242,6 +242,7 @@ int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, c
     float* wq = g->vbp_q + n_meas;
     SidArgs a = make_sid_args(g, first, n_meas);
     a.pitch = g->d.Nx + 1;
+    a.add = add;
     const uint32_t groups = (per + kFpqTPB - 1) / kFpqTPB;
     const uint32_t chunk = (per + groups - 1) / groups;
     const size_t lds_fp = (size_t)g->d.Ny * (g->d.Nx + 1) * sizeof(float);

@@ -4,7 +4,8 @@ Restates the subset loop of the reference's built-in reconstruction
 (source/cpp/reconstructionAF.h:1053-1531) for OSEM and OSL-OSEM with the RDP prior:
 
     for iter: for subset k:
-        yhat = A_k x;  r = y_k / (yhat + epps), NaN/Inf -> epps         (computeForwardProject.h:20-21,94,400-401)
+        yhat = A_k x;  r = y_k / ((yhat + epps) + a_k), NaN/Inf -> epps  (computeForwardProject.h:20-21,66-69,94,400-401)
+                       a_k: optional additive randoms / scatter term, 0 without it
         rhs = A_k^T r;  in iteration 0 also S_k = A_k^T 1, S_k < epps -> epps
                                                                          (reconstructionAF.h:1156-1264; functions.hpp:2996)
         OSEM:     x = x / S_k * (rhs + epps)                             (subiterStep.h:111-121)
@@ -34,7 +35,10 @@ class OSEMRunner:
     """OSEM / OSL-OSEM on a :class:`omega_amd.projectorClass` with subset-ordered data on the device."""
 
     def __init__(self, proj, meas, epps: float = 1e-5, beta: float = 0.0, gamma: float = 2.0,
-                 enforce_positivity: bool = True, rdp_boundary: int = 0, stream=None):
+                 enforce_positivity: bool = True, rdp_boundary: int = 0, stream=None, additive=None):
+        """``additive``: optional randoms (+ scatter) estimate, a float32 device tensor in the layout of
+        ``meas`` (subset order), added to the forward projection in the ratio (randoms_correction,
+        computeForwardProject.h:66-69)."""
         import torch
         self.torch = torch
         self.proj = proj
@@ -52,6 +56,12 @@ class OSEMRunner:
         self.beta = float(beta)
         self.gamma = float(gamma)
         self.pos = int(bool(enforce_positivity))
+        self.additive = None
+        if additive is not None:
+            self.additive = additive.reshape(-1).contiguous()
+            if self.additive.dtype != torch.float32 or not self.additive.is_cuda or \
+                    self.additive.numel() != self.meas.numel():
+                raise ValueError("additive must be a float32 device tensor shaped like the measurements")
         self.rdp_boundary = int(rdp_boundary)
         dev = self.meas.device
         self.S = torch.zeros((proj.subsets, self.N), dtype=torch.float32, device=dev)
@@ -70,7 +80,8 @@ class OSEMRunner:
         n = int(proj.nMeasSubset[k, 0])
         off = int(proj.nTotMeas[k])
         yk = self.meas[off:off + n]
-        _lib.call("omg_osem_subset_iteration", self.g, first, n, _p(x), _p(yk), _p(self.S[k]),
+        ak = None if self.additive is None else self.additive[off:off + n]
+        _lib.call("omg_osem_subset_iteration_ex", self.g, first, n, _p(x), _p(yk), _p(ak), _p(self.S[k]),
                   int(not self.sens_done), self.epps, self.beta, self.gamma, self.pos, self._s())
 
     def subset_step_unfused(self, x, k: int) -> None:
@@ -84,8 +95,10 @@ class OSEMRunner:
         if sens is not None:
             sens.zero_()
         yk = self.meas[off:off + n]
+        ak = None if self.additive is None else self.additive[off:off + n]
         self.rhs.zero_()
-        _lib.call("omg_osem_data_step", self.g, first, n, _p(x), _p(yk), _p(self.rhs), _p(sens), self.epps, st)
+        _lib.call("omg_osem_data_step_ex", self.g, first, n, _p(x), _p(yk), _p(ak), _p(self.rhs), _p(sens),
+                  self.epps, st)
         if sens is not None:
             _lib.call("omg_sens_clamp", _p(sens), self.N, self.epps, st)
         if self.beta == 0.0:

@@ -192,6 +192,18 @@ def ratio(y, yhat, epps):
     return r
 
 
+def ratio_additive(y, yhat, add, epps):
+    """The ratio with the randoms / scatter term, in float32 and the reference's order
+    (computeForwardProject.h:20-21, 66-69, 94): a = yhat + epps; a = a + add; r = y / a; NaN/Inf -> epps."""
+    f = np.float32
+    a = (np.asarray(yhat, f) + f(epps)).astype(f)
+    a = (a + np.asarray(add, f)).astype(f)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = (np.asarray(y, f) / a).astype(f)
+    r[~np.isfinite(r)] = f(epps)
+    return r
+
+
 def sens_clamp(s, epps):
     s = np.ascontiguousarray(s, dtype=np.float32).copy()
     lib().oracle_sens_clamp_f32(s, s.size, epps)
@@ -212,10 +224,13 @@ def osl_update(x, S, dU, rhs, epps, positivity=True):
     return x
 
 
-def _ratio64(y, yhat, epps):
-    """oracle_ratio_f32's steps in float64: r = y / (yhat + epps), NaN/Inf -> epps."""
+def _ratio64(y, yhat, epps, add=None):
+    """oracle_ratio_f32's steps in float64: r = y / ((yhat + epps) + add), NaN/Inf -> epps."""
     with np.errstate(divide="ignore", invalid="ignore"):
-        r = np.asarray(y, np.float64) / (np.asarray(yhat, np.float64) + epps)
+        den = np.asarray(yhat, np.float64) + epps
+        if add is not None:
+            den = den + np.asarray(add, np.float64)
+        r = np.asarray(y, np.float64) / den
     r[~np.isfinite(r)] = epps
     return r
 
@@ -235,14 +250,16 @@ def _update64(x, S, dU, rhs, epps, positivity):
 
 
 def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter, x0, epps=1e-5, beta=0.0,
-         gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None, dtype=np.float32):
+         gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None, dtype=np.float32,
+         additive=None):
     """OSEM / OSL-OSEM(RDP) on the CPU in the reference's step order (see omega_amd/recon.py).
 
     ``meas`` is the full measurement vector in subset order. With ``use_ref`` the forward and
     backward projections come from the reference build (oracle/_ref), the elementwise steps from
     the oracle. ``subsets_to_run`` limits the run to the first that many subset-iterations.
     ``dtype=np.float64`` is the FP64 re-run: the same projector and steps in double precision (not
-    with ``use_ref``). Returns the image (float32, or float64 for the FP64 re-run).
+    with ``use_ref``). ``additive`` (optional, laid out like ``meas``) is the randoms / scatter term
+    of the ratio. Returns the image (float32, or float64 for the FP64 re-run).
     """
     f64 = np.dtype(dtype) == np.float64
     assert not (f64 and use_ref), "the reference build is FP32 only"
@@ -260,7 +277,11 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
             case = SiddonCase(grid, size_x, size_y, xy, z_subset_ordered, plane_offset=int(starts[k]), dtype=dtype)
             yk = meas[starts[k] * per: starts[k] * per + n]
             yhat = case.ref_fp(x, n, nthreads) if use_ref else case.fp(x, n, nthreads)
-            r = _ratio64(yk, yhat, epps) if f64 else ratio(yk, yhat, epps)
+            if additive is not None:
+                ak = additive[starts[k] * per: starts[k] * per + n]
+                r = _ratio64(yk, yhat, epps, ak) if f64 else ratio_additive(yk, yhat, ak, epps)
+            else:
+                r = _ratio64(yk, yhat, epps) if f64 else ratio(yk, yhat, epps)
             if it == 0:
                 rhs, s = (case.ref_bp if use_ref else case.bp)(r, n, sens=True, nthreads=nthreads)
                 S[k] = _sens_clamp64(s, epps) if f64 else sens_clamp(s, epps)

@@ -97,16 +97,16 @@ def test_elementwise_bitexact(cuda):
     assert np.array_equal(xt.cpu().numpy(), O.osl_update(x, S, dU, rhs, 1e-5))
 
 
-def _oracle_run(p, y, n_sub, beta=0.0, dtype=np.float32):
+def _oracle_run(p, y, n_sub, beta=0.0, dtype=np.float32, additive=None):
     nmeas = np.diff(p.nMeas).astype(np.int64)
     return O.osem(p.grid, p.nRowsD, p.nColsD, p.x, p.z, nmeas, y.cpu().numpy(), 4,
                   np.ones(int(p.N[0]), np.float32), epps=1e-5, beta=beta, gamma=2.0, subsets_to_run=n_sub,
-                  dtype=dtype)
+                  dtype=dtype, additive=None if additive is None else additive.cpu().numpy())
 
 
-def _gpu_run(p, y, n_sub, beta=0.0, generic=False, unfused=False, voxel_bp=False):
+def _gpu_run(p, y, n_sub, beta=0.0, generic=False, unfused=False, voxel_bp=False, additive=None):
     import torch
-    run = OSEMRunner(p, y, beta=beta, gamma=2.0)
+    run = OSEMRunner(p, y, beta=beta, gamma=2.0, additive=additive)
     if generic or voxel_bp:
         d, keep = p._descriptor()
         if generic:
@@ -148,6 +148,29 @@ def test_osl_osem_rdp_iterations_match_oracle(pet2d):
     print(f"[osl-osem] rel-L2 vs oracle {e:.2e}")
     assert e < TOL
     assert rel_l2(_gpu_run(p, y, 10, beta=0.1, unfused=True), xo) < TOL
+
+
+@pytest.mark.parametrize("path", ["planar_lds", "generic", "unfused", "voxel_bp"])
+def test_osem_with_randoms_term(pet2d, path):
+    """The ratio's randoms / scatter term, y / ((A x + eps) + a): every path within 1e-5 of the oracle
+    run with the same term (the voxel-driven path against the FP64 re-run), and the term changes the
+    result (a run without it differs by far more than the tolerance)."""
+    import torch
+    p, y = pet2d
+    n_sub = 10
+    rng = np.random.default_rng(12)
+    add = torch.from_numpy((rng.random(y.numel()) * 0.3 + 0.05).astype(np.float32)).cuda()
+    print(f"[osem randoms] path={path}", flush=True)
+    xg = _gpu_run(p, y, n_sub, generic=path == "generic", unfused=path == "unfused", voxel_bp=path == "voxel_bp",
+                  additive=add)
+    if path == "voxel_bp":
+        xo = _oracle_run(p, y, n_sub, dtype=np.float64, additive=add)
+    else:
+        xo = _oracle_run(p, y, n_sub, additive=add)
+    e = rel_l2(xg, xo)
+    print(f"  rel-L2 vs oracle {e:.2e}; without the term {rel_l2(_oracle_run(p, y, n_sub), xo):.2e}", flush=True)
+    assert e < TOL
+    assert rel_l2(_oracle_run(p, y, n_sub), xo) > 100 * TOL
 
 
 @pytest.mark.parametrize("beta", [0.0, 0.1])

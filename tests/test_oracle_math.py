@@ -198,3 +198,19 @@ def test_rdp_gradient_matches_value_derivative():
         a = u[j]
         ghost_part = sum(a * (a + gamma * a) / (a + gamma * a) ** 2 for gh in ghosts if gh)
         assert abs(fd - (2 * g[j] - ghost_part)) < 1e-5 * max(1.0, abs(fd))
+
+
+def test_ratio_additive():
+    """The ratio with an additive randoms term: with a zero term it is the plain ratio bit for bit;
+    otherwise y / ((yhat + eps) + a) to float rounding; zero denominators give eps."""
+    rng = np.random.default_rng(8)
+    y = rng.poisson(3.0, 5000).astype(np.float32)
+    yhat = (rng.random(5000) * 4).astype(np.float32)
+    yhat[::97] = -1e-5                                   # yhat + eps == 0
+    r0 = O.ratio_additive(y, yhat, np.zeros_like(yhat), 1e-5)
+    assert np.array_equal(r0, O.ratio(y, yhat, 1e-5))
+    add = (rng.random(5000) * 0.5 + 0.01).astype(np.float32)
+    r = O.ratio_additive(y, yhat, add, 1e-5)
+    exact = y.astype(np.float64) / (yhat.astype(np.float64) + 1e-5 + add)
+    ok = np.isfinite(exact)
+    assert np.max(np.abs(r[ok] - exact[ok]) / np.maximum(np.abs(exact[ok]), 1e-30)) < 4e-7
