@@ -4,9 +4,9 @@
 // Layout: activations are channels-last, act[voxel][channel] with voxel = (z * H + y) * W + x, channel
 // counts multiples of 16; conv weights are w[tap][cout][cin] with tap = (dz * 3 + dy) * 3 + dx.
 // A 3x3x3 convolution (zero padding 1) is an implicit GEMM: Y[v][co] = sum_{tap, ci} W[tap][co][ci] *
-// X[v + off(tap)][ci]. One wave computes 16 voxels x all output channels; per (tap, 16-channel block)
-// each lane feeds the matrix core 4 consecutive input channels of one voxel (an 8-byte load) and 4
-// consecutive input channels of one weight row. Lane l of a 16x16x16 MFMA holds A[l % 16][4 (l / 16) + j],
+// X[v + off(tap)][ci]. One wave computes 4 tiles of 16 voxels x all output channels; per (tap, 16-channel
+// block) each lane feeds the matrix core 4 consecutive input channels of one voxel per tile (8-byte
+// loads) and 4 consecutive input channels of one weight row, reused by the 4 tiles. Lane l of a 16x16x16 MFMA holds A[l % 16][4 (l / 16) + j],
 // B[4 (l / 16) + j][l % 16] and receives D[4 (l / 16) + j][l % 16], j = 0..3.
 //
 // Backward: dX is the same convolution of dY with the flipped, transposed weights (k_conv_flipT);
@@ -39,58 +39,88 @@ __device__ __forceinline__ bool neighbour(const Vol& g, int v, int tap, int& nb)
 
 __device__ __forceinline__ bf4 load4(const uint16_t* p) { return *reinterpret_cast<const bf4*>(p); }
 
-// Y = act(conv(X) + b), X = [X1 | X2] (channel concatenation), MT = Cout / 16. 4 waves x 16 voxels per block.
+// Y = act(conv(X) + b), X = [X1 | X2] (channel concatenation), MT = Cout / 16. A wave computes NT tiles of
+// 16 consecutive voxels x all output channels: per (tap, 16-channel block) it loads NT activation
+// fragments and, for each of the MT output blocks, one weight fragment that feeds NT matrix-core
+// operations (weights reused NT times from registers). 4 waves per block.
+constexpr int kConvNT = 4;
+
 template <int MT>
 __global__ __launch_bounds__(256) void k_conv3d_fwd(const uint16_t* __restrict__ X1, int C1,
                                                     const uint16_t* __restrict__ X2, int C2,
                                                     const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                                     uint16_t* __restrict__ Y, Vol g, int act) {
+    constexpr int NT = kConvNT;
     const int V = g.D * g.H * g.W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int base = (blockIdx.x * 4 + wave) * 16;
+    const int base = (blockIdx.x * 4 + wave) * 16 * NT;
     if (base >= V) return;
     const int Cin = C1 + C2, Cout = MT * 16;
     const int col = lane & 15, kq = (lane >> 4) * 4;
-    const int v = base + col;
-    const bool vin = v < V;
-    f4v acc[MT];
+    int v[NT];
+    bool vin[NT];
 #pragma unroll
-    for (int m = 0; m < MT; m++) acc[m] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < NT; t++) {
+        v[t] = base + t * 16 + col;
+        vin[t] = v[t] < V;
+    }
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = f4v{0.f, 0.f, 0.f, 0.f};
     for (int tap = 0; tap < 27; tap++) {
-        int nb = 0;
-        const bool ok = vin && neighbour(g, v, tap, nb);
+        int nb[NT];
+        bool ok[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            nb[t] = 0;
+            ok[t] = vin[t] && neighbour(g, v[t], tap, nb[t]);
+        }
         for (int cb = 0; cb < Cin; cb += 16) {
             const int c = cb + kq;
-            bf4 b = bf4{0, 0, 0, 0};
-            if (ok) b = (c < C1) ? load4(X1 + (size_t)nb * C1 + c) : load4(X2 + (size_t)nb * C2 + (c - C1));
+            bf4 b[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                b[t] = bf4{0, 0, 0, 0};
+                if (ok[t]) b[t] = (c < C1) ? load4(X1 + (size_t)nb[t] * C1 + c) : load4(X2 + (size_t)nb[t] * C2 + (c - C1));
+            }
             const uint16_t* wrow = Wt + ((size_t)tap * Cout + col) * Cin + c;
 #pragma unroll
             for (int m = 0; m < MT; m++) {
                 const bf4 a = load4(wrow + (size_t)m * 16 * Cin);
-                acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[m][t], 0, 0, 0);
             }
         }
     }
-    if (!vin) return;
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
-        const int co = m * 16 + kq;
-        uint16_t o[4];
+    for (int t = 0; t < NT; t++) {
+        if (!vin[t]) continue;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            float val = acc[m][j] + (bias ? bias[co + j] : 0.f);
-            if (act == 1) val = fmaxf(val, 0.f);
-            o[j] = f2bf(val);
+        for (int m = 0; m < MT; m++) {
+            const int co = m * 16 + kq;
+            uint16_t o[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float val = acc[m][t][j] + (bias ? bias[co + j] : 0.f);
+                if (act == 1) val = fmaxf(val, 0.f);
+                o[j] = f2bf(val);
+            }
+            uint2 pk;
+            pk.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+            pk.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            *reinterpret_cast<uint2*>(Y + (size_t)v[t] * Cout + co) = pk;
         }
-        uint2 pk;
-        pk.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-        pk.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-        *reinterpret_cast<uint2*>(Y + (size_t)v * Cout + co) = pk;
     }
 }
 
 // dW[tap][co][ci] += sum over this wave's voxels of dY[v][co] * X[v + off(tap)][ci]. grid = (chunks, 27).
-template <int MT, int NT>
+// ROWS (W % 16 == 0, vox_per_wave % 16 == 0): a 16-voxel K block lies in one image row, so the row's
+// (y, z) and the tap's row offset are found once per block and each voxel's neighbour is v + offset with
+// only its x checked (instead of a division-based neighbour() per voxel).
+template <int MT, int NT, bool ROWS>
 __global__ __launch_bounds__(256) void k_conv3d_wgrad(const uint16_t* __restrict__ dY,
                                                       const uint16_t* __restrict__ X1, int C1,
                                                       const uint16_t* __restrict__ X2, int C2,
@@ -108,14 +138,28 @@ __global__ __launch_bounds__(256) void k_conv3d_wgrad(const uint16_t* __restrict
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[m][n] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int toff = (dz * g.H + dy) * g.W + dx;
     for (int kb = v0; kb < v1; kb += 16) {
         // this lane's 4 voxels of the K block
         int nbv[4];
         bool okv[4];
+        if (ROWS) {
+            const int row = kb / g.W, y = row % g.H, z = row / g.H;
+            const bool row_ok = z + dz >= 0 && z + dz < g.D && y + dy >= 0 && y + dy < g.H;
+            const int x0 = kb - row * g.W + kq + dx;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int v = kb + kq + j;
-            okv[j] = v < v1 && neighbour(g, v, tap, nbv[j]);
+            for (int j = 0; j < 4; j++) {
+                const int v = kb + kq + j;
+                nbv[j] = v + toff;
+                okv[j] = row_ok && v < v1 && x0 + j >= 0 && x0 + j < g.W;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int v = kb + kq + j;
+                okv[j] = v < v1 && neighbour(g, v, tap, nbv[j]);
+            }
         }
         bf4 a[MT];
 #pragma unroll
@@ -160,14 +204,21 @@ __global__ void k_conv_flipT(const uint16_t* __restrict__ W, uint16_t* __restric
     Wf[((size_t)(26 - tap) * Cin + ci) * Cout + co] = W[i];
 }
 
-// db[c] += sum_v dY[v][c]
-__global__ void k_bias_grad(const uint16_t* __restrict__ dY, float* __restrict__ db, int V, int C) {
+// db[c] += sum_v dY[v][c]: per-thread partial sums, reduced over the block's rows in LDS, then one
+// atomic per channel per block (a float atomic per thread onto C addresses serialises).
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ dY, float* __restrict__ db, int V, int C) {
+    __shared__ float part[256];
     const int c = threadIdx.x % C;
     const int lanes = 256 / C;     // C divides 256 for C in {16, 32, 64}
     const int r = threadIdx.x / C;
     float s = 0.f;
     for (int v = blockIdx.x * lanes + r; v < V; v += gridDim.x * lanes) s += bf2f(dY[(size_t)v * C + c]);
-    unsafeAtomicAdd(db + c, s);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (r == 0) {
+        for (int k = 1; k < lanes; k++) s += part[k * C + c];
+        unsafeAtomicAdd(db + c, s);
+    }
 }
 
 // dZ = (Y > 0) ? dY : 0 over n elements (in place allowed)
@@ -281,19 +332,25 @@ __global__ void k_head_bwd(const uint16_t* __restrict__ X, const float* __restri
     if (threadIdx.x == 0) unsafeAtomicAdd(db, red[C][0] + red[C][1] + red[C][2] + red[C][3]);
 }
 
-// Masked mean squared error: loss += sum m (out - t)^2 / n_valid (FP64 accumulation); g = 2 m (out - t) / n_valid
-__global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t, const float* __restrict__ mask,
-                      float* __restrict__ g, double* __restrict__ loss, int V, float inv_n) {
-    const int v = blockIdx.x * 256 + threadIdx.x;
+// Masked mean squared error: loss += sum m (out - t)^2 / n_valid (FP64 accumulation); g = 2 m (out - t) / n_valid.
+// Grid-stride over a capped grid with the block's sum reduced in LDS: one FP64 atomic per block (atomics
+// on the one loss address serialise, so one per wave costs more than the whole pass).
+constexpr int kMseBlocks = 512;
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out, const float* __restrict__ t,
+                                             const float* __restrict__ mask, float* __restrict__ g,
+                                             double* __restrict__ loss, int V, float inv_n) {
+    __shared__ double part[4];
     double e2 = 0.0;
-    if (v < V) {
+    for (int v = blockIdx.x * 256 + threadIdx.x; v < V; v += gridDim.x * 256) {
         const float m = mask ? mask[v] : 1.f;
         const float e = (out[v] - t[v]) * m;
         g[v] = 2.f * e * inv_n;
-        e2 = (double)e * e * inv_n;
+        e2 += (double)e * e * inv_n;
     }
     for (int o = 32; o > 0; o >>= 1) e2 += __shfl_xor(e2, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, e2);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e2;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
 }
 
 // Adam on FP32 master weights p with moments m, v; writes the BF16 copy q (q may be null).
@@ -335,8 +392,9 @@ template <int MT>
 static void launch_fwd(const uint16_t* X1, int C1, const uint16_t* X2, int C2, const uint16_t* W, const float* b,
                        uint16_t* Y, Vol g, int act, hipStream_t s) {
     const int V = g.D * g.H * g.W;
-    hipLaunchKernelGGL(k_conv3d_fwd<MT>, dim3((unsigned)((V + 63) / 64)), dim3(256), 0, s, X1, C1, X2, C2, W, b, Y, g,
-                       act);
+    const int per_block = 4 * 16 * kConvNT;
+    hipLaunchKernelGGL(k_conv3d_fwd<MT>, dim3((unsigned)((V + per_block - 1) / per_block)), dim3(256), 0, s, X1, C1, X2,
+                       C2, W, b, Y, g, act);
 }
 
 template <int MT, int NT>
@@ -344,8 +402,11 @@ static void launch_wgrad(const uint16_t* dY, const uint16_t* X1, int C1, const u
                          hipStream_t s) {
     const int V = g.D * g.H * g.W;
     const int vpw = 1024;
-    hipLaunchKernelGGL((k_conv3d_wgrad<MT, NT>), dim3((unsigned)((V + 4 * vpw - 1) / (4 * vpw)), 27), dim3(256), 0, s,
-                       dY, X1, C1, X2, C2, dW, g, vpw);
+    const dim3 grid((unsigned)((V + 4 * vpw - 1) / (4 * vpw)), 27);
+    if (g.W % 16 == 0)
+        hipLaunchKernelGGL((k_conv3d_wgrad<MT, NT, true>), grid, dim3(256), 0, s, dY, X1, C1, X2, C2, dW, g, vpw);
+    else
+        hipLaunchKernelGGL((k_conv3d_wgrad<MT, NT, false>), grid, dim3(256), 0, s, dY, X1, C1, X2, C2, dW, g, vpw);
 }
 
 }  // namespace omg
@@ -402,7 +463,7 @@ extern "C" int omg_conv3d_flipT(const uint16_t* W, uint16_t* Wf, int32_t Cout, i
 }
 
 extern "C" int omg_bias_grad(const uint16_t* dY, float* db, int32_t V, int32_t C, void* stream) {
-    OMG_REQUIRE(dY && db && (C == 16 || C == 32 || C == 64), "bias gradient needs C in {16, 32, 64}");
+    OMG_REQUIRE(dY && db && V > 0 && (C == 16 || C == 32 || C == 64), "bias gradient needs V > 0, C in {16, 32, 64}");
     hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), 0, as_stream(stream), dY, db, V, C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -417,7 +478,8 @@ extern "C" int omg_relu_backward(const uint16_t* Y, uint16_t* dY, int64_t n, voi
 
 extern "C" int omg_pool_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32_t H, int32_t W, int32_t C,
                                 void* stream) {
-    OMG_REQUIRE(X && Y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "pooling needs even dimensions");
+    OMG_REQUIRE(X && Y && C > 0 && vol_ok(D, H, W) && D % 2 == 0 && H % 2 == 0 && W % 2 == 0,
+                "pooling needs even positive dimensions");
     hipLaunchKernelGGL(k_pool_fwd, dim3(blocks((size_t)D * H * W * C / 8)), dim3(256), 0, as_stream(stream), X, Y,
                        Vol{D, H, W}, C);
     OMG_CHECK_LAUNCH();
@@ -426,7 +488,8 @@ extern "C" int omg_pool_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32
 
 extern "C" int omg_pool_backward(const uint16_t* dY, uint16_t* dX, int32_t D, int32_t H, int32_t W, int32_t C,
                                  int32_t accumulate, void* stream) {
-    OMG_REQUIRE(dY && dX && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "pooling needs even dimensions");
+    OMG_REQUIRE(dY && dX && C > 0 && vol_ok(D, H, W) && D % 2 == 0 && H % 2 == 0 && W % 2 == 0,
+                "pooling needs even positive dimensions");
     hipLaunchKernelGGL(k_pool_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, dX,
                        Vol{D, H, W}, C, accumulate);
     OMG_CHECK_LAUNCH();
@@ -435,7 +498,8 @@ extern "C" int omg_pool_backward(const uint16_t* dY, uint16_t* dX, int32_t D, in
 
 extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, int32_t H, int32_t W, int32_t C,
                                     void* stream) {
-    OMG_REQUIRE(X && Y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "upsampling needs an even fine grid");
+    OMG_REQUIRE(X && Y && C > 0 && vol_ok(D, H, W) && D % 2 == 0 && H % 2 == 0 && W % 2 == 0,
+                "upsampling needs an even positive fine grid");
     hipLaunchKernelGGL(k_up_fwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), X, Y,
                        Vol{D, H, W}, C);
     OMG_CHECK_LAUNCH();
@@ -444,7 +508,7 @@ extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, i
 
 extern "C" int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off, uint16_t* dX, int32_t D, int32_t H,
                                      int32_t W, int32_t C, void* stream) {
-    OMG_REQUIRE(dY && dX && off >= 0 && off + C <= Cs, "bad channel window");
+    OMG_REQUIRE(dY && dX && vol_ok(D, H, W) && C > 0 && off >= 0 && off + C <= Cs, "bad channel window");
     hipLaunchKernelGGL(k_up_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, Cs, off, dX,
                        Vol{D, H, W}, C);
     OMG_CHECK_LAUNCH();
@@ -453,7 +517,7 @@ extern "C" int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off
 
 extern "C" int omg_take_channels(const uint16_t* src, int32_t Cs, int32_t off, uint16_t* dst, int64_t V, int32_t C,
                                  int32_t accumulate, void* stream) {
-    OMG_REQUIRE(src && dst && off >= 0 && off + C <= Cs, "bad channel window");
+    OMG_REQUIRE(src && dst && V > 0 && C > 0 && off >= 0 && off + C <= Cs, "bad channel window");
     hipLaunchKernelGGL(k_take_channels, dim3(blocks((size_t)V * C)), dim3(256), 0, as_stream(stream), src, Cs, off, dst,
                        (size_t)V, C, accumulate);
     OMG_CHECK_LAUNCH();
@@ -462,7 +526,7 @@ extern "C" int omg_take_channels(const uint16_t* src, int32_t Cs, int32_t off, u
 
 extern "C" int omg_head_forward(const uint16_t* X, const float* w, const float* b, const float* base, float* out,
                                 int32_t V, int32_t C, void* stream) {
-    OMG_REQUIRE(X && w && b && base && out && C > 0 && C < 64, "bad arguments");
+    OMG_REQUIRE(X && w && b && base && out && V > 0 && C > 0 && C < 64, "bad arguments");
     hipLaunchKernelGGL(k_head_fwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, b, base, out, V, C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -470,7 +534,7 @@ extern "C" int omg_head_forward(const uint16_t* X, const float* w, const float* 
 
 extern "C" int omg_head_backward(const uint16_t* X, const float* w, const float* g, uint16_t* dX, float* dw, float* db,
                                  int32_t V, int32_t C, void* stream) {
-    OMG_REQUIRE(X && w && g && dX && dw && db && C > 0 && C < 64, "bad arguments");
+    OMG_REQUIRE(X && w && g && dX && dw && db && V > 0 && C > 0 && C < 64, "bad arguments");
     hipLaunchKernelGGL(k_head_bwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, g, dX, dw, db, V, C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -478,8 +542,9 @@ extern "C" int omg_head_backward(const uint16_t* X, const float* w, const float*
 
 extern "C" int omg_mse(const float* out, const float* t, const float* mask, float* g, double* loss, int32_t V,
                        float inv_n, void* stream) {
-    OMG_REQUIRE(out && t && g && loss, "bad arguments");
-    hipLaunchKernelGGL(k_mse, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), out, t, mask, g, loss, V, inv_n);
+    OMG_REQUIRE(out && t && g && loss && V > 0, "bad arguments");
+    const unsigned nb = (unsigned)std::min<size_t>(blocks((size_t)V), kMseBlocks);
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, as_stream(stream), out, t, mask, g, loss, V, inv_n);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -503,7 +568,7 @@ extern "C" int omg_to_bf16(const float* src, uint16_t* dst, int64_t n, void* str
 
 extern "C" int omg_embed(const float* img, uint16_t* X, int32_t D, int32_t H, int32_t W, int32_t Nz, float scale,
                          void* stream) {
-    OMG_REQUIRE(img && X && Nz <= D, "bad arguments");
+    OMG_REQUIRE(img && X && vol_ok(D, H, W) && Nz >= 1 && Nz <= D, "bad arguments");
     hipLaunchKernelGGL(k_embed, dim3(blocks((size_t)D * H * W * 16)), dim3(256), 0, as_stream(stream), img, X,
                        Vol{D, H, W}, Nz, scale);
     OMG_CHECK_LAUNCH();

@@ -109,3 +109,16 @@ def test_elementwise_argument_checks():
     assert lib.omg_ratio(None, None, 10, 1e-5, None) == _lib.OMG_ERR_ARG
     assert lib.omg_rdp_gradient(None, None, 4, 4, 4, 2.0, 1e-5, 0.1, 0, None) == _lib.OMG_ERR_ARG
     assert lib.omg_pdhg_update(C.c_void_p(8), C.c_void_p(8), None, 10, 1e-5, 1.0, 0.1, 1, 0, None) == _lib.OMG_ERR_ARG
+
+
+def test_unet_size_checks():
+    """Zero or negative sizes are refused before any launch (an empty grid is not a valid launch)."""
+    lib = _lib.load()
+    p = C.c_void_p(256)        # never dereferenced: each call fails its argument check first
+    assert lib.omg_mse(p, p, None, p, p, 0, C.c_float(1.0), None) == _lib.OMG_ERR_ARG
+    assert lib.omg_bias_grad(p, p, 0, 16, None) == _lib.OMG_ERR_ARG
+    assert lib.omg_pool_forward(p, p, 0, 2, 2, 16, None) == _lib.OMG_ERR_ARG
+    assert lib.omg_upsample_forward(p, p, 2, 2, 2, 0, None) == _lib.OMG_ERR_ARG
+    assert lib.omg_head_forward(p, p, p, p, p, 0, 16, None) == _lib.OMG_ERR_ARG
+    assert lib.omg_embed(p, p, 4, 4, 4, 0, C.c_float(1.0), None) == _lib.OMG_ERR_ARG
+    assert lib.omg_conv3d_forward(p, 16, None, 0, p, None, p, 16, 0, 4, 4, 0, None) == _lib.OMG_ERR_ARG
