@@ -81,6 +81,18 @@ typedef struct omg_geom_desc {
      * meas_index[first + i] (radial bin fastest, then angle, then plane), and `norm` is in the same
      * (subset) order. The direct-plane kernels do not apply; frame batching is not offered. */
     const uint32_t* meas_index; int64_t meas_index_len;
+    /* attenuation correction of PET data (projectorClass.attenuation_correction, CTAttenuation, vaimennus:
+     * source/Python/omegatomo/projector/proj.py:85,296,358; kernel options -DATN / -DATNM, init.py:318-321).
+     * atten_mode 0: off. 1: `atten` is the attenuation image (Nx*Ny*Nz floats, 1/mm, x fastest); the
+     * library computes each measurement's factor exp(-sum len * mu) over its Siddon walk once, here (the
+     * reference integrates it inside every projection, projector_functions.h:1754-1767,1938-1939). 2:
+     * `atten` holds one factor per measurement, in the order of `norm`. The factor multiplies the LOR
+     * weight of every projector and of the fused steps. */
+    int32_t atten_mode; const float* atten; int64_t atten_len;
+    /* optional multiplicative correction per measurement (projectorClass.additionalCorrection / corrVector,
+     * kernel option -DSCATTER, init.py:324,592-595; paramStruct::scatterCoefs, projector_functions.h:1436-1440),
+     * in the order of `norm`; it multiplies the LOR weight after norm. PET data only. */
+    const float* scat; int64_t scat_len;
 } omg_geom_desc;
 
 /* Library/version info and error text. */

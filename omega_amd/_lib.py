@@ -51,6 +51,8 @@ class GeomDesc(C.Structure):
         ("dPitchY", C.c_float),
         ("reserved", C.c_int32 * 8),
         ("meas_index", C.c_void_p), ("meas_index_len", C.c_int64),
+        ("atten_mode", C.c_int32), ("atten", C.c_void_p), ("atten_len", C.c_int64),
+        ("scat", C.c_void_p), ("scat_len", C.c_int64),
     ]
 
 

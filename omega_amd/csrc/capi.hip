@@ -82,6 +82,31 @@ static int upload(const float* src, int64_t n, float** dst) {
     return OMG_OK;
 }
 
+// Measurements the tables describe: the length of a per-measurement table (norm, LOR attenuation).
+static int64_t total_measurements(const omg_geom_desc& d) {
+    if (d.mode == OMG_MODE_LIST6) return d.x_len / 6;
+    if (d.meas_index) return d.meas_index_len;
+    return (int64_t)d.size_x * d.size_y * d.n_proj;
+}
+
+// Attenuation image -> one factor per measurement (g->atten_dev), computed once on the device.
+static int attenuation_prepare(omg_geom* g, const float* mu_host, int64_t n_total) {
+    const int64_t N = (int64_t)g->d.Nx * g->d.Ny * g->d.Nz;
+    float* mu = nullptr;
+    int rc = upload(mu_host, N, &mu);
+    if (!rc && hipMalloc(reinterpret_cast<void**>(&g->atten_dev), (size_t)n_total * sizeof(float)) != hipSuccess) {
+        set_error("could not allocate %lld attenuation factors", (long long)n_total);
+        rc = OMG_ERR_HIP;
+    }
+    if (!rc) rc = atten_factors(g, mu, g->atten_dev, n_total, nullptr);
+    if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) {
+        set_error("attenuation factor computation failed");
+        rc = OMG_ERR_HIP;
+    }
+    hipFree(mu);
+    return rc;
+}
+
 int bdd_prepare(omg_geom* g);   // BDD per-geometry set-up (bdd.hip)
 int vbp_prepare(omg_geom* g, const float* xt);  // voxel-driven planar backprojection set-up (vbp.hip)
 int orth_prepare(omg_geom* g);  // orthogonal projector set-up (orth.hip)
@@ -152,6 +177,29 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
                         (long long)i, d->meas_index[i], (unsigned long long)bins);
     }
 
+    const int64_t n_total = total_measurements(*d);
+    if (d->norm)
+        OMG_REQUIRE(d->norm_len >= n_total, "norm holds %lld floats, the data %lld measurements",
+                    (long long)d->norm_len, (long long)n_total);
+    if (d->scat) {
+        OMG_REQUIRE(d->mode != OMG_MODE_CT_FLAT, "the multiplicative (scatter) correction applies to PET data, not CT");
+        OMG_REQUIRE(d->scat_len >= n_total, "scat holds %lld floats, the data %lld measurements",
+                    (long long)d->scat_len, (long long)n_total);
+    }
+    OMG_REQUIRE(d->atten_mode >= 0 && d->atten_mode <= 2, "atten_mode %d is not 0 (off), 1 (image) or 2 (per LOR)",
+                d->atten_mode);
+    if (d->atten_mode) {
+        OMG_REQUIRE(d->mode != OMG_MODE_CT_FLAT, "attenuation correction applies to PET data, not CT");
+        OMG_REQUIRE(d->atten != nullptr, "atten_mode %d without an atten table", d->atten_mode);
+        if (d->atten_mode == 1)
+            OMG_REQUIRE(d->atten_len == (int64_t)d->Nx * d->Ny * d->Nz,
+                        "attenuation image holds %lld floats, the image grid %llu voxels", (long long)d->atten_len,
+                        (unsigned long long)d->Nx * d->Ny * d->Nz);
+        else
+            OMG_REQUIRE(d->atten_len >= n_total, "attenuation table holds %lld floats, the data %lld measurements",
+                        (long long)d->atten_len, (long long)n_total);
+    }
+
     omg_geom* g = new (std::nothrow) omg_geom();
     OMG_REQUIRE(g != nullptr, "out of host memory");
     g->d = *d;
@@ -173,6 +221,9 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
             rc = OMG_ERR_HIP;
         }
     }
+    if (!rc && d->scat) rc = upload(d->scat, n_total, &g->scat_dev);
+    if (!rc && d->atten_mode == 2) rc = upload(d->atten, n_total, &g->atten_dev);
+    if (!rc && d->atten_mode == 1) rc = attenuation_prepare(g, d->atten, n_total);
     if (!rc && d->mode == OMG_MODE_PET_SINO) {
         g->z_host.assign(d->z, d->z + d->z_len);
         g->planar = d->meas_index == nullptr;
@@ -182,7 +233,8 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     if (!rc && d->mode == OMG_MODE_PET_SINO) rc = vbp_prepare(g, d->x);
     if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
     if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);
-    g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr; g->d.meas_index = nullptr;
+    g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr; g->d.meas_index = nullptr; g->d.atten = nullptr;
+    g->d.scat = nullptr;
     if (rc) {
         omg_geom_destroy(g);
         return rc;
@@ -200,6 +252,8 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->x_dev);
     hipFree(g->z_dev);
     hipFree(g->norm_dev);
+    hipFree(g->atten_dev);
+    hipFree(g->scat_dev);
     hipFree(g->index_dev);
     hipFree(g->frames_t);
     hipFree(g->scratch);

@@ -10,6 +10,8 @@ struct omg_geom {
     float* x_dev = nullptr;   // coordinate tables on the device
     float* z_dev = nullptr;
     float* norm_dev = nullptr;
+    float* atten_dev = nullptr;      // per-measurement attenuation factors (norm's layout), or null
+    float* scat_dev = nullptr;       // per-measurement multiplicative correction (norm's layout), or null
     uint32_t* index_dev = nullptr;   // measurement-level subsets: sinogram bin per measurement (subset order)
     float bmaxx = 0.f, bmaxy = 0.f, bmaxz = 0.f;
     int device = 0;
@@ -61,16 +63,19 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.dPitchZ = g->d.dPitchZ;
     a.global_factor = g->d.global_factor;
     a.x = g->x_dev; a.z = g->z_dev;
-    a.norm = nullptr;
     a.pitch = g->d.Nx;
     a.index = g->index_dev;
     a.add = nullptr;
-    if (g->norm_dev) {
-        const int64_t per = (g->d.mode == MODE_LIST6 || g->index_dev) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
-        a.norm = g->norm_dev + first * per;
-    }
+    const int64_t per = (g->d.mode == MODE_LIST6 || g->index_dev) ? 1 : (int64_t)g->d.size_x * g->d.size_y;
+    a.norm = g->norm_dev ? g->norm_dev + first * per : nullptr;
+    a.atten = g->atten_dev ? g->atten_dev + first * per : nullptr;
+    a.scat = g->scat_dev ? g->scat_dev + first * per : nullptr;
+    a.atten_lor = g->d.atten_mode == 2;
     return a;
 }
+
+// Writes f[i] = exp(-sum len * mu) over the Siddon walk of measurements [0, n) (siddon.hip).
+int atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s);
 
 // Validates a [first, first + n) measurement range against the tables; returns OMG_OK or OMG_ERR_ARG.
 int check_range(const omg_geom* g, int64_t first, int64_t n);

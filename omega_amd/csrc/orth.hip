@@ -1,7 +1,7 @@
 // orth.hip -- orthogonal distance-based projector (projector type 2) for gfx950.
 //
 // Semantics are those of the reference's GPU kernel, projectorType123.cl built with -DORTH and
-// -DCRYSTXY / -DCRYSTZ (no TOF, attenuation, masks or multi-ray), because the reference's CPU version
+// -DCRYSTXY / -DCRYSTZ (no TOF, masks or multi-ray), because the reference's CPU version
 // of this projector is one-sided (projector_functions.h:1121-1123,1145-1147):
 //   * a Siddon walk (GPU set-up: siddon_pre_loop_2D/3D, general_opencl_functions.h:1825-1957) fixes the
 //     slab axis and, at every crossing of a slab boundary, the row/plane the ray leaves the slab through
@@ -12,7 +12,9 @@
 //   * slabs before the first and after the last crossing are swept until one yields no voxel
 //     (projectorType123.cl:1055-1128,1235-1313);
 //   * rays parallel to x or y walk their row of slabs directly (projectorType123.cl:513-791).
-// FP: y = temp * sum w x; BP: rhs += r * (w * temp), sens += w * temp, temp = norm * global_factor.
+// FP: y = temp * sum w x; BP: rhs += r * (w * temp), sens += w * temp, temp = atten * norm * scat * global_factor,
+// atten the LOR's attenuation factor exp(-sum len mu) over its Siddon walk (projectorType123.cl:1022-1048,
+// :1220-1233: the reference's -DORTH -DATN integrates mu along the same Siddon walk).
 //
 // One lane follows one LOR. FMAD of the reference is fmaf here; everything else is evaluated as written
 // (the library is built with -ffp-contract=off and IEEE division/sqrt), so a forward projection is
@@ -245,7 +247,9 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit) {
     const float L = sqrtf(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
     t.orl = L * oa.width;
     float w = 1.f;
+    if (g.atten) w *= g.atten[lo];
     if (g.norm) w *= g.norm[lo];
+    if (g.scat) w *= g.scat[lo];
     w *= g.global_factor;
     t.Nxy = g.Nx * g.Ny;
     t.Nz = g.Nz;
@@ -409,7 +413,9 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_bp(OrthArgs oa, const float
     const float r = yin[lo];
     if (r == 0.f && sens == nullptr) return;
     float temp = 1.f;
+    if (oa.a.atten) temp *= oa.a.atten[lo];
     if (oa.a.norm) temp *= oa.a.norm[lo];
+    if (oa.a.scat) temp *= oa.a.scat[lo];
     temp *= oa.a.global_factor;
     if (sens) {
         orth_lor(oa, lo, [&](int64_t ind, float w) {
@@ -526,7 +532,9 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
         for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
             const int64_t lo = base + l;
             float temp = 1.f;
+            if (a.atten) temp *= a.atten[lo];
             if (a.norm) temp *= a.norm[lo];
+            if (a.scat) temp *= a.scat[lo];
             temp *= a.global_factor;
             mr = fmaxf(mr, fabsf(yin[lo] * temp));
             ms = fmaxf(ms, fabsf(temp));
@@ -552,7 +560,9 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
                 const float r = FP ? R[l - l0] : yin[lo];
                 if (pass == 0 && r == 0.f) continue;
                 float temp = 1.f;
+                if (a.atten) temp *= a.atten[lo];
                 if (a.norm) temp *= a.norm[lo];
+                if (a.scat) temp *= a.scat[lo];
                 temp *= a.global_factor;
                 if (pass == 0) {
                     orth_lor(oa, lo, [&](int64_t ind, float w) {

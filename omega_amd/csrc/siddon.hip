@@ -69,6 +69,43 @@ __global__ __launch_bounds__(kBlock) void k_siddon_fp(SidArgs a, const float* __
     y[lo] = hit ? (a.mode == MODE_CT_FLAT ? ax : ax * w) : 0.f;
 }
 
+// ---- attenuation factors ---------------------------------------------------------------------------
+// f[lo] = exp(jelppi), jelppi = sum of len * -mu[voxel] over the LOR's walk in traversal order, with
+// the walk's (clipped) lengths: the reference's forward-projection sum (compute_attenuation,
+// projector_functions.h:597-601, :1938-1939, :2026-2027). An axis-parallel LOR integrates mu over its
+// whole image row with the voxel size as length (perpendicular_elements, :1193-1205). A LOR that misses
+// the image gets 1. `a` carries no weights (its atten and norm are null).
+__global__ __launch_bounds__(kBlock) void k_atten_factor(SidArgs a, const float* __restrict__ mu,
+                                                         float* __restrict__ f) {
+    const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (lo >= a.nmeas) return;
+    Walk w;
+    float weight, jel = 0.f;
+    if (siddon_setup<false>(a, lo, w, weight)) {
+        if (w.perp) {
+            for (uint32_t i = 0; i < w.p_nrun; i++) jel += w.p_d * -mu[w.p_row + i * w.p_rstride];
+        } else {
+            while (w.left > 0u) {
+                uint32_t ind;
+                float len;
+                walk_step<false>(a, w, ind, len);
+                if (len > 0.f) jel += len * -mu[ind];
+            }
+        }
+    }
+    f[lo] = expf(jel);
+}
+
+int atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s) {
+    SidArgs a = make_sid_args(g, 0, n);
+    a.norm = nullptr;
+    a.atten = nullptr;
+    a.scat = nullptr;
+    hipLaunchKernelGGL(k_atten_factor, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a, mu, f);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
 // ---- backprojection -----------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_siddon_bp(SidArgs a, const float* __restrict__ yin, float* __restrict__ rhs,
                                                       float* __restrict__ sens) {

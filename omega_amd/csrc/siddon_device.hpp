@@ -33,6 +33,9 @@ struct SidArgs {
     const float* __restrict__ x;   // coordinate tables (device)
     const float* __restrict__ z;
     const float* __restrict__ norm; // per-measurement normalisation, subset-local, or null
+    const float* __restrict__ atten; // per-measurement attenuation factor, subset-local, or null
+    const float* __restrict__ scat;  // per-measurement multiplicative (scatter) correction, subset-local, or null
+    bool atten_lor;                  // atten holds given per-LOR factors (not factors computed from an image)
     uint32_t pitch;                // row pitch of slice-relative (planar) voxel indices: Nx, or Nx + 1 in LDS
     const uint32_t* __restrict__ index;  // PET sinogram: sinogram bin per measurement (subset types 1-5), or null;
                                          // plane_offset is then the call's first measurement in this table
@@ -185,11 +188,18 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
     return false;
 }
 
-// LOR weight: PET 1/L * norm * global factor (projector_functions.h:1817-1836, :1181-1211), CT 1.
-__device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct) {
+// LOR weight: PET 1/L * attenuation * norm * scatter * global factor (projector_functions.h:1817-1836,
+// :1181-1211), CT 1. The attenuation factor multiplies in where the reference's backprojection applies
+// its exp(jelppi) or a given per-LOR factor (:1825-1828); on axis-parallel LORs (perp) a given per-LOR
+// factor comes after norm, as there (:1209-1210).
+__device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct, bool perp = false) {
     if (ct) return 1.f;
     float t = 1.f / L;
+    const bool late = perp && g.atten_lor;
+    if (g.atten && !late) t *= g.atten[lo];
     if (g.norm) t *= g.norm[lo];
+    if (g.atten && late) t *= g.atten[lo];
+    if (g.scat) t *= g.scat[lo];
     t *= g.global_factor;
     return t;
 }
@@ -215,6 +225,7 @@ struct Walk {
     uint32_t left;                 // Siddon steps still to take (0 = finished)
     bool perp;
     uint32_t p_ind, p_stride, p_a1, p_a2, p_nrun, p_lim;
+    uint32_t p_row, p_rstride;     // first voxel and stride of the whole image row the LOR runs along
     float p_dT1, p_dT2, p_d;
 };
 
@@ -278,7 +289,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         } else {
             return false;
         }
-        weight = make_weight(g, lo, L, ct);
+        weight = make_weight(g, lo, L, ct, true);
         uint32_t apu = 0;
         {
             float s = db - dd + ddb;
@@ -289,6 +300,9 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         }
         w.perp = true;
         w.trans = false;
+        const uint32_t zoff = PLANAR ? 0u : (uint32_t)tk * g.Nyx;
+        if (stride == 1u) { w.p_row = apu * (PLANAR ? g.pitch : g.Nx) + zoff; w.p_rstride = 1u; }
+        else { w.p_row = apu + zoff; w.p_rstride = stride; }
         w.p_ind = apu * Nm + (PLANAR ? 0u : (uint32_t)tk * Nrow * Nrun) + stride * (uint32_t)a1;
         w.p_stride = stride;
         w.p_a1 = (uint32_t)a1; w.p_a2 = (uint32_t)a2; w.p_nrun = Nrun;

@@ -92,6 +92,18 @@ class projectorClass:
         self.normalization_correction = False
         self.normalization = np.empty(0, dtype=np.float32)
         self.global_correction_factor = 1.0
+        # attenuation correction (proj.py:85,212-214,225,296,300,358)
+        # multiplicative per-measurement correction (proj.py:88,295 -> -DSCATTER)
+        self.additionalCorrection = False
+        self.corrVector = np.empty(0, dtype=np.float32)
+        self.attenuation_correction = False
+        self.CTAttenuation = True       # True: vaimennus is an attenuation image (1/mm); False: one factor per measurement
+        self.CT_attenuation = True
+        self.vaimennus = np.empty(0, dtype=np.float32)
+        self.attIncm = False            # the image is in 1/cm
+        self.rotateAttImage = 0
+        self.flipAttImageXY = False
+        self.flipAttImageZ = False
         self.epps = 1e-5
         # PET scanner (used when no coordinate tables are given)
         self.diameter = 0.0
@@ -251,6 +263,37 @@ class projectorClass:
         d = np.asarray(data, dtype=np.float32).reshape(-1, per)
         return np.ascontiguousarray(d[self.index].reshape(-1))
 
+    def _attenuation(self):
+        """The attenuation table handed to the library and its mode (1 image, 2 per measurement).
+
+        An image (CTAttenuation and CT_attenuation, the defaults) gets the reference's preparation
+        (prepass.py:126-150): N values, or an (Nx, Ny, Nz) array; rotated by rotateAttImage quarter turns
+        in the transaxial plane, flipped (flipAttImageXY, flipAttImageZ), divided by 10 when attIncm, and
+        passed x fastest. The library computes each LOR's factor exp(-sum len * mu) once. Per-measurement
+        factors (init.py:518-521, prepass.py:465-470) are reordered into subset order as the
+        normalisation is."""
+        v = np.asarray(self.vaimennus, dtype=np.float32)
+        if v.size == 0:
+            raise ValueError("attenuation_correction needs vaimennus: an attenuation image or one factor per measurement")
+        g = self.grid
+        if self.CTAttenuation and self.CT_attenuation:
+            if v.size != g.N:
+                raise ValueError(f"Attenuation data is of different size than the reconstructed image "
+                                 f"({v.size} values, {g.N} voxels)")
+            a = v.reshape((g.Nx, g.Ny, g.Nz), order="F") if v.ndim != 3 else v
+            if self.rotateAttImage != 0:
+                a = np.rot90(a, int(self.rotateAttImage))
+            if self.flipAttImageXY:
+                a = np.fliplr(a)
+            if self.flipAttImageZ:
+                a = np.flip(a, 2)
+            if self.attIncm:
+                a = a / np.float32(10.0)
+            return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F")), 1
+        if self.listmode:
+            return np.ascontiguousarray(v.ravel()), 2
+        return self.subset_order(v), 2
+
     def _descriptor(self):
         d = _lib.GeomDesc()
         d.struct_size = C.sizeof(_lib.GeomDesc)
@@ -274,6 +317,17 @@ class projectorClass:
                 np.ascontiguousarray(self.normalization, dtype=np.float32).ravel()
             keep.append(nrm)
             d.norm, d.norm_len = nrm.ctypes.data, nrm.size
+        if self.additionalCorrection and not self.CT:
+            if self.corrVector.size == 0:
+                raise ValueError("additionalCorrection needs corrVector (one factor per measurement)")
+            cv = self.subset_order(self.corrVector) if not self.listmode else \
+                np.ascontiguousarray(self.corrVector, dtype=np.float32).ravel()
+            keep.append(cv)
+            d.scat, d.scat_len = cv.ctypes.data, cv.size
+        if self.attenuation_correction and not self.CT:
+            atn, d.atten_mode = self._attenuation()
+            keep.append(atn)
+            d.atten, d.atten_len = atn.ctypes.data, atn.size
         if pt == 2:
             # the kernel's tube width is tube_width_z (init.py:652); the sweeps follow orthTransaxial /
             # orthAxial (proj.py:1041-1054 -> -DCRYSTXY / -DCRYSTZ, init.py:286-290)

@@ -52,6 +52,9 @@ struct Geo {
     int32_t ct_pitch;          // CT: z table holds 6 floats per projection
     T global_factor;
     const T* norm;             // optional per-measurement normalisation (subset-local), may be null
+    const T* atten;            // attenuation: image mu (atten_mode 1) or per-measurement factors (2), may be null
+    int32_t atten_mode;        // 0 none, 1 image (paramStruct::CTAttenuation), 2 per measurement
+    const T* scat;             // optional per-measurement multiplicative correction (scatterCoefs), may be null
 };
 
 // float -> uint32 the way g++/x86-64 converts (cvttss2si to 64 bit, keep the low word).
@@ -200,9 +203,19 @@ static inline int perp_start(T b, T d, T dd, uint32_t N) {
 
 // Traces LOR `lo` and calls visit(voxel_index, length) for every voxel with length > 0, in
 // traversal order. Returns false if the LOR is skipped (no contribution). `w` receives the
-// LOR weight (1/L * norm * global_factor for PET, 1 for CT).
+// LOR weight (1/L * norm * global_factor for PET, 1 for CT), with attenuation:
+//   * per-measurement factors (atten_mode 2): after 1/L on the Siddon path (projector_functions.h:1827-1828),
+//     after norm on the axis-parallel path (:1209-1210);
+//   * image (atten_mode 1), exp(jelppi), jelppi = sum of len * -mu (compute_attenuation, :597-601):
+//     axis-parallel LORs sum the whole image row with the voxel size as length, applied after 1/L
+//     (:1193-1206); forward projection (fp == 1) sums the walk's clipped lengths and multiplies the
+//     finished weight (:1938-1939, :2026-2027); backprojection (fp == 2) first walks the LOR with the
+//     unclipped lengths (t_next - t_c) * L and applies the factor after 1/L (:1754-1775, :1825-1826).
 template <typename T, typename F>
-static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w, F&& visit) {
+// `atn` (optional) receives the attenuation factor exp(jelppi) of atten_mode 1 (1 when the LOR is skipped).
+static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w, F&& visit, int fp = 1,
+                       T* atn = nullptr) {
+    if (atn) *atn = (T)1.;
     Ray<T> r;
     fetch_ray(g, lo, x, z, r);
     T ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
@@ -271,7 +284,19 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         uint32_t ind = apu * N + (uint32_t)tk * Nrow * Nrun;
         if (!ct) {
             T t = (T)1. / L;
+            if (g.atten_mode == 1) {
+                // the row's voxels with the image's own strides: x = apu (column) or y = apu (row)
+                T jelppi = (T)0.;
+                for (uint32_t iii = 0; iii < Nrun; iii++) {
+                    const uint32_t ax = (stride == 1u) ? iii : apu, ay = (stride == 1u) ? apu : iii;
+                    jelppi += (dd2 * -g.atten[ax + ay * g.Nx + (uint32_t)tk * Nyx]);
+                }
+                t *= std::exp(jelppi);
+                if (atn) *atn = std::exp(jelppi);
+            }
             if (g.norm) t *= lnorm;
+            if (g.atten_mode == 2) t *= g.atten[lo];
+            if (g.scat) t *= g.scat[lo];
             t *= g.global_factor;
             w = t;
         } else {
@@ -313,9 +338,31 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         skip = preloop3(g, r, xdf, ydf, zdf, bmaxx, bmaxy, bmaxz, ti, tj, tk, tyu, txu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0);
     }
     if (skip) return false;
+    const bool atn_fp = !ct && g.atten_mode == 1 && fp == 1;
+    T jelppi = (T)0.;
+    if (!ct && g.atten_mode == 1 && fp == 2) {
+        // backprojection pre-pass over the unclipped walk (projector_functions.h:1754-1775)
+        int32_t ci = ti, cj = tj, ck = tk;
+        T cx0 = tx0, cy0 = ty0, cz0 = tz0, cc = tc;
+        for (uint32_t ii = 0; ii < Np; ii++) {
+            const uint32_t ind = (uint32_t)cj * g.Nx + (uint32_t)ci + (uint32_t)ck * Nyx;
+            T ele;
+            if (cz0 < cy0 && cz0 < cx0) { ele = (cz0 - cc) * L; ck += uz; cc = cz0; cz0 += tzu; }
+            else if (cy0 < cx0) { ele = (cy0 - cc) * L; cj += uy; cc = cy0; cy0 += tyu; }
+            else { ele = (cx0 - cc) * L; ci += ux; cc = cx0; cx0 += txu; }
+            jelppi += (ele * -g.atten[ind]);
+            if (ci < 0 || (uint32_t)ci >= g.Nx || cj < 0 || (uint32_t)cj >= g.Ny || ck < 0 || (uint32_t)ck >= g.Nz) break;
+        }
+    }
     if (!ct) {
         T t = (T)1. / (L * (T)1);
+        if (g.atten_mode == 1 && fp == 2) {
+            t *= std::exp(jelppi);
+            if (atn) *atn = std::exp(jelppi);
+        }
+        else if (g.atten_mode == 2) t *= g.atten[lo];
         if (g.norm) t *= lnorm;
+        if (g.scat) t *= g.scat[lo];
         t *= g.global_factor;
         w = t;
     } else {
@@ -336,8 +383,15 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         if (axis == 2) { tk += uz; tz0 += tzu; }
         else if (axis == 1) { tj += uy; ty0 += tyu; }
         else { ti += ux; tx0 += txu; }
-        if (len > (T)0.) visit(ind, len);
+        if (len > (T)0.) {
+            if (atn_fp) jelppi += (len * -g.atten[ind]);
+            visit(ind, len);
+        }
         if (ti < 0 || (uint32_t)ti >= g.Nx || tj < 0 || (uint32_t)tj >= g.Ny || tk < 0 || (uint32_t)tk >= g.Nz) break;
+    }
+    if (atn_fp) {
+        w *= std::exp(jelppi);
+        if (atn) *atn = std::exp(jelppi);
     }
     return true;
 }
@@ -380,7 +434,7 @@ static void siddon_bp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
 #pragma omp atomic
                 sens[ind] += lw;
             }
-        });
+        }, 2);
     }
 }
 
@@ -423,39 +477,42 @@ using namespace oracle;
 
 template <typename T>
 static Geo<T> make_geo(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y, int32_t mode,
-                       int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, const T* norm) {
+                       int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, const T* norm,
+                       const T* atten, int32_t atten_mode, const T* scat) {
     Geo<T> g;
     g.Nx = n3[0]; g.Ny = n3[1]; g.Nz = n3[2];
     g.dx = d3[0]; g.dy = d3[1]; g.dz = d3[2];
     g.bx = b3[0]; g.by = b3[1]; g.bz = b3[2];
     g.size_x = size_x; g.size_y = size_y; g.mode = mode; g.plane_offset = plane_offset;
     g.dPitchZ = dPitchZ; g.ct_pitch = ct_pitch; g.global_factor = global_factor; g.norm = norm;
+    g.atten = atten; g.atten_mode = atten ? atten_mode : 0;
+    g.scat = scat;
     return g;
 }
 
 #define ORACLE_SIDDON_API(T, SUF)                                                                                     \
     extern "C" int oracle_siddon_fp_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,             \
                                           uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,            \
-                                          int32_t ct_pitch, T global_factor, const T* norm, int64_t nMeas,           \
+                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, int64_t nMeas,           \
                                           const T* x, const T* z, const T* img, T* y, int nthreads) {                \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat); \
         siddon_fp(g, nMeas, x, z, img, y, nthreads);                                                                   \
         return 0;                                                                                                      \
     }                                                                                                                  \
     extern "C" int oracle_siddon_bp_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,             \
                                           uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,            \
-                                          int32_t ct_pitch, T global_factor, const T* norm, int64_t nMeas,           \
+                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, int64_t nMeas,           \
                                           const T* x, const T* z, const T* yin, T* rhs, T* sens, int nthreads) {      \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat); \
         siddon_bp(g, nMeas, x, z, yin, rhs, sens, nthreads);                                                          \
         return 0;                                                                                                      \
     }                                                                                                                  \
     extern "C" int64_t oracle_siddon_trace_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,      \
                                                  uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,     \
-                                                 int32_t ct_pitch, T global_factor, const T* norm, int64_t nMeas,    \
+                                                 int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, int64_t nMeas,    \
                                                  const T* x, const T* z, int64_t* counts, int64_t cap,               \
                                                  uint32_t* idx_out, T* len_out, T* w_out) {                          \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat); \
         int64_t total = 0;                                                                                             \
         for (int64_t lo = 0; lo < nMeas; lo++) {                                                                       \
             int64_t c = 0;                                                                                             \
@@ -477,6 +534,28 @@ static Geo<T> make_geo(const uint32_t* n3, const T* d3, const T* b3, uint32_t si
 
 ORACLE_SIDDON_API(float, f32)
 ORACLE_SIDDON_API(double, f64)
+
+// Per-measurement attenuation factors exp(jelppi) with the forward projection's semantics (atten = mu,
+// atten_mode 1); 1 for skipped LORs.
+#define ORACLE_ATTEN_API(T, SUF)                                                                                      \
+    extern "C" int oracle_siddon_atten_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,          \
+                                             uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,         \
+                                             int32_t ct_pitch, T global_factor, const T* norm, const T* atten,       \
+                                             int32_t atten_mode, const T* scat, int64_t nMeas, const T* x, const T* z, T* out,      \
+                                             int nthreads) {                                                         \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, \
+                               atten, atten_mode, scat);                                                               \
+        set_threads(nthreads);                                                                                         \
+        _Pragma("omp parallel for schedule(dynamic, 1000)")                                                            \
+        for (int64_t lo = 0; lo < nMeas; lo++) {                                                                       \
+            T w = (T)0.;                                                                                               \
+            siddon_lor(g, lo, x, z, w, [](uint32_t, T) {}, 1, out + lo);                                               \
+        }                                                                                                              \
+        return 0;                                                                                                      \
+    }
+
+ORACLE_ATTEN_API(float, f32)
+ORACLE_ATTEN_API(double, f64)
 
 // Elementwise OSEM pieces, FP32, in the reference's operation order.
 // ratio: yhat += epps (non-CT); r = y / yhat; NaN/Inf -> epps  (computeForwardProject.h:20-21,94,400-401)
@@ -1092,7 +1171,9 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
     const float orl = L * o.width;
     float w = 1.f;
     if (g.mode != oracle::CT_FLAT) {
+        if (g.atten) w *= g.atten[lo];
         if (g.norm) w *= g.norm[lo];
+        if (g.scat) w *= g.scat[lo];
         w *= g.global_factor;
     }
     const uint32_t Nxy = g.Nx * g.Ny;
@@ -1253,10 +1334,10 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
 }  // namespace orth
 
 static orth::OGeo make_ogeo(const uint32_t* n3, const float* d3, const float* b3, uint32_t size_x, uint32_t size_y,
-                            int32_t mode, int64_t plane_offset, float global_factor, const float* norm, float width,
-                            int32_t flags) {
+                            int32_t mode, int64_t plane_offset, float global_factor, const float* norm,
+                            const float* atten, const float* scat, float width, int32_t flags) {
     orth::OGeo o;
-    o.g = make_geo<float>(n3, d3, b3, size_x, size_y, mode, plane_offset, 0.f, 0, global_factor, norm);
+    o.g = make_geo<float>(n3, d3, b3, size_x, size_y, mode, plane_offset, 0.f, 0, global_factor, norm, atten, 2, scat);
     o.bmaxx = (float)((double)b3[0] + (double)n3[0] * (double)d3[0]);
     o.bmaxy = (float)((double)b3[1] + (double)n3[1] * (double)d3[1]);
     o.bmaxz = (float)((double)b3[2] + (double)n3[2] * (double)d3[2]);
@@ -1268,11 +1349,11 @@ static orth::OGeo make_ogeo(const uint32_t* n3, const float* d3, const float* b3
 
 #define ORACLE_ORTH_ARGS                                                                                              \
     const uint32_t *n3, const float *d3, const float *b3, uint32_t size_x, uint32_t size_y, int32_t mode,             \
-        int64_t plane_offset, float global_factor, const float *norm, float width, int32_t flags, int64_t nMeas,     \
+        int64_t plane_offset, float global_factor, const float *norm, const float *atten, const float *scat, float width, int32_t flags, int64_t nMeas,     \
         const float *x, const float *z
 
 extern "C" int oracle_orth_fp_f32(ORACLE_ORTH_ARGS, const float* img, float* y, int nthreads) {
-    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, width, flags);
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags);
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t lo = 0; lo < nMeas; lo++) {
@@ -1284,7 +1365,7 @@ extern "C" int oracle_orth_fp_f32(ORACLE_ORTH_ARGS, const float* img, float* y, 
 }
 
 extern "C" int oracle_orth_bp_f32(ORACLE_ORTH_ARGS, const float* yin, float* rhs, float* sens, int nthreads) {
-    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, width, flags);
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags);
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t lo = 0; lo < nMeas; lo++) {
@@ -1308,7 +1389,7 @@ extern "C" int oracle_orth_bp_f32(ORACLE_ORTH_ARGS, const float* yin, float* rhs
 // Per-LOR voxel lists in visiting order (indices, tube weights w) and the LOR weight; returns the total count.
 extern "C" int64_t oracle_orth_trace_f32(ORACLE_ORTH_ARGS, int64_t* counts, int64_t cap, int64_t* idx_out,
                                          float* w_out, float* lor_w) {
-    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, width, flags);
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags);
     int64_t total = 0;
     for (int64_t lo = 0; lo < nMeas; lo++) {
         int64_t c = 0;

@@ -15,7 +15,7 @@
 template <typename T>
 static void fill(paramStruct<T>& p, const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y,
                  int32_t mode, int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, T* norm,
-                 uint32_t nProjections) {
+                 T* atten, int32_t atten_mode, T* scat, uint32_t nProjections) {
     p.Nx = n3[0]; p.Ny = n3[1]; p.Nz = n3[2];
     p.dx = d3[0]; p.dy = d3[1]; p.dz = d3[2];
     p.bx = b3[0]; p.by = b3[1]; p.bz = b3[2];
@@ -31,6 +31,12 @@ static void fill(paramStruct<T>& p, const uint32_t* n3, const T* d3, const T* b3
     p.globalFactor = global_factor;
     p.normalizationCorrection = norm != nullptr;
     p.normCoefs = norm;
+    // attenuation: mode 1 image (CTAttenuation), mode 2 per-measurement factors
+    p.attenuationCorrection = atten != nullptr && atten_mode != 0;
+    p.CTAttenuation = atten_mode != 2;
+    p.atten = atten;
+    p.scatterCorrectionMult = scat != nullptr;
+    p.scatterCoefs = scat;
     p.dPitchZ = dPitchZ; p.dPitchXY = dPitchZ;
     p.pitch = ct_pitch != 0;
     p.nProjections = nProjections;
@@ -40,10 +46,12 @@ static void fill(paramStruct<T>& p, const uint32_t* n3, const T* d3, const T* b3
 #define REF_API(T, SUF)                                                                                               \
     extern "C" int ref_siddon_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y,  \
                                     int32_t mode, int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, \
-                                    T* norm, uint32_t nProjections, int64_t nMeas, const T* x, const T* z, T* input,  \
-                                    T* output, T* sens, int fp, int nthreads) {                                       \
+                                    T* norm, T* atten, int32_t atten_mode, T* scat, uint32_t nProjections,            \
+                                    int64_t nMeas,                                                                    \
+                                    const T* x, const T* z, T* input, T* output, T* sens, int fp, int nthreads) {     \
         paramStruct<T> p;                                                                                              \
-        fill(p, n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, nProjections); \
+        fill(p, n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten,        \
+             atten_mode, scat, nProjections);                                                                          \
         p.noSensImage = (sens == nullptr);                                                                             \
         projectorType123Implementation4<T>(p, nMeas, output, x, z, input, mode == 1, false, (uint8_t)fp, sens,        \
                                            nullptr, (uint32_t)nthreads);                                               \

@@ -11,10 +11,11 @@ from omega_amd.projector import Geometry
 
 def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
               ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_flags=0, dPitchX=0.0, dPitchY=0.0,
-              generic=False, x_fastest_only=False):
+              generic=False, x_fastest_only=False, atten=None, atten_mode=0, scat=None):
     """Geometry handle through the C ABI; ``generic`` routes every call through the one-lane-per-LOR
     kernels (omg_geom_desc.reserved[0]); ``x_fastest_only`` keeps those kernels in the x-fastest image
-    (reserved[3])."""
+    (reserved[3]); ``atten`` is an attenuation image (``atten_mode`` 1) or per-measurement factors (2);
+    ``scat`` a per-measurement multiplicative correction."""
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
     d = _lib.GeomDesc()
@@ -38,6 +39,15 @@ def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, 
         norm = np.ascontiguousarray(norm, dtype=np.float32)
         keep.append(norm)
         d.norm, d.norm_len = norm.ctypes.data, norm.size
+    if atten is not None:
+        atten = np.ascontiguousarray(atten, dtype=np.float32).ravel()
+        keep.append(atten)
+        d.atten_mode = atten_mode
+        d.atten, d.atten_len = atten.ctypes.data, atten.size
+    if scat is not None:
+        scat = np.ascontiguousarray(scat, dtype=np.float32).ravel()
+        keep.append(scat)
+        d.scat, d.scat_len = scat.ctypes.data, scat.size
     d.orth_width = orth_width
     d.orth_flags = orth_flags
     d.dPitchX = dPitchX

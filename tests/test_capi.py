@@ -97,6 +97,28 @@ def test_create_rejects_bad_subset_index():
     assert "PET sinogram" in _lib.last_error()
 
 
+def test_create_rejects_bad_correction_tables():
+    """Per-measurement tables (norm, attenuation factors) must cover the 4 x 3 x 2 = 24 measurements; an
+    attenuation image must match the 4 x 4 x 4 grid; attenuation is for PET data; the mode is 0-2."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    short = np.ones(23, np.float32)
+    img_bad = np.ones(63, np.float32)
+    checks = [
+        (dict(norm=short.ctypes.data, norm_len=short.size), "norm holds 23"),
+        (dict(atten_mode=2, atten=short.ctypes.data, atten_len=short.size), "attenuation table holds 23"),
+        (dict(atten_mode=1, atten=img_bad.ctypes.data, atten_len=img_bad.size), "image grid 64"),
+        (dict(atten_mode=1), "without an atten table"),
+        (dict(atten_mode=3, atten=short.ctypes.data, atten_len=short.size), "atten_mode 3"),
+        (dict(atten_mode=2, atten=short.ctypes.data, atten_len=short.size, mode=_lib.MODE_CT_FLAT, x_len=12),
+         "PET data"),
+    ]
+    for kw, msg in checks:
+        d, keep = _desc(**kw)
+        assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG, msg
+        assert msg in _lib.last_error(), (msg, _lib.last_error())
+
+
 def test_bdd_requires_ct():
     d, keep = _desc(projector_type=5)
     h = C.c_void_p()
