@@ -2,6 +2,7 @@
 #include "geom.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -186,6 +187,16 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
         OMG_REQUIRE(d->scat_len >= n_total, "scat holds %lld floats, the data %lld measurements",
                     (long long)d->scat_len, (long long)n_total);
     }
+    if (d->tof_bins > 1) {
+        OMG_REQUIRE(d->tof_bins <= kTofMax, "tof_bins %u exceeds %u", d->tof_bins, kTofMax);
+        OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON && d->mode != OMG_MODE_CT_FLAT,
+                    "time of flight is provided for the improved Siddon projector (projector_type 1) on PET data");
+        OMG_REQUIRE(d->tof_sigma > 0.f && std::isfinite(d->tof_sigma), "tof_sigma must be a positive standard deviation");
+        OMG_REQUIRE(d->tof_centers != nullptr, "tof_bins %u without tof_centers", d->tof_bins);
+        for (uint32_t b = 0; b < d->tof_bins; b++)
+            OMG_REQUIRE(std::isfinite(d->tof_centers[b]), "tof_centers[%u] is not finite", b);
+        OMG_REQUIRE(d->tof_epps >= 0.f, "tof_epps must be >= 0");
+    }
     OMG_REQUIRE(d->atten_mode >= 0 && d->atten_mode <= 2, "atten_mode %d is not 0 (off), 1 (image) or 2 (per LOR)",
                 d->atten_mode);
     if (d->atten_mode) {
@@ -222,11 +233,12 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
         }
     }
     if (!rc && d->scat) rc = upload(d->scat, n_total, &g->scat_dev);
+    if (!rc && d->tof_bins > 1) rc = upload(d->tof_centers, d->tof_bins, &g->tof_c);
     if (!rc && d->atten_mode == 2) rc = upload(d->atten, n_total, &g->atten_dev);
     if (!rc && d->atten_mode == 1) rc = attenuation_prepare(g, d->atten, n_total);
     if (!rc && d->mode == OMG_MODE_PET_SINO) {
         g->z_host.assign(d->z, d->z + d->z_len);
-        g->planar = d->meas_index == nullptr;
+        g->planar = d->meas_index == nullptr && d->tof_bins <= 1;
         for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
     }
     if (!rc && g->planar) g->ring = ring_table(*d, d->x, g->bmaxx, g->bmaxy);
@@ -234,7 +246,7 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
     if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);
     g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr; g->d.meas_index = nullptr; g->d.atten = nullptr;
-    g->d.scat = nullptr;
+    g->d.scat = nullptr; g->d.tof_centers = nullptr;
     if (rc) {
         omg_geom_destroy(g);
         return rc;
@@ -254,6 +266,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->norm_dev);
     hipFree(g->atten_dev);
     hipFree(g->scat_dev);
+    hipFree(g->tof_c);
     hipFree(g->index_dev);
     hipFree(g->frames_t);
     hipFree(g->scratch);

@@ -12,6 +12,7 @@ struct omg_geom {
     float* norm_dev = nullptr;
     float* atten_dev = nullptr;      // per-measurement attenuation factors (norm's layout), or null
     float* scat_dev = nullptr;       // per-measurement multiplicative correction (norm's layout), or null
+    float* tof_c = nullptr;          // TOF bin centres (tof_bins floats), or null
     uint32_t* index_dev = nullptr;   // measurement-level subsets: sinogram bin per measurement (subset order)
     float bmaxx = 0.f, bmaxy = 0.f, bmaxz = 0.f;
     int device = 0;
@@ -71,8 +72,20 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.atten = g->atten_dev ? g->atten_dev + first * per : nullptr;
     a.scat = g->scat_dev ? g->scat_dev + first * per : nullptr;
     a.atten_lor = g->d.atten_mode == 2;
+    a.tof_bins = g->d.tof_bins > 1 ? g->d.tof_bins : 1u;
+    a.tof_sigma = g->d.tof_sigma;
+    a.tof_pf = g->d.tof_sigma > 0.f ? (float)0.3989422804014327 / g->d.tof_sigma : 0.f;
+    a.tof_epps = g->d.tof_epps;
+    a.tof_c = g->tof_c;
     return a;
 }
+
+// Time-of-flight projections (tof.hip): measurement arrays hold tof_bins blocks of n values.
+int tof_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
+int tof_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
+                 hipStream_t s);
+int tof_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
+             float* sens, float epps, hipStream_t s, const float* add);
 
 // Writes f[i] = exp(-sum len * mu) over the Siddon walk of measurements [0, n) (siddon.hip).
 int atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s);

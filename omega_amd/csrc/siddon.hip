@@ -581,6 +581,7 @@ static dim3 xy_tiles(const omg_geom* g) { return dim3((g->d.Nx + 31) / 32, (g->d
 // The forward projection gathers from the x-fastest image only: gathering x-major LORs from a y-fastest
 // copy measured slower (9.6 ms against 7.4 ms for a C2 forward projection).
 static int generic_fp(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
+    if (g->d.tof_bins > 1) return tof_forward(g, first, n, img, y, s);
     const SidArgs a = make_sid_args(g, first, n);
     hipEvent_t stop;
     timing_begin(s, &stop);
@@ -593,6 +594,7 @@ static int generic_fp(const omg_geom* g, int64_t first, int64_t n, const float* 
 // FP -> ratio -> BP into rhs (and sens), both already holding what the caller wants added to.
 static int generic_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
                         float* sens, float epps, hipStream_t s, const float* add) {
+    if (g->d.tof_bins > 1) return tof_osem(g, first, n, img, meas, rhs, sens, epps, s, add);
     SidArgs a = make_sid_args(g, first, n);
     a.add = add;
     float *rhs_t = nullptr, *sens_t = nullptr;
@@ -673,6 +675,7 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
         if (sens) return launch_planar<false, true, true>(g, first, n_meas, nullptr, y, nullptr, rhs, sens, 0.f, s);
         return launch_planar<false, true, false>(g, first, n_meas, nullptr, y, nullptr, rhs, nullptr, 0.f, s);
     }
+    if (g->d.tof_bins > 1) return tof_backward(g, first, n_meas, y, rhs, sens, s);
     SidArgs a = make_sid_args(g, first, n_meas);
     if (split_layout(g)) {
         // x-major LORs into y-fastest buffers (scratch), added back transposed

@@ -36,6 +36,9 @@ struct SidArgs {
     const float* __restrict__ atten; // per-measurement attenuation factor, subset-local, or null
     const float* __restrict__ scat;  // per-measurement multiplicative (scatter) correction, subset-local, or null
     bool atten_lor;                  // atten holds given per-LOR factors (not factors computed from an image)
+    uint32_t tof_bins;               // time-of-flight bins (<= 1: no TOF)
+    float tof_sigma, tof_pf, tof_epps;  // TOF standard deviation (mm), 0.3989422804014327f / sigma, weight-sum floor
+    const float* __restrict__ tof_c; // TOF bin centres (mm), tof_bins floats
     uint32_t pitch;                // row pitch of slice-relative (planar) voxel indices: Nx, or Nx + 1 in LDS
     const uint32_t* __restrict__ index;  // PET sinogram: sinogram bin per measurement (subset types 1-5), or null;
                                          // plane_offset is then the call's first measurement in this table
@@ -227,6 +230,7 @@ struct Walk {
     uint32_t p_ind, p_stride, p_a1, p_a2, p_nrun, p_lim;
     uint32_t p_row, p_rstride;     // first voxel and stride of the whole image row the LOR runs along
     float p_dT1, p_dT2, p_d;
+    float tofD;                    // TOF: signed distance of the walk's start from the LOR centre (TOF only)
 };
 
 // Set-up of LOR `lo` (projector_functions.h:1421-1840). Returns false when the LOR contributes
@@ -254,7 +258,9 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         int a1 = 0, a2;
         float dT1, dT2, db, dd, ddb, dd2;
         uint32_t Nm, Nrow, Nrun, stride;
+        bool along_x;   // the LOR runs along x (a row of constant y), else along y
         if ((double)fabsf(ydf) < 1e-8 && r.yd <= g.bmaxy && r.yd >= g.by) {
+            along_x = true;
             a2 = (int)g.Nx - 1; dT1 = g.dx; dT2 = g.dx; db = g.by; dd = r.yd; ddb = g.dy;
             float e1, e2;
             if (r.xs > r.xd) { e1 = g.bx - r.xd; e2 = g.bx + (float)g.Nx * g.dx - r.xs; }
@@ -272,6 +278,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
             // planar (slice-relative) indices require Nx == Ny and use the LDS row pitch
             Nm = PLANAR ? g.pitch : g.Ny; Nrow = g.Ny; Nrun = g.Nx; stride = 1u;
         } else if ((double)fabsf(xdf) < 1e-8 && r.xd <= g.bmaxx && r.xd >= g.bx) {
+            along_x = false;
             a2 = (int)g.Ny - 1; dT1 = g.dx; dT2 = g.dx; db = g.bx; dd = r.xd;
             float e1, e2;
             if (r.ys > r.yd) { e1 = g.by - r.yd; e2 = g.by + (float)g.Ny * g.dy - r.ys; }
@@ -301,7 +308,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         w.perp = true;
         w.trans = false;
         const uint32_t zoff = PLANAR ? 0u : (uint32_t)tk * g.Nyx;
-        if (stride == 1u) { w.p_row = apu * (PLANAR ? g.pitch : g.Nx) + zoff; w.p_rstride = 1u; }
+        if (along_x) { w.p_row = apu * (PLANAR ? g.pitch : g.Nx) + zoff; w.p_rstride = 1u; }
         else { w.p_row = apu + zoff; w.p_rstride = stride; }
         w.p_ind = apu * Nm + (PLANAR ? 0u : (uint32_t)tk * Nrow * Nrun) + stride * (uint32_t)a1;
         w.p_stride = stride;
@@ -310,6 +317,8 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         w.p_lim = PLANAR ? g.Ny * g.pitch : g.Nyx * g.Nz;
         w.p_dT1 = dT1; w.p_dT2 = dT2; w.p_d = dd2;
         w.left = 0;
+        // half the row, signed by the (near-zero) difference across the row (projector_functions.h:1641-1645)
+        if (g.tof_bins > 1) w.tofD = (dd2 * (float)Nrun) / copysignf(2.f, along_x ? ydf : xdf);
         return true;
     }
     w.perp = false;
@@ -337,6 +346,10 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
     }
     if (skip) return false;
     weight = make_weight(g, lo, L, ct);
+    if (g.tof_bins > 1) {   // TOFDis (projector_functions.h:588-594, :1741-1742)
+        const float xI = xdf * tc, yI = ydf * tc, zI = zdf * tc;
+        w.tofD = sqrtf(xI * xI + yI * yI + zI * zI) - L / 2.f;
+    }
     w.tx0 = tx0; w.ty0 = ty0; w.tz0 = tz0; w.txu = txu; w.tyu = tyu; w.tzu = tzu; w.tc = tc;
     w.ti = ti; w.tj = tj; w.tk = tk; w.ux = ux; w.uy = uy; w.uz = uz;
     if (PLANAR) {
@@ -399,6 +412,61 @@ __device__ __forceinline__ void walk_step(const SidArgs& g, Walk& w, uint32_t& i
     bool out = ((uint32_t)w.ti >= g.Nx) | ((uint32_t)w.tj >= g.Ny);
     if (!PLANAR) out = out | ((uint32_t)w.tk >= g.Nz);
     w.left = (alive & !out) ? w.left - 1u : 0u;
+}
+
+// ---- time of flight (projector_functions.h:559-594, 632-657) ---------------------------------------
+// A voxel's TOF weight for bin c is a 4-point trapezoid of a Gaussian over the voxel's stretch of the
+// LOR, from the distance D of its entry point from the LOR centre towards the far end. As in the
+// reference, the Gaussian's exponential is taken in double precision (its -0.5 is a double literal)
+// and its product with (0.3989422804014327f / sigma) rounded to float.
+constexpr uint32_t kTofMax = 64;   // TOF bins a geometry may have
+
+__device__ __forceinline__ float tof_pdf(const SidArgs& g, float x, float mu) {
+    const float a = (x - mu) / g.tof_sigma;
+    return (float)((double)g.tof_pf * exp(-0.5 * (double)a * (double)a));
+}
+
+// TOFWeight (projector_functions.h:565-573) with TRAPZ_BINS = 4
+__device__ __forceinline__ float tof_weight(const SidArgs& g, float elem, float D, float DD, float mu, float dX) {
+    float out = tof_pdf(g, D, mu);
+    const float dXs = copysignf(dX, DD);
+    out += tof_pdf(g, D - dXs * 1.f, mu) * 2.f;
+    out += tof_pdf(g, D - dXs * 2.f, mu) * 2.f;
+    out += tof_pdf(g, D - copysignf(elem, DD), mu);
+    return out;
+}
+
+// TOF walk of LOR `lo`: visit(voxel, length, D, DD, esum) for each voxel with length > 0 in traversal
+// order, D the TOF distance at the voxel and esum the length the reference's weight sum (TOFLoop) uses:
+// the length itself on a Siddon walk, the full voxel size on an axis-parallel row (whose end voxels
+// have partial lengths, :1664-1673). D moves by each step's length (:1990-1991; rows: the voxel size,
+// :1684-1685), also across steps of length 0. Returns false when the LOR contributes nothing.
+template <typename Visit>
+__device__ __forceinline__ bool siddon_tof_trace(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+    Walk w;
+    if (!siddon_setup<false>(g, lo, w, weight)) return false;
+    float D = w.tofD;
+    const float DD = D;
+    if (w.perp) {
+        uint32_t ind = w.p_ind;
+        for (uint32_t ii = w.p_a1; ii <= w.p_a2; ii++) {
+            float din = w.p_d;
+            if (w.p_a1 > 0 && ii == w.p_a1) din = w.p_dT1;
+            else if (w.p_a2 < w.p_nrun - 1 && ii == w.p_a2) din = w.p_dT2;
+            if (ind < w.p_lim) visit(ind, din, D, DD, w.p_d);
+            ind += w.p_stride;
+            D -= copysignf(w.p_d, DD);
+        }
+        return true;
+    }
+    while (w.left > 0u) {
+        uint32_t ind;
+        float len;
+        walk_step<false>(g, w, ind, len);
+        if (len > 0.f) visit(ind, len, D, DD, len);
+        D -= copysignf(len, DD);
+    }
+    return true;
 }
 
 // Axis-parallel walk: visit(voxel, length) for each voxel in order.

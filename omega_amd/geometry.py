@@ -205,6 +205,33 @@ def subset_index_measurements(subset_type: int, subsets: int, n_dist: int, n_ang
     return np.concatenate(parts), np.array([p.size for p in parts], dtype=np.int64)
 
 
+SPEED_OF_LIGHT_MM_S = 2.99792458e11
+
+
+def tof_sigma(fwhm_s: float) -> float:
+    """TOF standard deviation along the LOR (mm) for a timing resolution FWHM in seconds
+    (projectorClass, proj.py:693-694: the distance is c * t / 2)."""
+    return (SPEED_OF_LIGHT_MM_S * fwhm_s / 2.0) / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+
+
+def tof_centers(bins: int, width_s: float, offset_s: float = 0.0) -> np.ndarray:
+    """TOF bin centres along the LOR (mm), in the reference's bin order: the central bin, then
+    alternately the next bin on the + and the - side (proj.py:695-703), times -c/2. That ordering needs
+    an odd bin count (the reference's slice assignment fails for an even one)."""
+    if bins < 1 or bins % 2 == 0:
+        raise ValueError(f"TOF bin centres are generated for an odd number of bins (got {bins}); give TOFCenter instead")
+    edges = np.linspace(-width_s * bins / 2, width_s * bins / 2, bins + 1, dtype=np.float32)
+    mid = edges[:-1] + np.float32(width_s / 2.0)
+    c = np.zeros(mid.shape, dtype=np.float32)
+    h = mid.size // 2
+    c[0] = mid[h]
+    c[1::2] = mid[h + 1:]
+    c[2::2] = mid[h - 1::-1]
+    if offset_s > 0:
+        c = c + np.float32(offset_s)
+    return np.float32(-c * SPEED_OF_LIGHT_MM_S / 2.0)
+
+
 @dataclass
 class PETScanner:
     """A cylindrical PET scanner sampled as parallel-chord sinograms."""

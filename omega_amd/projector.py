@@ -92,10 +92,18 @@ class projectorClass:
         self.normalization_correction = False
         self.normalization = np.empty(0, dtype=np.float32)
         self.global_correction_factor = 1.0
-        # attenuation correction (proj.py:85,212-214,225,296,300,358)
         # multiplicative per-measurement correction (proj.py:88,295 -> -DSCATTER)
         self.additionalCorrection = False
         self.corrVector = np.empty(0, dtype=np.float32)
+        # time of flight (proj.py:81,166-168,332-333,455-456, 681-705)
+        self.TOF_bins = 1
+        self.TOF_bins_used = 1
+        self.TOF_FWHM = 0.0             # timing resolution (s)
+        self.TOF_width = 0.0            # bin width (s)
+        self.TOF_offset = 0.0           # (s)
+        self.TOFCenter = np.empty(0, dtype=np.float32)
+        self.sigma_x = 0.0
+        # attenuation correction (proj.py:85,212-214,225,296,300,358)
         self.attenuation_correction = False
         self.CTAttenuation = True       # True: vaimennus is an attenuation image (1/mm); False: one factor per measurement
         self.CT_attenuation = True
@@ -294,6 +302,38 @@ class projectorClass:
             return np.ascontiguousarray(v.ravel()), 2
         return self.subset_order(v), 2
 
+    @property
+    def nBins(self) -> int:
+        """TOF bins of every measurement (1 without TOF)."""
+        return int(self.TOF_bins) if self._tof() else 1
+
+    def _tof(self) -> bool:
+        """TOF is used when TOF_bins > 1 and TOF_bins_used equals it (0 meaning all bins); other
+        TOF_bins_used values ask the reference to sum the bins, i.e. a non-TOF reconstruction
+        (proj.py:681-691, :1208-1210)."""
+        pt = self.projector_type % 10 if self.projector_type > 10 else self.projector_type
+        used = int(self.TOF_bins_used) or int(self.TOF_bins)
+        return int(self.TOF_bins) > 1 and used == int(self.TOF_bins) and not self.CT and pt in (1, 2)
+
+    def _tof_tables(self):
+        """sigma_x (mm) and the bin centres (mm) the kernels get (proj.py:681-705, init errors :1247-1254)."""
+        bins = int(self.TOF_bins)
+        if self.TOFCenter.size == 0:
+            if self.TOF_width <= 0:
+                raise ValueError("TOF width (self.TOF_width) must be greater than zero.")
+            if self.TOF_FWHM <= 0:
+                raise ValueError("TOF FWHM (self.TOF_FWHM) must be greater than zero.")
+            self.sigma_x = G.tof_sigma(self.TOF_FWHM)
+            self.TOFCenter = G.tof_centers(bins, self.TOF_width, self.TOF_offset)
+        elif self.sigma_x <= 0:
+            if self.TOF_FWHM <= 0:
+                raise ValueError("TOF FWHM (self.TOF_FWHM) must be greater than zero.")
+            self.sigma_x = G.tof_sigma(self.TOF_FWHM)
+        c = np.ascontiguousarray(self.TOFCenter, dtype=np.float32).ravel()
+        if c.size != bins:
+            raise ValueError(f"TOFCenter holds {c.size} values for {bins} TOF bins")
+        return float(self.sigma_x), c
+
     def _descriptor(self):
         d = _lib.GeomDesc()
         d.struct_size = C.sizeof(_lib.GeomDesc)
@@ -317,6 +357,13 @@ class projectorClass:
                 np.ascontiguousarray(self.normalization, dtype=np.float32).ravel()
             keep.append(nrm)
             d.norm, d.norm_len = nrm.ctypes.data, nrm.size
+        if self._tof() and pt != 1:
+            raise ValueError("TOF is currently only supported with improved Siddon (projector_type = 1)")
+        if self._tof():
+            sig, tc = self._tof_tables()
+            keep.append(tc)
+            d.tof_bins, d.tof_sigma, d.tof_epps = int(self.TOF_bins), sig, float(self.epps)
+            d.tof_centers = tc.ctypes.data
         if self.additionalCorrection and not self.CT:
             if self.corrVector.size == 0:
                 raise ValueError("additionalCorrection needs corrVector (one factor per measurement)")
@@ -374,7 +421,7 @@ class projectorClass:
             raise ValueError("Multi-resolution volumes are not supported by this implementation!")
         first, n = self._range(subset)
         f = self._as_image(f)
-        y = torch.empty(n, dtype=torch.float32, device=f.device)
+        y = torch.empty(n * self.nBins, dtype=torch.float32, device=f.device)
         _lib.call("omg_forward_project", self._geom.handle, first, n, C.c_void_p(f.data_ptr()),
                   C.c_void_p(y.data_ptr()), _stream_ptr(torch))
         return y
@@ -388,7 +435,7 @@ class projectorClass:
         if not self.projectorInitialized:
             self.initProj()
         first, n = self._range(subset)
-        y = self._as_meas(y, n)
+        y = self._as_meas(y, n * self.nBins)
         f = torch.zeros(int(self.N[0]), dtype=torch.float32, device=y.device)
         sp = C.c_void_p(0)
         if sens is not None:

@@ -38,7 +38,9 @@ class OSEMRunner:
                  enforce_positivity: bool = True, rdp_boundary: int = 0, stream=None, additive=None):
         """``additive``: optional randoms (+ scatter) estimate, a float32 device tensor in the layout of
         ``meas`` (subset order), added to the forward projection in the ratio (randoms_correction,
-        computeForwardProject.h:66-69)."""
+        computeForwardProject.h:66-69). With TOF (``proj.nBins`` > 1) each subset's measurements are
+        nBins blocks of the subset's LORs, and ``additive`` holds one value per LOR, added to every bin
+        (af::tile, computeForwardProject.h:66-67)."""
         import torch
         self.torch = torch
         self.proj = proj
@@ -50,8 +52,10 @@ class OSEMRunner:
         self.meas = meas.reshape(-1).contiguous()
         if self.meas.dtype != torch.float32 or not self.meas.is_cuda:
             raise ValueError("measurements must be a float32 device tensor")
-        if self.meas.numel() != int(proj.nTotMeas[-1]):
-            raise ValueError(f"measurement vector has {self.meas.numel()} elements, expected {int(proj.nTotMeas[-1])}")
+        self.nb = int(getattr(proj, "nBins", 1))
+        if self.meas.numel() != int(proj.nTotMeas[-1]) * self.nb:
+            raise ValueError(f"measurement vector has {self.meas.numel()} elements, expected "
+                             f"{int(proj.nTotMeas[-1]) * self.nb}")
         self.epps = float(epps)
         self.beta = float(beta)
         self.gamma = float(gamma)
@@ -60,8 +64,8 @@ class OSEMRunner:
         if additive is not None:
             self.additive = additive.reshape(-1).contiguous()
             if self.additive.dtype != torch.float32 or not self.additive.is_cuda or \
-                    self.additive.numel() != self.meas.numel():
-                raise ValueError("additive must be a float32 device tensor shaped like the measurements")
+                    self.additive.numel() != int(proj.nTotMeas[-1]):
+                raise ValueError("additive must be a float32 device tensor with one value per LOR")
         self.rdp_boundary = int(rdp_boundary)
         dev = self.meas.device
         self.S = torch.zeros((proj.subsets, self.N), dtype=torch.float32, device=dev)
@@ -79,7 +83,7 @@ class OSEMRunner:
         first = int(proj.nMeas[k])
         n = int(proj.nMeasSubset[k, 0])
         off = int(proj.nTotMeas[k])
-        yk = self.meas[off:off + n]
+        yk = self.meas[off * self.nb:(off + n) * self.nb]
         ak = None if self.additive is None else self.additive[off:off + n]
         _lib.call("omg_osem_subset_iteration_ex", self.g, first, n, _p(x), _p(yk), _p(ak), _p(self.S[k]),
                   int(not self.sens_done), self.epps, self.beta, self.gamma, self.pos, self._s())
@@ -94,7 +98,7 @@ class OSEMRunner:
         sens = None if self.sens_done else self.S[k]
         if sens is not None:
             sens.zero_()
-        yk = self.meas[off:off + n]
+        yk = self.meas[off * self.nb:(off + n) * self.nb]
         ak = None if self.additive is None else self.additive[off:off + n]
         self.rhs.zero_()
         _lib.call("omg_osem_data_step_ex", self.g, first, n, _p(x), _p(yk), _p(ak), _p(self.rhs), _p(sens),

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -55,7 +56,46 @@ struct Geo {
     const T* atten;            // attenuation: image mu (atten_mode 1) or per-measurement factors (2), may be null
     int32_t atten_mode;        // 0 none, 1 image (paramStruct::CTAttenuation), 2 per measurement
     const T* scat;             // optional per-measurement multiplicative correction (scatterCoefs), may be null
+    uint32_t tof_bins;         // time-of-flight bins (<= 1: none); measurement arrays hold tof_bins blocks of nMeas
+    T tof_sigma;               // TOF standard deviation (mm), paramStruct::sigma_x
+    const T* tof_c;            // TOF bin centres (mm), paramStruct::TOFCenters
+    T tof_epps;                // floor of a voxel's weight sum, paramStruct::epps
 };
+
+// ---- time of flight (projector_functions.h:559-594) ----
+// normPDF: the exponent is evaluated in double (its -0.5 is a double literal), pow(., 1) is the identity.
+template <typename T>
+static inline T tof_pdf(const Geo<T>& g, T x, T mu) {
+    const T a = (x - mu) / g.tof_sigma;
+    return (T)(std::pow((T)0.3989422804014327 / g.tof_sigma, (T)1.) * std::exp(-0.5 * a * a));
+}
+
+// TOFWeight with TRAPZ_BINS = 4
+template <typename T>
+static inline T tof_weight(const Geo<T>& g, T elem, T D, T DD, T mu, T dX) {
+    T out = tof_pdf(g, D, mu);
+    dX = std::copysign(dX, DD);
+    for (int tr = 1; tr < 3; tr++) out += (tof_pdf(g, D - dX * (float)(tr), mu) * (T)2.);
+    out += tof_pdf(g, D - std::copysign(elem, DD), mu);
+    return out;
+}
+
+// TOFLoop: the sum of the bins' weights, floored at epps
+template <typename T>
+static inline T tof_loop(const Geo<T>& g, T DD, T elem, T D) {
+    T sum = (T)0.;
+    const T dX = elem / (T)(4. - 1.);
+    for (uint32_t to = 0; to < g.tof_bins; to++) sum += tof_weight(g, elem, D, DD, g.tof_c[to], dX) * dX;
+    if (sum < g.tof_epps) sum = g.tof_epps;
+    return sum;
+}
+
+// visit(ind, len) or, for a TOF visitor, visit(ind, len, D, DD, esum)
+template <typename T, typename F>
+static inline void call_visit(F& visit, uint32_t ind, T len, T D, T DD, T esum) {
+    if constexpr (std::is_invocable_v<F&, uint32_t, T, T, T, T>) visit(ind, len, D, DD, esum);
+    else visit(ind, len);
+}
 
 // float -> uint32 the way g++/x86-64 converts (cvttss2si to 64 bit, keep the low word).
 template <typename T>
@@ -236,7 +276,9 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         int a1 = 0, a2 = 0;
         T dT1, dT2, db, dd, ddb, dd2;
         uint32_t N, Nrow, Nrun, stride;
+        bool along_x;   // the LOR runs along x (a row of constant y), else along y
         if ((double)std::fabs(ydf) < 1e-8 && r.yd <= bmaxy && r.yd >= g.by) {
+            along_x = true;
             a1 = 0; a2 = (int)g.Nx - 1; dT1 = g.dx; dT2 = g.dx;
             db = g.by; dd = r.yd; ddb = g.dy;
             T e1, e2;
@@ -254,6 +296,7 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
             // row index is multiplied by Ny here, as the reference does (projector_functions.h:1570-1573 -> :1176)
             N = g.Ny; Nrow = g.Ny; Nrun = g.Nx; stride = 1u;
         } else if ((double)std::fabs(xdf) < 1e-8 && r.xd <= bmaxx && r.xd >= g.bx) {
+            along_x = false;
             a1 = 0; a2 = (int)g.Ny - 1; dT1 = g.dx; dT2 = g.dx;
             db = g.bx; dd = r.xd;
             T e1, e2;
@@ -288,7 +331,7 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
                 // the row's voxels with the image's own strides: x = apu (column) or y = apu (row)
                 T jelppi = (T)0.;
                 for (uint32_t iii = 0; iii < Nrun; iii++) {
-                    const uint32_t ax = (stride == 1u) ? iii : apu, ay = (stride == 1u) ? apu : iii;
+                    const uint32_t ax = along_x ? iii : apu, ay = along_x ? apu : iii;
                     jelppi += (dd2 * -g.atten[ax + ay * g.Nx + (uint32_t)tk * Nyx]);
                 }
                 t *= std::exp(jelppi);
@@ -302,14 +345,19 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         } else {
             w = (T)1.;
         }
+        // TOF: half the row, signed by the (near-zero) difference across it (projector_functions.h:1641-1645)
+        T D = (T)0.;
+        if (g.tof_bins > 1) D = (dd2 * (T)Nrun) / std::copysign((T)2., along_x ? ydf : xdf);
+        const T DD = D;
         for (uint32_t ii = (uint32_t)a1; ii <= (uint32_t)a2; ii++) {
             T din = dd2;
             if (ii == (uint32_t)a1) ind += stride * ii;
             if (a1 > 0 && ii == (uint32_t)a1) din = dT1;
             else if ((uint32_t)a2 < Nrun - 1 && ii == (uint32_t)a2) din = dT2;
             // with Nx != Ny the reference's row stride can index past the image (undefined there); dropped here
-            if (ind < Nyx * g.Nz) visit(ind, din);
+            if (ind < Nyx * g.Nz) call_visit(visit, ind, din, D, DD, dd2);
             ind += stride;
+            if (g.tof_bins > 1) D -= std::copysign(dd2, DD);
         }
         return true;
     }
@@ -338,6 +386,12 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         skip = preloop3(g, r, xdf, ydf, zdf, bmaxx, bmaxy, bmaxz, ti, tj, tk, tyu, txu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0);
     }
     if (skip) return false;
+    T D = (T)0.;
+    if (g.tof_bins > 1) {   // TOFDis (projector_functions.h:588-594, :1741-1742)
+        const T xI = xdf * tc, yI = ydf * tc, zI = zdf * tc;
+        D = std::sqrt(xI * xI + yI * yI + zI * zI) - L / (T)2.;
+    }
+    const T DD = D;
     const bool atn_fp = !ct && g.atten_mode == 1 && fp == 1;
     T jelppi = (T)0.;
     if (!ct && g.atten_mode == 1 && fp == 2) {
@@ -385,8 +439,9 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         else { ti += ux; tx0 += txu; }
         if (len > (T)0.) {
             if (atn_fp) jelppi += (len * -g.atten[ind]);
-            visit(ind, len);
+            call_visit(visit, ind, len, D, DD, len);
         }
+        if (g.tof_bins > 1) D -= std::copysign(len, DD);
         if (ti < 0 || (uint32_t)ti >= g.Nx || tj < 0 || (uint32_t)tj >= g.Ny || tk < 0 || (uint32_t)tk >= g.Nz) break;
     }
     if (atn_fp) {
@@ -410,8 +465,23 @@ static void siddon_fp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1000)
     for (int64_t lo = 0; lo < nMeas; lo++) {
-        T ax = (T)0.;
         T w = (T)0.;
+        if (g.tof_bins > 1) {
+            // denominator with TOF (projector_functions.h:609-630); bins in blocks of nMeas (:2030-2035)
+            std::vector<T> ax(g.tof_bins, (T)0.);
+            const bool hit = siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len, T D, T DD, T esum) {
+                const T apu = len * img[ind];
+                const T sum = tof_loop(g, DD, esum, D);
+                const T dX = len / (T)(4. - 1.);
+                for (uint32_t to = 0; to < g.tof_bins; to++) {
+                    const T joku = (tof_weight(g, len, D, DD, g.tof_c[to], dX) * dX);
+                    ax[to] += apu * joku / sum;
+                }
+            });
+            for (uint32_t to = 0; to < g.tof_bins; to++) y[lo + (int64_t)to * nMeas] = hit ? ax[to] * w : (T)0.;
+            continue;
+        }
+        T ax = (T)0.;
         const bool hit = siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len) { ax += len * img[ind]; });
         y[lo] = hit ? ((g.mode == CT_FLAT) ? ax : ax * w) : (T)0.;
     }
@@ -423,8 +493,30 @@ static void siddon_bp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1000)
     for (int64_t lo = 0; lo < nMeas; lo++) {
-        const T yi = yin[lo];
         T w = (T)0.;
+        if (g.tof_bins > 1) {
+            // rhs with TOF (projector_functions.h:632-657): dX = elem / (TRAPZ_BINS - (T)1.) is a double
+            // division rounded to T
+            siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len, T D, T DD, T esum) {
+                const T lw = len * w;
+                const T sum = tof_loop(g, DD, esum, D);
+                const T dX = (T)(len / (4. - (T)1.));
+                T yax = (T)0., val = (T)0.;
+                for (uint32_t to = 0; to < g.tof_bins; to++) {
+                    const T apu = lw * ((tof_weight(g, len, D, DD, g.tof_c[to], dX) * dX) / sum);
+                    val += apu;
+                    yax += apu * yin[lo + (int64_t)to * nMeas];
+                }
+#pragma omp atomic
+                rhs[ind] += yax;
+                if (sens) {
+#pragma omp atomic
+                    sens[ind] += val;
+                }
+            }, 2);
+            continue;
+        }
+        const T yi = yin[lo];
         siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len) {
             const T lw = len * w;
             const T c = yi * lw;
@@ -478,7 +570,8 @@ using namespace oracle;
 template <typename T>
 static Geo<T> make_geo(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y, int32_t mode,
                        int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, const T* norm,
-                       const T* atten, int32_t atten_mode, const T* scat) {
+                       const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins = 0, T tof_sigma = (T)0.,
+                       const T* tof_c = nullptr, T tof_epps = (T)0.) {
     Geo<T> g;
     g.Nx = n3[0]; g.Ny = n3[1]; g.Nz = n3[2];
     g.dx = d3[0]; g.dy = d3[1]; g.dz = d3[2];
@@ -487,32 +580,36 @@ static Geo<T> make_geo(const uint32_t* n3, const T* d3, const T* b3, uint32_t si
     g.dPitchZ = dPitchZ; g.ct_pitch = ct_pitch; g.global_factor = global_factor; g.norm = norm;
     g.atten = atten; g.atten_mode = atten ? atten_mode : 0;
     g.scat = scat;
+    g.tof_bins = tof_bins > 1 ? tof_bins : 1u;
+    g.tof_sigma = tof_sigma;
+    g.tof_c = tof_c;
+    g.tof_epps = tof_epps;
     return g;
 }
 
 #define ORACLE_SIDDON_API(T, SUF)                                                                                     \
     extern "C" int oracle_siddon_fp_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,             \
                                           uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,            \
-                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, int64_t nMeas,           \
+                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, int64_t nMeas,           \
                                           const T* x, const T* z, const T* img, T* y, int nthreads) {                \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps); \
         siddon_fp(g, nMeas, x, z, img, y, nthreads);                                                                   \
         return 0;                                                                                                      \
     }                                                                                                                  \
     extern "C" int oracle_siddon_bp_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,             \
                                           uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,            \
-                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, int64_t nMeas,           \
+                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, int64_t nMeas,           \
                                           const T* x, const T* z, const T* yin, T* rhs, T* sens, int nthreads) {      \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps); \
         siddon_bp(g, nMeas, x, z, yin, rhs, sens, nthreads);                                                          \
         return 0;                                                                                                      \
     }                                                                                                                  \
     extern "C" int64_t oracle_siddon_trace_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,      \
                                                  uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,     \
-                                                 int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, int64_t nMeas,    \
+                                                 int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, int64_t nMeas,    \
                                                  const T* x, const T* z, int64_t* counts, int64_t cap,               \
                                                  uint32_t* idx_out, T* len_out, T* w_out) {                          \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps); \
         int64_t total = 0;                                                                                             \
         for (int64_t lo = 0; lo < nMeas; lo++) {                                                                       \
             int64_t c = 0;                                                                                             \
@@ -541,10 +638,11 @@ ORACLE_SIDDON_API(double, f64)
     extern "C" int oracle_siddon_atten_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,          \
                                              uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,         \
                                              int32_t ct_pitch, T global_factor, const T* norm, const T* atten,       \
-                                             int32_t atten_mode, const T* scat, int64_t nMeas, const T* x, const T* z, T* out,      \
+                                             int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma,        \
+                                             const T* tof_c, T tof_epps, int64_t nMeas, const T* x, const T* z, T* out,      \
                                              int nthreads) {                                                         \
         Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, \
-                               atten, atten_mode, scat);                                                               \
+                               atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps);                         \
         set_threads(nthreads);                                                                                         \
         _Pragma("omp parallel for schedule(dynamic, 1000)")                                                            \
         for (int64_t lo = 0; lo < nMeas; lo++) {                                                                       \

@@ -46,7 +46,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         for T, suf, p in ((C.c_float, "f32", _f32p), (C.c_double, "f64", _f64p)):
             geo = [C.c_void_p, p, p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, T, C.c_int32, T, C.c_void_p,
-                   C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
+                   C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, T, C.c_void_p, T, C.c_int64]
             f = getattr(_lib, "oracle_siddon_fp_" + suf)
             f.argtypes = geo + [p, p, p, p, C.c_int]
             f.restype = C.c_int
@@ -89,8 +89,8 @@ def ref():
         for T, suf, p in ((C.c_float, "f32", _f32p), (C.c_double, "f64", _f64p)):
             f = getattr(_ref, "ref_siddon_" + suf)
             f.argtypes = [C.c_void_p, p, p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, T, C.c_int32, T,
-                          C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, C.c_int64, p, p, p, p,
-                          C.c_void_p, C.c_int, C.c_int]
+                          C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, T, C.c_void_p, T,
+                          C.c_uint32, C.c_int64, p, p, p, p, C.c_void_p, C.c_int, C.c_int]
             f.restype = C.c_int
         _ref.ref_rdp_f32.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
     return _ref
@@ -100,9 +100,12 @@ class SiddonCase:
     """Everything a Siddon call needs besides the data vectors."""
 
     def __init__(self, grid, size_x, size_y, x, z, mode=MODE_PET, plane_offset=0, dPitchZ=0.0, ct_pitch=0,
-                 global_factor=1.0, norm=None, n_proj=None, dtype=np.float32, atten=None, atten_mode=0, scat=None):
+                 global_factor=1.0, norm=None, n_proj=None, dtype=np.float32, atten=None, atten_mode=0, scat=None,
+                 tof_bins=0, tof_sigma=0.0, tof_centers=None, tof_epps=1e-5):
         """atten: attenuation image mu (atten_mode 1, N values, 1/mm, x fastest) or one factor per
-        measurement (atten_mode 2, norm's order); scat: per-measurement multiplicative correction."""
+        measurement (atten_mode 2, norm's order); scat: per-measurement multiplicative correction;
+        tof_bins > 1: time of flight with standard deviation tof_sigma (mm) and bin centres tof_centers
+        (mm); measurement vectors then hold tof_bins blocks of nmeas values."""
         self.dtype = np.dtype(dtype)
         self.n3 = np.ascontiguousarray(grid.n3())
         self.d3 = grid.d3(self.dtype)
@@ -121,6 +124,10 @@ class SiddonCase:
         self.atten = None if atten is None else np.ascontiguousarray(atten, dtype=self.dtype)
         self.atten_mode = int(atten_mode) if atten is not None else 0
         self.scat = None if scat is None else np.ascontiguousarray(scat, dtype=self.dtype)
+        self.tof_bins = int(tof_bins) if tof_bins and tof_bins > 1 else 1
+        self.tof_sigma = tof_sigma
+        self.tof_centers = None if tof_centers is None else np.ascontiguousarray(tof_centers, dtype=self.dtype)
+        self.tof_epps = tof_epps
 
     @property
     def suf(self):
@@ -130,12 +137,14 @@ class SiddonCase:
         normp = None if self.norm is None else self.norm.ctypes.data
         atnp = None if self.atten is None else self.atten.ctypes.data
         scp = None if self.scat is None else self.scat.ctypes.data
+        tcp = None if self.tof_centers is None else self.tof_centers.ctypes.data
         return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
-                self.dPitchZ, self.ct_pitch, self.global_factor, normp, atnp, self.atten_mode, scp, nmeas]
+                self.dPitchZ, self.ct_pitch, self.global_factor, normp, atnp, self.atten_mode, scp,
+                self.tof_bins, self.tof_sigma, tcp, self.tof_epps, nmeas]
 
     # ---- oracle -------------------------------------------------------------------------
     def fp(self, img, nmeas, nthreads=0):
-        y = np.zeros(nmeas, dtype=self.dtype)
+        y = np.zeros(nmeas * self.tof_bins, dtype=self.dtype)
         getattr(lib(), "oracle_siddon_fp_" + self.suf)(*self._geo(nmeas), self.x, self.z,
                                                        np.ascontiguousarray(img, dtype=self.dtype), y, nthreads)
         return y
@@ -171,7 +180,7 @@ class SiddonCase:
     # ---- reference checker ---------------------------------------------------------------
     def ref_fp(self, img, nmeas, nthreads=0):
         r = ref()
-        y = np.zeros(nmeas, dtype=self.dtype)
+        y = np.zeros(nmeas * self.tof_bins, dtype=self.dtype)
         g = self._geo(nmeas)
         getattr(r, "ref_siddon_" + self.suf)(*g[:-1], self.n_proj, nmeas, self.x, self.z,
                                              np.ascontiguousarray(img, dtype=self.dtype).copy(), y, None, 1, nthreads)
@@ -271,7 +280,7 @@ def _update64(x, S, dU, rhs, epps, positivity):
 
 def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter, x0, epps=1e-5, beta=0.0,
          gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None, dtype=np.float32,
-         additive=None, atten=None, atten_mode=0):
+         additive=None, atten=None, atten_mode=0, tof=None):
     """OSEM / OSL-OSEM(RDP) on the CPU in the reference's step order (see omega_amd/recon.py).
 
     ``meas`` is the full measurement vector in subset order. With ``use_ref`` the forward and
@@ -280,8 +289,12 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
     ``dtype=np.float64`` is the FP64 re-run: the same projector and steps in double precision (not
     with ``use_ref``). ``additive`` (optional, laid out like ``meas``) is the randoms / scatter term
     of the ratio. ``atten``: attenuation image (``atten_mode`` 1) or per-measurement factors laid out
-    like ``meas`` (``atten_mode`` 2). Returns the image (float32, or float64 for the FP64 re-run).
+    like ``meas`` (``atten_mode`` 2). ``tof``: optional dict of SiddonCase's tof_* arguments; each
+    subset's measurements are then tof_bins blocks of its LORs and ``additive`` (one value per LOR)
+    is added to every bin. Returns the image (float32, or float64 for the FP64 re-run).
     """
+    tof = dict(tof or {})
+    nb = int(tof.get("tof_bins", 1)) if int(tof.get("tof_bins", 1)) > 1 else 1
     f64 = np.dtype(dtype) == np.float64
     assert not (f64 and use_ref), "the reference build is FP32 only"
     per = size_x * size_y
@@ -299,11 +312,11 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
             if atten is not None and atten_mode == 2:
                 atn = atten[starts[k] * per: starts[k] * per + n]
             case = SiddonCase(grid, size_x, size_y, xy, z_subset_ordered, plane_offset=int(starts[k]), dtype=dtype,
-                              atten=atn, atten_mode=atten_mode)
-            yk = meas[starts[k] * per: starts[k] * per + n]
+                              atten=atn, atten_mode=atten_mode, **tof)
+            yk = meas[nb * starts[k] * per: nb * (starts[k] * per + n)]
             yhat = case.ref_fp(x, n, nthreads) if use_ref else case.fp(x, n, nthreads)
             if additive is not None:
-                ak = additive[starts[k] * per: starts[k] * per + n]
+                ak = np.tile(additive[starts[k] * per: starts[k] * per + n], nb)
                 r = _ratio64(yk, yhat, epps, ak) if f64 else ratio_additive(yk, yhat, ak, epps)
             else:
                 r = _ratio64(yk, yhat, epps) if f64 else ratio(yk, yhat, epps)

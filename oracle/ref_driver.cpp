@@ -15,7 +15,8 @@
 template <typename T>
 static void fill(paramStruct<T>& p, const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y,
                  int32_t mode, int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, T* norm,
-                 T* atten, int32_t atten_mode, T* scat, uint32_t nProjections) {
+                 T* atten, int32_t atten_mode, T* scat, uint32_t tof_bins, T tof_sigma, T* tof_c, T tof_epps,
+                 uint32_t nProjections) {
     p.Nx = n3[0]; p.Ny = n3[1]; p.Nz = n3[2];
     p.dx = d3[0]; p.dy = d3[1]; p.dz = d3[2];
     p.bx = b3[0]; p.by = b3[1]; p.bz = b3[2];
@@ -27,7 +28,11 @@ static void fill(paramStruct<T>& p, const uint32_t* n3, const T* d3, const T* b3
     p.nMeas = (uint64_t)plane_offset;
     p.listMode = (mode == 2) ? 1 : 0;
     p.raw = false;
-    p.nRays2D = 1; p.nRays3D = 1; p.nBins = 1; p.TOF = false;
+    p.nRays2D = 1; p.nRays3D = 1;
+    p.TOF = tof_bins > 1;
+    p.nBins = tof_bins > 1 ? tof_bins : 1u;
+    p.sigma_x = tof_sigma;
+    p.TOFCenters = tof_c;
     p.globalFactor = global_factor;
     p.normalizationCorrection = norm != nullptr;
     p.normCoefs = norm;
@@ -40,18 +45,19 @@ static void fill(paramStruct<T>& p, const uint32_t* n3, const T* d3, const T* b3
     p.dPitchZ = dPitchZ; p.dPitchXY = dPitchZ;
     p.pitch = ct_pitch != 0;
     p.nProjections = nProjections;
-    p.epps = (T)1e-5;
+    p.epps = tof_bins > 1 ? tof_epps : (T)1e-5;
 }
 
 #define REF_API(T, SUF)                                                                                               \
     extern "C" int ref_siddon_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y,  \
                                     int32_t mode, int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, \
-                                    T* norm, T* atten, int32_t atten_mode, T* scat, uint32_t nProjections,            \
+                                    T* norm, T* atten, int32_t atten_mode, T* scat, uint32_t tof_bins, T tof_sigma,  \
+                                    T* tof_c, T tof_epps, uint32_t nProjections,                                      \
                                     int64_t nMeas,                                                                    \
                                     const T* x, const T* z, T* input, T* output, T* sens, int fp, int nthreads) {     \
         paramStruct<T> p;                                                                                              \
         fill(p, n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten,        \
-             atten_mode, scat, nProjections);                                                                          \
+             atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps, nProjections);                                    \
         p.noSensImage = (sens == nullptr);                                                                             \
         projectorType123Implementation4<T>(p, nMeas, output, x, z, input, mode == 1, false, (uint8_t)fp, sens,        \
                                            nullptr, (uint32_t)nthreads);                                               \

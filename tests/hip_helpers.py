@@ -11,11 +11,11 @@ from omega_amd.projector import Geometry
 
 def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
               ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_flags=0, dPitchX=0.0, dPitchY=0.0,
-              generic=False, x_fastest_only=False, atten=None, atten_mode=0, scat=None):
+              generic=False, x_fastest_only=False, atten=None, atten_mode=0, scat=None, tof=None):
     """Geometry handle through the C ABI; ``generic`` routes every call through the one-lane-per-LOR
     kernels (omg_geom_desc.reserved[0]); ``x_fastest_only`` keeps those kernels in the x-fastest image
     (reserved[3]); ``atten`` is an attenuation image (``atten_mode`` 1) or per-measurement factors (2);
-    ``scat`` a per-measurement multiplicative correction."""
+    ``scat`` a per-measurement multiplicative correction; ``tof`` = (bins, sigma_mm, centres_mm, epps)."""
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
     d = _lib.GeomDesc()
@@ -48,6 +48,11 @@ def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, 
         scat = np.ascontiguousarray(scat, dtype=np.float32).ravel()
         keep.append(scat)
         d.scat, d.scat_len = scat.ctypes.data, scat.size
+    if tof is not None:
+        bins, sigma, centres, epps = tof
+        centres = np.ascontiguousarray(centres, dtype=np.float32)
+        keep.append(centres)
+        d.tof_bins, d.tof_sigma, d.tof_epps, d.tof_centers = int(bins), float(sigma), float(epps), centres.ctypes.data
     d.orth_width = orth_width
     d.orth_flags = orth_flags
     d.dPitchX = dPitchX
@@ -64,8 +69,10 @@ def ptr(t):
 
 
 def fp(geom, first, n, img_t):
+    """y = A img for measurements [first, first + n): n values per TOF bin."""
     import torch
-    y = torch.empty(n, dtype=torch.float32, device=img_t.device)
+    bins = max(1, int(geom.desc.tof_bins))
+    y = torch.empty(n * bins, dtype=torch.float32, device=img_t.device)
     _lib.call("omg_forward_project", geom.handle, first, n, ptr(img_t), ptr(y), None)
     return y
 

@@ -119,6 +119,23 @@ def test_create_rejects_bad_correction_tables():
         assert msg in _lib.last_error(), (msg, _lib.last_error())
 
 
+def test_create_rejects_bad_tof():
+    """TOF needs the Siddon projector on PET data, at most 64 bins, a positive sigma and the centres."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    c = np.zeros(65, np.float32)
+    checks = [
+        (dict(tof_bins=65, tof_sigma=10.0, tof_centers=c.ctypes.data), "exceeds 64"),
+        (dict(tof_bins=3, tof_sigma=0.0, tof_centers=c.ctypes.data), "tof_sigma"),
+        (dict(tof_bins=3, tof_sigma=10.0), "without tof_centers"),
+        (dict(tof_bins=3, tof_sigma=10.0, tof_centers=c.ctypes.data, projector_type=2), "projector_type 1"),
+    ]
+    for kw, msg in checks:
+        d, keep = _desc(**kw)
+        assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG, msg
+        assert msg in _lib.last_error(), (msg, _lib.last_error())
+
+
 def test_bdd_requires_ct():
     d, keep = _desc(projector_type=5)
     h = C.c_void_p()
