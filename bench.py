@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_traffic.txt"),
                     help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
-                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet3d_autograd"],
+                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet3d_autograd", "pet2d_tof"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     ap.add_argument("--voxel-bp", action="store_true",
@@ -62,7 +62,7 @@ def main():
 
     if args.workload != "pet2d_osem":
         {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth,
-         "pet3d_autograd": bench_autograd}[args.workload](args)
+         "pet3d_autograd": bench_autograd, "pet2d_tof": bench_tof}[args.workload](args)
         return
 
     import ctypes as C
@@ -402,6 +402,74 @@ def bench_orth(args):
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_orth_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt)},
+        "cpu_baseline": None,
+    })
+
+
+def bench_tof(args):
+    """2-D plane PET OSEM with time of flight and attenuation on the pet2d_osem geometry: 17 TOF bins of
+    140 ps (400 ps FWHM), a water cylinder (0.0096 / mm, radius 120 mm) as attenuation image.
+    value = subset-iterations/s (8 subsets x 1 iteration per step). The TOF kernels are bound by the
+    Gaussian's double-precision exponentials (4 per bin and voxel visit), reported as exps/s."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from omega_amd import _lib, workloads
+    from omega_amd.recon import OSEMRunner
+    dev = torch.device("cuda:0")
+    _lib.load()
+    p = workloads.pet2d_projector(subsets=8)
+    p.TOF_bins, p.TOF_bins_used, p.TOF_width, p.TOF_FWHM = 17, 17, 140e-12, 400e-12
+    g = p.grid
+    xc = (np.arange(g.Nx) + 0.5) * g.dx + g.bx
+    yc = (np.arange(g.Ny) + 0.5) * g.dy + g.by
+    mu2 = np.where(xc[:, None] ** 2 + yc[None, :] ** 2 < 120.0 ** 2, 0.0096, 0.0).astype(np.float32)
+    p.attenuation_correction = True
+    p.vaimennus = np.repeat(mu2[:, :, None], g.Nz, axis=2)
+    torch.cuda.synchronize()
+    t_set = time.perf_counter()
+    p.initProj()
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t_set) * 1e3
+    ph, y = workloads.pet2d_data(p, seed=0)
+    runner = OSEMRunner(p, y)
+    x = torch.ones(p.grid.N, device=dev)
+    for _ in range(args.warmup):
+        for k in range(p.subsets):
+            runner.subset_step(x, k)
+        runner.sens_done = True
+    torch.cuda.synchronize()
+    _progress("pet2d_tof warm-up done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for k in range(p.subsets):
+            runner.subset_step(x, k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # one subset step timed alone; its visits give the exponential count (FP and BP each: 4 per bin and
+    # visit, the BP's weights reused from its sum)
+    ntr = _trace_counts(p, 0, dev)
+    M0 = int(p.nMeasSubset[0, 0])
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    runner.subset_step(x, 0)
+    b.record()
+    b.synchronize()
+    s_s = a.elapsed_time(b) * 1e-3
+    exps = 2.0 * 4.0 * p.nBins * ntr
+    byt = 12.0 * ntr + 8.0 * M0 * p.nBins
+    _common(args, {
+        "metric": "OSEM subset-iters/s, improved Siddon with TOF and attenuation", "value": round(p.subsets * args.steps / dt, 2),
+        "unit": "subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "data": "synthetic (seeded ellipsoid phantom, Poisson TOF data at 5e6 counts)",
+        "config": {"workload": "pet2d_osem_tof", "image": [128, 128, 63], "sinogram": [128, 160, 63],
+                   "tof_bins": p.nBins, "tof_fwhm_ps": 400, "tof_bin_ps": 140, "attenuation": "water cylinder",
+                   "subsets": 8, "voxel_visits_subset0": ntr, "geometry_setup_ms": round(setup_ms, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_tof_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
+                     "algorithmic_bytes_per_launch": int(byt),
+                     "fp64_exp_per_s": round(exps / s_s, 1)},
         "cpu_baseline": None,
     })
 
