@@ -410,7 +410,8 @@ def bench_tof(args):
     """2-D plane PET OSEM with time of flight and attenuation on the pet2d_osem geometry: 17 TOF bins of
     140 ps (400 ps FWHM), a water cylinder (0.0096 / mm, radius 120 mm) as attenuation image.
     value = subset-iterations/s (8 subsets x 1 iteration per step). The TOF kernels are bound by the
-    Gaussian's double-precision exponentials (4 per bin and voxel visit), reported as exps/s."""
+    Gaussian's double-precision exponentials (3 per bin and voxel visit: the trapezoid's last point is
+    the next voxel's first), reported as exps/s."""
     import ctypes as C
     import numpy as np
     import torch
@@ -446,8 +447,8 @@ def bench_tof(args):
             runner.subset_step(x, k)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    # one subset step timed alone; its visits give the exponential count (FP and BP each: 4 per bin and
-    # visit, the BP's weights reused from its sum)
+    # one subset step timed alone; its visits give the exponential count (FP and BP each: 3 per bin and
+    # visit, the first point shared with the previous voxel; the BP's weights reused from its sum)
     ntr = _trace_counts(p, 0, dev)
     M0 = int(p.nMeasSubset[0, 0])
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -456,7 +457,7 @@ def bench_tof(args):
     b.record()
     b.synchronize()
     s_s = a.elapsed_time(b) * 1e-3
-    exps = 2.0 * 4.0 * p.nBins * ntr
+    exps = 2.0 * 3.0 * p.nBins * ntr
     byt = 12.0 * ntr + 8.0 * M0 * p.nBins
     _common(args, {
         "metric": "OSEM subset-iters/s, improved Siddon with TOF and attenuation", "value": round(p.subsets * args.steps / dt, 2),

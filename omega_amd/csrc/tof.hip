@@ -21,16 +21,38 @@ namespace omg {
 
 constexpr int kTofBlock = 64;
 
+// Per-lane cache of the Gaussians at the last trapezoid point of the previous voxel: on a Siddon walk
+// that point, D - copysign(len, DD), is bitwise the next voxel's D, so its values are the next
+// voxel's first terms (a quarter of the exponentials saved; keyed by the exact argument, so the
+// results do not change).
+struct TofCache {
+    float* x;   // the cached argument (one float per lane), NaN when empty
+    float* p;   // p[b * kTofBlock]: Gaussian of bin b at *x
+};
+
 // TOFLoop (projector_functions.h:575-586): the bins' weights for a voxel of length `elem` at distance
-// D into wl[b * kTofBlock], dX = elem / 3.f; returns their sum floored at epps.
-__device__ __forceinline__ float tof_weights(const SidArgs& a, float elem, float D, float DD, float* wl) {
+// D into wl[b * kTofBlock], dX = elem / 3.f; returns their sum floored at epps. Each weight is
+// TOFWeight's p(D) + 2 p(D - dX) + 2 p(D - 2 dX) + p(D - elem) (signs by DD), times dX.
+__device__ __forceinline__ float tof_weights(const SidArgs& a, float elem, float D, float DD, float* wl,
+                                             TofCache c) {
     const float dX = elem / 3.f;
+    const float dXs = copysignf(dX, DD);
+    const float x3 = D - copysignf(elem, DD);
+    const bool reuse = D == *c.x;
     float sum = 0.f;
     for (uint32_t b = 0; b < a.tof_bins; b++) {
-        const float v = tof_weight(a, elem, D, DD, a.tof_c[b], dX) * dX;
+        const float mu = a.tof_c[b];
+        float out = reuse ? c.p[b * kTofBlock] : tof_pdf(a, D, mu);
+        out += tof_pdf(a, D - dXs * 1.f, mu) * 2.f;
+        out += tof_pdf(a, D - dXs * 2.f, mu) * 2.f;
+        const float p3 = tof_pdf(a, x3, mu);
+        out += p3;
+        c.p[b * kTofBlock] = p3;
+        const float v = out * dX;
         wl[b * kTofBlock] = v;
         sum += v;
     }
+    *c.x = x3;
     return sum < a.tof_epps ? a.tof_epps : sum;
 }
 
@@ -45,22 +67,24 @@ __device__ __forceinline__ void tof_bp_weights(const SidArgs& a, float len, floa
 
 // FP walk: acc[b] += (len * src[v]) * w_b / sum over the LOR's voxels (denominator, :609-630).
 __device__ __forceinline__ bool tof_gather(const SidArgs& a, int64_t lo, const float* __restrict__ src, float* acc,
-                                           float* wl, float& w) {
+                                           float* wl, TofCache c, float& w) {
     for (uint32_t b = 0; b < a.tof_bins; b++) acc[b * kTofBlock] = 0.f;
+    *c.x = __builtin_nanf("");
     return siddon_tof_trace(a, lo, w, [&](uint32_t ind, float len, float D, float DD, float esum) {
         const float apu = len * src[ind];
-        const float sum = tof_weights(a, esum, D, DD, wl);
-        if (esum != len) tof_weights(a, len, D, DD, wl);
+        const float sum = tof_weights(a, esum, D, DD, wl, c);
+        if (esum != len) tof_weights(a, len, D, DD, wl, c);
         for (uint32_t b = 0; b < a.tof_bins; b++) acc[b * kTofBlock] = acc[b * kTofBlock] + apu * wl[b * kTofBlock] / sum;
     });
 }
 
 // BP walk: rhs[v] += sum_b (len w) (w_b / sum) yv[b]; sens[v] += sum_b (len w) (w_b / sum).
-__device__ __forceinline__ void tof_scatter(const SidArgs& a, int64_t lo, const float* yv, float* wl,
+__device__ __forceinline__ void tof_scatter(const SidArgs& a, int64_t lo, const float* yv, float* wl, TofCache c,
                                             float* __restrict__ rhs, float* __restrict__ sens) {
     float w = 0.f;
+    *c.x = __builtin_nanf("");
     siddon_tof_trace(a, lo, w, [&](uint32_t ind, float len, float D, float DD, float esum) {
-        const float sum = tof_weights(a, esum, D, DD, wl);
+        const float sum = tof_weights(a, esum, D, DD, wl, c);
         tof_bp_weights(a, len, esum, D, DD, wl);
         const float lw = len * w;
         float yax = 0.f, val = 0.f;
@@ -74,6 +98,10 @@ __device__ __forceinline__ void tof_scatter(const SidArgs& a, int64_t lo, const 
     });
 }
 
+__device__ __forceinline__ TofCache tof_cache(const SidArgs& a, float* lds) {
+    return TofCache{lds + 3 * a.tof_bins * kTofBlock + threadIdx.x, lds + 2 * a.tof_bins * kTofBlock + threadIdx.x};
+}
+
 // y[lo + b n] = w * acc[b]
 __global__ __launch_bounds__(kTofBlock) void k_tof_fp(SidArgs a, const float* __restrict__ img, float* __restrict__ y) {
     extern __shared__ float lds[];
@@ -81,8 +109,9 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_fp(SidArgs a, const float* __
     if (lo >= a.nmeas) return;
     float* acc = lds + threadIdx.x;
     float* wl = lds + a.tof_bins * kTofBlock + threadIdx.x;
+    const TofCache c = tof_cache(a, lds);
     float w = 0.f;
-    const bool hit = tof_gather(a, lo, img, acc, wl, w);
+    const bool hit = tof_gather(a, lo, img, acc, wl, c, w);
     for (uint32_t b = 0; b < a.tof_bins; b++) y[lo + (int64_t)b * a.nmeas] = hit ? acc[b * kTofBlock] * w : 0.f;
 }
 
@@ -93,6 +122,7 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_bp(SidArgs a, const float* __
     if (lo >= a.nmeas) return;
     float* yv = lds + threadIdx.x;
     float* wl = lds + a.tof_bins * kTofBlock + threadIdx.x;
+    const TofCache c = tof_cache(a, lds);
     bool any = false;
     for (uint32_t b = 0; b < a.tof_bins; b++) {
         const float v = yin[lo + (int64_t)b * a.nmeas];
@@ -100,7 +130,7 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_bp(SidArgs a, const float* __
         any |= v != 0.f;
     }
     if (!any && sens == nullptr) return;   // adds of +0 leave rhs unchanged
-    tof_scatter(a, lo, yv, wl, rhs, sens);
+    tof_scatter(a, lo, yv, wl, c, rhs, sens);
 }
 
 // Fused data step: yhat_b = w acc_b; r_b = meas_b / ((yhat_b + epps) + add); NaN/Inf -> epps; BP of r.
@@ -112,18 +142,20 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_osem(SidArgs a, const float* 
     if (lo >= a.nmeas) return;
     float* acc = lds + threadIdx.x;
     float* wl = lds + a.tof_bins * kTofBlock + threadIdx.x;
+    const TofCache c = tof_cache(a, lds);
     float w = 0.f;
-    if (!tof_gather(a, lo, img, acc, wl, w)) return;
+    if (!tof_gather(a, lo, img, acc, wl, c, w)) return;
     for (uint32_t b = 0; b < a.tof_bins; b++) {
         const float yhat = acc[b * kTofBlock] * w;
         float r = meas[lo + (int64_t)b * a.nmeas] / ratio_den(a, lo, yhat, epps);
         if (isnan(r) || isinf(r)) r = epps;
         acc[b * kTofBlock] = r;
     }
-    tof_scatter(a, lo, acc, wl, rhs, sens);
+    tof_scatter(a, lo, acc, wl, c, rhs, sens);
 }
 
-static size_t tof_lds(const SidArgs& a) { return 2 * (size_t)a.tof_bins * kTofBlock * sizeof(float); }
+// LDS per workgroup: [acc or yv: bins x lanes][wl: bins x lanes][cache p: bins x lanes][cache x: lanes]
+static size_t tof_lds(const SidArgs& a) { return (3 * (size_t)a.tof_bins + 1) * kTofBlock * sizeof(float); }
 
 int tof_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
     const SidArgs a = make_sid_args(g, first, n);
