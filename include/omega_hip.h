@@ -101,6 +101,12 @@ typedef struct omg_geom_desc {
      * tof_centers: the bins' centre offsets along the LOR (mm, tof_bins floats, copied at create);
      * tof_epps: the floor of a voxel's weight sum (the reference's epps). */
     uint32_t tof_bins; float tof_sigma; float tof_epps; const float* tof_centers;
+    /* several rays per LOR (projectorClass.n_rays_transaxial / n_rays_axial, -DN_RAYS2D / -DN_RAYS3D,
+     * init.py:357-360; paramStruct nRays2D, nRays3D, projector_functions.h:262-264,1444-1466): rays_xy x
+     * rays_z rays (0 or 1: one), their endpoints spread over cr_xy (transaxially, on x and y alike) and
+     * cr_z (axially) (paramStruct dPitchXY, dPitchZ; the operator API passes dPitchX and dPitchY); each
+     * ray weighs 1 / (L nRays), the forward projection sums them. Siddon, PET data, no TOF. */
+    uint32_t rays_xy, rays_z; float cr_xy, cr_z;
 } omg_geom_desc;
 
 /* Library/version info and error text. */

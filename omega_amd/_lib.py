@@ -54,6 +54,7 @@ class GeomDesc(C.Structure):
         ("atten_mode", C.c_int32), ("atten", C.c_void_p), ("atten_len", C.c_int64),
         ("scat", C.c_void_p), ("scat_len", C.c_int64),
         ("tof_bins", C.c_uint32), ("tof_sigma", C.c_float), ("tof_epps", C.c_float), ("tof_centers", C.c_void_p),
+        ("rays_xy", C.c_uint32), ("rays_z", C.c_uint32), ("cr_xy", C.c_float), ("cr_z", C.c_float),
     ]
 
 

@@ -197,6 +197,16 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
             OMG_REQUIRE(std::isfinite(d->tof_centers[b]), "tof_centers[%u] is not finite", b);
         OMG_REQUIRE(d->tof_epps >= 0.f, "tof_epps must be >= 0");
     }
+    const uint64_t rays = (uint64_t)std::max(d->rays_xy, 1u) * std::max(d->rays_z, 1u);
+    if (rays > 1) {
+        OMG_REQUIRE(rays <= 256, "%llu rays per LOR exceeds 256", (unsigned long long)rays);
+        OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON && d->mode != OMG_MODE_CT_FLAT,
+                    "several rays per LOR are provided for the improved Siddon projector on PET data");
+        OMG_REQUIRE(d->tof_bins <= 1, "several rays per LOR together with TOF are not provided");
+        OMG_REQUIRE(d->atten_mode != 1, "several rays per LOR together with an attenuation image are not provided "
+                                        "(each ray has its own attenuation factor)");
+        OMG_REQUIRE(std::isfinite(d->cr_xy) && std::isfinite(d->cr_z), "cr_xy and cr_z must be finite");
+    }
     OMG_REQUIRE(d->atten_mode >= 0 && d->atten_mode <= 2, "atten_mode %d is not 0 (off), 1 (image) or 2 (per LOR)",
                 d->atten_mode);
     if (d->atten_mode) {
@@ -238,7 +248,7 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     if (!rc && d->atten_mode == 1) rc = attenuation_prepare(g, d->atten, n_total);
     if (!rc && d->mode == OMG_MODE_PET_SINO) {
         g->z_host.assign(d->z, d->z + d->z_len);
-        g->planar = d->meas_index == nullptr && d->tof_bins <= 1;
+        g->planar = d->meas_index == nullptr && d->tof_bins <= 1 && rays <= 1;
         for (uint32_t p = 0; p < d->n_proj && g->planar; p++) g->planar = d->z[2 * p] == d->z[2 * p + 1];
     }
     if (!rc && g->planar) g->ring = ring_table(*d, d->x, g->bmaxx, g->bmaxy);

@@ -356,6 +356,7 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
     OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
+    OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     OMG_REQUIRE(x && meas && S && rhs, "null pointer argument");
     OMG_REQUIRE(n_frames >= 1, "n_frames must be >= 1");
     OMG_REQUIRE(rays_per_block == 64 || rays_per_block == 128 || rays_per_block == 256,
@@ -413,6 +414,7 @@ extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t 
                 "batched frames run the improved Siddon projector on emission data");
     OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
+    OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     const int rc = check_range(g, first, n_meas);
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
@@ -436,6 +438,7 @@ extern "C" int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_me
                 "batched frames run the improved Siddon projector on emission data");
     OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
+    OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     OMG_REQUIRE(n_frames == 1 || yhat_stride >= n_meas, "yhat_stride must cover the subset's measurements");
     const int rc = check_range(g, first, n_meas);
     if (rc) return rc;
@@ -496,6 +499,7 @@ extern "C" int omg_frames_graph_create(const omg_geom* g, int32_t n_subsets, con
     OMG_REQUIRE(n_subsets >= 1 && first && n_meas && meas_offset, "subset tables are required");
     OMG_REQUIRE(g != nullptr, "null geometry");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
+    OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
     OMG_REQUIRE(n_frames >= 1, "n_frames must be >= 1");
     float* rhs_t = nullptr;

@@ -77,6 +77,10 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.tof_pf = g->d.tof_sigma > 0.f ? (float)0.3989422804014327 / g->d.tof_sigma : 0.f;
     a.tof_epps = g->d.tof_epps;
     a.tof_c = g->tof_c;
+    a.rays_xy = g->d.rays_xy > 1 ? g->d.rays_xy : 1u;
+    a.rays_z = g->d.rays_z > 1 ? g->d.rays_z : 1u;
+    a.cr_xy = g->d.cr_xy;
+    a.cr_z = g->d.cr_z;
     return a;
 }
 

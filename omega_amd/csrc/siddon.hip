@@ -64,6 +64,15 @@ __global__ void k_fill_f32(float* __restrict__ p, uint32_t n, float v) {
 __global__ __launch_bounds__(kBlock) void k_siddon_fp(SidArgs a, const float* __restrict__ img, float* __restrict__ y) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (lo >= a.nmeas) return;
+    if (n_rays(a) > 1) {   // the rays' weighted sums, added in ray order (projector_functions.h:2036-2056)
+        float total = 0.f;
+        for (uint32_t ray = 0; ray < n_rays(a); ray++) {
+            float ax, w = 0.f;
+            if (siddon_gather<false>(a, lo, img, ax, w, (int)ray)) total += ax * w;
+        }
+        y[lo] = total;
+        return;
+    }
     float ax, w = 0.f;
     const bool hit = siddon_gather<false>(a, lo, img, ax, w);
     y[lo] = hit ? (a.mode == MODE_CT_FLAT ? ax : ax * w) : 0.f;
@@ -114,6 +123,15 @@ __global__ __launch_bounds__(kBlock) void k_siddon_bp(SidArgs a, const float* __
     const float yi = yin[lo];
     if (yi == 0.f && sens == nullptr) return;  // adds of +0 leave rhs unchanged
     float w = 0.f;
+    if (n_rays(a) > 1) {   // each ray with its own weight
+        for (uint32_t ray = 0; ray < n_rays(a); ray++)
+            siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
+                const float lw = len * w;
+                unsafeAtomicAdd(rhs + ind, yi * lw);
+                if (sens) unsafeAtomicAdd(sens + ind, lw);
+            }, (int)ray);
+        return;
+    }
     if (sens) {
         siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
             const float lw = len * w;
@@ -136,11 +154,16 @@ __global__ __launch_bounds__(kBlock) void k_siddon_bp_split(SidArgs a, const flo
     const float yi = yin[lo];
     if (!SENS && yi == 0.f) return;
     float w = 0.f;
-    siddon_trace_split(a, lo, w, [&](uint32_t ind, float len, bool t) {
+    const auto visit = [&](uint32_t ind, float len, bool t) {
         const float lw = len * w;
         unsafeAtomicAdd((t ? rhs_t : rhs) + ind, yi * lw);
         if (SENS) unsafeAtomicAdd((t ? sens_t : sens) + ind, lw);
-    });
+    };
+    if (n_rays(a) > 1) {
+        for (uint32_t ray = 0; ray < n_rays(a); ray++) siddon_trace_split(a, lo, w, visit, (int)ray);
+        return;
+    }
+    siddon_trace_split(a, lo, w, visit);
 }
 
 // dst[x + y*Nx + z*Nx*Ny] += src_t[y + x*Ny + z*Nx*Ny], through 32 x 32 LDS tiles (both sides coalesced).
@@ -170,12 +193,43 @@ __global__ __launch_bounds__(kBlock) void k_siddon_osem(SidArgs a, const float* 
                                                         float* __restrict__ sens_t, float epps) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (lo >= a.nmeas) return;
-    float ax, w = 0.f;
-    if (!siddon_gather<false>(a, lo, img, ax, w)) return;
-    const float yhat = ax * w;
+    const uint32_t nr = n_rays(a);
+    float ax, w = 0.f, yhat;
+    if (nr > 1) {   // all rays missing the image: nothing to backproject, as with one ray
+        yhat = 0.f;
+        bool any = false;
+        for (uint32_t ray = 0; ray < nr; ray++) {
+            if (siddon_gather<false>(a, lo, img, ax, w, (int)ray)) {
+                yhat += ax * w;
+                any = true;
+            }
+        }
+        if (!any) return;
+    } else {
+        if (!siddon_gather<false>(a, lo, img, ax, w)) return;
+        yhat = ax * w;
+    }
     float r = meas[lo] / ratio_den(a, lo, yhat, epps);
     if (isnan(r) || isinf(r)) r = epps;
     if (r == 0.f && sens == nullptr) return;
+    if (nr > 1) {
+        for (uint32_t ray = 0; ray < nr; ray++) {
+            if (rhs_t) {
+                siddon_trace_split(a, lo, w, [&](uint32_t ind, float len, bool t) {
+                    const float lw = len * w;
+                    unsafeAtomicAdd((t ? rhs_t : rhs) + ind, r * lw);
+                    if (sens) unsafeAtomicAdd((t ? sens_t : sens) + ind, lw);
+                }, (int)ray);
+            } else {
+                siddon_trace(a, lo, w, [&](uint32_t ind, float len) {
+                    const float lw = len * w;
+                    unsafeAtomicAdd(rhs + ind, r * lw);
+                    if (sens) unsafeAtomicAdd(sens + ind, lw);
+                }, (int)ray);
+            }
+        }
+        return;
+    }
     if (rhs_t) {
         siddon_trace_split(a, lo, w, [&](uint32_t ind, float len, bool t) {
             const float lw = len * w;
@@ -924,6 +978,8 @@ extern "C" int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas
     int rc = siddon_common_checks(g, first, n_meas);
     if (rc) return rc;
     OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "trace lists are produced by projector_type 1 and 2");
+    OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1,
+                "trace lists describe single-ray LORs (this geometry has several rays per LOR)");
     OMG_REQUIRE(counts != nullptr, "null counts");
     OMG_REQUIRE(offsets == nullptr || (idx && len), "offsets given without idx/len outputs");
     if (n_meas == 0) return OMG_OK;

@@ -39,11 +39,15 @@ struct SidArgs {
     uint32_t tof_bins;               // time-of-flight bins (<= 1: no TOF)
     float tof_sigma, tof_pf, tof_epps;  // TOF standard deviation (mm), 0.3989422804014327f / sigma, weight-sum floor
     const float* __restrict__ tof_c; // TOF bin centres (mm), tof_bins floats
+    uint32_t rays_xy, rays_z;        // rays per LOR, transaxial x axial (1 x 1: one ray)
+    float cr_xy, cr_z;               // the pitches the rays' shifts are spread over (paramStruct dPitchXY, dPitchZ)
     uint32_t pitch;                // row pitch of slice-relative (planar) voxel indices: Nx, or Nx + 1 in LDS
     const uint32_t* __restrict__ index;  // PET sinogram: sinogram bin per measurement (subset types 1-5), or null;
                                          // plane_offset is then the call's first measurement in this table
     const float* __restrict__ add;  // additive term of the fused ratio (randoms / scatter), call-local, or null
 };
+
+__device__ __forceinline__ uint32_t n_rays(const SidArgs& a) { return a.rays_xy * a.rays_z; }
 
 // Denominator of the fused EM ratio: (A x + epps) + additive (computeForwardProject.h:20-21, 66-69).
 __device__ __forceinline__ float ratio_den(const SidArgs& a, int64_t lo, float yv, float epps) {
@@ -60,7 +64,41 @@ __device__ __forceinline__ uint32_t to_u32(float v) { return (uint32_t)(int64_t)
 __device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }
 
-__device__ __forceinline__ void fetch_ray(const SidArgs& a, int64_t lo, Ray& r) {
+// Ray `ray` (axial index ray / rays_xy outer, transaxial index inner) of a multi-ray LOR: both
+// endpoints move by (cr / (2n) - cr / 2) + (cr / n) * index, the transaxial shift on x and y alike
+// (multirayCoordinateShiftZ / XY, projector_functions.h:439-461, applied Z first).
+__device__ __forceinline__ void ray_shift(const SidArgs& a, int ray, Ray& r) {
+    const int lorZ = ray / (int)a.rays_xy, lorXY = ray - lorZ * (int)a.rays_xy;
+    if (a.rays_z > 1) {
+        float interval = a.cr_z / (float)(a.rays_z * 2);
+        r.zs += (interval - a.cr_z / 2.f);
+        r.zd += (interval - a.cr_z / 2.f);
+        interval *= 2.f;
+        r.zs += interval * (float)lorZ;
+        r.zd += interval * (float)lorZ;
+    }
+    if (a.rays_xy > 1) {
+        float interval = a.cr_xy / (float)(a.rays_xy * 2);
+        r.xs += (interval - a.cr_xy / 2.f);
+        r.xd += (interval - a.cr_xy / 2.f);
+        r.ys += (interval - a.cr_xy / 2.f);
+        r.yd += (interval - a.cr_xy / 2.f);
+        interval *= 2.f;
+        r.xs += interval * (float)lorXY;
+        r.xd += interval * (float)lorXY;
+        r.ys += interval * (float)lorXY;
+        r.yd += interval * (float)lorXY;
+    }
+}
+
+__device__ __forceinline__ void fetch_ray_base(const SidArgs& a, int64_t lo, Ray& r);
+
+__device__ __forceinline__ void fetch_ray(const SidArgs& a, int64_t lo, Ray& r, int ray = 0) {
+    fetch_ray_base(a, lo, r);
+    if (n_rays(a) > 1) ray_shift(a, ray, r);
+}
+
+__device__ __forceinline__ void fetch_ray_base(const SidArgs& a, int64_t lo, Ray& r) {
     if (a.mode == MODE_LIST6) {
         const float* p = a.x + (a.plane_offset + lo) * 6;
         r.xs = p[0]; r.ys = p[1]; r.zs = p[2]; r.xd = p[3]; r.yd = p[4]; r.zd = p[5];
@@ -197,7 +235,8 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
 // factor comes after norm, as there (:1209-1210).
 __device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct, bool perp = false) {
     if (ct) return 1.f;
-    float t = 1.f / L;
+    // 1 / (L * nRays) with several rays per LOR (:1819-1821, :1189-1190)
+    float t = n_rays(g) > 1 ? 1.f / (L * (float)n_rays(g)) : 1.f / L;
     const bool late = perp && g.atten_lor;
     if (g.atten && !late) t *= g.atten[lo];
     if (g.norm) t *= g.norm[lo];
@@ -242,9 +281,9 @@ struct Walk {
 // cache lines (w.trans is set; the caller supplies the transposed image / accumulator).
 template <bool PLANAR>
 __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk& w, float& weight,
-                                             bool allow_trans = false) {
+                                             bool allow_trans = false, int ray = 0) {
     Ray r;
-    fetch_ray(g, lo, r);
+    fetch_ray(g, lo, r, ray);
     const float ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
     if ((ydf == 0.f && xdf == 0.f) || isinf(ydf) || isinf(xdf)) return false;
     const bool ct = g.mode == MODE_CT_FLAT;
@@ -485,9 +524,9 @@ __device__ __forceinline__ void perp_walk(const Walk& w, Visit&& visit) {
 // Trace LOR `lo`: visit(voxel, length) for each voxel with length > 0 in traversal order.
 // Returns false when the LOR contributes nothing; `weight` is the LOR weight.
 template <bool PLANAR = false, typename Visit>
-__device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+__device__ __forceinline__ bool siddon_trace(const SidArgs& g, int64_t lo, float& weight, Visit&& visit, int ray = 0) {
     Walk w;
-    if (!siddon_setup<PLANAR>(g, lo, w, weight)) return false;
+    if (!siddon_setup<PLANAR>(g, lo, w, weight, false, ray)) return false;
     if (w.perp) {
         perp_walk(w, visit);
         return true;
@@ -508,10 +547,10 @@ constexpr int kBatch = 8;
 
 template <bool PLANAR, bool RING = false>
 __device__ __forceinline__ bool siddon_gather(const SidArgs& g, int64_t lo, const float* __restrict__ src, float& ax,
-                                              float& weight) {
+                                              float& weight, int ray = 0) {
     ax = 0.f;
     Walk w;
-    if (!siddon_setup<PLANAR>(g, lo, w, weight)) return false;
+    if (!siddon_setup<PLANAR>(g, lo, w, weight, false, ray)) return false;
     if (w.perp) {
         perp_walk(w, [&](uint32_t ind, float len) { ax += len * src[ind]; });
         return true;
@@ -555,9 +594,10 @@ __device__ __forceinline__ bool siddon_scatter(const SidArgs& g, int64_t lo, flo
 // visit(ind, len, trans) for each voxel with len > 0, ind in the layout named by trans (see
 // siddon_setup's allow_trans). Returns false when the LOR contributes nothing.
 template <typename Visit>
-__device__ __forceinline__ bool siddon_trace_split(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+__device__ __forceinline__ bool siddon_trace_split(const SidArgs& g, int64_t lo, float& weight, Visit&& visit,
+                                                   int ray = 0) {
     Walk w;
-    if (!siddon_setup<false>(g, lo, w, weight, true)) return false;
+    if (!siddon_setup<false>(g, lo, w, weight, true, ray)) return false;
     if (w.perp) {
         perp_walk(w, [&](uint32_t ind, float len) { visit(ind, len, false); });
         return true;

@@ -95,6 +95,11 @@ class projectorClass:
         # multiplicative per-measurement correction (proj.py:88,295 -> -DSCATTER)
         self.additionalCorrection = False
         self.corrVector = np.empty(0, dtype=np.float32)
+        # rays per LOR (proj.py:263-264); their spread is dPitchX (transaxial) / dPitchY (axial), which the
+        # reference sets to cr_pz / cr_p for PET data (proj.py:937-939)
+        self.n_rays_transaxial = 1
+        self.n_rays_axial = 1
+        self.cr_p = 0.0
         # time of flight (proj.py:81,166-168,332-333,455-456, 681-705)
         self.TOF_bins = 1
         self.TOF_bins_used = 1
@@ -357,6 +362,12 @@ class projectorClass:
                 np.ascontiguousarray(self.normalization, dtype=np.float32).ravel()
             keep.append(nrm)
             d.norm, d.norm_len = nrm.ctypes.data, nrm.size
+        nr = (max(int(self.n_rays_transaxial), 1), max(int(self.n_rays_axial), 1))
+        if nr != (1, 1) and not self.CT:
+            if pt != 1:
+                raise ValueError("Several rays per LOR are supported with improved Siddon (projector_type = 1)")
+            d.rays_xy, d.rays_z = nr
+            d.cr_xy, d.cr_z = float(self.cr_pz), float(self.cr_p)
         if self._tof() and pt != 1:
             raise ValueError("TOF is currently only supported with improved Siddon (projector_type = 1)")
         if self._tof():

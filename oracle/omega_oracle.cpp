@@ -60,6 +60,8 @@ struct Geo {
     T tof_sigma;               // TOF standard deviation (mm), paramStruct::sigma_x
     const T* tof_c;            // TOF bin centres (mm), paramStruct::TOFCenters
     T tof_epps;                // floor of a voxel's weight sum, paramStruct::epps
+    uint32_t rays_xy, rays_z;  // rays per LOR (paramStruct nRays2D, nRays3D)
+    T cr_xy, cr_z;             // their spread (paramStruct dPitchXY, dPitchZ)
 };
 
 // ---- time of flight (projector_functions.h:559-594) ----
@@ -105,7 +107,38 @@ template <typename T>
 struct Ray { T xs, ys, zs, xd, yd, zd; };
 
 template <typename T>
-static inline void fetch_ray(const Geo<T>& g, int64_t lo, const T* x, const T* z, Ray<T>& r) {
+static inline void fetch_ray_base(const Geo<T>& g, int64_t lo, const T* x, const T* z, Ray<T>& r);
+
+// multirayCoordinateShiftZ, then XY (projector_functions.h:376-380, 439-461); ray = lorZ * rays_xy + lorXY
+template <typename T>
+static inline void fetch_ray(const Geo<T>& g, int64_t lo, const T* x, const T* z, Ray<T>& r, int ray = 0) {
+    fetch_ray_base(g, lo, x, z, r);
+    if (g.rays_xy * g.rays_z <= 1) return;
+    const int lorZ = ray / (int)g.rays_xy, lorXY = ray % (int)g.rays_xy;
+    if (g.rays_z > 1) {
+        T interval = g.cr_z / (static_cast<T>(g.rays_z * 2));
+        r.zs += (interval - g.cr_z / (T)2.);
+        r.zd += (interval - g.cr_z / (T)2.);
+        interval *= (T)2.;
+        r.zs += interval * lorZ;
+        r.zd += interval * lorZ;
+    }
+    if (g.rays_xy > 1) {
+        T interval = g.cr_xy / (static_cast<T>(g.rays_xy * 2));
+        r.xs += (interval - g.cr_xy / (T)2.);
+        r.xd += (interval - g.cr_xy / (T)2.);
+        r.ys += (interval - g.cr_xy / (T)2.);
+        r.yd += (interval - g.cr_xy / (T)2.);
+        interval *= (T)2.;
+        r.xs += interval * lorXY;
+        r.xd += interval * lorXY;
+        r.ys += interval * lorXY;
+        r.yd += interval * lorXY;
+    }
+}
+
+template <typename T>
+static inline void fetch_ray_base(const Geo<T>& g, int64_t lo, const T* x, const T* z, Ray<T>& r) {
     if (g.mode == LIST6) {
         r.xs = x[lo * 6]; r.ys = x[lo * 6 + 1]; r.zs = x[lo * 6 + 2];
         r.xd = x[lo * 6 + 3]; r.yd = x[lo * 6 + 4]; r.zd = x[lo * 6 + 5];
@@ -254,10 +287,11 @@ static inline int perp_start(T b, T d, T dd, uint32_t N) {
 template <typename T, typename F>
 // `atn` (optional) receives the attenuation factor exp(jelppi) of atten_mode 1 (1 when the LOR is skipped).
 static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w, F&& visit, int fp = 1,
-                       T* atn = nullptr) {
+                       T* atn = nullptr, int ray = 0) {
     if (atn) *atn = (T)1.;
+    const T nrays = (T)(g.rays_xy * g.rays_z);
     Ray<T> r;
-    fetch_ray(g, lo, x, z, r);
+    fetch_ray(g, lo, x, z, r, ray);
     T ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
     if ((ydf == (T)0. && xdf == (T)0. && zdf == (T)0.) || (ydf == (T)0. && xdf == (T)0.) || std::isinf(ydf) || std::isinf(xdf))
         return false;
@@ -326,7 +360,7 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         }
         uint32_t ind = apu * N + (uint32_t)tk * Nrow * Nrun;
         if (!ct) {
-            T t = (T)1. / L;
+            T t = nrays > (T)1. ? (T)1. / (L * nrays) : (T)1. / L;   // :1187-1190
             if (g.atten_mode == 1) {
                 // the row's voxels with the image's own strides: x = apu (column) or y = apu (row)
                 T jelppi = (T)0.;
@@ -409,7 +443,7 @@ static bool siddon_lor(const Geo<T>& g, int64_t lo, const T* x, const T* z, T& w
         }
     }
     if (!ct) {
-        T t = (T)1. / (L * (T)1);
+        T t = (T)1. / (L * nrays);   // :1819-1821
         if (g.atten_mode == 1 && fp == 2) {
             t *= std::exp(jelppi);
             if (atn) *atn = std::exp(jelppi);
@@ -481,6 +515,17 @@ static void siddon_fp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
             for (uint32_t to = 0; to < g.tof_bins; to++) y[lo + (int64_t)to * nMeas] = hit ? ax[to] * w : (T)0.;
             continue;
         }
+        if (g.rays_xy * g.rays_z > 1) {
+            // per ray ax[lor] *= temp, then the rays' values summed in order (projector_functions.h:2036-2056)
+            T total = (T)0.;
+            for (int ray = 0; ray < (int)(g.rays_xy * g.rays_z); ray++) {
+                T ax = (T)0.;
+                if (siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len) { ax += len * img[ind]; }, 1, (T*)nullptr, ray))
+                    total += ax * w;
+            }
+            y[lo] = total;
+            continue;
+        }
         T ax = (T)0.;
         const bool hit = siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len) { ax += len * img[ind]; });
         y[lo] = hit ? ((g.mode == CT_FLAT) ? ax : ax * w) : (T)0.;
@@ -517,16 +562,18 @@ static void siddon_bp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
             continue;
         }
         const T yi = yin[lo];
-        siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len) {
-            const T lw = len * w;
-            const T c = yi * lw;
+        for (int ray = 0; ray < (int)(g.rays_xy * g.rays_z); ray++) {
+            siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len) {
+                const T lw = len * w;
+                const T c = yi * lw;
 #pragma omp atomic
-            rhs[ind] += c;
-            if (sens) {
+                rhs[ind] += c;
+                if (sens) {
 #pragma omp atomic
-                sens[ind] += lw;
-            }
-        }, 2);
+                    sens[ind] += lw;
+                }
+            }, 2, (T*)nullptr, ray);
+        }
     }
 }
 
@@ -571,7 +618,8 @@ template <typename T>
 static Geo<T> make_geo(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x, uint32_t size_y, int32_t mode,
                        int64_t plane_offset, T dPitchZ, int32_t ct_pitch, T global_factor, const T* norm,
                        const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins = 0, T tof_sigma = (T)0.,
-                       const T* tof_c = nullptr, T tof_epps = (T)0.) {
+                       const T* tof_c = nullptr, T tof_epps = (T)0., uint32_t rays_xy = 1, uint32_t rays_z = 1,
+                       T cr_xy = (T)0., T cr_z = (T)0.) {
     Geo<T> g;
     g.Nx = n3[0]; g.Ny = n3[1]; g.Nz = n3[2];
     g.dx = d3[0]; g.dy = d3[1]; g.dz = d3[2];
@@ -584,32 +632,36 @@ static Geo<T> make_geo(const uint32_t* n3, const T* d3, const T* b3, uint32_t si
     g.tof_sigma = tof_sigma;
     g.tof_c = tof_c;
     g.tof_epps = tof_epps;
+    g.rays_xy = rays_xy > 1 ? rays_xy : 1u;
+    g.rays_z = rays_z > 1 ? rays_z : 1u;
+    g.cr_xy = cr_xy;
+    g.cr_z = cr_z;
     return g;
 }
 
 #define ORACLE_SIDDON_API(T, SUF)                                                                                     \
     extern "C" int oracle_siddon_fp_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,             \
                                           uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,            \
-                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, int64_t nMeas,           \
+                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, uint32_t rays_xy, uint32_t rays_z, T cr_xy, T cr_z, int64_t nMeas,           \
                                           const T* x, const T* z, const T* img, T* y, int nthreads) {                \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps, rays_xy, rays_z, cr_xy, cr_z); \
         siddon_fp(g, nMeas, x, z, img, y, nthreads);                                                                   \
         return 0;                                                                                                      \
     }                                                                                                                  \
     extern "C" int oracle_siddon_bp_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,             \
                                           uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,            \
-                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, int64_t nMeas,           \
+                                          int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, uint32_t rays_xy, uint32_t rays_z, T cr_xy, T cr_z, int64_t nMeas,           \
                                           const T* x, const T* z, const T* yin, T* rhs, T* sens, int nthreads) {      \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps, rays_xy, rays_z, cr_xy, cr_z); \
         siddon_bp(g, nMeas, x, z, yin, rhs, sens, nthreads);                                                          \
         return 0;                                                                                                      \
     }                                                                                                                  \
     extern "C" int64_t oracle_siddon_trace_##SUF(const uint32_t* n3, const T* d3, const T* b3, uint32_t size_x,      \
                                                  uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,     \
-                                                 int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, int64_t nMeas,    \
+                                                 int32_t ct_pitch, T global_factor, const T* norm, const T* atten, int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma, const T* tof_c, T tof_epps, uint32_t rays_xy, uint32_t rays_z, T cr_xy, T cr_z, int64_t nMeas,    \
                                                  const T* x, const T* z, int64_t* counts, int64_t cap,               \
                                                  uint32_t* idx_out, T* len_out, T* w_out) {                          \
-        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps); \
+        Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps, rays_xy, rays_z, cr_xy, cr_z); \
         int64_t total = 0;                                                                                             \
         for (int64_t lo = 0; lo < nMeas; lo++) {                                                                       \
             int64_t c = 0;                                                                                             \
@@ -639,10 +691,10 @@ ORACLE_SIDDON_API(double, f64)
                                              uint32_t size_y, int32_t mode, int64_t plane_offset, T dPitchZ,         \
                                              int32_t ct_pitch, T global_factor, const T* norm, const T* atten,       \
                                              int32_t atten_mode, const T* scat, uint32_t tof_bins, T tof_sigma,        \
-                                             const T* tof_c, T tof_epps, int64_t nMeas, const T* x, const T* z, T* out,      \
+                                             const T* tof_c, T tof_epps, uint32_t rays_xy, uint32_t rays_z, T cr_xy, T cr_z, int64_t nMeas, const T* x, const T* z, T* out,      \
                                              int nthreads) {                                                         \
         Geo<T> g = make_geo<T>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, \
-                               atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps);                         \
+                               atten, atten_mode, scat, tof_bins, tof_sigma, tof_c, tof_epps, rays_xy, rays_z, cr_xy, cr_z);                         \
         set_threads(nthreads);                                                                                         \
         _Pragma("omp parallel for schedule(dynamic, 1000)")                                                            \
         for (int64_t lo = 0; lo < nMeas; lo++) {                                                                       \

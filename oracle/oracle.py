@@ -46,7 +46,8 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         for T, suf, p in ((C.c_float, "f32", _f32p), (C.c_double, "f64", _f64p)):
             geo = [C.c_void_p, p, p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, T, C.c_int32, T, C.c_void_p,
-                   C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, T, C.c_void_p, T, C.c_int64]
+                   C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, T, C.c_void_p, T, C.c_uint32, C.c_uint32, T, T,
+                   C.c_int64]
             f = getattr(_lib, "oracle_siddon_fp_" + suf)
             f.argtypes = geo + [p, p, p, p, C.c_int]
             f.restype = C.c_int
@@ -90,7 +91,8 @@ def ref():
             f = getattr(_ref, "ref_siddon_" + suf)
             f.argtypes = [C.c_void_p, p, p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, T, C.c_int32, T,
                           C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, T, C.c_void_p, T,
-                          C.c_uint32, C.c_int64, p, p, p, p, C.c_void_p, C.c_int, C.c_int]
+                          C.c_uint32, C.c_uint32, T, T, C.c_uint32, C.c_int64, p, p, p, p, C.c_void_p, C.c_int,
+                          C.c_int]
             f.restype = C.c_int
         _ref.ref_rdp_f32.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
     return _ref
@@ -101,11 +103,12 @@ class SiddonCase:
 
     def __init__(self, grid, size_x, size_y, x, z, mode=MODE_PET, plane_offset=0, dPitchZ=0.0, ct_pitch=0,
                  global_factor=1.0, norm=None, n_proj=None, dtype=np.float32, atten=None, atten_mode=0, scat=None,
-                 tof_bins=0, tof_sigma=0.0, tof_centers=None, tof_epps=1e-5):
+                 tof_bins=0, tof_sigma=0.0, tof_centers=None, tof_epps=1e-5, rays=(1, 1), ray_pitch=(0.0, 0.0)):
         """atten: attenuation image mu (atten_mode 1, N values, 1/mm, x fastest) or one factor per
         measurement (atten_mode 2, norm's order); scat: per-measurement multiplicative correction;
         tof_bins > 1: time of flight with standard deviation tof_sigma (mm) and bin centres tof_centers
-        (mm); measurement vectors then hold tof_bins blocks of nmeas values."""
+        (mm); measurement vectors then hold tof_bins blocks of nmeas values; rays = (transaxial, axial)
+        rays per LOR spread over ray_pitch (mm)."""
         self.dtype = np.dtype(dtype)
         self.n3 = np.ascontiguousarray(grid.n3())
         self.d3 = grid.d3(self.dtype)
@@ -128,6 +131,8 @@ class SiddonCase:
         self.tof_sigma = tof_sigma
         self.tof_centers = None if tof_centers is None else np.ascontiguousarray(tof_centers, dtype=self.dtype)
         self.tof_epps = tof_epps
+        self.rays = (int(rays[0]), int(rays[1]))
+        self.ray_pitch = (float(ray_pitch[0]), float(ray_pitch[1]))
 
     @property
     def suf(self):
@@ -140,7 +145,8 @@ class SiddonCase:
         tcp = None if self.tof_centers is None else self.tof_centers.ctypes.data
         return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
                 self.dPitchZ, self.ct_pitch, self.global_factor, normp, atnp, self.atten_mode, scp,
-                self.tof_bins, self.tof_sigma, tcp, self.tof_epps, nmeas]
+                self.tof_bins, self.tof_sigma, tcp, self.tof_epps, self.rays[0], self.rays[1], self.ray_pitch[0],
+                self.ray_pitch[1], nmeas]
 
     # ---- oracle -------------------------------------------------------------------------
     def fp(self, img, nmeas, nthreads=0):
@@ -280,7 +286,7 @@ def _update64(x, S, dU, rhs, epps, positivity):
 
 def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter, x0, epps=1e-5, beta=0.0,
          gamma=2.0, use_ref=False, nthreads=0, positivity=True, subsets_to_run=None, dtype=np.float32,
-         additive=None, atten=None, atten_mode=0, tof=None):
+         additive=None, atten=None, atten_mode=0, tof=None, rays=(1, 1), ray_pitch=(0.0, 0.0)):
     """OSEM / OSL-OSEM(RDP) on the CPU in the reference's step order (see omega_amd/recon.py).
 
     ``meas`` is the full measurement vector in subset order. With ``use_ref`` the forward and
@@ -291,7 +297,8 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
     of the ratio. ``atten``: attenuation image (``atten_mode`` 1) or per-measurement factors laid out
     like ``meas`` (``atten_mode`` 2). ``tof``: optional dict of SiddonCase's tof_* arguments; each
     subset's measurements are then tof_bins blocks of its LORs and ``additive`` (one value per LOR)
-    is added to every bin. Returns the image (float32, or float64 for the FP64 re-run).
+    is added to every bin. ``rays`` / ``ray_pitch``: rays per LOR and their spread (SiddonCase). Returns the
+    image (float32, or float64 for the FP64 re-run).
     """
     tof = dict(tof or {})
     nb = int(tof.get("tof_bins", 1)) if int(tof.get("tof_bins", 1)) > 1 else 1
@@ -312,7 +319,7 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
             if atten is not None and atten_mode == 2:
                 atn = atten[starts[k] * per: starts[k] * per + n]
             case = SiddonCase(grid, size_x, size_y, xy, z_subset_ordered, plane_offset=int(starts[k]), dtype=dtype,
-                              atten=atn, atten_mode=atten_mode, **tof)
+                              atten=atn, atten_mode=atten_mode, rays=rays, ray_pitch=ray_pitch, **tof)
             yk = meas[nb * starts[k] * per: nb * (starts[k] * per + n)]
             yhat = case.ref_fp(x, n, nthreads) if use_ref else case.fp(x, n, nthreads)
             if additive is not None:

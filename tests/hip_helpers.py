@@ -11,11 +11,12 @@ from omega_amd.projector import Geometry
 
 def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
               ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_flags=0, dPitchX=0.0, dPitchY=0.0,
-              generic=False, x_fastest_only=False, atten=None, atten_mode=0, scat=None, tof=None):
+              generic=False, x_fastest_only=False, atten=None, atten_mode=0, scat=None, tof=None, rays=None):
     """Geometry handle through the C ABI; ``generic`` routes every call through the one-lane-per-LOR
     kernels (omg_geom_desc.reserved[0]); ``x_fastest_only`` keeps those kernels in the x-fastest image
     (reserved[3]); ``atten`` is an attenuation image (``atten_mode`` 1) or per-measurement factors (2);
-    ``scat`` a per-measurement multiplicative correction; ``tof`` = (bins, sigma_mm, centres_mm, epps)."""
+    ``scat`` a per-measurement multiplicative correction; ``tof`` = (bins, sigma_mm, centres_mm, epps);
+    ``rays`` = (transaxial, axial, pitch_xy, pitch_z)."""
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
     d = _lib.GeomDesc()
@@ -48,6 +49,8 @@ def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, 
         scat = np.ascontiguousarray(scat, dtype=np.float32).ravel()
         keep.append(scat)
         d.scat, d.scat_len = scat.ctypes.data, scat.size
+    if rays is not None:
+        d.rays_xy, d.rays_z, d.cr_xy, d.cr_z = int(rays[0]), int(rays[1]), float(rays[2]), float(rays[3])
     if tof is not None:
         bins, sigma, centres, epps = tof
         centres = np.ascontiguousarray(centres, dtype=np.float32)
