@@ -501,6 +501,9 @@ def bench_autograd(args):
         worst = max(worst, abs(lhs - rhs) / abs(lhs))
     _progress(f"autograd: adjoint worst {worst:.2e}")
     x = torch.as_tensor(ph, device=dev).clone().requires_grad_(True)
+    for _ in range(args.warmup):   # untimed passes (allocations, first launches)
+        x.grad = None
+        AG.penalised_nll(x, p, meas, gamma=2.0, beta=0.1).backward()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     reps = max(1, args.steps)
