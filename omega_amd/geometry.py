@@ -151,6 +151,57 @@ def subset_index_type8(n_proj: int, subsets: int):
     return np.concatenate(idx), np.array([len(i) for i in idx], dtype=np.int64)
 
 
+def _prime_factors(n: int):
+    f, d = [], 2
+    while n > 1:
+        while n % d == 0:
+            f.append(d)
+            n //= d
+        d += 1
+    return f
+
+
+def subset_index_projections(subset_type: int, n_proj: int, subsets: int, seed: int = -1):
+    """Projection-level subsets (indices.py:158-240): a permutation of the planes / projections cut into
+    ``subsets`` consecutive runs, the first n_proj % subsets of them one longer.
+
+    * 8: every ``subsets``-th projection (interleaved, not cut into runs);
+    * 9: a random permutation (numpy ``default_rng(seed).permutation``; unseeded when seed < 0);
+    * 11: the prime-factor ordering: with n_proj = f1 f2 ... fN (prime factors in increasing order),
+      projection r (digits d1 .. dN of r in that mixed radix, d1 fastest) goes to position
+      d1 f2..fN + d2 f3..fN + ... + dN, which spreads consecutive positions over the whole range.
+
+    Returns (index, nMeas): the projections in subset order and the number of projections per subset.
+    """
+    if subsets <= 1:
+        return np.arange(n_proj, dtype=np.int64), np.array([n_proj], dtype=np.int64)
+    if subset_type == 8:
+        idx = [np.arange(k, n_proj, subsets, dtype=np.int64) for k in range(subsets)]
+        return np.concatenate(idx), np.array([len(i) for i in idx], dtype=np.int64)
+    if subset_type == 9:
+        rng = np.random.default_rng() if seed < 0 else np.random.default_rng(seed)
+        order = rng.permutation(n_proj).astype(np.int64)
+    elif subset_type == 11:
+        f = _prime_factors(n_proj)
+        if not f:
+            raise ValueError("subset type 11 needs at least two projections")
+        weights = np.cumprod(np.asarray(f[1:][::-1], dtype=np.int64))[::-1]   # f(k+1) ... fN for k < N
+        digits = np.zeros((len(f), n_proj), dtype=np.int64)
+        r = np.arange(n_proj, dtype=np.int64)
+        step = 1
+        for k, fk in enumerate(f):
+            digits[k] = (r // step) % fk
+            step *= fk
+        order = (digits[: len(weights)] * weights[:, None]).sum(0) + digits[-1]
+    elif subset_type == 10:
+        raise ValueError("Subset type 10 (golden angle) is not supported in the Python version of the reference either")
+    else:
+        raise ValueError(f"subset type {subset_type} is not a projection-level subset type (8, 9, 11)")
+    base, extra = divmod(n_proj, subsets)
+    sizes = np.array([base + (1 if k < extra else 0) for k in range(subsets)], dtype=np.int64)
+    return order, sizes
+
+
 def subset_index_measurements(subset_type: int, subsets: int, n_dist: int, n_ang: int, n_sinos: int,
                               seed: int = -1):
     """Measurement-level subsets of a sinogram (indexMaker, indices.py:51-123).

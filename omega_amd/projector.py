@@ -156,10 +156,12 @@ class projectorClass:
         if self.projector_type in (5, 55) and not self.CT:
             raise ValueError("Projector type 5 is only supported with CT data!")
         if self.subsets > 1 and not self.listmode:
-            if self.CT and self.subsetType != 8:
-                raise ValueError("Only subset type 8 (every n-th projection) is supported for CT data!")
-            if self.subsetType not in (1, 2, 3, 4, 5, 8):
-                raise ValueError("Subset types 1-5 and 8 are supported for sinogram data!")
+            if self.subsetType == 10:
+                raise ValueError("Not supported in Python version!")
+            if self.CT and self.subsetType not in (8, 9, 11):
+                raise ValueError("Projection-level subset types 8, 9 and 11 are supported for CT data!")
+            if self.subsetType not in (1, 2, 3, 4, 5, 8, 9, 11):
+                raise ValueError("Subset types 1-5, 8, 9 and 11 are supported for sinogram data!")
         if self.subsets < 1:
             raise ValueError("Number of subsets must be at least 1!")
         if self.FOVa_x <= 0 or self.axial_fov <= 0:
@@ -245,7 +247,9 @@ class projectorClass:
                                                                  int(self.nColsD), int(self.NSinos), self.seed)
             kerroin = 1
         else:
-            self.index, per_subset = G.subset_index_type8(self.nProjections, self.subsets if self.subsets > 1 else 1)
+            st = self.subsetType if self.subsetType in (8, 9, 11) else 8
+            self.index, per_subset = G.subset_index_projections(st, int(self.nProjections),
+                                                                self.subsets if self.subsets > 1 else 1, int(self.seed))
             if self.subsets > 1:
                 if self.CT:
                     self.x = np.ascontiguousarray(self.x.reshape(-1, 6)[self.index].reshape(-1))

@@ -5,7 +5,9 @@ Nothing else pins these conventions (the reference's tests hold no vectors for t
 imports that one file by path -- it needs numpy only -- and records, for small sinograms
 (Ndist x Nang x NSinos) and several subset counts, the concatenated index vector and the
 per-subset lengths of subset types 1, 2, 3 (seeded), 4 and 5 (index into the sinogram, radial bin
-fastest, then angle, then plane), and of type 8 (plane numbers). Where the reference raises (subset
+fastest, then angle, then plane), of type 8 (plane numbers), and, for several projection counts
+(primes and composites), of the projection-level types 9 (seeded permutation) and 11 (prime-factor
+order). Where the reference raises (subset
 type 4 with Nang % subsets != 0 fails in its remainder branch, indices.py:58-66, with a numpy
 broadcasting error), the case is recorded as an error. Runs only where the reference sources are
 present:
@@ -29,6 +31,8 @@ CASES = [
     (5, 9, 7, 2),
 ]
 TYPES = [1, 2, 3, 4, 5, 8]
+# projection-level types 9 (seeded permutation) and 11 (prime-factor order): (nProjections, subsets)
+PROJ_CASES = [(12, 3), (18, 4), (30, 7), (7, 2), (20, 6), (63, 8)]
 
 
 def main():
@@ -47,6 +51,15 @@ def main():
             except ValueError as e:
                 out[key + "_error"] = np.array(str(e))
                 continue
+            out[key + "_index"] = np.asarray(o.index).astype(np.int64).ravel()
+            out[key + "_nmeas"] = np.asarray(o.nMeas).astype(np.int64).ravel()
+    for (npj, sub) in PROJ_CASES:
+        for st in (9, 11):
+            o = types.SimpleNamespace(subsets=sub, sampling=1, Ndist=4, Nang=3, NSinos=npj, CT=False,
+                                      subsetType=st, use_raw_data=0, listmode=0, Nt=1, seed=11,
+                                      nProjections=npj, PET=True, SPECT=False)
+            key = f"p{st}_{npj}_s{sub}"
+            mod.indexMaker(o)
             out[key + "_index"] = np.asarray(o.index).astype(np.int64).ravel()
             out[key + "_nmeas"] = np.asarray(o.nMeas).astype(np.int64).ravel()
     path = os.path.join(HERE, "subset_indices.npz")
