@@ -313,3 +313,157 @@ def pet_3d_rings(radius: float = 400.0, n_rings: int = 32, ring_pitch: float = 4
     n_ang = crystals // 2
     pitch = math.pi * radius / crystals
     return PETScanner(radius, n_ang, n_ang, pitch, ring_pair_planes_z(n_rings, ring_pitch, max_ring_diff))
+
+
+# ---- OMEGA's own PET scanner geometry (source/Python/omegatomo/projector/detcoord.py) ----------------
+# The reference builds a block-ring scanner's crystal coordinates, the sinogram's LOR endpoints and the
+# axial (ring-pair) coordinates procedurally; the tables here are float-for-float the same (checked
+# against the reference's own output), so data binned by OMEGA can be projected directly.
+
+def omega_crystal_ring(blocks_per_ring: int, cryst_per_block: int, cr_p: float, diameter: float,
+                       transaxial_multip: int = 1, doi: float = 0.0, flip_image: bool = False,
+                       offangle: float = 0.0):
+    """Transaxial crystal centres (x, y), float32, of a ring of flat blocks without pseudo detectors
+    (detectorCoordinates / computeCoordinates, detcoord.py:326-556): blocks at angles 90, 90 - 360/B, ...
+    degrees; crystals of pitch cr_p along each block; the gap between blocks chosen so that the ring
+    spans the diameter (+ 2 DOI). The first quarter of the ring is stepped crystal by crystal (each
+    position stored as float32 and the next one stepped from it, as there); the rest follows by the
+    ring's mirror symmetries."""
+    B = int(blocks_per_ring) * int(transaxial_multip)
+    c = int(cryst_per_block)
+    d = float(diameter) + float(doi) * 2
+    deg = np.arange(90, -271, -360 / B)
+    h = B // 2
+    w_x = c * cr_p * np.cos(np.radians(deg[:h + 1]))
+    w_y = c * cr_p * np.sin(np.radians(deg[:h]))
+    gx = (d - np.sum(np.abs(w_x))) / np.sum(np.cos(np.radians(deg[:h + 1]))) / 2
+    gy = (d - np.sum(np.abs(w_y))) / np.sum(np.abs(np.sin(np.radians(deg[:h])))) / 2
+    cs = lambda k: np.cos(np.radians(deg[k]))  # noqa: E731  (scalar calls, as the reference makes them)
+    sn = lambda k: np.sin(np.radians(deg[k]))  # noqa: E731
+    x = np.zeros(B * c, dtype=np.float32)
+    y = np.zeros(B * c, dtype=np.float32)
+    px, py = d / 2., -(c / 2. + 0.5) * cr_p
+
+    def block_start(k, px, py):
+        # first crystal of block k: half a pitch, the gaps on both sides of the block edge, half a pitch
+        return (px - (cr_p * 0.5) * cs(k) - gx * cs(k - 1) - gx * cs(k) - (cr_p * 0.5) * cs(k - 1),
+                py + (cr_p * 0.5) * sn(k) + gy * sn(k - 1) + gy * sn(k) + (cr_p * 0.5) * sn(k - 1))
+
+    n = 0
+    quarter = -(-B // 4)
+    for k in range(quarter):
+        for m in range(1, c + 1):
+            if k > 0 and m == 1:
+                x[n], y[n] = block_start(k, px, py)
+            else:
+                x[n], y[n] = px - cr_p * cs(k), py + cr_p * sn(k)
+            px, py = x[n], y[n]
+            n += 1
+    n4 = (B * c) // 4
+    if B % 4 == 0:
+        k = quarter
+        x[n], y[n] = block_start(k, px, py)
+        px, py = x[n], y[n]
+        n += 1
+        for _ in range(2, c + 1):
+            x[n], y[n] = px - cr_p * cs(k), py + cr_p * sn(k)
+            px, py = x[n], y[n]
+            n += 1
+        m = n4
+    else:
+        m = n4 + c // 2
+    x[n:n + m] = -np.flip(x[:m])
+    y[n:n + m] = np.flip(y[:m])
+    x[n + m:] = np.flip(x[c:(B * c) // 2])
+    y[n + m:] = -y[c:(B * c) // 2]
+    if flip_image:
+        x, y = np.flip(x), np.flip(y)
+    if offangle != 0:
+        x, y = np.roll(x, int(np.round(offangle))), np.roll(y, int(np.round(offangle)))
+    return x, y
+
+
+def omega_sinogram_xy(x, y, n_dist: int, n_ang: int, ndist_side: int = 1) -> np.ndarray:
+    """Sinogram LOR endpoints [xs, ys, xd, yd] per (radial, angle) bin, radial bin fastest, from the
+    crystal coordinates of one ring (sinogramCoordinates2D, detcoord.py:589-701): every crystal pair
+    (a <= b) gets angle ((a + b + det/2 + 1) mod det) / 2 and signed radial offset |b - a - det/2|
+    (negative where the pair wraps), endpoints swapped at the sinogram's corners; the pairs within
+    n_dist radial bins are kept (ndist_side picks the extra bin of an even n_dist). Angular mashing
+    (n_ang < det / 2) is not provided."""
+    det = len(x)
+    if n_ang < det // 2:
+        raise ValueError("Angular mashing (Nang < det_w_pseudo / 2) is not supported by this implementation")
+    a, b = np.triu_indices(det)
+    L = np.stack([a, b], 1).astype(np.int64)
+    hi, lo = b.astype(np.int64), a.astype(np.int64)
+    j = ((hi + lo + det // 2 + 1) % det) // 2
+    i = np.abs(hi - lo - det // 2)
+    i = np.where((lo < j) | (j + det // 2 < hi), -i, i)
+    swap = (j * 2 < -i) | (i <= (j - det // 2) * 2)
+    L[swap] = L[swap][:, ::-1]
+    if n_dist % 2 == 0:
+        keep = (i <= n_dist // 2 + min(0, ndist_side)) & (i >= -n_dist // 2 + max(0, ndist_side))
+    else:
+        keep = (i <= n_dist // 2) & (i >= -(n_dist // 2))
+    j = j // (det // 2 // n_ang)
+    i, j, L = i[keep], j[keep], L[keep]
+    if i.min() < 0:
+        i = i + np.abs(i.min())
+    grids = [np.zeros((n_dist, n_ang), dtype=np.float32, order="F") for _ in range(4)]
+    for g, v in zip(grids, (x[L[:, 0]], y[L[:, 0]], x[L[:, 1]], y[L[:, 1]])):
+        np.add.at(g, (i, j), v)
+    return np.ascontiguousarray(np.stack([g.ravel("F") for g in grids], 1).astype(np.float32))
+
+
+def omega_segment_table(rings: int, span: int, ring_difference: int, image_nz: int) -> np.ndarray:
+    """Sinograms per segment of a span-compressed ring-pair set (projectorClass, proj.py:668-670): the
+    direct segment (2 rings - 1), then each oblique segment twice (+ and - ring difference). As in the
+    reference the bound uses the image's Nz."""
+    first = rings * 2 - 1
+    rest = np.arange(first - (span + 1), max(image_nz - ring_difference * 2, rings - ring_difference), -span * 2)
+    return np.insert(np.repeat(rest, 2), 0, first).astype(np.int64)
+
+
+def omega_sinogram_z(rings: int, cr_pz: float, span: int, ring_difference: int, image_nz: int) -> np.ndarray:
+    """Axial endpoints [zs, zd] per sinogram (sinogramCoordinates3D, detcoord.py:703-896, single layer, no
+    ring gaps). Span 1: every ring pair (a, b), a outer. Span > 1: ring z_a = (a + 1) cr_pz - (rings + 1)
+    cr_pz / 2; sinogram s holds the mean (z_a, z_b) of the ring pairs binned into it: the direct segment
+    holds |a - b| <= span / 2 at position a + b, oblique segment i the differences d in
+    [k_i, k_i + span - 1] (k_i = ceil(span / 2) + span i), b - a = d in the segment's first copy and a - b = d
+    in its second, at position (a + b) - k_i within the segment."""
+    maxz = float(rings + 1) * cr_pz
+    if span <= 1:
+        t = np.repeat(np.arange(rings), rings).astype(np.float64)
+        ring = np.arange(0, cr_pz * rings, cr_pz)
+        z = np.stack([cr_pz * t * 1.0, np.tile(ring, rings)], 1)
+        return z - (maxz / 2 - cr_pz)
+    zr = np.linspace(cr_pz, maxz + cr_pz, rings + 2)[:rings]
+    zr = zr - maxz / 2
+    seg = omega_segment_table(rings, span, ring_difference, image_nz)
+    off = np.cumsum(seg)
+    tot = int(off[-1])
+    z = np.zeros((tot, 2))
+    cnt = np.zeros(tot)
+    nz = rings * 2 - 1
+    z[:nz + 1:2, 0] = zr
+    z[:nz + 1:2, 1] = zr
+    cnt[:nz + 1:2] = 1
+    for e in range(1, span // 2 + 1):              # direct segment, |a - b| = e: both orders per position
+        lo = np.arange(rings - e)
+        p = 2 * lo + e
+        z[p, 0] += zr[lo + e] + zr[lo]
+        z[p, 1] += zr[lo] + zr[lo + e]
+        cnt[p] += 2
+    k = [int(np.ceil(span / 2)) + span * i for i in range((ring_difference - int(np.ceil(span / 2))) // span + 1)]
+    for si in range(len(seg) // 2):
+        for dd in range(k[si], k[si] + span):
+            lo = np.arange(rings - dd)
+            pos = lo * 2 + dd - k[si]
+            z[off[2 * si] + pos, 0] += zr[lo + dd]     # first copy: pairs (a = b + d, b)
+            z[off[2 * si] + pos, 1] += zr[lo]
+            cnt[off[2 * si] + pos] += 1
+            z[off[2 * si + 1] + pos, 0] += zr[lo]      # second copy: pairs (a, b = a + d)
+            z[off[2 * si + 1] + pos, 1] += zr[lo + dd]
+            cnt[off[2 * si + 1] + pos] += 1
+    cnt[cnt == 0] = 1
+    return z / cnt[:, None]

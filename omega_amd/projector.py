@@ -124,6 +124,15 @@ class projectorClass:
         self.rings = 0
         self.cr_pz = 0.0
         self.ring_difference = -1
+        # OMEGA's block-ring scanner model (proj.py:127-169): used when these are set and x / z are not given
+        self.blocks_per_ring = 0
+        self.cryst_per_block = 0
+        self.transaxial_multip = 1
+        self.DOI = 0.0
+        self.flip_image = False
+        self.offangle = 0.0
+        self.ndist_side = 1
+        self.span = 3
         # CT geometry (used when no coordinate tables are given)
         self.sourceToCRot = 0.0
         self.sourceToDetector = 0.0
@@ -197,6 +206,8 @@ class projectorClass:
             self.nProjections = self.x.size // 6
             self.NSinos = self.nProjections
         else:
+            if self.x.size == 0 and self.blocks_per_ring > 0 and self.cryst_per_block > 0:
+                self._omega_scanner_tables()
             if self.x.size == 0:
                 if self.diameter <= 0 or self.det_per_ring <= 0:
                     raise ValueError("PET data needs either the x/z coordinate tables or diameter and det_per_ring!")
@@ -216,6 +227,34 @@ class projectorClass:
             self.nProjections = self.NSinos
         self.x = np.ascontiguousarray(self.x, dtype=np.float32).ravel()
         self.z = np.ascontiguousarray(self.z, dtype=np.float32).ravel()
+
+    def _omega_scanner_tables(self):
+        """Coordinate tables of OMEGA's block-ring scanner model from the scanner parameters (detcoord.py
+        getCoordinates -> detectorCoordinates, sinogramCoordinates2D/3D; proj.py:654-679 for the derived
+        sizes): blocks_per_ring x cryst_per_block crystals of pitch cr_p on a ring of diameter (+ 2 DOI),
+        rings of pitch cr_pz, Ndist x Nang sinograms, span-compressed ring pairs."""
+        det = int(self.blocks_per_ring) * int(self.transaxial_multip) * int(self.cryst_per_block)
+        if self.det_per_ring == 0:
+            self.det_per_ring = det
+        if self.det_per_ring != det:
+            raise ValueError("det_per_ring must equal blocks_per_ring * transaxial_multip * cryst_per_block "
+                             "(pseudo detectors are not supported)")
+        if self.cr_p <= 0 or self.diameter <= 0 or self.rings <= 0 or self.cr_pz <= 0:
+            raise ValueError("The scanner model needs cr_p, diameter, rings and cr_pz")
+        if self.Nang == 0:
+            self.Nang = det // 2
+        if self.Ndist == 0:
+            self.Ndist = det // 2
+        if self.ring_difference <= 0:
+            self.ring_difference = self.rings - 1
+        xc, yc = G.omega_crystal_ring(self.blocks_per_ring, self.cryst_per_block, self.cr_p, self.diameter,
+                                      self.transaxial_multip, self.DOI, self.flip_image, self.offangle)
+        self.x = G.omega_sinogram_xy(xc, yc, int(self.Ndist), int(self.Nang), int(self.ndist_side)).ravel()
+        nz = int(np.asarray(self.Nz).ravel()[0])
+        z = G.omega_sinogram_z(int(self.rings), float(self.cr_pz), int(self.span), int(self.ring_difference), nz)
+        if self.NSinos > 0:
+            z = z[: int(self.NSinos)]
+        self.z = z.astype(np.float32).ravel()
 
     def addProjector(self):
         """Validate the configuration, place the grid and form the subsets (proj.py:520-1066)."""

@@ -1,0 +1,89 @@
+"""OMEGA's block-ring PET scanner coordinates (omega_amd.geometry.omega_*) against the reference's own
+detcoord output (tests/golden/scanner_geometry.npz, written by tests/golden/make_scanner_geometry.py).
+
+Bar: crystal coordinates, sinogram tables [xs, ys, xd, yd] and axial tables [zs, zd] bit-identical,
+for block counts with and without a 4-fold symmetry, odd crystals per block, even and odd radial bin
+counts (both ndist_side), span 1 and span 3 with full and limited ring difference. The reference's
+span > 1 routine raises for span >= 5 (detcoord.py:791); there the tables here hold, by construction,
+the mean ring-pair coordinates of each sinogram, checked against a direct enumeration of the pairs
+(parity with the reference unpinned for span >= 5, where it has no output).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from omega_amd import geometry as G
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "scanner_geometry.npz"), allow_pickle=False)
+RINGS = [(16, 8, 4.0, 550.0, 64, 1), (18, 6, 3.2, 400.0, 40, -1), (24, 13, 4.0, 820.0, 101, 1),
+         (38, 8, 2.4, 665.0, 128, 1)]
+AXIAL = [(8, 4.0, 3, 7, 15), (13, 4.0, 5, 12, 25), (8, 4.0, 1, 7, 15), (10, 3.3, 3, 6, 19), (16, 4.0, 7, 15, 31)]
+
+
+@pytest.mark.parametrize("n", range(len(RINGS)))
+def test_crystals_and_sinogram_table(n):
+    bpr, cpb, cr_p, dia, nd, side = RINGS[n]
+    x, y = G.omega_crystal_ring(bpr, cpb, cr_p, dia)
+    assert np.array_equal(x, GOLD[f"ring{n}_x"]) and np.array_equal(y, GOLD[f"ring{n}_y"])
+    t = G.omega_sinogram_xy(x, y, nd, bpr * cpb // 2, side)
+    assert np.array_equal(t, GOLD[f"ring{n}_table"])
+    # neighbouring crystals of a block lie one pitch apart
+    p = np.stack([x, y], 1).astype(np.float64).reshape(bpr, cpb, 2)
+    step = np.linalg.norm(np.diff(p, axis=1), axis=2)
+    np.testing.assert_allclose(step, cr_p, atol=1e-3)
+
+
+@pytest.mark.parametrize("n", range(len(AXIAL)))
+def test_axial_table(n):
+    rings, cr_pz, span, rd, nz = AXIAL[n]
+    z = G.omega_sinogram_z(rings, cr_pz, span, rd, nz)
+    assert np.array_equal(G.omega_segment_table(rings, span, rd, nz), GOLD[f"axial{n}_seg"])
+    if f"axial{n}_z" in GOLD.files:
+        assert np.array_equal(z, GOLD[f"axial{n}_z"])
+    # direct enumeration: the mean (z_a, z_b) of each sinogram's ring pairs
+    zr = (np.arange(rings) + 1) * cr_pz - (rings + 1) * cr_pz / 2
+    if span == 1:
+        ref = np.stack([np.repeat(zr, rings), np.tile(zr, rings)], 1)
+    else:
+        seg = G.omega_segment_table(rings, span, rd, nz)
+        off = np.concatenate([[0], np.cumsum(seg)])
+        k0 = math.ceil(span / 2)
+        sums = np.zeros((off[-1], 2)); cnt = np.zeros(off[-1])
+        for a in range(rings):
+            for b in range(rings):
+                d = a - b
+                if abs(d) <= span // 2:
+                    s = a + b
+                else:
+                    i = (abs(d) - k0) // span
+                    if i >= len(seg) // 2:
+                        continue
+                    s = off[2 * i + (0 if d > 0 else 1) + 1] + a + b - (k0 + span * i)
+                sums[s] += (zr[a], zr[b]); cnt[s] += 1
+        assert np.all(cnt > 0)
+        ref = sums / cnt[:, None]
+    np.testing.assert_allclose(z, ref, rtol=0, atol=1e-9)
+
+
+def _scanner_projector(subsets=1):
+    from omega_amd.projector import projectorClass
+    p = projectorClass()
+    p.Nx, p.Ny, p.Nz = 64, 64, 15
+    p.FOVa_x, p.FOVa_y, p.axial_fov = 300.0, 300.0, 30.0
+    p.blocks_per_ring, p.cryst_per_block, p.cr_p, p.diameter = 16, 8, 4.0, 550.0
+    p.rings, p.cr_pz, p.span = 8, 4.0, 3
+    p.Ndist = 64
+    p.subsets, p.subsetType = subsets, 8
+    return p
+
+
+def test_projector_class_builds_the_scanner_tables():
+    p = _scanner_projector()
+    p.addProjector()
+    x, y = G.omega_crystal_ring(16, 8, 4.0, 550.0)
+    assert np.array_equal(p.x, G.omega_sinogram_xy(x, y, 64, 64, 1).ravel())
+    z = G.omega_sinogram_z(8, 4.0, 3, 7, 15).astype(np.float32).ravel()
+    assert np.array_equal(p.z, z)
+    assert (int(p.nRowsD), int(p.nColsD), int(p.nProjections)) == (64, 64, 47)
