@@ -459,6 +459,12 @@ def bench_tof(args):
     s_s = a.elapsed_time(b) * 1e-3
     exps = 2.0 * 3.0 * p.nBins * ntr
     byt = 12.0 * ntr + 8.0 * M0 * p.nBins
+    cpu = None
+    if not args.no_cpu:
+        _progress("pet2d_tof: CPU baseline")
+        tof = dict(tof_bins=p.nBins, tof_sigma=np.float32(p.sigma_x), tof_centers=p.TOFCenter, tof_epps=float(p.epps))
+        cpu = cpu_baseline(p, y, torch.ones(p.grid.N, device=dev), args.cpu_seconds, runner, 1,
+                           osem_kw=dict(atten=p._attenuation()[0], atten_mode=1, tof=tof))
     _common(args, {
         "metric": "OSEM subset-iters/s, improved Siddon with TOF and attenuation", "value": round(p.subsets * args.steps / dt, 2),
         "unit": "subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
@@ -471,7 +477,7 @@ def bench_tof(args):
                      "kernel": "k_tof_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt),
                      "fp64_exp_per_s": round(exps / s_s, 1)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     })
 
 
@@ -547,8 +553,9 @@ def read_traffic(path):
     return None
 
 
-def cpu_baseline(p, y, x0, budget_s, runner, n_iter):
-    """Reference CPU OSEM on a bounded number of subset-iterations; image compared with the GPU's."""
+def cpu_baseline(p, y, x0, budget_s, runner, n_iter, osem_kw=None):
+    """Reference CPU OSEM on a bounded number of subset-iterations; image compared with the GPU's.
+    ``osem_kw``: corrections of the run (attenuation, TOF) passed to the oracle's OSEM driver."""
     import numpy as np
     import torch
 
@@ -560,14 +567,15 @@ def cpu_baseline(p, y, x0, budget_s, runner, n_iter):
     meas = y.cpu().numpy()
     nmeas = np.diff(p.nMeas).astype(np.int64)
     # probe one subset-iteration to size the sample, then run as many as fit the budget
+    kw = dict(osem_kw or {})
     t = time.perf_counter()
     O.osem(p.grid, p.nRowsD, p.nColsD, p.x, p.z, nmeas, meas, 1, x0.cpu().numpy(), use_ref=use_ref,
-           nthreads=threads, subsets_to_run=1)
+           nthreads=threads, subsets_to_run=1, **kw)
     t1 = time.perf_counter() - t
     n_run = int(max(2, min(p.subsets * n_iter, budget_s / max(t1, 1e-6))))
     t = time.perf_counter()
     x_cpu = O.osem(p.grid, p.nRowsD, p.nColsD, p.x, p.z, nmeas, meas, n_iter, x0.cpu().numpy(), use_ref=use_ref,
-                   nthreads=threads, subsets_to_run=n_run)
+                   nthreads=threads, subsets_to_run=n_run, **kw)
     dt = time.perf_counter() - t
     # GPU image after the same number of subset-iterations
     runner.sens_done = False
@@ -588,8 +596,8 @@ def cpu_baseline(p, y, x0, budget_s, runner, n_iter):
         cpu_model = "unknown"
     return {"value": round(n_run / dt, 3), "unit": "subset-iters/s", "cores": threads,
             "kind": "reference" if use_ref else "port",
-            "sample": f"first {n_run} subset-iterations (of 32) of the same OSEM run, incl. sensitivity "
-                      f"images; OpenMP threads={threads}; {cpu_model}",
+            "sample": f"first {n_run} subset-iterations (of {p.subsets * n_iter}) of the same OSEM run, incl. "
+                      f"sensitivity images; OpenMP threads={threads}; {cpu_model}",
             "image_rel_l2_vs_gpu": rel}
 
 
