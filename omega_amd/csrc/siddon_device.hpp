@@ -233,10 +233,12 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
 // :1181-1211), CT 1. The attenuation factor multiplies in where the reference's backprojection applies
 // its exp(jelppi) or a given per-LOR factor (:1825-1828); on axis-parallel LORs (perp) a given per-LOR
 // factor comes after norm, as there (:1209-1210).
+// MULTI: the geometry may have several rays per LOR (never in the direct-plane kernels).
+template <bool MULTI = true>
 __device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct, bool perp = false) {
     if (ct) return 1.f;
     // 1 / (L * nRays) with several rays per LOR (:1819-1821, :1189-1190)
-    float t = n_rays(g) > 1 ? 1.f / (L * (float)n_rays(g)) : 1.f / L;
+    float t = (MULTI && n_rays(g) > 1) ? 1.f / (L * (float)n_rays(g)) : 1.f / L;
     const bool late = perp && g.atten_lor;
     if (g.atten && !late) t *= g.atten[lo];
     if (g.norm) t *= g.norm[lo];
@@ -283,7 +285,9 @@ template <bool PLANAR>
 __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk& w, float& weight,
                                              bool allow_trans = false, int ray = 0) {
     Ray r;
-    fetch_ray(g, lo, r, ray);
+    // the direct-plane kernels never see several rays per LOR or TOF (such geometries are not planar)
+    if (PLANAR) fetch_ray_base(g, lo, r);
+    else fetch_ray(g, lo, r, ray);
     const float ydf = r.yd - r.ys, xdf = r.xd - r.xs, zdf = r.zd - r.zs;
     if ((ydf == 0.f && xdf == 0.f) || isinf(ydf) || isinf(xdf)) return false;
     const bool ct = g.mode == MODE_CT_FLAT;
@@ -335,7 +339,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         } else {
             return false;
         }
-        weight = make_weight(g, lo, L, ct, true);
+        weight = make_weight<!PLANAR>(g, lo, L, ct, true);
         uint32_t apu = 0;
         {
             float s = db - dd + ddb;
@@ -346,9 +350,11 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         }
         w.perp = true;
         w.trans = false;
-        const uint32_t zoff = PLANAR ? 0u : (uint32_t)tk * g.Nyx;
-        if (along_x) { w.p_row = apu * (PLANAR ? g.pitch : g.Nx) + zoff; w.p_rstride = 1u; }
-        else { w.p_row = apu + zoff; w.p_rstride = stride; }
+        if (!PLANAR) {   // the whole row, for the attenuation line integral (k_atten_factor)
+            const uint32_t zoff = (uint32_t)tk * g.Nyx;
+            if (along_x) { w.p_row = apu * g.Nx + zoff; w.p_rstride = 1u; }
+            else { w.p_row = apu + zoff; w.p_rstride = stride; }
+        }
         w.p_ind = apu * Nm + (PLANAR ? 0u : (uint32_t)tk * Nrow * Nrun) + stride * (uint32_t)a1;
         w.p_stride = stride;
         w.p_a1 = (uint32_t)a1; w.p_a2 = (uint32_t)a2; w.p_nrun = Nrun;
@@ -357,7 +363,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         w.p_dT1 = dT1; w.p_dT2 = dT2; w.p_d = dd2;
         w.left = 0;
         // half the row, signed by the (near-zero) difference across the row (projector_functions.h:1641-1645)
-        if (g.tof_bins > 1) w.tofD = (dd2 * (float)Nrun) / copysignf(2.f, along_x ? ydf : xdf);
+        if (!PLANAR && g.tof_bins > 1) w.tofD = (dd2 * (float)Nrun) / copysignf(2.f, along_x ? ydf : xdf);
         return true;
     }
     w.perp = false;
@@ -384,8 +390,8 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         skip = preloop3(g, r, xdf, ydf, zdf, ti, tj, tk, tyu, txu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0);
     }
     if (skip) return false;
-    weight = make_weight(g, lo, L, ct);
-    if (g.tof_bins > 1) {   // TOFDis (projector_functions.h:588-594, :1741-1742)
+    weight = make_weight<!PLANAR>(g, lo, L, ct);
+    if (!PLANAR && g.tof_bins > 1) {   // TOFDis (projector_functions.h:588-594, :1741-1742)
         const float xI = xdf * tc, yI = ydf * tc, zI = zdf * tc;
         w.tofD = sqrtf(xI * xI + yI * yI + zI * zI) - L / 2.f;
     }
