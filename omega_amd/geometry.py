@@ -467,3 +467,76 @@ def omega_sinogram_z(rings: int, cr_pz: float, span: int, ring_difference: int, 
             cnt[off[2 * si + 1] + pos] += 1
     cnt[cnt == 0] = 1
     return z / cnt[:, None]
+
+
+# ---- volume-of-intersection projector (projector type 3) ------------------------------------------------
+
+def sphere_cylinder_volume(R: float, r: float, b):
+    """Volume of the intersection of a sphere of radius r whose centre lies at distance b from the axis
+    of an infinite cylinder of radius R (the closed form in complete elliptic integrals of the first and
+    second kind and a third-kind integral evaluated by quadrature, as the reference uses it,
+    coordinates.py:150-195). b: array; returns float32."""
+    import warnings
+    from scipy.integrate import quad
+    from scipy.special import ellipe, ellipk
+    with warnings.catch_warnings(), np.errstate(divide="ignore", invalid="ignore"):
+        warnings.simplefilter("ignore")   # b = R divides by zero and quad warns there, as in the reference
+        return _sphere_cylinder_volume(R, r, np.asarray(b), quad, ellipk, ellipe)
+
+
+def _sphere_cylinder_volume(R, r, b, quad, ellipk, ellipe):
+    A = np.maximum(r ** 2, (b + R) ** 2)
+    B = np.minimum(r ** 2, (b + R) ** 2)
+    C = (b - R) ** 2
+    k = np.abs((B - C) / (A - C))
+    alpha = (B - C) / C
+    s = (b + R) * (b - R)
+    third = np.array([quad(lambda x, al=al, kk=kk: 1. / ((1. + al * x ** 2) * np.sqrt(1. - x ** 2)
+                                                          * np.sqrt(1. - kk * x ** 2)), 0, 1)[0]
+                      for al, kk in zip(np.ravel(alpha), np.ravel(k))]).reshape(np.shape(alpha))
+    K, E = ellipk(k), ellipe(k)
+    step = np.zeros(b.shape[0], dtype=np.float32)
+    step[R - b > 0] = 1
+    step[R - b == 0] = 0.5
+    V = np.zeros(b.shape[0], dtype=np.float32)
+    sph = (4. * np.pi) / 3. * r ** 3
+    out = r > b + R
+    if np.any(out):
+        a_, b_, c_, s_ = A[out], B[out], C[out], s[out]
+        V[out] = sph * step[out] + 4. / (3. * np.sqrt(a_ - c_)) * (
+            third[out] * (a_ ** 2 * s_) / c_ - K[out] * (a_ * s_ - ((a_ - b_) * (a_ - c_)) / 3.)
+            - E[out] * (a_ - c_) * (s_ + (4. * a_ - 2. * b_ - 2. * c_) / 3.))
+    inn = r < b + R
+    if np.any(inn):
+        a_, b_, c_, s_ = A[inn], B[inn], C[inn], s[inn]
+        V[inn] = sph * step[inn] + ((4. / 3.) / (np.sqrt(a_ - c_))) * (
+            third[inn] * ((b_ ** 2 * s_) / c_) + K[inn] * (s_ * (a_ - 2. * b_) + (a_ - b_) * ((3. * b_ - c_ - 2 * a_) / 3.))
+            + E[inn] * (a_ - c_) * (-s_ + (2. * a_ - 4. * b_ + 2. * c_) / 3.))
+    return V
+
+
+def voxel_volume_table(tube_radius: float, voxel_radius: float, dx: float, dy: float, dz: float,
+                       return_grid: bool = False):
+    """Look-up table of the volume projector (computeVoxelVolumes, coordinates.py:133-207): a voxel is a
+    sphere of radius sqrt(2) * voxel_radius * max(d) / 2; V[k] is its volume of intersection with the tube
+    (radius tube_radius) at axis distance bmin + k * 1e-3 mm (the kernel indexes it by
+    rint((d - bmin) * 1e3)); closer than bmin the voxel counts Vmax, beyond bmax nothing.
+    Returns (V float32, bmin, bmax, Vmax, effective voxel radius), and with return_grid the distances
+    the entries were evaluated at: the reference drops the entries where the closed form is NaN or not
+    decreasing (at and next to b = tube_radius), so past that point V[k] holds the volume at a distance
+    one or more 1e-3 steps beyond bmin + k * 1e-3, and the kernel's lookup reads it there as well."""
+    # numpy float64 scalars where the reference has them (they set the dtype of the table's arithmetic)
+    dp = np.max(np.array([dx, dy, dz], dtype=np.float64))
+    vr = math.sqrt(2.) * voxel_radius * (dp / 2.)
+    bmax = tube_radius + vr
+    b = np.linspace(0, bmax, 10000, dtype=np.float32)
+    b = b[tube_radius <= (b + vr)]
+    b = np.unique(np.round(b * 10 ** 3) / 10 ** 3)
+    V = sphere_cylinder_volume(tube_radius, vr, b)
+    keep = np.append(np.diff(V), 0) <= 0
+    b = b[keep]
+    V = np.abs(V[keep]).astype(np.float32)
+    vmax = (4 * np.pi) / 3 * vr ** 3
+    if return_grid:
+        return V, float(np.min(b)), float(bmax), float(vmax), vr, b
+    return V, float(np.min(b)), float(bmax), float(vmax), vr
