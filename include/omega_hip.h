@@ -33,6 +33,7 @@ extern "C" {
 /* Projector types (source/Python/omegatomo/projector/proj.py:242, projector_type). */
 #define OMG_PROJ_SIDDON 1
 #define OMG_PROJ_ORTH 2
+#define OMG_PROJ_VOLUME 3
 #define OMG_PROJ_BDD 5
 
 typedef struct omg_geom omg_geom;
@@ -107,6 +108,13 @@ typedef struct omg_geom_desc {
      * cr_z (axially) (paramStruct dPitchXY, dPitchZ; the operator API passes dPitchX and dPitchY); each
      * ray weighs 1 / (L nRays), the forward projection sums them. Siddon, PET data, no TOF. */
     uint32_t rays_xy, rays_z; float cr_xy, cr_z;
+    /* volume-of-intersection projector (projector_type 3; the reference's -DORTH -DVOL kernel,
+     * opencl_functions_orth3D.h:132-140, projectorType123.cl:502-505,641-643,952-953): the tube of
+     * radius orth_width (tube_radius, init.py:653) sweeps like projector 2 (orth_flags); a voxel at axis
+     * distance d counts vol_vmax when d < vol_bmin, vol_V[rne((d - vol_bmin) * 1000)] when
+     * d <= vol_bmax (0 past the table's end), nothing beyond; every LOR weighs 1 / (L R^2 / pi).
+     * vol_V, vol_bmin, vol_bmax, vol_vmax: computeVoxelVolumes (coordinates.py:190-205), copied at create. */
+    const float* vol_V; int64_t vol_len; float vol_bmin, vol_bmax, vol_vmax;
 } omg_geom_desc;
 
 /* Library/version info and error text. */

@@ -16,7 +16,7 @@ OMG_ERR_HIP = -2
 OMG_ERR_UNSUPPORTED = -3
 
 MODE_PET_SINO, MODE_CT_FLAT, MODE_LIST6 = 0, 1, 2
-PROJ_SIDDON, PROJ_ORTH, PROJ_BDD = 1, 2, 5
+PROJ_SIDDON, PROJ_ORTH, PROJ_VOLUME, PROJ_BDD = 1, 2, 3, 5
 
 
 class OmegaHipError(RuntimeError):
@@ -55,6 +55,8 @@ class GeomDesc(C.Structure):
         ("scat", C.c_void_p), ("scat_len", C.c_int64),
         ("tof_bins", C.c_uint32), ("tof_sigma", C.c_float), ("tof_epps", C.c_float), ("tof_centers", C.c_void_p),
         ("rays_xy", C.c_uint32), ("rays_z", C.c_uint32), ("cr_xy", C.c_float), ("cr_z", C.c_float),
+        ("vol_V", C.c_void_p), ("vol_len", C.c_int64), ("vol_bmin", C.c_float), ("vol_bmax", C.c_float),
+        ("vol_vmax", C.c_float),
     ]
 
 

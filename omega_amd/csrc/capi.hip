@@ -150,8 +150,9 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     OMG_REQUIRE(d->mode == OMG_MODE_PET_SINO || d->mode == OMG_MODE_CT_FLAT || d->mode == OMG_MODE_LIST6,
                 "unknown data mode %d", d->mode);
     OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON || d->projector_type == OMG_PROJ_ORTH ||
-                    d->projector_type == OMG_PROJ_BDD,
-                "projector_type %d is not provided (1 Siddon, 2 orthogonal, 5 branchless distance-driven)",
+                    d->projector_type == OMG_PROJ_VOLUME || d->projector_type == OMG_PROJ_BDD,
+                "projector_type %d is not provided (1 Siddon, 2 orthogonal, 3 volume of intersection, 5 branchless "
+                "distance-driven)",
                 d->projector_type);
     OMG_REQUIRE(d->x && d->x_len > 0, "missing x coordinate table");
     if (d->mode == OMG_MODE_PET_SINO) {
@@ -168,6 +169,15 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     }
     if (d->projector_type == OMG_PROJ_BDD)
         OMG_REQUIRE(d->mode == OMG_MODE_CT_FLAT, "projector_type 5 (BDD) supports CT flat-panel data only");
+    if (d->projector_type == OMG_PROJ_VOLUME) {
+        OMG_REQUIRE(d->vol_V && d->vol_len > 0 && d->vol_len <= 0x7FFFFFFFll,
+                    "projector_type 3 needs the volume table vol_V (1 to 2^31 - 1 entries), got %lld",
+                    (long long)d->vol_len);
+        OMG_REQUIRE(std::isfinite(d->vol_bmin) && d->vol_bmin >= 0.f && std::isfinite(d->vol_bmax) &&
+                        d->vol_bmax > d->vol_bmin && std::isfinite(d->vol_vmax) && d->vol_vmax > 0.f,
+                    "volume table scalars need 0 <= vol_bmin < vol_bmax and vol_vmax > 0 (got %g, %g, %g)",
+                    (double)d->vol_bmin, (double)d->vol_bmax, (double)d->vol_vmax);
+    }
     if (d->meas_index) {
         OMG_REQUIRE(d->mode == OMG_MODE_PET_SINO, "meas_index (subset types 1-5) is for PET sinogram data");
         OMG_REQUIRE(d->meas_index_len > 0 && d->meas_index_len <= 0xFFFFFFFFll,
@@ -254,9 +264,10 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     if (!rc && g->planar) g->ring = ring_table(*d, d->x, g->bmaxx, g->bmaxy);
     if (!rc && d->mode == OMG_MODE_PET_SINO) rc = vbp_prepare(g, d->x);
     if (!rc && d->projector_type == OMG_PROJ_BDD) rc = bdd_prepare(g);
-    if (!rc && d->projector_type == OMG_PROJ_ORTH) rc = orth_prepare(g);
+    if (!rc && d->projector_type == OMG_PROJ_VOLUME) rc = upload(d->vol_V, d->vol_len, &g->vol_dev);
+    if (!rc && (d->projector_type == OMG_PROJ_ORTH || d->projector_type == OMG_PROJ_VOLUME)) rc = orth_prepare(g);
     g->d.x = nullptr; g->d.z = nullptr; g->d.norm = nullptr; g->d.meas_index = nullptr; g->d.atten = nullptr;
-    g->d.scat = nullptr; g->d.tof_centers = nullptr;
+    g->d.scat = nullptr; g->d.tof_centers = nullptr; g->d.vol_V = nullptr;
     if (rc) {
         omg_geom_destroy(g);
         return rc;
@@ -276,6 +287,7 @@ extern "C" int omg_geom_destroy(omg_geom* g) {
     hipFree(g->norm_dev);
     hipFree(g->atten_dev);
     hipFree(g->scat_dev);
+    hipFree(g->vol_dev);
     hipFree(g->tof_c);
     hipFree(g->index_dev);
     hipFree(g->frames_t);

@@ -12,6 +12,7 @@ struct omg_geom {
     float* norm_dev = nullptr;
     float* atten_dev = nullptr;      // per-measurement attenuation factors (norm's layout), or null
     float* scat_dev = nullptr;       // per-measurement multiplicative correction (norm's layout), or null
+    float* vol_dev = nullptr;        // projector 3: the volume-of-intersection table, or null
     float* tof_c = nullptr;          // TOF bin centres (tof_bins floats), or null
     uint32_t* index_dev = nullptr;   // measurement-level subsets: sinogram bin per measurement (subset order)
     float bmaxx = 0.f, bmaxy = 0.f, bmaxz = 0.f;
@@ -51,6 +52,11 @@ struct omg_geom {
 namespace omg {
 
 // Kernel argument block for Siddon-family launches covering measurements [first, first + n).
+// Projectors 2 and 3 share the tube sweep (orth.hip); they differ in the voxel and LOR weights.
+static inline bool is_tube(const omg_geom* g) {
+    return g->d.projector_type == OMG_PROJ_ORTH || g->d.projector_type == OMG_PROJ_VOLUME;
+}
+
 static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n) {
     SidArgs a;
     a.Nx = g->d.Nx; a.Ny = g->d.Ny; a.Nz = g->d.Nz; a.Nyx = g->d.Nx * g->d.Ny;

@@ -1,4 +1,5 @@
-// orth.hip -- orthogonal distance-based projector (projector type 2) for gfx950.
+// orth.hip -- orthogonal distance-based (projector type 2) and volume-of-intersection (type 3) projectors
+// for gfx950.
 //
 // Semantics are those of the reference's GPU kernel, projectorType123.cl built with -DORTH and
 // -DCRYSTXY / -DCRYSTZ (no TOF, masks or multi-ray), because the reference's CPU version
@@ -15,6 +16,10 @@
 // FP: y = temp * sum w x; BP: rhs += r * (w * temp), sens += w * temp, temp = atten * norm * scat * global_factor,
 // atten the LOR's attenuation factor exp(-sum len mu) over its Siddon walk (projectorType123.cl:1022-1048,
 // :1220-1233: the reference's -DORTH -DATN integrates mu along the same Siddon walk).
+// Projector 3 (-DVOL -DTOTLENGTH) sweeps the same tube of radius R = width with orl = L: a voxel at axis
+// distance d weighs Vmax inside bmin, else its volume of intersection from the look-up table, and is cut
+// at d > bmax; temp carries 1 / TotV, TotV = L R^2 / pi (opencl_functions_orth3D.h:132-140,
+// projectorType123.cl:502-505,641-643,952).
 //
 // One lane follows one LOR. FMAD of the reference is fmaf here; everything else is evaluated as written
 // (the library is built with -ffp-contract=off and IEEE division/sqrt), so a forward projection is
@@ -30,8 +35,11 @@ constexpr double kCosHi = -0.7071035822349067;   // cos(2.35619f), in double
 
 struct OrthArgs {
     SidArgs a;
-    float width;        // kernel tube width in mm (tube_width_z)
+    float width;        // kernel tube width in mm (tube_width_z); the tube radius for projector 3
     int32_t cxy, cz;    // transaxial row sweep, axial sweep
+    const float* vol;   // projector 3: volume-of-intersection table (null: projector 2)
+    uint32_t vol_len;
+    float bmin, bmax, vmax;
 };
 
 struct F3 { float x, y, z; };
@@ -141,6 +149,9 @@ struct Tube {
     float b2, d2, bz, dz, orl;
     uint32_t Nxy, N1, N2p, N3p, Nz;
     bool cxy, cz;
+    const float* V;     // projector 3 (null: projector 2)
+    uint32_t vlen;
+    float bmin, bmax, vmax;
 };
 
 // One voxel: true when it lies outside the cut-off (the row sweep stops there).
@@ -150,8 +161,22 @@ __device__ __forceinline__ bool o_voxel(const Tube& t, int tempi, int uu, int zz
     const float y1 = __builtin_fmaf(t.diff.z, x0, -l.y);
     const float z1 = __builtin_fmaf(-t.diff.x, x0, l.z);
     const float norm2 = __builtin_fmaf(l.x, l.x, __builtin_fmaf(y1, y1, z1 * z1));
-    if (norm2 >= (1.f - kOrthThr) * (1.f - kOrthThr) * t.orl * t.orl) return true;
-    const float w = 1.f - sqrtf(norm2) / t.orl;
+    float w;
+    if (t.V) {
+        // volume of intersection (opencl_functions_orth3D.h:132-140): orl = L, so w is the axis distance;
+        // past the table's end (which the reference leaves open) the voxel counts 0
+        if (norm2 > t.bmax * t.bmax * t.orl * t.orl) return true;
+        w = sqrtf(norm2) / t.orl;
+        if (w < t.bmin) {
+            w = t.vmax;
+        } else {
+            const uint32_t k = __float2uint_rn((w - t.bmin) * 1e3f);
+            w = k < t.vlen ? t.V[k] : 0.f;
+        }
+    } else {
+        if (norm2 >= (1.f - kOrthThr) * (1.f - kOrthThr) * t.orl * t.orl) return true;
+        w = 1.f - sqrtf(norm2) / t.orl;
+    }
     const int64_t ind = (int64_t)((uint32_t)tempi * t.N2p + (uint32_t)uu * t.N3p) + (int64_t)zz * (int64_t)t.Nxy;
     visit(ind, w);
     return false;
@@ -231,9 +256,10 @@ __device__ int o_sweep(const Tube& t, int tempi, float cx, int temp2, int tempk,
     return uu;
 }
 
-// One LOR: visit(ind, w) for every voxel of its tube. Returns the LOR weight temp (0: ray skipped).
+// One LOR: visit(ind, w) for every voxel of its tube. Returns the LOR weight temp (0: ray skipped); with
+// temp_out, temp is also stored there before the first visit (for callers whose visit applies it).
 template <class V>
-__device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit) {
+__device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out = nullptr) {
     const SidArgs& g = oa.a;
     Ray r;
     fetch_ray(g, lo, r);
@@ -245,12 +271,23 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit) {
     F3& diff = t.diff;
     if ((diff.x == 0.f && diff.y == 0.f) || !isfinite(diff.x) || !isfinite(diff.y) || !isfinite(diff.z)) return 0.f;
     const float L = sqrtf(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
-    t.orl = L * oa.width;
-    float w = 1.f;
-    if (g.atten) w *= g.atten[lo];
-    if (g.norm) w *= g.norm[lo];
-    if (g.scat) w *= g.scat[lo];
-    w *= g.global_factor;
+    const bool vol = oa.vol != nullptr;
+    t.orl = vol ? L : L * oa.width;
+    t.V = oa.vol;
+    t.vlen = oa.vol_len;
+    t.bmin = oa.bmin;
+    t.bmax = oa.bmax;
+    t.vmax = oa.vmax;
+    auto chain = [&](float w) {
+        if (g.atten) w *= g.atten[lo];
+        if (g.norm) w *= g.norm[lo];
+        if (g.scat) w *= g.scat[lo];
+        return w * g.global_factor;
+    };
+    // projector 3: TotV = L * (1/pi) * R^2 (projectorType123.cl:505); an axis-parallel ray's weight is the
+    // corrections' product times 1 / TotV (:641-643), any other ray's starts from 1 / TotV (:952)
+    const float inv_v = vol ? 1.f / (L * (float)M_1_PI * oa.width * oa.width) : 1.f;
+    const float w1 = chain(1.f);
     t.Nxy = g.Nx * g.Ny;
     t.Nz = g.Nz;
     t.bz = g.bz;
@@ -295,6 +332,8 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit) {
         } else {
             return 0.f;
         }
+        const float w = vol ? w1 * inv_v : w1;
+        if (temp_out) *temp_out = w;
         for (int ii = apuX1; ii <= apuX2; ii++) {
             const float cx = b1 + d1 * (float)ii + d1 / 2.f;
             o_sweep(t, ii, cx, indO, tempk, false, tempk_b, 0, visit);
@@ -345,6 +384,8 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit) {
         NNx = g.Nx;
         t.N1 = g.Ny; t.N2p = 1u; t.N3p = g.Nx;
     }
+    const float w = vol ? chain(inv_v) : w1;
+    if (temp_out) *temp_out = w;
     int tempi_a = tempi, tempj_a = tempj, tempk_a = tempk;
     bool swept = false;
     int tempiOld = 0;
@@ -412,19 +453,15 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_bp(OrthArgs oa, const float
     if (lo >= oa.a.nmeas) return;
     const float r = yin[lo];
     if (r == 0.f && sens == nullptr) return;
-    float temp = 1.f;
-    if (oa.a.atten) temp *= oa.a.atten[lo];
-    if (oa.a.norm) temp *= oa.a.norm[lo];
-    if (oa.a.scat) temp *= oa.a.scat[lo];
-    temp *= oa.a.global_factor;
+    float temp = 0.f;   // the LOR weight, set by orth_lor before its first visit
     if (sens) {
         orth_lor(oa, lo, [&](int64_t ind, float w) {
             const float e = w * temp;
             unsafeAtomicAdd(rhs + ind, r * e);
             unsafeAtomicAdd(sens + ind, e);
-        });
+        }, &temp);
     } else {
-        orth_lor(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); });
+        orth_lor(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); }, &temp);
     }
 }
 
@@ -595,6 +632,11 @@ static OrthArgs make_orth_args(const omg_geom* g, int64_t first, int64_t n) {
     oa.width = g->d.orth_width;
     oa.cxy = (g->d.orth_flags & 1) != 0;
     oa.cz = (g->d.orth_flags & 2) != 0;
+    oa.vol = g->vol_dev;
+    oa.vol_len = (uint32_t)g->d.vol_len;
+    oa.bmin = g->d.vol_bmin;
+    oa.bmax = g->d.vol_bmax;
+    oa.vmax = g->d.vol_vmax;
     return oa;
 }
 
@@ -674,12 +716,14 @@ int orth_planar_fused(const omg_geom* g, int64_t first, int64_t n, const float* 
 int orth_prepare(omg_geom* g) {
     const omg_geom_desc& d = g->d;
     OMG_REQUIRE(d.mode == OMG_MODE_PET_SINO || d.mode == OMG_MODE_LIST6,
-                "projector_type 2 is provided for PET sinogram and list-mode data");
-    OMG_REQUIRE(d.Nx == d.Ny, "projector_type 2 needs Nx == Ny (the reference kernel's axis-parallel branch "
-                              "indexes rows with the other axis' extent)");
-    OMG_REQUIRE(d.orth_width > 0.f && std::isfinite(d.orth_width), "orth_width must be a positive tube width");
+                "projector_type %d is provided for PET sinogram and list-mode data", d.projector_type);
+    OMG_REQUIRE(d.Nx == d.Ny, "projector_type %d needs Nx == Ny (the reference kernel's axis-parallel branch "
+                              "indexes rows with the other axis' extent)", d.projector_type);
+    OMG_REQUIRE(d.orth_width > 0.f && std::isfinite(d.orth_width),
+                "orth_width must be a positive tube width (projector 2) or radius (projector 3)");
     OMG_REQUIRE((d.orth_flags & ~3) == 0, "orth_flags holds bit 0 (transaxial sweep) and bit 1 (axial sweep) only");
-    g->orth_planar = orth_planar_check(g);
+    // volume weights exceed 1, outside the planar kernels' fixed-point bound: one lane per LOR
+    g->orth_planar = d.projector_type == OMG_PROJ_ORTH && orth_planar_check(g);
     return OMG_OK;
 }
 

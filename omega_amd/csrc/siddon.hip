@@ -682,8 +682,7 @@ static int generic_osem(const omg_geom* g, int64_t first, int64_t n, const float
 
 static int siddon_common_checks(const omg_geom* g, int64_t first, int64_t n) {
     OMG_REQUIRE(g != nullptr, "null geometry handle");
-    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON || g->d.projector_type == OMG_PROJ_ORTH ||
-                    g->d.projector_type == OMG_PROJ_BDD,
+    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON || is_tube(g) || g->d.projector_type == OMG_PROJ_BDD,
                 "unknown projector type %d", g->d.projector_type);
     return check_range(g, first, n);
 }
@@ -710,7 +709,7 @@ extern "C" int omg_forward_project(const omg_geom* g, int64_t first, int64_t n_m
     OMG_REQUIRE(img && y, "null image or output pointer");
     if (n_meas == 0) return OMG_OK;
     hipStream_t s = as_stream(stream);
-    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_forward(g, first, n_meas, img, y, s);
+    if (is_tube(g)) return orth_forward(g, first, n_meas, img, y, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_forward(g, first, n_meas, img, y, s);
     if (planar_ok(g) && !g->force_generic) return launch_planar<true, false, false>(g, first, n_meas, img, nullptr, y, nullptr, nullptr, 0.f, s);
     return generic_fp(g, first, n_meas, img, y, s);
@@ -723,7 +722,7 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
     OMG_REQUIRE(y && rhs, "null input or output pointer");
     if (n_meas == 0) return OMG_OK;
     hipStream_t s = as_stream(stream);
-    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_backward(g, first, n_meas, y, rhs, sens, s);
+    if (is_tube(g)) return orth_backward(g, first, n_meas, y, rhs, sens, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_backward(g, first, n_meas, y, rhs, sens, s);
     if (planar_ok(g) && !g->force_generic) {
         if (sens) return launch_planar<false, true, true>(g, first, n_meas, nullptr, y, nullptr, rhs, sens, 0.f, s);
@@ -778,7 +777,7 @@ extern "C" int omg_osem_data_step_ex(const omg_geom* g, int64_t first, int64_t n
     OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM data step is implemented for projector_type 1 and 2");
     if (n_meas == 0) return OMG_OK;
     hipStream_t s = as_stream(stream);
-    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_osem(g, first, n_meas, img, meas, rhs, sens, epps, s, additive);
+    if (is_tube(g)) return orth_osem(g, first, n_meas, img, meas, rhs, sens, epps, s, additive);
     if (planar_ok(g) && !g->force_generic) {
         if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, s, additive);
         return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, s, additive);
@@ -946,7 +945,7 @@ extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, in
         if (rc) return rc;
         return planar_update(g, first, n_meas, x, S, dU, compute_sens, epps, positivity, s);
     }
-    if (n_meas > 0 && g->d.projector_type == OMG_PROJ_ORTH && g->orth_planar && !g->force_generic &&
+    if (n_meas > 0 && is_tube(g) && g->orth_planar && !g->force_generic &&
         unique_slices(g, first, (uint32_t)(n_meas / per))) {
         rc = orth_planar_fused(g, first, n_meas, x, meas, compute_sens, epps, s, additive);
         if (rc) return rc;
@@ -960,7 +959,7 @@ extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, in
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, S, N, 0.f);
         OMG_CHECK_LAUNCH();
     }
-    if (n_meas > 0 && g->d.projector_type == OMG_PROJ_ORTH) {
+    if (n_meas > 0 && is_tube(g)) {
         rc = orth_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s, additive);
         if (rc) return rc;
     } else if (n_meas > 0) {
@@ -983,7 +982,7 @@ extern "C" int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas
     OMG_REQUIRE(counts != nullptr, "null counts");
     OMG_REQUIRE(offsets == nullptr || (idx && len), "offsets given without idx/len outputs");
     if (n_meas == 0) return OMG_OK;
-    if (g->d.projector_type == OMG_PROJ_ORTH) return orth_trace(g, first, n_meas, counts, offsets, idx, len, as_stream(stream));
+    if (is_tube(g)) return orth_trace(g, first, n_meas, counts, offsets, idx, len, as_stream(stream));
     SidArgs a = make_sid_args(g, first, n_meas);
     hipLaunchKernelGGL(k_siddon_trace, dim3(grid_for(n_meas, kBlock)), dim3(kBlock), 0, as_stream(stream), a, counts,
                        offsets, idx, len);

@@ -74,6 +74,8 @@ class projectorClass:
         self.projector_type = 1
         self.tube_width_xy = 0.0
         self.tube_width_z = 0.0
+        self.tube_radius = 1.0      # projector 3: radius of the tube of response (mm)
+        self.voxel_radius = 1.0     # projector 3: voxel sphere radius relative to half a voxel's diagonal
         # data
         self.CT = False
         self.listmode = 0
@@ -159,9 +161,10 @@ class projectorClass:
         """Configuration checks with the reference's ValueError behaviour (proj.py:1068-1285)."""
         if self.FOVa_x > 0 and self.FOVa_y == 0:
             self.FOVa_y = self.FOVa_x
-        if self.projector_type not in (1, 2, 5, 11, 22, 55):
-            raise ValueError("Only projector types 1 (improved Siddon), 2 (orthogonal distance-based) and "
-                             "5 (branchless distance-driven) are available in this implementation!")
+        if self.projector_type not in (1, 2, 3, 5, 11, 22, 33, 55):
+            raise ValueError("Only projector types 1 (improved Siddon), 2 (orthogonal distance-based), 3 (volume "
+                             "of intersection) and 5 (branchless distance-driven) are available in this "
+                             "implementation!")
         if self.projector_type in (5, 55) and not self.CT:
             raise ValueError("Projector type 5 is only supported with CT data!")
         if self.subsets > 1 and not self.listmode:
@@ -267,6 +270,13 @@ class projectorClass:
         self.dx = np.array([g.dx], dtype=np.float32)
         self.dy = np.array([g.dy], dtype=np.float32)
         self.dz = np.array([g.dz], dtype=np.float32)
+        pt = self.projector_type % 10 if self.projector_type > 10 else self.projector_type
+        if pt == 3 and getattr(self, "V", None) is None:
+            # computeVoxelVolumes (coordinates.py:190-205): the table and its scalars; voxel_radius becomes the
+            # voxel sphere's radius in mm, as there
+            self.V, self.bmin, self.bmax, self.Vmax, self.voxel_radius = G.voxel_volume_table(
+                float(self.tube_radius), float(self.voxel_radius), self.dx[0].item(), self.dy[0].item(),
+                self.dz[0].item())
         self.bx = np.array([g.bx], dtype=np.float32)
         self.by = np.array([g.by], dtype=np.float32)
         self.bz = np.array([g.bz], dtype=np.float32)
@@ -437,6 +447,17 @@ class projectorClass:
                                  "(with tube_width_z = 0 every voxel is outside the tube)")
             d.orth_width = float(self.tube_width_z)
             d.orth_flags = int(self.tube_width_xy > 0) | (int(self.tube_width_z > 0) << 1)
+        if pt == 3:
+            # the kernel's radius is tube_radius (init.py:653), both sweeps on (proj.py:1041-1054); bmin,
+            # bmax and Vmax reach the kernel as float32 (init.py:654)
+            if not self.tube_radius > 0:
+                raise ValueError("The volume-of-intersection projector needs tube_radius > 0")
+            V = np.ascontiguousarray(self.V, dtype=np.float32)
+            keep.append(V)
+            d.orth_width = float(np.float32(self.tube_radius))
+            d.orth_flags = 3
+            d.vol_V, d.vol_len = V.ctypes.data, V.size
+            d.vol_bmin, d.vol_bmax, d.vol_vmax = float(self.bmin), float(self.bmax), float(self.Vmax)
         if getattr(self, "voxel_backprojection", False):
             d.reserved[2] = 1
         if self._measurement_subsets():

@@ -1083,8 +1083,17 @@ namespace orth {
 struct OGeo {
     oracle::Geo<float> g;
     float bmaxx, bmaxy, bmaxz;   // b + N*d computed in double, rounded once (init.py:727)
-    float width;                 // kernel tube width (tube_width_z)
+    float width;                 // kernel tube width (tube_width_z); tube radius (tube_radius) for projector 3
     bool cxy, cz;                // -DCRYSTXY (transaxial row sweep), -DCRYSTZ (axial sweep)
+    const float* V = nullptr;    // projector 3 (-DVOL): the volume-of-intersection table, else null
+    uint32_t vlen = 0;
+    float bmin = 0.f, bmax = 0.f, vmax = 0.f;
+};
+
+// Per-LOR tube scale: orl = L * width (projector 2) or L (projector 3, whose voxel weight is a table value).
+struct TubeW {
+    float orl;
+    const OGeo* o;
 };
 
 struct F3 { float x, y, z; };
@@ -1196,13 +1205,29 @@ static bool pre3(const OGeo& o, F3 s, F3 d, F3 diff, int& ti, int& tj, int& tk, 
 // One voxel of the tube: returns true when the voxel is outside the cut-off (the sweep stops there).
 template <class V>
 static inline bool helper(int tempi, int uu, uint32_t N2p, uint32_t N3p, uint32_t Nxy, int zz, F3 s, F3 l, F3 diff,
-                          float orl, F3 center, V& visit) {
+                          const TubeW& tw, F3 center, V& visit) {
     const float x0 = std::fma(-1.f, s.y, center.y);
     const float y1 = std::fma(diff.z, x0, -l.y);
     const float z1 = std::fma(-diff.x, x0, l.z);
     const float norm2 = std::fma(l.x, l.x, std::fma(y1, y1, z1 * z1));
-    if (norm2 >= (1.f - kThr) * (1.f - kThr) * orl * orl) return true;
-    const float w = 1.f - std::sqrt(norm2) / orl;
+    const float orl = tw.orl;
+    float w;
+    if (tw.o->V) {
+        // opencl_functions_orth3D.h:132-140: cut at d > bmax, Vmax inside bmin, else the table entry at
+        // (d - bmin) * 1e3 rounded to nearest even (0 past the table's end, which the reference leaves open)
+        const OGeo& o = *tw.o;
+        if (norm2 > o.bmax * o.bmax * orl * orl) return true;
+        w = std::sqrt(norm2) / orl;
+        if (w < o.bmin) {
+            w = o.vmax;
+        } else {
+            const uint32_t k = (uint32_t)std::nearbyint((w - o.bmin) * 1e3f);
+            w = k < o.vlen ? o.V[k] : 0.f;
+        }
+    } else {
+        if (norm2 >= (1.f - kThr) * (1.f - kThr) * orl * orl) return true;
+        w = 1.f - std::sqrt(norm2) / orl;
+    }
     const int64_t ind = (int64_t)((uint32_t)tempi * N2p + (uint32_t)uu * N3p) + (int64_t)zz * (int64_t)Nxy;
     visit(ind, w);
     return false;
@@ -1211,7 +1236,7 @@ static inline bool helper(int tempi, int uu, uint32_t N2p, uint32_t N3p, uint32_
 // orthDistance3D: rows of slab `tempi` around row temp2, z rows around tempk (when the axial sweep is on).
 template <class V>
 static int sweep(const OGeo& o, int tempi, F3 diff, F3 center, F3 s, float b2, float d2, float bz, float dz,
-                 int temp2, int tempk, uint32_t Nxy, float orl, uint32_t N1, uint32_t N2p, uint32_t N3p,
+                 int temp2, int tempk, uint32_t Nxy, const TubeW& orl, uint32_t N1, uint32_t N2p, uint32_t N3p,
                  bool preStep, int& k, int ku, V& visit) {
     int uu = 0;
     bool brk = false;
@@ -1318,14 +1343,21 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
     if ((diff.x == 0.f && diff.y == 0.f) || !std::isfinite(diff.x) || !std::isfinite(diff.y) || !std::isfinite(diff.z))
         return;
     const float L = std::sqrt(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
-    const float orl = L * o.width;
-    float w = 1.f;
-    if (g.mode != oracle::CT_FLAT) {
-        if (g.atten) w *= g.atten[lo];
-        if (g.norm) w *= g.norm[lo];
-        if (g.scat) w *= g.scat[lo];
-        w *= g.global_factor;
-    }
+    const bool vol = o.V != nullptr;
+    const TubeW orl{vol ? L : L * o.width, &o};
+    // projector 3: TotV = L * (1/pi) * R^2 (projectorType123.cl:505); the LOR weight of an axis-parallel ray
+    // is the corrections' product times 1 / TotV (:641-643), that of any other ray starts from 1 / TotV (:952)
+    const float inv_v = vol ? 1.f / (L * (float)M_1_PI * o.width * o.width) : 1.f;
+    auto chain = [&](float w) {
+        if (g.mode != oracle::CT_FLAT) {
+            if (g.atten) w *= g.atten[lo];
+            if (g.norm) w *= g.norm[lo];
+            if (g.scat) w *= g.scat[lo];
+            w *= g.global_factor;
+        }
+        return w;
+    };
+    const float w = chain(1.f);
     const uint32_t Nxy = g.Nx * g.Ny;
     int tempk_b = 0;
     F3 center{0.f, 0.f, 0.f};
@@ -1367,7 +1399,7 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
         } else {
             return;
         }
-        temp = w;
+        temp = vol ? w * inv_v : w;
         for (int ii = apuX1; ii <= apuX2; ii++) {
             center.x = b1 + d1 * (float)ii + d1 / 2.f;
             sweep(o, ii, diff, center, s, b2, d2, g.bz, g.dz, indO, tempk, Nxy, orl, N1, N2p, N3p, false, tempk_b, 0,
@@ -1419,7 +1451,7 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
         NNx = g.Nx;
         N1 = g.Ny; N2p = 1u; N3p = g.Nx;
     }
-    temp = w;
+    temp = vol ? chain(inv_v) : w;
     float tx0_a = tx0, ty0_a = ty0, tz0_a = tz0;
     int tempi_a = tempi, tempj_a = tempj, tempk_a = tempk;
     bool swept = false;     // tempiOld holds a slab index
@@ -1485,7 +1517,8 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
 
 static orth::OGeo make_ogeo(const uint32_t* n3, const float* d3, const float* b3, uint32_t size_x, uint32_t size_y,
                             int32_t mode, int64_t plane_offset, float global_factor, const float* norm,
-                            const float* atten, const float* scat, float width, int32_t flags) {
+                            const float* atten, const float* scat, float width, int32_t flags, const float* vol,
+                            int64_t vol_len, const float* vol_s) {
     orth::OGeo o;
     o.g = make_geo<float>(n3, d3, b3, size_x, size_y, mode, plane_offset, 0.f, 0, global_factor, norm, atten, 2, scat);
     o.bmaxx = (float)((double)b3[0] + (double)n3[0] * (double)d3[0]);
@@ -1494,16 +1527,24 @@ static orth::OGeo make_ogeo(const uint32_t* n3, const float* d3, const float* b3
     o.width = width;
     o.cxy = (flags & 1) != 0;
     o.cz = (flags & 2) != 0;
+    if (vol) {   // vol_s: bmin, bmax, Vmax
+        o.V = vol;
+        o.vlen = (uint32_t)vol_len;
+        o.bmin = vol_s[0];
+        o.bmax = vol_s[1];
+        o.vmax = vol_s[2];
+    }
     return o;
 }
 
 #define ORACLE_ORTH_ARGS                                                                                              \
     const uint32_t *n3, const float *d3, const float *b3, uint32_t size_x, uint32_t size_y, int32_t mode,             \
-        int64_t plane_offset, float global_factor, const float *norm, const float *atten, const float *scat, float width, int32_t flags, int64_t nMeas,     \
+        int64_t plane_offset, float global_factor, const float *norm, const float *atten, const float *scat, float width, int32_t flags, \
+        const float *vol, int64_t vol_len, const float *vol_s, int64_t nMeas,                                          \
         const float *x, const float *z
 
 extern "C" int oracle_orth_fp_f32(ORACLE_ORTH_ARGS, const float* img, float* y, int nthreads) {
-    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags);
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags, vol, vol_len, vol_s);
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t lo = 0; lo < nMeas; lo++) {
@@ -1515,7 +1556,7 @@ extern "C" int oracle_orth_fp_f32(ORACLE_ORTH_ARGS, const float* img, float* y, 
 }
 
 extern "C" int oracle_orth_bp_f32(ORACLE_ORTH_ARGS, const float* yin, float* rhs, float* sens, int nthreads) {
-    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags);
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags, vol, vol_len, vol_s);
     set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t lo = 0; lo < nMeas; lo++) {
@@ -1539,7 +1580,7 @@ extern "C" int oracle_orth_bp_f32(ORACLE_ORTH_ARGS, const float* yin, float* rhs
 // Per-LOR voxel lists in visiting order (indices, tube weights w) and the LOR weight; returns the total count.
 extern "C" int64_t oracle_orth_trace_f32(ORACLE_ORTH_ARGS, int64_t* counts, int64_t cap, int64_t* idx_out,
                                          float* w_out, float* lor_w) {
-    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags);
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags, vol, vol_len, vol_s);
     int64_t total = 0;
     for (int64_t lo = 0; lo < nMeas; lo++) {
         int64_t c = 0;

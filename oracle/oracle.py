@@ -68,7 +68,8 @@ def lib():
         _lib.oracle_bdd_bp_f32.argtypes = bgeo + [_f32p, _f32p, C.c_void_p, C.c_int]
         _lib.oracle_bdd_geom5_f32.argtypes = bgeo + [_f32p]
         ogeo = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, C.c_float, C.c_void_p,
-                C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_int64, _f32p, _f32p]
+                C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, _f32p,
+                _f32p]
         _lib.oracle_orth_fp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_int]
         _lib.oracle_orth_bp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_void_p, C.c_int]
         _lib.oracle_orth_trace_f32.argtypes = ogeo + [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -390,10 +391,12 @@ class OrthCase:
 
     ``flags``: bit 0 transaxial row sweep (tube_width_xy > 0), bit 1 axial sweep (tube_width_z > 0);
     ``width`` is the kernel's tube width (tube_width_z); ``atten``, ``scat``: optional per-measurement
-    attenuation factors and multiplicative corrections (the weight is atten * norm * scat * global_factor)."""
+    attenuation factors and multiplicative corrections (the weight is atten * norm * scat * global_factor).
+    ``volume``: (V, bmin, bmax, Vmax) selects the volume-of-intersection projector (projector type 3),
+    ``width`` then being the tube radius."""
 
     def __init__(self, grid, size_x, size_y, x, z, width, flags=3, mode=MODE_PET, plane_offset=0,
-                 global_factor=1.0, norm=None, atten=None, scat=None):
+                 global_factor=1.0, norm=None, atten=None, scat=None, volume=None):
         self.n3 = np.ascontiguousarray(grid.n3())
         self.d3 = grid.d3(np.float32)
         self.b3 = grid.b3(np.float32)
@@ -406,13 +409,20 @@ class OrthCase:
         self.norm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
         self.atten = None if atten is None else np.ascontiguousarray(atten, dtype=np.float32)
         self.scat = None if scat is None else np.ascontiguousarray(scat, dtype=np.float32)
+        self.vol = self.vol_s = None
+        if volume is not None:
+            self.vol = np.ascontiguousarray(volume[0], dtype=np.float32)
+            self.vol_s = np.array(volume[1:4], dtype=np.float32)
 
     def _geo(self, nmeas):
         normp = None if self.norm is None else self.norm.ctypes.data
         atnp = None if self.atten is None else self.atten.ctypes.data
         scp = None if self.scat is None else self.scat.ctypes.data
+        volp = None if self.vol is None else self.vol.ctypes.data
+        vsp = None if self.vol is None else self.vol_s.ctypes.data
+        nv = 0 if self.vol is None else self.vol.size
         return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
-                self.global_factor, normp, atnp, scp, self.width, self.flags, nmeas, self.x, self.z]
+                self.global_factor, normp, atnp, scp, self.width, self.flags, volp, nv, vsp, nmeas, self.x, self.z]
 
     def fp(self, img, nmeas, nthreads=0):
         y = np.zeros(nmeas, dtype=np.float32)

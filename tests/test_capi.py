@@ -179,3 +179,22 @@ def test_unet_size_checks():
     assert lib.omg_head_forward(p, p, p, p, p, 0, 16, None) == _lib.OMG_ERR_ARG
     assert lib.omg_embed(p, p, 4, 4, 4, 0, C.c_float(1.0), None) == _lib.OMG_ERR_ARG
     assert lib.omg_conv3d_forward(p, 16, None, 0, p, None, p, 16, 0, 4, 4, 0, None) == _lib.OMG_ERR_ARG
+
+
+def test_create_rejects_bad_volume_table():
+    """Projector 3 needs its table and 0 <= bmin < bmax, Vmax > 0; it runs on PET data."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    V = np.ones(8, np.float32)
+    ok = dict(projector_type=3, orth_width=1.0, orth_flags=3, vol_V=V.ctypes.data, vol_len=V.size, vol_bmin=0.1,
+              vol_bmax=2.0, vol_vmax=4.0)
+    checks = [
+        (dict(ok, vol_V=None, vol_len=0), "needs the volume table"),
+        (dict(ok, vol_bmin=2.0), "vol_bmin < vol_bmax"),
+        (dict(ok, vol_vmax=0.0), "vol_vmax > 0"),
+        (dict(ok, vol_bmin=float("nan")), "vol_bmin < vol_bmax"),
+    ]
+    for kw, msg in checks:
+        d, keep = _desc(**kw)
+        assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG, msg
+        assert msg in _lib.last_error(), (msg, _lib.last_error())
