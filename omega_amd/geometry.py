@@ -540,3 +540,22 @@ def voxel_volume_table(tube_radius: float, voxel_radius: float, dx: float, dy: f
     if return_grid:
         return V, float(np.min(b)), float(bmax), float(vmax), vr, b
     return V, float(np.min(b)), float(bmax), float(vmax), vr
+
+
+def listmode_sensitivity_tables(crystal_xy, ring_z):
+    """LOR tables of the list-mode sensitivity image (the reference's -DSENS kernel: every ordered pair of
+    crystals (i, j) in every ordered pair of rings (a, b), projectorType123.cl:283-300,426-432,
+    getDetectorCoordinatesRaw general_opencl_functions.h:1527-1543) laid out as a sinogram: bin i + j * det
+    holds (x_i, y_i, x_j, y_j) and plane a * rings + b holds (z_a, z_b), so measurement
+    i + j * det + (a * rings + b) * det^2 is the kernel's work-item (i, j, a * rings + b).
+    crystal_xy: (det, 2) crystal centres; ring_z: (rings,) ring positions. Returns (x table, z table)."""
+    c = np.ascontiguousarray(crystal_xy, dtype=np.float32).reshape(-1, 2)
+    zr = np.ascontiguousarray(ring_z, dtype=np.float32).ravel()
+    det = c.shape[0]
+    i = np.tile(np.arange(det), det)
+    j = np.repeat(np.arange(det), det)
+    x = np.stack([c[i, 0], c[i, 1], c[j, 0], c[j, 1]], axis=1).ravel()
+    a = np.repeat(np.arange(zr.size), zr.size)
+    b = np.tile(np.arange(zr.size), zr.size)
+    z = np.stack([zr[a], zr[b]], axis=1).ravel()
+    return np.ascontiguousarray(x), np.ascontiguousarray(z)

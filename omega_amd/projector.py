@@ -521,6 +521,51 @@ class projectorClass:
                   C.c_void_p(f.data_ptr()), sp, _stream_ptr(torch))
         return f
 
+    def computeSensitivityImage(self, crystal_xy, ring_z, max_lors: int = 1 << 25):
+        """Sensitivity image of list-mode data (compute_sensitivity_image, proj.py:281; the reference's
+        -DSENS backprojection, ProjectorClass.h:3217-3264,3913-3978): the backprojection of ones over
+        every ordered crystal pair of every ordered ring pair, with this projector's type, tube settings,
+        rays and global correction factor (per-LOR corrections do not apply to these LORs).
+        crystal_xy: (det_per_ring, 2) crystal centres; ring_z: (rings,) ring positions. The LORs run
+        as a sinogram (geometry.listmode_sensitivity_tables), at most max_lors per launch."""
+        import torch
+        if not self.projectorInitialized:
+            self.initProj()
+        if self.CT:
+            raise ValueError("The sensitivity image of list-mode data is for PET")
+        if self._tof():
+            raise ValueError("The list-mode sensitivity image is computed without TOF")
+        x, z = G.listmode_sensitivity_tables(crystal_xy, ring_z)
+        det = x.size // 4
+        det = int(round(math.sqrt(det)))
+        planes = z.size // 2
+        d, keep = self._descriptor()
+        d.mode = _lib.MODE_PET_SINO
+        d.size_x = d.size_y = det
+        d.n_proj = planes
+        d.x, d.x_len = x.ctypes.data, x.size
+        d.z, d.z_len = z.ctypes.data, z.size
+        d.norm, d.norm_len = None, 0
+        d.meas_index, d.meas_index_len = None, 0
+        d.atten_mode, d.atten, d.atten_len = 0, None, 0
+        d.scat, d.scat_len = None, 0
+        geom = Geometry(d, keep + [x, z])
+        dev = torch.device("cuda")
+        sens = torch.zeros(int(self.N[0]), dtype=torch.float32, device=dev)
+        per = det * det
+        chunk = max(1, min(planes, max_lors // per))
+        ones = torch.ones(per * chunk, dtype=torch.float32, device=dev)
+        try:
+            for p0 in range(0, planes, chunk):
+                n = per * min(chunk, planes - p0)
+                # sinogram calls start at a plane: `first` is p0
+                _lib.call("omg_backward_project", geom.handle, p0, n, C.c_void_p(ones.data_ptr()),
+                          C.c_void_p(sens.data_ptr()), C.c_void_p(0), _stream_ptr(torch))
+        finally:
+            torch.cuda.synchronize()
+            geom.close()
+        return sens
+
     def T(self):
         self.trans = True
         return self
