@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_traffic.txt"),
                     help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
-                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet3d_autograd", "pet2d_tof"],
+                    choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet2d_volume", "pet3d_autograd",
+                             "pet2d_tof"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     ap.add_argument("--voxel-bp", action="store_true",
@@ -62,6 +63,7 @@ def main():
 
     if args.workload != "pet2d_osem":
         {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth,
+         "pet2d_volume": lambda a: bench_orth(a, volume=True),
          "pet3d_autograd": bench_autograd, "pet2d_tof": bench_tof}[args.workload](args)
         return
 
@@ -346,9 +348,11 @@ def bench_cgls(args):
     })
 
 
-def bench_orth(args):
+def bench_orth(args, volume=False):
     """2-D plane PET OSEM with the orthogonal distance-based projector (tube width 2.5 mm, transaxial
-    and axial sweeps) on the pet2d_osem geometry. value = subset-iterations/s (8 subsets x 1 iteration per step)."""
+    and axial sweeps) on the pet2d_osem geometry. value = subset-iterations/s (8 subsets x 1 iteration per step).
+    volume: the volume-of-intersection projector instead (projector type 3, the operator API's defaults
+    tube_radius = 1 mm and voxel_radius = 1: voxels count out to 2.77 mm from the LOR)."""
     import numpy as np
     import torch
     from omega_amd import _lib, geometry as G, workloads
@@ -361,8 +365,12 @@ def bench_orth(args):
     p = projectorClass()
     p.Nx, p.Ny, p.Nz = 128, 128, 63
     p.FOVa_x, p.FOVa_y, p.axial_fov = 320.0, 320.0, 157.5
-    p.projector_type = 2
-    p.tube_width_xy = p.tube_width_z = 2.5
+    if volume:
+        p.projector_type = 3
+        p.tube_radius, p.voxel_radius = 1.0, 1.0
+    else:
+        p.projector_type = 2
+        p.tube_width_xy = p.tube_width_z = 2.5
     p.x, p.z = sc.xy_table(), sc.z
     p.Ndist, p.Nang = sc.n_radial, sc.n_angles
     p.subsets, p.subsetType = 8, 8
@@ -392,12 +400,15 @@ def bench_orth(args):
     s_s = a.elapsed_time(b) * 1e-3
     byt = 12.0 * ntube + 8.0 * M0
     _common(args, {
-        "metric": "OSEM subset-iters/s, orthogonal distance-based projector", "value": round(p.subsets * args.steps / dt, 2),
+        "metric": "OSEM subset-iters/s, " + ("volume-of-intersection" if volume else "orthogonal distance-based")
+                  + " projector", "value": round(p.subsets * args.steps / dt, 2),
         "unit": "subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
         "data": "synthetic (seeded ellipsoid phantom, Poisson data at 5e6 counts)",
-        "config": {"workload": "pet2d_osem_orth", "image": [128, 128, 63], "sinogram": [128, 160, 63],
-                   "tube_width_mm": 2.5, "sweeps": "transaxial+axial", "subsets": 8,
-                   "tube_visits_subset0": ntube},
+        "config": {"workload": "pet2d_osem_volume" if volume else "pet2d_osem_orth", "image": [128, 128, 63],
+                   "sinogram": [128, 160, 63],
+                   **({"tube_radius_mm": 1.0, "voxel_radius": 1.0, "table_entries": int(p.V.size)} if volume
+                      else {"tube_width_mm": 2.5}),
+                   "sweeps": "transaxial+axial", "subsets": 8, "tube_visits_subset0": ntube},
         "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_orth_osem + update", "avg_launch_us": round(s_s * 1e6, 1),

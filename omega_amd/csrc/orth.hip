@@ -24,6 +24,8 @@
 // One lane follows one LOR. FMAD of the reference is fmaf here; everything else is evaluated as written
 // (the library is built with -ffp-contract=off and IEEE division/sqrt), so a forward projection is
 // bit-identical to the oracle's restatement of the same kernel.
+#include <type_traits>
+
 #include "geom.hpp"
 
 namespace omg {
@@ -149,20 +151,27 @@ struct Tube {
     float b2, d2, bz, dz, orl;
     uint32_t Nxy, N1, N2p, N3p, Nz;
     bool cxy, cz;
-    const float* V;     // projector 3 (null: projector 2)
+    static constexpr bool kVol = false;
+};
+
+// Projector 3's tube: the look-up table on top. A type of its own keeps these fields and the table
+// branch out of projector 2's kernels (their registers and occupancy stay as they were).
+struct VolTube : Tube {
+    static constexpr bool kVol = true;
+    const float* V;
     uint32_t vlen;
     float bmin, bmax, vmax;
 };
 
 // One voxel: true when it lies outside the cut-off (the row sweep stops there).
-template <class V>
-__device__ __forceinline__ bool o_voxel(const Tube& t, int tempi, int uu, int zz, const F3& l, float cy, V& visit) {
+template <class T, class V>
+__device__ __forceinline__ bool o_voxel(const T& t, int tempi, int uu, int zz, const F3& l, float cy, V& visit) {
     const float x0 = __builtin_fmaf(-1.f, t.s.y, cy);
     const float y1 = __builtin_fmaf(t.diff.z, x0, -l.y);
     const float z1 = __builtin_fmaf(-t.diff.x, x0, l.z);
     const float norm2 = __builtin_fmaf(l.x, l.x, __builtin_fmaf(y1, y1, z1 * z1));
     float w;
-    if (t.V) {
+    if constexpr (T::kVol) {
         // volume of intersection (opencl_functions_orth3D.h:132-140): orl = L, so w is the axis distance;
         // past the table's end (which the reference leaves open) the voxel counts 0
         if (norm2 > t.bmax * t.bmax * t.orl * t.orl) return true;
@@ -184,8 +193,8 @@ __device__ __forceinline__ bool o_voxel(const Tube& t, int tempi, int uu, int zz
 
 // Rows of one z plane. `recompute`: the upward rows take their centre from the row index (first z loop);
 // otherwise the centre accumulates from the start row (second z loop), as in the reference.
-template <class V>
-__device__ __forceinline__ void o_rows(const Tube& t, int tempi, int temp2, int zz, const F3& l, float centerY0,
+template <class T, class V>
+__device__ __forceinline__ void o_rows(const T& t, int tempi, int temp2, int zz, const F3& l, float centerY0,
                                        bool recompute, int& uu, int& uu1, int& uu2, bool& brk, V& visit) {
     const int maxXY = (int)t.N1;
     if (t.cxy) {
@@ -213,8 +222,8 @@ __device__ __forceinline__ void o_rows(const Tube& t, int tempi, int temp2, int 
 }
 
 // orthDistance3D: the tube's voxels in slab `tempi` (centre cx), rows from temp2, planes from tempk.
-template <class V>
-__device__ int o_sweep(const Tube& t, int tempi, float cx, int temp2, int tempk, bool preStep, int& k, int ku,
+template <class T, class V>
+__device__ int o_sweep(const T& t, int tempi, float cx, int temp2, int tempk, bool preStep, int& k, int ku,
                        V& visit) {
     int uu = 0, uu1 = 0, uu2 = 0;
     bool brk = false;
@@ -258,12 +267,12 @@ __device__ int o_sweep(const Tube& t, int tempi, float cx, int temp2, int tempk,
 
 // One LOR: visit(ind, w) for every voxel of its tube. Returns the LOR weight temp (0: ray skipped); with
 // temp_out, temp is also stored there before the first visit (for callers whose visit applies it).
-template <class V>
+template <bool VOL, class V>
 __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out = nullptr) {
     const SidArgs& g = oa.a;
     Ray r;
     fetch_ray(g, lo, r);
-    Tube t;
+    std::conditional_t<VOL, VolTube, Tube> t;
     t.s = F3{r.xs, r.ys, r.zs};
     const F3 d{r.xd, r.yd, r.zd};
     t.diff = F3{d.x - t.s.x, d.y - t.s.y, d.z - t.s.z};
@@ -271,13 +280,15 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp
     F3& diff = t.diff;
     if ((diff.x == 0.f && diff.y == 0.f) || !isfinite(diff.x) || !isfinite(diff.y) || !isfinite(diff.z)) return 0.f;
     const float L = sqrtf(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
-    const bool vol = oa.vol != nullptr;
+    constexpr bool vol = VOL;
     t.orl = vol ? L : L * oa.width;
-    t.V = oa.vol;
-    t.vlen = oa.vol_len;
-    t.bmin = oa.bmin;
-    t.bmax = oa.bmax;
-    t.vmax = oa.vmax;
+    if constexpr (VOL) {
+        t.V = oa.vol;
+        t.vlen = oa.vol_len;
+        t.bmin = oa.bmin;
+        t.bmax = oa.bmax;
+        t.vmax = oa.vmax;
+    }
     auto chain = [&](float w) {
         if (g.atten) w *= g.atten[lo];
         if (g.norm) w *= g.norm[lo];
@@ -438,15 +449,17 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp
     return w;
 }
 
+template <bool VOL>
 __global__ __launch_bounds__(kOrthBlock) void k_orth_fp(OrthArgs oa, const float* __restrict__ img,
                                                         float* __restrict__ y) {
     const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
     if (lo >= oa.a.nmeas) return;
     float ax = 0.f;
-    const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
+    const float temp = orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
     y[lo] = ax * temp;
 }
 
+template <bool VOL>
 __global__ __launch_bounds__(kOrthBlock) void k_orth_bp(OrthArgs oa, const float* __restrict__ yin,
                                                         float* __restrict__ rhs, float* __restrict__ sens) {
     const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
@@ -455,40 +468,42 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_bp(OrthArgs oa, const float
     if (r == 0.f && sens == nullptr) return;
     float temp = 0.f;   // the LOR weight, set by orth_lor before its first visit
     if (sens) {
-        orth_lor(oa, lo, [&](int64_t ind, float w) {
+        orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) {
             const float e = w * temp;
             unsafeAtomicAdd(rhs + ind, r * e);
             unsafeAtomicAdd(sens + ind, e);
         }, &temp);
     } else {
-        orth_lor(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); }, &temp);
+        orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); }, &temp);
     }
 }
 
 // FP, ratio r = y / (A x + eps) (NaN/Inf -> eps), BP of r; optional sensitivity.
+template <bool VOL>
 __global__ __launch_bounds__(kOrthBlock) void k_orth_osem(OrthArgs oa, const float* __restrict__ img,
                                                           const float* __restrict__ meas, float* __restrict__ rhs,
                                                           float* __restrict__ sens, float epps) {
     const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
     if (lo >= oa.a.nmeas) return;
     float ax = 0.f;
-    const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
+    const float temp = orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) { ax += w * img[ind]; });
     if (temp == 0.f) return;
     float r = meas[lo] / ratio_den(oa.a, lo, ax * temp, epps);
     if (isnan(r) || isinf(r)) r = epps;
     if (r == 0.f && sens == nullptr) return;
     if (sens) {
-        orth_lor(oa, lo, [&](int64_t ind, float w) {
+        orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) {
             const float e = w * temp;
             unsafeAtomicAdd(rhs + ind, r * e);
             unsafeAtomicAdd(sens + ind, e);
         });
     } else {
-        orth_lor(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); });
+        orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); });
     }
 }
 
 // Per-LOR voxel lists: counts only (offsets == null) or indices and tube weights at offsets[lo].
+template <bool VOL>
 __global__ __launch_bounds__(kOrthBlock) void k_orth_trace(OrthArgs oa, int32_t* __restrict__ counts,
                                                            const int64_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ idx, float* __restrict__ wout) {
@@ -497,9 +512,9 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_trace(OrthArgs oa, int32_t*
     int32_t c = 0;
     if (offsets) {
         int64_t o = offsets[lo];
-        orth_lor(oa, lo, [&](int64_t ind, float w) { idx[o] = (uint32_t)ind; wout[o] = w; o++; c++; });
+        orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) { idx[o] = (uint32_t)ind; wout[o] = w; o++; c++; });
     } else {
-        orth_lor(oa, lo, [&](int64_t, float) { c++; });
+        orth_lor<VOL>(oa, lo, [&](int64_t, float) { c++; });
     }
     counts[lo] = c;
 }
@@ -548,7 +563,7 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
         for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
             const int64_t lo = base + l;
             float ax = 0.f;
-            const float temp = orth_lor(oa, lo, [&](int64_t ind, float w) {
+            const float temp = orth_lor<false>(oa, lo, [&](int64_t ind, float w) {
                 const uint64_t j = (uint64_t)(ind - sbase);
                 if (j < Nyx) ax += w * lds[j];
             });
@@ -602,12 +617,12 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
                 if (a.scat) temp *= a.scat[lo];
                 temp *= a.global_factor;
                 if (pass == 0) {
-                    orth_lor(oa, lo, [&](int64_t ind, float w) {
+                    orth_lor<false>(oa, lo, [&](int64_t ind, float w) {
                         const uint64_t j = (uint64_t)(ind - sbase);
                         if (j < Nyx) lds_add_fixed(acc + j, r * (w * temp), up_r);
                     });
                 } else {
-                    orth_lor(oa, lo, [&](int64_t ind, float w) {
+                    orth_lor<false>(oa, lo, [&](int64_t ind, float w) {
                         const uint64_t j = (uint64_t)(ind - sbase);
                         if (j < Nyx) lds_add_fixed(acc + j, w * temp, up_s);
                     });
@@ -733,7 +748,8 @@ int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, 
     if (g->orth_planar && !g->force_generic)
         return orth_launch_planar<true, false, false>(g, first, n, img, nullptr, y, nullptr, nullptr, 0.f, s);
     const OrthArgs oa = make_orth_args(g, first, n);
-    hipLaunchKernelGGL(k_orth_fp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, y);
+    auto* kern = oa.vol ? k_orth_fp<true> : k_orth_fp<false>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, y);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -745,7 +761,8 @@ int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin,
         return orth_launch_planar<false, true, false>(g, first, n, nullptr, yin, nullptr, rhs, nullptr, 0.f, s);
     }
     const OrthArgs oa = make_orth_args(g, first, n);
-    hipLaunchKernelGGL(k_orth_bp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, yin, rhs, sens);
+    auto* kern = oa.vol ? k_orth_bp<true> : k_orth_bp<false>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, yin, rhs, sens);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -758,7 +775,8 @@ int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, con
     }
     OrthArgs oa = make_orth_args(g, first, n);
     oa.a.add = add;
-    hipLaunchKernelGGL(k_orth_osem, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, meas, rhs, sens,
+    auto* kern = oa.vol ? k_orth_osem<true> : k_orth_osem<false>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, meas, rhs, sens,
                        epps);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -767,7 +785,8 @@ int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, con
 int orth_trace(const omg_geom* g, int64_t first, int64_t n, int32_t* counts, const int64_t* offsets, uint32_t* idx,
                float* w, hipStream_t s) {
     const OrthArgs oa = make_orth_args(g, first, n);
-    hipLaunchKernelGGL(k_orth_trace, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, counts, offsets, idx,
+    auto* kern = oa.vol ? k_orth_trace<true> : k_orth_trace<false>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, counts, offsets, idx,
                        w);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
