@@ -1,4 +1,7 @@
-"""Builds libomega_hip.so from omega_amd/csrc/*.hip with hipcc for gfx950 (in-tree)."""
+"""Builds libomega_hip.so from omega_amd/csrc/*.hip with hipcc for gfx950 (in-tree).
+
+Each translation unit is compiled to an object in omega_amd/_build/ (in parallel), then linked.
+"""
 from __future__ import annotations
 
 import glob
@@ -6,17 +9,20 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libomega_hip.so")
+INCLUDE = os.path.join(HERE, "..", "include")
 
 # -ffp-contract=off + IEEE div/sqrt: Siddon index lists and forward projections are bit-identical
 #   to the reference CPU build;
 # -munsafe-fp-atomics: float atomicAdd lowers to global_atomic_add_f32 (no CAS loop);
 # -packed-fp32-ops disabled: no v_pk_*_f32 emitted.
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -34,19 +40,45 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source into omega_amd/libomega_hip.so (always recompiles)."""
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(HERE, "..", "include")] + sources() + ["-o", tmp]
+def _headers_mtime() -> float:
+    hs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(INCLUDE, "*.h"))
+    return max((os.path.getmtime(h) for h in hs), default=0.0)
+
+
+def _compile(src: str, incremental: bool, verbose: bool):
+    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
+    if incremental and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(src), _headers_mtime()):
+        return obj, None
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, "-c", src, "-o", obj + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
+        return obj, r.stdout + r.stderr
+    os.replace(obj + ".tmp", obj)
+    return obj, None
+
+
+def build(verbose: bool = False, incremental: bool = False, jobs: int = 0) -> str:
+    """Compile every HIP source and link omega_amd/libomega_hip.so. By default every source is
+    recompiled; incremental=True reuses objects newer than their source and the headers."""
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(lambda s: _compile(s, incremental, verbose), sources()))
+    errors = [e for _, e in results if e]
+    if errors:
+        sys.stderr.write("\n".join(errors))
         raise RuntimeError("hipcc failed building libomega_hip.so")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + [o for o, _ in results] + ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("linking libomega_hip.so failed")
     os.replace(tmp, LIB)
     return LIB
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))
