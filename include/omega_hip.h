@@ -170,6 +170,20 @@ int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, int64_t n_mea
                                  const float* additive, float* S, int32_t compute_sens, float epps, float beta,
                                  float gamma, int32_t positivity, void* stream);
 
+/* omg_osem_subset_iteration_ex without allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). The same kernels in the same order, with every work buffer (OSL gradient, rhs image, slice map,
+ * the direct-plane kernels' per-workgroup partial slices, the voxel-driven path's per-LOR buffers) carved
+ * out of one library-owned device array of 64 MiB (16,777,216 floats) that each GPU holds until the process
+ * ends. A call that needs more (2 N + Nz floats for an image of N voxels and Nz slices, plus, on the
+ * direct-plane path, 2 x planes x workgroups-per-plane x Nx Ny floats) fails with OMG_ERR_ARG before any
+ * launch. Every call on one GPU shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other threads, other handles) race; the caller serialises them. The current device must be the
+ * geometry's. Nothing in the array has to start at zero. */
+int omg_osem_subset_iteration_noalloc(const omg_geom* g, int64_t first, int64_t n_meas, float* x,
+                                      const float* meas, const float* additive, float* S, int32_t compute_sens,
+                                      float epps, float beta, float gamma, int32_t positivity, void* stream);
+
 /* Kernel timing of the projector launches with HIP events, the counterpart of the reference's
  * verbose >= 3 event timers (source/cpp/ProjectorClass.h:3594-3599). omg_timing_enable(1) records
  * every following projector kernel launch (Siddon FP/BP/fused kernels) until omg_timing_enable(0);

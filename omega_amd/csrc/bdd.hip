@@ -301,8 +301,8 @@ int bdd_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, f
     const size_t need = WX * H * a.Ny + WY * H * a.Nx;
     int rc = scratch_reserve(const_cast<omg_geom*>(g), need);
     if (rc) return rc;
-    float* IY = g->scratch;
-    float* IX = g->scratch + WX * H * a.Ny;
+    float* IY = scratch_ptr(g);
+    float* IX = scratch_ptr(g) + WX * H * a.Ny;
     hipLaunchKernelGGL(k_sat_vol_z, dim3(grid_for(a.Nx, 256), a.Ny), dim3(256), 0, s, a, img, IY, IX);
     OMG_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_sat_rows, dim3(grid_for((int64_t)(H - 1) * a.Ny, 256)), dim3(256), 0, s, IY, (uint32_t)WX,
@@ -324,7 +324,7 @@ int bdd_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, 
     const size_t W = a.nRows + 1, H = a.nCols + 1;
     int rc = scratch_reserve(const_cast<omg_geom*>(g), W * H * nproj);
     if (rc) return rc;
-    float* S = g->scratch;
+    float* S = scratch_ptr(g);
     hipLaunchKernelGGL(k_sat_sino_cols, dim3(grid_for(a.nRows, 256), nproj), dim3(256), 0, s, a, yin, S);
     OMG_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_sat_rows, dim3(grid_for((int64_t)(H - 1) * nproj, 256)), dim3(256), 0, s, S, (uint32_t)W,

@@ -40,7 +40,32 @@ int check_range(const omg_geom* g, int64_t first, int64_t n) {
     return OMG_OK;
 }
 
+static thread_local const FixedRegion* t_fixed = nullptr;
+
+FixedRegionGuard::FixedRegionGuard(const FixedRegion* r) { t_fixed = r; }
+FixedRegionGuard::~FixedRegionGuard() { t_fixed = nullptr; }
+
+// Floats of the work part of a fixed region: [dU: N][rhs: N][slice map: Nz int32], rounded up to 64
+// floats (256 B) so the scratch part that follows keeps the region's alignment.
+static size_t fixed_work_floats(const omg_geom* g) {
+    const size_t w = 2 * (size_t)g->d.Nx * g->d.Ny * g->d.Nz + g->d.Nz;
+    return (w + 63) & ~(size_t)63;
+}
+
+static int fixed_need(const omg_geom* g, size_t floats, const char* what) {
+    const size_t need = fixed_work_floats(g) + floats;
+    OMG_REQUIRE(need <= t_fixed->floats, "this call needs %zu floats of work memory (%s), more than the %zu floats "
+                "(%zu MiB) of the no-allocation scratch", need, what, t_fixed->floats,
+                t_fixed->floats * sizeof(float) >> 20);
+    return OMG_OK;
+}
+
+float* scratch_ptr(const omg_geom* g) { return t_fixed ? t_fixed->base + fixed_work_floats(g) : g->scratch; }
+float* work_ptr(const omg_geom* g) { return t_fixed ? t_fixed->base : g->work; }
+float* q_ptr(const omg_geom* g) { return t_fixed ? t_fixed->base + fixed_work_floats(g) : g->vbp_q; }
+
 int scratch_reserve(omg_geom* g, size_t floats) {
+    if (t_fixed) return fixed_need(g, floats, "partial images");
     if (g->scratch_floats >= floats) return OMG_OK;
     if (g->scratch) {
         OMG_CHECK_HIP(hipDeviceSynchronize());
@@ -54,6 +79,7 @@ int scratch_reserve(omg_geom* g, size_t floats) {
 }
 
 int work_reserve(omg_geom* g, uint32_t N) {
+    if (t_fixed) return fixed_need(g, 0, "image work buffers");
     const size_t need = 2 * (size_t)N + g->d.Nz;
     if (g->work_floats >= need) return OMG_OK;
     if (g->work) {
@@ -64,6 +90,20 @@ int work_reserve(omg_geom* g, uint32_t N) {
     }
     OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->work), need * sizeof(float)));
     g->work_floats = need;
+    return OMG_OK;
+}
+
+int q_reserve(omg_geom* g, size_t floats) {
+    if (t_fixed) return fixed_need(g, floats, "per-LOR q buffers");
+    if (g->vbp_q_floats >= floats) return OMG_OK;
+    if (g->vbp_q) {
+        OMG_CHECK_HIP(hipDeviceSynchronize());
+        OMG_CHECK_HIP(hipFree(g->vbp_q));
+        g->vbp_q = nullptr;
+        g->vbp_q_floats = 0;
+    }
+    OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->vbp_q), floats * sizeof(float)));
+    g->vbp_q_floats = floats;
     return OMG_OK;
 }
 

@@ -103,11 +103,31 @@ int atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipSt
 // Validates a [first, first + n) measurement range against the tables; returns OMG_OK or OMG_ERR_ARG.
 int check_range(const omg_geom* g, int64_t first, int64_t n);
 
-// Ensures g->scratch holds at least `floats` floats (hipMalloc on growth; synchronises the device then).
-int scratch_reserve(omg_geom* g, size_t floats);
+// Work buffers of one call come from the handle (grown with hipMalloc on demand) unless the calling
+// thread holds a fixed region (the no-allocation entry point, noalloc.hip): then they are carved out of
+// it as [work: 2N + Nz floats, rounded up to 64][scratch / planar q buffers: the rest], and a call that
+// needs more than the region holds fails with OMG_ERR_ARG instead of allocating.
+struct FixedRegion {
+    float* base;      // device address of the region
+    size_t floats;    // its size
+};
+// Installs `r` for the calling thread until the guard ends (nested use is not supported).
+struct FixedRegionGuard {
+    explicit FixedRegionGuard(const FixedRegion* r);
+    ~FixedRegionGuard();
+};
 
-// Ensures g->work holds the fused-iteration buffers for an image of N voxels.
+// Ensures scratch_ptr(g) holds at least `floats` floats (hipMalloc on growth; synchronises the device then).
+int scratch_reserve(omg_geom* g, size_t floats);
+float* scratch_ptr(const omg_geom* g);
+
+// Ensures work_ptr(g) holds the fused-iteration buffers for an image of N voxels.
 int work_reserve(omg_geom* g, uint32_t N);
+float* work_ptr(const omg_geom* g);
+
+// Ensures q_ptr(g) holds `floats` floats for the voxel-driven planar path's per-LOR q buffers.
+int q_reserve(omg_geom* g, size_t floats);
+float* q_ptr(const omg_geom* g);
 
 // ---- direct-plane (planar) launches, shared by the Siddon and orthogonal projectors (siddon.hip) ----
 // Workgroups of a planar launch over [first, first + n): n_planes planes, `chunk` LORs per workgroup,

@@ -705,8 +705,8 @@ static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const
     if (BP) {
         const int rc = scratch_reserve(const_cast<omg_geom*>(g), (size_t)sh.blocks * Nyx * (SENS ? 2 : 1));
         if (rc) return rc;
-        part = g->scratch;
-        part2 = SENS ? g->scratch + (size_t)sh.blocks * Nyx : nullptr;
+        part = scratch_ptr(g);
+        part2 = SENS ? scratch_ptr(g) + (size_t)sh.blocks * Nyx : nullptr;
     }
     hipEvent_t stop;
     timing_begin(s, &stop);
@@ -718,7 +718,7 @@ static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const
     return OMG_OK;
 }
 
-// Fused planar FP -> ratio -> BP of [first, first + n) into the partial slices of g->scratch (the caller
+// Fused planar FP -> ratio -> BP of [first, first + n) into the partial slices of scratch_ptr(g) (the caller
 // applies planar_update). Returns OMG_ERR_ARG when the planar kernels do not apply.
 int orth_planar_fused(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas,
                       int32_t compute_sens, float epps, hipStream_t s, const float* add) {

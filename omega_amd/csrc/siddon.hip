@@ -428,7 +428,7 @@ PlanarShape planar_shape(const omg_geom* g, int64_t n) {
     return s;
 }
 
-// Adds the partial slices in g->scratch (layout of planar_shape(g, n): slot b = group * n_planes + plane;
+// Adds the partial slices in scratch_ptr(g) (layout of planar_shape(g, n): slot b = group * n_planes + plane;
 // sensitivity partials after all rhs partials) of launch [first, first + n) into rhs (and sens).
 int planar_reduce_partials(const omg_geom* g, int64_t first, int64_t n, float* rhs, float* sens, hipStream_t s) {
     const PlanarShape sh = planar_shape(g, n);
@@ -436,11 +436,11 @@ int planar_reduce_partials(const omg_geom* g, int64_t first, int64_t n, float* r
     const uint32_t groups = sh.blocks / sh.n_planes;
     const SidArgs a = make_sid_args(g, first, n);
     const dim3 rg(grid_for(Nyx, 256), sh.n_planes);
-    hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, g->scratch, rhs);
+    hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups, scratch_ptr(g), rhs);
     OMG_CHECK_LAUNCH();
     if (sens) {
         hipLaunchKernelGGL(k_planar_reduce, rg, dim3(256), 0, s, a, sh.n_planes, groups,
-                           g->scratch + (size_t)sh.blocks * Nyx, sens);
+                           scratch_ptr(g) + (size_t)sh.blocks * Nyx, sens);
         OMG_CHECK_LAUNCH();
     }
     return OMG_OK;
@@ -476,8 +476,8 @@ static int launch_planar(const omg_geom* g, int64_t first, int64_t n, const floa
         const size_t need = (size_t)sh.blocks * Nyx * (SENS ? 2 : 1);
         int rc = scratch_reserve(const_cast<omg_geom*>(g), need);
         if (rc) return rc;
-        part = g->scratch;
-        part2 = SENS ? g->scratch + (size_t)sh.blocks * Nyx : nullptr;
+        part = scratch_ptr(g);
+        part2 = SENS ? scratch_ptr(g) + (size_t)sh.blocks * Nyx : nullptr;
     }
     hipEvent_t stop;
     timing_begin(s, &stop);
@@ -656,7 +656,7 @@ static int generic_osem(const omg_geom* g, int64_t first, int64_t n, const float
     if (split_layout(g)) {   // y-fastest accumulators only (see generic_fp)
         const int rc = scratch_reserve(const_cast<omg_geom*>(g), (size_t)N * (sens ? 2 : 1));
         if (rc) return rc;
-        float* t = g->scratch;
+        float* t = scratch_ptr(g);
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for((int64_t)N * (sens ? 2 : 1), 256)), dim3(256), 0, s, t,
                            N * (sens ? 2u : 1u), 0.f);
         OMG_CHECK_LAUNCH();
@@ -735,8 +735,8 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
         const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
         rc = scratch_reserve(const_cast<omg_geom*>(g), (size_t)N * (sens ? 2 : 1));
         if (rc) return rc;
-        float* rt = g->scratch;
-        float* st = sens ? g->scratch + N : nullptr;
+        float* rt = scratch_ptr(g);
+        float* st = sens ? scratch_ptr(g) + N : nullptr;
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for((int64_t)N * (sens ? 2 : 1), 256)), dim3(256), 0, s, rt,
                            N * (sens ? 2u : 1u), 0.f);
         OMG_CHECK_LAUNCH();
@@ -824,15 +824,14 @@ __global__ __launch_bounds__(256) void k_generic_update(uint32_t N, float* __res
 // True when the planes [first, first + n_planes) map to distinct image slices.
 // EM / OSL update of image x from the partial slices of a planar fused launch [first, first + n):
 // k_planar_update4 when x, S, dU and the partials are 16-byte aligned, else the slice map and
-// k_planar_update. gm->work must hold the fused-iteration buffers (work_reserve).
+// k_planar_update. work_ptr(g) must hold the fused-iteration buffers (work_reserve).
 int planar_update(const omg_geom* g, int64_t first, int64_t n_meas, float* x, float* S, const float* dU,
                   int32_t compute_sens, float epps, int32_t positivity, hipStream_t s) {
-    omg_geom* gm = const_cast<omg_geom*>(g);
     const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
     const PlanarShape sh = planar_shape(g, n_meas);
     const uint32_t Nyx = g->d.Nx * g->d.Ny;
-    const float* part = g->scratch;
-    const float* part2 = g->scratch + (size_t)sh.blocks * Nyx;
+    const float* part = scratch_ptr(g);
+    const float* part2 = scratch_ptr(g) + (size_t)sh.blocks * Nyx;
     const uint32_t groups = sh.blocks / sh.n_planes;
     SidArgs a = make_sid_args(g, first, n_meas);
     auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
@@ -859,7 +858,7 @@ int planar_update(const omg_geom* g, int64_t first, int64_t n_meas, float* x, fl
         OMG_CHECK_LAUNCH();
         return OMG_OK;
     }
-    int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
+    int32_t* map = reinterpret_cast<int32_t*>(work_ptr(g) + 2 * (size_t)N);
     hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a, sh.n_planes, map);
     OMG_CHECK_LAUNCH();
     dim3 grid(grid_for(N, 256));
@@ -900,9 +899,13 @@ extern "C" int omg_osem_subset_iteration(const omg_geom* g, int64_t first, int64
                                         positivity, stream);
 }
 
-extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, int64_t n_meas, float* x,
-                                            const float* meas, const float* additive, float* S, int32_t compute_sens,
-                                            float epps, float beta, float gamma, int32_t positivity, void* stream) {
+namespace omg {
+
+// The subset iteration behind omg_osem_subset_iteration_ex and omg_osem_subset_iteration_noalloc (which
+// holds a fixed work region while it calls this).
+int omg_osem_subset_iteration_impl(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas,
+                                   const float* additive, float* S, int32_t compute_sens, float epps, float beta,
+                                   float gamma, int32_t positivity, void* stream) {
     int rc = siddon_common_checks(g, first, n_meas);
     if (rc) return rc;
     OMG_REQUIRE(x && meas && S, "null pointer argument");
@@ -915,7 +918,7 @@ extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, in
     if (rc) return rc;
     float* dU = nullptr;
     if (beta != 0.f) {
-        dU = gm->work;
+        dU = work_ptr(g);
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, dU, N, 0.f);
         OMG_CHECK_LAUNCH();
         rc = omg_rdp_gradient(x, dU, g->d.Nx, g->d.Ny, g->d.Nz, gamma, epps, beta, 0, stream);
@@ -926,7 +929,7 @@ extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, in
         unique_slices(g, first, (uint32_t)(n_meas / per))) {
         if (g->vbp_ok) {
             // scatter-free path: ray-driven FP -> q = r w, voxel-driven BP with the update fused (vbp.hip)
-            int32_t* map = reinterpret_cast<int32_t*>(gm->work + 2 * (size_t)N);
+            int32_t* map = reinterpret_cast<int32_t*>(work_ptr(g) + 2 * (size_t)N);
             SidArgs a = make_sid_args(g, first, n_meas);
             hipLaunchKernelGGL(k_slice_plane, dim3(grid_for(g->d.Nz, 256)), dim3(256), 0, s, a,
                                (uint32_t)(n_meas / per), map);
@@ -952,7 +955,7 @@ extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, in
         return planar_update(g, first, n_meas, x, S, dU, compute_sens, epps, positivity, s);
     }
     // one-lane-per-LOR path: rhs (and S) accumulate with global atomics, then the update
-    float* rhs = gm->work + N;
+    float* rhs = work_ptr(g) + N;
     hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, rhs, N, 0.f);
     OMG_CHECK_LAUNCH();
     if (compute_sens) {
@@ -970,6 +973,15 @@ extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, in
                        (int)(compute_sens != 0), (int)positivity);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
+}
+
+}  // namespace omg
+
+extern "C" int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, int64_t n_meas, float* x,
+                                            const float* meas, const float* additive, float* S, int32_t compute_sens,
+                                            float epps, float beta, float gamma, int32_t positivity, void* stream) {
+    return omg_osem_subset_iteration_impl(g, first, n_meas, x, meas, additive, S, compute_sens, epps, beta, gamma,
+                                          positivity, stream);
 }
 
 extern "C" int omg_siddon_trace(const omg_geom* g, int64_t first, int64_t n_meas, int32_t* counts,

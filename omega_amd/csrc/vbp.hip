@@ -229,17 +229,10 @@ int vbp_subset_iteration(omg_geom* g, int64_t first, int64_t n_meas, float* x, c
                          hipStream_t s, const float* add) {
     const uint32_t per = g->d.size_x * g->d.size_y;
     const uint32_t n_planes = (uint32_t)(n_meas / per);
-    const size_t need = 2 * (size_t)n_meas;
-    if (g->vbp_q_floats < need) {
-        OMG_CHECK_HIP(hipDeviceSynchronize());
-        hipFree(g->vbp_q);
-        g->vbp_q = nullptr;
-        g->vbp_q_floats = 0;
-        OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g->vbp_q), need * sizeof(float)));
-        g->vbp_q_floats = need;
-    }
-    float* q = g->vbp_q;
-    float* wq = g->vbp_q + n_meas;
+    const int rc = q_reserve(g, 2 * (size_t)n_meas);
+    if (rc) return rc;
+    float* q = q_ptr(g);
+    float* wq = q + n_meas;
     SidArgs a = make_sid_args(g, first, n_meas);
     a.pitch = g->d.Nx + 1;
     a.add = add;
