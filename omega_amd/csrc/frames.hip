@@ -458,7 +458,7 @@ extern "C" int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_me
 
 extern "C" int omg_frames_loglik(const float* y, const float* yhat, int64_t n, int32_t n_frames, double* out,
                                  void* stream) {
-    OMG_REQUIRE(y && yhat && out && n >= 0 && n_frames >= 1, "bad arguments");
+    OMG_REQUIRE(out && n >= 0 && n_frames >= 1 && (n == 0 || (y && yhat)), "bad arguments");
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_frames_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<float*>(out), (size_t)(2 * n_frames),
                        0.f);

@@ -283,7 +283,7 @@ extern "C" int omg_axpby(float* y, const float* x, int64_t n, float a, float b, 
 }
 
 extern "C" int omg_poisson_loglik(const float* y, const float* yhat, int64_t n, double* out, void* stream) {
-    OMG_REQUIRE(y && yhat && out && n >= 0, "bad argument");
+    OMG_REQUIRE(out && n >= 0 && (n == 0 || (y && yhat)), "bad argument (null pointer or negative length)");
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, s, out);
     OMG_CHECK_LAUNCH();

@@ -344,20 +344,67 @@ def osem(grid, size_x, size_y, xy, z_subset_ordered, nmeas_planes, meas, n_iter,
     return x
 
 
-def cgls(case, meas, n_iter, nmeas=None):
+def ct_osem(cases, meas_parts, n_iter, x0, flat, epps=1e-5, beta=0.0, gamma=2.0, positivity=True, shape=None,
+            subsets_to_run=None, nthreads=0):
+    """Transmission (CT) OSEM / OSL-OSEM without randoms in the reference's step order, one oracle case
+    (Siddon or BDD) per subset and the subset's measured counts in ``meas_parts``:
+    S_k = A_k^T y_k in iteration 0 with S_k < epps -> epps (reconstructionAF.h:322-327,1224-1260);
+    yhat = A_k x, yhat < 0 -> 0, r = exp(-yhat) (computeForwardProject.h:80-88); rhs = A_k^T r;
+    x = x / S_k * (flat * rhs) (subiterStep.h:117-118) or x / (S_k + dU) * (flat * rhs) (:223-224),
+    dU = beta * grad RDP(x) (``shape`` = (Nx, Ny, Nz)); positivity x < epps -> epps. The arithmetic is in
+    the cases' dtype; the exponential is evaluated in double and rounded once, as the library does."""
+    dt = np.dtype(getattr(cases[0], "dtype", np.float32))
+    f = dt.type
+
+    def fp(case, v, n):
+        return case.fp(v, nthreads=nthreads) if isinstance(case, BDDCase) else case.fp(v, n, nthreads)
+
+    def bp(case, v, n):
+        return case.bp(v, nthreads=nthreads) if isinstance(case, BDDCase) else case.bp(v, n, nthreads=nthreads)
+
+    x = np.ascontiguousarray(x0, dtype=dt).copy()
+    S = [None] * len(cases)
+    done = 0
+    for it in range(n_iter):
+        for k, case in enumerate(cases):
+            if subsets_to_run is not None and done >= subsets_to_run:
+                return x
+            y = np.asarray(meas_parts[k], dtype=dt)
+            n = y.size
+            if S[k] is None:
+                s = bp(case, y, n)
+                S[k] = np.where(s < f(epps), f(epps), s).astype(dt)
+            yhat = fp(case, x, n)
+            yhat = np.where(yhat < 0, f(0), yhat).astype(dt)
+            r = np.exp(-yhat.astype(np.float64)).astype(dt)   # exp in double, rounded once
+            rhs = bp(case, r, n)
+            num = (f(flat) * rhs).astype(dt)
+            if beta != 0.0:
+                dU = rdp_gradient(x, shape, gamma, epps, beta, dtype=dt, nthreads=nthreads)
+                v = (x / (S[k] + dU) * num).astype(dt)
+            else:
+                v = (x / S[k] * num).astype(dt)
+            if positivity:
+                v = np.where(v < f(epps), f(epps), v).astype(dt)
+            x = v
+            done += 1
+    return x
+
+
+def cgls(case, meas, n_iter, nmeas=None, nthreads=0):
     """CGLS recurrences (functions.hpp:2315-2340, computeForwardProject.h:197-206, algorithms.h:243-269)
     with the oracle projector of ``case`` (float64 vectors)."""
     n = meas.size if nmeas is None else nmeas
     y = np.asarray(meas, dtype=case.dtype)
     f = np.zeros(case.grid.N, dtype=case.dtype)
     r = y.copy()
-    p = case.bp(r, n)
+    p = case.bp(r, n, nthreads=nthreads)
     gamma = float(np.dot(p.astype(np.float64), p.astype(np.float64)))
     for _ in range(n_iter):
-        q = case.fp(p, n)
+        q = case.fp(p, n, nthreads)
         alpha = gamma / float(np.dot(q.astype(np.float64), q.astype(np.float64)))
         r = r - alpha * q
-        s = case.bp(r, n)
+        s = case.bp(r, n, nthreads=nthreads)
         gamma_new = float(np.dot(s.astype(np.float64), s.astype(np.float64)))
         f = f + alpha * p
         p = s + (gamma_new / gamma) * p
@@ -365,7 +412,7 @@ def cgls(case, meas, n_iter, nmeas=None):
     return f
 
 
-def pdhg(cases, meas_parts, n_iter, tau, sigma=1.0, theta=1.0, epps=1e-5):
+def pdhg(cases, meas_parts, n_iter, tau, sigma=1.0, theta=1.0, epps=1e-5, nthreads=0):
     """PDHG with subsets, least-squares data term (CBCT_custom_algorithm_example_PyTorchCUDA.py:235-264,
     PDHGUpdate auxKernels.cl:2034-2059), one oracle case per subset."""
     N = cases[0].grid.N
@@ -376,9 +423,9 @@ def pdhg(cases, meas_parts, n_iter, tau, sigma=1.0, theta=1.0, epps=1e-5):
     for _ in range(n_iter):
         for k, c in enumerate(cases):
             n = meas_parts[k].size
-            ax = c.fp(f.astype(c.dtype), n).astype(np.float64)
+            ax = c.fp(f.astype(c.dtype), n, nthreads).astype(np.float64)
             pl = (p[k] + sigma * (ax - meas_parts[k])) / (1.0 + sigma)
-            dg = c.bp((pl - p[k]).astype(c.dtype), n).astype(np.float64)
+            dg = c.bp((pl - p[k]).astype(c.dtype), n, nthreads=nthreads).astype(np.float64)
             p[k] = pl
             g1 = g1 + dg
             g = g1 + (theta * S) * dg
@@ -391,12 +438,14 @@ class OrthCase:
 
     ``flags``: bit 0 transaxial row sweep (tube_width_xy > 0), bit 1 axial sweep (tube_width_z > 0);
     ``width`` is the kernel's tube width (tube_width_z); ``atten``, ``scat``: optional per-measurement
-    attenuation factors and multiplicative corrections (the weight is atten * norm * scat * global_factor).
+    attenuation factors and multiplicative corrections: the weight is atten * norm * scat * global_factor
+    for factors made from an attenuation image, norm * scat * global_factor * atten for factors given per
+    LOR (``atten_lor``; norm * scat * atten * global_factor on the axis-parallel path).
     ``volume``: (V, bmin, bmax, Vmax) selects the volume-of-intersection projector (projector type 3),
     ``width`` then being the tube radius."""
 
     def __init__(self, grid, size_x, size_y, x, z, width, flags=3, mode=MODE_PET, plane_offset=0,
-                 global_factor=1.0, norm=None, atten=None, scat=None, volume=None):
+                 global_factor=1.0, norm=None, atten=None, scat=None, volume=None, atten_lor=False):
         self.n3 = np.ascontiguousarray(grid.n3())
         self.d3 = grid.d3(np.float32)
         self.b3 = grid.b3(np.float32)
@@ -404,7 +453,8 @@ class OrthCase:
         self.size_x, self.size_y = int(size_x), int(size_y)
         self.x = np.ascontiguousarray(x, dtype=np.float32)
         self.z = np.ascontiguousarray(z, dtype=np.float32)
-        self.width, self.flags = float(width), int(flags)
+        # flags bit 2: `atten` holds given per-LOR factors (-DATNM order), not factors made from an image
+        self.width, self.flags = float(width), int(flags) | (4 if atten_lor else 0)
         self.mode, self.plane_offset, self.global_factor = mode, int(plane_offset), global_factor
         self.norm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
         self.atten = None if atten is None else np.ascontiguousarray(atten, dtype=np.float32)

@@ -88,11 +88,11 @@ def _osl(case, y, beta, eps, n_iter, dtype):
     S = case.bp(np.ones(n, dtype), n)
     S = O._sens_clamp64(S, eps) if f64 else O.sens_clamp(S, eps)
     x = np.ones(case.grid.N, dtype)
-    for _ in range(n_iter):
-        yhat = case.fp(x, n)
+    for _ in range(n_iter):   # one thread: thousands of tiny calls
+        yhat = case.fp(x, n, nthreads=1)
         r = O._ratio64(yd, yhat, eps) if f64 else O.ratio(yd, yhat, eps)
-        rhs = case.bp(r, n)
-        dU = O.rdp_gradient(x, SHAPE, GAMMA, eps, beta, dtype=dtype)
+        rhs = case.bp(r, n, nthreads=1)
+        dU = O.rdp_gradient(x, SHAPE, GAMMA, eps, beta, dtype=dtype, nthreads=1)
         x = O._update64(x, S, dU, rhs, eps, True) if f64 else O.osl_update(x, S, dU, rhs, eps)
     return x
 

@@ -23,7 +23,7 @@ def test_cgls_reaches_least_squares():
     rng = np.random.default_rng(0)
     xt = rng.random(A.shape[1])
     y = A @ xt + 0.01 * rng.standard_normal(n)
-    f = O.cgls(c, y, 1000)   # rank-deficient A (rank 149 of 280 columns): CGLS converges in ~rank steps, slowly
+    f = O.cgls(c, y, 1000, nthreads=1)   # rank-deficient A (rank 149 of 280 columns): CGLS converges in ~rank steps, slowly
     ls = np.linalg.lstsq(A, y, rcond=None)[0]
     # compare in the data space (A may have a null space on voxels no ray crosses)
     assert np.linalg.norm(A @ f - A @ ls) <= 1e-6 * np.linalg.norm(A @ ls)
@@ -35,6 +35,6 @@ def test_pdhg_reaches_nonnegative_least_squares():
     rng = np.random.default_rng(1)
     y = A @ rng.random(A.shape[1])
     L = np.linalg.norm(A, 2) ** 2
-    f = O.pdhg([c], [y], 3000, tau=1.0 / L, sigma=1.0, theta=1.0, epps=0.0)
+    f = O.pdhg([c], [y], 3000, tau=1.0 / L, sigma=1.0, theta=1.0, epps=0.0, nthreads=1)
     ref = nnls(A, y, maxiter=5000)[0]
     assert np.linalg.norm(A @ f - A @ ref) <= 2e-3 * np.linalg.norm(A @ ref)
