@@ -170,6 +170,19 @@ int omg_osem_subset_iteration_ex(const omg_geom* g, int64_t first, int64_t n_mea
                                  const float* additive, float* S, int32_t compute_sens, float epps, float beta,
                                  float gamma, int32_t positivity, void* stream);
 
+/* One OSEM (beta == 0) or OSL-OSEM with the RDP prior (beta != 0) subset iteration of transmission (CT) data on
+ * image x, in place, with any projector of the geometry: if compute_sens, S = A_k^T meas with S < epps -> epps
+ * (stored; else S is read); yhat = A_k x, yhat < 0 -> 0; r = exp(-yhat); rhs = A_k^T r; OSEM
+ * x = x / S * (flat * rhs), OSL x = x / (S + dU) * (flat * rhs) with dU = beta * grad RDP(x; gamma, epps), zero
+ * boundary; positivity x < epps -> epps when positivity != 0. meas: the measured counts of the subset's
+ * projections; flat: the blank-scan (flat-field) value. Replaces the CT branch of reconstructionAF for OSEM /
+ * OSL-OSEM without randoms (source/cpp/reconstructionAF.h:322-327,1224-1260; computeForwardProject.h:76-90;
+ * subiterStep.h:111-121,219-231). Uses work buffers of the geometry handle: calls on one handle must not
+ * overlap. */
+int omg_ct_osem_subset_iteration(const omg_geom* g, int64_t first, int64_t n_meas, float* x, const float* meas,
+                                 float* S, int32_t compute_sens, float flat, float epps, float beta, float gamma,
+                                 int32_t positivity, void* stream);
+
 /* omg_osem_subset_iteration_ex without allocation. No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid
  * hipMalloc). The same kernels in the same order, with every work buffer (OSL gradient, rhs image, slice map,

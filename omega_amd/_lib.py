@@ -78,6 +78,7 @@ SIGNATURES = {
     "omg_osem_data_step_ex": [_P, _I64, _I64, _P, _P, _P, _P, _P, _F, _P],
     "omg_osem_subset_iteration_ex": [_P, _I64, _I64, _P, _P, _P, _P, _I32, _F, _F, _F, _I32, _P],
     "omg_osem_subset_iteration_noalloc": [_P, _I64, _I64, _P, _P, _P, _P, _I32, _F, _F, _F, _I32, _P],
+    "omg_ct_osem_subset_iteration": [_P, _I64, _I64, _P, _P, _P, _I32, _F, _F, _F, _F, _I32, _P],
     "omg_frames_sensitivity": [_P, _I64, _I64, _P, _F, _P],
     "omg_frames_forward": [_P, _I64, _I64, _I32, _P, _P, _I64, _P],
     "omg_frames_loglik": [_P, _P, _I64, _I32, _P, _P],

@@ -289,16 +289,22 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp
         t.bmax = oa.bmax;
         t.vmax = oa.vmax;
     }
-    auto chain = [&](float w) {
-        if (g.atten) w *= g.atten[lo];
+    // corrections in the reference kernel's order: factors made from an attenuation image first (-DATN,
+    // projectorType123.cl:975-976); given per-LOR factors (-DATNM) after the global factor (:986-988), or
+    // before it on the axis-parallel path (perpendicular_elements, general_opencl_functions.h:1722-1734)
+    auto chain = [&](float w, bool perp) {
+        if (g.atten && !g.atten_lor) w *= g.atten[lo];
         if (g.norm) w *= g.norm[lo];
         if (g.scat) w *= g.scat[lo];
-        return w * g.global_factor;
+        if (g.atten && g.atten_lor && perp) w *= g.atten[lo];
+        w *= g.global_factor;
+        if (g.atten && g.atten_lor && !perp) w *= g.atten[lo];
+        return w;
     };
     // projector 3: TotV = L * (1/pi) * R^2 (projectorType123.cl:505); an axis-parallel ray's weight is the
     // corrections' product times 1 / TotV (:641-643), any other ray's starts from 1 / TotV (:952)
     const float inv_v = vol ? 1.f / (L * (float)M_1_PI * oa.width * oa.width) : 1.f;
-    const float w1 = chain(1.f);
+    const float w1 = chain(1.f, true);
     t.Nxy = g.Nx * g.Ny;
     t.Nz = g.Nz;
     t.bz = g.bz;
@@ -395,7 +401,7 @@ __device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp
         NNx = g.Nx;
         t.N1 = g.Ny; t.N2p = 1u; t.N3p = g.Nx;
     }
-    const float w = vol ? chain(inv_v) : w1;
+    const float w = chain(inv_v, false);
     if (temp_out) *temp_out = w;
     int tempi_a = tempi, tempj_a = tempj, tempk_a = tempk;
     bool swept = false;

@@ -120,6 +120,7 @@ class projectorClass:
         self.flipAttImageXY = False
         self.flipAttImageZ = False
         self.epps = 1e-5
+        self.flat = 0.0                 # CT: blank-scan (flat-field) value of the measured counts (proj.py:359)
         # PET scanner (used when no coordinate tables are given)
         self.diameter = 0.0
         self.det_per_ring = 0
@@ -368,10 +369,10 @@ class projectorClass:
     def _tof(self) -> bool:
         """TOF is used when TOF_bins > 1 and TOF_bins_used equals it (0 meaning all bins); other
         TOF_bins_used values ask the reference to sum the bins, i.e. a non-TOF reconstruction
-        (proj.py:681-691, :1208-1210)."""
-        pt = self.projector_type % 10 if self.projector_type > 10 else self.projector_type
+        (proj.py:681-691, :1208-1210). The reference accepts it for projector types 1-3 (proj.py:1247-1248);
+        here initProj refuses it for any type but 1."""
         used = int(self.TOF_bins_used) or int(self.TOF_bins)
-        return int(self.TOF_bins) > 1 and used == int(self.TOF_bins) and not self.CT and pt in (1, 2)
+        return int(self.TOF_bins) > 1 and used == int(self.TOF_bins) and not self.CT
 
     def _tof_tables(self):
         """sigma_x (mm) and the bin centres (mm) the kernels get (proj.py:681-705, init errors :1247-1254)."""
@@ -525,7 +526,8 @@ class projectorClass:
         """Sensitivity image of list-mode data (compute_sensitivity_image, proj.py:281; the reference's
         -DSENS backprojection, ProjectorClass.h:3217-3264,3913-3978): the backprojection of ones over
         every ordered crystal pair of every ordered ring pair, with this projector's type, tube settings,
-        rays and global correction factor (per-LOR corrections do not apply to these LORs).
+        rays, global correction factor and attenuation image (per-event tables -- normalisation,
+        multiplicative corrections, attenuation factors -- do not describe these LORs).
         crystal_xy: (det_per_ring, 2) crystal centres; ring_z: (rings,) ring positions. The LORs run
         as a sinogram (geometry.listmode_sensitivity_tables), at most max_lors per launch."""
         import torch
@@ -547,7 +549,11 @@ class projectorClass:
         d.z, d.z_len = z.ctypes.data, z.size
         d.norm, d.norm_len = None, 0
         d.meas_index, d.meas_index_len = None, 0
-        d.atten_mode, d.atten, d.atten_len = 0, None, 0
+        if d.atten_mode == 2:
+            raise ValueError("The list-mode sensitivity image takes attenuation from an attenuation image "
+                             "(CTAttenuation); per-event attenuation factors do not describe its LORs")
+        # an attenuation image stays: the library computes every sensitivity LOR's factor from it, as the
+        # reference's -DSENS kernel integrates it along each LOR (projectorType123.cl, ProjectorClass.h:3937-3953)
         d.scat, d.scat_len = None, 0
         geom = Geometry(d, keep + [x, z])
         dev = torch.device("cuda")

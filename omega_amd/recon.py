@@ -12,6 +12,11 @@ Restates the subset loop of the reference's built-in reconstruction
         OSL-OSEM: dU = beta * grad RDP(x);  x = x / (S_k + dU) * rhs    (priors.h:447-451; subiterStep.h:219-231)
         positivity x < epps -> epps                                      (reconstructionAF.h:1461-1465)
 
+For transmission (CT) data the same loop follows the reference's CT branch (reconstructionAF.h:322-327,
+1224-1260; computeForwardProject.h:76-90; subiterStep.h:111-121,219-231): S_k = A_k^T y_k in iteration 0,
+r = exp(-max(A_k x, 0)), rhs = A_k^T r, x = x / S_k * (flat * rhs) or x / (S_k + dU) * (flat * rhs), with
+``flat`` the blank-scan value (omg_ct_osem_subset_iteration).
+
 A subset iteration is one C-ABI call (omg_osem_subset_iteration): the fused FP -> ratio -> BP
 projector kernel and one update kernel (plus the RDP gradient kernel for OSL-OSEM). Sensitivity
 images are kept per subset (compute_norm_matrix = 2, reconstructionAF.h:313-317) and frozen after
@@ -35,12 +40,14 @@ class OSEMRunner:
     """OSEM / OSL-OSEM on a :class:`omega_amd.projectorClass` with subset-ordered data on the device."""
 
     def __init__(self, proj, meas, epps: float = 1e-5, beta: float = 0.0, gamma: float = 2.0,
-                 enforce_positivity: bool = True, rdp_boundary: int = 0, stream=None, additive=None):
+                 enforce_positivity: bool = True, rdp_boundary: int = 0, stream=None, additive=None,
+                 flat: float = 0.0):
         """``additive``: optional randoms (+ scatter) estimate, a float32 device tensor in the layout of
         ``meas`` (subset order), added to the forward projection in the ratio (randoms_correction,
         computeForwardProject.h:66-69). With TOF (``proj.nBins`` > 1) each subset's measurements are
         nBins blocks of the subset's LORs, and ``additive`` holds one value per LOR, added to every bin
-        (af::tile, computeForwardProject.h:66-67)."""
+        (af::tile, computeForwardProject.h:66-67). ``flat``: the blank-scan value of transmission (CT) data
+        (projectorClass.flat; required for CT, whose ``meas`` are the measured counts)."""
         import torch
         self.torch = torch
         self.proj = proj
@@ -66,6 +73,13 @@ class OSEMRunner:
             if self.additive.dtype != torch.float32 or not self.additive.is_cuda or \
                     self.additive.numel() != int(proj.nTotMeas[-1]):
                 raise ValueError("additive must be a float32 device tensor with one value per LOR")
+        self.ct = bool(proj.CT)
+        self.flat = float(flat if flat else getattr(proj, "flat", 0.0))
+        if self.ct:
+            if not self.flat > 0.0:
+                raise ValueError("OSEM of CT data needs the blank-scan value flat > 0")
+            if self.additive is not None:
+                raise ValueError("OSEM of CT data is provided without a randoms / scatter term")
         self.rdp_boundary = int(rdp_boundary)
         dev = self.meas.device
         self.S = torch.zeros((proj.subsets, self.N), dtype=torch.float32, device=dev)
@@ -84,12 +98,18 @@ class OSEMRunner:
         n = int(proj.nMeasSubset[k, 0])
         off = int(proj.nTotMeas[k])
         yk = self.meas[off * self.nb:(off + n) * self.nb]
+        if self.ct:
+            _lib.call("omg_ct_osem_subset_iteration", self.g, first, n, _p(x), _p(yk), _p(self.S[k]),
+                      int(not self.sens_done), self.flat, self.epps, self.beta, self.gamma, self.pos, self._s())
+            return
         ak = None if self.additive is None else self.additive[off:off + n]
         _lib.call("omg_osem_subset_iteration_ex", self.g, first, n, _p(x), _p(yk), _p(ak), _p(self.S[k]),
                   int(not self.sens_done), self.epps, self.beta, self.gamma, self.pos, self._s())
 
     def subset_step_unfused(self, x, k: int) -> None:
         """The same iteration through the separate entry points (data step, clamp, RDP, update)."""
+        if self.ct:
+            raise ValueError("the unfused iteration is for emission data")
         proj = self.proj
         first = int(proj.nMeas[k])
         n = int(proj.nMeasSubset[k, 0])

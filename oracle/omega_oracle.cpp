@@ -1088,6 +1088,7 @@ struct OGeo {
     const float* V = nullptr;    // projector 3 (-DVOL): the volume-of-intersection table, else null
     uint32_t vlen = 0;
     float bmin = 0.f, bmax = 0.f, vmax = 0.f;
+    bool atten_lor = false;      // the attenuation factors are given per LOR (-DATNM), not made from an image
 };
 
 // Per-LOR tube scale: orl = L * width (projector 2) or L (projector 3, whose voxel weight is a table value).
@@ -1348,16 +1349,22 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
     // projector 3: TotV = L * (1/pi) * R^2 (projectorType123.cl:505); the LOR weight of an axis-parallel ray
     // is the corrections' product times 1 / TotV (:641-643), that of any other ray starts from 1 / TotV (:952)
     const float inv_v = vol ? 1.f / (L * (float)M_1_PI * o.width * o.width) : 1.f;
-    auto chain = [&](float w) {
+    // Corrections in the reference kernel's order: attenuation factors from an image first (-DATN,
+    // projectorType123.cl:975-976); given per-LOR factors (-DATNM) last, after the global factor
+    // (:986-988), or before it on the axis-parallel path (perpendicular_elements,
+    // general_opencl_functions.h:1722-1734).
+    auto chain = [&](float w, bool perp) {
         if (g.mode != oracle::CT_FLAT) {
-            if (g.atten) w *= g.atten[lo];
+            if (g.atten && !o.atten_lor) w *= g.atten[lo];
             if (g.norm) w *= g.norm[lo];
             if (g.scat) w *= g.scat[lo];
+            if (g.atten && o.atten_lor && perp) w *= g.atten[lo];
             w *= g.global_factor;
+            if (g.atten && o.atten_lor && !perp) w *= g.atten[lo];
         }
         return w;
     };
-    const float w = chain(1.f);
+    const float w = chain(1.f, true);
     const uint32_t Nxy = g.Nx * g.Ny;
     int tempk_b = 0;
     F3 center{0.f, 0.f, 0.f};
@@ -1451,7 +1458,7 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
         NNx = g.Nx;
         N1 = g.Ny; N2p = 1u; N3p = g.Nx;
     }
-    temp = vol ? chain(inv_v) : w;
+    temp = chain(inv_v, false);
     float tx0_a = tx0, ty0_a = ty0, tz0_a = tz0;
     int tempi_a = tempi, tempj_a = tempj, tempk_a = tempk;
     bool swept = false;     // tempiOld holds a slab index
@@ -1527,6 +1534,7 @@ static orth::OGeo make_ogeo(const uint32_t* n3, const float* d3, const float* b3
     o.width = width;
     o.cxy = (flags & 1) != 0;
     o.cz = (flags & 2) != 0;
+    o.atten_lor = (flags & 4) != 0;
     if (vol) {   // vol_s: bmin, bmax, Vmax
         o.V = vol;
         o.vlen = (uint32_t)vol_len;

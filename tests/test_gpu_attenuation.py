@@ -194,18 +194,34 @@ def test_osem_with_attenuation_image(pet2d_atn, generic):
     assert rel_l2(plain, xo) > 1e-3
 
 
-def test_orth_with_factors(cuda):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_orth_with_factors(cuda, mode):
+    """Orthogonal projector with attenuation, normalisation and a global factor, in the reference kernel's
+    order: factors made from an image (mode 1) first, given per-LOR factors (mode 2) after the global
+    factor (projectorType123.cl:975-988); FP bit-identical to the oracle with given factors, within FP_RTOL
+    with factors from the image (exp on the device and on the host)."""
     import torch
     grid, sc = cases.orth_pet2d()
     n = sc.n_radial * sc.n_angles * sc.n_planes
-    fac = _rand(n, 12, lo=0.3, scale=0.7)
-    print(f"[atten orth] n={n}", flush=True)
-    case = O.OrthCase(grid, sc.n_radial, sc.n_angles, sc.xy_table(), sc.z, 3.0, flags=3, atten=fac)
+    norm = _rand(n, 11, lo=0.5, scale=1.0)
+    if mode == 2:
+        atn = fac = _rand(n, 12, lo=0.3, scale=0.7)
+    else:
+        atn = _rand(grid.N, 12, lo=0.0, scale=0.02)
+        fac = O.SiddonCase(grid, sc.n_radial, sc.n_angles, sc.xy_table(), sc.z, atten=atn,
+                           atten_mode=1).atten_factors(n)
+    print(f"[atten orth] mode={mode} n={n}", flush=True)
+    case = O.OrthCase(grid, sc.n_radial, sc.n_angles, sc.xy_table(), sc.z, 3.0, flags=3, atten=fac, norm=norm,
+                      global_factor=1.37, atten_lor=mode == 2)
     for generic in (False, True):
         geom = make_geom(grid, sc.xy_table(), sc.z, sc.n_radial, sc.n_angles, projector_type=2, orth_width=3.0,
-                         orth_flags=3, atten=fac, atten_mode=2, generic=generic)
+                         orth_flags=3, atten=atn, atten_mode=mode, norm=norm, global_factor=1.37, generic=generic)
         img = _rand(grid.N, 13)
-        assert np.array_equal(fp(geom, 0, n, torch.from_numpy(img).to(cuda)).cpu().numpy(), case.fp(img, n))
+        got = fp(geom, 0, n, torch.from_numpy(img).to(cuda)).cpu().numpy()
+        if mode == 2:
+            assert np.array_equal(got, case.fp(img, n))
+        else:   # factors from the image: the device's expf against the host's exp
+            _fp_close(got, case.fp(img, n))
         yin = _rand(n, 14)
         rhs, sens = bp(geom, 0, n, torch.from_numpy(yin).to(cuda), grid.N, sens=True)
         rhs_o, sens_o = case.bp(yin, n, sens=True)
