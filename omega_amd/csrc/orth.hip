@@ -222,8 +222,11 @@ __device__ __forceinline__ void o_rows(const T& t, int tempi, int temp2, int zz,
 }
 
 // orthDistance3D: the tube's voxels in slab `tempi` (centre cx), rows from temp2, planes from tempk.
+// Inlined into every caller, as is orth_lor: as real calls they kept the tube in a stack frame in scratch
+// memory (312-328 B per lane in the one-lane-per-LOR kernels, 3 waves per SIMD); inlined, those kernels
+// use no scratch and ~95 VGPRs (5 waves per SIMD).
 template <class T, class V>
-__device__ int o_sweep(const T& t, int tempi, float cx, int temp2, int tempk, bool preStep, int& k, int ku,
+__device__ __forceinline__ int o_sweep(const T& t, int tempi, float cx, int temp2, int tempk, bool preStep, int& k, int ku,
                        V& visit) {
     int uu = 0, uu1 = 0, uu2 = 0;
     bool brk = false;
@@ -268,7 +271,7 @@ __device__ int o_sweep(const T& t, int tempi, float cx, int temp2, int tempk, bo
 // One LOR: visit(ind, w) for every voxel of its tube. Returns the LOR weight temp (0: ray skipped); with
 // temp_out, temp is also stored there before the first visit (for callers whose visit applies it).
 template <bool VOL, class V>
-__device__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out = nullptr) {
+__device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out = nullptr) {
     const SidArgs& g = oa.a;
     Ray r;
     fetch_ray(g, lo, r);
