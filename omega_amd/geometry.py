@@ -322,15 +322,20 @@ def pet_3d_rings(radius: float = 400.0, n_rings: int = 32, ring_pitch: float = 4
 
 def omega_crystal_ring(blocks_per_ring: int, cryst_per_block: int, cr_p: float, diameter: float,
                        transaxial_multip: int = 1, doi: float = 0.0, flip_image: bool = False,
-                       offangle: float = 0.0):
-    """Transaxial crystal centres (x, y), float32, of a ring of flat blocks without pseudo detectors
-    (detectorCoordinates / computeCoordinates, detcoord.py:326-556): blocks at angles 90, 90 - 360/B, ...
-    degrees; crystals of pitch cr_p along each block; the gap between blocks chosen so that the ring
-    spans the diameter (+ 2 DOI). The first quarter of the ring is stepped crystal by crystal (each
-    position stored as float32 and the next one stepped from it, as there); the rest follows by the
-    ring's mirror symmetries."""
+                       offangle: float = 0.0, pseudo: bool = False):
+    """Transaxial crystal centres (x, y), float32, of a ring of flat blocks (detectorCoordinates /
+    computeCoordinates, detcoord.py:326-556, m-files/computeCoordinates.m): blocks at angles 90,
+    90 - 360/B, ... degrees; crystals of pitch cr_p along each block; the gap between blocks chosen so that
+    the ring spans the diameter (+ 2 DOI). Positions are stepped crystal by crystal, each stored as float32
+    and the next one stepped from it. Without pseudo detectors the first quarter of the ring is stepped
+    and the rest follows by the ring's mirror symmetries. With ``pseudo`` every block ends in a pseudo
+    detector one pitch past its last crystal (det_w_pseudo = B (cryst_per_block + 1) positions), the next
+    block stepping on from the last real crystal, and the whole ring is stepped block by block, as the
+    reference's MATLAB version does (its Python translation steps one block past the end of the array and
+    raises IndexError)."""
     B = int(blocks_per_ring) * int(transaxial_multip)
     c = int(cryst_per_block)
+    cc = c + 1 if pseudo else c          # positions per block
     d = float(diameter) + float(doi) * 2
     deg = np.arange(90, -271, -360 / B)
     h = B // 2
@@ -340,42 +345,55 @@ def omega_crystal_ring(blocks_per_ring: int, cryst_per_block: int, cr_p: float, 
     gy = (d - np.sum(np.abs(w_y))) / np.sum(np.abs(np.sin(np.radians(deg[:h])))) / 2
     cs = lambda k: np.cos(np.radians(deg[k]))  # noqa: E731  (scalar calls, as the reference makes them)
     sn = lambda k: np.sin(np.radians(deg[k]))  # noqa: E731
-    x = np.zeros(B * c, dtype=np.float32)
-    y = np.zeros(B * c, dtype=np.float32)
-    px, py = d / 2., -(c / 2. + 0.5) * cr_p
+    x = np.zeros(B * cc, dtype=np.float32)
+    y = np.zeros(B * cc, dtype=np.float32)
 
     def block_start(k, px, py):
         # first crystal of block k: half a pitch, the gaps on both sides of the block edge, half a pitch
         return (px - (cr_p * 0.5) * cs(k) - gx * cs(k - 1) - gx * cs(k) - (cr_p * 0.5) * cs(k - 1),
                 py + (cr_p * 0.5) * sn(k) + gy * sn(k - 1) + gy * sn(k) + (cr_p * 0.5) * sn(k - 1))
 
-    n = 0
+    def step_blocks(blocks, n, px, py):
+        # every position of the listed blocks; after a block's pseudo detector the walk steps back to its
+        # last real crystal
+        for k in blocks:
+            for m in range(1, cc + 1):
+                if k > 0 and m == 1:
+                    x[n], y[n] = block_start(k, px, py)
+                else:
+                    x[n], y[n] = px - cr_p * cs(k), py + cr_p * sn(k)
+                if pseudo and m == cc:
+                    px, py = x[n] + cr_p * cs(k), y[n] - cr_p * sn(k)
+                else:
+                    px, py = x[n], y[n]
+                n += 1
+        return n, px, py
+
     quarter = -(-B // 4)
-    for k in range(quarter):
-        for m in range(1, c + 1):
-            if k > 0 and m == 1:
-                x[n], y[n] = block_start(k, px, py)
-            else:
-                x[n], y[n] = px - cr_p * cs(k), py + cr_p * sn(k)
-            px, py = x[n], y[n]
-            n += 1
-    n4 = (B * c) // 4
+    n, px, py = step_blocks(range(quarter), 0, d / 2., -(c / 2. + 0.5) * cr_p)
+    n4 = (B * cc) // 4
     if B % 4 == 0:
-        k = quarter
+        k = quarter                      # the top block, stepped without the pseudo step-back
         x[n], y[n] = block_start(k, px, py)
         px, py = x[n], y[n]
         n += 1
-        for _ in range(2, c + 1):
+        for _ in range(2, cc + 1):
             x[n], y[n] = px - cr_p * cs(k), py + cr_p * sn(k)
             px, py = x[n], y[n]
             n += 1
+        if pseudo:
+            px, py = x[n - 1] + cr_p * cs(k), y[n - 1] - cr_p * sn(k)
+            step_blocks(range(quarter + 1, B), n, px, py)
         m = n4
+    elif pseudo:
+        step_blocks(range(quarter, B), n, px, py)
     else:
         m = n4 + c // 2
-    x[n:n + m] = -np.flip(x[:m])
-    y[n:n + m] = np.flip(y[:m])
-    x[n + m:] = np.flip(x[c:(B * c) // 2])
-    y[n + m:] = -y[c:(B * c) // 2]
+    if not pseudo:
+        x[n:n + m] = -np.flip(x[:m])
+        y[n:n + m] = np.flip(y[:m])
+        x[n + m:] = np.flip(x[c:(B * c) // 2])
+        y[n + m:] = -y[c:(B * c) // 2]
     if flip_image:
         x, y = np.flip(x), np.flip(y)
     if offangle != 0:
@@ -389,10 +407,16 @@ def omega_sinogram_xy(x, y, n_dist: int, n_ang: int, ndist_side: int = 1) -> np.
     (a <= b) gets angle ((a + b + det/2 + 1) mod det) / 2 and signed radial offset |b - a - det/2|
     (negative where the pair wraps), endpoints swapped at the sinogram's corners; the pairs within
     n_dist radial bins are kept (ndist_side picks the extra bin of an even n_dist). Angular mashing
-    (n_ang < det / 2) is not provided."""
+    (n_ang < det / 2, detcoord.py:616-620,688-695): the det / 2 angles are binned as without it and every m
+    = det / 2 / n_ang consecutive angle columns are averaged (the reference's block_reduce(..., np.mean):
+    a float32 sum in column order divided by m)."""
     det = len(x)
+    mashing = 1
     if n_ang < det // 2:
-        raise ValueError("Angular mashing (Nang < det_w_pseudo / 2) is not supported by this implementation")
+        mashing = det // n_ang // 2
+        if n_ang * mashing != det // 2:
+            raise ValueError(f"Angular mashing needs Nang to divide det_w_pseudo / 2 ({det // 2}); got Nang = {n_ang}")
+        n_ang = n_ang * mashing
     a, b = np.triu_indices(det)
     L = np.stack([a, b], 1).astype(np.int64)
     hi, lo = b.astype(np.int64), a.astype(np.int64)
@@ -412,6 +436,13 @@ def omega_sinogram_xy(x, y, n_dist: int, n_ang: int, ndist_side: int = 1) -> np.
     grids = [np.zeros((n_dist, n_ang), dtype=np.float32, order="F") for _ in range(4)]
     for g, v in zip(grids, (x[L[:, 0]], y[L[:, 0]], x[L[:, 1]], y[L[:, 1]])):
         np.add.at(g, (i, j), v)
+    if mashing > 1:
+        def mash(g):
+            acc = g[:, 0::mashing].copy()
+            for q in range(1, mashing):
+                acc = (acc + g[:, q::mashing]).astype(np.float32)
+            return (acc / np.float32(mashing)).astype(np.float32)
+        grids = [mash(g) for g in grids]
     return np.ascontiguousarray(np.stack([g.ravel("F") for g in grids], 1).astype(np.float32))
 
 
@@ -424,20 +455,44 @@ def omega_segment_table(rings: int, span: int, ring_difference: int, image_nz: i
     return np.insert(np.repeat(rest, 2), 0, first).astype(np.int64)
 
 
-def omega_sinogram_z(rings: int, cr_pz: float, span: int, ring_difference: int, image_nz: int) -> np.ndarray:
-    """Axial endpoints [zs, zd] per sinogram (sinogramCoordinates3D, detcoord.py:703-896, single layer, no
-    ring gaps). Span 1: every ring pair (a, b), a outer. Span > 1: ring z_a = (a + 1) cr_pz - (rings + 1)
+def omega_sinogram_z(rings: int, cr_pz: float, span: int, ring_difference: int, image_nz: int,
+                     ring_gaps=None, cryst_per_block_axial: int = 0) -> np.ndarray:
+    """Axial endpoints [zs, zd] per sinogram (sinogramCoordinates3D, detcoord.py:703-896, single layer).
+    Span 1: every ring pair (a, b), a outer. Span > 1: ring z_a = (a + 1) cr_pz - (rings + 1)
     cr_pz / 2; sinogram s holds the mean (z_a, z_b) of the ring pairs binned into it: the direct segment
     holds |a - b| <= span / 2 at position a + b, oblique segment i the differences d in
     [k_i, k_i + span - 1] (k_i = ceil(span / 2) + span i), b - a = d in the segment's first copy and a - b = d
-    in its second, at position (a + b) - k_i within the segment."""
-    maxz = float(rings + 1) * cr_pz
+    in its second, at position (a + b) - k_i within the segment.
+    ring_gaps (mm, one per boundary between axial blocks of cryst_per_block_axial rings) lengthen the axial
+    extent by their sum and shift the rings as the reference does: with span > 1 block k's rings move by
+    the cumulative gap before it (the ring pitch itself stretched by sum(gaps) / (rings + 1), :726-748);
+    with span 1 the second ring of a pair moves by its block's cumulative gap and the first by
+    n * gaps[n - 1], n counting the multiples of cryst_per_block_axial + 1 up to the pair's first ring
+    (:872-880)."""
+    gaps = np.asarray(ring_gaps if ring_gaps is not None else [], dtype=np.float64).ravel()
+    has_gaps = gaps.size > 0 and gaps.sum() > 0
+    if has_gaps and cryst_per_block_axial <= 0:
+        raise ValueError("ring gaps need cryst_per_block_axial (rings per axial block)")
+    maxz = float(rings + 1) * cr_pz + (np.sum(gaps) if has_gaps else 0.0)
     if span <= 1:
         t = np.repeat(np.arange(rings), rings).astype(np.float64)
         ring = np.arange(0, cr_pz * rings, cr_pz)
         z = np.stack([cr_pz * t * 1.0, np.tile(ring, rings)], 1)
+        if has_gaps:
+            per_ring = np.repeat(np.insert(np.cumsum(gaps), 0, 0), cryst_per_block_axial)[:rings]
+            n = 0
+            for a in range(rings):
+                z[a * rings:(a + 1) * rings, 1] += per_ring
+                if (a + 1) % (cryst_per_block_axial + 1) == 0:
+                    n += 1
+                if n > 0:
+                    z[a * rings:(a + 1) * rings, 0] += n * gaps[n - 1]
         return z - (maxz / 2 - cr_pz)
     zr = np.linspace(cr_pz, maxz + cr_pz, rings + 2)[:rings]
+    if has_gaps:
+        cg = np.cumsum(gaps)
+        for kk in range(1, int(rings / cryst_per_block_axial)):
+            zr[cryst_per_block_axial * kk:cryst_per_block_axial * (kk + 1)] += cg[kk - 1]
     zr = zr - maxz / 2
     seg = omega_segment_table(rings, span, ring_difference, image_nz)
     off = np.cumsum(seg)

@@ -136,6 +136,10 @@ class projectorClass:
         self.offangle = 0.0
         self.ndist_side = 1
         self.span = 3
+        self.det_w_pseudo = 0            # > det_per_ring: a pseudo detector after every block (proj.py:220)
+        self.ringGaps = np.empty(0, dtype=np.float32)   # axial gaps between blocks of rings (mm, proj.py:201)
+        self.cryst_per_block_axial = 0   # rings per axial block (proj.py:129; default: all rings)
+        self.nLayers = 1                 # crystal layers (only one is provided)
         # CT geometry (used when no coordinate tables are given)
         self.sourceToCRot = 0.0
         self.sourceToDetector = 0.0
@@ -236,26 +240,42 @@ class projectorClass:
         """Coordinate tables of OMEGA's block-ring scanner model from the scanner parameters (detcoord.py
         getCoordinates -> detectorCoordinates, sinogramCoordinates2D/3D; proj.py:654-679 for the derived
         sizes): blocks_per_ring x cryst_per_block crystals of pitch cr_p on a ring of diameter (+ 2 DOI),
-        rings of pitch cr_pz, Ndist x Nang sinograms, span-compressed ring pairs."""
-        det = int(self.blocks_per_ring) * int(self.transaxial_multip) * int(self.cryst_per_block)
+        with a pseudo detector after every block when det_w_pseudo says so; rings of pitch cr_pz, with
+        ringGaps between axial blocks of cryst_per_block_axial rings; Ndist x Nang sinograms, angles mashed
+        when Nang < det_w_pseudo / 2; span-compressed ring pairs."""
+        blocks = int(self.blocks_per_ring) * int(self.transaxial_multip)
+        det = blocks * int(self.cryst_per_block)
         if self.det_per_ring == 0:
             self.det_per_ring = det
         if self.det_per_ring != det:
-            raise ValueError("det_per_ring must equal blocks_per_ring * transaxial_multip * cryst_per_block "
-                             "(pseudo detectors are not supported)")
+            raise ValueError("det_per_ring must equal blocks_per_ring * transaxial_multip * cryst_per_block")
+        if self.det_w_pseudo == 0:
+            self.det_w_pseudo = det
+        pseudo = int(self.det_w_pseudo) > det
+        if pseudo and int(self.det_w_pseudo) != blocks * (int(self.cryst_per_block) + 1):
+            raise ValueError("det_w_pseudo must be det_per_ring (no pseudo detectors) or "
+                             "blocks_per_ring * transaxial_multip * (cryst_per_block + 1) (one after every block)")
         if self.cr_p <= 0 or self.diameter <= 0 or self.rings <= 0 or self.cr_pz <= 0:
             raise ValueError("The scanner model needs cr_p, diameter, rings and cr_pz")
+        if int(self.nLayers) > 1:
+            raise ValueError("Multiple crystal layers (nLayers > 1) are not supported by this implementation")
+        detw = int(self.det_w_pseudo)
         if self.Nang == 0:
-            self.Nang = det // 2
+            self.Nang = detw // 2
         if self.Ndist == 0:
-            self.Ndist = det // 2
+            self.Ndist = detw // 2
+        if int(self.Nang) > detw // 2:
+            raise ValueError(f"Number of sinogram angles can be at most the number of detectors per ring divided "
+                             f"by two ({detw // 2})!")
         if self.ring_difference <= 0:
             self.ring_difference = self.rings - 1
         xc, yc = G.omega_crystal_ring(self.blocks_per_ring, self.cryst_per_block, self.cr_p, self.diameter,
-                                      self.transaxial_multip, self.DOI, self.flip_image, self.offangle)
+                                      self.transaxial_multip, self.DOI, self.flip_image, self.offangle, pseudo)
         self.x = G.omega_sinogram_xy(xc, yc, int(self.Ndist), int(self.Nang), int(self.ndist_side)).ravel()
         nz = int(np.asarray(self.Nz).ravel()[0])
-        z = G.omega_sinogram_z(int(self.rings), float(self.cr_pz), int(self.span), int(self.ring_difference), nz)
+        cpba = int(self.cryst_per_block_axial) or int(self.rings)
+        z = G.omega_sinogram_z(int(self.rings), float(self.cr_pz), int(self.span), int(self.ring_difference), nz,
+                               ring_gaps=self.ringGaps, cryst_per_block_axial=cpba)
         if self.NSinos > 0:
             z = z[: int(self.NSinos)]
         self.z = z.astype(np.float32).ravel()

@@ -5,9 +5,10 @@ source/Python/omegatomo/projector/detcoord.py:326-896).
 Nothing else pins these tables (the reference's tests hold none), so this script imports that module
 (numpy only) from the reference sources and records, for several block-ring scanners: the crystal
 coordinates, the sinogram table [xs, ys, xd, yd] per (radial, angle) bin and the axial table [zs, zd]
-per sinogram (span 1 and span > 1, with the reference's segment table; where the reference raises,
-as its span > 1 path does for span >= 5, the error is recorded). Runs only where the reference
-sources are present:
+per sinogram (span 1 and span > 1, with the reference's segment table; with ring gaps between axial
+blocks; where the reference raises, as its span > 1 path does for span >= 5 and its pseudo-detector and
+angular-mashing paths always do, the error is recorded). Runs only where the reference sources are
+present:
 
     python tests/golden/make_scanner_geometry.py
 """
@@ -37,6 +38,12 @@ AXIAL = [  # rings, cr_pz, span, ring_difference, image Nz
     (10, 3.3, 3, 6, 19),
     (16, 4.0, 7, 15, 31),
 ]
+GAPS = [  # rings, cr_pz, span, ring_difference, image Nz, cryst_per_block_axial, ring gaps (mm)
+    (12, 4.0, 1, 11, 23, 4, [1.5, 2.25]),
+    (12, 4.0, 3, 11, 23, 4, [1.5, 2.25]),
+    (15, 3.3, 1, 14, 29, 5, [0.8, 0.8]),
+    (15, 3.3, 3, 10, 29, 5, [2.0, 0.5]),
+]
 
 
 def main():
@@ -62,6 +69,27 @@ def main():
             out[f"axial{n}_z"] = np.asarray(detcoord.sinogramCoordinates3D(o), np.float64)
         except ValueError as e:   # the reference's span > 1 path fails for span >= 5 (detcoord.py:791)
             out[f"axial{n}_error"] = np.array(str(e))
+    for n, (rings, cr_pz, span, rd, nz, cpba, gaps) in enumerate(GAPS):
+        seg = G.omega_segment_table(rings, span, rd, nz)
+        tot = rings * rings if span == 1 else int(seg.sum())
+        o = types.SimpleNamespace(cr_pz=cr_pz, rings=rings, ringGaps=np.asarray(gaps), nLayers=1, span=span,
+                                  ring_difference=rd, segment_table=seg, TotSinos=tot,
+                                  cryst_per_block_axial=cpba, linear_multip=rings // cpba)
+        out[f"gaps{n}_z"] = np.asarray(detcoord.sinogramCoordinates3D(o), np.float64)
+    # pseudo detectors and angular mashing: the reference's Python raises on both (recorded; the tests pin
+    # these paths by invariants against the tables above)
+    for name, (bpr, cpb, mash) in {"pseudo": (16, 8, 1), "mashing": (16, 8, 2)}.items():
+        det = bpr * cpb
+        detp = bpr * (cpb + 1) if name == "pseudo" else det
+        o = types.SimpleNamespace(cr_p=4.0, diameter=550.0, cryst_per_block=cpb, blocks_per_ring=bpr, DOI=0.0,
+                                  transaxial_multip=1, det_w_pseudo=detp, det_per_ring=det, nLayers=1,
+                                  flip_image=False, offangle=0.0, Nang=detp // 2 // mash, Ndist=64, ndist_side=1)
+        try:
+            x, y = detcoord.detectorCoordinates(o)
+            detcoord.sinogramCoordinates2D(o, x, y)
+            out[f"{name}_error"] = np.array("")
+        except Exception as e:  # noqa: BLE001 -- the reference's own failure is the record
+            out[f"{name}_error"] = np.array(f"{type(e).__name__}: {e}")
     path = os.path.join(HERE, "scanner_geometry.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path}: " + ", ".join(f"{k} {v.shape}" for k, v in out.items()))

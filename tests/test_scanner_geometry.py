@@ -87,3 +87,89 @@ def test_projector_class_builds_the_scanner_tables():
     z = G.omega_sinogram_z(8, 4.0, 3, 7, 15).astype(np.float32).ravel()
     assert np.array_equal(p.z, z)
     assert (int(p.nRowsD), int(p.nColsD), int(p.nProjections)) == (64, 64, 47)
+
+
+GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scanner_geometry.npz"))
+GAPS = [(12, 4.0, 1, 11, 23, 4, [1.5, 2.25]), (12, 4.0, 3, 11, 23, 4, [1.5, 2.25]),
+        (15, 3.3, 1, 14, 29, 5, [0.8, 0.8]), (15, 3.3, 3, 10, 29, 5, [2.0, 0.5])]
+
+
+@pytest.mark.parametrize("n", range(len(GAPS)))
+def test_ring_gaps_against_reference(n):
+    """Axial tables with gaps between blocks of rings (detcoord.py:726-748, 872-880), span 1 and 3, against
+    the reference's own output."""
+    rings, cr_pz, span, rd, nz, cpba, gaps = GAPS[n]
+    z = G.omega_sinogram_z(rings, cr_pz, span, rd, nz, ring_gaps=gaps, cryst_per_block_axial=cpba)
+    ref = GOLD[f"gaps{n}_z"]
+    assert z.shape == ref.shape
+    np.testing.assert_allclose(z, ref, rtol=0, atol=1e-12)
+    # and the gaps change the table (the comparison is not vacuous)
+    assert np.abs(z - G.omega_sinogram_z(rings, cr_pz, span, rd, nz)[:len(z)]).max() > 0.1
+
+
+@pytest.mark.parametrize("ring", range(4))
+def test_pseudo_detectors_invariants(ring):
+    """Pseudo detectors (det_w_pseudo = B (c + 1)). The reference's Python raises IndexError on this path
+    (recorded in the golden file), its MATLAB version defines it: the real crystals sit where the ring
+    without pseudo detectors has them (to float32 stepping, 1e-3 mm), and each block's pseudo detector is its
+    last real crystal stepped one more pitch along the block."""
+    bpr, cpb, cr_p, dia = [(16, 8, 4.0, 550.0), (18, 6, 3.2, 400.0), (24, 13, 4.0, 820.0), (38, 8, 2.4, 665.0)][ring]
+    assert "IndexError" in str(GOLD["pseudo_error"])
+    xp, yp = G.omega_crystal_ring(bpr, cpb, cr_p, dia, pseudo=True)
+    assert xp.size == bpr * (cpb + 1)
+    pos = np.arange(xp.size)
+    real = pos % (cpb + 1) != cpb
+    np.testing.assert_allclose(xp[real], GOLD[f"ring{ring}_x"], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(yp[real], GOLD[f"ring{ring}_y"], rtol=0, atol=1e-3)
+    deg = np.arange(90, -271, -360 / bpr)[:bpr]
+    last = pos[~real] - 1
+    np.testing.assert_allclose(xp[~real], xp[last] - cr_p * np.cos(np.radians(deg)), rtol=0, atol=1e-3)
+    np.testing.assert_allclose(yp[~real], yp[last] + cr_p * np.sin(np.radians(deg)), rtol=0, atol=1e-3)
+    t = G.omega_sinogram_xy(xp, yp, 64, xp.size // 2, 1)
+    assert t.shape == (64 * (xp.size // 2), 4) and np.isfinite(t).all()
+
+
+@pytest.mark.parametrize("ring,mash", [(0, 2), (0, 4), (1, 3)])
+def test_angular_mashing(ring, mash):
+    """Angular mashing (detcoord.py:616-620, 688-695; the reference's block_reduce needs scikit-image, absent
+    here, as the golden file records): every `mash` consecutive angle columns of the unmashed table -- the
+    reference's own output -- averaged. Mashing by 2 equals the float64 mean rounded to float32 bit for bit
+    (a float32 sum halved is the rounded mean); larger factors within 2 ulp."""
+    assert "skimage" in str(GOLD["mashing_error"])
+    bpr, cpb, nd, side = [(16, 8, 64, 1), (18, 6, 40, -1)][ring]
+    det = bpr * cpb
+    x, y = G.omega_crystal_ring(*[(16, 8, 4.0, 550.0), (18, 6, 3.2, 400.0)][ring])
+    t = G.omega_sinogram_xy(x, y, nd, det // 2 // mash, side)
+    full = GOLD[f"ring{ring}_table"].astype(np.float64).reshape(det // 2, nd, 4)     # [angle][radial][4]
+    ref = full.reshape(det // 2 // mash, mash, nd, 4).mean(axis=1).reshape(-1, 4).astype(np.float32)
+    if mash == 2:
+        assert np.array_equal(t, ref)
+    else:
+        np.testing.assert_allclose(t, ref, rtol=2.5e-7, atol=1e-6)
+    with pytest.raises(ValueError, match="Nang"):
+        G.omega_sinogram_xy(x, y, nd, det // 2 // mash - 1, side)
+
+
+def test_projector_class_pseudo_mashing_gaps():
+    p = _scanner_projector()
+    p.det_w_pseudo = 16 * 9
+    p.Nang = 36
+    p.ringGaps = np.array([2.0])
+    p.cryst_per_block_axial = 4
+    p.addProjector()
+    xc, yc = G.omega_crystal_ring(16, 8, 4.0, 550.0, pseudo=True)
+    assert np.array_equal(p.x, G.omega_sinogram_xy(xc, yc, 64, 36, 1).ravel())
+    z = G.omega_sinogram_z(8, 4.0, 3, 7, 15, ring_gaps=[2.0], cryst_per_block_axial=4).astype(np.float32).ravel()
+    assert np.array_equal(p.z, z)
+    assert (int(p.nRowsD), int(p.nColsD)) == (64, 36)
+    bad = _scanner_projector()
+    bad.det_w_pseudo = 16 * 8 + 3
+    with pytest.raises(ValueError, match="det_w_pseudo"):
+        bad.addProjector()
+
+
+def test_multiple_layers_are_refused():
+    p = _scanner_projector()
+    p.nLayers = 2
+    with pytest.raises(ValueError, match="layers"):
+        p.addProjector()
