@@ -34,6 +34,7 @@ extern "C" {
 #define OMG_PROJ_SIDDON 1
 #define OMG_PROJ_ORTH 2
 #define OMG_PROJ_VOLUME 3
+#define OMG_PROJ_INTERP 4
 #define OMG_PROJ_BDD 5
 
 typedef struct omg_geom omg_geom;
@@ -115,6 +116,13 @@ typedef struct omg_geom_desc {
      * d <= vol_bmax (0 past the table's end), nothing beyond; every LOR weighs 1 / (L R^2 / pi).
      * vol_V, vol_bmin, vol_bmax, vol_vmax: computeVoxelVolumes (coordinates.py:190-205), copied at create. */
     const float* vol_V; int64_t vol_len; float vol_bmin, vol_bmax, vol_vmax;
+    /* interpolation projector (projector_type 4; projectorType4Forward / projectorType4Backward,
+     * source/opencl/projectorType4.cl:84,718): samples every interp_dl mm along the ray (interp_dl <= 0: one
+     * sample per voxel plane of the ray's dominant axis, the Joseph variant), trilinear in the image; CT data
+     * are backprojected voxel by voxel, each panel sampled bilinearly and weighted by L^3 / |v|^2 *
+     * interp_kerroin, interp_kerroin = (dx dy dz) / (dPitchX dPitchY sourceToDetector)
+     * (computeProjectorScalingValues, coordinates.py:241-262). */
+    float interp_dl; float interp_kerroin;
 } omg_geom_desc;
 
 /* Library/version info and error text. */

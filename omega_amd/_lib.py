@@ -16,7 +16,7 @@ OMG_ERR_HIP = -2
 OMG_ERR_UNSUPPORTED = -3
 
 MODE_PET_SINO, MODE_CT_FLAT, MODE_LIST6 = 0, 1, 2
-PROJ_SIDDON, PROJ_ORTH, PROJ_VOLUME, PROJ_BDD = 1, 2, 3, 5
+PROJ_SIDDON, PROJ_ORTH, PROJ_VOLUME, PROJ_INTERP, PROJ_BDD = 1, 2, 3, 4, 5
 
 
 class OmegaHipError(RuntimeError):
@@ -57,6 +57,7 @@ class GeomDesc(C.Structure):
         ("rays_xy", C.c_uint32), ("rays_z", C.c_uint32), ("cr_xy", C.c_float), ("cr_z", C.c_float),
         ("vol_V", C.c_void_p), ("vol_len", C.c_int64), ("vol_bmin", C.c_float), ("vol_bmax", C.c_float),
         ("vol_vmax", C.c_float),
+        ("interp_dl", C.c_float), ("interp_kerroin", C.c_float),
     ]
 
 

@@ -139,7 +139,8 @@ static int attenuation_prepare(omg_geom* g, const float* mu_host, int64_t n_tota
         set_error("could not allocate %lld attenuation factors", (long long)n_total);
         rc = OMG_ERR_HIP;
     }
-    if (!rc) rc = atten_factors(g, mu, g->atten_dev, n_total, nullptr);
+    if (!rc) rc = g->d.projector_type == OMG_PROJ_INTERP ? interp_atten_factors(g, mu, g->atten_dev, n_total, nullptr)
+                                                         : atten_factors(g, mu, g->atten_dev, n_total, nullptr);
     if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) {
         set_error("attenuation factor computation failed");
         rc = OMG_ERR_HIP;
@@ -190,9 +191,10 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     OMG_REQUIRE(d->mode == OMG_MODE_PET_SINO || d->mode == OMG_MODE_CT_FLAT || d->mode == OMG_MODE_LIST6,
                 "unknown data mode %d", d->mode);
     OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON || d->projector_type == OMG_PROJ_ORTH ||
-                    d->projector_type == OMG_PROJ_VOLUME || d->projector_type == OMG_PROJ_BDD,
-                "projector_type %d is not provided (1 Siddon, 2 orthogonal, 3 volume of intersection, 5 branchless "
-                "distance-driven)",
+                    d->projector_type == OMG_PROJ_VOLUME || d->projector_type == OMG_PROJ_INTERP ||
+                    d->projector_type == OMG_PROJ_BDD,
+                "projector_type %d is not provided (1 Siddon, 2 orthogonal, 3 volume of intersection, 4 interpolation, "
+                "5 branchless distance-driven)",
                 d->projector_type);
     OMG_REQUIRE(d->x && d->x_len > 0, "missing x coordinate table");
     if (d->mode == OMG_MODE_PET_SINO) {
@@ -209,6 +211,13 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     }
     if (d->projector_type == OMG_PROJ_BDD)
         OMG_REQUIRE(d->mode == OMG_MODE_CT_FLAT, "projector_type 5 (BDD) supports CT flat-panel data only");
+    if (d->projector_type == OMG_PROJ_INTERP) {
+        OMG_REQUIRE(std::isfinite(d->interp_dl), "interp_dl must be finite");
+        if (d->mode == OMG_MODE_CT_FLAT)
+            OMG_REQUIRE(std::isfinite(d->interp_kerroin) && d->interp_kerroin > 0.f && d->dPitchX > 0.f &&
+                            d->dPitchY > 0.f,
+                        "projector_type 4 on CT data needs interp_kerroin > 0 and the detector pitches dPitchX, dPitchY");
+    }
     if (d->projector_type == OMG_PROJ_VOLUME) {
         OMG_REQUIRE(d->vol_V && d->vol_len > 0 && d->vol_len <= 0x7FFFFFFFll,
                     "projector_type 3 needs the volume table vol_V (1 to 2^31 - 1 entries), got %lld",

@@ -100,6 +100,15 @@ int tof_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, cons
 // Writes f[i] = exp(-sum len * mu) over the Siddon walk of measurements [0, n) (siddon.hip).
 int atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s);
 
+// Interpolation projector (projector type 4, interp.hip): forward and backward projection, the fused
+// emission data step FP -> ratio -> BP, and attenuation factors by its own sampling walk.
+int interp_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s);
+int interp_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
+                    hipStream_t s);
+int interp_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
+                float* sens, float epps, hipStream_t s, const float* add);
+int interp_atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s);
+
 // Validates a [first, first + n) measurement range against the tables; returns OMG_OK or OMG_ERR_ARG.
 int check_range(const omg_geom* g, int64_t first, int64_t n);
 
@@ -125,7 +134,8 @@ float* scratch_ptr(const omg_geom* g);
 int work_reserve(omg_geom* g, uint32_t N);
 float* work_ptr(const omg_geom* g);
 
-// Ensures q_ptr(g) holds `floats` floats for the voxel-driven planar path's per-LOR q buffers.
+// Ensures q_ptr(g) holds `floats` floats of per-measurement work memory (the voxel-driven planar path's q and w
+// buffers, the forward projections of the transmission OSEM iteration and of projector 4's fused data step).
 int q_reserve(omg_geom* g, size_t floats);
 float* q_ptr(const omg_geom* g);
 

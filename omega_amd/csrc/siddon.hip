@@ -682,7 +682,8 @@ static int generic_osem(const omg_geom* g, int64_t first, int64_t n, const float
 
 static int siddon_common_checks(const omg_geom* g, int64_t first, int64_t n) {
     OMG_REQUIRE(g != nullptr, "null geometry handle");
-    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON || is_tube(g) || g->d.projector_type == OMG_PROJ_BDD,
+    OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON || is_tube(g) || g->d.projector_type == OMG_PROJ_BDD ||
+                    g->d.projector_type == OMG_PROJ_INTERP,
                 "unknown projector type %d", g->d.projector_type);
     return check_range(g, first, n);
 }
@@ -711,6 +712,7 @@ extern "C" int omg_forward_project(const omg_geom* g, int64_t first, int64_t n_m
     hipStream_t s = as_stream(stream);
     if (is_tube(g)) return orth_forward(g, first, n_meas, img, y, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_forward(g, first, n_meas, img, y, s);
+    if (g->d.projector_type == OMG_PROJ_INTERP) return interp_forward(g, first, n_meas, img, y, s);
     if (planar_ok(g) && !g->force_generic) return launch_planar<true, false, false>(g, first, n_meas, img, nullptr, y, nullptr, nullptr, 0.f, s);
     return generic_fp(g, first, n_meas, img, y, s);
 }
@@ -724,6 +726,7 @@ extern "C" int omg_backward_project(const omg_geom* g, int64_t first, int64_t n_
     hipStream_t s = as_stream(stream);
     if (is_tube(g)) return orth_backward(g, first, n_meas, y, rhs, sens, s);
     if (g->d.projector_type == OMG_PROJ_BDD) return bdd_backward(g, first, n_meas, y, rhs, sens, s);
+    if (g->d.projector_type == OMG_PROJ_INTERP) return interp_backward(g, first, n_meas, y, rhs, sens, s);
     if (planar_ok(g) && !g->force_generic) {
         if (sens) return launch_planar<false, true, true>(g, first, n_meas, nullptr, y, nullptr, rhs, sens, 0.f, s);
         return launch_planar<false, true, false>(g, first, n_meas, nullptr, y, nullptr, rhs, nullptr, 0.f, s);
@@ -774,10 +777,11 @@ extern "C" int omg_osem_data_step_ex(const omg_geom* g, int64_t first, int64_t n
     if (rc) return rc;
     OMG_REQUIRE(img && meas && rhs, "null pointer argument");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "the fused OSEM data step is for emission (PET) data");
-    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM data step is implemented for projector_type 1 and 2");
+    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM data step is implemented for projector_type 1-4");
     if (n_meas == 0) return OMG_OK;
     hipStream_t s = as_stream(stream);
     if (is_tube(g)) return orth_osem(g, first, n_meas, img, meas, rhs, sens, epps, s, additive);
+    if (g->d.projector_type == OMG_PROJ_INTERP) return interp_osem(g, first, n_meas, img, meas, rhs, sens, epps, s, additive);
     if (planar_ok(g) && !g->force_generic) {
         if (sens) return launch_planar<true, true, true>(g, first, n_meas, img, meas, nullptr, rhs, sens, epps, s, additive);
         return launch_planar<true, true, false>(g, first, n_meas, img, meas, nullptr, rhs, nullptr, epps, s, additive);
@@ -910,7 +914,7 @@ int omg_osem_subset_iteration_impl(const omg_geom* g, int64_t first, int64_t n_m
     if (rc) return rc;
     OMG_REQUIRE(x && meas && S, "null pointer argument");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "OSEM subset iterations here are for emission (PET) data");
-    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM iteration is implemented for projector_type 1 and 2");
+    OMG_REQUIRE(g->d.projector_type != OMG_PROJ_BDD, "the fused OSEM iteration is implemented for projector_type 1-4");
     hipStream_t s = as_stream(stream);
     omg_geom* gm = const_cast<omg_geom*>(g);
     const uint32_t N = g->d.Nx * g->d.Ny * g->d.Nz;
@@ -964,6 +968,9 @@ int omg_osem_subset_iteration_impl(const omg_geom* g, int64_t first, int64_t n_m
     }
     if (n_meas > 0 && is_tube(g)) {
         rc = orth_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s, additive);
+        if (rc) return rc;
+    } else if (n_meas > 0 && g->d.projector_type == OMG_PROJ_INTERP) {
+        rc = interp_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s, additive);
         if (rc) return rc;
     } else if (n_meas > 0) {
         rc = generic_osem(g, first, n_meas, x, meas, rhs, compute_sens ? S : nullptr, epps, s, additive);

@@ -143,6 +143,9 @@ class projectorClass:
         # CT geometry (used when no coordinate tables are given)
         self.sourceToCRot = 0.0
         self.sourceToDetector = 0.0
+        # interpolation projector (projector_type 4; proj.py:287): interpolation length in voxels of FOVa_x / Nx
+        # (0: one voxel; negative: the Joseph variant, one sample per voxel plane)
+        self.dL = 0.0
         self.dPitchX = 0.0
         self.dPitchY = 0.0
         self.angles = np.empty(0, dtype=np.float32)
@@ -166,10 +169,10 @@ class projectorClass:
         """Configuration checks with the reference's ValueError behaviour (proj.py:1068-1285)."""
         if self.FOVa_x > 0 and self.FOVa_y == 0:
             self.FOVa_y = self.FOVa_x
-        if self.projector_type not in (1, 2, 3, 5, 11, 22, 33, 55):
+        if self.projector_type not in (1, 2, 3, 4, 5, 11, 22, 33, 44, 55):
             raise ValueError("Only projector types 1 (improved Siddon), 2 (orthogonal distance-based), 3 (volume "
-                             "of intersection) and 5 (branchless distance-driven) are available in this "
-                             "implementation!")
+                             "of intersection), 4 (interpolation-based) and 5 (branchless distance-driven) are "
+                             "available in this implementation!")
         if self.projector_type in (5, 55) and not self.CT:
             raise ValueError("Projector type 5 is only supported with CT data!")
         if self.subsets > 1 and not self.listmode:
@@ -487,7 +490,29 @@ class projectorClass:
             d.meas_index, d.meas_index_len = idx.ctypes.data, idx.size
         d.dPitchX = float(self.dPitchX)
         d.dPitchY = float(self.dPitchY)
+        if pt == 4:
+            d.interp_dl, d.interp_kerroin = self._interp_scalars()
         return d, keep
+
+    def _interp_scalars(self):
+        """interp_dl (mm) and interp_kerroin of projector type 4 (computeProjectorScalingValues,
+        coordinates.py:241-262): dL = FOVa_x / Nx, or dL * FOVa_x / Nx when dL is set (a negative dL, the
+        Joseph variant, stays negative); kerroin = (dx dy dz) / (dPitchX dPitchY sourceToDetector) for CT,
+        sourceToDetector taken from the first projection's source-panel distance when it is not set."""
+        vox = float(self.FOVa_x) / float(np.asarray(self.Nx).ravel()[0])
+        dl = vox if self.dL == 0 else float(self.dL) * vox
+        kerroin = 0.0
+        if self.CT:
+            sdd = float(self.sourceToDetector)
+            if sdd <= 0:
+                t = np.asarray(self.x, dtype=np.float64).reshape(-1, 6)[0]
+                sdd = float(np.linalg.norm(t[3:] - t[:3]))
+            if self.dPitchX <= 0 or self.dPitchY <= 0 or sdd <= 0:
+                raise ValueError("The interpolation projector on CT data needs dPitchX, dPitchY and the source-detector "
+                                 "distance")
+            g = self.grid
+            kerroin = (float(g.dx) * float(g.dy) * float(g.dz)) / (float(self.dPitchX) * float(self.dPitchY) * sdd)
+        return float(np.float32(dl)), float(np.float32(kerroin))
 
     def initProj(self):
         """Create the device-side geometry (proj.py:1913 -> init.py:39)."""

@@ -1603,3 +1603,334 @@ extern "C" int64_t oracle_orth_trace_f32(ORACLE_ORTH_ARGS, int64_t* counts, int6
     }
     return total;
 }
+
+// ------------------------------------------------------------------------------------------------
+// Interpolation projector (projector type 4). There is no CPU version in the reference: this restates
+// its GPU kernels (source/opencl/projectorType4.cl) with the 3-D image texture (normalised coordinates,
+// linear filtering, clamp to edge: samplerForw / samplerIm, general_opencl_functions.h:456-458) replaced
+// by exact FP32 trilinear interpolation, evaluated as nested lerps in x, then y, then z:
+//   u = p * N - 0.5, i0 = floor(u), a = u - i0, corners clamped to [0, N - 1].
+//   fp        projectorType4Forward with -DFP (:84-674): the ray from s to d clipped to the box
+//             [b, bmax]; samples every tStep = dL / L from tStart (or, dL <= 0, the Joseph variant: one
+//             sample per voxel plane of the dominant axis, :434-461), the normalised position recomputed
+//             every 64 samples and stepped by dp = v * scale * tStep between (:531-541); ax += stepLen *
+//             tex(p); PET: y = ax * temp, temp = 1 / L * norm * scat * atnm * global (* exp(jelppi) for an
+//             attenuation image, last, :612-626); CT: y = ax.
+//   bp (PET)  projectorType4Forward with -DBP: the same samples, each depositing stepLen * temp * y into the
+//             voxel containing p (truncated normalised coordinates; index 0 and indices past the image are
+//             skipped, :570-576); with an attenuation image temp starts with exp(jelppi) and the samples
+//             start a tenth of a step later (:485-510).
+//   bp (CT)   projectorType4Backward (:718-1172, no offset / FDK / helical / STYPE 12): every voxel centre,
+//             in z groups of NVOXELS = 8 (the first voxel of a group from its centre, the next by
+//             v.z += dz, lowerPart -= dz * cP.z), projected from each source onto its panel; the panel
+//             sampled bilinearly at (px, py) (projection pz = (k + 0.5) / nProj); weight = L^3 / |v|^2 * kerroin;
+//             out += sum yVar * weight, sens += sum weight, both over the samples with yVar != 0.
+namespace interp {
+
+struct F3 { float x, y, z; };
+static inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+static inline F3 sub(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+struct IGeo {
+    oracle::Geo<float> g;
+    F3 b, bmax, scale;     // box corners and 1 / (N d) per axis
+    float dL;              // interpolation length (mm); <= 0: Joseph
+    float kerroin;         // CT backprojection scale (dx dy dz) / (dPitchX dPitchY sourceToDetector)
+    float dPitchX, dPitchY;
+    int32_t atn_image;     // atten holds factors made from an attenuation image (ATN order), not given ones
+};
+
+// Trilinear sample of an Nx x Ny x Nz image (x fastest) at normalised coordinates, clamp to edge.
+static inline float tex3(const float* img, uint32_t Nx, uint32_t Ny, uint32_t Nz, float px, float py, float pz) {
+    auto axis = [](float p, uint32_t N, uint32_t& i0, uint32_t& i1, float& a) {
+        const float u = p * (float)N - 0.5f;
+        const float f = std::floor(u);
+        a = u - f;
+        const float lo = std::fmin(std::fmax(f, 0.f), (float)(N - 1));
+        const float hi = std::fmin(std::fmax(f + 1.f, 0.f), (float)(N - 1));
+        i0 = (uint32_t)lo;
+        i1 = (uint32_t)hi;
+    };
+    uint32_t x0, x1, y0, y1, z0, z1;
+    float a, b, c;
+    axis(px, Nx, x0, x1, a);
+    axis(py, Ny, y0, y1, b);
+    axis(pz, Nz, z0, z1, c);
+    const size_t Nxy = (size_t)Nx * Ny;
+    auto at = [&](uint32_t i, uint32_t j, uint32_t k) { return img[(size_t)k * Nxy + (size_t)j * Nx + i]; };
+    const float c00 = (1.f - a) * at(x0, y0, z0) + a * at(x1, y0, z0);
+    const float c10 = (1.f - a) * at(x0, y1, z0) + a * at(x1, y1, z0);
+    const float c01 = (1.f - a) * at(x0, y0, z1) + a * at(x1, y0, z1);
+    const float c11 = (1.f - a) * at(x0, y1, z1) + a * at(x1, y1, z1);
+    const float c0 = (1.f - b) * c00 + b * c10;
+    const float c1 = (1.f - b) * c01 + b * c11;
+    return (1.f - c) * c0 + c * c1;
+}
+
+// Sampling of one ray: start, parametric step, step length, count. Returns false when it misses the box.
+struct Samples { F3 s, v; float t0, tStep, stepLen, L, tStart, tEnd; uint32_t n; };
+
+static bool samples(const IGeo& G, int64_t lo, const float* x, const float* z, Samples& S) {
+    oracle::Ray<float> r;
+    oracle::fetch_ray(G.g, lo, x, z, r);
+    S.s = f3(r.xs, r.ys, r.zs);
+    const F3 d = f3(r.xd, r.yd, r.zd);
+    S.v = sub(d, S.s);
+    const F3 tBack{(G.b.x - S.s.x) / S.v.x, (G.b.y - S.s.y) / S.v.y, (G.b.z - S.s.z) / S.v.z};
+    const F3 tFront{(G.bmax.x - S.s.x) / S.v.x, (G.bmax.y - S.s.y) / S.v.y, (G.bmax.z - S.s.z) / S.v.z};
+    const F3 tMin{std::fmin(tFront.x, tBack.x), std::fmin(tFront.y, tBack.y), std::fmin(tFront.z, tBack.z)};
+    const F3 tMax{std::fmax(tFront.x, tBack.x), std::fmax(tFront.y, tBack.y), std::fmax(tFront.z, tBack.z)};
+    const float tStart = std::fmax(std::fmax(tMin.x, tMin.y), tMin.z);
+    const float tEnd = std::fmin(std::fmin(tMax.x, tMax.y), tMax.z);
+    if (!(tStart < tEnd)) return false;
+    S.tStart = tStart;
+    S.tEnd = tEnd;
+    S.L = std::sqrt(S.v.x * S.v.x + S.v.y * S.v.y + S.v.z * S.v.z);
+    if (G.dL <= 0.f) {
+        const float crx = std::fabs(S.v.x) * G.scale.x * (float)G.g.Nx;
+        const float cry = std::fabs(S.v.y) * G.scale.y * (float)G.g.Ny;
+        const float crz = std::fabs(S.v.z) * G.scale.z * (float)G.g.Nz;
+        S.tStep = 1.f / std::fmax(std::fmax(crx, cry), crz);
+        float sInd, vDim;
+        if (crx >= cry && crx >= crz) {
+            sInd = (S.s.x + S.v.x * tStart - G.b.x) * G.scale.x * (float)G.g.Nx - 0.5f;
+            vDim = S.v.x;
+        } else if (cry >= crz) {
+            sInd = (S.s.y + S.v.y * tStart - G.b.y) * G.scale.y * (float)G.g.Ny - 0.5f;
+            vDim = S.v.y;
+        } else {
+            sInd = (S.s.z + S.v.z * tStart - G.b.z) * G.scale.z * (float)G.g.Nz - 0.5f;
+            vDim = S.v.z;
+        }
+        S.t0 = vDim > 0.f ? tStart + (std::floor(sInd) - sInd) * S.tStep : tStart + (sInd - std::floor(sInd)) * S.tStep;
+        S.stepLen = S.tStep * S.L;
+    } else {
+        S.tStep = G.dL / S.L;
+        S.stepLen = G.dL;
+        S.t0 = tStart;
+    }
+    S.n = S.t0 <= tEnd ? (uint32_t)((tEnd - S.t0) / S.tStep) + 1u : 0u;
+    return true;
+}
+
+// Visit the normalised sample positions of a ray (recomputed every 64 samples, stepped between).
+template <class V>
+static void walk(const IGeo& G, const Samples& S, float t0, uint32_t n, V&& visit) {
+    const F3 dp{S.v.x * G.scale.x * S.tStep, S.v.y * G.scale.y * S.tStep, S.v.z * G.scale.z * S.tStep};
+    F3 p{0.f, 0.f, 0.f};
+    for (uint32_t ii = 0; ii < n; ii++) {
+        if ((ii & 63u) == 0u) {
+            const float t = t0 + (float)ii * S.tStep;
+            p = F3{S.v.x * t + S.s.x, S.v.y * t + S.s.y, S.v.z * t + S.s.z};
+            p = F3{(p.x - G.b.x) * G.scale.x, (p.y - G.b.y) * G.scale.y, (p.z - G.b.z) * G.scale.z};
+        } else {
+            p = F3{p.x + dp.x, p.y + dp.y, p.z + dp.z};
+        }
+        visit(p);
+    }
+}
+
+static float lor_weight(const IGeo& G, int64_t lo, float L, bool fp) {
+    const oracle::Geo<float>& g = G.g;
+    float temp = 1.f / L;
+    const bool atn = g.atten && G.atn_image;
+    if (atn && !fp) temp *= g.atten[lo];
+    if (g.norm) temp *= g.norm[lo];
+    if (g.scat) temp *= g.scat[lo];
+    if (g.atten && !G.atn_image) temp *= g.atten[lo];
+    temp *= g.global_factor;
+    if (atn && fp) temp *= g.atten[lo];
+    return temp;
+}
+
+static void fp(const IGeo& G, int64_t nMeas, const float* x, const float* z, const float* img, float* y, int nthreads) {
+    oracle::set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        Samples S;
+        if (!samples(G, lo, x, z, S)) { y[lo] = 0.f; continue; }
+        float ax = 0.f;
+        walk(G, S, S.t0, S.n, [&](F3 p) { ax += S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z); });
+        y[lo] = G.g.mode == oracle::CT_FLAT ? ax : ax * lor_weight(G, lo, S.L, true);
+    }
+}
+
+static void bp_rays(const IGeo& G, int64_t nMeas, const float* x, const float* z, const float* yin, float* rhs,
+                    float* sens, int nthreads) {
+    const uint32_t Nx = G.g.Nx, Ny = G.g.Ny, Nz = G.g.Nz;
+    const uint64_t N = (uint64_t)Nx * Ny * Nz;
+    oracle::set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        Samples S;
+        if (!samples(G, lo, x, z, S)) continue;
+        float t0 = S.t0;
+        uint32_t n = S.n;
+        if (G.g.atten && G.atn_image && G.dL > 0.f) {
+            // with an attenuation image the samples start a tenth of a step in (projectorType4.cl:505-509)
+            t0 = S.tStart + S.tStep / 10.f;
+            n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
+        }
+        const float e = S.stepLen * lor_weight(G, lo, S.L, false);
+        const float yv = yin[lo];
+        walk(G, S, t0, n, [&](F3 p) {
+            const uint64_t ind = (uint64_t)(uint32_t)std::fmax(p.x * (float)Nx, 0.f) +
+                                 (uint64_t)(uint32_t)std::fmax(p.y * (float)Ny, 0.f) * Nx +
+                                 (uint64_t)(uint32_t)std::fmax(p.z * (float)Nz, 0.f) * Nx * Ny;
+            if (ind == 0 || ind >= N) return;
+            const float val = yv * e;
+#pragma omp atomic
+            rhs[ind] += val;
+            if (sens) {
+#pragma omp atomic
+                sens[ind] += e;
+            }
+        });
+    }
+}
+
+// Bilinear sample of projection k's panel (size_x rows fastest, size_y columns) at normalised (px, py), with
+// the 3-D texture's third coordinate pz interpolating between projections (clamp to edge).
+static inline float tex_panel(const float* y, uint32_t sx, uint32_t sy, uint32_t np, float px, float py, float pz) {
+    return tex3(y, sx, sy, np, px, py, pz);
+}
+
+static void bp_ct(const IGeo& G, int64_t nProj, const float* xyz, const float* uv, const float* yin, float* out,
+                  float* sens, int nthreads) {
+    const oracle::Geo<float>& g = G.g;
+    const uint32_t Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    constexpr uint32_t NV = 8;
+    const uint32_t groups = (Nz + NV - 1) / NV;
+    const size_t NA = g.ct_pitch ? 6 : 2;
+    const float kx = (float)g.size_x * G.dPitchX, ky = (float)g.size_y * G.dPitchY;
+    const float ix = (float)g.size_x / 2.f, iy = (float)g.size_y / 2.f;
+    oracle::set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < (int64_t)Nx * Ny * groups; q++) {
+        const uint32_t gi = (uint32_t)(q % Nx), gj = (uint32_t)((q / Nx) % Ny), gk = (uint32_t)(q / ((int64_t)Nx * Ny)) * NV;
+        float temp[NV], wsum[NV];
+        for (uint32_t zz = 0; zz < NV; zz++) { temp[zz] = 0.f; wsum[zz] = 0.f; }
+        const F3 dV{(float)gi * g.dx + g.dx / 2.f + g.bx, (float)gj * g.dy + g.dy / 2.f + g.by,
+                    (float)gk * g.dz + g.dz / 2.f + g.bz};
+        for (int64_t kk = 0; kk < nProj; kk++) {
+            const float* xp = xyz + (size_t)(g.plane_offset + kk) * 6;
+            const float* up = uv + (size_t)(g.plane_offset + kk) * NA;
+            const F3 s{xp[0], xp[1], xp[2]}, d1{xp[3], xp[4], xp[5]};
+            F3 apuX, apuY;
+            if (g.ct_pitch) {
+                apuX = F3{up[0] * ix, up[1] * ix, up[2] * ix};
+                apuY = F3{up[3] * iy, up[4] * iy, up[5] * iy};
+            } else {
+                apuX = F3{ix * up[0], ix * up[1], 0.f};
+                apuY = F3{0.f, 0.f, iy * G.dPitchY};
+            }
+            const F3 d2 = sub(apuX, apuY);
+            const F3 d3 = sub(sub(d1, apuX), apuY);
+            const float nx = std::sqrt(dot3(apuX, apuX)), ny = std::sqrt(dot3(apuY, apuY));
+            const F3 normX{apuX.x / nx / kx, apuX.y / nx / kx, apuX.z / nx / kx};
+            const F3 normY{apuY.x / ny / ky, apuY.y / ny / ky, apuY.z / ny / ky};
+            const F3 e = sub(d3, d1);
+            const F3 cP{d2.y * e.z - d2.z * e.y, d2.z * e.x - d2.x * e.z, d2.x * e.y - d2.y * e.x};
+            const float pz = ((float)kk + 0.5f) / (float)nProj;
+            const float dApu = g.dz * cP.z;
+            const float upper = dot3(cP, sub(s, d1));
+            F3 v = sub(dV, s);
+            float lower = -dot3(v, cP);
+            const float vApu = v.x * v.x + v.y * v.y;
+            for (uint32_t zz = 0; zz < NV; zz++) {
+                if (gk + zz >= Nz) break;
+                const float t = upper / lower;
+                F3 p{s.x + v.x * t, s.y + v.y * t, s.z + v.z * t};
+                const float l1 = vApu + v.z * v.z;
+                const F3 ps = sub(p, s);
+                const float L = std::sqrt(dot3(ps, ps));
+                const float weight = (L * L * L) / l1 * G.kerroin;
+                p = sub(p, d3);
+                const float px = dot3(p, normX), py = dot3(p, normY);
+                float yv = 0.f;
+                if (px <= 1.f && py <= 1.f && pz <= 1.f && px >= 0.f && py >= 0.f && pz >= 0.f)
+                    yv = tex_panel(yin, g.size_x, g.size_y, (uint32_t)nProj, px, py, pz);
+                if (yv != 0.f) {
+                    temp[zz] += yv * weight;
+                    wsum[zz] += weight;
+                }
+                v.z += g.dz;
+                lower -= dApu;
+            }
+        }
+        for (uint32_t zz = 0; zz < NV; zz++) {
+            if (gk + zz >= Nz) break;
+            const size_t idx = (size_t)gi + (size_t)gj * Nx + (size_t)(gk + zz) * Nx * Ny;
+            out[idx] += temp[zz];
+            if (sens) sens[idx] += wsum[zz];
+        }
+    }
+}
+
+}  // namespace interp
+
+static interp::IGeo make_igeo(const uint32_t* n3, const float* d3, const float* b3, uint32_t size_x, uint32_t size_y,
+                              int32_t mode, int64_t plane_offset, float dPitchZ, int32_t ct_pitch, float global_factor,
+                              const float* norm, const float* atten, int32_t atten_image, const float* scat,
+                              const float* bmax3, const float* scale3, float dL, float kerroin, float dPitchX,
+                              float dPitchY) {
+    interp::IGeo G;
+    G.g = make_geo<float>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm,
+                          atten, 2, scat);
+    G.b = interp::F3{b3[0], b3[1], b3[2]};
+    G.bmax = interp::F3{bmax3[0], bmax3[1], bmax3[2]};
+    G.scale = interp::F3{scale3[0], scale3[1], scale3[2]};
+    G.dL = dL;
+    G.kerroin = kerroin;
+    G.dPitchX = dPitchX;
+    G.dPitchY = dPitchY;
+    G.atn_image = atten_image;
+    return G;
+}
+
+#define ORACLE_INTERP_ARGS                                                                                            \
+    const uint32_t *n3, const float *d3, const float *b3, uint32_t size_x, uint32_t size_y, int32_t mode,             \
+        int64_t plane_offset, float dPitchZ, int32_t ct_pitch, float global_factor, const float *norm,               \
+        const float *atten, int32_t atten_image, const float *scat, const float *bmax3, const float *scale3, float dL, \
+        float kerroin, float dPitchX, float dPitchY, int64_t nMeas, const float *x, const float *z
+
+#define ORACLE_INTERP_GEO                                                                                             \
+    make_igeo(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten,          \
+              atten_image, scat, bmax3, scale3, dL, kerroin, dPitchX, dPitchY)
+
+// Forward projection of nMeas measurements (CT: nMeas = projections x panel pixels).
+extern "C" int oracle_interp_fp_f32(ORACLE_INTERP_ARGS, const float* img, float* y, int nthreads) {
+    interp::fp(ORACLE_INTERP_GEO, nMeas, x, z, img, y, nthreads);
+    return 0;
+}
+
+// Backprojection: PET / list mode ray-driven (nMeas measurements); CT voxel-driven over nMeas / (size_x size_y)
+// projections.
+extern "C" int oracle_interp_bp_f32(ORACLE_INTERP_ARGS, const float* yin, float* rhs, float* sens, int nthreads) {
+    const interp::IGeo G = ORACLE_INTERP_GEO;
+    if (mode == oracle::CT_FLAT)
+        interp::bp_ct(G, nMeas / ((int64_t)size_x * size_y), x, z, yin, rhs, sens, nthreads);
+    else
+        interp::bp_rays(G, nMeas, x, z, yin, rhs, sens, nthreads);
+    return 0;
+}
+
+// Per-measurement attenuation factors exp(jelppi) of the interpolation projector: jelppi accumulates
+// stepLen * -mu(p) over the forward projection's samples (compute_attenuation, general_opencl_functions.h:
+// 1633-1653, trilinear as above); 1 for measurements whose ray misses the image.
+extern "C" int oracle_interp_atten_f32(ORACLE_INTERP_ARGS, const float* mu, float* out, int nthreads) {
+    const interp::IGeo G = ORACLE_INTERP_GEO;
+    oracle::set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        interp::Samples S;
+        if (!interp::samples(G, lo, x, z, S)) { out[lo] = 1.f; continue; }
+        float jel = 0.f;
+        interp::walk(G, S, S.t0, S.n, [&](interp::F3 p) {
+            jel += S.stepLen * -interp::tex3(mu, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z);
+        });
+        out[lo] = std::exp(jel);
+    }
+    return 0;
+}

@@ -74,6 +74,12 @@ def lib():
         _lib.oracle_orth_bp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_void_p, C.c_int]
         _lib.oracle_orth_trace_f32.argtypes = ogeo + [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.oracle_orth_trace_f32.restype = C.c_int64
+        igeo = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, C.c_float, C.c_int32,
+                C.c_float, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, _f32p, _f32p, C.c_float, C.c_float,
+                C.c_float, C.c_float, C.c_int64, _f32p, _f32p]
+        _lib.oracle_interp_fp_f32.argtypes = igeo + [_f32p, _f32p, C.c_int]
+        _lib.oracle_interp_bp_f32.argtypes = igeo + [_f32p, _f32p, C.c_void_p, C.c_int]
+        _lib.oracle_interp_atten_f32.argtypes = igeo + [_f32p, _f32p, C.c_int]
         _lib.oracle_ratio_f32.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
         _lib.oracle_sens_clamp_f32.argtypes = [_f32p, C.c_int64, C.c_float]
         _lib.oracle_osem_update_f32.argtypes = [_f32p, _f32p, _f32p, C.c_int64, C.c_float, C.c_int]
@@ -496,6 +502,60 @@ class OrthCase:
         w = np.zeros(max(total, 1), dtype=np.float32)
         f(*self._geo(nmeas), None, total, idx.ctypes.data, w.ctypes.data, None)
         return counts, idx[:total], w[:total], lw
+
+
+class InterpCase:
+    """Interpolation projector (projector type 4, GPU-kernel semantics with exact FP32 trilinear sampling) of
+    the oracle, float32. ``dL``: interpolation length in mm (<= 0: the Joseph variant); ``kerroin``: the CT
+    backprojection scale (dx dy dz) / (dPitchX dPitchY sourceToDetector); ``atten``: per-measurement
+    attenuation factors, made from an attenuation image (``atten_image``) or given per LOR; CT
+    backprojection is voxel-driven over the (size_x x size_y)-pixel panels of the call's projections."""
+
+    def __init__(self, grid, size_x, size_y, x, z, dL, mode=MODE_PET, plane_offset=0, dPitchZ=0.0, ct_pitch=0,
+                 global_factor=1.0, norm=None, atten=None, atten_image=False, scat=None, kerroin=0.0, dPitchX=0.0,
+                 dPitchY=0.0):
+        self.grid = grid
+        self.n3 = np.ascontiguousarray(grid.n3())
+        self.d3 = grid.d3(np.float32)
+        self.b3 = grid.b3(np.float32)
+        # box far corner b + N d in double, rounded once; 1 / (N d) (init.py:705-735, coordinates.py:224-226)
+        self.bmax3 = np.array([float(b) + n * float(d) for b, n, d in zip(self.b3, self.n3, self.d3)], np.float32)
+        self.scale3 = np.array([1.0 / (float(d) * n) for d, n in zip(self.d3, self.n3)], np.float32)
+        self.size_x, self.size_y = int(size_x), int(size_y)
+        self.x = np.ascontiguousarray(x, dtype=np.float32)
+        self.z = np.ascontiguousarray(z, dtype=np.float32)
+        self.dL, self.mode, self.plane_offset = float(dL), mode, int(plane_offset)
+        self.dPitchZ, self.ct_pitch, self.global_factor = float(dPitchZ), int(ct_pitch), float(global_factor)
+        self.norm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        self.atten = None if atten is None else np.ascontiguousarray(atten, dtype=np.float32)
+        self.atten_image = int(bool(atten_image))
+        self.scat = None if scat is None else np.ascontiguousarray(scat, dtype=np.float32)
+        self.kerroin, self.dPitchX, self.dPitchY = float(kerroin), float(dPitchX), float(dPitchY)
+
+    def _geo(self, nmeas):
+        p = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
+                self.dPitchZ, self.ct_pitch, self.global_factor, p(self.norm), p(self.atten), self.atten_image,
+                p(self.scat), self.bmax3, self.scale3, self.dL, self.kerroin, self.dPitchX, self.dPitchY, nmeas,
+                self.x, self.z]
+
+    def fp(self, img, nmeas, nthreads=0):
+        y = np.zeros(nmeas, dtype=np.float32)
+        lib().oracle_interp_fp_f32(*self._geo(nmeas), np.ascontiguousarray(img, dtype=np.float32), y, nthreads)
+        return y
+
+    def atten_factors(self, mu, nmeas, nthreads=0):
+        """exp(-sum stepLen * mu(p)) over the forward projection's samples (1 where the ray misses)."""
+        out = np.ones(nmeas, dtype=np.float32)
+        lib().oracle_interp_atten_f32(*self._geo(nmeas), np.ascontiguousarray(mu, dtype=np.float32), out, nthreads)
+        return out
+
+    def bp(self, yin, nmeas, sens=False, nthreads=0):
+        rhs = np.zeros(self.grid.N, dtype=np.float32)
+        s = np.zeros(self.grid.N, dtype=np.float32) if sens else None
+        lib().oracle_interp_bp_f32(*self._geo(nmeas), np.ascontiguousarray(yin, dtype=np.float32), rhs,
+                                   None if s is None else s.ctypes.data, nthreads)
+        return (rhs, s) if sens else rhs
 
 
 class BDDCase:

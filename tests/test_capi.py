@@ -69,7 +69,7 @@ def _desc(**kw):
     ("Nx", 0, "empty image"),
     ("dx", -1.0, "positive"),
     ("mode", 7, "unknown data mode"),
-    ("projector_type", 4, "projector_type 4"),
+    ("projector_type", 6, "projector_type 6"),
     ("x_len", 3, "x table"),
     ("n_proj", 9, "z table"),
 ])
@@ -80,6 +80,21 @@ def test_create_rejects_bad_descriptors(field, value, msg):
     rc = lib.omg_geom_create(C.byref(d), C.byref(h))
     assert rc == _lib.OMG_ERR_ARG
     assert msg in _lib.last_error()
+
+
+def test_create_rejects_bad_interp_settings():
+    """Projector 4 needs a finite interpolation length and, on CT data, the backprojection scale and pitches."""
+    lib = _lib.load()
+    d, keep = _desc(projector_type=4, interp_dl=float("nan"))
+    h = C.c_void_p()
+    assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG
+    assert "interp_dl" in _lib.last_error()
+    x = np.zeros(6 * 2, np.float32)
+    uv = np.ones(2 * 2, np.float32)
+    d, keep = _desc(projector_type=4, mode=_lib.MODE_CT_FLAT, interp_dl=0.5)
+    d.x, d.x_len, d.z, d.z_len = x.ctypes.data, x.size, uv.ctypes.data, uv.size
+    assert lib.omg_geom_create(C.byref(d), C.byref(h)) == _lib.OMG_ERR_ARG
+    assert "interp_kerroin" in _lib.last_error()
 
 
 def test_create_rejects_bad_subset_index():
