@@ -1,4 +1,5 @@
-"""Builds libomega_hip.so from omega_amd/csrc/*.hip with hipcc for gfx950 (in-tree).
+"""Builds libomega_hip.so from omega_amd/csrc/*.hip with hipcc for gfx950 (in-tree), then the three
+omegaMain libraries (omega_amd/csrc/main/omega_main.hip, one per measurement type) linked against it.
 
 Each translation unit is compiled to an object in omega_amd/_build/ (in parallel), then linked.
 """
@@ -27,6 +28,15 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+
+
+# omegaMain variants: library name -> measurement element type (include/omega_main.h)
+MAIN_SRC = os.path.join(CSRC, "main", "omega_main.hip")
+MAIN_LIBS = {
+    "libomega_matrixfree.so": "float",
+    "libomega_matrixfree_uint16.so": "uint16_t",
+    "libomega_matrixfree_uint8.so": "uint8_t",
+}
 
 
 def hipcc() -> str:
@@ -59,6 +69,23 @@ def _compile(src: str, incremental: bool, verbose: bool):
     return obj, None
 
 
+def _build_main(name: str, meas_type: str, verbose: bool):
+    """Compile and link one omegaMain library; returns an error text or None."""
+    out = os.path.join(HERE, name)
+    obj = os.path.join(OBJ, name[:-3] + ".o")
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, f"-DOMG_MAIN_MEAS_TYPE={meas_type}", "-c", MAIN_SRC, "-o", obj]
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", obj, "-L", HERE, "-lomega_hip",
+            "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    for c in (cmd, link):
+        if verbose:
+            print(" ".join(c), flush=True)
+        r = subprocess.run(c, capture_output=True, text=True)
+        if r.returncode != 0:
+            return r.stdout + r.stderr
+    os.replace(out + ".tmp", out)
+    return None
+
+
 def build(verbose: bool = False, incremental: bool = False, jobs: int = 0) -> str:
     """Compile every HIP source and link omega_amd/libomega_hip.so. By default every source is
     recompiled; incremental=True reuses objects newer than their source and the headers."""
@@ -77,6 +104,11 @@ def build(verbose: bool = False, incremental: bool = False, jobs: int = 0) -> st
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("linking libomega_hip.so failed")
     os.replace(tmp, LIB)
+    with ThreadPoolExecutor(max_workers=min(jobs, len(MAIN_LIBS))) as ex:
+        errors = [e for e in ex.map(lambda kv: _build_main(kv[0], kv[1], verbose), MAIN_LIBS.items()) if e]
+    if errors:
+        sys.stderr.write("\n".join(errors))
+        raise RuntimeError("hipcc failed building the omegaMain libraries")
     return LIB
 
 
