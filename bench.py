@@ -45,7 +45,7 @@ def parse():
                     help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
                     choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet2d_volume", "pet3d_autograd",
-                             "pet2d_tof"],
+                             "pet2d_tof", "unet"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     ap.add_argument("--voxel-bp", action="store_true",
@@ -64,7 +64,7 @@ def main():
     if args.workload != "pet2d_osem":
         {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth,
          "pet2d_volume": lambda a: bench_orth(a, volume=True),
-         "pet3d_autograd": bench_autograd, "pet2d_tof": bench_tof}[args.workload](args)
+         "pet3d_autograd": bench_autograd, "pet2d_tof": bench_tof, "unet": bench_unet}[args.workload](args)
         return
 
     import ctypes as C
@@ -551,6 +551,87 @@ def bench_autograd(args):
         "adjoint_pairs": 20, "adjoint_worst_rel_mismatch": worst,
         "fd_voxels": 64, "fd_h": h, "fd_max_rel_error": max(errs), "fd_median_rel_error": float(np.median(errs)),
         "roofline": None, "cpu_baseline": None,
+    })
+
+
+def bench_unet(args):
+    """The warm start's 3-D U-Net on the 3-D PET image (128 x 128 x 63, z padded to 64): one training step
+    (forward, masked MSE, backward, Adam) per step. value = training steps/s. Each forward convolution is
+    timed alone with HIP events on the launch stream; the roofline is that of the convolution kernel over
+    all ten forward layers (2 * 27 * Cin * Cout FLOP per voxel, BF16 dense MFMA peak), its launches queued
+    behind a device-side sleep so that host launch overhead does not count."""
+    import torch
+    from omega_amd import _lib
+    from omega_amd.unet import LAYERS, UNet3D
+    dev = torch.device("cuda:0")
+    _lib.load()
+    net = UNet3D(128, 128, 63, seed=0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    img = torch.rand(128 * 128 * 63, generator=g, device=dev)
+    tgt = torch.rand(128 * 128 * 63, generator=g, device=dev)
+    mask = net.mask()
+
+    def step():
+        net.forward(img, 1.0)
+        net.loss(tgt, mask, 1.0)
+        net.zero_grad()
+        net.backward(net.act["gout"])
+        net.adam_step(1e-4)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # forward pass alone, and each forward convolution alone
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+
+    def queued(fn):
+        """Seconds per call of fn, its launches queued behind a device-side sleep so that the host's
+        launch overhead does not open gaps between them."""
+        torch.cuda._sleep(int(5e6))
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) * 1e-3 / reps
+
+    fwd_s = queued(lambda: net.forward(img, 1.0))
+    inputs = {"e1a": ("x0", None), "e1b": ("a1", None), "e2a": ("p1", None), "e2b": ("a2", None),
+              "bta": ("p2", None), "btb": ("a3", None), "d2a": ("u2", "s2"), "d2b": ("a4", None),
+              "d1a": ("u1", "s1"), "d1b": ("a5", None)}
+    outputs = {"e1a": "a1", "e1b": "s1", "e2a": "a2", "e2b": "s2", "bta": "a3", "btb": "b3", "d2a": "a4",
+               "d2b": "d2", "d1a": "a5", "d1b": "d1"}
+    layers = {}
+    tot_flop, tot_s = 0.0, 0.0
+    for name, cins, cout, lvl in LAYERS:
+        x1, x2 = inputs[name]
+        s_l = queued(lambda: net._conv(name, net.act[x1], None if x2 is None else net.act[x2], net.act[outputs[name]]))
+        flop = 2.0 * 27 * sum(cins) * cout * net.V[lvl]
+        layers[name] = {"cin": sum(cins), "cout": cout, "voxels": net.V[lvl], "us": round(s_l * 1e6, 1),
+                        "tflops": round(flop / s_l / 1e12, 1)}
+        tot_flop += flop
+        tot_s += s_l
+    peak = 2500.0
+    achieved = tot_flop / tot_s / 1e12
+    _common(args, {
+        "metric": "3-D U-Net training steps/s (BF16 MFMA convolutions)", "value": round(args.steps / dt, 2),
+        "unit": "train-steps/s", "ms_per_step": round(dt / args.steps * 1e3, 3), "dtype": "bf16",
+        "data": "synthetic (uniform random image and target)",
+        "config": {"workload": "unet3d_warmstart", "image": [128, 128, 63], "padded_depth": 64,
+                   "channels": [16, 32, 64], "forward_ms": round(fwd_s * 1e3, 3), "layers": layers},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_conv3d_fwd_tile (ten forward convolutions)",
+                     "avg_launch_us": round(tot_s / len(LAYERS) * 1e6, 1),
+                     "algorithmic_flop_per_forward": int(tot_flop)},
+        "cpu_baseline": None,
     })
 
 

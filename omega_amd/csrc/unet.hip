@@ -116,6 +116,139 @@ __global__ __launch_bounds__(256) void k_conv3d_fwd(const uint16_t* __restrict__
     }
 }
 
+// The same convolution on gfx950's K = 32 matrix-core operation (v_mfma_f32_16x16x32_bf16) with the input
+// staged in LDS. A workgroup of WAVES waves owns a block of 16 (x) x TY (y) x TZ (z) output voxels and all
+// Cout channels, TY * TZ = NT * WAVES rows of 16 voxels; it first copies the block's input halo,
+// 18 x (TY + 2) x (TZ + 2) voxels x CIN channels (zero outside the volume), into LDS (all of a lane's loads
+// issued before its stores), so every input voxel is read from memory once per workgroup instead of once per
+// tap. The reduction runs over K = 27 CIN in chunks of 32, fully unrolled: lane l takes the 8 consecutive k
+// of its quarter, k = 32 kc + 8 (l / 16) + j, i.e. tap k / CIN and channels k % CIN .. +7 (CIN % 8 == 0, so the
+// 8 never straddle a tap), reads them as one 16-byte LDS load of its voxel's shifted neighbour (B[k][voxel])
+// and one 16-byte load of weight row co = 16 m + l % 16 (A[co][k]), the latter issued two chunks ahead; the
+// result D[co][voxel] holds output channels 16 m + 4 (l / 16) + j of voxel l % 16. Wave w computes rows
+// NT w .. NT w + NT - 1 for all MT output blocks: every weight fragment feeds NT operations and every
+// activation fragment MT. LDS slots are CIN elements apart (measured faster than padded strides).
+// Tile shapes by volume (measured on 128 x 128 x 64, 64 x 64 x 32 and 32 x 32 x 16): 8 waves on 16 x 8 x 4
+// where the volume gives at least 512 such blocks, 4 waves on 16 x 4 x 4 where it gives 256, otherwise 4
+// waves on 16 x 4 x 1 rows (one row per wave), so that small volumes still occupy the chip.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <int CIN, int WAVES, int NT, int TY>
+constexpr size_t conv_tile_lds() {
+    return (size_t)18 * (TY + 2) * (NT * WAVES / TY + 2) * CIN * sizeof(uint16_t);
+}
+
+template <int CIN, int MT, int WAVES, int NT, int TY>
+__global__ __launch_bounds__(64 * WAVES) void k_conv3d_fwd_tile(const uint16_t* __restrict__ X1, int C1,
+                                                                const uint16_t* __restrict__ X2,
+                                                                const uint16_t* __restrict__ Wt,
+                                                                const float* __restrict__ bias,
+                                                                uint16_t* __restrict__ Y, Vol g, int act, int tiles_x,
+                                                                int tiles_y) {
+    constexpr int TX = 16, TZ = NT * WAVES / TY, HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HS = HX * HY * HZ;
+    constexpr int S = CIN, Cout = MT * 16, K = 27 * CIN, KC = (K + 31) / 32, NTH = 64 * WAVES, PF = 2;
+    static_assert(TZ * TY == NT * WAVES, "rows of the block");
+    extern __shared__ uint16_t halo[];
+    const int bx = blockIdx.x % tiles_x, rest = blockIdx.x / tiles_x;
+    const int x0 = bx * TX, y0 = (rest % tiles_y) * TY, z0 = (rest / tiles_y) * TZ;
+    const int C2 = CIN - C1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kg = lane >> 4;
+    const bf8 zero8 = {};
+    auto wload = [&](int kc, int m) -> bf8 {
+        const int k = kc * 32 + kg * 8;
+        const bool in = kc < KC && ((K % 32 == 0) || k < K);
+        const int tap = in ? k / CIN : 0, ci = k - tap * CIN;
+        return in ? *reinterpret_cast<const bf8*>(Wt + ((size_t)tap * Cout + m * 16 + col) * CIN + ci) : zero8;
+    };
+    bf8 wq[PF][MT];
+#pragma unroll
+    for (int p = 0; p < PF; p++)
+#pragma unroll
+        for (int m = 0; m < MT; m++) wq[p][m] = wload(p, m);
+    // halo fill, in batches of up to 8 16-byte loads per lane, all issued before their stores
+    constexpr int CH = CIN / 8, FILL = (HS * CH + NTH - 1) / NTH, FB = FILL < 8 ? FILL : 8;
+#pragma unroll
+    for (int it0 = 0; it0 < FILL; it0 += FB) {
+        uint4 buf[FB];
+#pragma unroll
+        for (int j = 0; j < FB; j++) {
+            const int i = threadIdx.x + (it0 + j) * NTH;
+            buf[j] = uint4{0u, 0u, 0u, 0u};
+            if (it0 + j < FILL && i < HS * CH) {
+                const int slot = i / CH, c8 = (i - slot * CH) * 8;
+                const int hx = slot % HX, r = slot / HX, hy = r % HY, hz = r / HY;
+                const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+                if (x >= 0 && x < g.W && y >= 0 && y < g.H && z >= 0 && z < g.D) {
+                    const size_t v = ((size_t)z * g.H + y) * g.W + x;
+                    buf[j] = (c8 < C1) ? *reinterpret_cast<const uint4*>(X1 + v * C1 + c8)
+                                       : *reinterpret_cast<const uint4*>(X2 + v * C2 + (c8 - C1));
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < FB; j++) {
+            const int i = threadIdx.x + (it0 + j) * NTH;
+            if (it0 + j < FILL && i < HS * CH) {
+                const int slot = i / CH, c8 = (i - slot * CH) * 8;
+                *reinterpret_cast<uint4*>(halo + slot * S + c8) = buf[j];
+            }
+        }
+    }
+    __syncthreads();
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+        const int k = kc * 32 + kg * 8;
+        const bool kin = (K % 32 == 0) || k < K;
+        const int tap = kin ? k / CIN : 0, ci = k - tap * CIN;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;   // halo coordinates start one voxel before the block
+        bf8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            a[m] = wq[kc % PF][m];
+            wq[kc % PF][m] = wload(kc + PF, m);
+        }
+        bf8 b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int row = wave * NT + t, yy = row % TY, zz = row / TY;
+            const int slot = ((zz + dz) * HY + (yy + dy)) * HX + col + dx;
+            b[t] = kin ? *reinterpret_cast<const bf8*>(halo + slot * S + ci) : zero8;
+        }
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int row = wave * NT + t;
+        const int x = x0 + col, y = y0 + row % TY, z = z0 + row / TY;
+        if (x >= g.W || y >= g.H || z >= g.D) continue;
+        const size_t v = ((size_t)z * g.H + y) * g.W + x;
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const int co = m * 16 + kg * 4;
+            uint16_t o[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float val = acc[m][t][j] + (bias ? bias[co + j] : 0.f);
+                if (act == 1) val = fmaxf(val, 0.f);
+                o[j] = f2bf(val);
+            }
+            uint2 pk;
+            pk.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+            pk.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            *reinterpret_cast<uint2*>(Y + v * Cout + co) = pk;
+        }
+    }
+}
+
 // dW[tap][co][ci] += sum over this wave's voxels of dY[v][co] * X[v + off(tap)][ci]. grid = (chunks, 27).
 // ROWS (W % 16 == 0, vox_per_wave % 16 == 0): a 16-voxel K block lies in one image row, so the row's
 // (y, z) and the tap's row offset are found once per block and each voxel's neighbour is v + offset with
@@ -397,6 +530,52 @@ static void launch_fwd(const uint16_t* X1, int C1, const uint16_t* X2, int C2, c
                        C2, W, b, Y, g, act);
 }
 
+template <int CIN, int MT, int WAVES, int NT, int TY>
+static int launch_fwd_tile(const uint16_t* X1, int C1, const uint16_t* X2, const uint16_t* W, const float* b,
+                           uint16_t* Y, Vol g, int act, hipStream_t s) {
+    constexpr int TZ = NT * WAVES / TY;
+    const int tx = (g.W + 15) / 16, ty = (g.H + TY - 1) / TY, tz = (g.D + TZ - 1) / TZ;
+    const size_t lds = conv_tile_lds<CIN, WAVES, NT, TY>();
+    static bool attr = false;   // above 64 KiB of LDS the kernel must opt in once
+    if (!attr) {
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_conv3d_fwd_tile<CIN, MT, WAVES, NT, TY>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_conv3d_fwd_tile<CIN, MT, WAVES, NT, TY>), dim3((unsigned)(tx * ty * tz)), dim3(64 * WAVES), lds,
+                       s, X1, C1, X2, W, b, Y, g, act, tx, ty);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+// Tile shape by the number of blocks the volume gives (see k_conv3d_fwd_tile).
+template <int CIN, int MT>
+static int launch_fwd_shape(const uint16_t* X1, int C1, const uint16_t* X2, const uint16_t* W, const float* b,
+                            uint16_t* Y, Vol g, int act, hipStream_t s) {
+    const long bx = (g.W + 15) / 16;
+    constexpr bool big_fits = conv_tile_lds<CIN, 8, 4, 8>() <= 160 * 1024;   // LDS of one CU
+    if (big_fits && bx * ((g.H + 7) / 8) * ((g.D + 3) / 4) >= 512)
+        return launch_fwd_tile<CIN, MT, big_fits ? 8 : 4, 4, big_fits ? 8 : 4>(X1, C1, X2, W, b, Y, g, act, s);
+    if (bx * ((g.H + 3) / 4) * ((g.D + 3) / 4) >= 256)
+        return launch_fwd_tile<CIN, MT, 4, 4, 4>(X1, C1, X2, W, b, Y, g, act, s);
+    return launch_fwd_tile<CIN, MT, 4, 1, 4>(X1, C1, X2, W, b, Y, g, act, s);
+}
+
+template <int CIN>
+static int dispatch_fwd_tile(int MT, const uint16_t* X1, int C1, const uint16_t* X2, const uint16_t* W, const float* b,
+                             uint16_t* Y, Vol g, int act, hipStream_t s) {
+    switch (MT) {
+        case 1: return launch_fwd_shape<CIN, 1>(X1, C1, X2, W, b, Y, g, act, s);
+        case 2: return launch_fwd_shape<CIN, 2>(X1, C1, X2, W, b, Y, g, act, s);
+        case 3: return launch_fwd_shape<CIN, 3>(X1, C1, X2, W, b, Y, g, act, s);
+        case 4: return launch_fwd_shape<CIN, 4>(X1, C1, X2, W, b, Y, g, act, s);
+        case 6:   // 96 x 96 would spill: it takes the register kernel
+            if constexpr (CIN != 96) return launch_fwd_shape<CIN, 6>(X1, C1, X2, W, b, Y, g, act, s);
+            break;
+    }
+    return OMG_ERR_UNSUPPORTED;
+}
+
 template <int MT, int NT>
 static void launch_wgrad(const uint16_t* dY, const uint16_t* X1, int C1, const uint16_t* X2, int C2, float* dW, Vol g,
                          hipStream_t s) {
@@ -423,6 +602,22 @@ extern "C" int omg_conv3d_forward(const uint16_t* X1, int32_t C1, const uint16_t
                 "inputs must have a multiple of 16 channels");
     const Vol g{D, H, Wd};
     hipStream_t s = as_stream(stream);
+    OMG_REQUIRE(Cout == 16 || Cout == 32 || Cout == 48 || Cout == 64 || Cout == 96,
+                "Cout %d is not one of 16, 32, 48, 64, 96", Cout);
+    // the LDS-tiled K = 32 kernel (16-byte operand loads) for the input widths the network uses; other widths
+    // and pointers that are not 16-byte aligned take the register kernel
+    const bool al16 = ((uintptr_t)X1 % 16 == 0) && (C2 == 0 || (uintptr_t)X2 % 16 == 0) && ((uintptr_t)W % 16 == 0) &&
+                      ((uintptr_t)Y % 8 == 0);
+    int rc = OMG_ERR_UNSUPPORTED;
+    switch (al16 ? C1 + C2 : 0) {
+        case 16: rc = dispatch_fwd_tile<16>(Cout / 16, X1, C1, X2, W, bias, Y, g, relu, s); break;
+        case 32: rc = dispatch_fwd_tile<32>(Cout / 16, X1, C1, X2, W, bias, Y, g, relu, s); break;
+        case 48: rc = dispatch_fwd_tile<48>(Cout / 16, X1, C1, X2, W, bias, Y, g, relu, s); break;
+        case 64: rc = dispatch_fwd_tile<64>(Cout / 16, X1, C1, X2, W, bias, Y, g, relu, s); break;
+        case 96: rc = dispatch_fwd_tile<96>(Cout / 16, X1, C1, X2, W, bias, Y, g, relu, s); break;
+        default: break;
+    }
+    if (rc != OMG_ERR_UNSUPPORTED) return rc;
     switch (Cout) {
         case 16: launch_fwd<1>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;
         case 32: launch_fwd<2>(X1, C1, X2, C2, W, bias, Y, g, relu, s); break;

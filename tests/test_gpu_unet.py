@@ -76,6 +76,39 @@ def test_conv_forward_dx_dw(cuda, cins, cout, dhw):
     assert np.abs(dw.cpu().numpy() - ref_dw).max() <= 1e-4 * np.abs(ref_dw).max()
 
 
+@pytest.mark.parametrize("dhw,cins,cout", [
+    ((32, 128, 64), (16,), 16),      # 8-wave blocks of 16 x 8 x 4 (the full-resolution layers)
+    ((32, 128, 64), (32, 16), 16),   # the same with two inputs, 48 channels
+    ((16, 64, 64), (16,), 32),       # 4-wave blocks of 16 x 4 x 4
+    ((16, 64, 64), (64, 32), 32),    # 96 input channels
+    ((16, 32, 32), (64,), 64),       # one-row blocks (small volumes)
+    ((13, 30, 40), (32,), 32),       # partial one-row blocks at every edge
+    ((34, 126, 70), (16,), 16),      # partial 8-wave blocks at every edge
+])
+def test_conv_forward_tile_shapes(cuda, dhw, cins, cout):
+    """The LDS-tiled convolution on volumes that select each of its block shapes, against the float64
+    oracle with the bound of test_conv_forward_dx_dw."""
+    import torch
+    rng = np.random.default_rng(7 + sum(cins) + cout)
+    D, H, W = dhw
+    cin = sum(cins)
+    print(f"[unet tile] {dhw} cins={cins} cout={cout}", flush=True)
+    Xs = [U.bf16_round(rng.normal(size=(D, H, W, c))) for c in cins]
+    Wt = U.bf16_round(rng.normal(0, 0.2, size=(27, cout, cin)))
+    b = rng.normal(0, 0.1, size=cout).astype(np.float32)
+    X = np.concatenate(Xs, -1)
+    ref = np.maximum(U.conv3d(X, Wt, b), 0.0)
+    xt = [_bf(torch, x, cuda) for x in Xs]
+    wt = _bf(torch, Wt, cuda)            # held: the kernel reads them after the call returns
+    bt = torch.from_numpy(b).to(cuda)
+    y = torch.empty((D * H * W, cout), dtype=torch.bfloat16, device=cuda)
+    _lib.call("omg_conv3d_forward", _p(xt[0]), cins[0], _p(xt[1] if len(xt) > 1 else None),
+              cins[1] if len(cins) > 1 else 0, _p(wt), _p(bt), _p(y), cout, D, H, W, 1, None)
+    got = y.float().cpu().numpy().reshape(D, H, W, cout)
+    acc = 27 * cin * 2.0 ** -24 * U.conv3d(np.abs(X), np.abs(Wt), np.abs(b))
+    assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + acc), np.abs(got - ref).max()
+
+
 def test_network_forward_and_gradients(cuda):
     import torch
     nx = ny = 8
