@@ -32,6 +32,8 @@ if ROOT not in sys.path:
 
 METRIC = "OSEM subset-iters/s; wall-time to plain 40-iter log-lik, learned vs plain"
 HBM_PEAK_GBS = 8000.0
+EXP64_INSTS = 19                                 # FP64 VALU instructions of exp(double) on gfx950
+EXP64_PEAK = 78.6e12 / 2 / EXP64_INSTS          # exp(double) per second at the FP64 vector rate
 
 
 def parse():
@@ -43,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_traffic.txt"),
                     help="summary of the rocprofv3 --pmc HBM traffic of the dominant kernel, if present")
+    ap.add_argument("--issue", default=os.path.join(ROOT, "profiles", "siddon_planar_pet2d_issue.txt"),
+                    help="summary of the rocprofv3 --pmc VALU issue counters of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
                     choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet2d_volume", "pet3d_autograd",
                              "pet2d_tof", "unet"],
@@ -151,6 +155,10 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel_name, "launches_timed": int(cnt.value),
                 "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    issue = None if args.voxel_bp else read_issue(args.issue)
+    if issue:
+        # the kernel's real limiter: its VALU issue rate, from the committed counter summary
+        roofline["issue_bound"] = issue
 
     cpu = None
     if not args.no_cpu:
@@ -272,6 +280,16 @@ def bench_frames(args):
         b.synchronize()
         t_ser += a.elapsed_time(b) * 1e-3
     serial_step_s = t_ser / min(F, 4) * F
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        _progress("frames: CPU baseline")
+        case = O.SiddonCase(p.grid, p.nRowsD, p.nColsD, p.x, p.z, plane_offset=int(p.nMeas[0]))
+        rate, per, reps = _cpu_subset_iteration(p, y[0, :M0].cpu().numpy(), case, M0, args.cpu_seconds)
+        cpu = {"value": round(rate, 3), "unit": "frame-subset-iters/s", "cores": _threads(), "kind": "port",
+               "sample": f"{reps} OSEM subset-iterations of frame 0, subset 0 ({per:.2f} s each) with the oracle's "
+                         f"Siddon projector, one frame after another as the reference's frame loop does; OpenMP "
+                         f"threads={_threads()}; {_cpu_model()}"}
     _common(args, {
         "metric": "dynamic PET OSEM frame-subset-iterations/s (all frames per launch, hipGraph, autotuned)",
         "value": round(fsi, 2), "unit": "frame-subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
@@ -289,7 +307,7 @@ def bench_frames(args):
         "serial_frames_subset_step_ms": round(serial_step_s * 1e3, 2),
         "batched_subset_step_ms": round(step_s * 1e3, 2),
         "speedup_vs_serial_frames": round(serial_step_s / step_s, 2),
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     })
 
 
@@ -329,9 +347,16 @@ def bench_cgls(args):
     b.record()
     b.synchronize()
     bp_s = a.elapsed_time(b) * 1e-3
-    n = 256
-    sat_bytes = 4.0 * ((n + 1) * (n + 1) * n) * 2
-    fp_bytes = sat_bytes + 4.0 * yy.numel() + 4.0 * p.grid.N
+    # algorithmic bytes of one FP: 4 B per summed-area-table tap. Each detector pixel sums over the slabs of
+    # the volume's dominant axis; grouping 8 slabs per lane as the reference does (projectorType5.cl:64-337)
+    # costs (4 + 2 * 7) / 8 = 2.25 taps per pixel and slab; plus the 4 B output per pixel.
+    slabs = max(p.grid.Nx, p.grid.Ny)
+    taps = yy.numel() * slabs * (4.0 + 2.0 * 7) / 8.0
+    fp_bytes = 4.0 * taps + 4.0 * yy.numel()
+    cpu = None
+    if not args.no_cpu:
+        _progress("cgls: CPU baseline")
+        cpu = _cpu_cgls(p, mu, args.cpu_seconds)
     _common(args, {
         "metric": "CBCT CGLS iterations/s (branchless distance-driven FP/BP)", "value": round(n_iter * args.steps / dt, 3),
         "unit": "CGLS-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
@@ -343,9 +368,63 @@ def bench_cgls(args):
         "roofline": {"bound": "hbm", "achieved": round(fp_bytes / fp_s / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(fp_bytes / fp_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "forward projection (k_sat_* + k_bdd_fp)", "avg_launch_us": round(fp_s * 1e6, 1),
-                     "algorithmic_bytes_per_launch": int(fp_bytes)},
-        "cpu_baseline": None,
+                     "algorithmic_bytes_per_launch": int(fp_bytes), "taps_per_fp": int(taps)},
+        "cpu_baseline": cpu,
     })
+
+
+def _threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def _cpu_model():
+    try:
+        return [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        return "unknown"
+
+
+def _cpu_cgls(p, mu, budget_s):
+    """The oracle's distance-driven FP and BP (OpenMP) on the first views of the same CBCT geometry, scaled
+    to one CGLS iteration (one FP and one BP of all views)."""
+    from oracle import oracle as O
+    threads = _threads()
+    nv = 2
+    while True:
+        case = O.BDDCase(p.grid, p.nRowsD, p.nColsD, p.x[: nv * 6], p.z[: nv * 2], p.dPitchX, p.dPitchY)
+        t = time.perf_counter()
+        y = case.fp(mu, nthreads=threads)
+        t_fp = time.perf_counter() - t
+        t = time.perf_counter()
+        case.bp(y, nthreads=threads)
+        t_bp = time.perf_counter() - t
+        if (t_fp + t_bp) * 4 > budget_s or nv >= 32:
+            break
+        nv *= 2
+    per_iter = (t_fp + t_bp) * p.nProjections / nv
+    return {"value": round(1.0 / per_iter, 5), "unit": "CGLS-iters/s", "cores": threads, "kind": "port",
+            "sample": f"oracle distance-driven FP ({t_fp:.2f} s) and BP ({t_bp:.2f} s) of {nv} of the "
+                      f"{p.nProjections} views at 256^3, scaled to all views; OpenMP threads={threads}; {_cpu_model()}"}
+
+
+def _cpu_subset_iteration(p, meas, case, n, budget_s):
+    """One OSEM subset-iteration (FP, ratio, BP with sensitivity, update) of an oracle case on the host cores;
+    repeated while the budget allows. Returns (subset-iterations/s, seconds per subset-iteration, repeats)."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = _threads()
+    x = np.ones(p.grid.N, np.float32)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        yhat = case.fp(x, n, nthreads=threads)
+        r = O.ratio(meas, yhat, 1e-5)
+        rhs, sens = case.bp(r, n, sens=True, nthreads=threads)
+        x = O.osem_update(x, O.sens_clamp(sens, 1e-5), rhs, 1e-5)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el * (reps + 1) / reps > budget_s:
+            break
+    return reps / el, el / reps, reps
 
 
 def bench_orth(args, volume=False):
@@ -399,6 +478,19 @@ def bench_orth(args, volume=False):
     b.synchronize()
     s_s = a.elapsed_time(b) * 1e-3
     byt = 12.0 * ntube + 8.0 * M0
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        _progress("orth: CPU baseline")
+        vol = (p.V, p.bmin, p.bmax, p.Vmax) if volume else None
+        width = float(np.float32(p.tube_radius)) if volume else 2.5
+        case = O.OrthCase(p.grid, p.nRowsD, p.nColsD, p.x, p.z, width, flags=3, plane_offset=int(p.nMeas[0]),
+                          volume=vol)
+        rate, per, reps = _cpu_subset_iteration(p, y[:M0].cpu().numpy(), case, M0, args.cpu_seconds)
+        cpu = {"value": round(rate, 3), "unit": "subset-iters/s", "cores": _threads(), "kind": "port",
+               "sample": f"{reps} OSEM subset-iterations of subset 0 ({per:.2f} s each) with the oracle's "
+                         f"{'volume-of-intersection' if volume else 'orthogonal'} projector; OpenMP threads="
+                         f"{_threads()}; {_cpu_model()}"}
     _common(args, {
         "metric": "OSEM subset-iters/s, " + ("volume-of-intersection" if volume else "orthogonal distance-based")
                   + " projector", "value": round(p.subsets * args.steps / dt, 2),
@@ -413,7 +505,7 @@ def bench_orth(args, volume=False):
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_orth_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     })
 
 
@@ -483,11 +575,17 @@ def bench_tof(args):
         "config": {"workload": "pet2d_osem_tof", "image": [128, 128, 63], "sinogram": [128, 160, 63],
                    "tof_bins": p.nBins, "tof_fwhm_ps": 400, "tof_bin_ps": 140, "attenuation": "water cylinder",
                    "subsets": 8, "voxel_visits_subset0": ntr, "geometry_setup_ms": round(setup_ms, 1)},
-        "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+        # bound: the double-precision exponentials. exp(double) compiles for gfx950 to 19 FP64 VALU
+        # instructions (13 FMA, mul, rndne, ldexp, cvt, 2 compares); at the FP64 vector rate of 78.6 TFLOP/s
+        # (39.3 T lane-instructions/s) that is 2.07e12 exp/s with no other work
+        "roofline": {"bound": "fp64", "achieved": round(exps / s_s / 1e9, 1), "peak": round(EXP64_PEAK / 1e9, 1),
+                     "unit": "G exp/s", "frac": round(exps / s_s / EXP64_PEAK, 4), "traffic": None,
+                     "peak_basis": f"78.6 TFLOP/s FP64 / 2 = 39.3e12 lane-instructions/s over {EXP64_INSTS} "
+                                   "FP64 instructions per exp(double)",
                      "kernel": "k_tof_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
-                     "algorithmic_bytes_per_launch": int(byt),
-                     "fp64_exp_per_s": round(exps / s_s, 1)},
+                     "exps_per_launch": int(exps),
+                     "hbm": {"algorithmic_bytes_per_launch": int(byt), "achieved": round(byt / s_s / 1e9, 1),
+                             "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
     })
 
@@ -542,6 +640,27 @@ def bench_autograd(args):
         fd = (float(AG.penalised_nll(xp, p, meas)) - float(AG.penalised_nll(xm, p, meas))) / (2 * h)
         errs.append(abs(fd - grad[j]) / max(1.0, abs(grad[j])))
     _progress(f"autograd: finite differences max {max(errs):.2e}")
+    # roofline of the pass: one FP and one BP of the whole sinogram; algorithmic bytes 4 B per (LOR, voxel)
+    # visit for the FP gather, 8 B for the BP update, 8 B per LOR (measurement and table)
+    import ctypes as C
+    M = int(p.nMeasSubset[0, 0])
+    visits = _trace_counts(p, 0, dev)
+    byt = 12.0 * visits + 8.0 * M
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        _progress("autograd: CPU baseline")
+        per = p.nRowsD * p.nColsD
+        planes = 16
+        case = O.SiddonCase(p.grid, p.nRowsD, p.nColsD, p.x, p.z)
+        xs = np.asarray(ph, np.float32)
+        t = time.perf_counter()
+        yc = case.fp(xs, planes * per, nthreads=_threads())
+        case.bp(yc, planes * per, nthreads=_threads())
+        t_s = (time.perf_counter() - t) * (p.nProjections / planes)
+        cpu = {"value": round(1.0 / t_s, 5), "unit": "fwd+bwd/s", "cores": _threads(), "kind": "port",
+               "sample": f"oracle Siddon FP + BP of {planes} of the {p.nProjections} sinogram planes, scaled to all "
+                         f"planes (the RDP terms are negligible); OpenMP threads={_threads()}; {_cpu_model()}"}
     _common(args, {
         "metric": "differentiable 3-D PET projector: forward+backward of -L(x)+R(x) per second",
         "value": round(1.0 / dt, 3), "unit": "fwd+bwd/s", "ms_per_step": round(dt * 1e3, 2),
@@ -550,7 +669,12 @@ def bench_autograd(args):
                    "rings": 32, "crystals": 448, "max_ring_difference": 31, "gamma": 2.0, "beta": 0.1},
         "adjoint_pairs": 20, "adjoint_worst_rel_mismatch": worst,
         "fd_voxels": 64, "fd_h": h, "fd_max_rel_error": max(errs), "fd_median_rel_error": float(np.median(errs)),
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "achieved": round(byt / dt / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(byt / dt / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "forward + backward projection of the pass (k_siddon_* )",
+                     "avg_launch_us": round(dt * 1e6, 1), "algorithmic_bytes_per_launch": int(byt),
+                     "visits": visits},
+        "cpu_baseline": cpu,
     })
 
 
@@ -643,6 +767,24 @@ def read_traffic(path):
         if line.startswith("hbm_bytes_per_launch:"):
             return int(float(line.split(":", 1)[1]))
     return None
+
+
+def read_issue(path):
+    """VALU issue figures of the dominant kernel from a committed rocprofv3 --pmc summary, if present:
+    lines 'valu_busy: f' (rocprof's VALUBusy), 'valu_busy_vs_sq_busy: f', 'valu_insts_per_launch: n',
+    'waves_per_simd: w'."""
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        for key in ("valu_busy", "valu_busy_vs_sq_busy", "valu_insts_per_launch", "waves_per_simd"):
+            if line.startswith(key + ":"):
+                out[key] = float(line.split(":", 1)[1])
+    if "valu_busy" not in out:
+        return None
+    out["bound"] = "valu_issue"
+    out["source"] = os.path.relpath(path, ROOT)
+    return out
 
 
 def cpu_baseline(p, y, x0, budget_s, runner, n_iter, osem_kw=None):
