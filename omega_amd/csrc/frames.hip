@@ -16,7 +16,8 @@
 //     captured hipGraph with no memset nodes.
 // Per frame, each FP sum, ratio and BP term is computed exactly as the single-frame fused kernel
 // computes it (k_siddon_osem in siddon.hip), so frames agree with single-frame runs up to the order of
-// the float atomics.
+// the float atomics. Subsets are those of the geometry: plane subsets (types 8-11) or the measurement
+// index table of types 1-7, which siddon_setup reads per LOR as the single-frame kernels do.
 #include "geom.hpp"
 #include "rdp_device.hpp"
 
@@ -354,7 +355,6 @@ static int make_step(const omg_geom* g, int64_t first, int64_t n_meas, int32_t n
     OMG_REQUIRE(g != nullptr, "null geometry");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON, "batched frames run the improved Siddon projector");
     OMG_REQUIRE(g->d.mode != OMG_MODE_CT_FLAT, "batched frames are emission (PET) data");
-    OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
     OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     OMG_REQUIRE(x && meas && S && rhs, "null pointer argument");
@@ -412,7 +412,6 @@ extern "C" int omg_frames_sensitivity(const omg_geom* g, int64_t first, int64_t 
     OMG_REQUIRE(g != nullptr && S != nullptr, "null argument");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
                 "batched frames run the improved Siddon projector on emission data");
-    OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
     OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     const int rc = check_range(g, first, n_meas);
@@ -436,7 +435,6 @@ extern "C" int omg_frames_forward(const omg_geom* g, int64_t first, int64_t n_me
     OMG_REQUIRE(g != nullptr && x && yhat && n_frames >= 1, "bad arguments");
     OMG_REQUIRE(g->d.projector_type == OMG_PROJ_SIDDON && g->d.mode != OMG_MODE_CT_FLAT,
                 "batched frames run the improved Siddon projector on emission data");
-    OMG_REQUIRE(g->index_dev == nullptr, "batched frames take plane subsets (type 8), not a measurement index table");
     OMG_REQUIRE(g->d.tof_bins <= 1, "batched frames do not take time-of-flight data");
     OMG_REQUIRE(std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u) == 1, "batched frames trace one ray per LOR");
     OMG_REQUIRE(n_frames == 1 || yhat_stride >= n_meas, "yhat_stride must cover the subset's measurements");

@@ -53,11 +53,12 @@ def pet2d_data(p: projectorClass, seed: int = 0, total_counts: float = 5e6, devi
 
 def pet3d_projector(subsets: int = 1, n_rings: int = 32, crystals: int = 448, ring_pitch: float = 4.0,
                     max_ring_diff: int = 31, radius: float = 400.0, image=(128, 128, 63),
-                    fov_xy: float = 320.0) -> projectorClass:
+                    fov_xy: float = 320.0, subset_type: int = 8) -> projectorClass:
     """3-D PET: ``n_rings`` rings at ``ring_pitch`` mm, ``crystals`` per ring, span-1 ring pairs up to
     ``max_ring_diff``; ``crystals/2`` angles x ``crystals/2`` radial bins; image ``image`` voxels over a
     ``fov_xy`` mm transaxial FOV and the rings' axial extent. Defaults: 224 x 224 x 1024 sinogram,
-    128 x 128 x 63 image (2.5 mm transaxial, 128/63 mm axial voxels)."""
+    128 x 128 x 63 image (2.5 mm transaxial, 128/63 mm axial voxels). ``subset_type``: the reference's
+    subset types (8: every n-th sinogram plane; 4: every n-th angle of every plane; ...)."""
     sc = G.pet_3d_rings(radius=radius, n_rings=n_rings, ring_pitch=ring_pitch, crystals=crystals,
                         max_ring_diff=max_ring_diff)
     p = projectorClass()
@@ -68,7 +69,7 @@ def pet3d_projector(subsets: int = 1, n_rings: int = 32, crystals: int = 448, ri
     p.z = sc.z
     p.Ndist, p.Nang = sc.n_radial, sc.n_angles
     p.subsets = subsets
-    p.subsetType = 8
+    p.subsetType = subset_type
     p.addProjector()
     return p
 
