@@ -39,7 +39,11 @@ class FramesOSEM:
     """OSEM of ``F`` frames that share the geometry of ``proj`` (improved Siddon, emission data)."""
 
     def __init__(self, proj, meas, epps: float = 1e-5, enforce_positivity: bool = True, x0=None, beta: float = 0.0,
-                 gamma: float = 2.0):
+                 gamma: float = 2.0, support=None):
+        """``support`` (optional bool device tensor [N]): the voxels that are reconstructed. The others have
+        their sensitivity set to 1e30, so every OSEM / OSL update sends them below ``epps``, where the
+        positivity clamp holds them; they then add epps-level terms to the projections and act as near-zero
+        neighbours for the prior."""
         import torch
         self.torch = torch
         if not proj.projectorInitialized:
@@ -64,6 +68,12 @@ class FramesOSEM:
         self.dU = torch.zeros((self.F, self.N), dtype=torch.float32, device=dev) if self.beta != 0.0 else None
         self.epps = float(epps)
         self.pos = int(bool(enforce_positivity))
+        if support is not None:
+            if support.dtype != torch.bool or support.numel() != self.N or support.device != dev:
+                raise ValueError("support must be a bool tensor of N voxels on the measurements' device")
+            if not self.pos:
+                raise ValueError("a support mask relies on the positivity clamp")
+        self.support = support
         self.sens_done = False
         self._graphs = {}
 
@@ -80,6 +90,8 @@ class FramesOSEM:
         for k in range(self.proj.subsets):
             first, n, _ = self._subset(k)
             _lib.call("omg_frames_sensitivity", self.g, first, n, _p(self.S[k]), self.epps, self._stream())
+        if self.support is not None:
+            self.S[:, ~self.support] = 1e30
         self.sens_done = True
 
     def step(self, k: int, rays_per_block: int = 256, frames_per_tile: int = 8, grid_blocks: int = 0, x=None) -> None:
