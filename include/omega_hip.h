@@ -107,7 +107,9 @@ typedef struct omg_geom_desc {
      * init.py:357-360; paramStruct nRays2D, nRays3D, projector_functions.h:262-264,1444-1466): rays_xy x
      * rays_z rays (0 or 1: one), their endpoints spread over cr_xy (transaxially, on x and y alike) and
      * cr_z (axially) (paramStruct dPitchXY, dPitchZ; the operator API passes dPitchX and dPitchY); each
-     * ray weighs 1 / (L nRays), the forward projection sums them. Siddon, PET data, no TOF. */
+     * ray weighs 1 / (L nRays), the forward projection sums them (per TOF bin with TOF). Siddon, PET data.
+     * With an attenuation image each ray has its own factor, exp of its own walk's sum (the library keeps
+     * rays x measurements factors). */
     uint32_t rays_xy, rays_z; float cr_xy, cr_z;
     /* volume-of-intersection projector (projector_type 3; the reference's -DORTH -DVOL kernel,
      * opencl_functions_orth3D.h:132-140, projectorType123.cl:502-505,641-643,952-953): the tube of

@@ -130,13 +130,17 @@ static int64_t total_measurements(const omg_geom_desc& d) {
     return (int64_t)d.size_x * d.size_y * d.n_proj;
 }
 
-// Attenuation image -> one factor per measurement (g->atten_dev), computed once on the device.
+// Attenuation image -> one factor per measurement (g->atten_dev), computed once on the device; with
+// several rays per LOR one factor per ray and measurement, ray-major.
 static int attenuation_prepare(omg_geom* g, const float* mu_host, int64_t n_total) {
     const int64_t N = (int64_t)g->d.Nx * g->d.Ny * g->d.Nz;
+    const int64_t rays = (int64_t)std::max(g->d.rays_xy, 1u) * std::max(g->d.rays_z, 1u);
+    g->atten_ray_stride = rays > 1 ? n_total : 0;
     float* mu = nullptr;
     int rc = upload(mu_host, N, &mu);
-    if (!rc && hipMalloc(reinterpret_cast<void**>(&g->atten_dev), (size_t)n_total * sizeof(float)) != hipSuccess) {
-        set_error("could not allocate %lld attenuation factors", (long long)n_total);
+    if (!rc && hipMalloc(reinterpret_cast<void**>(&g->atten_dev), (size_t)(n_total * rays) * sizeof(float)) !=
+                   hipSuccess) {
+        set_error("could not allocate %lld attenuation factors", (long long)(n_total * rays));
         rc = OMG_ERR_HIP;
     }
     if (!rc) rc = g->d.projector_type == OMG_PROJ_INTERP ? interp_atten_factors(g, mu, g->atten_dev, n_total, nullptr)
@@ -261,9 +265,6 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
         OMG_REQUIRE(rays <= 256, "%llu rays per LOR exceeds 256", (unsigned long long)rays);
         OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON && d->mode != OMG_MODE_CT_FLAT,
                     "several rays per LOR are provided for the improved Siddon projector on PET data");
-        OMG_REQUIRE(d->tof_bins <= 1, "several rays per LOR together with TOF are not provided");
-        OMG_REQUIRE(d->atten_mode != 1, "several rays per LOR together with an attenuation image are not provided "
-                                        "(each ray has its own attenuation factor)");
         OMG_REQUIRE(std::isfinite(d->cr_xy) && std::isfinite(d->cr_z), "cr_xy and cr_z must be finite");
     }
     OMG_REQUIRE(d->atten_mode >= 0 && d->atten_mode <= 2, "atten_mode %d is not 0 (off), 1 (image) or 2 (per LOR)",

@@ -11,6 +11,7 @@ struct omg_geom {
     float* z_dev = nullptr;
     float* norm_dev = nullptr;
     float* atten_dev = nullptr;      // per-measurement attenuation factors (norm's layout), or null
+    int64_t atten_ray_stride = 0;    // factors of an image with several rays: one block of them per ray
     float* scat_dev = nullptr;       // per-measurement multiplicative correction (norm's layout), or null
     float* vol_dev = nullptr;        // projector 3: the volume-of-intersection table, or null
     float* tof_c = nullptr;          // TOF bin centres (tof_bins floats), or null
@@ -78,6 +79,7 @@ static inline SidArgs make_sid_args(const omg_geom* g, int64_t first, int64_t n)
     a.atten = g->atten_dev ? g->atten_dev + first * per : nullptr;
     a.scat = g->scat_dev ? g->scat_dev + first * per : nullptr;
     a.atten_lor = g->d.atten_mode == 2;
+    a.atten_ray_stride = g->atten_ray_stride;
     a.tof_bins = g->d.tof_bins > 1 ? g->d.tof_bins : 1u;
     a.tof_sigma = g->d.tof_sigma;
     a.tof_pf = g->d.tof_sigma > 0.f ? (float)0.3989422804014327 / g->d.tof_sigma : 0.f;

@@ -83,14 +83,16 @@ __global__ __launch_bounds__(kBlock) void k_siddon_fp(SidArgs a, const float* __
 // the walk's (clipped) lengths: the reference's forward-projection sum (compute_attenuation,
 // projector_functions.h:597-601, :1938-1939, :2026-2027). An axis-parallel LOR integrates mu over its
 // whole image row with the voxel size as length (perpendicular_elements, :1193-1205). A LOR that misses
-// the image gets 1. `a` carries no weights (its atten and norm are null).
+// the image gets 1. `a` carries no weights (its atten and norm are null). With several rays per LOR,
+// blockIdx.y is the ray and its factors form block ray of f (each ray's own walk, :1480, :1826, :2027).
 __global__ __launch_bounds__(kBlock) void k_atten_factor(SidArgs a, const float* __restrict__ mu,
                                                          float* __restrict__ f) {
     const int64_t lo = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (lo >= a.nmeas) return;
+    const int ray = (int)blockIdx.y;
     Walk w;
     float weight, jel = 0.f;
-    if (siddon_setup<false>(a, lo, w, weight)) {
+    if (siddon_setup<false>(a, lo, w, weight, false, ray)) {
         if (w.perp) {
             for (uint32_t i = 0; i < w.p_nrun; i++) jel += w.p_d * -mu[w.p_row + i * w.p_rstride];
         } else {
@@ -102,15 +104,16 @@ __global__ __launch_bounds__(kBlock) void k_atten_factor(SidArgs a, const float*
             }
         }
     }
-    f[lo] = expf(jel);
+    f[(int64_t)ray * a.nmeas + lo] = expf(jel);
 }
 
+// f holds n_rays blocks of n factors.
 int atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s) {
     SidArgs a = make_sid_args(g, 0, n);
     a.norm = nullptr;
     a.atten = nullptr;
     a.scat = nullptr;
-    hipLaunchKernelGGL(k_atten_factor, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a, mu, f);
+    hipLaunchKernelGGL(k_atten_factor, dim3(grid_for(n, kBlock), a.rays_xy * a.rays_z), dim3(kBlock), 0, s, a, mu, f);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }

@@ -36,6 +36,8 @@ struct SidArgs {
     const float* __restrict__ atten; // per-measurement attenuation factor, subset-local, or null
     const float* __restrict__ scat;  // per-measurement multiplicative (scatter) correction, subset-local, or null
     bool atten_lor;                  // atten holds given per-LOR factors (not factors computed from an image)
+    int64_t atten_ray_stride;        // factors made from an image with several rays per LOR: ray r's factor of
+                                     // measurement lo is atten[r * atten_ray_stride + lo]; else 0
     uint32_t tof_bins;               // time-of-flight bins (<= 1: no TOF)
     float tof_sigma, tof_pf, tof_epps;  // TOF standard deviation (mm), 0.3989422804014327f / sigma, weight-sum floor
     const float* __restrict__ tof_c; // TOF bin centres (mm), tof_bins floats
@@ -234,15 +236,19 @@ __device__ __forceinline__ bool preloop3(const SidArgs& g, const Ray& r, float x
 // its exp(jelppi) or a given per-LOR factor (:1825-1828); on axis-parallel LORs (perp) a given per-LOR
 // factor comes after norm, as there (:1209-1210).
 // MULTI: the geometry may have several rays per LOR (never in the direct-plane kernels).
+// With several rays and an attenuation image each ray has its own factor, exp(jelppi) of its own walk
+// (jelppi reset per ray, :1480, :1826, :2027).
 template <bool MULTI = true>
-__device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct, bool perp = false) {
+__device__ __forceinline__ float make_weight(const SidArgs& g, int64_t lo, float L, bool ct, bool perp = false,
+                                             int ray = 0) {
     if (ct) return 1.f;
     // 1 / (L * nRays) with several rays per LOR (:1819-1821, :1189-1190)
     float t = (MULTI && n_rays(g) > 1) ? 1.f / (L * (float)n_rays(g)) : 1.f / L;
     const bool late = perp && g.atten_lor;
-    if (g.atten && !late) t *= g.atten[lo];
+    const float* atn = g.atten ? g.atten + (int64_t)ray * g.atten_ray_stride : nullptr;
+    if (atn && !late) t *= atn[lo];
     if (g.norm) t *= g.norm[lo];
-    if (g.atten && late) t *= g.atten[lo];
+    if (atn && late) t *= atn[lo];
     if (g.scat) t *= g.scat[lo];
     t *= g.global_factor;
     return t;
@@ -339,7 +345,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         } else {
             return false;
         }
-        weight = make_weight<!PLANAR>(g, lo, L, ct, true);
+        weight = make_weight<!PLANAR>(g, lo, L, ct, true, ray);
         uint32_t apu = 0;
         {
             float s = db - dd + ddb;
@@ -390,7 +396,7 @@ __device__ __forceinline__ bool siddon_setup(const SidArgs& g, int64_t lo, Walk&
         skip = preloop3(g, r, xdf, ydf, zdf, ti, tj, tk, tyu, txu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0);
     }
     if (skip) return false;
-    weight = make_weight<!PLANAR>(g, lo, L, ct);
+    weight = make_weight<!PLANAR>(g, lo, L, ct, false, ray);
     if (!PLANAR && g.tof_bins > 1) {   // TOFDis (projector_functions.h:588-594, :1741-1742)
         const float xI = xdf * tc, yI = ydf * tc, zI = zdf * tc;
         w.tofD = sqrtf(xI * xI + yI * yI + zI * zI) - L / 2.f;
@@ -487,9 +493,10 @@ __device__ __forceinline__ float tof_weight(const SidArgs& g, float elem, float 
 // have partial lengths, :1664-1673). D moves by each step's length (:1990-1991; rows: the voxel size,
 // :1684-1685), also across steps of length 0. Returns false when the LOR contributes nothing.
 template <typename Visit>
-__device__ __forceinline__ bool siddon_tof_trace(const SidArgs& g, int64_t lo, float& weight, Visit&& visit) {
+__device__ __forceinline__ bool siddon_tof_trace(const SidArgs& g, int64_t lo, float& weight, Visit&& visit,
+                                                 int ray = 0) {
     Walk w;
-    if (!siddon_setup<false>(g, lo, w, weight)) return false;
+    if (!siddon_setup<false>(g, lo, w, weight, false, ray)) return false;
     float D = w.tofD;
     const float DD = D;
     if (w.perp) {

@@ -11,6 +11,10 @@
 // One lane per LOR, one wave per workgroup. A lane's per-bin accumulators and weights live in LDS,
 // laid out [bin][lane] (conflict-free), so any bin count up to kTofMax runs without register arrays
 // indexed at run time (which would go to scratch memory).
+//
+// Several rays per LOR (:1441-1446, :2036-2056): the forward projection sums each ray's bins times that
+// ray's weight (1 / (L nRays), its own attenuation factor, norm, ...) into a fourth per-lane LDS array,
+// in ray order; the backprojection traces every ray with its own weight.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -67,7 +71,7 @@ __device__ __forceinline__ void tof_bp_weights(const SidArgs& a, float len, floa
 
 // FP walk: acc[b] += (len * src[v]) * w_b / sum over the LOR's voxels (denominator, :609-630).
 __device__ __forceinline__ bool tof_gather(const SidArgs& a, int64_t lo, const float* __restrict__ src, float* acc,
-                                           float* wl, TofCache c, float& w) {
+                                           float* wl, TofCache c, float& w, int ray = 0) {
     for (uint32_t b = 0; b < a.tof_bins; b++) acc[b * kTofBlock] = 0.f;
     *c.x = __builtin_nanf("");
     return siddon_tof_trace(a, lo, w, [&](uint32_t ind, float len, float D, float DD, float esum) {
@@ -75,12 +79,12 @@ __device__ __forceinline__ bool tof_gather(const SidArgs& a, int64_t lo, const f
         const float sum = tof_weights(a, esum, D, DD, wl, c);
         if (esum != len) tof_weights(a, len, D, DD, wl, c);
         for (uint32_t b = 0; b < a.tof_bins; b++) acc[b * kTofBlock] = acc[b * kTofBlock] + apu * wl[b * kTofBlock] / sum;
-    });
+    }, ray);
 }
 
 // BP walk: rhs[v] += sum_b (len w) (w_b / sum) yv[b]; sens[v] += sum_b (len w) (w_b / sum).
 __device__ __forceinline__ void tof_scatter(const SidArgs& a, int64_t lo, const float* yv, float* wl, TofCache c,
-                                            float* __restrict__ rhs, float* __restrict__ sens) {
+                                            float* __restrict__ rhs, float* __restrict__ sens, int ray = 0) {
     float w = 0.f;
     *c.x = __builtin_nanf("");
     siddon_tof_trace(a, lo, w, [&](uint32_t ind, float len, float D, float DD, float esum) {
@@ -95,11 +99,31 @@ __device__ __forceinline__ void tof_scatter(const SidArgs& a, int64_t lo, const 
         }
         unsafeAtomicAdd(rhs + ind, yax);
         if (sens) unsafeAtomicAdd(sens + ind, val);
-    });
+    }, ray);
+}
+
+// Several rays: tot[b] = sum over the rays (in order) of the ray's bin b times its weight; false when
+// every ray misses the image.
+__device__ __forceinline__ bool tof_gather_rays(const SidArgs& a, int64_t lo, const float* __restrict__ src, float* acc,
+                                                float* wl, TofCache c, float* tot) {
+    for (uint32_t b = 0; b < a.tof_bins; b++) tot[b * kTofBlock] = 0.f;
+    bool any = false;
+    for (uint32_t ray = 0; ray < n_rays(a); ray++) {
+        float w = 0.f;
+        if (!tof_gather(a, lo, src, acc, wl, c, w, (int)ray)) continue;
+        any = true;
+        for (uint32_t b = 0; b < a.tof_bins; b++) tot[b * kTofBlock] = tot[b * kTofBlock] + acc[b * kTofBlock] * w;
+    }
+    return any;
 }
 
 __device__ __forceinline__ TofCache tof_cache(const SidArgs& a, float* lds) {
     return TofCache{lds + 3 * a.tof_bins * kTofBlock + threadIdx.x, lds + 2 * a.tof_bins * kTofBlock + threadIdx.x};
+}
+
+// the rays' bin sums, after the cache's argument slot
+__device__ __forceinline__ float* tof_tot(const SidArgs& a, float* lds) {
+    return lds + (3 * a.tof_bins + 1) * kTofBlock + threadIdx.x;
 }
 
 // y[lo + b n] = w * acc[b]
@@ -110,6 +134,12 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_fp(SidArgs a, const float* __
     float* acc = lds + threadIdx.x;
     float* wl = lds + a.tof_bins * kTofBlock + threadIdx.x;
     const TofCache c = tof_cache(a, lds);
+    if (n_rays(a) > 1) {
+        float* tot = tof_tot(a, lds);
+        tof_gather_rays(a, lo, img, acc, wl, c, tot);
+        for (uint32_t b = 0; b < a.tof_bins; b++) y[lo + (int64_t)b * a.nmeas] = tot[b * kTofBlock];
+        return;
+    }
     float w = 0.f;
     const bool hit = tof_gather(a, lo, img, acc, wl, c, w);
     for (uint32_t b = 0; b < a.tof_bins; b++) y[lo + (int64_t)b * a.nmeas] = hit ? acc[b * kTofBlock] * w : 0.f;
@@ -130,7 +160,7 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_bp(SidArgs a, const float* __
         any |= v != 0.f;
     }
     if (!any && sens == nullptr) return;   // adds of +0 leave rhs unchanged
-    tof_scatter(a, lo, yv, wl, c, rhs, sens);
+    for (uint32_t ray = 0; ray < n_rays(a); ray++) tof_scatter(a, lo, yv, wl, c, rhs, sens, (int)ray);
 }
 
 // Fused data step: yhat_b = w acc_b; r_b = meas_b / ((yhat_b + epps) + add); NaN/Inf -> epps; BP of r.
@@ -143,6 +173,17 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_osem(SidArgs a, const float* 
     float* acc = lds + threadIdx.x;
     float* wl = lds + a.tof_bins * kTofBlock + threadIdx.x;
     const TofCache c = tof_cache(a, lds);
+    if (n_rays(a) > 1) {   // every ray missing the image: nothing to backproject, as with one ray
+        float* tot = tof_tot(a, lds);
+        if (!tof_gather_rays(a, lo, img, acc, wl, c, tot)) return;
+        for (uint32_t b = 0; b < a.tof_bins; b++) {
+            float r = meas[lo + (int64_t)b * a.nmeas] / ratio_den(a, lo, tot[b * kTofBlock], epps);
+            if (isnan(r) || isinf(r)) r = epps;
+            tot[b * kTofBlock] = r;
+        }
+        for (uint32_t ray = 0; ray < n_rays(a); ray++) tof_scatter(a, lo, tot, wl, c, rhs, sens, (int)ray);
+        return;
+    }
     float w = 0.f;
     if (!tof_gather(a, lo, img, acc, wl, c, w)) return;
     for (uint32_t b = 0; b < a.tof_bins; b++) {
@@ -155,10 +196,29 @@ __global__ __launch_bounds__(kTofBlock) void k_tof_osem(SidArgs a, const float* 
 }
 
 // LDS per workgroup: [acc or yv: bins x lanes][wl: bins x lanes][cache p: bins x lanes][cache x: lanes]
-static size_t tof_lds(const SidArgs& a) { return (3 * (size_t)a.tof_bins + 1) * kTofBlock * sizeof(float); }
+// and, with several rays per LOR, [tot: bins x lanes]: at most (4 * 64 + 1) * 64 * 4 B = 65,792 B.
+static size_t tof_lds(const SidArgs& a) {
+    return (3 * (size_t)a.tof_bins + 1 + (a.rays_xy * a.rays_z > 1 ? (size_t)a.tof_bins : 0)) * kTofBlock * sizeof(float);
+}
+
+// Lifts the three kernels' dynamic LDS limit above the default 64 KiB once (several rays, > 63 bins).
+static int tof_lds_attr() {
+    static bool done = false;
+    if (done) return OMG_OK;
+    const int bytes = (int)((4 * (size_t)kTofMax + 1) * kTofBlock * sizeof(float));
+    OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tof_fp),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tof_bp),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tof_osem),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    done = true;
+    return OMG_OK;
+}
 
 int tof_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
     const SidArgs a = make_sid_args(g, first, n);
+    if (int rc = tof_lds_attr()) return rc;
     hipLaunchKernelGGL(k_tof_fp, dim3(grid_for(n, kTofBlock)), dim3(kTofBlock), tof_lds(a), s, a, img, y);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -167,6 +227,7 @@ int tof_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, f
 int tof_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
                  hipStream_t s) {
     const SidArgs a = make_sid_args(g, first, n);
+    if (int rc = tof_lds_attr()) return rc;
     hipLaunchKernelGGL(k_tof_bp, dim3(grid_for(n, kTofBlock)), dim3(kTofBlock), tof_lds(a), s, a, yin, rhs, sens);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -176,6 +237,7 @@ int tof_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, cons
              float* sens, float epps, hipStream_t s, const float* add) {
     SidArgs a = make_sid_args(g, first, n);
     a.add = add;
+    if (int rc = tof_lds_attr()) return rc;
     hipLaunchKernelGGL(k_tof_osem, dim3(grid_for(n, kTofBlock)), dim3(kTofBlock), tof_lds(a), s, a, img, meas, rhs,
                        sens, epps);
     OMG_CHECK_LAUNCH();

@@ -501,18 +501,29 @@ static void siddon_fp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
     for (int64_t lo = 0; lo < nMeas; lo++) {
         T w = (T)0.;
         if (g.tof_bins > 1) {
-            // denominator with TOF (projector_functions.h:609-630); bins in blocks of nMeas (:2030-2035)
-            std::vector<T> ax(g.tof_bins, (T)0.);
-            const bool hit = siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len, T D, T DD, T esum) {
-                const T apu = len * img[ind];
-                const T sum = tof_loop(g, DD, esum, D);
-                const T dX = len / (T)(4. - 1.);
-                for (uint32_t to = 0; to < g.tof_bins; to++) {
-                    const T joku = (tof_weight(g, len, D, DD, g.tof_c[to], dX) * dX);
-                    ax[to] += apu * joku / sum;
+            // denominator with TOF (projector_functions.h:609-630); bins in blocks of nMeas (:2030-2035).
+            // Several rays: each ray's bins times that ray's weight, then the rays added per bin in ray
+            // order (ax[to + nBins * lor] *= temp, :2036-2056)
+            const int nr = (int)(g.rays_xy * g.rays_z);
+            std::vector<T> ax(g.tof_bins), tot(g.tof_bins, (T)0.);
+            for (int ray = 0; ray < nr; ray++) {
+                std::fill(ax.begin(), ax.end(), (T)0.);
+                const bool hit = siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len, T D, T DD, T esum) {
+                    const T apu = len * img[ind];
+                    const T sum = tof_loop(g, DD, esum, D);
+                    const T dX = len / (T)(4. - 1.);
+                    for (uint32_t to = 0; to < g.tof_bins; to++) {
+                        const T joku = (tof_weight(g, len, D, DD, g.tof_c[to], dX) * dX);
+                        ax[to] += apu * joku / sum;
+                    }
+                }, 1, (T*)nullptr, ray);
+                if (nr == 1) {
+                    for (uint32_t to = 0; to < g.tof_bins; to++) tot[to] = hit ? ax[to] * w : (T)0.;
+                } else if (hit) {
+                    for (uint32_t to = 0; to < g.tof_bins; to++) tot[to] += ax[to] * w;
                 }
-            });
-            for (uint32_t to = 0; to < g.tof_bins; to++) y[lo + (int64_t)to * nMeas] = hit ? ax[to] * w : (T)0.;
+            }
+            for (uint32_t to = 0; to < g.tof_bins; to++) y[lo + (int64_t)to * nMeas] = tot[to];
             continue;
         }
         if (g.rays_xy * g.rays_z > 1) {
@@ -541,7 +552,8 @@ static void siddon_bp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
         T w = (T)0.;
         if (g.tof_bins > 1) {
             // rhs with TOF (projector_functions.h:632-657): dX = elem / (TRAPZ_BINS - (T)1.) is a double
-            // division rounded to T
+            // division rounded to T; each ray backprojects with its own weight
+            for (int ray = 0; ray < (int)(g.rays_xy * g.rays_z); ray++)
             siddon_lor(g, lo, x, z, w, [&](uint32_t ind, T len, T D, T DD, T esum) {
                 const T lw = len * w;
                 const T sum = tof_loop(g, DD, esum, D);
@@ -558,7 +570,7 @@ static void siddon_bp(const Geo<T>& g, int64_t nMeas, const T* x, const T* z, co
 #pragma omp atomic
                     sens[ind] += val;
                 }
-            }, 2);
+            }, 2, (T*)nullptr, ray);
             continue;
         }
         const T yi = yin[lo];

@@ -152,16 +152,13 @@ def test_create_rejects_bad_tof():
 
 
 def test_create_rejects_bad_multiray():
-    """Several rays per LOR: Siddon on PET data, without TOF or an attenuation image, at most 256."""
+    """Several rays per LOR: Siddon on PET data (with or without TOF and attenuation), at most 256."""
     lib = _lib.load()
     h = C.c_void_p()
-    c = np.zeros(3, np.float32)
-    img = np.zeros(64, np.float32)
     checks = [
         (dict(rays_xy=17, rays_z=16), "exceeds 256"),
         (dict(rays_xy=2, projector_type=2), "improved Siddon"),
-        (dict(rays_xy=2, tof_bins=3, tof_sigma=10.0, tof_centers=c.ctypes.data), "TOF"),
-        (dict(rays_xy=2, atten_mode=1, atten=img.ctypes.data, atten_len=img.size), "attenuation image"),
+        (dict(rays_xy=2, cr_xy=float("inf")), "must be finite"),
     ]
     for kw, msg in checks:
         d, keep = _desc(**kw)
