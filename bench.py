@@ -503,7 +503,8 @@ def bench_orth(args, volume=False):
                    "sweeps": "transaxial+axial", "subsets": 8, "tube_visits_subset0": ntube},
         "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_orth_osem + update", "avg_launch_us": round(s_s * 1e6, 1),
+                     "kernel": ("k_tube_vol_planar (tube lists kept on the device) + update" if volume
+                                else "k_orth_osem + update"), "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt)},
         "cpu_baseline": cpu,
     })

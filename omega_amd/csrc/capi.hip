@@ -44,6 +44,7 @@ static thread_local const FixedRegion* t_fixed = nullptr;
 
 FixedRegionGuard::FixedRegionGuard(const FixedRegion* r) { t_fixed = r; }
 FixedRegionGuard::~FixedRegionGuard() { t_fixed = nullptr; }
+bool fixed_region_active() { return t_fixed != nullptr; }
 
 // Floats of the work part of a fixed region: [dU: N][rhs: N][slice map: Nz int32], rounded up to 64
 // floats (256 B) so the scratch part that follows keeps the region's alignment.

@@ -30,6 +30,11 @@ struct omg_geom {
     size_t frames_t_floats = 0;
     // Orthogonal projector: direct-plane sinogram whose tubes stay in their slice (orth.hip planar kernels).
     bool orth_planar = false;
+    // Volume projector: direct-plane PET sinogram (orth.hip k_orth_vol_planar for the fused step).
+    bool vol_planar = false;
+    // Volume projector: the tube lists of the measurement ranges used so far, kept on the device
+    // (orth.hip; omg_geom_desc.reserved[4] sets the budget).
+    struct TubeCache* tubes = nullptr;
     // Workgroups per plane for the planar kernels (omg_geom_desc.reserved[1]; 0 = automatic).
     int32_t planar_groups = 0;
     // Library-owned scratch (floats) for partial images; grown on demand, freed with the handle.
@@ -127,6 +132,8 @@ struct FixedRegionGuard {
     explicit FixedRegionGuard(const FixedRegion* r);
     ~FixedRegionGuard();
 };
+// True while the calling thread holds a fixed region (no device allocation may happen).
+bool fixed_region_active();
 
 // Ensures scratch_ptr(g) holds at least `floats` floats (hipMalloc on growth; synchronises the device then).
 int scratch_reserve(omg_geom* g, size_t floats);

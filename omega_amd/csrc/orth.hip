@@ -28,6 +28,9 @@
 
 #include "geom.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace omg {
 
 constexpr int kOrthBlock = 256;
@@ -646,6 +649,243 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
     }
 }
 
+// ---- projector 3 on direct-plane sinograms: fused FP -> ratio -> BP with LDS slice accumulators ----
+// The volume tube reaches the slices next to the LOR's own (a voxel counts out to bmax = 2.77 mm with the
+// defaults): 64% of a 2-D plane subset's visits land in slice tk, 18% in each neighbour. A workgroup
+// owns (plane, chunk of LORs), all in slice tk:
+//   * FP: slice tk staged in LDS, the others read from img, summed in the visit order (bit-identical to
+//     k_orth_osem's forward projection); r and the LOR weight are kept per LOR in LDS;
+//   * BP: slices tk and tk + 1 accumulate in LDS (float ds_add: ~0.33 lanes / clock / CU, still ~8x the
+//     scattered global float atomics), slice tk - 1 takes global atomics; one coalesced flush of the
+//     two slices (non-zero entries) into rhs at the end.
+// r * (w * temp) is the value k_orth_osem adds, so rhs differs from it only in the order of the float
+// additions. No sensitivity image (the first iteration keeps k_orth_osem).
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_orth_vol_planar(OrthArgs oa, uint32_t n_planes, uint32_t chunk,
+                                                         const float* __restrict__ img, const float* __restrict__ meas,
+                                                         float* __restrict__ rhs, float epps) {
+    extern __shared__ float lds[];
+    const SidArgs& a = oa.a;
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    const uint32_t per = a.size_x * a.size_y;
+    const uint32_t l0 = grp * chunk;
+    if (l0 >= per) return;
+    const uint32_t l1 = min(per, l0 + chunk);
+    const uint32_t Nyx = a.Nyx;
+    const int64_t base = (int64_t)plane * per;
+    const float zs = a.z[(a.plane_offset + plane) * 2];
+    const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+    if (tk < 0 || (uint32_t)tk >= a.Nz) return;   // every LOR of the plane misses the image
+    const int64_t sbase = (int64_t)tk * Nyx;
+    const uint32_t nacc = ((uint32_t)tk + 1u < a.Nz) ? 2u * Nyx : Nyx;   // slices tk (and tk + 1) in LDS
+    // LDS: [2 Nyx floats: slice tk for the FP, then the accumulators][R: chunk floats][T: chunk floats]
+    float* R = lds + 2 * Nyx;
+    float* T = R + chunk;
+    {
+        const float4* src = reinterpret_cast<const float4*>(img + sbase);
+        float4* dst = reinterpret_cast<float4*>(lds);
+        for (uint32_t i = threadIdx.x; i < Nyx / 4; i += TPB) dst[i] = src[i];
+    }
+    __syncthreads();
+    for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+        const int64_t lo = base + l;
+        float ax = 0.f;
+        const float temp = orth_lor<true>(oa, lo, [&](int64_t ind, float w) {
+            const uint64_t j = (uint64_t)(ind - sbase);
+            ax += w * (j < Nyx ? lds[j] : img[ind]);
+        });
+        float r = 0.f;   // 0: nothing to backproject (a LOR that misses the image, or y = 0)
+        if (temp != 0.f) {
+            r = meas[lo] / ratio_den(a, lo, ax * temp, epps);
+            if (isnan(r) || isinf(r)) r = epps;
+        }
+        R[l - l0] = r;
+        T[l - l0] = temp;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nacc; i += TPB) lds[i] = 0.f;
+    __syncthreads();
+    for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+        const float r = R[l - l0];
+        if (r == 0.f) continue;
+        const float temp = T[l - l0];
+        orth_lor<true>(oa, base + l, [&](int64_t ind, float w) {
+            const float v = r * (w * temp);
+            const uint64_t j = (uint64_t)(ind - sbase);
+            if (j < nacc) __hip_atomic_fetch_add(lds + j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else unsafeAtomicAdd(rhs + ind, v);
+        });
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nacc; i += TPB) {
+        const float v = lds[i];
+        if (v != 0.f) unsafeAtomicAdd(rhs + sbase + i, v);
+    }
+}
+
+// ---- projector 3: tube lists kept in device memory ------------------------------------------------
+// A LOR's tube (the voxels it visits and their table weights) depends on the geometry only, and tracing
+// it costs ~290 VALU instructions per visit. The lists of a measurement range are traced once and kept:
+// lane lo's k-th visit at base[lo / 64] + 64 k + lo % 64 (the 64 LORs of a block side by side, so each
+// load of a wave is one coalesced row), the block padded to its longest list. The projections then
+// stream them: the same (voxel, weight) pairs in the same order, so the forward projection is
+// bit-identical to the tracing kernels' and the backprojection adds the same values.
+struct TubeView {
+    const uint32_t* ind;
+    const float* w;
+    const int64_t* base;
+    const int32_t* count;
+    const float* temp;   // the LOR weight orth_lor returns (0: nothing visited)
+    int64_t n;
+};
+
+// The walk is the counting pass's (same code, no contraction), so it visits count[lo] voxels; the writes
+// are bounded by that count all the same.
+template <bool VOL>
+__global__ __launch_bounds__(kOrthBlock) void k_tube_fill(OrthArgs oa, const int64_t* __restrict__ base,
+                                                          const int32_t* __restrict__ count,
+                                                          uint32_t* __restrict__ ind, float* __restrict__ w,
+                                                          float* __restrict__ temp) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    const int64_t p = base[lo >> 6] + (lo & 63);
+    const int32_t c = count[lo];
+    int32_t k = 0;
+    temp[lo] = orth_lor<VOL>(oa, lo, [&](int64_t i, float wt) {
+        if (k < c) {
+            ind[p + 64 * (int64_t)k] = (uint32_t)i;
+            w[p + 64 * (int64_t)k] = wt;
+        }
+        k++;
+    });
+}
+
+// sum_k w_k img[ind_k] in visit order
+__device__ __forceinline__ float tube_gather(const TubeView& t, int64_t lo, const float* __restrict__ img) {
+    const int32_t c = t.count[lo];
+    const int64_t p = t.base[lo >> 6] + (lo & 63);
+    float ax = 0.f;
+    for (int32_t k = 0; k < c; k++) ax += t.w[p + 64 * k] * img[t.ind[p + 64 * k]];
+    return ax;
+}
+
+__global__ __launch_bounds__(kOrthBlock) void k_tube_fp(TubeView t, const float* __restrict__ img,
+                                                        float* __restrict__ y) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= t.n) return;
+    y[lo] = tube_gather(t, lo, img) * t.temp[lo];
+}
+
+// The backprojection of r over one LOR: rhs += r (w temp), sens += w temp (k_orth_bp's values).
+__device__ __forceinline__ void tube_scatter(const TubeView& t, int64_t lo, float r, float temp, float* __restrict__ rhs,
+                                             float* __restrict__ sens) {
+    const int32_t c = t.count[lo];
+    const int64_t p = t.base[lo >> 6] + (lo & 63);
+    for (int32_t k = 0; k < c; k++) {
+        const float e = t.w[p + 64 * k] * temp;
+        const uint32_t i = t.ind[p + 64 * k];
+        unsafeAtomicAdd(rhs + i, r * e);
+        if (sens) unsafeAtomicAdd(sens + i, e);
+    }
+}
+
+__global__ __launch_bounds__(kOrthBlock) void k_tube_bp(TubeView t, const float* __restrict__ yin,
+                                                        float* __restrict__ rhs, float* __restrict__ sens) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= t.n) return;
+    const float r = yin[lo];
+    if (r == 0.f && sens == nullptr) return;
+    tube_scatter(t, lo, r, t.temp[lo], rhs, sens);
+}
+
+// k_orth_osem from the lists.
+__global__ __launch_bounds__(kOrthBlock) void k_tube_osem(TubeView t, SidArgs a, const float* __restrict__ img,
+                                                          const float* __restrict__ meas, float* __restrict__ rhs,
+                                                          float* __restrict__ sens, float epps) {
+    const int64_t lo = (int64_t)blockIdx.x * kOrthBlock + threadIdx.x;
+    if (lo >= t.n) return;
+    const float temp = t.temp[lo];
+    if (temp == 0.f) return;
+    const float ax = tube_gather(t, lo, img);
+    float r = meas[lo] / ratio_den(a, lo, ax * temp, epps);
+    if (isnan(r) || isinf(r)) r = epps;
+    if (r == 0.f && sens == nullptr) return;
+    tube_scatter(t, lo, r, temp, rhs, sens);
+}
+
+// k_orth_vol_planar from the lists: slice tk staged in LDS for the gathers, slices tk and tk + 1
+// accumulated in LDS, slice tk - 1 through global atomics. SENS: the sensitivity image instead, the sum of
+// w temp over every LOR that visits the image (no forward projection; `rhs` receives it).
+template <int TPB, bool SENS>
+__global__ __launch_bounds__(TPB) void k_tube_vol_planar(TubeView t, SidArgs a, uint32_t n_planes, uint32_t chunk,
+                                                         const float* __restrict__ img, const float* __restrict__ meas,
+                                                         float* __restrict__ rhs, float epps) {
+    extern __shared__ float lds[];
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    const uint32_t per = a.size_x * a.size_y;
+    const uint32_t l0 = grp * chunk;
+    if (l0 >= per) return;
+    const uint32_t l1 = min(per, l0 + chunk);
+    const uint32_t Nyx = a.Nyx;
+    const int64_t base = (int64_t)plane * per;
+    const float zs = a.z[(a.plane_offset + plane) * 2];
+    const int32_t tk = (int32_t)(fabsf(zs - a.bz) / a.dz);
+    if (tk < 0 || (uint32_t)tk >= a.Nz) return;
+    const int64_t sbase = (int64_t)tk * Nyx;
+    const uint32_t nacc = ((uint32_t)tk + 1u < a.Nz) ? 2u * Nyx : Nyx;
+    float* R = lds + 2 * Nyx;
+    if (!SENS) {
+        const float4* src = reinterpret_cast<const float4*>(img + sbase);
+        float4* dst = reinterpret_cast<float4*>(lds);
+        for (uint32_t i = threadIdx.x; i < Nyx / 4; i += TPB) dst[i] = src[i];
+    }
+    __syncthreads();
+    for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+        const int64_t lo = base + l;
+        const float temp = t.temp[lo];
+        float r = 0.f;
+        if (SENS) {
+            r = temp != 0.f ? 1.f : 0.f;
+        } else if (temp != 0.f) {
+            const int32_t c = t.count[lo];
+            const int64_t p = t.base[lo >> 6] + (lo & 63);
+            float ax = 0.f;
+            for (int32_t k = 0; k < c; k++) {
+                const uint32_t i = t.ind[p + 64 * k];
+                const uint64_t j = (uint64_t)((int64_t)i - sbase);
+                ax += t.w[p + 64 * k] * (j < Nyx ? lds[j] : img[i]);
+            }
+            r = meas[lo] / ratio_den(a, lo, ax * temp, epps);
+            if (isnan(r) || isinf(r)) r = epps;
+        }
+        R[l - l0] = r;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nacc; i += TPB) lds[i] = 0.f;
+    __syncthreads();
+    for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
+        const float r = R[l - l0];
+        if (r == 0.f) continue;
+        const int64_t lo = base + l;
+        const float temp = t.temp[lo];
+        const int32_t c = t.count[lo];
+        const int64_t p = t.base[lo >> 6] + (lo & 63);
+        for (int32_t k = 0; k < c; k++) {
+            const uint32_t i = t.ind[p + 64 * k];
+            const float e = t.w[p + 64 * k] * temp;
+            const float v = SENS ? e : r * e;
+            const uint64_t j = (uint64_t)((int64_t)i - sbase);
+            if (j < nacc) __hip_atomic_fetch_add(lds + j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else unsafeAtomicAdd(rhs + i, v);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nacc; i += TPB) {
+        const float v = lds[i];
+        if (v != 0.f) unsafeAtomicAdd(rhs + sbase + i, v);
+    }
+}
+
 static OrthArgs make_orth_args(const omg_geom* g, int64_t first, int64_t n) {
     OrthArgs oa;
     oa.a = make_sid_args(g, first, n);
@@ -746,16 +986,190 @@ int orth_prepare(omg_geom* g) {
     OMG_REQUIRE(d.orth_width > 0.f && std::isfinite(d.orth_width),
                 "orth_width must be a positive tube width (projector 2) or radius (projector 3)");
     OMG_REQUIRE((d.orth_flags & ~3) == 0, "orth_flags holds bit 0 (transaxial sweep) and bit 1 (axial sweep) only");
-    // volume weights exceed 1, outside the planar kernels' fixed-point bound: one lane per LOR
+    // volume weights exceed 1, outside the planar kernels' fixed-point bound: one lane per LOR, and the
+    // float-accumulator slice kernel for the fused step on direct-plane sinograms
     g->orth_planar = d.projector_type == OMG_PROJ_ORTH && orth_planar_check(g);
+    g->vol_planar = d.projector_type == OMG_PROJ_VOLUME && g->planar && d.mode == OMG_MODE_PET_SINO && d.Nx % 4 == 0;
     return OMG_OK;
 }
 
-void orth_release(omg_geom*) {}
+// ---- the tube-list cache -----------------------------------------------------------------------------
+struct TubeEntry {
+    int64_t first = 0, n = 0;
+    bool cached = false;   // false: over the budget (kept so the range is not counted again)
+    uint32_t* ind = nullptr;
+    float* w = nullptr;
+    int64_t* base = nullptr;
+    int32_t* count = nullptr;
+    float* temp = nullptr;
+    size_t bytes = 0;
+};
+struct TubeCache {
+    std::vector<TubeEntry*> e;
+    size_t bytes = 0;
+};
+
+static void tube_free(TubeEntry* e) {
+    hipFree(e->ind);
+    hipFree(e->w);
+    hipFree(e->base);
+    hipFree(e->count);
+    hipFree(e->temp);
+    delete e;
+}
+
+static size_t tube_budget(const omg_geom* g) {
+    const int32_t r = g->d.reserved[4];
+    if (r < 0) return 0;
+    return (r == 0 ? (size_t)16384 : (size_t)r) << 20;
+}
+
+// *out: the lists of [first, first + n) of a projector-3 geometry, built on the first call for the range
+// (two tracing passes: counts, then the lists); nullptr when they do not apply (budget, a no-allocation
+// call, stream capture, a one-lane-per-LOR routing) or would not fit the budget.
+static int tube_lists(const omg_geom* gc, int64_t first, int64_t n, hipStream_t s, const TubeEntry** out) {
+    *out = nullptr;
+    omg_geom* g = const_cast<omg_geom*>(gc);
+    if (g->d.projector_type != OMG_PROJ_VOLUME || g->force_generic || fixed_region_active() || n <= 0) return OMG_OK;
+    const size_t budget = tube_budget(g);
+    if (budget == 0) return OMG_OK;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return OMG_OK;
+    if (!g->tubes) g->tubes = new TubeCache();
+    for (TubeEntry* e : g->tubes->e)
+        if (e->first == first && e->n == n) {
+            *out = e->cached ? e : nullptr;
+            return OMG_OK;
+        }
+    TubeEntry* e = new TubeEntry();
+    e->first = first;
+    e->n = n;
+    g->tubes->e.push_back(e);
+    const OrthArgs oa = make_orth_args(g, first, n);
+    const int64_t blocks = (n + 63) / 64;
+    if (hipMalloc(reinterpret_cast<void**>(&e->count), (size_t)n * sizeof(int32_t)) != hipSuccess) {
+        set_error("could not allocate %lld tube counts", (long long)n);
+        return OMG_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_orth_trace<true>, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, e->count,
+                       nullptr, nullptr, nullptr);
+    OMG_CHECK_LAUNCH();
+    std::vector<int32_t> cnt((size_t)n);
+    OMG_CHECK_HIP(hipMemcpyAsync(cnt.data(), e->count, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    OMG_CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> base((size_t)blocks);
+    int64_t total = 0;
+    for (int64_t b = 0; b < blocks; b++) {
+        int32_t m = 0;
+        for (int64_t l = b * 64; l < std::min(n, b * 64 + 64); l++) m = std::max(m, cnt[(size_t)l]);
+        base[(size_t)b] = total;
+        total += 64 * (int64_t)m;
+    }
+    e->bytes = (size_t)total * 8 + (size_t)blocks * 8 + (size_t)n * 8;
+    if (g->tubes->bytes + e->bytes > budget) {   // keep tracing this range: the lists would not fit
+        hipFree(e->count);
+        e->count = nullptr;
+        e->bytes = 0;
+        return OMG_OK;
+    }
+    bool ok = hipMalloc(reinterpret_cast<void**>(&e->ind), (size_t)std::max<int64_t>(total, 1) * 4) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&e->w), (size_t)std::max<int64_t>(total, 1) * 4) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&e->base), (size_t)blocks * 8) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&e->temp), (size_t)n * 4) == hipSuccess;
+    if (!ok) {   // device memory short of the budget: trace instead
+        (void)hipGetLastError();
+        hipFree(e->ind); hipFree(e->w); hipFree(e->base); hipFree(e->temp); hipFree(e->count);
+        e->ind = nullptr; e->w = nullptr; e->base = nullptr; e->temp = nullptr; e->count = nullptr;
+        e->bytes = 0;
+        return OMG_OK;
+    }
+    OMG_CHECK_HIP(hipMemcpyAsync(e->base, base.data(), (size_t)blocks * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_tube_fill<true>, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, e->base,
+                       e->count, e->ind, e->w, e->temp);
+    OMG_CHECK_LAUNCH();
+    OMG_CHECK_HIP(hipStreamSynchronize(s));   // base (a host vector) must outlive the copy
+    e->cached = true;
+    g->tubes->bytes += e->bytes;
+    *out = e;
+    return OMG_OK;
+}
+
+static TubeView tube_view(const TubeEntry* e) {
+    return TubeView{e->ind, e->w, e->base, e->count, e->temp, e->n};
+}
+
+// k_orth_vol_planar's launch, or OMG_ERR_UNSUPPORTED when its LDS would not fit.
+static int vol_launch_planar(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas,
+                             float* rhs, float epps, hipStream_t s, const float* add) {
+    const PlanarShape sh = planar_shape(g, n);
+    const size_t Nyx = (size_t)g->d.Nx * g->d.Ny;
+    const size_t lds = sizeof(float) * (2 * Nyx + 2 * (size_t)sh.chunk);
+    if (lds > 160 * 1024) return OMG_ERR_UNSUPPORTED;
+    if (sh.blocks == 0) return OMG_OK;
+    auto kern = k_orth_vol_planar<kPlanarTPB>;
+    static size_t attr = 0;
+    if (lds > attr) {
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = lds;
+    }
+    OrthArgs oa = make_orth_args(g, first, n);
+    oa.a.add = add;
+    hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, oa, sh.n_planes, sh.chunk, img, meas, rhs,
+                       epps);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
+}
+
+void orth_release(omg_geom* g) {
+    if (!g->tubes) return;
+    for (TubeEntry* e : g->tubes->e) tube_free(e);
+    delete g->tubes;
+    g->tubes = nullptr;
+}
+
+// The tube-list form of k_orth_vol_planar (its LDS: 2 Nyx + chunk floats); with sens, a second launch
+// accumulates the sensitivity image the same way.
+static int tube_launch_planar(const omg_geom* g, const TubeEntry* e, int64_t first, int64_t n, const float* img,
+                              const float* meas, float* rhs, float* sens, float epps, hipStream_t s, const float* add) {
+    const PlanarShape sh = planar_shape(g, n);
+    const size_t Nyx = (size_t)g->d.Nx * g->d.Ny;
+    const size_t lds = sizeof(float) * (2 * Nyx + (size_t)sh.chunk);
+    if (lds > 160 * 1024) return OMG_ERR_UNSUPPORTED;
+    if (sh.blocks == 0) return OMG_OK;
+    auto kern = k_tube_vol_planar<kPlanarTPB, false>;
+    auto kern_s = k_tube_vol_planar<kPlanarTPB, true>;
+    static size_t attr = 0;
+    if (lds > attr) {
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern_s),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = lds;
+    }
+    SidArgs a = make_sid_args(g, first, n);
+    a.add = add;
+    hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, tube_view(e), a, sh.n_planes, sh.chunk, img,
+                       meas, rhs, epps);
+    OMG_CHECK_LAUNCH();
+    if (sens) {
+        hipLaunchKernelGGL(kern_s, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, tube_view(e), a, sh.n_planes, sh.chunk,
+                           img, meas, sens, epps);
+        OMG_CHECK_LAUNCH();
+    }
+    return OMG_OK;
+}
 
 int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
     if (g->orth_planar && !g->force_generic)
         return orth_launch_planar<true, false, false>(g, first, n, img, nullptr, y, nullptr, nullptr, 0.f, s);
+    const TubeEntry* e = nullptr;
+    if (int rc = tube_lists(g, first, n, s, &e)) return rc;
+    if (e) {
+        hipLaunchKernelGGL(k_tube_fp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, tube_view(e), img, y);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
     const OrthArgs oa = make_orth_args(g, first, n);
     auto* kern = oa.vol ? k_orth_fp<true> : k_orth_fp<false>;
     hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, img, y);
@@ -769,6 +1183,14 @@ int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin,
         if (sens) return orth_launch_planar<false, true, true>(g, first, n, nullptr, yin, nullptr, rhs, sens, 0.f, s);
         return orth_launch_planar<false, true, false>(g, first, n, nullptr, yin, nullptr, rhs, nullptr, 0.f, s);
     }
+    const TubeEntry* e = nullptr;
+    if (int rc = tube_lists(g, first, n, s, &e)) return rc;
+    if (e) {
+        hipLaunchKernelGGL(k_tube_bp, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, tube_view(e), yin, rhs,
+                           sens);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
     const OrthArgs oa = make_orth_args(g, first, n);
     auto* kern = oa.vol ? k_orth_bp<true> : k_orth_bp<false>;
     hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, yin, rhs, sens);
@@ -781,6 +1203,24 @@ int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, con
     if (g->orth_planar && !g->force_generic) {
         if (sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, rhs, sens, epps, s, add);
         return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, rhs, nullptr, epps, s, add);
+    }
+    const TubeEntry* e = nullptr;
+    if (int rc = tube_lists(g, first, n, s, &e)) return rc;
+    if (e) {
+        if (g->vol_planar) {
+            const int rc = tube_launch_planar(g, e, first, n, img, meas, rhs, sens, epps, s, add);
+            if (rc != OMG_ERR_UNSUPPORTED) return rc;
+        }
+        SidArgs a = make_sid_args(g, first, n);
+        a.add = add;
+        hipLaunchKernelGGL(k_tube_osem, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, tube_view(e), a, img,
+                           meas, rhs, sens, epps);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
+    if (g->vol_planar && !sens && !g->force_generic) {
+        const int rc = vol_launch_planar(g, first, n, img, meas, rhs, epps, s, add);
+        if (rc != OMG_ERR_UNSUPPORTED) return rc;
     }
     OrthArgs oa = make_orth_args(g, first, n);
     oa.a.add = add;

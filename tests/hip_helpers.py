@@ -12,13 +12,14 @@ from omega_amd.projector import Geometry
 def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, projector_type=1, dPitchZ=0.0,
               ct_pitch=0, global_factor=1.0, norm=None, orth_width=0.0, orth_flags=0, dPitchX=0.0, dPitchY=0.0,
               generic=False, x_fastest_only=False, atten=None, atten_mode=0, scat=None, tof=None, rays=None,
-              volume=None, interp=None):
+              volume=None, interp=None, tube_budget_mib=0):
     """Geometry handle through the C ABI; ``generic`` routes every call through the one-lane-per-LOR
     kernels (omg_geom_desc.reserved[0]); ``x_fastest_only`` keeps those kernels in the x-fastest image
     (reserved[3]); ``atten`` is an attenuation image (``atten_mode`` 1) or per-measurement factors (2);
     ``scat`` a per-measurement multiplicative correction; ``tof`` = (bins, sigma_mm, centres_mm, epps);
     ``rays`` = (transaxial, axial, pitch_xy, pitch_z); ``volume`` = (V, bmin, bmax, Vmax) for projector 3;
-    ``interp`` = (dL mm, kerroin) for projector 4."""
+    ``interp`` = (dL mm, kerroin) for projector 4; ``tube_budget_mib``: projector 3's tube-list budget
+    (reserved[4]; 0 default, < 0 off: every call traces the tubes)."""
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     z = np.ascontiguousarray(z, dtype=np.float32).ravel() if z is not None else np.zeros(2, np.float32)
     d = _lib.GeomDesc()
@@ -73,6 +74,7 @@ def make_geom(grid, x, z, size_x, size_y, mode=_lib.MODE_PET_SINO, n_proj=None, 
         d.reserved[0] = 1
     if x_fastest_only:
         d.reserved[3] = 1
+    d.reserved[4] = int(tube_budget_mib)
     return Geometry(d, keep)
 
 
