@@ -73,7 +73,7 @@ typedef struct omg_geom_desc {
      * qualify keep the LDS fixed-point planar kernel; reserved[3] != 0 keeps the one-lane-per-LOR kernels
      * on PET sinograms in the x-fastest image only (by default LORs closer to the x axis accumulate
      * their backprojection into y-fastest buffers, where the lanes of a wave share cache lines);
-     * reserved[4] is the device-memory budget in MiB of the volume projector's tube lists (each
+     * reserved[4] is the device-memory budget in MiB of projectors 2 and 3's tube lists (each
      * measurement range's voxel indices and table weights, traced once and then streamed by the
      * projections; 0: 16384 MiB, < 0: off, every call traces the tubes).
      * Zero = automatic. */

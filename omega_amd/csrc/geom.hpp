@@ -32,7 +32,9 @@ struct omg_geom {
     bool orth_planar = false;
     // Volume projector: direct-plane PET sinogram (orth.hip k_orth_vol_planar for the fused step).
     bool vol_planar = false;
-    // Volume projector: the tube lists of the measurement ranges used so far, kept on the device
+    // Projectors 2 and 3 on a direct-plane PET sinogram: the tube lists' LDS-slice kernel applies.
+    bool tube_planar = false;
+    // Projectors 2 and 3: the tube lists of the measurement ranges used so far, kept on the device
     // (orth.hip; omg_geom_desc.reserved[4] sets the budget).
     struct TubeCache* tubes = nullptr;
     // Workgroups per plane for the planar kernels (omg_geom_desc.reserved[1]; 0 = automatic).

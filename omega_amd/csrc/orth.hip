@@ -540,8 +540,32 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_trace(OrthArgs oa, int32_t*
 // in LDS (same visit order: bit-identical to k_orth_fp); the BP accumulates t = r * (w * temp) in a
 // 64-bit fixed-point LDS slice (the scale bound uses w <= 1 and, as margin, up to four visits of a
 // voxel per LOR) and stores a partial slice, reduced by the shared planar reduce / update.
-template <int TPB, bool FP, bool BP, bool SENS>
-__global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_planes, uint32_t chunk,
+// Tube lists kept in device memory (see k_tube_fill): lane lo's k-th visit at base[lo / 64] + 64 k + lo % 64.
+struct TubeView {
+    const uint32_t* ind;
+    const float* w;
+    const int64_t* base;
+    const int32_t* count;
+    const float* temp;   // the LOR weight orth_lor returns (0: nothing visited)
+    int64_t n;
+};
+
+// LISTS: the tubes come from the kept lists (tv) instead of being traced: the same pairs in the same order,
+// so every sum is the traced kernel's, bit for bit.
+template <bool LISTS, class V>
+__device__ __forceinline__ float planar_tube(const OrthArgs& oa, const TubeView& tv, int64_t lo, V&& visit) {
+    if constexpr (LISTS) {
+        const int32_t c = tv.count[lo];
+        const int64_t p = tv.base[lo >> 6] + (lo & 63);
+        for (int32_t k = 0; k < c; k++) visit((int64_t)tv.ind[p + 64 * k], tv.w[p + 64 * k]);
+        return tv.temp[lo];
+    } else {
+        return orth_lor<false>(oa, lo, visit);
+    }
+}
+
+template <int TPB, bool FP, bool BP, bool SENS, bool LISTS>
+__global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, TubeView tv, uint32_t n_planes, uint32_t chunk,
                                                      const float* __restrict__ img, const float* __restrict__ yin,
                                                      float* __restrict__ yout, float* __restrict__ rhs,
                                                      float* __restrict__ sens, float epps) {
@@ -575,7 +599,7 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
         for (uint32_t l = l0 + threadIdx.x; l < l1; l += TPB) {
             const int64_t lo = base + l;
             float ax = 0.f;
-            const float temp = orth_lor<false>(oa, lo, [&](int64_t ind, float w) {
+            const float temp = planar_tube<LISTS>(oa, tv, lo, [&](int64_t ind, float w) {
                 const uint64_t j = (uint64_t)(ind - sbase);
                 if (j < Nyx) ax += w * lds[j];
             });
@@ -629,12 +653,12 @@ __global__ __launch_bounds__(TPB) void k_orth_planar(OrthArgs oa, uint32_t n_pla
                 if (a.scat) temp *= a.scat[lo];
                 temp *= a.global_factor;
                 if (pass == 0) {
-                    orth_lor<false>(oa, lo, [&](int64_t ind, float w) {
+                    planar_tube<LISTS>(oa, tv, lo, [&](int64_t ind, float w) {
                         const uint64_t j = (uint64_t)(ind - sbase);
                         if (j < Nyx) lds_add_fixed(acc + j, r * (w * temp), up_r);
                     });
                 } else {
-                    orth_lor<false>(oa, lo, [&](int64_t ind, float w) {
+                    planar_tube<LISTS>(oa, tv, lo, [&](int64_t ind, float w) {
                         const uint64_t j = (uint64_t)(ind - sbase);
                         if (j < Nyx) lds_add_fixed(acc + j, w * temp, up_s);
                     });
@@ -723,21 +747,13 @@ __global__ __launch_bounds__(TPB) void k_orth_vol_planar(OrthArgs oa, uint32_t n
     }
 }
 
-// ---- projector 3: tube lists kept in device memory ------------------------------------------------
-// A LOR's tube (the voxels it visits and their table weights) depends on the geometry only, and tracing
-// it costs ~290 VALU instructions per visit. The lists of a measurement range are traced once and kept:
+// ---- projectors 2 and 3: tube lists kept in device memory ------------------------------------------
+// A LOR's tube (the voxels it visits and their distance or table weights) depends on the geometry only,
+// and tracing it costs ~290 VALU instructions per visit (projector 3). The lists of a measurement range are traced once and kept:
 // lane lo's k-th visit at base[lo / 64] + 64 k + lo % 64 (the 64 LORs of a block side by side, so each
 // load of a wave is one coalesced row), the block padded to its longest list. The projections then
 // stream them: the same (voxel, weight) pairs in the same order, so the forward projection is
 // bit-identical to the tracing kernels' and the backprojection adds the same values.
-struct TubeView {
-    const uint32_t* ind;
-    const float* w;
-    const int64_t* base;
-    const int32_t* count;
-    const float* temp;   // the LOR weight orth_lor returns (0: nothing visited)
-    int64_t n;
-};
 
 // The walk is the counting pass's (same code, no contraction), so it visits count[lo] voxels; the writes
 // are bounded by that count all the same.
@@ -932,6 +948,10 @@ static bool orth_planar_check(const omg_geom* g) {
     return true;
 }
 
+struct TubeEntry;
+static int tube_lists(const omg_geom* gc, int64_t first, int64_t n, hipStream_t s, const TubeEntry** out);
+static TubeView tube_view(const TubeEntry* e);
+
 template <bool FP, bool BP, bool SENS>
 static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* yin,
                               float* yout, float* rhs, float* sens, float epps, hipStream_t s,
@@ -940,12 +960,15 @@ static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const
     const uint32_t Nyx = g->d.Nx * g->d.Ny;
     const size_t lds = BP ? (size_t)Nyx * 8 + sizeof(float) * ((size_t)sh.chunk + kPlanarTPB / 64)
                           : sizeof(float) * (size_t)Nyx;
-    auto kern = k_orth_planar<kPlanarTPB, FP, BP, SENS>;
-    static size_t attr_bytes = 0;
-    if (lds > attr_bytes) {
+    const TubeEntry* e = nullptr;   // the kept tube lists of the range, when there are
+    if (int rc = tube_lists(g, first, n, s, &e)) return rc;
+    const TubeView tv = e ? tube_view(e) : TubeView{};
+    auto kern = e ? k_orth_planar<kPlanarTPB, FP, BP, SENS, true> : k_orth_planar<kPlanarTPB, FP, BP, SENS, false>;
+    static size_t attr_bytes[2] = {0, 0};
+    if (lds > attr_bytes[e ? 1 : 0]) {
         OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_bytes = lds;
+        attr_bytes[e ? 1 : 0] = lds;
     }
     OrthArgs oa = make_orth_args(g, first, n);
     oa.a.add = add;
@@ -959,7 +982,7 @@ static int orth_launch_planar(const omg_geom* g, int64_t first, int64_t n, const
     }
     hipEvent_t stop;
     timing_begin(s, &stop);
-    hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, oa, sh.n_planes, sh.chunk, img, yin, yout,
+    hipLaunchKernelGGL(kern, dim3(sh.blocks), dim3(kPlanarTPB), lds, s, oa, tv, sh.n_planes, sh.chunk, img, yin, yout,
                        part, part2, epps);
     timing_end(s, stop);
     OMG_CHECK_LAUNCH();
@@ -990,6 +1013,7 @@ int orth_prepare(omg_geom* g) {
     // float-accumulator slice kernel for the fused step on direct-plane sinograms
     g->orth_planar = d.projector_type == OMG_PROJ_ORTH && orth_planar_check(g);
     g->vol_planar = d.projector_type == OMG_PROJ_VOLUME && g->planar && d.mode == OMG_MODE_PET_SINO && d.Nx % 4 == 0;
+    g->tube_planar = g->planar && d.mode == OMG_MODE_PET_SINO && d.Nx % 4 == 0;
     return OMG_OK;
 }
 
@@ -1024,13 +1048,13 @@ static size_t tube_budget(const omg_geom* g) {
     return (r == 0 ? (size_t)16384 : (size_t)r) << 20;
 }
 
-// *out: the lists of [first, first + n) of a projector-3 geometry, built on the first call for the range
+// *out: the lists of [first, first + n) of a projector-2 or -3 geometry, built on the first call for the range
 // (two tracing passes: counts, then the lists); nullptr when they do not apply (budget, a no-allocation
 // call, stream capture, a one-lane-per-LOR routing) or would not fit the budget.
 static int tube_lists(const omg_geom* gc, int64_t first, int64_t n, hipStream_t s, const TubeEntry** out) {
     *out = nullptr;
     omg_geom* g = const_cast<omg_geom*>(gc);
-    if (g->d.projector_type != OMG_PROJ_VOLUME || g->force_generic || fixed_region_active() || n <= 0) return OMG_OK;
+    if (!is_tube(g) || g->force_generic || fixed_region_active() || n <= 0) return OMG_OK;
     const size_t budget = tube_budget(g);
     if (budget == 0) return OMG_OK;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -1051,8 +1075,9 @@ static int tube_lists(const omg_geom* gc, int64_t first, int64_t n, hipStream_t 
         set_error("could not allocate %lld tube counts", (long long)n);
         return OMG_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_orth_trace<true>, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, e->count,
-                       nullptr, nullptr, nullptr);
+    const bool vol = oa.vol != nullptr;
+    hipLaunchKernelGGL(vol ? k_orth_trace<true> : k_orth_trace<false>, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock),
+                       0, s, oa, e->count, nullptr, nullptr, nullptr);
     OMG_CHECK_LAUNCH();
     std::vector<int32_t> cnt((size_t)n);
     OMG_CHECK_HIP(hipMemcpyAsync(cnt.data(), e->count, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1084,8 +1109,8 @@ static int tube_lists(const omg_geom* gc, int64_t first, int64_t n, hipStream_t 
         return OMG_OK;
     }
     OMG_CHECK_HIP(hipMemcpyAsync(e->base, base.data(), (size_t)blocks * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_tube_fill<true>, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0, s, oa, e->base,
-                       e->count, e->ind, e->w, e->temp);
+    hipLaunchKernelGGL(vol ? k_tube_fill<true> : k_tube_fill<false>, dim3(grid_for(n, kOrthBlock)), dim3(kOrthBlock), 0,
+                       s, oa, e->base, e->count, e->ind, e->w, e->temp);
     OMG_CHECK_LAUNCH();
     OMG_CHECK_HIP(hipStreamSynchronize(s));   // base (a host vector) must outlive the copy
     e->cached = true;
@@ -1207,7 +1232,7 @@ int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, con
     const TubeEntry* e = nullptr;
     if (int rc = tube_lists(g, first, n, s, &e)) return rc;
     if (e) {
-        if (g->vol_planar) {
+        if (g->tube_planar) {
             const int rc = tube_launch_planar(g, e, first, n, img, meas, rhs, sens, epps, s, add);
             if (rc != OMG_ERR_UNSUPPORTED) return rc;
         }

@@ -145,21 +145,23 @@ def test_projector_class_orth(cuda):
 
 
 @pytest.mark.parametrize("flags,width", [(1, 3.0), (3, 2.2)])
-@pytest.mark.parametrize("generic", [False, True])
-def test_planar_tube_in_slice(cuda, flags, width, generic):
+@pytest.mark.parametrize("path", ["lists", "traced", "generic"])
+def test_planar_tube_in_slice(cuda, flags, width, path):
     """Direct planes whose tube stays in its slice (transaxial sweep; or axial sweep with 0.99 * width
-    below the 2.5 mm slice pitch): the direct-plane LDS kernels (default) and the one-lane-per-LOR
-    kernels give FP bit-identical to the oracle, BP / sensitivity / the fused data step within 1e-5,
-    OSEM and OSL-OSEM subset iterations within 1e-5; the LDS path is bit-identical run to run."""
+    below the 2.5 mm slice pitch): the direct-plane LDS kernels fed by the kept tube lists (default),
+    the same kernels tracing the tubes (lists off) and the one-lane-per-LOR kernels give FP bit-identical
+    to the oracle, BP / sensitivity / the fused data step within 1e-5, OSEM and OSL-OSEM subset
+    iterations within 1e-5; both LDS paths are bit-identical run to run (64-bit fixed-point sums)."""
+    generic = path == "generic"
     import torch
     grid, sc = cases.orth_pet2d()
     per = sc.n_radial * sc.n_angles
     nplanes = sc.z.size // 2
     n = per * nplanes
-    print(f"[orth planar] flags={flags} width={width} generic={generic} n={n}", flush=True)
+    print(f"[orth planar] flags={flags} width={width} path={path} n={n}", flush=True)
     case = O.OrthCase(grid, sc.n_radial, sc.n_angles, sc.xy_table(), sc.z, width, flags=flags)
     geom = make_geom(grid, sc.xy_table(), sc.z, sc.n_radial, sc.n_angles, projector_type=2, orth_width=width,
-                     orth_flags=flags, generic=generic)
+                     orth_flags=flags, generic=generic, tube_budget_mib=-1 if path == "traced" else 0)
     rng = np.random.default_rng(20 + flags)
     img = rng.random(grid.N).astype(np.float32) + 0.05
     img_t = torch.from_numpy(img).to(cuda)
