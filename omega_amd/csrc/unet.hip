@@ -15,6 +15,8 @@
 // the loss and Adam are elementwise kernels.
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace omg {
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
@@ -329,6 +331,149 @@ __global__ __launch_bounds__(256) void k_conv3d_wgrad(const uint16_t* __restrict
             }
 }
 
+// ---- weight gradient on the K = 32 BF16 matrix cores ---------------------------------------------------
+// dW[tap][co][ci] = sum_v dY[v][co] X[v + off(tap)][ci] is a product whose K dimension is the voxels. The
+// MFMA's lanes hold 8 consecutive K values of one row (A) or column (B), so both operands are staged
+// transposed in LDS: dYt[co][voxel] for the block's 32 x 4 x 2 voxels and, for one 16-channel slice of X
+// (blockIdx.y), Xt[dx + 1][ci][halo row][x] -- three copies of the (2 + 2) x (4 + 2) halo rows shifted by
+// dx = -1, 0, 1, so that every operand is one aligned 16-byte LDS read. The 8 waves split the 27 taps
+// (wave w: w, w + 8, w + 16, w + 24) and reuse each row's dY operand across theirs. Blocks persist over
+// the tiles (blockIdx.x, step gridDim.x) and store their sums as one partial per block, added into dW by
+// k_wgrad_reduce (no atomics).
+constexpr int kWgTX = 32, kWgTY = 4, kWgTZ = 2, kWgRows = kWgTY * kWgTZ, kWgVox = kWgTX * kWgRows;
+constexpr int kWgHY = kWgTY + 2, kWgHZ = kWgTZ + 2, kWgHRows = kWgHY * kWgHZ, kWgThreads = 512;
+// channel-row strides of the LDS operands, padded by 16 B so that the 16 lanes of a read phase (one
+// channel each, 16 B apart otherwise a multiple of the 256-byte bank cycle) fall on distinct banks
+constexpr int kWgDS = kWgVox + 8, kWgXS = kWgHRows * kWgTX + 8;
+
+// u[e] holds 8 channels of voxel e; channel j's 4 voxels go to dst + j * stride as one 8-byte write.
+__device__ __forceinline__ void wg_store4(uint16_t* dst, size_t stride, const uint4 (&u)[4]) {
+    const uint32_t w[4][4] = {{u[0].x, u[0].y, u[0].z, u[0].w}, {u[1].x, u[1].y, u[1].z, u[1].w},
+                              {u[2].x, u[2].y, u[2].z, u[2].w}, {u[3].x, u[3].y, u[3].z, u[3].w}};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int sh = 16 * (j & 1), k = j >> 1;
+        const uint32_t lo = ((w[0][k] >> sh) & 0xffffu) | (((w[1][k] >> sh) & 0xffffu) << 16);
+        const uint32_t hi = ((w[2][k] >> sh) & 0xffffu) | (((w[3][k] >> sh) & 0xffffu) << 16);
+        *reinterpret_cast<uint2*>(dst + (size_t)j * stride) = uint2{lo, hi};
+    }
+}
+
+template <int MT>
+constexpr size_t wgrad_tile_lds() { return sizeof(uint16_t) * ((size_t)MT * 16 * kWgDS + 3 * 16 * (size_t)kWgXS); }
+
+template <int MT>
+__global__ __launch_bounds__(kWgThreads) void k_conv3d_wgrad_tile(const uint16_t* __restrict__ dY,
+                                                                  const uint16_t* __restrict__ X1, int C1,
+                                                                  const uint16_t* __restrict__ X2, int C2,
+                                                                  float* __restrict__ part, Vol g, int tiles_x,
+                                                                  int tiles_y, int n_tiles) {
+    constexpr int Cout = MT * 16;
+    extern __shared__ uint16_t wl[];
+    uint16_t* dYt = wl;                              // [Cout][kWgVox], row stride kWgDS
+    uint16_t* Xt = wl + (size_t)Cout * kWgDS;        // [3][16][kWgHRows][kWgTX], channel stride kWgXS
+    const int Cin = C1 + C2;
+    const int n = blockIdx.y;                        // input-channel slice [16 n, 16 n + 16)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, q = lane >> 4;
+    const int ntap = wave < 3 ? 4 : 3;
+    f4v acc[4][MT];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int m = 0; m < MT; m++) acc[t][m] = f4v{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* Xs = (n * 16 < C1) ? X1 : X2;    // the slice lies in one input (C1 % 16 == 0)
+    const int Cs = (n * 16 < C1) ? C1 : C2;
+    const int cs0 = (n * 16 < C1) ? n * 16 : n * 16 - C1;
+    for (int ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+        const int bx = ti % tiles_x, rest = ti / tiles_x;
+        const int x0 = bx * kWgTX, y0 = (rest % tiles_y) * kWgTY, z0 = (rest / tiles_y) * kWgTZ;
+        // Transposing fills: an item is 4 consecutive voxels of 8 channels (four 16-byte loads), stored as one
+        // 8-byte LDS write per channel. dY tile:
+        for (int i = threadIdx.x; i < (kWgVox / 4) * (Cout / 8); i += kWgThreads) {
+            const int quad = i / (Cout / 8), c8 = (i - quad * (Cout / 8)) * 8;
+            const int vox = quad * 4;
+            const int vx = vox % kWgTX, r = vox / kWgTX, vy = r % kWgTY, vz = r / kWgTY;
+            const int y = y0 + vy, z = z0 + vz;
+            uint4 u[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int x = x0 + vx + e;
+                u[e] = uint4{0u, 0u, 0u, 0u};
+                if (x < g.W && y < g.H && z < g.D)
+                    u[e] = *reinterpret_cast<const uint4*>(dY + (((size_t)z * g.H + y) * g.W + x) * Cout + c8);
+            }
+            wg_store4(dYt + (size_t)c8 * kWgDS + vox, kWgDS, u);
+        }
+        // X slice: halo rows, and in copy d the voxels x0 + p + d - 1 at p = 0..31 (x from x0 - 1 to x0 + 32)
+        for (int i = threadIdx.x; i < kWgHRows * (kWgTX / 4) * 2 * 3; i += kWgThreads) {
+            const int d = i % 3, r2 = i / 3;
+            const int c8 = (r2 & 1) * 8, r3 = r2 >> 1;
+            const int xq = r3 % (kWgTX / 4), hrow = r3 / (kWgTX / 4);
+            const int hy = hrow % kWgHY, hz = hrow / kWgHY;
+            const int y = y0 + hy - 1, z = z0 + hz - 1;
+            const bool row_in = y >= 0 && y < g.H && z >= 0 && z < g.D;
+            uint4 u[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int x = x0 + xq * 4 + e + d - 1;
+                u[e] = uint4{0u, 0u, 0u, 0u};
+                if (row_in && x >= 0 && x < g.W)
+                    u[e] = *reinterpret_cast<const uint4*>(Xs + (((size_t)z * g.H + y) * g.W + x) * Cs + cs0 + c8);
+            }
+            wg_store4(Xt + (size_t)(d * 16 + c8) * kWgXS + hrow * kWgTX + xq * 4, kWgXS, u);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kWgRows; r++) {
+            const int ry = r % kWgTY, rz = r / kWgTY;
+            bf8 a[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+                a[m] = *reinterpret_cast<const bf8*>(dYt + (size_t)(m * 16 + col) * kWgDS + r * kWgTX + 8 * q);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (t >= ntap) break;
+                const int tap = wave + 8 * t;
+                const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;   // 0..2: offset + 1
+                const int hrow = (rz + dz) * kWgHY + (ry + dy);
+                const bf8 b = *reinterpret_cast<const bf8*>(Xt + (size_t)(dx * 16 + col) * kWgXS + hrow * kWgTX + 8 * q);
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[t][m], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // this block's sums: part[block][tap][co][ci]
+    const size_t per = (size_t)27 * Cout * Cin;
+    float* pb = part + (size_t)blockIdx.x * per;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (t >= ntap) break;
+        const int tap = wave + 8 * t;
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                pb[((size_t)tap * Cout + m * 16 + 4 * q + j) * Cin + n * 16 + col] = acc[t][m][j];
+    }
+}
+
+// dW[i] += sum over the P partials of part[p][i]: a block takes 64 consecutive outputs (lane l: i0 + l,
+// coalesced), its 4 waves the partials p = wave, wave + 4, ..., combined through LDS.
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, size_t per,
+                                                      int P) {
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t i = (size_t)blockIdx.x * 64 + lane;
+    float s = 0.f;
+    if (i < per)
+        for (int p = wave; p < P; p += 4) s += part[(size_t)p * per + i];
+    red[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && i < per) dW[i] += (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
 // Wf[tap][ci][co] = W[26 - tap][co][ci]: dX = conv(dY) with these weights.
 __global__ void k_conv_flipT(const uint16_t* __restrict__ W, uint16_t* __restrict__ Wf, int Cout, int Cin) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -548,7 +693,9 @@ static int launch_fwd_tile(const uint16_t* X1, int C1, const uint16_t* X2, const
     return OMG_OK;
 }
 
-// Tile shape by the number of blocks the volume gives (see k_conv3d_fwd_tile).
+// Tile shape by the number of blocks the volume gives (see k_conv3d_fwd_tile). (Splitting the output
+// channels over blocks as well, one 16-channel slice per block, was measured slower on 5 of the 7 layers
+// with more than 16 of them: each slice block refills the halo.)
 template <int CIN, int MT>
 static int launch_fwd_shape(const uint16_t* X1, int C1, const uint16_t* X2, const uint16_t* W, const float* b,
                             uint16_t* Y, Vol g, int act, hipStream_t s) {
@@ -574,6 +721,41 @@ static int dispatch_fwd_tile(int MT, const uint16_t* X1, int C1, const uint16_t*
             break;
     }
     return OMG_ERR_UNSUPPORTED;
+}
+
+// Partial sums of the tiled weight gradient: a library buffer grown on demand (kept for the process).
+static float* g_wg_part = nullptr;
+static size_t g_wg_floats = 0;
+
+template <int MT>
+static int launch_wgrad_tile(const uint16_t* dY, const uint16_t* X1, int C1, const uint16_t* X2, int C2, float* dW,
+                             Vol g, hipStream_t s) {
+    const int Cin = C1 + C2, NT = Cin / 16;
+    const int tiles_x = (g.W + kWgTX - 1) / kWgTX, tiles_y = (g.H + kWgTY - 1) / kWgTY;
+    const int n_tiles = tiles_x * tiles_y * ((g.D + kWgTZ - 1) / kWgTZ);
+    const int P = std::max(1, std::min(n_tiles, 256));   // persistent blocks per channel slice
+    const size_t per = (size_t)27 * MT * 16 * Cin;
+    if ((size_t)P * per > g_wg_floats) {
+        OMG_CHECK_HIP(hipStreamSynchronize(s));
+        OMG_CHECK_HIP(hipFree(g_wg_part));
+        g_wg_part = nullptr;
+        g_wg_floats = 0;
+        OMG_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_wg_part), (size_t)P * per * sizeof(float)));
+        g_wg_floats = (size_t)P * per;
+    }
+    const size_t lds = wgrad_tile_lds<MT>();
+    static bool attr = false;
+    if (!attr) {
+        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_conv3d_wgrad_tile<MT>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_conv3d_wgrad_tile<MT>, dim3((unsigned)P, (unsigned)NT), dim3(kWgThreads), lds, s, dY, X1, C1,
+                       X2, C2, g_wg_part, g, tiles_x, tiles_y, n_tiles);
+    OMG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((per + 63) / 64)), dim3(256), 0, s, g_wg_part, dW, per, P);
+    OMG_CHECK_LAUNCH();
+    return OMG_OK;
 }
 
 template <int MT, int NT>
@@ -637,6 +819,13 @@ extern "C" int omg_conv3d_wgrad(const uint16_t* dY, int32_t Cout, const uint16_t
     const Vol g{D, H, Wd};
     hipStream_t s = as_stream(stream);
     const int MT = Cout / 16, NT = (C1 + C2) / 16;
+    // the tiled matrix-core kernel for 16-byte aligned operands (channel-last rows of 16 B multiples)
+    const bool al16 = ((uintptr_t)dY % 16 == 0) && ((uintptr_t)X1 % 16 == 0) && (C2 == 0 || (uintptr_t)X2 % 16 == 0);
+    if (al16 && (MT == 1 || MT == 2 || MT == 4) && NT >= 1) {
+        if (MT == 1) return launch_wgrad_tile<1>(dY, X1, C1, X2, C2, dW, g, s);
+        if (MT == 2) return launch_wgrad_tile<2>(dY, X1, C1, X2, C2, dW, g, s);
+        return launch_wgrad_tile<4>(dY, X1, C1, X2, C2, dW, g, s);
+    }
     if (MT == 1 && NT == 1) launch_wgrad<1, 1>(dY, X1, C1, X2, C2, dW, g, s);
     else if (MT == 2 && NT == 1) launch_wgrad<2, 1>(dY, X1, C1, X2, C2, dW, g, s);
     else if (MT == 2 && NT == 2) launch_wgrad<2, 2>(dY, X1, C1, X2, C2, dW, g, s);
