@@ -499,10 +499,91 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ 
     }
 }
 
+// k_bias_grad reading 8 channels (16 bytes) per thread: the block's 256 / (C / 8) rows keep 8 partial sums
+// each, folded in two LDS passes (8 rows per thread, then 256 / C per channel), then one atomic per channel.
+__global__ __launch_bounds__(256) void k_bias_grad8(const uint16_t* __restrict__ dY, float* __restrict__ db, int V,
+                                                    int C) {
+    __shared__ float part[2048];
+    const int g8 = C / 8;                // 16-byte groups per voxel row
+    const int rows = 256 / g8;
+    const int r = threadIdx.x / g8, c8 = (threadIdx.x - r * g8) * 8;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int v = blockIdx.x * rows + r; v < V; v += gridDim.x * rows) {
+        const uint4 u = *reinterpret_cast<const uint4*>(dY + (size_t)v * C + c8);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += bf2f((uint16_t)(w[j >> 1] >> (16 * (j & 1))));
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) part[r * C + c8 + j] = s[j];   // rows * C == 2048
+    __syncthreads();
+    const int c = threadIdx.x % C, q = threadIdx.x / C, step = 256 / C;
+    float t = 0.f;
+    for (int k = q; k < rows; k += step) t += part[k * C + c];
+    __syncthreads();
+    part[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float u = 0.f;
+        for (int k = 0; k < step; k++) u += part[k * C + threadIdx.x];
+        unsafeAtomicAdd(db + threadIdx.x, u);
+    }
+}
+
 // dZ = (Y > 0) ? dY : 0 over n elements (in place allowed)
 __global__ void k_relu_bwd(const uint16_t* __restrict__ Y, uint16_t* dY, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n && !(bf2f(Y[i]) > 0.f)) dY[i] = 0;
+}
+
+// The same, 8 elements (16 bytes) per thread; n8 = n / 8.
+__global__ void k_relu_bwd8(const uint4* __restrict__ Y, uint4* dY, size_t n8) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n8) return;
+    const uint4 y = Y[i];
+    uint4 d = dY[i];
+    const uint32_t yw[4] = {y.x, y.y, y.z, y.w};
+    uint32_t dw[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!(bf2f((uint16_t)(yw[k] & 0xffffu)) > 0.f)) dw[k] &= 0xffff0000u;
+        if (!(bf2f((uint16_t)(yw[k] >> 16)) > 0.f)) dw[k] &= 0x0000ffffu;
+    }
+    dY[i] = uint4{dw[0], dw[1], dw[2], dw[3]};
+}
+
+// Nearest 2x upsampling, 8 channels (16 bytes) per thread: grid = (ceil(W * C / 8 / 256), H, D) of the fine grid.
+__global__ void k_up_fwd8(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, Vol g, int C) {
+    const int c8n = C / 8;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.W * c8n) return;
+    const int x = i / c8n, c8 = (i - x * c8n) * 8, y = blockIdx.y, z = blockIdx.z;
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    const size_t vo = ((size_t)(z / 2) * Ho + y / 2) * Wo + x / 2;
+    const size_t v = ((size_t)z * g.H + y) * g.W + x;
+    *reinterpret_cast<uint4*>(Y + v * C + c8) = *reinterpret_cast<const uint4*>(X + vo * C + c8);
+}
+
+// k_up_bwd, 8 channels per thread: grid = (ceil(W * C / 8 / 256), H, D) of the coarse grid g.
+__global__ void k_up_bwd8(const uint16_t* __restrict__ dY, int Cs, int off, uint16_t* __restrict__ dX, Vol g, int C) {
+    const int c8n = C / 8;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.W * c8n) return;
+    const int xo = i / c8n, c8 = (i - xo * c8n) * 8, yo = blockIdx.y, zo = blockIdx.z;
+    const int H2 = g.H * 2, W2 = g.W * 2;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int d = 0; d < 8; d++) {   // children in the order of k_up_bwd
+        const int z = 2 * zo + (d >> 2), y = 2 * yo + ((d >> 1) & 1), x = 2 * xo + (d & 1);
+        const uint4 u = *reinterpret_cast<const uint4*>(dY + ((size_t)(z * H2 + y) * W2 + x) * Cs + off + c8);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += bf2f((uint16_t)(w[j >> 1] >> (16 * (j & 1))));
+    }
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (uint32_t)f2bf(s[2 * k]) | ((uint32_t)f2bf(s[2 * k + 1]) << 16);
+    const size_t vo = ((size_t)zo * g.H + yo) * g.W + xo;
+    *reinterpret_cast<uint4*>(dX + vo * C + c8) = uint4{o[0], o[1], o[2], o[3]};
 }
 
 // 2x2x2 average pooling of X[D][H][W][C] (even D, H, W)
@@ -848,14 +929,21 @@ extern "C" int omg_conv3d_flipT(const uint16_t* W, uint16_t* Wf, int32_t Cout, i
 
 extern "C" int omg_bias_grad(const uint16_t* dY, float* db, int32_t V, int32_t C, void* stream) {
     OMG_REQUIRE(dY && db && V > 0 && (C == 16 || C == 32 || C == 64), "bias gradient needs V > 0, C in {16, 32, 64}");
-    hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), 0, as_stream(stream), dY, db, V, C);
+    if ((uintptr_t)dY % 16 == 0)
+        hipLaunchKernelGGL(k_bias_grad8, dim3(512), dim3(256), 0, as_stream(stream), dY, db, V, C);
+    else
+        hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), 0, as_stream(stream), dY, db, V, C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
 
 extern "C" int omg_relu_backward(const uint16_t* Y, uint16_t* dY, int64_t n, void* stream) {
     OMG_REQUIRE(Y && dY && n >= 0, "bad arguments");
-    if (n) hipLaunchKernelGGL(k_relu_bwd, dim3(blocks((size_t)n)), dim3(256), 0, as_stream(stream), Y, dY, (size_t)n);
+    if (n && n % 8 == 0 && (uintptr_t)Y % 16 == 0 && (uintptr_t)dY % 16 == 0)
+        hipLaunchKernelGGL(k_relu_bwd8, dim3(blocks((size_t)n / 8)), dim3(256), 0, as_stream(stream),
+                           reinterpret_cast<const uint4*>(Y), reinterpret_cast<uint4*>(dY), (size_t)n / 8);
+    else if (n)
+        hipLaunchKernelGGL(k_relu_bwd, dim3(blocks((size_t)n)), dim3(256), 0, as_stream(stream), Y, dY, (size_t)n);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -884,8 +972,12 @@ extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, i
                                     void* stream) {
     OMG_REQUIRE(X && Y && C > 0 && vol_ok(D, H, W) && D % 2 == 0 && H % 2 == 0 && W % 2 == 0,
                 "upsampling needs an even positive fine grid");
-    hipLaunchKernelGGL(k_up_fwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), X, Y,
-                       Vol{D, H, W}, C);
+    if (C % 8 == 0 && (uintptr_t)X % 16 == 0 && (uintptr_t)Y % 16 == 0)
+        hipLaunchKernelGGL(k_up_fwd8, dim3(blocks((size_t)W * (C / 8)), (unsigned)H, (unsigned)D), dim3(256), 0,
+                           as_stream(stream), X, Y, Vol{D, H, W}, C);
+    else
+        hipLaunchKernelGGL(k_up_fwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), X, Y,
+                           Vol{D, H, W}, C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -893,8 +985,12 @@ extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, i
 extern "C" int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off, uint16_t* dX, int32_t D, int32_t H,
                                      int32_t W, int32_t C, void* stream) {
     OMG_REQUIRE(dY && dX && vol_ok(D, H, W) && C > 0 && off >= 0 && off + C <= Cs, "bad channel window");
-    hipLaunchKernelGGL(k_up_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, Cs, off, dX,
-                       Vol{D, H, W}, C);
+    if (C % 8 == 0 && Cs % 8 == 0 && off % 8 == 0 && (uintptr_t)dY % 16 == 0 && (uintptr_t)dX % 16 == 0)
+        hipLaunchKernelGGL(k_up_bwd8, dim3(blocks((size_t)W * (C / 8)), (unsigned)H, (unsigned)D), dim3(256), 0,
+                           as_stream(stream), dY, Cs, off, dX, Vol{D, H, W}, C);
+    else
+        hipLaunchKernelGGL(k_up_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, Cs, off,
+                           dX, Vol{D, H, W}, C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
