@@ -508,7 +508,7 @@ __global__ __launch_bounds__(256) void k_bias_grad8(const uint16_t* __restrict__
     const int rows = 256 / g8;
     const int r = threadIdx.x / g8, c8 = (threadIdx.x - r * g8) * 8;
     float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int v = blockIdx.x * rows + r; v < V; v += gridDim.x * rows) {
+    for (int64_t v = (int64_t)blockIdx.x * rows + r; v < V; v += (int64_t)gridDim.x * rows) {
         const uint4 u = *reinterpret_cast<const uint4*>(dY + (size_t)v * C + c8);
         const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
@@ -745,6 +745,110 @@ __global__ void k_embed(const float* __restrict__ img, uint16_t* __restrict__ X,
     X[i] = (c == 0 && z < Nz) ? f2bf(img[v] * scale) : (uint16_t)0;
 }
 
+// Unpacks 8 BF16 values from 16 bytes.
+__device__ inline void bf8_unpack(const uint4 u, float (&f)[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) f[j] = bf2f((uint16_t)(w[j >> 1] >> (16 * (j & 1))));
+}
+
+__device__ inline uint4 bf8_pack(const float (&f)[8]) {
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    return uint4{o[0], o[1], o[2], o[3]};
+}
+
+// k_pool_bwd, 8 channels per thread: grid = (ceil(W * C / 8 / 256), H, D) of the fine grid g.
+__global__ void k_pool_bwd8(const uint16_t* __restrict__ dY, uint16_t* __restrict__ dX, Vol g, int C, int accumulate) {
+    const int c8n = C / 8;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.W * c8n) return;
+    const int x = i / c8n, c8 = (i - x * c8n) * 8, y = blockIdx.y, z = blockIdx.z;
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    float a[8];
+    bf8_unpack(*reinterpret_cast<const uint4*>(dY + ((size_t)(z / 2) * Ho + y / 2) * Wo * C + (size_t)(x / 2) * C + c8), a);
+    uint4* d = reinterpret_cast<uint4*>(dX + (((size_t)z * g.H + y) * g.W + x) * C + c8);
+    float o[8];
+    if (accumulate) bf8_unpack(*d, o);
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = accumulate ? o[j] + a[j] * 0.125f : a[j] * 0.125f;
+    *d = bf8_pack(o);
+}
+
+// k_head_fwd with 16-byte channel loads (C a multiple of 8); channels summed in the same order.
+__global__ void k_head_fwd8(const uint16_t* __restrict__ X, const float* __restrict__ w, const float* __restrict__ b,
+                            const float* __restrict__ base, float* __restrict__ out, int V, int C) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= V) return;
+    float s = b[0];
+    for (int c8 = 0; c8 < C; c8 += 8) {
+        float f[8];
+        bf8_unpack(*reinterpret_cast<const uint4*>(X + (size_t)v * C + c8), f);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += w[c8 + j] * f[j];
+    }
+    out[v] = base[v] + s;
+}
+
+// k_head_bwd, 8 channels per thread over a capped grid: dX rows written as 16-byte stores, the weight and
+// bias gradients kept as per-thread partial sums, folded in LDS, one atomic per channel per block.
+__global__ __launch_bounds__(256) void k_head_bwd8(const uint16_t* __restrict__ X, const float* __restrict__ w,
+                                                   const float* __restrict__ gout, uint16_t* __restrict__ dX,
+                                                   float* __restrict__ dw, float* __restrict__ db, int V, int C) {
+    __shared__ float part[2048];
+    __shared__ float gpart[4];
+    const int g8 = C / 8;
+    const int rows = 256 / g8;
+    const int r = threadIdx.x / g8, c8 = (threadIdx.x - r * g8) * 8;
+    float wc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) wc[j] = w[c8 + j];
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float sg = 0.f;
+    for (int64_t v = (int64_t)blockIdx.x * rows + r; v < V; v += (int64_t)gridDim.x * rows) {
+        const float gv = gout[v];
+        float f[8], d[8];
+        bf8_unpack(*reinterpret_cast<const uint4*>(X + (size_t)v * C + c8), f);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            s[j] += gv * f[j];
+            d[j] = gv * wc[j];
+        }
+        *reinterpret_cast<uint4*>(dX + (size_t)v * C + c8) = bf8_pack(d);
+        if (c8 == 0) sg += gv;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) part[r * C + c8 + j] = s[j];   // rows * C == 2048
+    for (int o = 32; o > 0; o >>= 1) sg += __shfl_xor(sg, o, 64);
+    if ((threadIdx.x & 63) == 0) gpart[threadIdx.x >> 6] = sg;
+    __syncthreads();
+    const int c = threadIdx.x % C, q = threadIdx.x / C, step = 256 / C;
+    float t = 0.f;
+    for (int k = q; k < rows; k += step) t += part[k * C + c];
+    __syncthreads();
+    part[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float u = 0.f;
+        for (int k = 0; k < step; k++) u += part[k * C + threadIdx.x];
+        unsafeAtomicAdd(dw + threadIdx.x, u);
+    }
+    if (threadIdx.x == 0) unsafeAtomicAdd(db, gpart[0] + gpart[1] + gpart[2] + gpart[3]);
+}
+
+// k_embed, one voxel (two 16-byte rows of 8 channels) per thread.
+__global__ void k_embed16(const float* __restrict__ img, uint16_t* __restrict__ X, Vol g, int Nz, float scale) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= g.D * g.H * g.W) return;
+    const int z = v / (g.H * g.W);
+    uint32_t c0 = 0u;
+    if (z < Nz) c0 = f2bf(img[v] * scale);
+    uint4* d = reinterpret_cast<uint4*>(X + (size_t)v * 16);
+    d[0] = uint4{c0, 0u, 0u, 0u};
+    d[1] = uint4{0u, 0u, 0u, 0u};
+}
+
 static inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
 template <int MT>
@@ -962,8 +1066,12 @@ extern "C" int omg_pool_backward(const uint16_t* dY, uint16_t* dX, int32_t D, in
                                  int32_t accumulate, void* stream) {
     OMG_REQUIRE(dY && dX && C > 0 && vol_ok(D, H, W) && D % 2 == 0 && H % 2 == 0 && W % 2 == 0,
                 "pooling needs even positive dimensions");
-    hipLaunchKernelGGL(k_pool_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, dX,
-                       Vol{D, H, W}, C, accumulate);
+    if (H <= 65535 && D <= 65535 && C % 8 == 0 && (uintptr_t)dY % 16 == 0 && (uintptr_t)dX % 16 == 0)
+        hipLaunchKernelGGL(k_pool_bwd8, dim3(blocks((size_t)W * (C / 8)), (unsigned)H, (unsigned)D), dim3(256), 0,
+                           as_stream(stream), dY, dX, Vol{D, H, W}, C, accumulate);
+    else
+        hipLaunchKernelGGL(k_pool_bwd, dim3(blocks((size_t)D * H * W * C)), dim3(256), 0, as_stream(stream), dY, dX,
+                           Vol{D, H, W}, C, accumulate);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -972,7 +1080,7 @@ extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, i
                                     void* stream) {
     OMG_REQUIRE(X && Y && C > 0 && vol_ok(D, H, W) && D % 2 == 0 && H % 2 == 0 && W % 2 == 0,
                 "upsampling needs an even positive fine grid");
-    if (C % 8 == 0 && (uintptr_t)X % 16 == 0 && (uintptr_t)Y % 16 == 0)
+    if (H <= 65535 && D <= 65535 && C % 8 == 0 && (uintptr_t)X % 16 == 0 && (uintptr_t)Y % 16 == 0)
         hipLaunchKernelGGL(k_up_fwd8, dim3(blocks((size_t)W * (C / 8)), (unsigned)H, (unsigned)D), dim3(256), 0,
                            as_stream(stream), X, Y, Vol{D, H, W}, C);
     else
@@ -985,7 +1093,8 @@ extern "C" int omg_upsample_forward(const uint16_t* X, uint16_t* Y, int32_t D, i
 extern "C" int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off, uint16_t* dX, int32_t D, int32_t H,
                                      int32_t W, int32_t C, void* stream) {
     OMG_REQUIRE(dY && dX && vol_ok(D, H, W) && C > 0 && off >= 0 && off + C <= Cs, "bad channel window");
-    if (C % 8 == 0 && Cs % 8 == 0 && off % 8 == 0 && (uintptr_t)dY % 16 == 0 && (uintptr_t)dX % 16 == 0)
+    if (H <= 65535 && D <= 65535 && C % 8 == 0 && Cs % 8 == 0 && off % 8 == 0 && (uintptr_t)dY % 16 == 0 &&
+        (uintptr_t)dX % 16 == 0)
         hipLaunchKernelGGL(k_up_bwd8, dim3(blocks((size_t)W * (C / 8)), (unsigned)H, (unsigned)D), dim3(256), 0,
                            as_stream(stream), dY, Cs, off, dX, Vol{D, H, W}, C);
     else
@@ -1007,7 +1116,12 @@ extern "C" int omg_take_channels(const uint16_t* src, int32_t Cs, int32_t off, u
 extern "C" int omg_head_forward(const uint16_t* X, const float* w, const float* b, const float* base, float* out,
                                 int32_t V, int32_t C, void* stream) {
     OMG_REQUIRE(X && w && b && base && out && V > 0 && C > 0 && C < 64, "bad arguments");
-    hipLaunchKernelGGL(k_head_fwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, b, base, out, V, C);
+    if (C % 8 == 0 && (uintptr_t)X % 16 == 0)
+        hipLaunchKernelGGL(k_head_fwd8, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, b, base, out, V,
+                           C);
+    else
+        hipLaunchKernelGGL(k_head_fwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, b, base, out, V,
+                           C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -1015,7 +1129,11 @@ extern "C" int omg_head_forward(const uint16_t* X, const float* w, const float* 
 extern "C" int omg_head_backward(const uint16_t* X, const float* w, const float* g, uint16_t* dX, float* dw, float* db,
                                  int32_t V, int32_t C, void* stream) {
     OMG_REQUIRE(X && w && g && dX && dw && db && V > 0 && C > 0 && C < 64, "bad arguments");
-    hipLaunchKernelGGL(k_head_bwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, g, dX, dw, db, V, C);
+    if ((C == 8 || C == 16 || C == 32) && (uintptr_t)X % 16 == 0 && (uintptr_t)dX % 16 == 0)
+        hipLaunchKernelGGL(k_head_bwd8, dim3(512), dim3(256), 0, as_stream(stream), X, w, g, dX, dw, db, V, C);
+    else
+        hipLaunchKernelGGL(k_head_bwd, dim3(blocks((size_t)V)), dim3(256), 0, as_stream(stream), X, w, g, dX, dw, db, V,
+                           C);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -1049,8 +1167,12 @@ extern "C" int omg_to_bf16(const float* src, uint16_t* dst, int64_t n, void* str
 extern "C" int omg_embed(const float* img, uint16_t* X, int32_t D, int32_t H, int32_t W, int32_t Nz, float scale,
                          void* stream) {
     OMG_REQUIRE(img && X && vol_ok(D, H, W) && Nz >= 1 && Nz <= D, "bad arguments");
-    hipLaunchKernelGGL(k_embed, dim3(blocks((size_t)D * H * W * 16)), dim3(256), 0, as_stream(stream), img, X,
-                       Vol{D, H, W}, Nz, scale);
+    if ((uintptr_t)X % 16 == 0)
+        hipLaunchKernelGGL(k_embed16, dim3(blocks((size_t)D * H * W)), dim3(256), 0, as_stream(stream), img, X,
+                           Vol{D, H, W}, Nz, scale);
+    else
+        hipLaunchKernelGGL(k_embed, dim3(blocks((size_t)D * H * W * 16)), dim3(256), 0, as_stream(stream), img, X,
+                           Vol{D, H, W}, Nz, scale);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
