@@ -461,17 +461,25 @@ __global__ __launch_bounds__(kWgThreads) void k_conv3d_wgrad_tile(const uint16_t
 
 // dW[i] += sum over the P partials of part[p][i]: a block takes 64 consecutive outputs (lane l: i0 + l,
 // coalesced), its 4 waves the partials p = wave, wave + 4, ..., combined through LDS.
-__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, size_t per,
-                                                      int P) {
-    __shared__ float red[4][64];
+__global__ __launch_bounds__(1024) void k_wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW,
+                                                       size_t per, int P) {
+    // 16 waves per 64 outputs, wave w summing partials w, w + 16, ...; folded in a fixed order
+    __shared__ float red[16][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t i = (size_t)blockIdx.x * 64 + lane;
     float s = 0.f;
-    if (i < per)
-        for (int p = wave; p < P; p += 4) s += part[(size_t)p * per + i];
+    if (i < per) {
+#pragma unroll 4
+        for (int p = wave; p < P; p += 16) s += part[(size_t)p * per + i];
+    }
     red[wave][lane] = s;
     __syncthreads();
-    if (wave == 0 && i < per) dW[i] += (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (wave == 0 && i < per) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < 16; w++) t += red[w][lane];
+        dW[i] += t;
+    }
 }
 
 // Wf[tap][ci][co] = W[26 - tap][co][ci]: dX = conv(dY) with these weights.
@@ -849,6 +857,28 @@ __global__ void k_embed16(const float* __restrict__ img, uint16_t* __restrict__ 
     d[1] = uint4{0u, 0u, 0u, 0u};
 }
 
+// k_take_channels, 8 channels (16 bytes) per thread (C, Cs and off multiples of 8).
+__global__ void k_take_channels8(const uint16_t* __restrict__ src, int Cs, int off, uint16_t* __restrict__ dst,
+                                 size_t V, int C, int accumulate) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int c8n = C / 8;
+    if (i >= V * c8n) return;
+    const size_t v = i / c8n;
+    const int c8 = (int)(i - v * c8n) * 8;
+    const uint4 u = *reinterpret_cast<const uint4*>(src + v * Cs + off + c8);
+    uint4* d = reinterpret_cast<uint4*>(dst + v * C + c8);
+    if (!accumulate) {
+        *d = u;
+        return;
+    }
+    float a[8], o[8];
+    bf8_unpack(u, a);
+    bf8_unpack(*d, o);
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] += a[j];
+    *d = bf8_pack(o);
+}
+
 static inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
 template <int MT>
@@ -938,7 +968,7 @@ static int launch_wgrad_tile(const uint16_t* dY, const uint16_t* X1, int C1, con
     hipLaunchKernelGGL(k_conv3d_wgrad_tile<MT>, dim3((unsigned)P, (unsigned)NT), dim3(kWgThreads), lds, s, dY, X1, C1,
                        X2, C2, g_wg_part, g, tiles_x, tiles_y, n_tiles);
     OMG_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((per + 63) / 64)), dim3(256), 0, s, g_wg_part, dW, per, P);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((per + 63) / 64)), dim3(1024), 0, s, g_wg_part, dW, per, P);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -1107,8 +1137,12 @@ extern "C" int omg_upsample_backward(const uint16_t* dY, int32_t Cs, int32_t off
 extern "C" int omg_take_channels(const uint16_t* src, int32_t Cs, int32_t off, uint16_t* dst, int64_t V, int32_t C,
                                  int32_t accumulate, void* stream) {
     OMG_REQUIRE(src && dst && V > 0 && C > 0 && off >= 0 && off + C <= Cs, "bad channel window");
-    hipLaunchKernelGGL(k_take_channels, dim3(blocks((size_t)V * C)), dim3(256), 0, as_stream(stream), src, Cs, off, dst,
-                       (size_t)V, C, accumulate);
+    if (C % 8 == 0 && Cs % 8 == 0 && off % 8 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0)
+        hipLaunchKernelGGL(k_take_channels8, dim3(blocks((size_t)V * (C / 8))), dim3(256), 0, as_stream(stream), src,
+                           Cs, off, dst, (size_t)V, C, accumulate);
+    else
+        hipLaunchKernelGGL(k_take_channels, dim3(blocks((size_t)V * C)), dim3(256), 0, as_stream(stream), src, Cs, off,
+                           dst, (size_t)V, C, accumulate);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
