@@ -273,8 +273,22 @@ __device__ __forceinline__ int o_sweep(const T& t, int tempi, float cx, int temp
 
 // One LOR: visit(ind, w) for every voxel of its tube. Returns the LOR weight temp (0: ray skipped); with
 // temp_out, temp is also stored there before the first visit (for callers whose visit applies it).
-template <bool VOL, class V>
-__device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out = nullptr) {
+// No time of flight: nothing to do per stretch of the centre ray.
+struct NoSeg {
+    __device__ void operator()(float, float, float) const {}
+};
+
+// The OpenCL built-in sign(): +-1, the argument itself for +-0, 0 for NaN.
+__device__ __forceinline__ float cl_sign(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : (v == v ? v : 0.f)); }
+
+// One LOR: visit(ind, w) for every voxel of its tube; returns the LOR weight (0 for a skipped ray). With TOF,
+// seg(elem, D, DDsign) announces the centre ray's current stretch before the planes swept for it
+// (projectorType123.cl:653-686 axis-parallel, :845-847, :1022-1054, :1215-1220 otherwise): axis-parallel rays
+// start at D = (d N / 2) sign(diff_y) after the x/y swap and step by the voxel size (first / last partial
+// voxels: dT1 / dT2); other rays start at |diff tc| - L / 2 and step by each stretch
+// (min(t0, 1) - max(tc, 0)) L; the planes swept after the walk keep the last stretch's weights.
+template <bool VOL, bool TOF, class V, class S>
+__device__ __forceinline__ float orth_lor_t(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out, S&& seg) {
     const SidArgs& g = oa.a;
     Ray r;
     fetch_ray(g, lo, r);
@@ -323,21 +337,33 @@ __device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& vi
         const int tempk = (int)(fabsf(s.z - g.bz) / g.dz);
         if (tempk < 0 || tempk >= (int)g.Nz) return 0.f;
         int apuX1, apuX2, indO;
-        float b1, d1;
+        float b1, d1, dT1 = 0.f, dT2 = 0.f;
+        uint32_t Nrun = 0;
         auto span = [&](float sa, float da, float ba, float dd, uint32_t N) {
             apuX1 = 0;
             apuX2 = (int)N - 1;
+            if constexpr (TOF) {
+                dT1 = dd;
+                dT2 = dd;
+                Nrun = N;
+            }
             float dist1, dist2;
             if (sa > da) { dist1 = ba - da; dist2 = ba + (float)N * dd - sa; }
             else { dist1 = ba - sa; dist2 = ba + (float)N * dd - da; }
             if (dist1 < 0.f) {
                 const int kk = (int)ceilf(-dist1 / dd);
-                if (kk < (int)N) apuX1 = kk;
+                if (kk < (int)N) {
+                    apuX1 = kk;
+                    if constexpr (TOF) dT1 = fminf(dist1 + (float)kk * dd, dd);
+                }
             }
             if (dist2 > 0.f) {
                 const int mm = (int)ceilf(dist2 / dd);
                 const int kk = (int)N - 1 - mm;
-                if (kk >= apuX1) apuX2 = kk;
+                if (kk >= apuX1) {
+                    apuX2 = kk;
+                    if constexpr (TOF) dT2 = fminf((float)mm * dd - dist2, dd);
+                }
             }
         };
         if (fabsf(diff.y) < 1e-6f && d.y <= g.bmaxy && d.y >= g.by && s.y <= g.bmaxy && s.y >= g.by) {
@@ -357,9 +383,21 @@ __device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& vi
         }
         const float w = vol ? w1 * inv_v : w1;
         if (temp_out) *temp_out = w;
+        float D = 0.f, DDsign = 0.f;
+        if constexpr (TOF) {
+            D = ((d1 * (float)Nrun) / 2.f) * cl_sign(diff.y);
+            DDsign = cl_sign(D);
+        }
         for (int ii = apuX1; ii <= apuX2; ii++) {
+            if constexpr (TOF) {
+                float d_in = d1;
+                if (apuX1 > 0 && ii == apuX1) d_in = dT1;
+                else if (apuX2 < (int)Nrun - 1 && ii == apuX2) d_in = dT2;
+                seg(d_in, D, DDsign);
+            }
             const float cx = b1 + d1 * (float)ii + d1 / 2.f;
             o_sweep(t, ii, cx, indO, tempk, false, tempk_b, 0, visit);
+            if constexpr (TOF) D -= d1 * DDsign;
         }
         return w;
     }
@@ -389,6 +427,12 @@ __device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& vi
         skip = o_pre3(g, s, d, diff, tempi, tempj, tempk, txu, tyu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0, XY);
     }
     if (skip) return 0.f;
+    float D = 0.f, DDsign = 0.f, tcv = tc;
+    if constexpr (TOF) {   // TOFDis with the unswapped difference vector
+        const float ax = diff.x * tc, ay = diff.y * tc, az = diff.z * tc;
+        D = sqrtf(ax * ax + ay * ay + az * az) - L / 2.f;
+        DDsign = cl_sign(D);
+    }
 
     float b1, d1;
     uint32_t NNx;
@@ -416,9 +460,18 @@ __device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& vi
         const int li = tempi, lj = tempj, lk = tempk;
         const float tx0_a = tx0, ty0_a = ty0, tz0_a = tz0;
         tempi_a = tempi; tempj_a = tempj; tempk_a = tempk;
-        if (tz0 < ty0 && tz0 < tx0) { tempk += uz; tz0 += tzu; }
-        else if (ty0 < tx0) { tempj += uy; ty0 += tyu; }
-        else { tempi += ux; tx0 += txu; }
+        float t0;
+        if (tz0 < ty0 && tz0 < tx0) { t0 = tz0; tempk += uz; tz0 += tzu; }
+        else if (ty0 < tx0) { t0 = ty0; tempj += uy; ty0 += tyu; }
+        else { t0 = tx0; tempi += ux; tx0 += txu; }
+        float elem = 0.f;
+        if constexpr (TOF) {   // compute_element, general_opencl_functions.h:1809-1818
+            const bool pass = (t0 >= 0.f && t0 <= 1.f) || (tcv >= 0.f && tcv <= 1.f);
+            if (pass) elem = (fminf(t0, 1.f) - fmaxf(tcv, 0.f)) * L;
+            tcv = t0;
+            seg(elem, D, DDsign);
+        }
+        (void)t0;
         if (ii == 0u) {
             tempk_b = tempk_a;
             if (ux >= 0) {
@@ -441,6 +494,7 @@ __device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& vi
             tempiOld = tempi_a;
             swept = true;
         }
+        if constexpr (TOF) D -= elem * DDsign;
         if (tempi < 0 || tempi >= (int)g.Nx || tempj < 0 || tempj >= (int)g.Ny || tempk < 0 || tempk >= (int)g.Nz)
             break;
     }
@@ -459,6 +513,11 @@ __device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& vi
         }
     }
     return w;
+}
+
+template <bool VOL, class V>
+__device__ __forceinline__ float orth_lor(const OrthArgs& oa, int64_t lo, V&& visit, float* temp_out = nullptr) {
+    return orth_lor_t<VOL, false>(oa, lo, visit, temp_out, NoSeg{});
 }
 
 template <bool VOL>
@@ -512,6 +571,140 @@ __global__ __launch_bounds__(kOrthBlock) void k_orth_osem(OrthArgs oa, const flo
     } else {
         orth_lor<VOL>(oa, lo, [&](int64_t ind, float w) { unsafeAtomicAdd(rhs + ind, r * (w * temp)); });
     }
+}
+
+// ---- Time of flight (projector 3; projectorType123.cl -DTOF with -DORTH -DVOL) ----
+// One lane per LOR, one wave per workgroup; a lane's bins (accumulators, measurements or ratios) and the
+// current stretch's bin weights live in LDS, laid out [bin][lane].
+constexpr int kOrthTofBlock = 64;
+
+// The bin weights of one centre-ray stretch (TOFLoop / TOFWeight, general_opencl_functions.h:1074-1116 with
+// -DUSEMAD, TRAPZ_BINS 4): normPDF(x) = (0.3989423f / sigma) exp(-0.5 ((x - mu) / sigma)^2), weight_b =
+// (p(D) + 2 p(D - dX s) + 2 p(D - 2 dX s) + p(D - elem s)) dX with dX = elem / 3, s = DDsign, and their sum
+// floored at epps. The walk announces every stretch; the weights are evaluated only when a tube voxel of the
+// stretch's planes needs them. Before the first stretch the weights are zero.
+struct OrthTof {
+    const SidArgs& a;
+    float* wl;
+    float inv, pf, sum, elem, D, sgn;
+    bool dirty;
+    __device__ OrthTof(const SidArgs& args, float* w)
+        : a(args), wl(w), inv(1.f / args.tof_sigma), pf(0.3989423f * (1.f / args.tof_sigma)), sum(1.f), elem(0.f),
+          D(0.f), sgn(0.f), dirty(false) {
+        for (uint32_t b = 0; b < a.tof_bins; b++) wl[b * kOrthTofBlock] = 0.f;
+    }
+    __device__ void set(float e, float d, float s) {
+        elem = e;
+        D = d;
+        sgn = s;
+        dirty = true;
+    }
+    __device__ float pdf(float x, float mu) const {
+        const float q = (x - mu) * inv;
+        return pf * expf(-0.5f * q * q);
+    }
+    __device__ void ready() {
+        if (!dirty) return;
+        dirty = false;
+        const float dX = elem / 3.f;
+        const float dXs = dX * sgn;
+        float s = 0.f;
+        for (uint32_t b = 0; b < a.tof_bins; b++) {
+            const float mu = a.tof_c[b];
+            float out = pdf(D, mu);
+            out += pdf(fmaf(-dXs, 1.f, D), mu) * 2.f;
+            out += pdf(fmaf(-dXs, 2.f, D), mu) * 2.f;
+            out += pdf(fmaf(-elem, sgn, D), mu);
+            const float v = out * dX;
+            wl[b * kOrthTofBlock] = v;
+            s += v;
+        }
+        sum = s < a.tof_epps ? a.tof_epps : s;
+    }
+};
+
+// FP (denominator, general_opencl_functions.h:1178-1206): ax_b += weight_b ((w x[v]) / sum); y_b = ax_b temp.
+template <bool VOL>
+__device__ __forceinline__ float orth_tof_gather(const OrthArgs& oa, int64_t lo, const float* __restrict__ img,
+                                                 float* ax, OrthTof& t) {
+    for (uint32_t b = 0; b < oa.a.tof_bins; b++) ax[b * kOrthTofBlock] = 0.f;
+    return orth_lor_t<VOL, true>(oa, lo, [&](int64_t ind, float w) {
+        t.ready();
+        const float dX = (w * img[ind]) / t.sum;
+        for (uint32_t b = 0; b < oa.a.tof_bins; b++) ax[b * kOrthTofBlock] += t.wl[b * kOrthTofBlock] * dX;
+    }, nullptr, [&](float e, float D, float sg) { t.set(e, D, sg); });
+}
+
+// BP (rhs, :1210-1245): rhs[v] += sum_b ((w temp) / sum) weight_b r_b; sens[v] += sum_b ((w temp) / sum) weight_b.
+template <bool VOL>
+__device__ __forceinline__ void orth_tof_scatter(const OrthArgs& oa, int64_t lo, const float* r, OrthTof& t,
+                                                 float* __restrict__ rhs, float* __restrict__ sens) {
+    float temp = 0.f;   // the LOR weight, set before the first visit
+    orth_lor_t<VOL, true>(oa, lo, [&](int64_t ind, float w) {
+        t.ready();
+        const float dX = (w * temp) / t.sum;
+        float val = 0.f, yax = 0.f;
+        for (uint32_t b = 0; b < oa.a.tof_bins; b++) {
+            const float apu = dX * t.wl[b * kOrthTofBlock];
+            val += apu;
+            yax += apu * r[b * kOrthTofBlock];
+        }
+        unsafeAtomicAdd(rhs + ind, yax);
+        if (sens) unsafeAtomicAdd(sens + ind, val);
+    }, &temp, [&](float e, float D, float sg) { t.set(e, D, sg); });
+}
+
+template <bool VOL>
+__global__ __launch_bounds__(kOrthTofBlock) void k_orth_tof_fp(OrthArgs oa, const float* __restrict__ img,
+                                                               float* __restrict__ y) {
+    extern __shared__ float lds[];
+    const int64_t lo = (int64_t)blockIdx.x * kOrthTofBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    float* ax = lds + threadIdx.x;
+    OrthTof t(oa.a, lds + oa.a.tof_bins * kOrthTofBlock + threadIdx.x);
+    const float temp = orth_tof_gather<VOL>(oa, lo, img, ax, t);
+    for (uint32_t b = 0; b < oa.a.tof_bins; b++) y[lo + (int64_t)b * oa.a.nmeas] = ax[b * kOrthTofBlock] * temp;
+}
+
+template <bool VOL>
+__global__ __launch_bounds__(kOrthTofBlock) void k_orth_tof_bp(OrthArgs oa, const float* __restrict__ yin,
+                                                               float* __restrict__ rhs, float* __restrict__ sens) {
+    extern __shared__ float lds[];
+    const int64_t lo = (int64_t)blockIdx.x * kOrthTofBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    float* yv = lds + threadIdx.x;
+    bool any = false;
+    for (uint32_t b = 0; b < oa.a.tof_bins; b++) {
+        const float v = yin[lo + (int64_t)b * oa.a.nmeas];
+        yv[b * kOrthTofBlock] = v;
+        any |= v != 0.f;
+    }
+    if (!any && sens == nullptr) return;   // adds of +0 leave rhs unchanged
+    OrthTof t(oa.a, lds + oa.a.tof_bins * kOrthTofBlock + threadIdx.x);
+    orth_tof_scatter<VOL>(oa, lo, yv, t, rhs, sens);
+}
+
+// Fused data step: yhat_b = ax_b temp; r_b = meas_b / ((yhat_b + epps) + add); NaN/Inf -> epps; BP of r.
+template <bool VOL>
+__global__ __launch_bounds__(kOrthTofBlock) void k_orth_tof_osem(OrthArgs oa, const float* __restrict__ img,
+                                                                 const float* __restrict__ meas,
+                                                                 float* __restrict__ rhs, float* __restrict__ sens,
+                                                                 float epps) {
+    extern __shared__ float lds[];
+    const int64_t lo = (int64_t)blockIdx.x * kOrthTofBlock + threadIdx.x;
+    if (lo >= oa.a.nmeas) return;
+    float* ax = lds + threadIdx.x;
+    float* wl = lds + oa.a.tof_bins * kOrthTofBlock + threadIdx.x;
+    OrthTof t(oa.a, wl);
+    const float temp = orth_tof_gather<VOL>(oa, lo, img, ax, t);
+    if (temp == 0.f) return;
+    for (uint32_t b = 0; b < oa.a.tof_bins; b++) {
+        float r = meas[lo + (int64_t)b * oa.a.nmeas] / ratio_den(oa.a, lo, ax[b * kOrthTofBlock] * temp, epps);
+        if (isnan(r) || isinf(r)) r = epps;
+        ax[b * kOrthTofBlock] = r;
+    }
+    OrthTof t2(oa.a, wl);
+    orth_tof_scatter<VOL>(oa, lo, ax, t2, rhs, sens);
 }
 
 // Per-LOR voxel lists: counts only (offsets == null) or indices and tube weights at offsets[lo].
@@ -1185,7 +1378,18 @@ static int tube_launch_planar(const omg_geom* g, const TubeEntry* e, int64_t fir
     return OMG_OK;
 }
 
+// LDS of the TOF kernels: [bins: accumulators / measurements / ratios][bins: weights] x 64 lanes.
+static size_t orth_tof_lds(const omg_geom* g) { return 2 * (size_t)g->d.tof_bins * kOrthTofBlock * sizeof(float); }
+
 int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
+    if (g->d.tof_bins > 1) {
+        const OrthArgs oa = make_orth_args(g, first, n);
+        auto* kern = oa.vol ? k_orth_tof_fp<true> : k_orth_tof_fp<false>;
+        hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthTofBlock)), dim3(kOrthTofBlock), orth_tof_lds(g), s, oa, img,
+                           y);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
     if (g->orth_planar && !g->force_generic)
         return orth_launch_planar<true, false, false>(g, first, n, img, nullptr, y, nullptr, nullptr, 0.f, s);
     const TubeEntry* e = nullptr;
@@ -1204,6 +1408,14 @@ int orth_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, 
 
 int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
                   hipStream_t s) {
+    if (g->d.tof_bins > 1) {
+        const OrthArgs oa = make_orth_args(g, first, n);
+        auto* kern = oa.vol ? k_orth_tof_bp<true> : k_orth_tof_bp<false>;
+        hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthTofBlock)), dim3(kOrthTofBlock), orth_tof_lds(g), s, oa, yin,
+                           rhs, sens);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
     if (g->orth_planar && !g->force_generic) {
         if (sens) return orth_launch_planar<false, true, true>(g, first, n, nullptr, yin, nullptr, rhs, sens, 0.f, s);
         return orth_launch_planar<false, true, false>(g, first, n, nullptr, yin, nullptr, rhs, nullptr, 0.f, s);
@@ -1225,6 +1437,15 @@ int orth_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin,
 
 int orth_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
               float* sens, float epps, hipStream_t s, const float* add) {
+    if (g->d.tof_bins > 1) {
+        OrthArgs oa = make_orth_args(g, first, n);
+        oa.a.add = add;
+        auto* kern = oa.vol ? k_orth_tof_osem<true> : k_orth_tof_osem<false>;
+        hipLaunchKernelGGL(kern, dim3(grid_for(n, kOrthTofBlock)), dim3(kOrthTofBlock), orth_tof_lds(g), s, oa, img,
+                           meas, rhs, sens, epps);
+        OMG_CHECK_LAUNCH();
+        return OMG_OK;
+    }
     if (g->orth_planar && !g->force_generic) {
         if (sens) return orth_launch_planar<true, true, true>(g, first, n, img, meas, nullptr, rhs, sens, epps, s, add);
         return orth_launch_planar<true, true, false>(g, first, n, img, meas, nullptr, rhs, nullptr, epps, s, add);

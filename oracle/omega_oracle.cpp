@@ -1343,9 +1343,55 @@ static int sweep(const OGeo& o, int tempi, F3 diff, F3 center, F3 s, float b2, f
     return uu;
 }
 
+// No time of flight: nothing to do per stretch of the centre ray.
+struct NoSeg {
+    void operator()(float, float, float) const {}
+};
+
+// The OpenCL built-in sign(): +-1, the argument itself for +-0, 0 for NaN.
+static inline float cl_sign(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : (v == v ? v : 0.f)); }
+
+// Time-of-flight bin weights of the GPU kernel (general_opencl_functions.h:1074-1116, -DUSEMAD, TRAPZ_BINS 4):
+// normPDF(x) = (0.3989423f / sigma) exp(-0.5 ((x - mu) / sigma)^2) with the reciprocal taken once; for a
+// stretch of length `elem` at distance D, weight_b = (p(D) + 2 p(D - dX s) + 2 p(D - 2 dX s) + p(D - elem s)) dX,
+// dX = elem / 3, s = DDsign; returns the weights' sum floored at epps.
+struct TofCL {
+    uint32_t bins;
+    const float* c;
+    float inv, pf, epps;
+    TofCL(uint32_t b, const float* centres, float sigma, float e)
+        : bins(b), c(centres), inv(1.f / sigma), pf(0.3989423f * (1.f / sigma)), epps(e) {}
+    float pdf(float x, float mu) const {
+        const float a = (x - mu) * inv;
+        return pf * std::exp(-0.5f * a * a);
+    }
+    float loop(float DDsign, float elem, float D, float* w) const {
+        float sum = 0.f;
+        const float dX = elem / 3.f;
+        for (uint32_t to = 0; to < bins; to++) {
+            const float dXs = dX * DDsign;
+            float out = pdf(D, c[to]);
+            for (int tr = 1; tr < 3; tr++) out += pdf(std::fma(-dXs, (float)tr, D), c[to]) * 2.f;
+            out += pdf(std::fma(-elem, DDsign, D), c[to]);
+            w[to] = out * dX;
+            sum += w[to];
+        }
+        if (sum < epps) sum = epps;
+        return sum;
+    }
+};
+
 // One LOR: visit(ind, w) for every voxel of its tube; `temp` receives the LOR weight (0 for a skipped ray).
-template <class V>
-static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt, float& temp, V&& visit) {
+// With time of flight, seg(elem, D, DDsign) announces the centre ray's current stretch before the planes it
+// sweeps (projectorType123.cl:653-686 for axis-parallel rays, :845-847, :1022-1054, :1215-1220 otherwise):
+// axis-parallel rays start at D = (d * N / 2) sign(diff_y) (diff after the kernel's x/y swap) and step by the
+// voxel size d, the first and last partial voxels' stretches dT1 / dT2; other rays start at
+// D = |diff tc| - L / 2 and step by each Siddon stretch (min(t0, 1) - max(tc, 0)) L; the planes swept after the
+// walk take the last stretch's weights.
+template <class V, class S = NoSeg>
+static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt, float& temp, V&& visit,
+                     S&& seg = S{}) {
+    constexpr bool TOF = !std::is_same<typename std::decay<S>::type, NoSeg>::value;
     const oracle::Geo<float>& g = o.g;
     oracle::Ray<float> r;
     oracle::fetch_ray(g, lo, xt, zt, r);
@@ -1385,22 +1431,31 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
         const int tempk = (int)(std::fabs(s.z - g.bz) / g.dz);
         if (tempk < 0 || tempk >= (int)g.Nz) return;
         int apuX1, apuX2, indO;
-        float b1, b2, d1, d2;
-        uint32_t N1, N2p, N3p;
+        float b1, b2, d1, d2, dT1, dT2;
+        uint32_t N1, N2p, N3p, Nrun = 0;
         auto span = [&](float sa, float da, float ba, float dd, uint32_t N) {
             apuX1 = 0;
             apuX2 = (int)N - 1;
+            dT1 = dd;
+            dT2 = dd;
+            Nrun = N;
             float dist1, dist2;
             if (sa > da) { dist1 = ba - da; dist2 = ba + (float)N * dd - sa; }
             else { dist1 = ba - sa; dist2 = ba + (float)N * dd - da; }
             if (dist1 < 0.f) {
                 const int kk = (int)std::ceil(-dist1 / dd);
-                if (kk < (int)N) apuX1 = kk;
+                if (kk < (int)N) {
+                    apuX1 = kk;
+                    dT1 = std::fmin(dist1 + (float)kk * dd, dd);
+                }
             }
             if (dist2 > 0.f) {
                 const int mm = (int)std::ceil(dist2 / dd);
                 const int kk = (int)N - 1 - mm;
-                if (kk >= apuX1) apuX2 = kk;
+                if (kk >= apuX1) {
+                    apuX2 = kk;
+                    dT2 = std::fmin((float)mm * dd - dist2, dd);
+                }
             }
         };
         if (std::fabs(diff.y) < 1e-6f && d.y <= o.bmaxy && d.y >= g.by && s.y <= o.bmaxy && s.y >= g.by) {
@@ -1419,10 +1474,22 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
             return;
         }
         temp = vol ? w * inv_v : w;
+        float D = 0.f, DDsign = 0.f;
+        if (TOF) {
+            D = ((d1 * (float)Nrun) / 2.f) * cl_sign(diff.y);
+            DDsign = cl_sign(D);
+        }
         for (int ii = apuX1; ii <= apuX2; ii++) {
+            if (TOF) {
+                float d_in = d1;
+                if (apuX1 > 0 && ii == apuX1) d_in = dT1;
+                else if (apuX2 < (int)Nrun - 1 && ii == apuX2) d_in = dT2;
+                seg(d_in, D, DDsign);
+            }
             center.x = b1 + d1 * (float)ii + d1 / 2.f;
             sweep(o, ii, diff, center, s, b2, d2, g.bz, g.dz, indO, tempk, Nxy, orl, N1, N2p, N3p, false, tempk_b, 0,
                   visit);
+            if (TOF) D -= d1 * DDsign;
         }
         return;
     }
@@ -1452,6 +1519,12 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
         skip = pre3(o, s, d, diff, tempi, tempj, tempk, txu, tyu, tzu, Np, tc, ux, uy, uz, tx0, ty0, tz0, XY);
     }
     if (skip) return;
+    float D = 0.f, DDsign = 0.f, tcv = tc;
+    if (TOF) {   // TOFDis with the unswapped difference vector
+        const float ax = diff.x * tc, ay = diff.y * tc, az = diff.z * tc;
+        D = std::sqrt(ax * ax + ay * ay + az * az) - L / 2.f;
+        DDsign = cl_sign(D);
+    }
 
     float b1, b2, d1, d2;
     uint32_t N1, N2p, N3p, NNx;
@@ -1479,9 +1552,17 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
         const int li = tempi, lj = tempj, lk = tempk;
         tx0_a = tx0; ty0_a = ty0; tz0_a = tz0;
         tempi_a = tempi; tempj_a = tempj; tempk_a = tempk;
-        if (tz0 < ty0 && tz0 < tx0) { tempk += uz; tz0 += tzu; }
-        else if (ty0 < tx0) { tempj += uy; ty0 += tyu; }
-        else { tempi += ux; tx0 += txu; }
+        float t0;
+        if (tz0 < ty0 && tz0 < tx0) { t0 = tz0; tempk += uz; tz0 += tzu; }
+        else if (ty0 < tx0) { t0 = ty0; tempj += uy; ty0 += tyu; }
+        else { t0 = tx0; tempi += ux; tx0 += txu; }
+        float elem = 0.f;
+        if (TOF) {   // compute_element, general_opencl_functions.h:1809-1818
+            const bool pass = (t0 >= 0.f && t0 <= 1.f) || (tcv >= 0.f && tcv <= 1.f);
+            if (pass) elem = (std::fmin(t0, 1.f) - std::fmax(tcv, 0.f)) * L;
+            tcv = t0;
+            seg(elem, D, DDsign);
+        }
         if (ii == 0u) {
             tempk_b = tempk_a;
             if (ux >= 0) {
@@ -1508,6 +1589,7 @@ static void orth_lor(const OGeo& o, int64_t lo, const float* xt, const float* zt
             tempiOld = tempi_a;
             swept = true;
         }
+        if (TOF) D -= elem * DDsign;
         if (tempi < 0 || tempi >= (int)g.Nx || tempj < 0 || tempj >= (int)g.Ny || tempk < 0 ||
             tempk >= (int)g.Nz)
             break;
@@ -1593,6 +1675,61 @@ extern "C" int oracle_orth_bp_f32(ORACLE_ORTH_ARGS, const float* yin, float* rhs
                 sens[p.first] += e;
             }
         }
+    }
+    return 0;
+}
+
+// Time of flight (projectorType123.cl -DTOF with -DORTH / -DVOL): measurement arrays hold tof_bins blocks of
+// nMeas. FP: ax_b += weight_b * ((w x[v]) / sum) over the tube (denominator, general_opencl_functions.h:1178-1206),
+// y_b = ax_b * temp; BP: rhs[v] += sum_b ((w temp) / sum) weight_b y_b, sens[v] += sum_b ((w temp) / sum) weight_b
+// (rhs, :1210-1245).
+extern "C" int oracle_orth_tof_fp_f32(ORACLE_ORTH_ARGS, uint32_t tof_bins, float tof_sigma, const float* tof_c,
+                                      float tof_epps, const float* img, float* y, int nthreads) {
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags, vol, vol_len, vol_s);
+    const orth::TofCL tc(tof_bins, tof_c, tof_sigma, tof_epps);
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        std::vector<float> ax(tof_bins, 0.f), wt(tof_bins, 0.f);
+        float temp = 0.f, sum = 1.f;
+        orth::orth_lor(o, lo, x, z, temp,
+                       [&](int64_t ind, float w) {
+                           const float dX = (w * img[ind]) / sum;
+                           for (uint32_t to = 0; to < tof_bins; to++) ax[to] += wt[to] * dX;
+                       },
+                       [&](float elem, float D, float sgn) { sum = tc.loop(sgn, elem, D, wt.data()); });
+        for (uint32_t to = 0; to < tof_bins; to++) y[lo + (int64_t)to * nMeas] = ax[to] * temp;
+    }
+    return 0;
+}
+
+extern "C" int oracle_orth_tof_bp_f32(ORACLE_ORTH_ARGS, uint32_t tof_bins, float tof_sigma, const float* tof_c,
+                                      float tof_epps, const float* yin, float* rhs, float* sens, int nthreads) {
+    const orth::OGeo o = make_ogeo(n3, d3, b3, size_x, size_y, mode, plane_offset, global_factor, norm, atten, scat, width, flags, vol, vol_len, vol_s);
+    const orth::TofCL tc(tof_bins, tof_c, tof_sigma, tof_epps);
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        std::vector<float> r(tof_bins), wt(tof_bins, 0.f);
+        for (uint32_t to = 0; to < tof_bins; to++) r[to] = yin[lo + (int64_t)to * nMeas];
+        float temp = 0.f, sum = 1.f;
+        orth::orth_lor(o, lo, x, z, temp,
+                       [&](int64_t ind, float w) {
+                           const float dX = (w * temp) / sum;
+                           float val = 0.f, yax = 0.f;
+                           for (uint32_t to = 0; to < tof_bins; to++) {
+                               const float apu = dX * wt[to];
+                               val += apu;
+                               yax += apu * r[to];
+                           }
+#pragma omp atomic
+                           rhs[ind] += yax;
+                           if (sens) {
+#pragma omp atomic
+                               sens[ind] += val;
+                           }
+                       },
+                       [&](float elem, float D, float sgn) { sum = tc.loop(sgn, elem, D, wt.data()); });
     }
     return 0;
 }

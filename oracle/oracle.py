@@ -72,6 +72,9 @@ def lib():
                 _f32p]
         _lib.oracle_orth_fp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_int]
         _lib.oracle_orth_bp_f32.argtypes = ogeo + [_f32p, _f32p, C.c_void_p, C.c_int]
+        tofa = [C.c_uint32, C.c_float, _f32p, C.c_float]
+        _lib.oracle_orth_tof_fp_f32.argtypes = ogeo + tofa + [_f32p, _f32p, C.c_int]
+        _lib.oracle_orth_tof_bp_f32.argtypes = ogeo + tofa + [_f32p, _f32p, C.c_void_p, C.c_int]
         _lib.oracle_orth_trace_f32.argtypes = ogeo + [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.oracle_orth_trace_f32.restype = C.c_int64
         igeo = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, C.c_float, C.c_int32,
@@ -490,6 +493,26 @@ class OrthCase:
         s = np.zeros(self.grid.N, dtype=np.float32) if sens else None
         lib().oracle_orth_bp_f32(*self._geo(nmeas), np.ascontiguousarray(yin, dtype=np.float32), rhs,
                                  None if s is None else s.ctypes.data, nthreads)
+        return (rhs, s) if sens else rhs
+
+    def _tof(self, tof):
+        bins, sigma, centres, epps = tof
+        c = np.ascontiguousarray(centres, dtype=np.float32)
+        assert c.size == bins
+        return [int(bins), float(sigma), c, float(epps)]
+
+    def tof_fp(self, img, nmeas, tof, nthreads=0):
+        """Time-of-flight forward projection, ``tof`` = (bins, sigma, centres, epps): bins blocks of nmeas."""
+        y = np.zeros(nmeas * int(tof[0]), dtype=np.float32)
+        lib().oracle_orth_tof_fp_f32(*self._geo(nmeas), *self._tof(tof), np.ascontiguousarray(img, dtype=np.float32),
+                                     y, nthreads)
+        return y
+
+    def tof_bp(self, yin, nmeas, tof, sens=False, nthreads=0):
+        rhs = np.zeros(self.grid.N, dtype=np.float32)
+        s = np.zeros(self.grid.N, dtype=np.float32) if sens else None
+        lib().oracle_orth_tof_bp_f32(*self._geo(nmeas), *self._tof(tof), np.ascontiguousarray(yin, dtype=np.float32),
+                                     rhs, None if s is None else s.ctypes.data, nthreads)
         return (rhs, s) if sens else rhs
 
     def trace(self, nmeas):

@@ -21,16 +21,24 @@ def _pet2d(pt):
     return p
 
 
-@pytest.mark.parametrize("pt", [2, 3])
-def test_tof_with_tube_projectors_is_refused(pt):
-    """TOF data (TOF_bins == TOF_bins_used > 1) with projector types 2 and 3 raises instead of being
-    projected as non-TOF data of the wrong size."""
-    p = _pet2d(pt)
+def test_tof_with_the_orthogonal_projector_is_refused():
+    """TOF data (TOF_bins == TOF_bins_used > 1) with projector type 2 raises (as proj.py:1247-1248 does)
+    instead of being projected as non-TOF data of the wrong size."""
+    p = _pet2d(2)
     p.TOF_bins, p.TOF_bins_used, p.TOF_width, p.TOF_FWHM = 7, 7, 100e-12, 300e-12
     p.addProjector()
     assert p.nBins == 7
     with pytest.raises(ValueError, match="TOF"):
         p._descriptor()
+
+
+def test_tof_with_the_volume_projector_reaches_the_descriptor():
+    p = _pet2d(3)
+    p.TOF_bins, p.TOF_bins_used, p.TOF_width, p.TOF_FWHM = 7, 7, 100e-12, 300e-12
+    p.addProjector()
+    assert p.nBins == 7
+    d, _ = p._descriptor()
+    assert d.tof_bins == 7 and d.projector_type == 3 and d.tof_sigma > 0
 
 
 @pytest.mark.parametrize("pt", [1, 2, 3])
