@@ -135,33 +135,39 @@ __global__ __launch_bounds__(256) void k_conv3d_fwd(const uint16_t* __restrict__
 // waves on 16 x 4 x 1 rows (one row per wave), so that small volumes still occupy the chip.
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-template <int CIN, int WAVES, int NT, int TY>
+template <int CIN, int WAVES, int NT, int TY, int MW = 1>
 constexpr size_t conv_tile_lds() {
-    return (size_t)18 * (TY + 2) * (NT * WAVES / TY + 2) * CIN * sizeof(uint16_t);
+    return (size_t)18 * (TY + 2) * (NT * WAVES / MW / TY + 2) * CIN * sizeof(uint16_t);
 }
 
-template <int CIN, int MT, int WAVES, int NT, int TY>
+// MW: the block's waves split the output channels MW ways (wave w takes channel group w % MW and row group
+// w / MW), so a wave streams 1 / MW of the weights and reuses each weight fragment over its NT rows; the
+// halo in LDS is shared by all of them.
+template <int CIN, int MT, int WAVES, int NT, int TY, int MW = 1>
 __global__ __launch_bounds__(64 * WAVES) void k_conv3d_fwd_tile(const uint16_t* __restrict__ X1, int C1,
                                                                 const uint16_t* __restrict__ X2,
                                                                 const uint16_t* __restrict__ Wt,
                                                                 const float* __restrict__ bias,
                                                                 uint16_t* __restrict__ Y, Vol g, int act, int tiles_x,
                                                                 int tiles_y) {
-    constexpr int TX = 16, TZ = NT * WAVES / TY, HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HS = HX * HY * HZ;
-    constexpr int S = CIN, Cout = MT * 16, K = 27 * CIN, KC = (K + 31) / 32, NTH = 64 * WAVES, PF = 2;
-    static_assert(TZ * TY == NT * WAVES, "rows of the block");
+    constexpr int RW = WAVES / MW, TX = 16, TZ = NT * RW / TY, HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+    constexpr int HS = HX * HY * HZ;
+    constexpr int S = CIN, Cout = MT * 16 * MW, K = 27 * CIN, KC = (K + 31) / 32, NTH = 64 * WAVES, PF = 2;
+    static_assert(WAVES % MW == 0 && TZ * TY == NT * RW, "rows of the block");
     extern __shared__ uint16_t halo[];
     const int bx = blockIdx.x % tiles_x, rest = blockIdx.x / tiles_x;
     const int x0 = bx * TX, y0 = (rest % tiles_y) * TY, z0 = (rest / tiles_y) * TZ;
     const int C2 = CIN - C1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
+    const int cg = wave % MW, rw = wave / MW;
     const bf8 zero8 = {};
     auto wload = [&](int kc, int m) -> bf8 {
         const int k = kc * 32 + kg * 8;
         const bool in = kc < KC && ((K % 32 == 0) || k < K);
         const int tap = in ? k / CIN : 0, ci = k - tap * CIN;
-        return in ? *reinterpret_cast<const bf8*>(Wt + ((size_t)tap * Cout + m * 16 + col) * CIN + ci) : zero8;
+        return in ? *reinterpret_cast<const bf8*>(Wt + ((size_t)tap * Cout + (cg * MT + m) * 16 + col) * CIN + ci)
+                  : zero8;
     };
     bf8 wq[PF][MT];
 #pragma unroll
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv3d_fwd_tile(const uint16_t* 
         bf8 b[NT];
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const int row = wave * NT + t, yy = row % TY, zz = row / TY;
+            const int row = rw * NT + t, yy = row % TY, zz = row / TY;
             const int slot = ((zz + dz) * HY + (yy + dy)) * HX + col + dx;
             b[t] = kin ? *reinterpret_cast<const bf8*>(halo + slot * S + ci) : zero8;
         }
@@ -229,13 +235,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv3d_fwd_tile(const uint16_t* 
     }
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        const int row = wave * NT + t;
+        const int row = rw * NT + t;
         const int x = x0 + col, y = y0 + row % TY, z = z0 + row / TY;
         if (x >= g.W || y >= g.H || z >= g.D) continue;
         const size_t v = ((size_t)z * g.H + y) * g.W + x;
 #pragma unroll
         for (int m = 0; m < MT; m++) {
-            const int co = m * 16 + kg * 4;
+            const int co = (cg * MT + m) * 16 + kg * 4;
             uint16_t o[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -890,20 +896,22 @@ static void launch_fwd(const uint16_t* X1, int C1, const uint16_t* X2, int C2, c
                        C2, W, b, Y, g, act);
 }
 
-template <int CIN, int MT, int WAVES, int NT, int TY>
+// MT: output-channel tiles of 16 per wave; MW: channel groups across the block's waves (Cout = 16 MT MW).
+template <int CIN, int MT, int WAVES, int NT, int TY, int MW = 1>
 static int launch_fwd_tile(const uint16_t* X1, int C1, const uint16_t* X2, const uint16_t* W, const float* b,
                            uint16_t* Y, Vol g, int act, hipStream_t s) {
-    constexpr int TZ = NT * WAVES / TY;
+    constexpr int TZ = NT * WAVES / MW / TY;
     const int tx = (g.W + 15) / 16, ty = (g.H + TY - 1) / TY, tz = (g.D + TZ - 1) / TZ;
-    const size_t lds = conv_tile_lds<CIN, WAVES, NT, TY>();
+    const size_t lds = conv_tile_lds<CIN, WAVES, NT, TY, MW>();
     static bool attr = false;   // above 64 KiB of LDS the kernel must opt in once
     if (!attr) {
-        OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_conv3d_fwd_tile<CIN, MT, WAVES, NT, TY>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        OMG_CHECK_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void*>(k_conv3d_fwd_tile<CIN, MT, WAVES, NT, TY, MW>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr = true;
     }
-    hipLaunchKernelGGL((k_conv3d_fwd_tile<CIN, MT, WAVES, NT, TY>), dim3((unsigned)(tx * ty * tz)), dim3(64 * WAVES), lds,
-                       s, X1, C1, X2, W, b, Y, g, act, tx, ty);
+    hipLaunchKernelGGL((k_conv3d_fwd_tile<CIN, MT, WAVES, NT, TY, MW>), dim3((unsigned)(tx * ty * tz)),
+                       dim3(64 * WAVES), lds, s, X1, C1, X2, W, b, Y, g, act, tx, ty);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
@@ -918,8 +926,13 @@ static int launch_fwd_shape(const uint16_t* X1, int C1, const uint16_t* X2, cons
     constexpr bool big_fits = conv_tile_lds<CIN, 8, 4, 8>() <= 160 * 1024;   // LDS of one CU
     if (big_fits && bx * ((g.H + 7) / 8) * ((g.D + 3) / 4) >= 512)
         return launch_fwd_tile<CIN, MT, big_fits ? 8 : 4, 4, big_fits ? 8 : 4>(X1, C1, X2, W, b, Y, g, act, s);
-    if (bx * ((g.H + 3) / 4) * ((g.D + 3) / 4) >= 256)
+    // with several output-channel tiles the waves split them (each streams 1 / MW of the weights, and reuses
+    // each fragment over more rows); one 16-voxel row per wave would stream every weight once per row
+    if (bx * ((g.H + 3) / 4) * ((g.D + 3) / 4) >= 256) {
+        if constexpr (MT % 2 == 0) return launch_fwd_tile<CIN, MT / 2, 4, 8, 4, 2>(X1, C1, X2, W, b, Y, g, act, s);
         return launch_fwd_tile<CIN, MT, 4, 4, 4>(X1, C1, X2, W, b, Y, g, act, s);
+    }
+    if constexpr (MT % 4 == 0) return launch_fwd_tile<CIN, MT / 4, 4, 4, 4, 4>(X1, C1, X2, W, b, Y, g, act, s);
     return launch_fwd_tile<CIN, MT, 4, 1, 4>(X1, C1, X2, W, b, Y, g, act, s);
 }
 
