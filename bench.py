@@ -49,7 +49,7 @@ def parse():
                     help="summary of the rocprofv3 --pmc VALU issue counters of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
                     choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet2d_volume", "pet3d_autograd",
-                             "pet2d_tof", "unet"],
+                             "pet2d_tof", "pet2d_volume_tof", "unet"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     ap.add_argument("--voxel-bp", action="store_true",
@@ -68,6 +68,7 @@ def main():
     if args.workload != "pet2d_osem":
         {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth,
          "pet2d_volume": lambda a: bench_orth(a, volume=True),
+         "pet2d_volume_tof": lambda a: bench_orth(a, volume=True, tof=True),
          "pet3d_autograd": bench_autograd, "pet2d_tof": bench_tof, "unet": bench_unet}[args.workload](args)
         return
 
@@ -407,18 +408,22 @@ def _cpu_cgls(p, mu, budget_s):
                       f"{p.nProjections} views at 256^3, scaled to all views; OpenMP threads={threads}; {_cpu_model()}"}
 
 
-def _cpu_subset_iteration(p, meas, case, n, budget_s):
+def _cpu_subset_iteration(p, meas, case, n, budget_s, tof=None):
     """One OSEM subset-iteration (FP, ratio, BP with sensitivity, update) of an oracle case on the host cores;
-    repeated while the budget allows. Returns (subset-iterations/s, seconds per subset-iteration, repeats)."""
+    repeated while the budget allows. ``tof``: (bins, sigma, centres, epps) for a TOF orthogonal / volume case.
+    Returns (subset-iterations/s, seconds per subset-iteration, repeats)."""
     import numpy as np
     from oracle import oracle as O
     threads = _threads()
     x = np.ones(p.grid.N, np.float32)
     reps, t0 = 0, time.perf_counter()
     while True:
-        yhat = case.fp(x, n, nthreads=threads)
+        yhat = case.fp(x, n, nthreads=threads) if tof is None else case.tof_fp(x, n, tof, nthreads=threads)
         r = O.ratio(meas, yhat, 1e-5)
-        rhs, sens = case.bp(r, n, sens=True, nthreads=threads)
+        if tof is None:
+            rhs, sens = case.bp(r, n, sens=True, nthreads=threads)
+        else:
+            rhs, sens = case.tof_bp(r, n, tof, sens=True, nthreads=threads)
         x = O.osem_update(x, O.sens_clamp(sens, 1e-5), rhs, 1e-5)
         reps += 1
         el = time.perf_counter() - t0
@@ -427,11 +432,12 @@ def _cpu_subset_iteration(p, meas, case, n, budget_s):
     return reps / el, el / reps, reps
 
 
-def bench_orth(args, volume=False):
+def bench_orth(args, volume=False, tof=False):
     """2-D plane PET OSEM with the orthogonal distance-based projector (tube width 2.5 mm, transaxial
     and axial sweeps) on the pet2d_osem geometry. value = subset-iterations/s (8 subsets x 1 iteration per step).
     volume: the volume-of-intersection projector instead (projector type 3, the operator API's defaults
-    tube_radius = 1 mm and voxel_radius = 1: voxels count out to 2.77 mm from the LOR)."""
+    tube_radius = 1 mm and voxel_radius = 1: voxels count out to 2.77 mm from the LOR). tof: with time of
+    flight, 17 bins of 140 ps at 400 ps FWHM (as the pet2d_tof workload)."""
     import numpy as np
     import torch
     from omega_amd import _lib, geometry as G, workloads
@@ -453,8 +459,11 @@ def bench_orth(args, volume=False):
     p.x, p.z = sc.xy_table(), sc.z
     p.Ndist, p.Nang = sc.n_radial, sc.n_angles
     p.subsets, p.subsetType = 8, 8
+    if tof:
+        p.TOF_bins, p.TOF_bins_used, p.TOF_width, p.TOF_FWHM = 17, 17, 140e-12, 400e-12
     p.addProjector()
     p.initProj()
+    nb = p.nBins
     ph, y = workloads.pet2d_data(p, seed=0)
     runner = OSEMRunner(p, y)
     x = torch.ones(p.grid.N, device=dev)
@@ -477,7 +486,7 @@ def bench_orth(args, volume=False):
     b.record()
     b.synchronize()
     s_s = a.elapsed_time(b) * 1e-3
-    byt = 12.0 * ntube + 8.0 * M0
+    byt = 12.0 * ntube + 8.0 * M0 * nb
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -486,25 +495,32 @@ def bench_orth(args, volume=False):
         width = float(np.float32(p.tube_radius)) if volume else 2.5
         case = O.OrthCase(p.grid, p.nRowsD, p.nColsD, p.x, p.z, width, flags=3, plane_offset=int(p.nMeas[0]),
                           volume=vol)
-        rate, per, reps = _cpu_subset_iteration(p, y[:M0].cpu().numpy(), case, M0, args.cpu_seconds)
+        tofp = (nb, float(p.sigma_x), p.TOFCenter, float(p.epps)) if tof else None
+        rate, per, reps = _cpu_subset_iteration(p, y[:M0 * nb].cpu().numpy(), case, M0, args.cpu_seconds, tofp)
         cpu = {"value": round(rate, 3), "unit": "subset-iters/s", "cores": _threads(), "kind": "port",
                "sample": f"{reps} OSEM subset-iterations of subset 0 ({per:.2f} s each) with the oracle's "
-                         f"{'volume-of-intersection' if volume else 'orthogonal'} projector; OpenMP threads="
-                         f"{_threads()}; {_cpu_model()}"}
+                         f"{'volume-of-intersection' if volume else 'orthogonal'} projector"
+                         f"{' with TOF' if tof else ''}; OpenMP threads={_threads()}; {_cpu_model()}"}
+    if tof:
+        kern = "k_orth_tof_osem (one lane per LOR, tubes traced) + update"
+    elif volume:
+        kern = "k_tube_vol_planar (tube lists kept on the device) + update"
+    else:
+        kern = "k_orth_osem + update"
     _common(args, {
         "metric": "OSEM subset-iters/s, " + ("volume-of-intersection" if volume else "orthogonal distance-based")
-                  + " projector", "value": round(p.subsets * args.steps / dt, 2),
+                  + " projector" + (" with TOF" if tof else ""), "value": round(p.subsets * args.steps / dt, 2),
         "unit": "subset-iters/s", "ms_per_step": round(dt / args.steps * 1e3, 2),
         "data": "synthetic (seeded ellipsoid phantom, Poisson data at 5e6 counts)",
-        "config": {"workload": "pet2d_osem_volume" if volume else "pet2d_osem_orth", "image": [128, 128, 63],
-                   "sinogram": [128, 160, 63],
+        "config": {"workload": ("pet2d_osem_volume" + ("_tof" if tof else "")) if volume else "pet2d_osem_orth",
+                   "image": [128, 128, 63], "sinogram": [128, 160, 63],
                    **({"tube_radius_mm": 1.0, "voxel_radius": 1.0, "table_entries": int(p.V.size)} if volume
                       else {"tube_width_mm": 2.5}),
+                   **({"tof_bins": nb, "tof_fwhm_ps": 400, "tof_bin_ps": 140} if tof else {}),
                    "sweeps": "transaxial+axial", "subsets": 8, "tube_visits_subset0": ntube},
         "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("k_tube_vol_planar (tube lists kept on the device) + update" if volume
-                                else "k_orth_osem + update"), "avg_launch_us": round(s_s * 1e6, 1),
+                     "kernel": kern, "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt)},
         "cpu_baseline": cpu,
     })

@@ -583,14 +583,21 @@ constexpr int kOrthTofBlock = 64;
 // (p(D) + 2 p(D - dX s) + 2 p(D - 2 dX s) + p(D - elem s)) dX with dX = elem / 3, s = DDsign, and their sum
 // floored at epps. The walk announces every stretch; the weights are evaluated only when a tube voxel of the
 // stretch's planes needs them. Before the first stretch the weights are zero.
+//
+// Every tube voxel of a stretch's planes takes the same weights, so the bins are visited once per stretch,
+// not per voxel: the forward projection sums the stretch's (w x[v]) / sum and adds weight_b times that sum to
+// bin b when the stretch changes (the reference adds weight_b ((w x[v]) / sum) per voxel: the same terms,
+// summed in another order); the backprojection forms sum_b weight_b r_b and sum_b weight_b once per stretch.
 struct OrthTof {
     const SidArgs& a;
     float* wl;
     float inv, pf, sum, elem, D, sgn;
+    float part;      // FP: this stretch's sum of (w x[v]) / sum, not yet added to the bins
+    float s1, sr;    // BP: sum_b weight_b and sum_b weight_b r_b of the current stretch
     bool dirty;
     __device__ OrthTof(const SidArgs& args, float* w)
         : a(args), wl(w), inv(1.f / args.tof_sigma), pf(0.3989423f * (1.f / args.tof_sigma)), sum(1.f), elem(0.f),
-          D(0.f), sgn(0.f), dirty(false) {
+          D(0.f), sgn(0.f), part(0.f), s1(0.f), sr(0.f), dirty(false) {
         for (uint32_t b = 0; b < a.tof_bins; b++) wl[b * kOrthTofBlock] = 0.f;
     }
     __device__ void set(float e, float d, float s) {
@@ -603,22 +610,45 @@ struct OrthTof {
         const float q = (x - mu) * inv;
         return pf * expf(-0.5f * q * q);
     }
-    __device__ void ready() {
-        if (!dirty) return;
-        dirty = false;
+    __device__ float weight(uint32_t b) const {
         const float dX = elem / 3.f;
         const float dXs = dX * sgn;
+        const float mu = a.tof_c[b];
+        float out = pdf(D, mu);
+        out += pdf(fmaf(-dXs, 1.f, D), mu) * 2.f;
+        out += pdf(fmaf(-dXs, 2.f, D), mu) * 2.f;
+        out += pdf(fmaf(-elem, sgn, D), mu);
+        return out * dX;
+    }
+    // FP: add the pending stretch sum to the bins with the weights it was gathered under
+    __device__ void flush(float* ax) {
+        if (part != 0.f)
+            for (uint32_t b = 0; b < a.tof_bins; b++) ax[b * kOrthTofBlock] += wl[b * kOrthTofBlock] * part;
+        part = 0.f;
+    }
+    __device__ void ready_fp(float* ax) {
+        if (!dirty) return;
+        dirty = false;
+        flush(ax);
         float s = 0.f;
         for (uint32_t b = 0; b < a.tof_bins; b++) {
-            const float mu = a.tof_c[b];
-            float out = pdf(D, mu);
-            out += pdf(fmaf(-dXs, 1.f, D), mu) * 2.f;
-            out += pdf(fmaf(-dXs, 2.f, D), mu) * 2.f;
-            out += pdf(fmaf(-elem, sgn, D), mu);
-            const float v = out * dX;
+            const float v = weight(b);
             wl[b * kOrthTofBlock] = v;
             s += v;
         }
+        sum = s < a.tof_epps ? a.tof_epps : s;
+    }
+    __device__ void ready_bp(const float* r) {
+        if (!dirty) return;
+        dirty = false;
+        float s = 0.f, q = 0.f;
+        for (uint32_t b = 0; b < a.tof_bins; b++) {
+            const float v = weight(b);
+            s += v;
+            q += v * r[b * kOrthTofBlock];
+        }
+        s1 = s;
+        sr = q;
         sum = s < a.tof_epps ? a.tof_epps : s;
     }
 };
@@ -628,11 +658,12 @@ template <bool VOL>
 __device__ __forceinline__ float orth_tof_gather(const OrthArgs& oa, int64_t lo, const float* __restrict__ img,
                                                  float* ax, OrthTof& t) {
     for (uint32_t b = 0; b < oa.a.tof_bins; b++) ax[b * kOrthTofBlock] = 0.f;
-    return orth_lor_t<VOL, true>(oa, lo, [&](int64_t ind, float w) {
-        t.ready();
-        const float dX = (w * img[ind]) / t.sum;
-        for (uint32_t b = 0; b < oa.a.tof_bins; b++) ax[b * kOrthTofBlock] += t.wl[b * kOrthTofBlock] * dX;
+    const float temp = orth_lor_t<VOL, true>(oa, lo, [&](int64_t ind, float w) {
+        t.ready_fp(ax);
+        t.part += (w * img[ind]) / t.sum;
     }, nullptr, [&](float e, float D, float sg) { t.set(e, D, sg); });
+    t.flush(ax);
+    return temp;
 }
 
 // BP (rhs, :1210-1245): rhs[v] += sum_b ((w temp) / sum) weight_b r_b; sens[v] += sum_b ((w temp) / sum) weight_b.
@@ -641,16 +672,10 @@ __device__ __forceinline__ void orth_tof_scatter(const OrthArgs& oa, int64_t lo,
                                                  float* __restrict__ rhs, float* __restrict__ sens) {
     float temp = 0.f;   // the LOR weight, set before the first visit
     orth_lor_t<VOL, true>(oa, lo, [&](int64_t ind, float w) {
-        t.ready();
+        t.ready_bp(r);
         const float dX = (w * temp) / t.sum;
-        float val = 0.f, yax = 0.f;
-        for (uint32_t b = 0; b < oa.a.tof_bins; b++) {
-            const float apu = dX * t.wl[b * kOrthTofBlock];
-            val += apu;
-            yax += apu * r[b * kOrthTofBlock];
-        }
-        unsafeAtomicAdd(rhs + ind, yax);
-        if (sens) unsafeAtomicAdd(sens + ind, val);
+        unsafeAtomicAdd(rhs + ind, dX * t.sr);
+        if (sens) unsafeAtomicAdd(sens + ind, dX * t.s1);
     }, &temp, [&](float e, float D, float sg) { t.set(e, D, sg); });
 }
 
