@@ -99,9 +99,9 @@ typedef struct omg_geom_desc {
      * in the order of `norm`; it multiplies the LOR weight after norm. PET data only. */
     const float* scat; int64_t scat_len;
     /* time of flight (projectorClass.TOF_bins, TOF_FWHM, TOF_width, TOFCenter; paramStruct TOF, nBins, sigma_x,
-     * TOFCenters, projector_functions.h:306-313,559-657; for projector 3 the GPU kernel's TOFLoop,
-     * projectorType123.cl -DTOF -DVOL). tof_bins > 1 turns it on (at most 64; Siddon and volume of
-     * intersection, PET data): every measurement array of a call then holds tof_bins blocks of n_meas
+     * TOFCenters, projector_functions.h:306-313,559-657; for projectors 3 and 4 the GPU kernels' TOFLoop,
+     * projectorType123.cl -DTOF -DVOL, projectorType4.cl -DTOF). tof_bins > 1 turns it on (at most 64;
+     * projectors 1, 3 and 4, PET data): every measurement array of a call then holds tof_bins blocks of n_meas
      * values, bin b of measurement i at i + b * n_meas (projector_functions.h:2030-2035); the additive term of the ratio
      * stays one value per measurement, added to every bin. tof_sigma: the TOF standard deviation (mm);
      * tof_centers: the bins' centre offsets along the LOR (mm, tof_bins floats, copied at create);

@@ -253,10 +253,10 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     }
     if (d->tof_bins > 1) {
         OMG_REQUIRE(d->tof_bins <= kTofMax, "tof_bins %u exceeds %u", d->tof_bins, kTofMax);
-        OMG_REQUIRE((d->projector_type == OMG_PROJ_SIDDON || d->projector_type == OMG_PROJ_VOLUME) &&
-                        d->mode != OMG_MODE_CT_FLAT,
-                    "time of flight is provided for the improved Siddon (projector_type 1) and volume-of-intersection "
-                    "(projector_type 3) projectors on PET data");
+        OMG_REQUIRE((d->projector_type == OMG_PROJ_SIDDON || d->projector_type == OMG_PROJ_VOLUME ||
+                     d->projector_type == OMG_PROJ_INTERP) && d->mode != OMG_MODE_CT_FLAT,
+                    "time of flight is provided for the improved Siddon (projector_type 1), volume-of-intersection "
+                    "(projector_type 3) and interpolation (projector_type 4) projectors on PET data");
         OMG_REQUIRE(d->tof_sigma > 0.f && std::isfinite(d->tof_sigma), "tof_sigma must be a positive standard deviation");
         OMG_REQUIRE(d->tof_centers != nullptr, "tof_bins %u without tof_centers", d->tof_bins);
         for (uint32_t b = 0; b < d->tof_bins; b++)

@@ -199,6 +199,132 @@ __global__ __launch_bounds__(kInterpBlock) void k_interp_bp_rays(InterpArgs ia, 
     });
 }
 
+// ---- Time of flight on PET data (projectorType4.cl -DTOF) ----
+// One lane per LOR, one wave per workgroup; a lane's bins (accumulators or measurements) and the current
+// sample's bin weights live in LDS, laid out [bin][lane]. D starts at |v tStart| - L / 2 (TOFDis, :419-423),
+// moved to the first Joseph sample by (t0 - tStart) L DDsign (:463-466); each sample takes the weights of
+// its own stretch of length stepLen (TOFLoop with the weight-sum floor 1e-6, :566-568), D then moves by
+// stepLen DDsign (:600-602).
+constexpr int kInterpTofBlock = 64;
+
+__device__ __forceinline__ float cl_sign4(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : (v == v ? v : 0.f)); }
+
+struct InterpTof {
+    const SidArgs& a;
+    float* wl;
+    float inv, pf;
+    __device__ InterpTof(const SidArgs& args, float* w)
+        : a(args), wl(w), inv(1.f / args.tof_sigma), pf(0.3989423f * (1.f / args.tof_sigma)) {}
+    __device__ float pdf(float x, float mu) const {
+        const float q = (x - mu) * inv;
+        return pf * expf(-0.5f * q * q);
+    }
+    // TOFLoop / TOFWeight (general_opencl_functions.h:1074-1116, -DUSEMAD, TRAPZ_BINS 4) into wl; the sum
+    // floored at 1e-6
+    __device__ float loop(float sgn, float elem, float D) const {
+        const float dX = elem / 3.f;
+        const float dXs = dX * sgn;
+        float s = 0.f;
+        for (uint32_t b = 0; b < a.tof_bins; b++) {
+            const float mu = a.tof_c[b];
+            float out = pdf(D, mu);
+            out += pdf(fmaf(-dXs, 1.f, D), mu) * 2.f;
+            out += pdf(fmaf(-dXs, 2.f, D), mu) * 2.f;
+            out += pdf(fmaf(-elem, sgn, D), mu);
+            const float v = out * dX;
+            wl[b * kInterpTofBlock] = v;
+            s += v;
+        }
+        return s < 1e-6f ? 1e-6f : s;
+    }
+};
+
+__device__ __forceinline__ float interp_tof_start(const InterpArgs& ia, const Samples& S, float& sgn) {
+    const float ax = S.v.x * S.tStart, ay = S.v.y * S.tStart, az = S.v.z * S.tStart;
+    float D = sqrtf(ax * ax + ay * ay + az * az) - S.L / 2.f;
+    sgn = cl_sign4(D);
+    if (ia.dL <= 0.f) D -= ((S.t0 - S.tStart) * S.L * sgn);
+    return D;
+}
+
+// FP (denominator): ax_b += weight_b ((stepLen x(p)) / sum); y_b = ax_b temp.
+__global__ __launch_bounds__(kInterpTofBlock) void k_interp_tof_fp(InterpArgs ia, const float* __restrict__ img,
+                                                                   float* __restrict__ y) {
+    extern __shared__ float lds[];
+    const int64_t lo = (int64_t)blockIdx.x * kInterpTofBlock + threadIdx.x;
+    if (lo >= ia.a.nmeas) return;
+    const uint32_t nb = ia.a.tof_bins;
+    Samples S;
+    if (!interp_samples(ia, lo, S)) {
+        for (uint32_t b = 0; b < nb; b++) y[lo + (int64_t)b * ia.a.nmeas] = 0.f;
+        return;
+    }
+    float* ax = lds + threadIdx.x;
+    const InterpTof t(ia.a, lds + nb * kInterpTofBlock + threadIdx.x);
+    for (uint32_t b = 0; b < nb; b++) ax[b * kInterpTofBlock] = 0.f;
+    float sgn;
+    float D = interp_tof_start(ia, S, sgn);
+    const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
+    interp_walk(ia, S, S.t0, S.n, [&](IF3 p) {
+        const float sum = t.loop(sgn, S.stepLen, D);
+        const float dX = (S.stepLen * tex3(img, Nx, Ny, Nz, p.x, p.y, p.z)) / sum;
+        for (uint32_t b = 0; b < nb; b++) ax[b * kInterpTofBlock] += t.wl[b * kInterpTofBlock] * dX;
+        D -= S.stepLen * sgn;
+    });
+    const float temp = interp_weight(ia, lo, S.L, true);
+    for (uint32_t b = 0; b < nb; b++) y[lo + (int64_t)b * ia.a.nmeas] = ax[b * kInterpTofBlock] * temp;
+}
+
+// BP (rhs): the sample's voxel gets sum_b ((stepLen temp) / sum) weight_b y_b (and the factors' sum in
+// sens); a sample whose voxel is skipped (index 0 or past the image) leaves D where it was, as the kernel's
+// `continue` there skips the update (:570-576).
+__global__ __launch_bounds__(kInterpTofBlock) void k_interp_tof_bp(InterpArgs ia, const float* __restrict__ yin,
+                                                                   float* __restrict__ rhs, float* __restrict__ sens) {
+    extern __shared__ float lds[];
+    const int64_t lo = (int64_t)blockIdx.x * kInterpTofBlock + threadIdx.x;
+    if (lo >= ia.a.nmeas) return;
+    const uint32_t nb = ia.a.tof_bins;
+    float* yv = lds + threadIdx.x;
+    bool any = false;
+    for (uint32_t b = 0; b < nb; b++) {
+        const float v = yin[lo + (int64_t)b * ia.a.nmeas];
+        yv[b * kInterpTofBlock] = v;
+        any |= v != 0.f;
+    }
+    if (!any && sens == nullptr) return;
+    Samples S;
+    if (!interp_samples(ia, lo, S)) return;
+    float t0 = S.t0;
+    uint32_t n = S.n;
+    if (ia.a.atten && ia.atn_image && ia.dL > 0.f) {
+        t0 = S.tStart + S.tStep / 10.f;
+        n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
+    }
+    const InterpTof t(ia.a, lds + nb * kInterpTofBlock + threadIdx.x);
+    float sgn;
+    float D = interp_tof_start(ia, S, sgn);
+    const float e = S.stepLen * interp_weight(ia, lo, S.L, false);
+    const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
+    const uint64_t N = (uint64_t)Nx * Ny * Nz;
+    interp_walk(ia, S, t0, n, [&](IF3 p) {
+        const float sum = t.loop(sgn, S.stepLen, D);
+        const uint64_t ind = (uint64_t)(uint32_t)fmaxf(p.x * (float)Nx, 0.f) +
+                             (uint64_t)(uint32_t)fmaxf(p.y * (float)Ny, 0.f) * Nx +
+                             (uint64_t)(uint32_t)fmaxf(p.z * (float)Nz, 0.f) * Nx * Ny;
+        if (ind == 0 || ind >= N) return;
+        const float dX = e / sum;
+        float val = 0.f, yax = 0.f;
+        for (uint32_t b = 0; b < nb; b++) {
+            const float apu = dX * t.wl[b * kInterpTofBlock];
+            val += apu;
+            yax += apu * yv[b * kInterpTofBlock];
+        }
+        unsafeAtomicAdd(rhs + ind, yax);
+        if (sens) unsafeAtomicAdd(sens + ind, val);
+        D -= S.stepLen * sgn;
+    });
+}
+
 // CT backprojection, voxel-driven: one lane per (x, y, z group of kInterpNV voxels), projections
 // [plane_offset, plane_offset + n_proj) of the panel stack yin (n_proj panels of size_x x size_y).
 __global__ __launch_bounds__(kInterpBlock) void k_interp_bp_ct(InterpArgs ia, uint32_t n_proj,
@@ -281,12 +407,14 @@ __global__ __launch_bounds__(kInterpBlock) void k_interp_bp_ct(InterpArgs ia, ui
 }
 
 // r = meas / ((yhat + epps) + additive), NaN/Inf -> epps, in place on yhat.
+// n values (TOF: bins x n_lor); the additive term is one value per LOR, added to every bin.
 __global__ __launch_bounds__(256) void k_interp_ratio(const float* __restrict__ meas, float* __restrict__ q,
-                                                      const float* __restrict__ add, int64_t n, float epps) {
+                                                      const float* __restrict__ add, int64_t n, int64_t n_lor,
+                                                      float epps) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     float d = q[i] + epps;
-    if (add) d += add[i];
+    if (add) d += add[i % n_lor];
     float r = meas[i] / d;
     if (isnan(r) || isinf(r)) r = epps;
     q[i] = r;
@@ -311,11 +439,17 @@ static InterpArgs make_interp_args(const omg_geom* g, int64_t first, int64_t n) 
     return ia;
 }
 
+static size_t interp_tof_lds(const omg_geom* g) { return 2 * (size_t)g->d.tof_bins * kInterpTofBlock * sizeof(float); }
+
 int interp_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
     const InterpArgs ia = make_interp_args(g, first, n);
     hipEvent_t stop;
     timing_begin(s, &stop);
-    hipLaunchKernelGGL(k_interp_fp, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, img, y);
+    if (g->d.tof_bins > 1)
+        hipLaunchKernelGGL(k_interp_tof_fp, dim3(grid_for(n, kInterpTofBlock)), dim3(kInterpTofBlock),
+                           interp_tof_lds(g), s, ia, img, y);
+    else
+        hipLaunchKernelGGL(k_interp_fp, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, img, y);
     timing_end(s, stop);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
@@ -332,6 +466,9 @@ int interp_backward(const omg_geom* g, int64_t first, int64_t n, const float* yi
         const uint64_t lanes = (uint64_t)g->d.Nx * g->d.Ny * ((g->d.Nz + kInterpNV - 1) / kInterpNV);
         hipLaunchKernelGGL(k_interp_bp_ct, dim3((unsigned)((lanes + kInterpBlock - 1) / kInterpBlock)),
                            dim3(kInterpBlock), 0, s, ia, nproj, yin, rhs, sens);
+    } else if (g->d.tof_bins > 1) {
+        hipLaunchKernelGGL(k_interp_tof_bp, dim3(grid_for(n, kInterpTofBlock)), dim3(kInterpTofBlock),
+                           interp_tof_lds(g), s, ia, yin, rhs, sens);
     } else {
         hipLaunchKernelGGL(k_interp_bp_rays, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, yin, rhs,
                            sens);
@@ -344,12 +481,13 @@ int interp_backward(const omg_geom* g, int64_t first, int64_t n, const float* yi
 // FP -> ratio -> BP into rhs (and sens) for emission data, through a per-measurement work buffer.
 int interp_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, const float* meas, float* rhs,
                 float* sens, float epps, hipStream_t s, const float* add) {
-    int rc = q_reserve(const_cast<omg_geom*>(g), (size_t)n);
+    const int64_t nq = n * (g->d.tof_bins > 1 ? (int64_t)g->d.tof_bins : 1);
+    int rc = q_reserve(const_cast<omg_geom*>(g), (size_t)nq);
     if (rc) return rc;
     float* q = q_ptr(g);
     rc = interp_forward(g, first, n, img, q, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_interp_ratio, dim3(grid_for(n, 256)), dim3(256), 0, s, meas, q, add, n, epps);
+    hipLaunchKernelGGL(k_interp_ratio, dim3(grid_for(nq, 256)), dim3(256), 0, s, meas, q, add, nq, n, epps);
     OMG_CHECK_LAUNCH();
     return interp_backward(g, first, n, q, rhs, sens, s);
 }

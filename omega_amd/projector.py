@@ -445,9 +445,10 @@ class projectorClass:
                 raise ValueError("Several rays per LOR are supported with improved Siddon (projector_type = 1)")
             d.rays_xy, d.rays_z = nr
             d.cr_xy, d.cr_z = float(self.cr_pz), float(self.cr_p)
-        if self._tof() and pt not in (1, 3):
-            # proj.py:1247-1248 admits 1, 3 and 4; projector 4 takes no TOF here
-            raise ValueError("TOF is supported with improved Siddon (projector_type = 1) and volume of intersection "
+        if self._tof() and pt not in (1, 3, 4):
+            # proj.py:1247-1248
+            raise ValueError("TOF is currently only supported with improved Siddon (projector_type = 1), "
+                             "interpolation-based projector (projector_type = 4) and volume of intersection "
                              "(projector_type = 3)")
         if self._tof():
             sig, tc = self._tof_tables()

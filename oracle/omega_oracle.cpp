@@ -1939,6 +1939,88 @@ static void bp_rays(const IGeo& G, int64_t nMeas, const float* x, const float* z
     }
 }
 
+// Time of flight (projectorType4.cl -DTOF, :402-425, :463-466, :566-600): D starts at |v tStart| - L / 2
+// (TOFDis), moved to the first Joseph sample by (t0 - tStart) L DDsign; every sample takes the bin weights of
+// its own stretch of length stepLen (TOFLoop with the weight-sum floor 1e-6) and D then moves by
+// stepLen DDsign. FP: ax_b += weight_b ((stepLen x(p)) / sum), y_b = ax_b temp. BP: the sample's voxel gets
+// sum_b ((stepLen temp) / sum) weight_b y_b (and the factors' sum in sens); a sample whose voxel is skipped
+// (index 0 or past the image) leaves D where it was, as the kernel's `continue` there skips the update.
+static float tof_start(const Samples& S, float& DDsign) {
+    const float ax = S.v.x * S.tStart, ay = S.v.y * S.tStart, az = S.v.z * S.tStart;
+    float D = std::sqrt(ax * ax + ay * ay + az * az) - S.L / 2.f;
+    DDsign = orth::cl_sign(D);
+    return D;
+}
+
+static void fp_tof(const IGeo& G, int64_t nMeas, const float* x, const float* z, const orth::TofCL& tc,
+                   const float* img, float* y, int nthreads) {
+    oracle::set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        Samples S;
+        if (!samples(G, lo, x, z, S)) {
+            for (uint32_t b = 0; b < tc.bins; b++) y[lo + (int64_t)b * nMeas] = 0.f;
+            continue;
+        }
+        float sgn;
+        float D = tof_start(S, sgn);
+        if (G.dL <= 0.f) D -= ((S.t0 - S.tStart) * S.L * sgn);
+        std::vector<float> ax(tc.bins, 0.f), wt(tc.bins);
+        walk(G, S, S.t0, S.n, [&](F3 p) {
+            const float sum = tc.loop(sgn, S.stepLen, D, wt.data());
+            const float dX = (S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z)) / sum;
+            for (uint32_t b = 0; b < tc.bins; b++) ax[b] += wt[b] * dX;
+            D -= S.stepLen * sgn;
+        });
+        const float temp = lor_weight(G, lo, S.L, true);
+        for (uint32_t b = 0; b < tc.bins; b++) y[lo + (int64_t)b * nMeas] = ax[b] * temp;
+    }
+}
+
+static void bp_rays_tof(const IGeo& G, int64_t nMeas, const float* x, const float* z, const orth::TofCL& tc,
+                        const float* yin, float* rhs, float* sens, int nthreads) {
+    const uint32_t Nx = G.g.Nx, Ny = G.g.Ny, Nz = G.g.Nz;
+    const uint64_t N = (uint64_t)Nx * Ny * Nz;
+    oracle::set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t lo = 0; lo < nMeas; lo++) {
+        Samples S;
+        if (!samples(G, lo, x, z, S)) continue;
+        float t0 = S.t0;
+        uint32_t n = S.n;
+        if (G.g.atten && G.atn_image && G.dL > 0.f) {
+            t0 = S.tStart + S.tStep / 10.f;
+            n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
+        }
+        float sgn;
+        float D = tof_start(S, sgn);
+        if (G.dL <= 0.f) D -= ((S.t0 - S.tStart) * S.L * sgn);
+        const float e = S.stepLen * lor_weight(G, lo, S.L, false);
+        std::vector<float> wt(tc.bins);
+        walk(G, S, t0, n, [&](F3 p) {
+            const float sum = tc.loop(sgn, S.stepLen, D, wt.data());
+            const uint64_t ind = (uint64_t)(uint32_t)std::fmax(p.x * (float)Nx, 0.f) +
+                                 (uint64_t)(uint32_t)std::fmax(p.y * (float)Ny, 0.f) * Nx +
+                                 (uint64_t)(uint32_t)std::fmax(p.z * (float)Nz, 0.f) * Nx * Ny;
+            if (ind == 0 || ind >= N) return;
+            const float dX = e / sum;
+            float val = 0.f, yax = 0.f;
+            for (uint32_t b = 0; b < tc.bins; b++) {
+                const float apu = dX * wt[b];
+                val += apu;
+                yax += apu * yin[lo + (int64_t)b * nMeas];
+            }
+#pragma omp atomic
+            rhs[ind] += yax;
+            if (sens) {
+#pragma omp atomic
+                sens[ind] += val;
+            }
+            D -= S.stepLen * sgn;
+        });
+    }
+}
+
 // Bilinear sample of projection k's panel (size_x rows fastest, size_y columns) at normalised (px, py), with
 // the 3-D texture's third coordinate pz interpolating between projections (clamp to edge).
 static inline float tex_panel(const float* y, uint32_t sx, uint32_t sy, uint32_t np, float px, float py, float pz) {
@@ -2062,6 +2144,21 @@ extern "C" int oracle_interp_bp_f32(ORACLE_INTERP_ARGS, const float* yin, float*
         interp::bp_ct(G, nMeas / ((int64_t)size_x * size_y), x, z, yin, rhs, sens, nthreads);
     else
         interp::bp_rays(G, nMeas, x, z, yin, rhs, sens, nthreads);
+    return 0;
+}
+
+// Time of flight on PET data: measurement arrays hold tof_bins blocks of nMeas (weight-sum floor 1e-6).
+extern "C" int oracle_interp_tof_fp_f32(ORACLE_INTERP_ARGS, uint32_t tof_bins, float tof_sigma, const float* tof_c,
+                                        const float* img, float* y, int nthreads) {
+    const orth::TofCL tc(tof_bins, tof_c, tof_sigma, 1e-6f);
+    interp::fp_tof(ORACLE_INTERP_GEO, nMeas, x, z, tc, img, y, nthreads);
+    return 0;
+}
+
+extern "C" int oracle_interp_tof_bp_f32(ORACLE_INTERP_ARGS, uint32_t tof_bins, float tof_sigma, const float* tof_c,
+                                        const float* yin, float* rhs, float* sens, int nthreads) {
+    const orth::TofCL tc(tof_bins, tof_c, tof_sigma, 1e-6f);
+    interp::bp_rays_tof(ORACLE_INTERP_GEO, nMeas, x, z, tc, yin, rhs, sens, nthreads);
     return 0;
 }
 

@@ -81,6 +81,9 @@ def lib():
                 C.c_float, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, _f32p, _f32p, C.c_float, C.c_float,
                 C.c_float, C.c_float, C.c_int64, _f32p, _f32p]
         _lib.oracle_interp_fp_f32.argtypes = igeo + [_f32p, _f32p, C.c_int]
+        _lib.oracle_interp_tof_fp_f32.argtypes = igeo + [C.c_uint32, C.c_float, _f32p, _f32p, _f32p, C.c_int]
+        _lib.oracle_interp_tof_bp_f32.argtypes = igeo + [C.c_uint32, C.c_float, _f32p, _f32p, _f32p, C.c_void_p,
+                                                         C.c_int]
         _lib.oracle_interp_bp_f32.argtypes = igeo + [_f32p, _f32p, C.c_void_p, C.c_int]
         _lib.oracle_interp_atten_f32.argtypes = igeo + [_f32p, _f32p, C.c_int]
         _lib.oracle_ratio_f32.argtypes = [_f32p, _f32p, C.c_int64, C.c_float]
@@ -572,6 +575,25 @@ class InterpCase:
         out = np.ones(nmeas, dtype=np.float32)
         lib().oracle_interp_atten_f32(*self._geo(nmeas), np.ascontiguousarray(mu, dtype=np.float32), out, nthreads)
         return out
+
+    def tof_fp(self, img, nmeas, tof, nthreads=0):
+        """Time-of-flight forward projection (PET), ``tof`` = (bins, sigma, centres): bins blocks of nmeas."""
+        bins, sigma, centres = tof[:3]
+        y = np.zeros(nmeas * int(bins), dtype=np.float32)
+        lib().oracle_interp_tof_fp_f32(*self._geo(nmeas), int(bins), float(sigma),
+                                       np.ascontiguousarray(centres, dtype=np.float32),
+                                       np.ascontiguousarray(img, dtype=np.float32), y, nthreads)
+        return y
+
+    def tof_bp(self, yin, nmeas, tof, sens=False, nthreads=0):
+        bins, sigma, centres = tof[:3]
+        rhs = np.zeros(self.grid.N, dtype=np.float32)
+        s = np.zeros(self.grid.N, dtype=np.float32) if sens else None
+        lib().oracle_interp_tof_bp_f32(*self._geo(nmeas), int(bins), float(sigma),
+                                       np.ascontiguousarray(centres, dtype=np.float32),
+                                       np.ascontiguousarray(yin, dtype=np.float32), rhs,
+                                       None if s is None else s.ctypes.data, nthreads)
+        return (rhs, s) if sens else rhs
 
     def bp(self, yin, nmeas, sens=False, nthreads=0):
         rhs = np.zeros(self.grid.N, dtype=np.float32)
