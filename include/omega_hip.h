@@ -111,7 +111,8 @@ typedef struct omg_geom_desc {
      * init.py:357-360; paramStruct nRays2D, nRays3D, projector_functions.h:262-264,1444-1466): rays_xy x
      * rays_z rays (0 or 1: one), their endpoints spread over cr_xy (transaxially, on x and y alike) and
      * cr_z (axially) (paramStruct dPitchXY, dPitchZ; the operator API passes dPitchX and dPitchY); each
-     * ray weighs 1 / (L nRays), the forward projection sums them (per TOF bin with TOF). Siddon, PET data.
+     * ray weighs 1 / (L nRays), the forward projection sums them (per TOF bin with TOF). Siddon and
+     * interpolation (projectorType4.cl -DN_RAYS), PET data.
      * With an attenuation image each ray has its own factor, exp of its own walk's sum (the library keeps
      * rays x measurements factors). */
     uint32_t rays_xy, rays_z; float cr_xy, cr_z;

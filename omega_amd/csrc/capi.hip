@@ -266,8 +266,10 @@ extern "C" int omg_geom_create(const omg_geom_desc* d, omg_geom** out) {
     const uint64_t rays = (uint64_t)std::max(d->rays_xy, 1u) * std::max(d->rays_z, 1u);
     if (rays > 1) {
         OMG_REQUIRE(rays <= 256, "%llu rays per LOR exceeds 256", (unsigned long long)rays);
-        OMG_REQUIRE(d->projector_type == OMG_PROJ_SIDDON && d->mode != OMG_MODE_CT_FLAT,
-                    "several rays per LOR are provided for the improved Siddon projector on PET data");
+        OMG_REQUIRE((d->projector_type == OMG_PROJ_SIDDON || d->projector_type == OMG_PROJ_INTERP) &&
+                        d->mode != OMG_MODE_CT_FLAT,
+                    "several rays per LOR are provided for the improved Siddon (projector_type 1) and interpolation "
+                    "(projector_type 4) projectors on PET data");
         OMG_REQUIRE(std::isfinite(d->cr_xy) && std::isfinite(d->cr_z), "cr_xy and cr_z must be finite");
     }
     OMG_REQUIRE(d->atten_mode >= 0 && d->atten_mode <= 2, "atten_mode %d is not 0 (off), 1 (image) or 2 (per LOR)",

@@ -76,9 +76,9 @@ __device__ __forceinline__ float tex3(const float* __restrict__ img, uint32_t Nx
 
 struct Samples { IF3 s, v; float t0, tStep, stepLen, L, tStart, tEnd; uint32_t n; };
 
-__device__ __forceinline__ bool interp_samples(const InterpArgs& ia, int64_t lo, Samples& S) {
+__device__ __forceinline__ bool interp_samples(const InterpArgs& ia, int64_t lo, Samples& S, int ray = 0) {
     Ray r;
-    fetch_ray(ia.a, lo, r);
+    fetch_ray(ia.a, lo, r, ray);
     S.s = IF3{r.xs, r.ys, r.zs};
     S.v = IF3{r.xd - r.xs, r.yd - r.ys, r.zd - r.zs};
     const IF3 tBack{(ia.b.x - S.s.x) / S.v.x, (ia.b.y - S.s.y) / S.v.y, (ia.b.z - S.s.z) / S.v.z};
@@ -132,17 +132,20 @@ __device__ __forceinline__ void interp_walk(const InterpArgs& ia, const Samples&
     }
 }
 
-// temp of a PET measurement: 1 / L, the corrections, the image attenuation factor last (FP) or first (BP).
-__device__ __forceinline__ float interp_weight(const InterpArgs& ia, int64_t lo, float L, bool fp) {
+// temp of a PET measurement: 1 / (L nRays) (projectorType4.cl:612-617), the corrections, the image attenuation
+// factor of this ray last (FP) or first (BP).
+__device__ __forceinline__ float interp_weight(const InterpArgs& ia, int64_t lo, float L, bool fp, int ray = 0) {
     const SidArgs& g = ia.a;
-    float temp = 1.f / L;
+    const uint32_t nr = n_rays(g);
+    float temp = nr > 1 ? 1.f / (L * (float)nr) : 1.f / L;
     const bool atn = g.atten && ia.atn_image;
-    if (atn && !fp) temp *= g.atten[lo];
+    const float fa = atn ? g.atten[(int64_t)ray * g.atten_ray_stride + lo] : 0.f;
+    if (atn && !fp) temp *= fa;
     if (g.norm) temp *= g.norm[lo];
     if (g.scat) temp *= g.scat[lo];
     if (g.atten && !ia.atn_image) temp *= g.atten[lo];
     temp *= g.global_factor;
-    if (atn && fp) temp *= g.atten[lo];
+    if (atn && fp) temp *= fa;
     return temp;
 }
 
@@ -150,12 +153,25 @@ __global__ __launch_bounds__(kInterpBlock) void k_interp_fp(InterpArgs ia, const
                                                             float* __restrict__ y) {
     const int64_t lo = (int64_t)blockIdx.x * kInterpBlock + threadIdx.x;
     if (lo >= ia.a.nmeas) return;
-    Samples S;
-    if (!interp_samples(ia, lo, S)) { y[lo] = 0.f; return; }
-    float ax = 0.f;
     const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
-    interp_walk(ia, S, S.t0, S.n, [&](IF3 p) { ax += S.stepLen * tex3(img, Nx, Ny, Nz, p.x, p.y, p.z); });
-    y[lo] = ia.a.mode == MODE_CT_FLAT ? ax : ax * interp_weight(ia, lo, S.L, true);
+    if (ia.a.mode == MODE_CT_FLAT) {
+        Samples S;
+        if (!interp_samples(ia, lo, S)) { y[lo] = 0.f; return; }
+        float ax = 0.f;
+        interp_walk(ia, S, S.t0, S.n, [&](IF3 p) { ax += S.stepLen * tex3(img, Nx, Ny, Nz, p.x, p.y, p.z); });
+        y[lo] = ax;
+        return;
+    }
+    // PET: every ray's samples times its own weight, added in ray order (a ray missing the image adds nothing)
+    float tot = 0.f;
+    for (uint32_t ray = 0; ray < n_rays(ia.a); ray++) {
+        Samples S;
+        if (!interp_samples(ia, lo, S, (int)ray)) continue;
+        float ax = 0.f;
+        interp_walk(ia, S, S.t0, S.n, [&](IF3 p) { ax += S.stepLen * tex3(img, Nx, Ny, Nz, p.x, p.y, p.z); });
+        tot += ax * interp_weight(ia, lo, S.L, true, (int)ray);
+    }
+    y[lo] = tot;
 }
 
 // exp(-sum stepLen mu(p)) over the forward projection's samples; 1 where the ray misses the image.
@@ -163,8 +179,10 @@ __global__ __launch_bounds__(kInterpBlock) void k_interp_atten(InterpArgs ia, co
                                                                float* __restrict__ f) {
     const int64_t lo = (int64_t)blockIdx.x * kInterpBlock + threadIdx.x;
     if (lo >= ia.a.nmeas) return;
+    const int ray = (int)blockIdx.y;   // several rays: ray r's factors at f[r * nmeas + lo]
+    f += (int64_t)ray * ia.a.nmeas;
     Samples S;
-    if (!interp_samples(ia, lo, S)) { f[lo] = 1.f; return; }
+    if (!interp_samples(ia, lo, S, ray)) { f[lo] = 1.f; return; }
     float jel = 0.f;
     const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
     interp_walk(ia, S, S.t0, S.n, [&](IF3 p) { jel += S.stepLen * -tex3(mu, Nx, Ny, Nz, p.x, p.y, p.z); });
@@ -178,25 +196,27 @@ __global__ __launch_bounds__(kInterpBlock) void k_interp_bp_rays(InterpArgs ia, 
     if (lo >= ia.a.nmeas) return;
     const float yv = yin[lo];
     if (yv == 0.f && sens == nullptr) return;
-    Samples S;
-    if (!interp_samples(ia, lo, S)) return;
-    float t0 = S.t0;
-    uint32_t n = S.n;
-    if (ia.a.atten && ia.atn_image && ia.dL > 0.f) {
-        t0 = S.tStart + S.tStep / 10.f;
-        n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
-    }
-    const float e = S.stepLen * interp_weight(ia, lo, S.L, false);
     const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
     const uint64_t N = (uint64_t)Nx * Ny * Nz;
-    interp_walk(ia, S, t0, n, [&](IF3 p) {
-        const uint64_t ind = (uint64_t)(uint32_t)fmaxf(p.x * (float)Nx, 0.f) +
-                             (uint64_t)(uint32_t)fmaxf(p.y * (float)Ny, 0.f) * Nx +
-                             (uint64_t)(uint32_t)fmaxf(p.z * (float)Nz, 0.f) * Nx * Ny;
-        if (ind == 0 || ind >= N) return;
-        unsafeAtomicAdd(rhs + ind, yv * e);
-        if (sens) unsafeAtomicAdd(sens + ind, e);
-    });
+    for (uint32_t ray = 0; ray < n_rays(ia.a); ray++) {   // each ray with its own weight
+        Samples S;
+        if (!interp_samples(ia, lo, S, (int)ray)) continue;
+        float t0 = S.t0;
+        uint32_t n = S.n;
+        if (ia.a.atten && ia.atn_image && ia.dL > 0.f) {
+            t0 = S.tStart + S.tStep / 10.f;
+            n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
+        }
+        const float e = S.stepLen * interp_weight(ia, lo, S.L, false, (int)ray);
+        interp_walk(ia, S, t0, n, [&](IF3 p) {
+            const uint64_t ind = (uint64_t)(uint32_t)fmaxf(p.x * (float)Nx, 0.f) +
+                                 (uint64_t)(uint32_t)fmaxf(p.y * (float)Ny, 0.f) * Nx +
+                                 (uint64_t)(uint32_t)fmaxf(p.z * (float)Nz, 0.f) * Nx * Ny;
+            if (ind == 0 || ind >= N) return;
+            unsafeAtomicAdd(rhs + ind, yv * e);
+            if (sens) unsafeAtomicAdd(sens + ind, e);
+        });
+    }
 }
 
 // ---- Time of flight on PET data (projectorType4.cl -DTOF) ----
@@ -254,25 +274,27 @@ __global__ __launch_bounds__(kInterpTofBlock) void k_interp_tof_fp(InterpArgs ia
     const int64_t lo = (int64_t)blockIdx.x * kInterpTofBlock + threadIdx.x;
     if (lo >= ia.a.nmeas) return;
     const uint32_t nb = ia.a.tof_bins;
-    Samples S;
-    if (!interp_samples(ia, lo, S)) {
-        for (uint32_t b = 0; b < nb; b++) y[lo + (int64_t)b * ia.a.nmeas] = 0.f;
-        return;
-    }
-    float* ax = lds + threadIdx.x;
-    const InterpTof t(ia.a, lds + nb * kInterpTofBlock + threadIdx.x);
-    for (uint32_t b = 0; b < nb; b++) ax[b * kInterpTofBlock] = 0.f;
-    float sgn;
-    float D = interp_tof_start(ia, S, sgn);
+    float* tot = lds + threadIdx.x;                              // the LOR's bins, over its rays
+    float* ax = lds + nb * kInterpTofBlock + threadIdx.x;        // one ray's bins
+    const InterpTof t(ia.a, lds + 2 * nb * kInterpTofBlock + threadIdx.x);
+    for (uint32_t b = 0; b < nb; b++) tot[b * kInterpTofBlock] = 0.f;
     const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
-    interp_walk(ia, S, S.t0, S.n, [&](IF3 p) {
-        const float sum = t.loop(sgn, S.stepLen, D);
-        const float dX = (S.stepLen * tex3(img, Nx, Ny, Nz, p.x, p.y, p.z)) / sum;
-        for (uint32_t b = 0; b < nb; b++) ax[b * kInterpTofBlock] += t.wl[b * kInterpTofBlock] * dX;
-        D -= S.stepLen * sgn;
-    });
-    const float temp = interp_weight(ia, lo, S.L, true);
-    for (uint32_t b = 0; b < nb; b++) y[lo + (int64_t)b * ia.a.nmeas] = ax[b * kInterpTofBlock] * temp;
+    for (uint32_t ray = 0; ray < n_rays(ia.a); ray++) {   // each ray's bins times its weight (:633-641)
+        Samples S;
+        if (!interp_samples(ia, lo, S, (int)ray)) continue;
+        for (uint32_t b = 0; b < nb; b++) ax[b * kInterpTofBlock] = 0.f;
+        float sgn;
+        float D = interp_tof_start(ia, S, sgn);
+        interp_walk(ia, S, S.t0, S.n, [&](IF3 p) {
+            const float sum = t.loop(sgn, S.stepLen, D);
+            const float dX = (S.stepLen * tex3(img, Nx, Ny, Nz, p.x, p.y, p.z)) / sum;
+            for (uint32_t b = 0; b < nb; b++) ax[b * kInterpTofBlock] += t.wl[b * kInterpTofBlock] * dX;
+            D -= S.stepLen * sgn;
+        });
+        const float temp = interp_weight(ia, lo, S.L, true, (int)ray);
+        for (uint32_t b = 0; b < nb; b++) tot[b * kInterpTofBlock] += ax[b * kInterpTofBlock] * temp;
+    }
+    for (uint32_t b = 0; b < nb; b++) y[lo + (int64_t)b * ia.a.nmeas] = tot[b * kInterpTofBlock];
 }
 
 // BP (rhs): the sample's voxel gets sum_b ((stepLen temp) / sum) weight_b y_b (and the factors' sum in
@@ -292,37 +314,39 @@ __global__ __launch_bounds__(kInterpTofBlock) void k_interp_tof_bp(InterpArgs ia
         any |= v != 0.f;
     }
     if (!any && sens == nullptr) return;
-    Samples S;
-    if (!interp_samples(ia, lo, S)) return;
-    float t0 = S.t0;
-    uint32_t n = S.n;
-    if (ia.a.atten && ia.atn_image && ia.dL > 0.f) {
-        t0 = S.tStart + S.tStep / 10.f;
-        n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
-    }
     const InterpTof t(ia.a, lds + nb * kInterpTofBlock + threadIdx.x);
-    float sgn;
-    float D = interp_tof_start(ia, S, sgn);
-    const float e = S.stepLen * interp_weight(ia, lo, S.L, false);
     const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz;
     const uint64_t N = (uint64_t)Nx * Ny * Nz;
-    interp_walk(ia, S, t0, n, [&](IF3 p) {
-        const float sum = t.loop(sgn, S.stepLen, D);
-        const uint64_t ind = (uint64_t)(uint32_t)fmaxf(p.x * (float)Nx, 0.f) +
-                             (uint64_t)(uint32_t)fmaxf(p.y * (float)Ny, 0.f) * Nx +
-                             (uint64_t)(uint32_t)fmaxf(p.z * (float)Nz, 0.f) * Nx * Ny;
-        if (ind == 0 || ind >= N) return;
-        const float dX = e / sum;
-        float val = 0.f, yax = 0.f;
-        for (uint32_t b = 0; b < nb; b++) {
-            const float apu = dX * t.wl[b * kInterpTofBlock];
-            val += apu;
-            yax += apu * yv[b * kInterpTofBlock];
+    for (uint32_t ray = 0; ray < n_rays(ia.a); ray++) {
+        Samples S;
+        if (!interp_samples(ia, lo, S, (int)ray)) continue;
+        float t0 = S.t0;
+        uint32_t n = S.n;
+        if (ia.a.atten && ia.atn_image && ia.dL > 0.f) {
+            t0 = S.tStart + S.tStep / 10.f;
+            n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
         }
-        unsafeAtomicAdd(rhs + ind, yax);
-        if (sens) unsafeAtomicAdd(sens + ind, val);
-        D -= S.stepLen * sgn;
-    });
+        float sgn;
+        float D = interp_tof_start(ia, S, sgn);
+        const float e = S.stepLen * interp_weight(ia, lo, S.L, false, (int)ray);
+        interp_walk(ia, S, t0, n, [&](IF3 p) {
+            const float sum = t.loop(sgn, S.stepLen, D);
+            const uint64_t ind = (uint64_t)(uint32_t)fmaxf(p.x * (float)Nx, 0.f) +
+                                 (uint64_t)(uint32_t)fmaxf(p.y * (float)Ny, 0.f) * Nx +
+                                 (uint64_t)(uint32_t)fmaxf(p.z * (float)Nz, 0.f) * Nx * Ny;
+            if (ind == 0 || ind >= N) return;
+            const float dX = e / sum;
+            float val = 0.f, yax = 0.f;
+            for (uint32_t b = 0; b < nb; b++) {
+                const float apu = dX * t.wl[b * kInterpTofBlock];
+                val += apu;
+                yax += apu * yv[b * kInterpTofBlock];
+            }
+            unsafeAtomicAdd(rhs + ind, yax);
+            if (sens) unsafeAtomicAdd(sens + ind, val);
+            D -= S.stepLen * sgn;
+        });
+    }
 }
 
 // CT backprojection, voxel-driven: one lane per (x, y, z group of kInterpNV voxels), projections
@@ -439,7 +463,8 @@ static InterpArgs make_interp_args(const omg_geom* g, int64_t first, int64_t n) 
     return ia;
 }
 
-static size_t interp_tof_lds(const omg_geom* g) { return 2 * (size_t)g->d.tof_bins * kInterpTofBlock * sizeof(float); }
+// LDS of the TOF kernels: FP [bins: the LOR's sums][bins: one ray's sums][bins: weights], BP the first two
+static size_t interp_tof_lds(const omg_geom* g) { return 3 * (size_t)g->d.tof_bins * kInterpTofBlock * sizeof(float); }
 
 int interp_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
     const InterpArgs ia = make_interp_args(g, first, n);
@@ -495,7 +520,8 @@ int interp_osem(const omg_geom* g, int64_t first, int64_t n, const float* img, c
 // Attenuation image -> one factor per measurement by the interpolation walk (the geometry's own LORs).
 int interp_atten_factors(const omg_geom* g, const float* mu, float* f, int64_t n, hipStream_t s) {
     const InterpArgs ia = make_interp_args(g, 0, n);
-    hipLaunchKernelGGL(k_interp_atten, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, mu, f);
+    hipLaunchKernelGGL(k_interp_atten, dim3(grid_for(n, kInterpBlock), ia.a.rays_xy * ia.a.rays_z), dim3(kInterpBlock),
+                       0, s, ia, mu, f);
     OMG_CHECK_LAUNCH();
     return OMG_OK;
 }

@@ -441,8 +441,9 @@ class projectorClass:
             d.norm, d.norm_len = nrm.ctypes.data, nrm.size
         nr = (max(int(self.n_rays_transaxial), 1), max(int(self.n_rays_axial), 1))
         if nr != (1, 1) and not self.CT:
-            if pt != 1:
-                raise ValueError("Several rays per LOR are supported with improved Siddon (projector_type = 1)")
+            if pt not in (1, 4):   # -DN_RAYS for FPType / BPType 1 and 4 (init.py:357-360)
+                raise ValueError("Several rays per LOR are supported with improved Siddon (projector_type = 1) and "
+                                 "the interpolation-based projector (projector_type = 4)")
             d.rays_xy, d.rays_z = nr
             d.cr_xy, d.cr_z = float(self.cr_pz), float(self.cr_p)
         if self._tof() and pt not in (1, 3, 4):

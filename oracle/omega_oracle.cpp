@@ -1788,7 +1788,10 @@ struct IGeo {
     float kerroin;         // CT backprojection scale (dx dy dz) / (dPitchX dPitchY sourceToDetector)
     float dPitchX, dPitchY;
     int32_t atn_image;     // atten holds factors made from an attenuation image (ATN order), not given ones
+    int64_t atten_ray_stride = 0;   // several rays: ray r's image factor of measurement lo at r * stride + lo
 };
+
+static inline uint32_t n_rays(const IGeo& G) { return G.g.rays_xy * G.g.rays_z; }
 
 // Trilinear sample of an Nx x Ny x Nz image (x fastest) at normalised coordinates, clamp to edge.
 static inline float tex3(const float* img, uint32_t Nx, uint32_t Ny, uint32_t Nz, float px, float py, float pz) {
@@ -1820,9 +1823,9 @@ static inline float tex3(const float* img, uint32_t Nx, uint32_t Ny, uint32_t Nz
 // Sampling of one ray: start, parametric step, step length, count. Returns false when it misses the box.
 struct Samples { F3 s, v; float t0, tStep, stepLen, L, tStart, tEnd; uint32_t n; };
 
-static bool samples(const IGeo& G, int64_t lo, const float* x, const float* z, Samples& S) {
+static bool samples(const IGeo& G, int64_t lo, const float* x, const float* z, Samples& S, int ray = 0) {
     oracle::Ray<float> r;
-    oracle::fetch_ray(G.g, lo, x, z, r);
+    oracle::fetch_ray(G.g, lo, x, z, r, ray);
     S.s = f3(r.xs, r.ys, r.zs);
     const F3 d = f3(r.xd, r.yd, r.zd);
     S.v = sub(d, S.s);
@@ -1880,16 +1883,20 @@ static void walk(const IGeo& G, const Samples& S, float t0, uint32_t n, V&& visi
     }
 }
 
-static float lor_weight(const IGeo& G, int64_t lo, float L, bool fp) {
+// temp of a PET measurement: 1 / (L nRays) (projectorType4.cl:612-617), the corrections, the image attenuation
+// factor (of this ray) last for FP, first for BP.
+static float lor_weight(const IGeo& G, int64_t lo, float L, bool fp, int ray = 0) {
     const oracle::Geo<float>& g = G.g;
-    float temp = 1.f / L;
+    const uint32_t nr = n_rays(G);
+    float temp = nr > 1 ? 1.f / (L * (float)nr) : 1.f / L;
     const bool atn = g.atten && G.atn_image;
-    if (atn && !fp) temp *= g.atten[lo];
+    const float fa = atn ? g.atten[(int64_t)ray * G.atten_ray_stride + lo] : 0.f;
+    if (atn && !fp) temp *= fa;
     if (g.norm) temp *= g.norm[lo];
     if (g.scat) temp *= g.scat[lo];
     if (g.atten && !G.atn_image) temp *= g.atten[lo];
     temp *= g.global_factor;
-    if (atn && fp) temp *= g.atten[lo];
+    if (atn && fp) temp *= fa;
     return temp;
 }
 
@@ -1897,11 +1904,24 @@ static void fp(const IGeo& G, int64_t nMeas, const float* x, const float* z, con
     oracle::set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1000)
     for (int64_t lo = 0; lo < nMeas; lo++) {
-        Samples S;
-        if (!samples(G, lo, x, z, S)) { y[lo] = 0.f; continue; }
-        float ax = 0.f;
-        walk(G, S, S.t0, S.n, [&](F3 p) { ax += S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z); });
-        y[lo] = G.g.mode == oracle::CT_FLAT ? ax : ax * lor_weight(G, lo, S.L, true);
+        if (G.g.mode == oracle::CT_FLAT) {
+            Samples S;
+            if (!samples(G, lo, x, z, S)) { y[lo] = 0.f; continue; }
+            float ax = 0.f;
+            walk(G, S, S.t0, S.n, [&](F3 p) { ax += S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z); });
+            y[lo] = ax;
+            continue;
+        }
+        // PET: every ray's samples times its own weight, added in ray order (a ray missing the image adds nothing)
+        float tot = 0.f;
+        for (uint32_t ray = 0; ray < n_rays(G); ray++) {
+            Samples S;
+            if (!samples(G, lo, x, z, S, (int)ray)) continue;
+            float ax = 0.f;
+            walk(G, S, S.t0, S.n, [&](F3 p) { ax += S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z); });
+            tot += ax * lor_weight(G, lo, S.L, true, (int)ray);
+        }
+        y[lo] = tot;
     }
 }
 
@@ -1911,9 +1931,9 @@ static void bp_rays(const IGeo& G, int64_t nMeas, const float* x, const float* z
     const uint64_t N = (uint64_t)Nx * Ny * Nz;
     oracle::set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1000)
-    for (int64_t lo = 0; lo < nMeas; lo++) {
+    for (int64_t lo = 0; lo < nMeas; lo++) for (uint32_t ray = 0; ray < n_rays(G); ray++) {
         Samples S;
-        if (!samples(G, lo, x, z, S)) continue;
+        if (!samples(G, lo, x, z, S, (int)ray)) continue;
         float t0 = S.t0;
         uint32_t n = S.n;
         if (G.g.atten && G.atn_image && G.dL > 0.f) {
@@ -1921,7 +1941,7 @@ static void bp_rays(const IGeo& G, int64_t nMeas, const float* x, const float* z
             t0 = S.tStart + S.tStep / 10.f;
             n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
         }
-        const float e = S.stepLen * lor_weight(G, lo, S.L, false);
+        const float e = S.stepLen * lor_weight(G, lo, S.L, false, (int)ray);
         const float yv = yin[lo];
         walk(G, S, t0, n, [&](F3 p) {
             const uint64_t ind = (uint64_t)(uint32_t)std::fmax(p.x * (float)Nx, 0.f) +
@@ -1957,23 +1977,24 @@ static void fp_tof(const IGeo& G, int64_t nMeas, const float* x, const float* z,
     oracle::set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1000)
     for (int64_t lo = 0; lo < nMeas; lo++) {
-        Samples S;
-        if (!samples(G, lo, x, z, S)) {
-            for (uint32_t b = 0; b < tc.bins; b++) y[lo + (int64_t)b * nMeas] = 0.f;
-            continue;
+        std::vector<float> tot(tc.bins, 0.f), ax(tc.bins), wt(tc.bins);
+        for (uint32_t ray = 0; ray < n_rays(G); ray++) {   // each ray's bins times its weight (:633-641)
+            Samples S;
+            if (!samples(G, lo, x, z, S, (int)ray)) continue;
+            float sgn;
+            float D = tof_start(S, sgn);
+            if (G.dL <= 0.f) D -= ((S.t0 - S.tStart) * S.L * sgn);
+            std::fill(ax.begin(), ax.end(), 0.f);
+            walk(G, S, S.t0, S.n, [&](F3 p) {
+                const float sum = tc.loop(sgn, S.stepLen, D, wt.data());
+                const float dX = (S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z)) / sum;
+                for (uint32_t b = 0; b < tc.bins; b++) ax[b] += wt[b] * dX;
+                D -= S.stepLen * sgn;
+            });
+            const float temp = lor_weight(G, lo, S.L, true, (int)ray);
+            for (uint32_t b = 0; b < tc.bins; b++) tot[b] += ax[b] * temp;
         }
-        float sgn;
-        float D = tof_start(S, sgn);
-        if (G.dL <= 0.f) D -= ((S.t0 - S.tStart) * S.L * sgn);
-        std::vector<float> ax(tc.bins, 0.f), wt(tc.bins);
-        walk(G, S, S.t0, S.n, [&](F3 p) {
-            const float sum = tc.loop(sgn, S.stepLen, D, wt.data());
-            const float dX = (S.stepLen * tex3(img, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z)) / sum;
-            for (uint32_t b = 0; b < tc.bins; b++) ax[b] += wt[b] * dX;
-            D -= S.stepLen * sgn;
-        });
-        const float temp = lor_weight(G, lo, S.L, true);
-        for (uint32_t b = 0; b < tc.bins; b++) y[lo + (int64_t)b * nMeas] = ax[b] * temp;
+        for (uint32_t b = 0; b < tc.bins; b++) y[lo + (int64_t)b * nMeas] = tot[b];
     }
 }
 
@@ -1983,9 +2004,9 @@ static void bp_rays_tof(const IGeo& G, int64_t nMeas, const float* x, const floa
     const uint64_t N = (uint64_t)Nx * Ny * Nz;
     oracle::set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1000)
-    for (int64_t lo = 0; lo < nMeas; lo++) {
+    for (int64_t lo = 0; lo < nMeas; lo++) for (uint32_t ray = 0; ray < n_rays(G); ray++) {
         Samples S;
-        if (!samples(G, lo, x, z, S)) continue;
+        if (!samples(G, lo, x, z, S, (int)ray)) continue;
         float t0 = S.t0;
         uint32_t n = S.n;
         if (G.g.atten && G.atn_image && G.dL > 0.f) {
@@ -1995,7 +2016,7 @@ static void bp_rays_tof(const IGeo& G, int64_t nMeas, const float* x, const floa
         float sgn;
         float D = tof_start(S, sgn);
         if (G.dL <= 0.f) D -= ((S.t0 - S.tStart) * S.L * sgn);
-        const float e = S.stepLen * lor_weight(G, lo, S.L, false);
+        const float e = S.stepLen * lor_weight(G, lo, S.L, false, (int)ray);
         std::vector<float> wt(tc.bins);
         walk(G, S, t0, n, [&](F3 p) {
             const float sum = tc.loop(sgn, S.stepLen, D, wt.data());
@@ -2105,7 +2126,7 @@ static interp::IGeo make_igeo(const uint32_t* n3, const float* d3, const float* 
                               int32_t mode, int64_t plane_offset, float dPitchZ, int32_t ct_pitch, float global_factor,
                               const float* norm, const float* atten, int32_t atten_image, const float* scat,
                               const float* bmax3, const float* scale3, float dL, float kerroin, float dPitchX,
-                              float dPitchY) {
+                              float dPitchY, const float* rays4, int64_t nMeas) {
     interp::IGeo G;
     G.g = make_geo<float>(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm,
                           atten, 2, scat);
@@ -2117,6 +2138,13 @@ static interp::IGeo make_igeo(const uint32_t* n3, const float* d3, const float* 
     G.dPitchX = dPitchX;
     G.dPitchY = dPitchY;
     G.atn_image = atten_image;
+    if (rays4 && rays4[0] * rays4[1] > 1.f) {   // rays_xy, rays_z, cr_xy, cr_z
+        G.g.rays_xy = (uint32_t)rays4[0];
+        G.g.rays_z = (uint32_t)rays4[1];
+        G.g.cr_xy = rays4[2];
+        G.g.cr_z = rays4[3];
+        if (atten_image) G.atten_ray_stride = nMeas;
+    }
     return G;
 }
 
@@ -2124,11 +2152,11 @@ static interp::IGeo make_igeo(const uint32_t* n3, const float* d3, const float* 
     const uint32_t *n3, const float *d3, const float *b3, uint32_t size_x, uint32_t size_y, int32_t mode,             \
         int64_t plane_offset, float dPitchZ, int32_t ct_pitch, float global_factor, const float *norm,               \
         const float *atten, int32_t atten_image, const float *scat, const float *bmax3, const float *scale3, float dL, \
-        float kerroin, float dPitchX, float dPitchY, int64_t nMeas, const float *x, const float *z
+        float kerroin, float dPitchX, float dPitchY, const float *rays4, int64_t nMeas, const float *x, const float *z
 
 #define ORACLE_INTERP_GEO                                                                                             \
     make_igeo(n3, d3, b3, size_x, size_y, mode, plane_offset, dPitchZ, ct_pitch, global_factor, norm, atten,          \
-              atten_image, scat, bmax3, scale3, dL, kerroin, dPitchX, dPitchY)
+              atten_image, scat, bmax3, scale3, dL, kerroin, dPitchX, dPitchY, rays4, nMeas)
 
 // Forward projection of nMeas measurements (CT: nMeas = projections x panel pixels).
 extern "C" int oracle_interp_fp_f32(ORACLE_INTERP_ARGS, const float* img, float* y, int nthreads) {
@@ -2168,15 +2196,17 @@ extern "C" int oracle_interp_tof_bp_f32(ORACLE_INTERP_ARGS, uint32_t tof_bins, f
 extern "C" int oracle_interp_atten_f32(ORACLE_INTERP_ARGS, const float* mu, float* out, int nthreads) {
     const interp::IGeo G = ORACLE_INTERP_GEO;
     oracle::set_threads(nthreads);
+    // several rays: ray r's factors at out[r * nMeas + lo]
 #pragma omp parallel for schedule(dynamic, 1000)
-    for (int64_t lo = 0; lo < nMeas; lo++) {
+    for (int64_t lo = 0; lo < nMeas; lo++) for (uint32_t ray = 0; ray < interp::n_rays(G); ray++) {
+        float* o = out + (int64_t)ray * nMeas + lo;
         interp::Samples S;
-        if (!interp::samples(G, lo, x, z, S)) { out[lo] = 1.f; continue; }
+        if (!interp::samples(G, lo, x, z, S, (int)ray)) { *o = 1.f; continue; }
         float jel = 0.f;
         interp::walk(G, S, S.t0, S.n, [&](interp::F3 p) {
             jel += S.stepLen * -interp::tex3(mu, G.g.Nx, G.g.Ny, G.g.Nz, p.x, p.y, p.z);
         });
-        out[lo] = std::exp(jel);
+        *o = std::exp(jel);
     }
     return 0;
 }

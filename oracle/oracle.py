@@ -79,7 +79,7 @@ def lib():
         _lib.oracle_orth_trace_f32.restype = C.c_int64
         igeo = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int64, C.c_float, C.c_int32,
                 C.c_float, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, _f32p, _f32p, C.c_float, C.c_float,
-                C.c_float, C.c_float, C.c_int64, _f32p, _f32p]
+                C.c_float, C.c_float, C.c_void_p, C.c_int64, _f32p, _f32p]
         _lib.oracle_interp_fp_f32.argtypes = igeo + [_f32p, _f32p, C.c_int]
         _lib.oracle_interp_tof_fp_f32.argtypes = igeo + [C.c_uint32, C.c_float, _f32p, _f32p, _f32p, C.c_int]
         _lib.oracle_interp_tof_bp_f32.argtypes = igeo + [C.c_uint32, C.c_float, _f32p, _f32p, _f32p, C.c_void_p,
@@ -535,11 +535,13 @@ class InterpCase:
     the oracle, float32. ``dL``: interpolation length in mm (<= 0: the Joseph variant); ``kerroin``: the CT
     backprojection scale (dx dy dz) / (dPitchX dPitchY sourceToDetector); ``atten``: per-measurement
     attenuation factors, made from an attenuation image (``atten_image``) or given per LOR; CT
-    backprojection is voxel-driven over the (size_x x size_y)-pixel panels of the call's projections."""
+    backprojection is voxel-driven over the (size_x x size_y)-pixel panels of the call's projections.
+    ``rays``: (transaxial, axial, pitch_xy, pitch_z) for several rays per LOR (PET); image attenuation factors
+    are then per ray, ray r's at r * nmeas + lo (as ``atten_factors`` returns them)."""
 
     def __init__(self, grid, size_x, size_y, x, z, dL, mode=MODE_PET, plane_offset=0, dPitchZ=0.0, ct_pitch=0,
                  global_factor=1.0, norm=None, atten=None, atten_image=False, scat=None, kerroin=0.0, dPitchX=0.0,
-                 dPitchY=0.0):
+                 dPitchY=0.0, rays=None):
         self.grid = grid
         self.n3 = np.ascontiguousarray(grid.n3())
         self.d3 = grid.d3(np.float32)
@@ -557,13 +559,15 @@ class InterpCase:
         self.atten_image = int(bool(atten_image))
         self.scat = None if scat is None else np.ascontiguousarray(scat, dtype=np.float32)
         self.kerroin, self.dPitchX, self.dPitchY = float(kerroin), float(dPitchX), float(dPitchY)
+        self.rays = None if rays is None else np.asarray(rays, dtype=np.float32)
+        self.n_rays = 1 if rays is None else int(rays[0]) * int(rays[1])
 
     def _geo(self, nmeas):
         p = lambda a: None if a is None else a.ctypes.data  # noqa: E731
         return [self.n3.ctypes.data, self.d3, self.b3, self.size_x, self.size_y, self.mode, self.plane_offset,
                 self.dPitchZ, self.ct_pitch, self.global_factor, p(self.norm), p(self.atten), self.atten_image,
-                p(self.scat), self.bmax3, self.scale3, self.dL, self.kerroin, self.dPitchX, self.dPitchY, nmeas,
-                self.x, self.z]
+                p(self.scat), self.bmax3, self.scale3, self.dL, self.kerroin, self.dPitchX, self.dPitchY,
+                p(self.rays), nmeas, self.x, self.z]
 
     def fp(self, img, nmeas, nthreads=0):
         y = np.zeros(nmeas, dtype=np.float32)
@@ -572,7 +576,7 @@ class InterpCase:
 
     def atten_factors(self, mu, nmeas, nthreads=0):
         """exp(-sum stepLen * mu(p)) over the forward projection's samples (1 where the ray misses)."""
-        out = np.ones(nmeas, dtype=np.float32)
+        out = np.ones(nmeas * self.n_rays, dtype=np.float32)
         lib().oracle_interp_atten_f32(*self._geo(nmeas), np.ascontiguousarray(mu, dtype=np.float32), out, nthreads)
         return out
 
