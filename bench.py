@@ -49,7 +49,7 @@ def parse():
                     help="summary of the rocprofv3 --pmc VALU issue counters of the dominant kernel, if present")
     ap.add_argument("--workload", default="pet2d_osem",
                     choices=["pet2d_osem", "pet3d_frames", "ct_cgls_bdd", "pet2d_orth", "pet2d_volume", "pet3d_autograd",
-                             "pet2d_tof", "pet2d_volume_tof", "unet"],
+                             "pet2d_tof", "pet2d_volume_tof", "pet2d_interp", "pet2d_interp_tof", "unet"],
                     help="pet2d_osem (default, the headline metric); the others print their own line")
     ap.add_argument("--frames", type=int, default=24, help="pet3d_frames: frames of the dynamic study")
     ap.add_argument("--voxel-bp", action="store_true",
@@ -69,6 +69,7 @@ def main():
         {"pet3d_frames": bench_frames, "ct_cgls_bdd": bench_cgls, "pet2d_orth": bench_orth,
          "pet2d_volume": lambda a: bench_orth(a, volume=True),
          "pet2d_volume_tof": lambda a: bench_orth(a, volume=True, tof=True),
+         "pet2d_interp": bench_interp, "pet2d_interp_tof": lambda a: bench_interp(a, tof=True),
          "pet3d_autograd": bench_autograd, "pet2d_tof": bench_tof, "unet": bench_unet}[args.workload](args)
         return
 
@@ -522,6 +523,97 @@ def bench_orth(args, volume=False, tof=False):
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": kern, "avg_launch_us": round(s_s * 1e6, 1),
                      "algorithmic_bytes_per_launch": int(byt)},
+        "cpu_baseline": cpu,
+    })
+
+
+def bench_interp(args, tof=False):
+    """2-D plane PET OSEM with the interpolation projector (projector type 4, the operator API's default
+    interpolation length dL = dx / 2) on the pet2d_osem geometry. value = subset-iterations/s (8 subsets x 1
+    iteration per step). tof: with time of flight, 17 bins of 140 ps at 400 ps FWHM. Algorithmic bytes per
+    subset step: 12 per sample (the forward projection's interpolated read, the backprojection's atomic
+    read-modify-write) plus 8 per measurement and bin; the samples are counted from the LORs' chords through
+    the image (chord / dL + 1 per LOR and plane)."""
+    import numpy as np
+    import torch
+    from omega_amd import _lib, geometry as G, workloads
+    from omega_amd.projector import projectorClass
+    from omega_amd.recon import OSEMRunner
+    dev = torch.device("cuda:0")
+    _lib.load()
+    grid = G.Grid.from_fov(128, 128, 63, 320.0, 320.0, 157.5)
+    sc = G.pet_2d_planes(grid, radius=400.0, n_radial=128, n_angles=160, radial_pitch=2.5)
+    p = projectorClass()
+    p.Nx, p.Ny, p.Nz = 128, 128, 63
+    p.FOVa_x, p.FOVa_y, p.axial_fov = 320.0, 320.0, 157.5
+    p.projector_type = 4
+    p.x, p.z = sc.xy_table(), sc.z
+    p.Ndist, p.Nang = sc.n_radial, sc.n_angles
+    p.subsets, p.subsetType = 8, 8
+    if tof:
+        p.TOF_bins, p.TOF_bins_used, p.TOF_width, p.TOF_FWHM = 17, 17, 140e-12, 400e-12
+    p.addProjector()
+    p.initProj()
+    nb = p.nBins
+    dl = float(p._interp_scalars()[0])
+    ph, y = workloads.pet2d_data(p, seed=0)
+    runner = OSEMRunner(p, y)
+    x = torch.ones(p.grid.N, device=dev)
+    for _ in range(args.warmup):
+        for k in range(p.subsets):
+            runner.subset_step(x, k)
+        runner.sens_done = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for k in range(p.subsets):
+            runner.subset_step(x, k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    M0 = int(p.nMeasSubset[0, 0])
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    runner.subset_step(x, 0)
+    b.record()
+    b.synchronize()
+    s_s = a.elapsed_time(b) * 1e-3
+    # samples: every LOR's chord through the transaxial box, per plane of subset 0
+    xy = np.asarray(sc.xy_table(), np.float64).reshape(-1, 4)
+    s0, v = xy[:, 0:2], xy[:, 2:4] - xy[:, 0:2]
+    lo, hi = np.array([grid.bx, grid.by]), np.array([grid.bx + 320.0, grid.by + 320.0])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ta, tb = (lo - s0) / v, (hi - s0) / v
+    tmin = np.nanmax(np.where(np.isfinite(ta), np.minimum(ta, tb), -np.inf), axis=1)
+    tmax = np.nanmin(np.where(np.isfinite(ta), np.maximum(ta, tb), np.inf), axis=1)
+    chord = np.clip(tmax - tmin, 0, None) * np.linalg.norm(v, axis=1)
+    per_plane = float(np.sum(np.where(chord > 0, np.floor(chord / dl) + 1, 0)))
+    samples = per_plane * (M0 // (sc.n_radial * sc.n_angles))
+    byt = 12.0 * samples + 8.0 * M0 * nb
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        _progress("interp: CPU baseline")
+        case = O.InterpCase(p.grid, p.nRowsD, p.nColsD, p.x, p.z, dl, plane_offset=int(p.nMeas[0]))
+        tofp = (nb, float(p.sigma_x), p.TOFCenter) if tof else None
+        rate, per, reps = _cpu_subset_iteration(p, y[:M0 * nb].cpu().numpy(), case, M0, args.cpu_seconds, tofp)
+        cpu = {"value": round(rate, 3), "unit": "subset-iters/s", "cores": _threads(), "kind": "port",
+               "sample": f"{reps} OSEM subset-iterations of subset 0 ({per:.2f} s each) with the oracle's "
+                         f"interpolation projector{' with TOF' if tof else ''}; OpenMP threads={_threads()}; "
+                         f"{_cpu_model()}"}
+    _common(args, {
+        "metric": "OSEM subset-iters/s, interpolation projector" + (" with TOF" if tof else ""),
+        "value": round(p.subsets * args.steps / dt, 2), "unit": "subset-iters/s",
+        "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "data": "synthetic (seeded ellipsoid phantom, Poisson data at 5e6 counts)",
+        "config": {"workload": "pet2d_osem_interp" + ("_tof" if tof else ""), "image": [128, 128, 63],
+                   "sinogram": [128, 160, 63], "interp_dl_mm": round(dl, 4),
+                   **({"tof_bins": nb, "tof_fwhm_ps": 400, "tof_bin_ps": 140} if tof else {}),
+                   "subsets": 8, "samples_subset0": int(samples)},
+        "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": ("k_interp_tof_fp + ratio + k_interp_tof_bp + update" if tof
+                                else "k_interp_fp + ratio + k_interp_bp_rays + update"),
+                     "avg_launch_us": round(s_s * 1e6, 1), "algorithmic_bytes_per_launch": int(byt)},
         "cpu_baseline": cpu,
     })
 
