@@ -349,6 +349,70 @@ __global__ __launch_bounds__(kInterpTofBlock) void k_interp_tof_bp(InterpArgs ia
     }
 }
 
+// Direct-plane PET sinograms (every LOR of a plane in one image slice: zs == zd, so every sample's z is the
+// plane's): k_interp_bp_rays with the slice accumulated in LDS. A workgroup owns (plane, chunk of the plane's
+// LORs), adds each sample's stepLen temp y (and stepLen temp) to its LDS slice, then adds the slice to rhs
+// (and sens) with one global atomic per touched voxel, instead of one per sample. The slice is the one the
+// walk's arithmetic gives for the plane's z, trunc(max(((0 t + z) - bz) sz Nz, 0)); a sample on the image's
+// far x face of its last row has the linear index of the next slice's first voxel (x = Nx wraps), where the
+// reference deposits it: such samples go to global memory directly.
+constexpr int kInterpPlanarTPB = 512;
+
+template <bool SENS>
+__global__ __launch_bounds__(kInterpPlanarTPB) void k_interp_bp_planar(InterpArgs ia, uint32_t n_planes, uint32_t chunk,
+                                                                       const float* __restrict__ yin,
+                                                                       float* __restrict__ rhs,
+                                                                       float* __restrict__ sens) {
+    extern __shared__ float slice[];   // [Nyx] rhs, then [Nyx] sens
+    const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz, Nyx = Nx * Ny;
+    const uint32_t per = ia.a.size_x * ia.a.size_y;
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    const float pz = ((0.f + ia.a.z[(ia.a.plane_offset + plane) * 2]) - ia.b.z) * ia.scale.z;
+    const uint64_t zb = (uint64_t)(uint32_t)fmaxf(pz * (float)Nz, 0.f);
+    for (uint32_t v = threadIdx.x; v < (SENS ? 2 : 1) * Nyx; v += kInterpPlanarTPB) slice[v] = 0.f;
+    __syncthreads();
+    const uint64_t N = (uint64_t)Nyx * Nz;
+    const uint32_t i_end = min(per, (grp + 1) * chunk);
+    for (uint32_t i = grp * chunk + threadIdx.x; i < i_end; i += kInterpPlanarTPB) {
+        const int64_t lo = (int64_t)plane * per + i;
+        const float yv = yin[lo];
+        if (yv == 0.f && !SENS) continue;
+        Samples S;
+        if (!interp_samples(ia, lo, S)) continue;
+        float t0 = S.t0;
+        uint32_t n = S.n;
+        if (ia.a.atten && ia.atn_image && ia.dL > 0.f) {
+            t0 = S.tStart + S.tStep / 10.f;
+            n = t0 <= S.tEnd ? (uint32_t)((S.tEnd - t0) / S.tStep) + 1u : 0u;
+        }
+        const float e = S.stepLen * interp_weight(ia, lo, S.L, false);
+        interp_walk(ia, S, t0, n, [&](IF3 p) {
+            const uint64_t ind = (uint64_t)(uint32_t)fmaxf(p.x * (float)Nx, 0.f) +
+                                 (uint64_t)(uint32_t)fmaxf(p.y * (float)Ny, 0.f) * Nx +
+                                 (uint64_t)(uint32_t)fmaxf(p.z * (float)Nz, 0.f) * Nyx;
+            if (ind == 0 || ind >= N) return;
+            if (ind / Nyx == zb) {
+                const uint32_t v = (uint32_t)(ind - zb * Nyx);
+                atomicAdd(slice + v, yv * e);
+                if (SENS) atomicAdd(slice + Nyx + v, e);
+            } else {
+                unsafeAtomicAdd(rhs + ind, yv * e);
+                if (SENS) unsafeAtomicAdd(sens + ind, e);
+            }
+        });
+    }
+    __syncthreads();
+    if (zb >= Nz) return;   // the plane lies outside the image: nothing was added to the slice
+    for (uint32_t v = threadIdx.x; v < Nyx; v += kInterpPlanarTPB) {
+        const float a = slice[v];
+        if (a != 0.f) unsafeAtomicAdd(rhs + zb * Nyx + v, a);
+        if (SENS) {
+            const float b = slice[Nyx + v];
+            if (b != 0.f) unsafeAtomicAdd(sens + zb * Nyx + v, b);
+        }
+    }
+}
+
 // CT backprojection, voxel-driven: one lane per (x, y, z group of kInterpNV voxels), projections
 // [plane_offset, plane_offset + n_proj) of the panel stack yin (n_proj panels of size_x x size_y).
 __global__ __launch_bounds__(kInterpBlock) void k_interp_bp_ct(InterpArgs ia, uint32_t n_proj,
@@ -480,6 +544,25 @@ int interp_forward(const omg_geom* g, int64_t first, int64_t n, const float* img
     return OMG_OK;
 }
 
+// The slice-in-LDS backprojection: direct-plane sinograms without TOF or several rays (the geometry's planar
+// flag), whole planes per call, rhs and sens slices within one CU's LDS.
+static bool interp_planar_ok(const omg_geom* g, int64_t n) {
+    const int64_t per = (int64_t)g->d.size_x * g->d.size_y;
+    return g->planar && !g->force_generic && per > 0 && n % per == 0 &&
+           2 * (size_t)g->d.Nx * g->d.Ny * sizeof(float) <= 160 * 1024 - 256;
+}
+
+static int interp_planar_attr(size_t bytes) {   // above 64 KiB of LDS the kernels must opt in
+    static size_t granted = 64 * 1024;
+    if (bytes <= granted) return OMG_OK;
+    OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_interp_bp_planar<true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_interp_bp_planar<false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    granted = bytes;
+    return OMG_OK;
+}
+
 int interp_backward(const omg_geom* g, int64_t first, int64_t n, const float* yin, float* rhs, float* sens,
                     hipStream_t s) {
     const InterpArgs ia = make_interp_args(g, first, n);
@@ -494,6 +577,18 @@ int interp_backward(const omg_geom* g, int64_t first, int64_t n, const float* yi
     } else if (g->d.tof_bins > 1) {
         hipLaunchKernelGGL(k_interp_tof_bp, dim3(grid_for(n, kInterpTofBlock)), dim3(kInterpTofBlock),
                            interp_tof_lds(g), s, ia, yin, rhs, sens);
+    } else if (interp_planar_ok(g, n)) {
+        const uint32_t per = g->d.size_x * g->d.size_y, n_planes = (uint32_t)(n / per);
+        const uint32_t groups = (per + kInterpPlanarTPB - 1) / kInterpPlanarTPB;   // one LOR per lane
+        const uint32_t chunk = (per + groups - 1) / groups;
+        const size_t lds = (sens ? 2 : 1) * (size_t)g->d.Nx * g->d.Ny * sizeof(float);
+        if (int rc = interp_planar_attr(2 * (size_t)g->d.Nx * g->d.Ny * sizeof(float))) return rc;
+        if (sens)
+            hipLaunchKernelGGL(k_interp_bp_planar<true>, dim3(n_planes * groups), dim3(kInterpPlanarTPB), lds, s, ia,
+                               n_planes, chunk, yin, rhs, sens);
+        else
+            hipLaunchKernelGGL(k_interp_bp_planar<false>, dim3(n_planes * groups), dim3(kInterpPlanarTPB), lds, s, ia,
+                               n_planes, chunk, yin, rhs, sens);
     } else {
         hipLaunchKernelGGL(k_interp_bp_rays, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, yin, rhs,
                            sens);
