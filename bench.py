@@ -612,7 +612,7 @@ def bench_interp(args, tof=False):
         "roofline": {"bound": "hbm", "achieved": round(byt / s_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(byt / s_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": ("k_interp_tof_fp + ratio + k_interp_tof_bp + update" if tof
-                                else "k_interp_fp + ratio + k_interp_bp_planar (slice in LDS) + update"),
+                                else "k_interp_fp_planar + ratio + k_interp_bp_planar (slices in LDS) + update"),
                      "avg_launch_us": round(s_s * 1e6, 1), "algorithmic_bytes_per_launch": int(byt)},
         "cpu_baseline": cpu,
     })

@@ -74,6 +74,25 @@ __device__ __forceinline__ float tex3(const float* __restrict__ img, uint32_t Nx
     return (1.f - c) * c0 + c * c1;
 }
 
+// tex3 with its two z slices given (the planar forward projection's LDS copies): the same lerps.
+__device__ __forceinline__ float tex_slices(const float* k0, const float* k1, uint32_t Nx, uint32_t Ny, float px,
+                                            float py, float c) {
+    uint32_t x0, x1, y0, y1;
+    float a, b;
+    tex_axis(px, Nx, x0, x1, a);
+    tex_axis(py, Ny, y0, y1, b);
+    const uint32_t r0 = y0 * Nx, r1 = y1 * Nx;
+    const float v000 = k0[r0 + x0], v100 = k0[r0 + x1], v010 = k0[r1 + x0], v110 = k0[r1 + x1];
+    const float v001 = k1[r0 + x0], v101 = k1[r0 + x1], v011 = k1[r1 + x0], v111 = k1[r1 + x1];
+    const float c00 = (1.f - a) * v000 + a * v100;
+    const float c10 = (1.f - a) * v010 + a * v110;
+    const float c01 = (1.f - a) * v001 + a * v101;
+    const float c11 = (1.f - a) * v011 + a * v111;
+    const float c0 = (1.f - b) * c00 + b * c10;
+    const float c1 = (1.f - b) * c01 + b * c11;
+    return (1.f - c) * c0 + c * c1;
+}
+
 struct Samples { IF3 s, v; float t0, tStep, stepLen, L, tStart, tEnd; uint32_t n; };
 
 __device__ __forceinline__ bool interp_samples(const InterpArgs& ia, int64_t lo, Samples& S, int ray = 0) {
@@ -349,6 +368,39 @@ __global__ __launch_bounds__(kInterpTofBlock) void k_interp_tof_bp(InterpArgs ia
     }
 }
 
+// Forward projection of direct-plane PET sinograms: every sample of a plane has the plane's z, so a
+// workgroup (plane, chunk of its LORs) stages the two image slices its trilinear samples blend in LDS and
+// gathers the corners there; values as k_interp_fp's (the same lerps on the same numbers).
+constexpr int kInterpPlanarFpTPB = 512;
+
+__global__ __launch_bounds__(kInterpPlanarFpTPB) void k_interp_fp_planar(InterpArgs ia, uint32_t n_planes,
+                                                                        uint32_t chunk,
+                                                                        const float* __restrict__ img,
+                                                                        float* __restrict__ y) {
+    extern __shared__ float slices[];   // [Nyx] slice z0, [Nyx] slice z1
+    const uint32_t Nx = ia.a.Nx, Ny = ia.a.Ny, Nz = ia.a.Nz, Nyx = Nx * Ny;
+    const uint32_t per = ia.a.size_x * ia.a.size_y;
+    const uint32_t plane = blockIdx.x % n_planes, grp = blockIdx.x / n_planes;
+    // the walk's z for this plane: ((0 t + z) - bz) sz, constant along every LOR of the plane
+    const float pz = ((0.f + ia.a.z[(ia.a.plane_offset + plane) * 2]) - ia.b.z) * ia.scale.z;
+    uint32_t z0, z1;
+    float c;
+    tex_axis(pz, Nz, z0, z1, c);
+    for (uint32_t v = threadIdx.x; v < Nyx; v += kInterpPlanarFpTPB) {
+        slices[v] = img[(size_t)z0 * Nyx + v];
+        slices[Nyx + v] = img[(size_t)z1 * Nyx + v];
+    }
+    __syncthreads();
+    const uint32_t i = grp * chunk + threadIdx.x;
+    if (i >= min(per, (grp + 1) * chunk)) return;
+    const int64_t lo = (int64_t)plane * per + i;
+    Samples S;
+    if (!interp_samples(ia, lo, S)) { y[lo] = 0.f; return; }
+    float ax = 0.f;
+    interp_walk(ia, S, S.t0, S.n, [&](IF3 p) { ax += S.stepLen * tex_slices(slices, slices + Nyx, Nx, Ny, p.x, p.y, c); });
+    y[lo] = 0.f + ax * interp_weight(ia, lo, S.L, true);
+}
+
 // Direct-plane PET sinograms (every LOR of a plane in one image slice: zs == zd, so every sample's z is the
 // plane's): k_interp_bp_rays with the slice accumulated in LDS. A workgroup owns (plane, chunk of the plane's
 // LORs), adds each sample's stepLen temp y (and stepLen temp) to its LDS slice, then adds the slice to rhs
@@ -528,24 +580,8 @@ static InterpArgs make_interp_args(const omg_geom* g, int64_t first, int64_t n) 
 }
 
 // LDS of the TOF kernels: FP [bins: the LOR's sums][bins: one ray's sums][bins: weights], BP the first two
-static size_t interp_tof_lds(const omg_geom* g) { return 3 * (size_t)g->d.tof_bins * kInterpTofBlock * sizeof(float); }
-
-int interp_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
-    const InterpArgs ia = make_interp_args(g, first, n);
-    hipEvent_t stop;
-    timing_begin(s, &stop);
-    if (g->d.tof_bins > 1)
-        hipLaunchKernelGGL(k_interp_tof_fp, dim3(grid_for(n, kInterpTofBlock)), dim3(kInterpTofBlock),
-                           interp_tof_lds(g), s, ia, img, y);
-    else
-        hipLaunchKernelGGL(k_interp_fp, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, img, y);
-    timing_end(s, stop);
-    OMG_CHECK_LAUNCH();
-    return OMG_OK;
-}
-
-// The slice-in-LDS backprojection: direct-plane sinograms without TOF or several rays (the geometry's planar
-// flag), whole planes per call, rhs and sens slices within one CU's LDS.
+// The slice-in-LDS kernels: direct-plane sinograms without TOF or several rays (the geometry's planar flag),
+// whole planes per call, two slices within one CU's LDS.
 static bool interp_planar_ok(const omg_geom* g, int64_t n) {
     const int64_t per = (int64_t)g->d.size_x * g->d.size_y;
     return g->planar && !g->force_generic && per > 0 && n % per == 0 &&
@@ -559,7 +595,33 @@ static int interp_planar_attr(size_t bytes) {   // above 64 KiB of LDS the kerne
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_interp_bp_planar<false>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    OMG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_interp_fp_planar),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     granted = bytes;
+    return OMG_OK;
+}
+
+static size_t interp_tof_lds(const omg_geom* g) { return 3 * (size_t)g->d.tof_bins * kInterpTofBlock * sizeof(float); }
+
+int interp_forward(const omg_geom* g, int64_t first, int64_t n, const float* img, float* y, hipStream_t s) {
+    const InterpArgs ia = make_interp_args(g, first, n);
+    hipEvent_t stop;
+    timing_begin(s, &stop);
+    if (g->d.tof_bins > 1) {
+        hipLaunchKernelGGL(k_interp_tof_fp, dim3(grid_for(n, kInterpTofBlock)), dim3(kInterpTofBlock),
+                           interp_tof_lds(g), s, ia, img, y);
+    } else if (interp_planar_ok(g, n)) {   // one LOR per lane, a plane's LORs over several workgroups
+        const uint32_t per = g->d.size_x * g->d.size_y, n_planes = (uint32_t)(n / per);
+        const uint32_t groups = (per + kInterpPlanarFpTPB - 1) / kInterpPlanarFpTPB;
+        const uint32_t chunk = (per + groups - 1) / groups;
+        const size_t lds = 2 * (size_t)g->d.Nx * g->d.Ny * sizeof(float);
+        if (int rc = interp_planar_attr(lds)) return rc;
+        hipLaunchKernelGGL(k_interp_fp_planar, dim3(n_planes * groups), dim3(kInterpPlanarFpTPB), lds, s, ia,
+                           n_planes, chunk, img, y);
+    } else
+        hipLaunchKernelGGL(k_interp_fp, dim3(grid_for(n, kInterpBlock)), dim3(kInterpBlock), 0, s, ia, img, y);
+    timing_end(s, stop);
+    OMG_CHECK_LAUNCH();
     return OMG_OK;
 }
 
