@@ -1,5 +1,5 @@
 """Learned warm start for OSEM-RDP: pairs made by the method itself, a 3-D U-Net trained on them with the
-library's own kernels, and the evaluation that keeps the plain method as fallback and judge.
+library's own kernels, and the evaluation that keeps the plain method as fallback and yardstick.
 
 * Pairs: for each seeded ellipsoid phantom, Poisson data of ``counts`` total counts; the input is the
   image after ``n_short`` plain OSEM iterations, the target the image after ``n_long`` OSEM-RDP
@@ -17,7 +17,7 @@ library's own kernels, and the evaluation that keeps the plain method as fallbac
   (``segment_search``, two forward projections): it never scores below its input, but OSL continued
   from it has needed more iterations than from the network's image itself.
 * Support (optional): every run reconstructs the same voxel support (``support_mask``: the voxels with
-  at least a given fraction of the peak sensitivity); plain OSEM-RDP on that support is the judge.
+  at least a given fraction of the peak sensitivity); plain OSEM-RDP on that support is the yardstick.
   Wall time is compared on the GPU's own clock (HIP events); when the warm start does not reach the
   target within ``n_long`` iterations the plain result stands.
 """

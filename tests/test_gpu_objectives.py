@@ -1,7 +1,7 @@
 """The objective kernels against their closed forms in float64 (numpy).
 
 omg_poisson_loglik and omg_frames_loglik evaluate L = sum_i y_i log(yhat_i) - yhat_i (terms with y_i = 0
-are -yhat_i); the learned warm start's judge (the plain run's log-likelihood) stands on them. Both read
+are -yhat_i); the learned warm start's yardstick (the plain run's log-likelihood) stands on them. Both read
 float32 inputs and accumulate in double, so they agree with numpy's float64 sum of the same terms to
 summation-order rounding (1e-12 relative here). Sizes: the 2-D plane PET sinogram (128 x 160 x 63 =
 1,290,240 measurements) and a ragged length that is not a multiple of any block size.

@@ -3,7 +3,7 @@ U-Net training on the library's kernels, held-out evaluation against the plain O
 
 Settings of the full-size workload: beta 0.1, angle subsets (type 4), and the support of voxels with at
 least 30% of the peak sensitivity, used by every run (plain, short, continuations), so plain OSEM-RDP
-on that support stays the judge. Bar: the training loss falls; on held-out seeds, OSEM-RDP continued from the network's image
+on that support stays the yardstick. Bar: the training loss falls; on held-out seeds, OSEM-RDP continued from the network's image
 ends its first iteration at a strictly higher OSEM-RDP objective than the same continuation from the
 network's input (the 2-iteration OSEM image) on average, and reaches the plain run's objective in
 strictly fewer iterations on average (a network that degrades its input fails both); the objective of
